@@ -1,0 +1,15 @@
+"""geomjax_amd: geomjax's Riemannian integrator hot path as gfx950 HIP kernels.
+
+Public names follow the reference package (geomjax/__init__.py:1-58) for the
+samplers this build serves.
+"""
+from . import metrics, models, random
+from .base import SamplingAlgorithm
+from .metrics import MongeMetric
+from .models import NealFunnel
+from .rmhmc import rmhmc
+
+__version__ = "0.1.0"
+
+__all__ = ["rmhmc", "SamplingAlgorithm", "NealFunnel", "MongeMetric", "metrics", "models", "random",
+           "__version__"]

@@ -1,0 +1,131 @@
+"""ctypes binding of the C ABI in include/geomjax_amd.h.
+
+The shared library is built in-tree by ``geomjax_amd.build`` and loaded from
+``geomjax_amd/_lib/libgeomjax_amd.so``.  There is no other implementation of
+the hot path: if the library is missing or fails to load, every sampler call
+raises ``NativeLibraryError``.
+"""
+import ctypes
+import os
+import threading
+
+import torch
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "libgeomjax_amd.so")
+
+GJX_F64 = 0
+GJX_F32 = 1
+
+_c_int32 = ctypes.c_int32
+_c_double = ctypes.c_double
+_c_u64 = ctypes.c_uint64
+_vp = ctypes.c_void_p
+
+
+class NativeLibraryError(RuntimeError):
+    """The HIP library is missing, failed to load, or returned an error."""
+
+
+class FunnelParams(ctypes.Structure):
+    _fields_ = [
+        ("num_chains", _c_int32),
+        ("dim", _c_int32),
+        ("dtype", _c_int32),
+        ("num_integration_steps", _c_int32),
+        ("max_iters", _c_int32),
+        ("num_transitions", _c_int32),
+        ("lanes_per_chain", _c_int32),
+        ("reserved", _c_int32),
+        ("step_size", _c_double),
+        ("alpha2", _c_double),
+        ("sigma", _c_double),
+        ("convergence_tol", _c_double),
+        ("divergence_tol", _c_double),
+        ("divergence_threshold", _c_double),
+        ("seed", _c_u64),
+        ("counter", _c_u64),
+    ]
+
+
+class RmhmcStepIO(ctypes.Structure):
+    _fields_ = [(n, _vp) for n in (
+        "position", "logdensity", "logdensity_grad", "z", "u",
+        "new_position", "new_logdensity", "new_logdensity_grad",
+        "momentum", "acceptance_rate", "is_accepted", "is_divergent", "energy",
+        "proposal_position", "proposal_momentum", "proposal_velocity", "proposal_logdensity",
+        "proposal_logdensity_grad", "proposal_weight", "proposal_sum_log_p_accept",
+        "fp_iters", "fp_norm", "fp_success",
+    )]
+
+
+class RmhmcSampleIO(ctypes.Structure):
+    _fields_ = [(n, _vp) for n in (
+        "position", "logdensity", "z", "u",
+        "final_position", "final_logdensity", "final_logdensity_grad",
+        "samples", "acceptance_rate", "num_accepted", "num_fevals",
+    )]
+
+
+# entry point name -> (argtypes) ; every entry point returns int
+_SIGNATURES = {
+    "gjx_rmhmc_monge_funnel_step": [ctypes.POINTER(FunnelParams), ctypes.POINTER(RmhmcStepIO), _vp],
+    "gjx_rmhmc_monge_funnel_sample": [ctypes.POINTER(FunnelParams), ctypes.POINTER(RmhmcSampleIO), _vp],
+    "gjx_funnel_value_and_grad": [_c_int32, _c_int32, _c_double, _c_int32, _vp, _vp, _vp, _vp],
+}
+_STRING_FUNCS = ("gjx_version", "gjx_last_error")
+
+_lib = None
+_lock = threading.Lock()
+
+
+def load(path=None):
+    """Load (once) and return the ctypes handle of the native library."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        p = path or LIB_PATH
+        if not os.path.exists(p):
+            raise NativeLibraryError(
+                f"{p} does not exist: build it with `python -m geomjax_amd.build` "
+                "(there is no non-HIP implementation of the sampler kernels)"
+            )
+        try:
+            lib = ctypes.CDLL(p)
+        except OSError as e:
+            raise NativeLibraryError(f"cannot load {p}: {e}") from e
+        for name, argtypes in _SIGNATURES.items():
+            f = getattr(lib, name)
+            f.argtypes = argtypes
+            f.restype = ctypes.c_int
+        for name in _STRING_FUNCS:
+            f = getattr(lib, name)
+            f.argtypes = []
+            f.restype = ctypes.c_char_p
+        _lib = lib
+        return lib
+
+
+def exported_symbols():
+    return list(_SIGNATURES) + list(_STRING_FUNCS)
+
+
+def last_error():
+    return load().gjx_last_error().decode(errors="replace")
+
+
+def check(rc, what):
+    if rc != 0:
+        raise NativeLibraryError(f"{what} failed with status {rc}: {last_error()}")
+
+
+def ptr(t):
+    """Device pointer of a tensor (None -> NULL)."""
+    if t is None:
+        return None
+    return t.data_ptr()
+
+
+def stream_handle(device):
+    """hipStream_t of torch's current stream on ``device``."""
+    return torch.cuda.current_stream(device).cuda_stream

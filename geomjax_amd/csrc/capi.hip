@@ -1,0 +1,176 @@
+// extern "C" entry points: argument validation, dispatch, error reporting.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/geomjax_amd.h"
+#include "funnel_rmhmc.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* where) {
+  return fail(GJX_EHIP, "%s: %s", where, hipGetErrorString(e));
+}
+
+#define GJX_REQUIRE(cond, ...)                     \
+  do {                                             \
+    if (!(cond)) return fail(GJX_EINVAL, __VA_ARGS__); \
+  } while (0)
+
+// Lanes per chain when the caller does not choose: enough lanes that each
+// holds at most 8 coordinates, at least 4 and at most 64.
+int default_lpc(int dim) {
+  int lpc = 4;
+  while (lpc < 64 && (dim - 1 + lpc - 1) / lpc > 8) lpc <<= 1;
+  return lpc;
+}
+
+int check_funnel_params(const gjx_funnel_params* p, bool sample, int* lpc, int* epl) {
+  GJX_REQUIRE(p != nullptr, "params is NULL");
+  GJX_REQUIRE(p->num_chains > 0, "num_chains must be > 0 (got %d)", p->num_chains);
+  GJX_REQUIRE(p->dim >= 2, "the funnel needs dim >= 2 (got %d)", p->dim);
+  GJX_REQUIRE(p->dtype == GJX_F64 || p->dtype == GJX_F32, "dtype must be GJX_F64 or GJX_F32 (got %d)",
+              p->dtype);
+  GJX_REQUIRE(p->num_integration_steps >= 0, "num_integration_steps must be >= 0 (got %d)",
+              p->num_integration_steps);
+  GJX_REQUIRE(p->max_iters >= 0, "max_iters must be >= 0 (got %d)", p->max_iters);
+  GJX_REQUIRE(std::isfinite(p->step_size), "step_size must be finite");
+  GJX_REQUIRE(std::isfinite(p->alpha2) && p->alpha2 >= 0.0, "alpha2 must be finite and >= 0");
+  GJX_REQUIRE(std::isfinite(p->sigma) && p->sigma > 0.0, "sigma must be finite and > 0");
+  if (sample) GJX_REQUIRE(p->num_transitions >= 1, "num_transitions must be >= 1 (got %d)", p->num_transitions);
+  int l = p->lanes_per_chain == 0 ? default_lpc(p->dim) : p->lanes_per_chain;
+  GJX_REQUIRE(l == 4 || l == 8 || l == 16 || l == 32 || l == 64,
+              "lanes_per_chain must be 0, 4, 8, 16, 32 or 64 (got %d)", p->lanes_per_chain);
+  int e = gjx::funnel_pick_epl(p->dim, l);
+  if (e < 0)
+    return fail(GJX_EUNSUPPORTED, "dim %d needs more than 16 coordinates per lane at %d lanes per chain",
+                p->dim, l);
+  *lpc = l;
+  *epl = e;
+  return GJX_OK;
+}
+
+gjx::FunnelArgs base_args(const gjx_funnel_params* p) {
+  gjx::FunnelArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.C = p->num_chains;
+  a.D = p->dim;
+  a.L = p->num_integration_steps;
+  a.max_iters = p->max_iters;
+  a.eps = p->step_size;
+  a.alpha2 = p->alpha2;
+  a.sigma = p->sigma;
+  a.conv_tol = p->convergence_tol;
+  a.div_tol = p->divergence_tol;
+  a.div_threshold = p->divergence_threshold;
+  a.seed = p->seed;
+  a.counter = p->counter;
+  return a;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* gjx_version(void) { return "geomjax_amd 0.1.0 (gfx950)"; }
+
+const char* gjx_last_error(void) { return g_last_error.c_str(); }
+
+int gjx_rmhmc_monge_funnel_step(const gjx_funnel_params* params, const gjx_rmhmc_step_io* io, void* stream) {
+  int lpc = 0, epl = 0;
+  int rc = check_funnel_params(params, false, &lpc, &epl);
+  if (rc != GJX_OK) return rc;
+  GJX_REQUIRE(io != nullptr, "io is NULL");
+  GJX_REQUIRE(io->position && io->logdensity, "position and logdensity are required");
+  GJX_REQUIRE(io->new_position && io->new_logdensity, "new_position and new_logdensity are required");
+  GJX_REQUIRE(!io->new_logdensity_grad || io->logdensity_grad,
+              "new_logdensity_grad needs logdensity_grad (copied through on rejection)");
+  GJX_REQUIRE((io->z == nullptr) == (io->u == nullptr), "z and u must both be given or both be NULL");
+  gjx::FunnelArgs a = base_args(params);
+  a.T = 1;
+  a.q_in = io->position;
+  a.logp_in = io->logdensity;
+  a.g_in = io->logdensity_grad;
+  a.z = io->z;
+  a.u = io->u;
+  a.q_out = io->new_position;
+  a.logp_out = io->new_logdensity;
+  a.g_out = io->new_logdensity_grad;
+  a.momentum = io->momentum;
+  a.acc_rate = io->acceptance_rate;
+  a.accepted = io->is_accepted;
+  a.divergent = io->is_divergent;
+  a.energy = io->energy;
+  a.prop_q = io->proposal_position;
+  a.prop_p = io->proposal_momentum;
+  a.prop_v = io->proposal_velocity;
+  a.prop_logp = io->proposal_logdensity;
+  a.prop_g = io->proposal_logdensity_grad;
+  a.prop_weight = io->proposal_weight;
+  a.prop_slpa = io->proposal_sum_log_p_accept;
+  a.fp_iters = io->fp_iters;
+  a.fp_norm = io->fp_norm;
+  a.fp_success = io->fp_success;
+  hipError_t e = gjx::launch_funnel_rmhmc(a, params->dtype, lpc, epl, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "gjx_rmhmc_monge_funnel_step launch");
+  return GJX_OK;
+}
+
+int gjx_rmhmc_monge_funnel_sample(const gjx_funnel_params* params, const gjx_rmhmc_sample_io* io,
+                                  void* stream) {
+  int lpc = 0, epl = 0;
+  int rc = check_funnel_params(params, true, &lpc, &epl);
+  if (rc != GJX_OK) return rc;
+  GJX_REQUIRE(io != nullptr, "io is NULL");
+  GJX_REQUIRE(io->position && io->logdensity, "position and logdensity are required");
+  GJX_REQUIRE(io->final_position && io->final_logdensity, "final_position and final_logdensity are required");
+  GJX_REQUIRE((io->z == nullptr) == (io->u == nullptr), "z and u must both be given or both be NULL");
+  gjx::FunnelArgs a = base_args(params);
+  a.T = params->num_transitions;
+  a.q_in = io->position;
+  a.logp_in = io->logdensity;
+  a.z = io->z;
+  a.u = io->u;
+  a.final_q = io->final_position;
+  a.final_logp = io->final_logdensity;
+  a.final_g = io->final_logdensity_grad;
+  a.samples = io->samples;
+  a.sample_acc = io->acceptance_rate;
+  a.num_accepted = io->num_accepted;
+  a.num_fevals = io->num_fevals;
+  hipError_t e = gjx::launch_funnel_rmhmc(a, params->dtype, lpc, epl, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "gjx_rmhmc_monge_funnel_sample launch");
+  return GJX_OK;
+}
+
+int gjx_funnel_value_and_grad(int32_t num_chains, int32_t dim, double sigma, int32_t dtype, const void* position,
+                              double* logdensity, void* logdensity_grad, void* stream) {
+  GJX_REQUIRE(num_chains > 0, "num_chains must be > 0 (got %d)", num_chains);
+  GJX_REQUIRE(dim >= 2, "the funnel needs dim >= 2 (got %d)", dim);
+  GJX_REQUIRE(dtype == GJX_F64 || dtype == GJX_F32, "dtype must be GJX_F64 or GJX_F32 (got %d)", dtype);
+  GJX_REQUIRE(std::isfinite(sigma) && sigma > 0.0, "sigma must be finite and > 0");
+  GJX_REQUIRE(position && logdensity && logdensity_grad, "position, logdensity and logdensity_grad are required");
+  hipError_t e = gjx::launch_funnel_value_and_grad(num_chains, dim, sigma, dtype, position, logdensity,
+                                                   logdensity_grad, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "gjx_funnel_value_and_grad launch");
+  return GJX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,284 @@
+"""RMHMC with the implicit midpoint integrator, on the HIP kernels.
+
+Mirrors the reference's public surface (geomjax/rmhmc/rmhmc.py:30-311 and
+geomjax/rmhmc/integrators.py:25-156): ``RMHMCState``, ``RMHMCInfo``,
+``IntegratorState``, ``FixedPointIterationInfo``, ``init``, ``build_kernel``
+returning ``kernel(rng_key, state, logdensity_fn, step_size, metric_fn,
+num_integration_steps)``, and the ``rmhmc`` algorithm class whose
+``__new__`` returns ``SamplingAlgorithm(init, step)``.  Keyword names are the
+reference's, so callers that pass them by keyword (the adaptation drivers,
+geomjax/adaptation/step_size_adaptation.py:147-153) work unchanged.
+
+Differences forced by the GPU (documented, not silent):
+  * states are batched: ``position`` is [C, D] (a [D] vector is one chain);
+  * ``logdensity_fn`` / ``metric_fn`` are model and metric descriptors
+    (``geomjax_amd.models``, ``geomjax_amd.metrics``), not callables;
+  * ``rng_key`` is a ``geomjax_amd.random.HostNoise`` or ``PRNGKey``.
+The fixed-point solver's keywords (``max_iters``, ``convergence_tol``,
+``divergence_tol``) are bound with ``functools.partial(implicit_midpoint,
+...)`` exactly as with the reference.
+"""
+import functools
+from typing import NamedTuple
+
+import torch
+
+from . import _lib
+from .base import SamplingAlgorithm
+from .metrics import MongeMetric
+from .models import NealFunnel, _dtype_code
+from .proposal import Proposal
+from .random import HostNoise, PRNGKey
+
+__all__ = ["RMHMCState", "RMHMCInfo", "IntegratorState", "FixedPointIterationInfo", "init", "build_kernel",
+           "rmhmc", "implicit_midpoint"]
+
+
+class IntegratorState(NamedTuple):  # geomjax/rmhmc/integrators.py:25-36
+    position: torch.Tensor
+    momentum: torch.Tensor
+    velocity: torch.Tensor
+    logdensity: torch.Tensor
+    logdensity_grad: torch.Tensor
+
+
+class FixedPointIterationInfo(NamedTuple):  # geomjax/rmhmc/integrators.py:47-50
+    success: torch.Tensor
+    norm: torch.Tensor
+    iters: torch.Tensor
+
+
+class RMHMCState(NamedTuple):  # geomjax/rmhmc/rmhmc.py:30-41
+    position: torch.Tensor
+    logdensity: torch.Tensor
+    logdensity_grad: torch.Tensor
+
+
+class RMHMCInfo(NamedTuple):  # geomjax/rmhmc/rmhmc.py:58-93
+    momentum: torch.Tensor
+    acceptance_rate: torch.Tensor
+    is_accepted: torch.Tensor
+    is_divergent: torch.Tensor
+    energy: torch.Tensor
+    proposal: Proposal
+    num_integration_steps: int
+
+
+_SOLVER_DEFAULTS = dict(convergence_tol=1e-6, divergence_tol=1e10, max_iters=100)
+
+
+def implicit_midpoint(logdensity_fn=None, kinetic_energy_fn=None, inverse_metric_vector_product=None, *,
+                      solver=None, return_info=False, **solver_kwargs):
+    """Marker for the implicit midpoint integrator (integrators.py:92-156).
+
+    The integration itself runs inside the HIP kernel; this function only
+    exists so that ``functools.partial(implicit_midpoint, max_iters=8)`` binds
+    solver keywords the way it does in the reference.
+    """
+    raise TypeError("implicit_midpoint is executed inside the HIP kernel; pass it to build_kernel/rmhmc")
+
+
+def _solver_kwargs(integrator):
+    kw = dict(_SOLVER_DEFAULTS)
+    if integrator is implicit_midpoint:
+        return kw
+    if isinstance(integrator, functools.partial) and integrator.func is implicit_midpoint:
+        extra = dict(integrator.keywords)
+        for k in ("solver", "return_info"):
+            extra.pop(k, None)
+        unknown = set(extra) - set(_SOLVER_DEFAULTS)
+        if unknown:
+            raise TypeError(f"unknown solver keywords {sorted(unknown)}")
+        kw.update(extra)
+        return kw
+    raise NotImplementedError("only implicit_midpoint (optionally functools.partial-bound) is implemented on the GPU")
+
+
+def _as_batch(t):
+    return (t[None], True) if t.dim() == 1 else (t, False)
+
+
+def init(position, logdensity_fn):
+    """geomjax/rmhmc/rmhmc.py:96-98: value and gradient of the log density."""
+    if not isinstance(logdensity_fn, NealFunnel):
+        raise NotImplementedError(f"no HIP log density for {logdensity_fn!r}")
+    logdensity, grad = logdensity_fn.value_and_grad(position)
+    return RMHMCState(position, logdensity, grad)
+
+
+def _check_problem(logdensity_fn, metric_fn, position):
+    if not isinstance(logdensity_fn, NealFunnel):
+        raise NotImplementedError(f"no HIP RMHMC kernel for logdensity_fn={logdensity_fn!r}")
+    if not isinstance(metric_fn, MongeMetric):
+        raise NotImplementedError(f"no HIP RMHMC kernel for metric_fn={metric_fn!r}")
+    if position.shape[-1] != logdensity_fn.D:
+        raise ValueError(f"position has {position.shape[-1]} coordinates, the model has D={logdensity_fn.D}")
+    if not position.is_cuda:
+        raise ValueError("RMHMC states must live on a HIP device (cuda:N)")
+
+
+def _params(C, D, dtype_code, step_size, L, solver, alpha2, sigma, divergence_threshold, T=1,
+            lanes_per_chain=0, seed=0, counter=0):
+    return _lib.FunnelParams(
+        num_chains=C, dim=D, dtype=dtype_code, num_integration_steps=int(L),
+        max_iters=int(solver["max_iters"]), num_transitions=int(T), lanes_per_chain=int(lanes_per_chain),
+        reserved=0, step_size=float(step_size), alpha2=float(alpha2), sigma=float(sigma),
+        convergence_tol=float(solver["convergence_tol"]), divergence_tol=float(solver["divergence_tol"]),
+        divergence_threshold=float(divergence_threshold), seed=int(seed), counter=int(counter))
+
+
+def build_kernel(integrator=implicit_midpoint, divergence_threshold: float = 1000, *,
+                 return_fixed_point_info=False, lanes_per_chain=0):
+    """geomjax/rmhmc/rmhmc.py:110-176.
+
+    With ``return_fixed_point_info=True`` the kernel returns a third element,
+    ``FixedPointIterationInfo`` of shape [C, num_integration_steps] (the info
+    the reference computes at integrators.py:88 and discards at :153).
+    """
+    solver = _solver_kwargs(integrator)
+
+    def kernel(rng_key, state, logdensity_fn, step_size, metric_fn, num_integration_steps):
+        position, single = _as_batch(state.position)
+        _check_problem(logdensity_fn, metric_fn, position)
+        position = position.contiguous()
+        dev = position.device
+        C, D = position.shape
+        logdensity = state.logdensity.reshape(C).to(torch.float64).contiguous()
+        grad = state.logdensity_grad.reshape(C, D).to(position.dtype).contiguous()
+        L = int(num_integration_steps)
+        if isinstance(rng_key, HostNoise):
+            z = rng_key.z.reshape(C, D).to(device=dev, dtype=position.dtype).contiguous()
+            u = rng_key.u.reshape(C).to(device=dev, dtype=torch.float64).contiguous()
+            seed, counter = 0, 0
+        elif isinstance(rng_key, PRNGKey):
+            z = u = None
+            seed, counter = rng_key.seed, rng_key.counter
+        else:
+            raise TypeError("rng_key must be geomjax_amd.random.HostNoise or PRNGKey")
+
+        def vec():
+            return torch.empty(C, D, dtype=position.dtype, device=dev)
+
+        def sca(dt=torch.float64):
+            return torch.empty(C, dtype=dt, device=dev)
+
+        out = dict(
+            new_position=vec(), new_logdensity=sca(), new_logdensity_grad=vec(), momentum=vec(),
+            acceptance_rate=sca(), is_accepted=sca(torch.uint8), is_divergent=sca(torch.uint8), energy=sca(),
+            proposal_position=vec(), proposal_momentum=vec(), proposal_velocity=vec(), proposal_logdensity=sca(),
+            proposal_logdensity_grad=vec(), proposal_weight=sca(), proposal_sum_log_p_accept=sca(),
+            fp_iters=torch.empty(C, max(L, 1), dtype=torch.int32, device=dev),
+            fp_norm=torch.empty(C, max(L, 1), dtype=torch.float64, device=dev),
+            fp_success=torch.empty(C, max(L, 1), dtype=torch.uint8, device=dev),
+        )
+        io = _lib.RmhmcStepIO(position=position.data_ptr(), logdensity=logdensity.data_ptr(),
+                              logdensity_grad=grad.data_ptr(), z=_lib.ptr(z), u=_lib.ptr(u),
+                              **{k: v.data_ptr() for k, v in out.items()})
+        params = _params(C, D, _dtype_code(position), step_size, L, solver, metric_fn.alpha2, logdensity_fn.sigma,
+                         divergence_threshold, lanes_per_chain=lanes_per_chain, seed=seed, counter=counter)
+        rc = _lib.load().gjx_rmhmc_monge_funnel_step(params, io, _lib.stream_handle(dev))
+        _lib.check(rc, "gjx_rmhmc_monge_funnel_step")
+
+        def sq(t):
+            return t[0] if single else t
+
+        new_state = RMHMCState(sq(out["new_position"]), sq(out["new_logdensity"]), sq(out["new_logdensity_grad"]))
+        pstate = IntegratorState(sq(out["proposal_position"]), sq(out["proposal_momentum"]),
+                                 sq(out["proposal_velocity"]), sq(out["proposal_logdensity"]),
+                                 sq(out["proposal_logdensity_grad"]))
+        proposal = Proposal(pstate, sq(out["energy"]), sq(out["proposal_weight"]),
+                            sq(out["proposal_sum_log_p_accept"]))
+        info = RMHMCInfo(sq(out["momentum"]), sq(out["acceptance_rate"]), sq(out["is_accepted"].bool()),
+                         sq(out["is_divergent"].bool()), sq(out["energy"]), proposal, L)
+        if return_fixed_point_info:
+            fp = FixedPointIterationInfo(sq(out["fp_success"][:, :L].bool()), sq(out["fp_norm"][:, :L]),
+                                         sq(out["fp_iters"][:, :L]))
+            return new_state, info, fp
+        return new_state, info
+
+    kernel.solver_kwargs = solver
+    kernel.divergence_threshold = divergence_threshold
+    return kernel
+
+
+def sample(rng_key, state, logdensity_fn, step_size, metric_fn, num_integration_steps, num_samples, *,
+           integrator=implicit_midpoint, divergence_threshold=1000, store_samples=True, lanes_per_chain=0):
+    """``num_samples`` transitions of every chain in ONE persistent launch.
+
+    The reference runs this loop as ``jax.lax.scan`` over ``jax.vmap(kernel)``
+    (examples/funnel/main.py:16-25); here the chain state stays in registers
+    across transitions.  ``rng_key`` is a PRNGKey (device Philox; transition t
+    uses counter ``key.counter + t``) or HostNoise with z [T, C, D], u [T, C].
+    Returns ``SampleResult``.
+    """
+    from .sampling import SampleResult
+
+    solver = _solver_kwargs(integrator)
+    position, single = _as_batch(state.position)
+    if single:
+        raise ValueError("sample() runs a batch of chains: position must be [C, D]")
+    _check_problem(logdensity_fn, metric_fn, position)
+    position = position.contiguous()
+    dev = position.device
+    C, D = position.shape
+    T = int(num_samples)
+    if T < 1:
+        raise ValueError("num_samples must be >= 1")
+    logdensity = state.logdensity.reshape(C).to(torch.float64).contiguous()
+    if isinstance(rng_key, HostNoise):
+        z = rng_key.z.reshape(T, C, D).to(device=dev, dtype=position.dtype).contiguous()
+        u = rng_key.u.reshape(T, C).to(device=dev, dtype=torch.float64).contiguous()
+        seed, counter = 0, 0
+    elif isinstance(rng_key, PRNGKey):
+        z = u = None
+        seed, counter = rng_key.seed, rng_key.counter
+    else:
+        raise TypeError("rng_key must be geomjax_amd.random.HostNoise or PRNGKey")
+    final_q = torch.empty_like(position)
+    final_l = torch.empty(C, dtype=torch.float64, device=dev)
+    final_g = torch.empty_like(position)
+    samples = torch.empty(T, C, D, dtype=position.dtype, device=dev) if store_samples else None
+    acc = torch.empty(T, C, dtype=torch.float32, device=dev)
+    n_acc = torch.empty(C, dtype=torch.int32, device=dev)
+    n_fev = torch.empty(C, dtype=torch.int64, device=dev)
+    io = _lib.RmhmcSampleIO(position=position.data_ptr(), logdensity=logdensity.data_ptr(), z=_lib.ptr(z),
+                            u=_lib.ptr(u), final_position=final_q.data_ptr(), final_logdensity=final_l.data_ptr(),
+                            final_logdensity_grad=final_g.data_ptr(), samples=_lib.ptr(samples),
+                            acceptance_rate=acc.data_ptr(), num_accepted=n_acc.data_ptr(),
+                            num_fevals=n_fev.data_ptr())
+    params = _params(C, D, _dtype_code(position), step_size, num_integration_steps, solver, metric_fn.alpha2,
+                     logdensity_fn.sigma, divergence_threshold, T=T, lanes_per_chain=lanes_per_chain, seed=seed,
+                     counter=counter)
+    rc = _lib.load().gjx_rmhmc_monge_funnel_sample(params, io, _lib.stream_handle(dev))
+    _lib.check(rc, "gjx_rmhmc_monge_funnel_sample")
+    return SampleResult(RMHMCState(final_q, final_l, final_g), samples, acc, n_acc, n_fev)
+
+
+class rmhmc:
+    """geomjax/rmhmc/rmhmc.py:247-311.
+
+    The returned ``SamplingAlgorithm`` also carries ``sample_fn(rng_key,
+    state, num_samples, **kw)``, the persistent multi-transition launch used
+    by ``geomjax_amd.sampling.run_chains``.
+    """
+
+    init = staticmethod(init)
+    build_kernel = staticmethod(build_kernel)
+
+    def __new__(cls, logdensity_fn, step_size, metric_fn, num_integration_steps, *,
+                divergence_threshold: int = 1000, integrator=implicit_midpoint) -> SamplingAlgorithm:
+        from .sampling import BoundAlgorithm
+
+        kernel = cls.build_kernel(integrator, divergence_threshold)
+
+        def init_fn(position):
+            return cls.init(position, logdensity_fn)
+
+        def step_fn(rng_key, state):
+            return kernel(rng_key, state, logdensity_fn, step_size, metric_fn, num_integration_steps)
+
+        def sample_fn(rng_key, state, num_samples, **kw):
+            return sample(rng_key, state, logdensity_fn, step_size, metric_fn, num_integration_steps, num_samples,
+                          integrator=integrator, divergence_threshold=divergence_threshold, **kw)
+
+        return BoundAlgorithm(init_fn, step_fn, sample_fn)

@@ -1,0 +1,144 @@
+/*
+ * geomjax_amd C ABI: the Riemannian integrator hot path of geomjax as
+ * hand-written gfx950 HIP kernels.
+ *
+ * Conventions
+ *  - Every buffer is device memory owned by the caller.  The library never
+ *    allocates on these entry points and never frees caller memory.
+ *  - Work is enqueued on `stream` (a hipStream_t passed as void*; NULL is
+ *    the null stream).  No entry point synchronises the device.
+ *  - Chain-major layouts: a [C, D] array holds chain c's vector at c*D.
+ *  - "state" arrays (positions, momenta, velocities, gradients, samples)
+ *    have the element type given by `dtype` (GJX_F64 or GJX_F32); every
+ *    per-chain scalar (log densities, energies, weights, acceptance
+ *    probabilities, uniforms) is float64 in both precisions.
+ *  - Return value: 0 on success, a negative GJX_E* code otherwise; the text
+ *    of the last error on the calling thread is gjx_last_error().
+ *  - Numerical failure never raises, as in the reference: a NaN energy
+ *    difference becomes -inf and the proposal is rejected
+ *    (geomjax/mcmc/proposal.py:107-108).
+ *  - Re-entrant; calls on different streams may overlap.  hipGraph capture of
+ *    these calls is legal (they only launch kernels).
+ */
+#ifndef GEOMJAX_AMD_H
+#define GEOMJAX_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum gjx_dtype { GJX_F64 = 0, GJX_F32 = 1 };
+
+enum gjx_status {
+  GJX_OK = 0,
+  GJX_EINVAL = -1,      /* invalid argument (shape, pointer, parameter) */
+  GJX_EUNSUPPORTED = -2,/* size outside the compiled tile configurations */
+  GJX_EHIP = -3         /* a HIP runtime call failed */
+};
+
+/* Sizes and scalars of one RMHMC call on Neal's funnel with the Monge metric
+ * G(q) = I + alpha2 * grad l(q) grad l(q)^T.
+ * Mirrors the arguments of the reference kernel
+ *   kernel(rng_key, state, logdensity_fn, step_size, metric_fn,
+ *          num_integration_steps)            geomjax/rmhmc/rmhmc.py:131-138
+ * with the fixed-point solver's keywords            (integrators.py:57-59)
+ * and build_kernel's divergence_threshold            (rmhmc.py:110-113). */
+typedef struct {
+  int32_t num_chains;            /* C */
+  int32_t dim;                   /* D >= 2; coordinate D-1 is the log-scale v */
+  int32_t dtype;                 /* enum gjx_dtype of the state arrays */
+  int32_t num_integration_steps; /* L >= 0 */
+  int32_t max_iters;             /* fixed-point iteration cap (>= 0) */
+  int32_t num_transitions;       /* T >= 1 (sample entry points) */
+  int32_t lanes_per_chain;       /* 0 = library default; else 4,8,16,32 or 64 */
+  int32_t reserved;
+  double step_size;              /* epsilon */
+  double alpha2;                 /* Monge warp alpha^2 >= 0 */
+  double sigma;                  /* funnel scale of v (examples/funnel/main.py:29) */
+  double convergence_tol;        /* 1e-6 in the reference */
+  double divergence_tol;         /* 1e10 in the reference */
+  double divergence_threshold;   /* 1000 in the reference */
+  uint64_t seed;                 /* device PRNG key (when z == NULL) */
+  uint64_t counter;              /* device PRNG transition offset */
+} gjx_funnel_params;
+
+/* Buffers of one RMHMC transition (gjx_rmhmc_monge_funnel_step).
+ * Inputs are RMHMCState (geomjax/rmhmc/rmhmc.py:30-41); outputs are the new
+ * RMHMCState and RMHMCInfo (rmhmc.py:58-93) with its Proposal
+ * (geomjax/mcmc/proposal.py:22-40) over IntegratorState
+ * (geomjax/rmhmc/integrators.py:25-36).  Optional outputs may be NULL. */
+typedef struct {
+  const void* position;          /* [C, D] dtype */
+  const double* logdensity;      /* [C] */
+  const void* logdensity_grad;   /* [C, D] dtype */
+  const void* z;                 /* [C, D] dtype standard normals, or NULL: device PRNG */
+  const double* u;               /* [C] uniforms in [0,1), or NULL: device PRNG */
+  void* new_position;            /* [C, D] dtype */
+  double* new_logdensity;        /* [C] */
+  void* new_logdensity_grad;     /* [C, D] dtype */
+  void* momentum;                /* [C, D] info.momentum (the p0 drawn), optional */
+  double* acceptance_rate;       /* [C] optional */
+  uint8_t* is_accepted;          /* [C] optional */
+  uint8_t* is_divergent;         /* [C] optional */
+  double* energy;                /* [C] optional: energy of the proposal */
+  void* proposal_position;       /* [C, D] optional */
+  void* proposal_momentum;       /* [C, D] optional (flipped) */
+  void* proposal_velocity;       /* [C, D] optional (flipped) */
+  double* proposal_logdensity;   /* [C] optional */
+  void* proposal_logdensity_grad;/* [C, D] optional */
+  double* proposal_weight;       /* [C] optional */
+  double* proposal_sum_log_p_accept; /* [C] optional */
+  int32_t* fp_iters;             /* [C, L] optional: FixedPointIterationInfo.iters */
+  double* fp_norm;               /* [C, L] optional: .norm */
+  uint8_t* fp_success;           /* [C, L] optional: .success */
+} gjx_rmhmc_step_io;
+
+/* Buffers of T consecutive transitions in one persistent launch
+ * (gjx_rmhmc_monge_funnel_sample): the reference's user loop
+ * jax.lax.scan(jax.vmap(kernel)) (examples/funnel/main.py:16-25). */
+typedef struct {
+  const void* position;          /* [C, D] dtype initial positions */
+  const double* logdensity;      /* [C] */
+  const void* z;                 /* [T, C, D] dtype, or NULL: device PRNG */
+  const double* u;               /* [T, C], or NULL: device PRNG */
+  void* final_position;          /* [C, D] dtype */
+  double* final_logdensity;      /* [C] */
+  void* final_logdensity_grad;   /* [C, D] dtype, optional */
+  void* samples;                 /* [T, C, D] dtype positions after each transition, optional */
+  float* acceptance_rate;        /* [T, C] optional */
+  int32_t* num_accepted;         /* [C] optional: accepted transitions */
+  int64_t* num_fevals;           /* [C] optional: evaluations of the vector field executed */
+} gjx_rmhmc_sample_io;
+
+/* One RMHMC transition per chain.  Replaces the reference kernel
+ * geomjax/rmhmc/rmhmc.py:131-174 (momentum draw :161, velocity :162,
+ * rmhmc_proposal.generate :416-438) with implicit_midpoint
+ * (geomjax/rmhmc/integrators.py:92-156) and metric_fn = Monge on the funnel
+ * logdensity (examples/funnel/main.py:34-39). */
+int gjx_rmhmc_monge_funnel_step(const gjx_funnel_params* params, const gjx_rmhmc_step_io* io,
+                                void* stream);
+
+/* T transitions per chain in one launch; state stays in registers between
+ * transitions.  Same per-transition semantics as gjx_rmhmc_monge_funnel_step. */
+int gjx_rmhmc_monge_funnel_sample(const gjx_funnel_params* params, const gjx_rmhmc_sample_io* io,
+                                  void* stream);
+
+/* Log density and its gradient of Neal's funnel for C positions: the
+ * jax.value_and_grad(logdensity_fn)(position) of the reference's init
+ * (geomjax/rmhmc/rmhmc.py:96-98, geomjax/lmcmonge/lmc.py:101-109) with the
+ * funnel of examples/funnel/main.py:34-39. */
+int gjx_funnel_value_and_grad(int32_t num_chains, int32_t dim, double sigma, int32_t dtype,
+                              const void* position, double* logdensity, void* logdensity_grad,
+                              void* stream);
+
+/* Library identification and diagnostics. */
+const char* gjx_version(void);
+const char* gjx_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GEOMJAX_AMD_H */
