@@ -3,7 +3,7 @@
 Public names follow the reference package (geomjax/__init__.py:1-58) for the
 samplers this build serves.
 """
-from . import metrics, models, random
+from . import diagnostics, metrics, models, random, sampling
 from .base import SamplingAlgorithm
 from .metrics import MongeMetric
 from .models import NealFunnel
@@ -11,5 +11,8 @@ from .rmhmc import rmhmc
 
 __version__ = "0.1.0"
 
-__all__ = ["rmhmc", "SamplingAlgorithm", "NealFunnel", "MongeMetric", "metrics", "models", "random",
-           "__version__"]
+ess = diagnostics.effective_sample_size
+rhat = diagnostics.potential_scale_reduction
+
+__all__ = ["rmhmc", "SamplingAlgorithm", "NealFunnel", "MongeMetric", "metrics", "models", "random", "sampling",
+           "diagnostics", "ess", "rhat", "__version__"]

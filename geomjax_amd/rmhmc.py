@@ -202,14 +202,16 @@ def build_kernel(integrator=implicit_midpoint, divergence_threshold: float = 100
 
 
 def sample(rng_key, state, logdensity_fn, step_size, metric_fn, num_integration_steps, num_samples, *,
-           integrator=implicit_midpoint, divergence_threshold=1000, store_samples=True, lanes_per_chain=0):
+           integrator=implicit_midpoint, divergence_threshold=1000, store_samples=True, lanes_per_chain=0,
+           samples_out=None):
     """``num_samples`` transitions of every chain in ONE persistent launch.
 
     The reference runs this loop as ``jax.lax.scan`` over ``jax.vmap(kernel)``
     (examples/funnel/main.py:16-25); here the chain state stays in registers
     across transitions.  ``rng_key`` is a PRNGKey (device Philox; transition t
     uses counter ``key.counter + t``) or HostNoise with z [T, C, D], u [T, C].
-    Returns ``SampleResult``.
+    ``samples_out`` (a contiguous [T, C, D] tensor of the state dtype) receives
+    the samples in place of a new allocation.  Returns ``SampleResult``.
     """
     from .sampling import SampleResult
 
@@ -237,7 +239,12 @@ def sample(rng_key, state, logdensity_fn, step_size, metric_fn, num_integration_
     final_q = torch.empty_like(position)
     final_l = torch.empty(C, dtype=torch.float64, device=dev)
     final_g = torch.empty_like(position)
-    samples = torch.empty(T, C, D, dtype=position.dtype, device=dev) if store_samples else None
+    if samples_out is not None:
+        if samples_out.shape != (T, C, D) or samples_out.dtype != position.dtype or not samples_out.is_contiguous():
+            raise ValueError(f"samples_out must be a contiguous [{T}, {C}, {D}] {position.dtype} tensor")
+        samples = samples_out
+    else:
+        samples = torch.empty(T, C, D, dtype=position.dtype, device=dev) if store_samples else None
     acc = torch.empty(T, C, dtype=torch.float32, device=dev)
     n_acc = torch.empty(C, dtype=torch.int32, device=dev)
     n_fev = torch.empty(C, dtype=torch.int64, device=dev)
