@@ -7,6 +7,7 @@ from . import diagnostics, metrics, models, random, sampling
 from .base import SamplingAlgorithm
 from .metrics import MongeMetric
 from .models import NealFunnel
+from .lmcmonge import lmc as lmcmonge
 from .rmhmc import rmhmc
 
 __version__ = "0.1.0"
@@ -14,5 +15,5 @@ __version__ = "0.1.0"
 ess = diagnostics.effective_sample_size
 rhat = diagnostics.potential_scale_reduction
 
-__all__ = ["rmhmc", "SamplingAlgorithm", "NealFunnel", "MongeMetric", "metrics", "models", "random", "sampling",
+__all__ = ["rmhmc", "lmcmonge", "SamplingAlgorithm", "NealFunnel", "MongeMetric", "metrics", "models", "random", "sampling",
            "diagnostics", "ess", "rhat", "__version__"]

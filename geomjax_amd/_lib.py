@@ -35,7 +35,7 @@ class FunnelParams(ctypes.Structure):
         ("max_iters", _c_int32),
         ("num_transitions", _c_int32),
         ("lanes_per_chain", _c_int32),
-        ("reserved", _c_int32),
+        ("flags", _c_int32),
         ("step_size", _c_double),
         ("alpha2", _c_double),
         ("sigma", _c_double),
@@ -66,10 +66,35 @@ class RmhmcSampleIO(ctypes.Structure):
     )]
 
 
+class LmcStepIO(ctypes.Structure):
+    _fields_ = [(n, _vp) for n in (
+        "inverse_mass_matrix", "position", "logdensity", "logdensity_grad", "volume_adjustment", "z", "u",
+        "new_position", "new_logdensity", "new_logdensity_grad", "new_volume_adjustment",
+        "velocity", "acceptance_rate", "is_accepted", "is_divergent", "energy",
+        "proposal_position", "proposal_momentum", "proposal_velocity", "proposal_logdensity",
+        "proposal_logdensity_grad_norm", "proposal_dl_ig", "proposal_Hdl_ig", "proposal_ig_Hdl_ig",
+        "proposal_logdensity_hvp_velocity_norm", "proposal_determinant_metric", "proposal_volume_adjustment",
+        "proposal_weight", "proposal_sum_log_p_accept",
+    )]
+
+
+class LmcSampleIO(ctypes.Structure):
+    _fields_ = [(n, _vp) for n in (
+        "inverse_mass_matrix", "position", "logdensity", "volume_adjustment", "z", "u",
+        "final_position", "final_logdensity", "final_logdensity_grad", "final_volume_adjustment",
+        "samples", "acceptance_rate", "num_accepted",
+    )]
+
+
+LMC_EXACT_SHERMAN_MORRISON = 1
+LMC_EXACT_MOMENTUM = 2
+
 # entry point name -> (argtypes) ; every entry point returns int
 _SIGNATURES = {
     "gjx_rmhmc_monge_funnel_step": [ctypes.POINTER(FunnelParams), ctypes.POINTER(RmhmcStepIO), _vp],
     "gjx_rmhmc_monge_funnel_sample": [ctypes.POINTER(FunnelParams), ctypes.POINTER(RmhmcSampleIO), _vp],
+    "gjx_lmcmonge_funnel_step": [ctypes.POINTER(FunnelParams), ctypes.POINTER(LmcStepIO), _vp],
+    "gjx_lmcmonge_funnel_sample": [ctypes.POINTER(FunnelParams), ctypes.POINTER(LmcSampleIO), _vp],
     "gjx_funnel_value_and_grad": [_c_int32, _c_int32, _c_double, _c_int32, _vp, _vp, _vp, _vp],
 }
 _STRING_FUNCS = ("gjx_version", "gjx_last_error")

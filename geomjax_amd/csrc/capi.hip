@@ -8,6 +8,7 @@
 #include <string>
 
 #include "../../include/geomjax_amd.h"
+#include "funnel_lmc.h"
 #include "funnel_rmhmc.h"
 
 namespace {
@@ -170,6 +171,107 @@ int gjx_funnel_value_and_grad(int32_t num_chains, int32_t dim, double sigma, int
   hipError_t e = gjx::launch_funnel_value_and_grad(num_chains, dim, sigma, dtype, position, logdensity,
                                                    logdensity_grad, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "gjx_funnel_value_and_grad launch");
+  return GJX_OK;
+}
+
+namespace {
+
+gjx::FunnelLmcArgs lmc_args(const gjx_funnel_params* p) {
+  gjx::FunnelLmcArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.C = p->num_chains;
+  a.D = p->dim;
+  a.L = p->num_integration_steps;
+  a.exact_sm = (p->flags & GJX_LMC_EXACT_SHERMAN_MORRISON) ? 1 : 0;
+  a.exact_momentum = (p->flags & GJX_LMC_EXACT_MOMENTUM) ? 1 : 0;
+  a.eps = p->step_size;
+  a.alpha2 = p->alpha2;
+  a.sigma = p->sigma;
+  a.div_threshold = p->divergence_threshold;
+  a.seed = p->seed;
+  a.counter = p->counter;
+  return a;
+}
+
+}  // namespace
+
+int gjx_lmcmonge_funnel_step(const gjx_funnel_params* params, const gjx_lmc_step_io* io, void* stream) {
+  int lpc = 0, epl = 0;
+  int rc = check_funnel_params(params, false, &lpc, &epl);
+  if (rc != GJX_OK) return rc;
+  GJX_REQUIRE((params->flags & ~3) == 0, "unknown flags 0x%x", params->flags);
+  GJX_REQUIRE(io != nullptr, "io is NULL");
+  GJX_REQUIRE(io->inverse_mass_matrix, "inverse_mass_matrix is required");
+  GJX_REQUIRE(io->position && io->logdensity && io->volume_adjustment,
+              "position, logdensity and volume_adjustment are required");
+  GJX_REQUIRE(io->new_position && io->new_logdensity, "new_position and new_logdensity are required");
+  GJX_REQUIRE(!io->new_logdensity_grad || io->logdensity_grad,
+              "new_logdensity_grad needs logdensity_grad (copied through on rejection)");
+  GJX_REQUIRE((io->z == nullptr) == (io->u == nullptr), "z and u must both be given or both be NULL");
+  gjx::FunnelLmcArgs a = lmc_args(params);
+  a.T = 1;
+  a.inv_mass = io->inverse_mass_matrix;
+  a.q_in = io->position;
+  a.logp_in = io->logdensity;
+  a.g_in = io->logdensity_grad;
+  a.J_in = io->volume_adjustment;
+  a.z = io->z;
+  a.u = io->u;
+  a.q_out = io->new_position;
+  a.logp_out = io->new_logdensity;
+  a.g_out = io->new_logdensity_grad;
+  a.J_out = io->new_volume_adjustment;
+  a.velocity = io->velocity;
+  a.acc_rate = io->acceptance_rate;
+  a.accepted = io->is_accepted;
+  a.divergent = io->is_divergent;
+  a.energy = io->energy;
+  a.prop_q = io->proposal_position;
+  a.prop_p = io->proposal_momentum;
+  a.prop_v = io->proposal_velocity;
+  a.prop_logp = io->proposal_logdensity;
+  a.prop_gnorm = io->proposal_logdensity_grad_norm;
+  a.prop_dl_ig = io->proposal_dl_ig;
+  a.prop_Hdl_ig = io->proposal_Hdl_ig;
+  a.prop_ig_Hdl_ig = io->proposal_ig_Hdl_ig;
+  a.prop_hvn = io->proposal_logdensity_hvp_velocity_norm;
+  a.prop_L = io->proposal_determinant_metric;
+  a.prop_J = io->proposal_volume_adjustment;
+  a.prop_weight = io->proposal_weight;
+  a.prop_slpa = io->proposal_sum_log_p_accept;
+  hipError_t e = gjx::launch_funnel_lmc(a, params->dtype, lpc, epl, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "gjx_lmcmonge_funnel_step launch");
+  return GJX_OK;
+}
+
+int gjx_lmcmonge_funnel_sample(const gjx_funnel_params* params, const gjx_lmc_sample_io* io, void* stream) {
+  int lpc = 0, epl = 0;
+  int rc = check_funnel_params(params, true, &lpc, &epl);
+  if (rc != GJX_OK) return rc;
+  GJX_REQUIRE((params->flags & ~3) == 0, "unknown flags 0x%x", params->flags);
+  GJX_REQUIRE(io != nullptr, "io is NULL");
+  GJX_REQUIRE(io->inverse_mass_matrix, "inverse_mass_matrix is required");
+  GJX_REQUIRE(io->position && io->logdensity && io->volume_adjustment,
+              "position, logdensity and volume_adjustment are required");
+  GJX_REQUIRE(io->final_position && io->final_logdensity, "final_position and final_logdensity are required");
+  GJX_REQUIRE((io->z == nullptr) == (io->u == nullptr), "z and u must both be given or both be NULL");
+  gjx::FunnelLmcArgs a = lmc_args(params);
+  a.T = params->num_transitions;
+  a.inv_mass = io->inverse_mass_matrix;
+  a.q_in = io->position;
+  a.logp_in = io->logdensity;
+  a.J_in = io->volume_adjustment;
+  a.z = io->z;
+  a.u = io->u;
+  a.final_q = io->final_position;
+  a.final_logp = io->final_logdensity;
+  a.final_g = io->final_logdensity_grad;
+  a.final_J = io->final_volume_adjustment;
+  a.samples = io->samples;
+  a.sample_acc = io->acceptance_rate;
+  a.num_accepted = io->num_accepted;
+  hipError_t e = gjx::launch_funnel_lmc(a, params->dtype, lpc, epl, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "gjx_lmcmonge_funnel_sample launch");
   return GJX_OK;
 }
 

@@ -141,4 +141,21 @@ enum : uint32_t {
 template <class T> struct is_f64 { static constexpr bool value = false; };
 template <> struct is_f64<double> { static constexpr bool value = true; };
 
+constexpr double kLog2Pi = 1.8378770664093454835606594728112;
+
+// max that propagates NaN (jnp.max does; a plain a > b ? a : b would drop it)
+template <class R>
+__device__ __forceinline__ R nan_max(R a, R b) {
+  return (a > b || a != a) ? a : b;
+}
+
+struct NanMax {
+  template <class T> __device__ __forceinline__ static T f(T a, T b) { return nan_max(a, b); }
+};
+
+template <class T>
+__device__ __forceinline__ T ld(const void* p, size_t i) { return static_cast<const T*>(p)[i]; }
+template <class T>
+__device__ __forceinline__ void st(void* p, size_t i, T v) { static_cast<T*>(p)[i] = v; }
+
 }  // namespace gjx

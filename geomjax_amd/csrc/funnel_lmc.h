@@ -1,0 +1,59 @@
+// Internal launch interface of the LMC-Monge funnel kernel (not part of the
+// public C ABI; capi.hip validates and forwards).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gjx {
+
+struct FunnelLmcArgs {
+  int C, D, L, T;
+  int exact_sm;        // 1: Sherman-Morrison correction with its alpha2 factor (reference omits it)
+  int exact_momentum;  // 1: momentum = M v + alpha2 (v.g) g (reference scales the rank-1 term by L)
+  double eps, alpha2, sigma, div_threshold;
+  uint64_t seed, counter;
+  const void* inv_mass;  // [D] dtype, diagonal inverse mass matrix
+  // inputs: LMCState
+  const void* q_in;
+  const double* logp_in;
+  const void* g_in;
+  const double* J_in;
+  const void* z;    // [T, C, D] or null
+  const double* u;  // [T, C] or null
+  // single-transition outputs
+  void* q_out;
+  double* logp_out;
+  void* g_out;
+  double* J_out;
+  void* velocity;  // info.velocity (v0)
+  double* acc_rate;
+  uint8_t* accepted;
+  uint8_t* divergent;
+  double* energy;
+  void* prop_q;
+  void* prop_p;
+  void* prop_v;
+  double* prop_logp;
+  void* prop_gnorm;
+  void* prop_dl_ig;
+  void* prop_Hdl_ig;
+  void* prop_ig_Hdl_ig;
+  void* prop_hvn;
+  double* prop_L;
+  double* prop_J;
+  double* prop_weight;
+  double* prop_slpa;
+  // multi-transition outputs
+  void* final_q;
+  double* final_logp;
+  void* final_g;
+  double* final_J;
+  void* samples;
+  float* sample_acc;
+  int32_t* num_accepted;
+};
+
+hipError_t launch_funnel_lmc(const FunnelLmcArgs& a, int dtype, int lpc, int epl, hipStream_t stream);
+
+}  // namespace gjx

@@ -1,0 +1,473 @@
+// Lagrangian Monte Carlo with the Monge metric (lmcmonge) on Neal's funnel.
+//
+// Reference: geomjax/lmcmonge/lmc.py:130-237,472-565 (transition, proposal,
+// flip), geomjax/lmcmonge/integrators.py:52-194 (lan_integrator,
+// half_step_omega), geomjax/lmcmonge/metrics.py:104-284 (velocity draw,
+// kinetic energy, weighted gradients, momentum), examples/funnel/main.py:34-39.
+//
+// Metric G = M + alpha2 g g^T with diagonal M = diag(1/mi) (mi = the
+// reference's inverse_mass_matrix), g = grad l, L = 1 + alpha2 g^T M^-1 g.
+// With x = q[0:D-1], v = q[D-1], e = exp(-v) every vector of the half step
+// is a per-coordinate expression in (x_k, vel_k, mi_k) and per-chain scalars:
+//   ghat_x = -e x / sqrt(L)              (logdensity_grad_norm)
+//   dl_x = mi ghat_x                     (dl_ig)
+//   Hdl_x = e (-dl_x + x dl_v) / sqrt(L) (Hdl_ig: funnel Hessian times dl_ig)
+//   igHdl_x = mi Hdl_x                   (ig_Hdl_ig)
+//   hv_x = e (-vel_x + x vel_v) / sqrt(L)  (logdensity_hvp_velocity_norm)
+// and the dot products half_step_omega needs are linear in the velocity, so
+// one group reduction of eight sums per half step gives all of them:
+//   v_t.hv  = v.hv - (eps/2) sqrt(L) (dphi_ig.hv - alpha2 (dphi.dl) (dl.hv))
+//   Hdl.v'  = Hdl.v - (eps/2) sqrt(L) (Hdl.dphi_ig - alpha2 (dphi.dl)(Hdl.dl)) - c (Hdl.dl).
+// The velocity draw v0 = chol(M^-1 - alpha2 dl dl^T) z uses the exact lower
+// factor of a diagonal-minus-rank-one matrix:
+//   a_j = sqrt(alpha2) dl_j / sqrt(mi_j),  t_j = 1 - sum_{k<=j} a_k^2,
+//   v_j = sqrt(mi_j) (sqrt(t_j/t_{j-1}) z_j - a_j sum_{k<j} a_k z_k / sqrt(t_k t_{k-1})).
+#include "common.h"
+#include "funnel_lmc.h"
+
+namespace gjx {
+
+namespace {
+
+template <class R>
+struct PosScalars {
+  R e, Sx, Smx, gv, L, sL, isL, dlv, Hdlv, igHdlv;
+};
+
+template <class R>
+__device__ __forceinline__ PosScalars<R> pos_scalars(R v, R Sx, R Smx, R a2, R inv_s2, R halfDx, R miv) {
+  PosScalars<R> P;
+  P.e = exp(-v);
+  P.Sx = Sx;
+  P.Smx = Smx;
+  P.gv = -v * inv_s2 - halfDx + R(0.5) * P.e * Sx;
+  P.L = R(1) + a2 * (P.e * P.e * Smx + miv * P.gv * P.gv);  // normalizing_constant (metrics.py:193-197)
+  P.sL = sqrt(P.L);
+  P.isL = R(1) / P.sL;
+  P.dlv = miv * P.gv * P.isL;
+  // Hdl_v = (e (x . dl_x) - (1/sigma^2 + e Sx / 2) dl_v) / sqrt(L),  x . dl_x = -e Smx / sqrt(L)
+  P.Hdlv = (-P.e * P.e * Smx * P.isL - (inv_s2 + R(0.5) * P.e * Sx) * P.dlv) * P.isL;
+  P.igHdlv = miv * P.Hdlv;
+  return P;
+}
+
+// One half_step_omega (geomjax/lmcmonge/integrators.py:158-194) at the
+// position described by P, updating the velocity (vx, vv) and dJ.
+template <class R, int LPC, int EPL>
+__device__ __forceinline__ void half_step(const PosScalars<R>& P, const R (&x)[EPL], const R (&mi)[EPL], R (&vx)[EPL],
+                                          R& vv, double& dJ, R a2, R inv_s2, R h, bool exact_sm) {
+  R dl[EPL], dphig[EPL];
+  R s0 = R(0), s1 = R(0), s2 = R(0), s3 = R(0), s4 = R(0), s5 = R(0), s6 = R(0), s7 = R(0);
+#pragma unroll
+  for (int k = 0; k < EPL; ++k) {
+    const R gn = -P.e * x[k] * P.isL;
+    dl[k] = mi[k] * gn;
+    const R Hdl = P.e * (-dl[k] + x[k] * P.dlv) * P.isL;
+    const R igH = mi[k] * Hdl;
+    const R dphi = -gn + a2 * Hdl * P.isL;
+    dphig[k] = -dl[k] + a2 * igH * P.isL;
+    const R hv = P.e * (-vx[k] + x[k] * vv) * P.isL;
+    s0 += hv * dl[k];
+    s1 += dphi * dl[k];
+    s2 += x[k] * vx[k];
+    s3 += vx[k] * hv;
+    s4 += dphig[k] * hv;
+    s5 += Hdl * vx[k];
+    s6 += Hdl * dphig[k];
+    s7 += Hdl * dl[k];
+  }
+  s0 = group_sum<LPC>(s0);
+  s1 = group_sum<LPC>(s1);
+  s2 = group_sum<LPC>(s2);
+  s3 = group_sum<LPC>(s3);
+  s4 = group_sum<LPC>(s4);
+  s5 = group_sum<LPC>(s5);
+  s6 = group_sum<LPC>(s6);
+  s7 = group_sum<LPC>(s7);
+  // log-scale coordinate contributions
+  const R hv_v = (P.e * s2 - (inv_s2 + R(0.5) * P.e * P.Sx) * vv) * P.isL;
+  const R gn_v = P.gv * P.isL;
+  const R dphi_v = -gn_v + a2 * P.Hdlv * P.isL;
+  const R dphig_v = -P.dlv + a2 * P.igHdlv * P.isL;
+  const R P1 = s0 + hv_v * P.dlv;     // hv . dl
+  const R P2 = s1 + dphi_v * P.dlv;   // dphi . dl
+  const R P3 = s3 + vv * hv_v;        // v . hv
+  const R P4 = s4 + dphig_v * hv_v;   // dphi_ig . hv
+  const R P5 = s5 + P.Hdlv * vv;      // Hdl . v
+  const R P6 = s6 + P.Hdlv * dphig_v; // Hdl . dphi_ig
+  const R P7 = s7 + P.Hdlv * P.dlv;   // Hdl . dl
+  // 1st volume adjustment (:178-179)
+  const R d1 = R(1) + h * a2 * P1;
+  dJ -= log(fabs((double)d1));
+  // v_t = v - (eps/2) sqrt(L) (dphi_ig - alpha2 (dphi.dl) dl)   (:182-185)
+  const R vtH = P3 - h * P.sL * (P4 - a2 * P2 * P1);
+  R coef = h * vtH / d1;  // (:186-189); the reference has no alpha2 here
+  if (exact_sm) coef *= a2;
+  const R HdlVn = P5 - h * P.sL * (P6 - a2 * P2 * P7) - coef * P7;
+  // 2nd volume adjustment (:192)
+  dJ += log(fabs((double)(R(1) - h * a2 * HdlVn)));
+  const R kq = h * P.sL, kd = a2 * P2;
+#pragma unroll
+  for (int k = 0; k < EPL; ++k) vx[k] = vx[k] - kq * (dphig[k] - kd * dl[k]) - coef * dl[k];
+  vv = vv - kq * (dphig_v - kd * P.dlv) - coef * P.dlv;
+}
+
+}  // namespace
+
+template <class T, int LPC, int EPL, bool STEP>
+__global__ void __launch_bounds__(256) funnel_lmc_kernel(const FunnelLmcArgs a) {
+  using R = T;
+  const int tid = blockIdx.x * 256 + threadIdx.x;
+  const int chain = tid / LPC;
+  const int lg = tid % LPC;
+  const bool live = chain < a.C;
+  const size_t c = live ? (size_t)chain : 0;
+  const int D = a.D, Dx = a.D - 1;
+  const int j0 = lg * EPL;
+  const R eps = R(a.eps);
+  const R h = R(0.5 * a.eps);
+  const R a2 = R(a.alpha2);
+  const R alpha = R(sqrt(a.alpha2));
+  const R inv_s2 = R(1.0 / (a.sigma * a.sigma));
+  const R halfDx = R(0.5 * Dx);
+  const bool exact_sm = a.exact_sm != 0;
+  const int T_steps = STEP ? 1 : a.T;
+#define GJX_ON(k) (j0 + (k) < Dx)
+
+  // diagonal inverse mass (the reference's inverse_mass_matrix, "fixed" power)
+  R mi[EPL];
+#pragma unroll
+  for (int k = 0; k < EPL; ++k) mi[k] = GJX_ON(k) ? ld<T>(a.inv_mass, j0 + k) : R(1);
+  const R miv = ld<T>(a.inv_mass, Dx);
+  double sum_log_mi;  // sum_j log(inverse mass_j) = -sum_j log(mass_j)
+  {
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < EPL; ++k)
+      if (GJX_ON(k)) s += log((double)mi[k]);
+    sum_log_mi = group_sum<LPC>(s) + log((double)miv);
+  }
+
+  // chain state: LMCState(position, logdensity, logdensity_grad, volume_adjustment)
+  R x[EPL];
+  R v;
+  {
+    const size_t base = c * D;
+#pragma unroll
+    for (int k = 0; k < EPL; ++k) x[k] = (live && GJX_ON(k)) ? ld<T>(a.q_in, base + j0 + k) : R(0);
+    v = live ? ld<T>(a.q_in, base + Dx) : R(0);
+  }
+  double logp = live ? a.logp_in[c] : 0.0;
+  double J = live ? a.J_in[c] : 0.0;
+  int n_acc = 0;
+
+  for (int t = 0; t < T_steps; ++t) {
+    // ---- position quantities at q0 (lmc.py:177-200) ----
+    PosScalars<R> P;
+    R loc_inc[EPL];
+    {
+      R sx = R(0), smx = R(0);
+#pragma unroll
+      for (int k = 0; k < EPL; ++k) {
+        sx += x[k] * x[k];
+        smx += mi[k] * x[k] * x[k];
+        loc_inc[k] = smx;
+      }
+      const R Smx_lane = smx;
+      const R off = group_exclusive_scan<LPC>(Smx_lane, lg);
+#pragma unroll
+      for (int k = 0; k < EPL; ++k) loc_inc[k] += off;  // inclusive prefix of mi x^2 over coordinates
+      sx = group_sum<LPC>(sx);
+      smx = group_sum<LPC>(smx);
+      P = pos_scalars<R>(v, sx, smx, a2, inv_s2, halfDx, miv);
+    }
+
+    // ---- noise ----
+    R z[EPL];
+    R zv;
+    double uacc;
+    {
+      const uint64_t tt = a.counter + (uint64_t)t;
+      const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
+      if (a.z != nullptr) {
+        const size_t base = ((size_t)t * a.C + c) * D;
+#pragma unroll
+        for (int k = 0; k < EPL; ++k) z[k] = (live && GJX_ON(k)) ? ld<T>(a.z, base + j0 + k) : R(0);
+        zv = live ? ld<T>(a.z, base + Dx) : R(0);
+        uacc = live ? a.u[(size_t)t * a.C + c] : 1.0;
+      } else {
+#pragma unroll
+        for (int k = 0; k < EPL; k += 2) {
+          u32x4 r = philox4x32_10(
+              u32x4{(uint32_t)c, (uint32_t)tt, kStreamMomentum + (uint32_t)(j0 + k), (uint32_t)(tt >> 32)}, k0, k1);
+          R n0, n1;
+          normal2(r, n0, n1);
+          z[k] = GJX_ON(k) ? n0 : R(0);
+          if (k + 1 < EPL) z[k + 1] = GJX_ON(k + 1) ? n1 : R(0);
+        }
+        {
+          u32x4 r = philox4x32_10(u32x4{(uint32_t)c, (uint32_t)tt, kStreamMomentumV, (uint32_t)(tt >> 32)}, k0, k1);
+          R n0, n1;
+          normal2(r, n0, n1);
+          zv = n0;
+        }
+        u32x4 r = philox4x32_10(u32x4{(uint32_t)c, (uint32_t)tt, kStreamAccept, (uint32_t)(tt >> 32)}, k0, k1);
+        uacc = u01_f64(r.x, r.y) - 0.5 * (1.0 / 9007199254740992.0);
+      }
+    }
+
+    // ---- velocity v0 = chol(M^-1 - alpha2 dl dl^T) z  (metrics.py:155-166) ----
+    R vx[EPL];
+    R vv;
+    {
+      // a_j^2 = alpha2 dl_j^2 / mi_j = (alpha2 e^2 / L) mi_j x_j^2
+      const R w = a2 * P.e * P.e / P.L;
+      R cj[EPL];
+      R acc2 = R(0);
+#pragma unroll
+      for (int k = 0; k < EPL; ++k) {
+        const R tj = R(1) - w * loc_inc[k];
+        const R tjm = R(1) - w * (loc_inc[k] - mi[k] * x[k] * x[k]);
+        const R aj = -alpha * P.e * sqrt(mi[k]) * x[k] * P.isL;
+        cj[k] = aj * z[k] / sqrt(tj * tjm);
+        vx[k] = sqrt(tj / tjm) * z[k];
+        acc2 += cj[k];
+      }
+      const R off2 = group_exclusive_scan<LPC>(acc2, lg);
+      const R tot2 = group_sum<LPC>(acc2);
+      R run = off2;
+#pragma unroll
+      for (int k = 0; k < EPL; ++k) {
+        const R aj = -alpha * P.e * sqrt(mi[k]) * x[k] * P.isL;
+        vx[k] = GJX_ON(k) ? sqrt(mi[k]) * (vx[k] - aj * run) : R(0);
+        run += cj[k];
+      }
+      // t after all x coordinates is 1 - alpha2 e^2 Smx / L; then the log-scale coordinate
+      const R txe = R(1) - w * P.Smx;
+      const R av = alpha * sqrt(miv) * P.gv * P.isL;
+      const R tv = txe - av * av;
+      vv = sqrt(miv) * (sqrt(tv / txe) * zv - av * tot2);
+    }
+    if (STEP && a.velocity != nullptr && live) {
+#pragma unroll
+      for (int k = 0; k < EPL; ++k)
+        if (GJX_ON(k)) st<T>(a.velocity, c * D + j0 + k, vx[k]);
+      if (lg == 0) st<T>(a.velocity, c * D + Dx, vv);
+    }
+
+    // ---- kinetic energy (metrics.py:168-186), float64 sums ----
+    auto kinetic = [&](const PosScalars<R>& Q, const R (&xx)[EPL], const R (&vel)[EPL], R velv) -> double {
+      double svm = 0.0, xv = 0.0;
+#pragma unroll
+      for (int k = 0; k < EPL; ++k) {
+        if (GJX_ON(k)) svm += (double)vel[k] * (double)vel[k] / (double)mi[k];
+        xv += (double)xx[k] * (double)vel[k];
+      }
+      svm = group_sum<LPC>(svm) + (double)velv * (double)velv / (double)miv;
+      xv = group_sum<LPC>(xv);
+      const double vg = -(double)Q.e * xv + (double)velv * (double)Q.gv;  // v . g
+      return -0.5 * (log((double)Q.L) - sum_log_mi) + 0.5 * svm + 0.5 * a.alpha2 * vg * vg;
+    };
+    const double K0 = kinetic(P, x, vx, vv);
+
+    // ---- trajectory (trajectory.py:121-139 over lan_integrator.one_step) ----
+    R xc[EPL];
+    R vc = v;
+#pragma unroll
+    for (int k = 0; k < EPL; ++k) xc[k] = x[k];
+    double dJ = 0.0;
+    for (int s = 0; s < a.L; ++s) {
+      half_step<R, LPC, EPL>(P, xc, mi, vx, vv, dJ, a2, inv_s2, h, exact_sm);
+      R sx = R(0), smx = R(0);
+#pragma unroll
+      for (int k = 0; k < EPL; ++k) {
+        xc[k] = xc[k] + eps * vx[k];
+        sx += xc[k] * xc[k];
+        smx += mi[k] * xc[k] * xc[k];
+      }
+      vc = vc + eps * vv;
+      sx = group_sum<LPC>(sx);
+      smx = group_sum<LPC>(smx);
+      P = pos_scalars<R>(vc, sx, smx, a2, inv_s2, halfDx, miv);
+      half_step<R, LPC, EPL>(P, xc, mi, vx, vv, dJ, a2, inv_s2, h, exact_sm);
+    }
+
+    // ---- energy at the end (lmcmonge_energy, metrics.py:267-284) ----
+    const double K1 = kinetic(P, xc, vx, vv);
+    double logp1;
+    {
+      double d1 = 0.0;
+#pragma unroll
+      for (int k = 0; k < EPL; ++k) d1 += (double)xc[k] * (double)xc[k];
+      const double Sx = group_sum<LPC>(d1);
+      const double vd = (double)vc, s2 = a.sigma * a.sigma;
+      logp1 = -0.5 * vd * vd / s2 - log(a.sigma) - 0.5 * kLog2Pi - 0.5 * exp(-vd) * Sx - 0.5 * (double)Dx * vd -
+              0.5 * (double)Dx * kLog2Pi;
+    }
+    const double J1 = J + dJ;
+    const double H1 = -logp1 + K1 - J1;
+    double dH = (K0 - logp) - (K1 - logp1) + dJ;  // H0 - H1 with the carried J cancelled exactly
+    if (isnan(dH)) dH = -INFINITY;
+    const double p_acc = fmin(exp(dH), 1.0);
+    const bool accept = uacc < p_acc;
+
+    if (STEP && live) {
+      if (lg == 0) {
+        if (a.acc_rate) a.acc_rate[c] = p_acc;
+        if (a.accepted) a.accepted[c] = (uint8_t)accept;
+        if (a.divergent) a.divergent[c] = (uint8_t)(-dH > a.div_threshold);
+        if (a.energy) a.energy[c] = H1;
+        if (a.prop_logp) a.prop_logp[c] = logp1;
+        if (a.prop_L) a.prop_L[c] = (double)P.L;
+        if (a.prop_J) a.prop_J[c] = J1;
+        if (a.prop_weight) a.prop_weight[c] = dH;
+        if (a.prop_slpa) a.prop_slpa[c] = fmin(dH, 0.0);
+      }
+      // v . g and x . v at the end for the momentum and hv
+      R xv = R(0);
+#pragma unroll
+      for (int k = 0; k < EPL; ++k) xv += xc[k] * vx[k];
+      xv = group_sum<LPC>(xv);
+      const R vg = -P.e * xv + vv * P.gv;
+      const R mom_coef = a.exact_momentum ? a2 * vg : a2 * P.L * vg;  // metrics.py:243-255 (extra L)
+      const R hv_v = (P.e * xv - (inv_s2 + R(0.5) * P.e * P.Sx) * vv) * P.isL;
+#pragma unroll
+      for (int k = 0; k < EPL; ++k) {
+        if (!GJX_ON(k)) continue;
+        const size_t i = c * D + j0 + k;
+        const R gx = -P.e * xc[k];
+        const R gn = gx * P.isL;
+        const R dl = mi[k] * gn;
+        const R Hdl = P.e * (-dl + xc[k] * P.dlv) * P.isL;
+        if (a.prop_q) st<T>(a.prop_q, i, xc[k]);
+        if (a.prop_p) st<T>(a.prop_p, i, -(vx[k] / mi[k] + mom_coef * gx));
+        if (a.prop_v) st<T>(a.prop_v, i, -vx[k]);
+        if (a.prop_gnorm) st<T>(a.prop_gnorm, i, gn);
+        if (a.prop_dl_ig) st<T>(a.prop_dl_ig, i, dl);
+        if (a.prop_Hdl_ig) st<T>(a.prop_Hdl_ig, i, Hdl);
+        if (a.prop_ig_Hdl_ig) st<T>(a.prop_ig_Hdl_ig, i, mi[k] * Hdl);
+        if (a.prop_hvn) st<T>(a.prop_hvn, i, P.e * (-vx[k] + xc[k] * vv) * P.isL);
+      }
+      if (lg == 0) {
+        const size_t i = c * D + Dx;
+        if (a.prop_q) st<T>(a.prop_q, i, vc);
+        if (a.prop_p) st<T>(a.prop_p, i, -(vv / miv + mom_coef * P.gv));
+        if (a.prop_v) st<T>(a.prop_v, i, -vv);
+        if (a.prop_gnorm) st<T>(a.prop_gnorm, i, P.gv * P.isL);
+        if (a.prop_dl_ig) st<T>(a.prop_dl_ig, i, P.dlv);
+        if (a.prop_Hdl_ig) st<T>(a.prop_Hdl_ig, i, P.Hdlv);
+        if (a.prop_ig_Hdl_ig) st<T>(a.prop_ig_Hdl_ig, i, P.igHdlv);
+        if (a.prop_hvn) st<T>(a.prop_hvn, i, hv_v);
+      }
+      // new LMCState (lmc.py:226-233): logdensity_grad = ghat * sqrt(L) = g
+#pragma unroll
+      for (int k = 0; k < EPL; ++k) {
+        if (!GJX_ON(k)) continue;
+        const size_t i = c * D + j0 + k;
+        st<T>(a.q_out, i, accept ? xc[k] : x[k]);
+        if (a.g_out) st<T>(a.g_out, i, accept ? (-P.e * xc[k] * P.isL) * P.sL : ld<T>(a.g_in, i));
+      }
+      if (lg == 0) {
+        const size_t i = c * D + Dx;
+        st<T>(a.q_out, i, accept ? vc : v);
+        if (a.g_out) st<T>(a.g_out, i, accept ? (P.gv * P.isL) * P.sL : ld<T>(a.g_in, i));
+        a.logp_out[c] = accept ? logp1 : logp;
+        if (a.J_out) a.J_out[c] = accept ? J1 : J;
+      }
+    }
+
+    if (accept) {
+#pragma unroll
+      for (int k = 0; k < EPL; ++k) x[k] = xc[k];
+      v = vc;
+      logp = logp1;
+      J = J1;
+      ++n_acc;
+    }
+    if (!STEP) {
+      if (a.samples != nullptr && live) {
+        const size_t base = ((size_t)t * a.C + c) * D;
+#pragma unroll
+        for (int k = 0; k < EPL; ++k)
+          if (GJX_ON(k)) st<T>(a.samples, base + j0 + k, x[k]);
+        if (lg == 0) st<T>(a.samples, base + Dx, v);
+      }
+      if (a.sample_acc != nullptr && live && lg == 0) a.sample_acc[(size_t)t * a.C + c] = (float)p_acc;
+    }
+  }
+
+  if (!STEP) {
+    if (live) {
+#pragma unroll
+      for (int k = 0; k < EPL; ++k)
+        if (GJX_ON(k)) st<T>(a.final_q, c * D + j0 + k, x[k]);
+      if (lg == 0) {
+        st<T>(a.final_q, c * D + Dx, v);
+        a.final_logp[c] = logp;
+        if (a.final_J) a.final_J[c] = J;
+        if (a.num_accepted) a.num_accepted[c] = n_acc;
+      }
+    }
+    if (a.final_g != nullptr) {
+      R s1 = R(0);
+#pragma unroll
+      for (int k = 0; k < EPL; ++k) s1 += x[k] * x[k];
+      const R sxf = group_sum<LPC>(s1);
+      const R e = exp(-v);
+      const R gv = -v * inv_s2 - halfDx + R(0.5) * e * sxf;
+      if (live) {
+#pragma unroll
+        for (int k = 0; k < EPL; ++k)
+          if (GJX_ON(k)) st<T>(a.final_g, c * D + j0 + k, -e * x[k]);
+        if (lg == 0) st<T>(a.final_g, c * D + Dx, gv);
+      }
+    }
+  }
+#undef GJX_ON
+}
+
+namespace {
+
+template <class T, int LPC, int EPL>
+hipError_t launch_one(const FunnelLmcArgs& a, hipStream_t stream) {
+  const long long lanes = (long long)a.C * LPC;
+  const unsigned blocks = (unsigned)((lanes + 255) / 256);
+  if (a.q_out != nullptr)
+    hipLaunchKernelGGL((funnel_lmc_kernel<T, LPC, EPL, true>), dim3(blocks), dim3(256), 0, stream, a);
+  else
+    hipLaunchKernelGGL((funnel_lmc_kernel<T, LPC, EPL, false>), dim3(blocks), dim3(256), 0, stream, a);
+  return hipGetLastError();
+}
+
+template <class T, int LPC>
+hipError_t launch_epl(const FunnelLmcArgs& a, int epl, hipStream_t stream) {
+  switch (epl) {
+    case 1: return launch_one<T, LPC, 1>(a, stream);
+    case 2: return launch_one<T, LPC, 2>(a, stream);
+    case 4: return launch_one<T, LPC, 4>(a, stream);
+    case 8: return launch_one<T, LPC, 8>(a, stream);
+    case 16: return launch_one<T, LPC, 16>(a, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <class T>
+hipError_t launch_lpc(const FunnelLmcArgs& a, int lpc, int epl, hipStream_t stream) {
+  switch (lpc) {
+    case 4: return launch_epl<T, 4>(a, epl, stream);
+    case 8: return launch_epl<T, 8>(a, epl, stream);
+    case 16: return launch_epl<T, 16>(a, epl, stream);
+    case 32: return launch_epl<T, 32>(a, epl, stream);
+    case 64: return launch_epl<T, 64>(a, epl, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace
+
+hipError_t launch_funnel_lmc(const FunnelLmcArgs& a, int dtype, int lpc, int epl, hipStream_t stream) {
+  if (dtype == 0) return launch_lpc<double>(a, lpc, epl, stream);
+  return launch_lpc<float>(a, lpc, epl, stream);
+}
+
+}  // namespace gjx

@@ -35,8 +35,6 @@ namespace gjx {
 
 namespace {
 
-constexpr double kLog2Pi = 1.8378770664093454835606594728112;
-
 // Per-chain scalar coefficients of the vector field at one iterate.
 template <class R>
 struct FieldCoef {
@@ -69,20 +67,6 @@ __device__ __forceinline__ FieldCoef<R> field_coef(R v, R pv, R sx, R xpx, R a2,
   f.dHv = k * hwv - gv;
   return f;
 }
-
-template <class R>
-__device__ __forceinline__ R nan_max(R a, R b) {
-  return (a > b || a != a) ? a : b;
-}
-
-struct NanMax {
-  template <class T> __device__ __forceinline__ static T f(T a, T b) { return nan_max(a, b); }
-};
-
-template <class T>
-__device__ __forceinline__ T ld(const void* p, size_t i) { return static_cast<const T*>(p)[i]; }
-template <class T>
-__device__ __forceinline__ void st(void* p, size_t i, T v) { static_cast<T*>(p)[i] = v; }
 
 }  // namespace
 

@@ -18,3 +18,8 @@ class MongeMetric:
 
     def __repr__(self):
         return f"MongeMetric(alpha2={self.alpha2})"
+
+
+def _check_alpha2(alpha2):
+    if not float(alpha2) >= 0:
+        raise ValueError(f"alpha2 must be >= 0 (got {alpha2})")

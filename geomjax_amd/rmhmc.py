@@ -122,7 +122,7 @@ def _params(C, D, dtype_code, step_size, L, solver, alpha2, sigma, divergence_th
     return _lib.FunnelParams(
         num_chains=C, dim=D, dtype=dtype_code, num_integration_steps=int(L),
         max_iters=int(solver["max_iters"]), num_transitions=int(T), lanes_per_chain=int(lanes_per_chain),
-        reserved=0, step_size=float(step_size), alpha2=float(alpha2), sigma=float(sigma),
+        flags=0, step_size=float(step_size), alpha2=float(alpha2), sigma=float(sigma),
         convergence_tol=float(solver["convergence_tol"]), divergence_tol=float(solver["divergence_tol"]),
         divergence_threshold=float(divergence_threshold), seed=int(seed), counter=int(counter))
 

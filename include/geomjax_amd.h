@@ -31,6 +31,14 @@ extern "C" {
 
 enum gjx_dtype { GJX_F64 = 0, GJX_F32 = 1 };
 
+/* Bits of gjx_funnel_params.flags for the LMC-Monge entry points.  0 reproduces
+ * the reference exactly, including two algebra slips:
+ *  - half_step_omega's Sherman-Morrison correction lacks alpha2
+ *    (geomjax/lmcmonge/integrators.py:186-189 vs :178, :192);
+ *  - the momentum's rank-one term carries an extra factor L
+ *    (geomjax/lmcmonge/lmc.py:202-204, metrics.py:243-255). */
+enum gjx_lmc_flags { GJX_LMC_EXACT_SHERMAN_MORRISON = 1, GJX_LMC_EXACT_MOMENTUM = 2 };
+
 enum gjx_status {
   GJX_OK = 0,
   GJX_EINVAL = -1,      /* invalid argument (shape, pointer, parameter) */
@@ -53,7 +61,7 @@ typedef struct {
   int32_t max_iters;             /* fixed-point iteration cap (>= 0) */
   int32_t num_transitions;       /* T >= 1 (sample entry points) */
   int32_t lanes_per_chain;       /* 0 = library default; else 4,8,16,32 or 64 */
-  int32_t reserved;
+  int32_t flags;                 /* LMC-Monge only: GJX_LMC_EXACT_* bits; 0 = as the reference */
   double step_size;              /* epsilon */
   double alpha2;                 /* Monge warp alpha^2 >= 0 */
   double sigma;                  /* funnel scale of v (examples/funnel/main.py:29) */
@@ -124,6 +132,70 @@ int gjx_rmhmc_monge_funnel_step(const gjx_funnel_params* params, const gjx_rmhmc
  * transitions.  Same per-transition semantics as gjx_rmhmc_monge_funnel_step. */
 int gjx_rmhmc_monge_funnel_sample(const gjx_funnel_params* params, const gjx_rmhmc_sample_io* io,
                                   void* stream);
+
+/* Buffers of one LMC-Monge transition (gjx_lmcmonge_funnel_step).  Inputs are
+ * LMCState (geomjax/lmcmonge/lmc.py:32-44); outputs the new LMCState and
+ * LMCInfo (:63-98) whose Proposal state is RiemannianIntegratorState
+ * (geomjax/lmcmonge/integrators.py:26-44; alpha2 is params->alpha2).
+ * Optional outputs may be NULL. */
+typedef struct {
+  const void* inverse_mass_matrix;  /* [D] dtype: diagonal inverse mass (metrics.py:44-60, "fixed") */
+  const void* position;             /* [C, D] dtype */
+  const double* logdensity;         /* [C] */
+  const void* logdensity_grad;      /* [C, D] dtype */
+  const double* volume_adjustment;  /* [C] */
+  const void* z;                    /* [C, D] dtype standard normals, or NULL: device PRNG */
+  const double* u;                  /* [C] uniforms, or NULL: device PRNG */
+  void* new_position;               /* [C, D] */
+  double* new_logdensity;           /* [C] */
+  void* new_logdensity_grad;        /* [C, D] optional */
+  double* new_volume_adjustment;    /* [C] optional */
+  void* velocity;                   /* [C, D] optional: info.velocity (the v0 drawn) */
+  double* acceptance_rate;          /* [C] optional */
+  uint8_t* is_accepted;             /* [C] optional */
+  uint8_t* is_divergent;            /* [C] optional */
+  double* energy;                   /* [C] optional */
+  void* proposal_position;          /* [C, D] optional */
+  void* proposal_momentum;          /* [C, D] optional (flipped) */
+  void* proposal_velocity;          /* [C, D] optional (flipped) */
+  double* proposal_logdensity;      /* [C] optional */
+  void* proposal_logdensity_grad_norm; /* [C, D] optional */
+  void* proposal_dl_ig;             /* [C, D] optional */
+  void* proposal_Hdl_ig;            /* [C, D] optional */
+  void* proposal_ig_Hdl_ig;         /* [C, D] optional */
+  void* proposal_logdensity_hvp_velocity_norm; /* [C, D] optional */
+  double* proposal_determinant_metric; /* [C] optional */
+  double* proposal_volume_adjustment;  /* [C] optional */
+  double* proposal_weight;          /* [C] optional */
+  double* proposal_sum_log_p_accept;/* [C] optional */
+} gjx_lmc_step_io;
+
+/* Buffers of T LMC-Monge transitions in one persistent launch. */
+typedef struct {
+  const void* inverse_mass_matrix;  /* [D] dtype */
+  const void* position;             /* [C, D] */
+  const double* logdensity;         /* [C] */
+  const double* volume_adjustment;  /* [C] */
+  const void* z;                    /* [T, C, D] or NULL */
+  const double* u;                  /* [T, C] or NULL */
+  void* final_position;             /* [C, D] */
+  double* final_logdensity;         /* [C] */
+  void* final_logdensity_grad;      /* [C, D] optional */
+  double* final_volume_adjustment;  /* [C] optional */
+  void* samples;                    /* [T, C, D] optional */
+  float* acceptance_rate;           /* [T, C] optional */
+  int32_t* num_accepted;            /* [C] optional */
+} gjx_lmc_sample_io;
+
+/* One LMC-Monge transition per chain: the reference kernel
+ * geomjax/lmcmonge/lmc.py:151-235 (velocity draw :201, lmc_proposal
+ * :512-534) with lan_integrator/half_step_omega
+ * (geomjax/lmcmonge/integrators.py:52-194) on the funnel
+ * (examples/funnel/main.py:34-39). */
+int gjx_lmcmonge_funnel_step(const gjx_funnel_params* params, const gjx_lmc_step_io* io, void* stream);
+
+/* T LMC-Monge transitions per chain in one launch. */
+int gjx_lmcmonge_funnel_sample(const gjx_funnel_params* params, const gjx_lmc_sample_io* io, void* stream);
 
 /* Log density and its gradient of Neal's funnel for C positions: the
  * jax.value_and_grad(logdensity_fn)(position) of the reference's init
