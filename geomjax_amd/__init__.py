@@ -5,8 +5,8 @@ samplers this build serves.
 """
 from . import diagnostics, metrics, models, random, sampling
 from .base import SamplingAlgorithm
-from .metrics import MongeMetric
-from .models import NealFunnel
+from .metrics import FisherMetric, MongeMetric
+from .models import LogisticBatch, LogisticRegression, NealFunnel
 from .lmcmonge import lmc as lmcmonge
 from .rmhmc import rmhmc
 
@@ -15,5 +15,5 @@ __version__ = "0.1.0"
 ess = diagnostics.effective_sample_size
 rhat = diagnostics.potential_scale_reduction
 
-__all__ = ["rmhmc", "lmcmonge", "SamplingAlgorithm", "NealFunnel", "MongeMetric", "metrics", "models", "random", "sampling",
+__all__ = ["rmhmc", "lmcmonge", "SamplingAlgorithm", "NealFunnel", "LogisticRegression", "LogisticBatch", "MongeMetric", "FisherMetric", "metrics", "models", "random", "sampling",
            "diagnostics", "ess", "rhat", "__version__"]

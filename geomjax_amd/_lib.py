@@ -86,6 +86,35 @@ class LmcSampleIO(ctypes.Structure):
     )]
 
 
+class LogisticParams(ctypes.Structure):
+    _fields_ = [
+        ("num_chains", _c_int32),
+        ("dim", _c_int32),
+        ("dtype", _c_int32),
+        ("num_integration_steps", _c_int32),
+        ("max_iters", _c_int32),
+        ("num_transitions", _c_int32),
+        ("flags", _c_int32),
+        ("reserved", _c_int32),
+        ("step_size", _c_double),
+        ("prior_var", _c_double),
+        ("convergence_tol", _c_double),
+        ("divergence_tol", _c_double),
+        ("divergence_threshold", _c_double),
+        ("seed", _c_u64),
+        ("counter", _c_u64),
+    ]
+
+
+class LogisticData(ctypes.Structure):
+    _fields_ = [
+        ("num_posteriors", _c_int32),
+        ("max_rows", _c_int32),
+        ("max_features", _c_int32),
+        ("reserved", _c_int32),
+    ] + [(n, _vp) for n in ("num_rows", "num_features", "x_offset", "y_offset", "chain_posterior", "x", "y")]
+
+
 LMC_EXACT_SHERMAN_MORRISON = 1
 LMC_EXACT_MOMENTUM = 2
 
@@ -96,6 +125,12 @@ _SIGNATURES = {
     "gjx_lmcmonge_funnel_step": [ctypes.POINTER(FunnelParams), ctypes.POINTER(LmcStepIO), _vp],
     "gjx_lmcmonge_funnel_sample": [ctypes.POINTER(FunnelParams), ctypes.POINTER(LmcSampleIO), _vp],
     "gjx_funnel_value_and_grad": [_c_int32, _c_int32, _c_double, _c_int32, _vp, _vp, _vp, _vp],
+    "gjx_rmhmc_fisher_logistic_step": [ctypes.POINTER(LogisticParams), ctypes.POINTER(LogisticData),
+                                       ctypes.POINTER(RmhmcStepIO), _vp],
+    "gjx_rmhmc_fisher_logistic_sample": [ctypes.POINTER(LogisticParams), ctypes.POINTER(LogisticData),
+                                         ctypes.POINTER(RmhmcSampleIO), _vp],
+    "gjx_logistic_value_and_grad": [ctypes.POINTER(LogisticParams), ctypes.POINTER(LogisticData), _vp, _vp, _vp,
+                                    _vp],
 }
 _STRING_FUNCS = ("gjx_version", "gjx_last_error")
 

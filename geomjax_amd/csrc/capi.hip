@@ -10,6 +10,7 @@
 #include "../../include/geomjax_amd.h"
 #include "funnel_lmc.h"
 #include "funnel_rmhmc.h"
+#include "logistic.h"
 
 namespace {
 
@@ -272,6 +273,130 @@ int gjx_lmcmonge_funnel_sample(const gjx_funnel_params* params, const gjx_lmc_sa
   a.num_accepted = io->num_accepted;
   hipError_t e = gjx::launch_funnel_lmc(a, params->dtype, lpc, epl, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "gjx_lmcmonge_funnel_sample launch");
+  return GJX_OK;
+}
+
+namespace {
+
+int check_logistic(const gjx_logistic_params* p, const gjx_logistic_data* d, bool sample, gjx::LogisticArgs* a) {
+  GJX_REQUIRE(p != nullptr && d != nullptr, "params and data are required");
+  GJX_REQUIRE(p->num_chains > 0, "num_chains must be > 0 (got %d)", p->num_chains);
+  GJX_REQUIRE(p->dtype == GJX_F64, "the logistic kernels take dtype GJX_F64 (got %d)", p->dtype);
+  GJX_REQUIRE(p->num_integration_steps >= 0, "num_integration_steps must be >= 0");
+  GJX_REQUIRE(p->max_iters >= 0, "max_iters must be >= 0");
+  GJX_REQUIRE(std::isfinite(p->step_size), "step_size must be finite");
+  GJX_REQUIRE(std::isfinite(p->prior_var) && p->prior_var > 0.0, "prior_var must be finite and > 0");
+  if (sample) GJX_REQUIRE(p->num_transitions >= 1, "num_transitions must be >= 1");
+  GJX_REQUIRE(d->num_posteriors >= 1, "num_posteriors must be >= 1");
+  GJX_REQUIRE(d->max_rows >= 1, "max_rows must be >= 1");
+  GJX_REQUIRE(d->max_features >= 1 && d->max_features <= p->dim,
+              "max_features must be in [1, dim] (got %d, dim %d)", d->max_features, p->dim);
+  if (d->max_features > gjx::kLogisticMaxD)
+    return fail(GJX_EUNSUPPORTED, "D = %d exceeds the float64 kernel's maximum of %d features", d->max_features,
+                gjx::kLogisticMaxD);
+  GJX_REQUIRE(d->num_rows && d->num_features && d->x_offset && d->y_offset && d->x && d->y,
+              "data arrays are required");
+  std::memset(a, 0, sizeof(*a));
+  a->C = p->num_chains;
+  a->L = p->num_integration_steps;
+  a->T = sample ? p->num_transitions : 1;
+  a->max_iters = p->max_iters;
+  a->Dmax = p->dim;
+  a->P = d->num_posteriors;
+  a->eps = p->step_size;
+  a->prior_var = p->prior_var;
+  a->conv_tol = p->convergence_tol;
+  a->div_tol = p->divergence_tol;
+  a->div_threshold = p->divergence_threshold;
+  a->seed = p->seed;
+  a->counter = p->counter;
+  a->chain_post = d->chain_posterior;
+  a->post_N = d->num_rows;
+  a->post_D = d->num_features;
+  a->post_xoff = d->x_offset;
+  a->post_yoff = d->y_offset;
+  a->X = static_cast<const double*>(d->x);
+  a->y = static_cast<const double*>(d->y);
+  return GJX_OK;
+}
+
+}  // namespace
+
+int gjx_rmhmc_fisher_logistic_step(const gjx_logistic_params* params, const gjx_logistic_data* data,
+                                   const gjx_rmhmc_step_io* io, void* stream) {
+  gjx::LogisticArgs a;
+  int rc = check_logistic(params, data, false, &a);
+  if (rc != GJX_OK) return rc;
+  GJX_REQUIRE(io != nullptr, "io is NULL");
+  GJX_REQUIRE(io->position && io->logdensity, "position and logdensity are required");
+  GJX_REQUIRE(io->new_position && io->new_logdensity, "new_position and new_logdensity are required");
+  GJX_REQUIRE(!io->new_logdensity_grad || io->logdensity_grad,
+              "new_logdensity_grad needs logdensity_grad (copied through on rejection)");
+  GJX_REQUIRE((io->z == nullptr) == (io->u == nullptr), "z and u must both be given or both be NULL");
+  a.q_in = io->position;
+  a.logp_in = io->logdensity;
+  a.g_in = io->logdensity_grad;
+  a.z = io->z;
+  a.u = io->u;
+  a.q_out = io->new_position;
+  a.logp_out = io->new_logdensity;
+  a.g_out = io->new_logdensity_grad;
+  a.momentum = io->momentum;
+  a.acc_rate = io->acceptance_rate;
+  a.accepted = io->is_accepted;
+  a.divergent = io->is_divergent;
+  a.energy = io->energy;
+  a.prop_q = io->proposal_position;
+  a.prop_p = io->proposal_momentum;
+  a.prop_v = io->proposal_velocity;
+  a.prop_logp = io->proposal_logdensity;
+  a.prop_g = io->proposal_logdensity_grad;
+  a.prop_weight = io->proposal_weight;
+  a.prop_slpa = io->proposal_sum_log_p_accept;
+  a.fp_iters = io->fp_iters;
+  a.fp_norm = io->fp_norm;
+  a.fp_success = io->fp_success;
+  hipError_t e = gjx::launch_logistic_rmhmc_f64(a, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "gjx_rmhmc_fisher_logistic_step launch");
+  return GJX_OK;
+}
+
+int gjx_rmhmc_fisher_logistic_sample(const gjx_logistic_params* params, const gjx_logistic_data* data,
+                                     const gjx_rmhmc_sample_io* io, void* stream) {
+  gjx::LogisticArgs a;
+  int rc = check_logistic(params, data, true, &a);
+  if (rc != GJX_OK) return rc;
+  GJX_REQUIRE(io != nullptr, "io is NULL");
+  GJX_REQUIRE(io->position && io->logdensity, "position and logdensity are required");
+  GJX_REQUIRE(io->final_position && io->final_logdensity, "final_position and final_logdensity are required");
+  GJX_REQUIRE((io->z == nullptr) == (io->u == nullptr), "z and u must both be given or both be NULL");
+  a.q_in = io->position;
+  a.logp_in = io->logdensity;
+  a.z = io->z;
+  a.u = io->u;
+  a.final_q = io->final_position;
+  a.final_logp = io->final_logdensity;
+  a.final_g = io->final_logdensity_grad;
+  a.samples = io->samples;
+  a.sample_acc = io->acceptance_rate;
+  a.num_accepted = io->num_accepted;
+  a.num_fevals = io->num_fevals;
+  hipError_t e = gjx::launch_logistic_rmhmc_f64(a, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "gjx_rmhmc_fisher_logistic_sample launch");
+  return GJX_OK;
+}
+
+int gjx_logistic_value_and_grad(const gjx_logistic_params* params, const gjx_logistic_data* data,
+                                const void* position, double* logdensity, void* logdensity_grad, void* stream) {
+  gjx::LogisticArgs a;
+  int rc = check_logistic(params, data, false, &a);
+  if (rc != GJX_OK) return rc;
+  GJX_REQUIRE(position && logdensity && logdensity_grad, "position, logdensity and logdensity_grad are required");
+  a.q_in = position;
+  a.logp_out = logdensity;
+  a.g_out = logdensity_grad;
+  hipError_t e = gjx::launch_logistic_value_and_grad(a, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "gjx_logistic_value_and_grad launch");
   return GJX_OK;
 }
 

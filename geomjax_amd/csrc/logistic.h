@@ -1,0 +1,61 @@
+// Internal launch interface of the logistic-regression kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gjx {
+
+struct LogisticArgs {
+  int C, L, T, max_iters, Dmax, P;
+  double eps, prior_var, conv_tol, div_tol, div_threshold;
+  uint64_t seed, counter;
+  // data: posteriors p = 0..P-1 with X_p [N_p, D_p] row-major at X + post_xoff[p]
+  const int32_t* chain_post;  // [C] or null (posterior 0)
+  const int32_t* post_N;
+  const int32_t* post_D;
+  const int64_t* post_xoff;
+  const int64_t* post_yoff;
+  const double* X;
+  const double* y;
+  // inputs (chain-major, row stride Dmax)
+  const void* q_in;
+  const double* logp_in;
+  const void* g_in;
+  const void* z;
+  const double* u;
+  // single-transition outputs
+  void* q_out;
+  double* logp_out;
+  void* g_out;
+  void* momentum;
+  double* acc_rate;
+  uint8_t* accepted;
+  uint8_t* divergent;
+  double* energy;
+  void* prop_q;
+  void* prop_p;
+  void* prop_v;
+  double* prop_logp;
+  void* prop_g;
+  double* prop_weight;
+  double* prop_slpa;
+  int32_t* fp_iters;
+  double* fp_norm;
+  uint8_t* fp_success;
+  // multi-transition outputs
+  void* final_q;
+  double* final_logp;
+  void* final_g;
+  void* samples;
+  float* sample_acc;
+  int32_t* num_accepted;
+  int64_t* num_fevals;
+};
+
+constexpr int kLogisticMaxD = 64;
+
+hipError_t launch_logistic_rmhmc_f64(const LogisticArgs& a, hipStream_t stream);
+hipError_t launch_logistic_value_and_grad(const LogisticArgs& a, hipStream_t stream);
+
+}  // namespace gjx

@@ -20,6 +20,18 @@ class MongeMetric:
         return f"MongeMetric(alpha2={self.alpha2})"
 
 
+class FisherMetric:
+    """The Fisher information metric of the model plus its prior precision.
+
+    For LogisticRegression: G(q) = X^T diag(s(1-s)) X + I / prior_var.
+    """
+
+    kind = "fisher"
+
+    def __repr__(self):
+        return "FisherMetric()"
+
+
 def _check_alpha2(alpha2):
     if not float(alpha2) >= 0:
         raise ValueError(f"alpha2 must be >= 0 (got {alpha2})")

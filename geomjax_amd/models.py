@@ -65,3 +65,146 @@ def funnel_value_and_grad(position, sigma):
     if squeeze:
         return logp[0], grad[0]
     return logp, grad
+
+
+class LogisticRegression:
+    """Bayesian logistic regression: y_n ~ Bernoulli(sigmoid(x_n . q)), q ~ N(0, prior_var I).
+
+    The reference leaves the model user-supplied; the log density is
+    sum_n [y_n eta_n - log(1 + e^eta_n)] - |q|^2 / (2 prior_var)
+    - (D/2) log(2 pi prior_var) with eta = X q.  ``X`` [N, D] and ``y`` [N]
+    are copied to the device once (float64).
+    """
+
+    kind = "logistic"
+
+    def __init__(self, X, y, prior_var=100.0, device="cuda:0"):
+        X = torch.as_tensor(X, dtype=torch.float64)
+        y = torch.as_tensor(y, dtype=torch.float64)
+        if X.dim() != 2 or y.shape != (X.shape[0],):
+            raise ValueError(f"X must be [N, D] and y [N] (got {tuple(X.shape)} and {tuple(y.shape)})")
+        if not prior_var > 0:
+            raise ValueError("prior_var must be > 0")
+        self.N, self.D = X.shape
+        self.prior_var = float(prior_var)
+        self.X = X.to(device).contiguous()
+        self.y = y.to(device).contiguous()
+        self._table = _posterior_table([self], None, self.X.device)
+
+    @property
+    def dim(self):
+        return self.D
+
+    def data(self, chain_posterior=None):
+        return self._table.struct(chain_posterior)
+
+    def value_and_grad(self, position):
+        return logistic_value_and_grad(self, position)
+
+    def __repr__(self):
+        return f"LogisticRegression(N={self.N}, D={self.D}, prior_var={self.prior_var})"
+
+
+class LogisticBatch:
+    """Many independent logistic-regression posteriors sampled in one launch.
+
+    ``models`` are LogisticRegression descriptors (same prior variance);
+    ``chains_per_posterior`` chains are assigned to each, in order.  States are
+    [C, Dmax] with Dmax = max D_p; coordinates past a chain's own D_p are zero
+    and stay zero.
+    """
+
+    kind = "logistic"
+
+    def __init__(self, models, chains_per_posterior):
+        if not models:
+            raise ValueError("LogisticBatch needs at least one model")
+        pv = {m.prior_var for m in models}
+        if len(pv) != 1:
+            raise ValueError("all posteriors of a batch must share prior_var")
+        self.models = list(models)
+        self.prior_var = pv.pop()
+        counts = [int(chains_per_posterior)] * len(models) if isinstance(chains_per_posterior, int) \
+            else [int(c) for c in chains_per_posterior]
+        if len(counts) != len(models) or min(counts) < 1:
+            raise ValueError("chains_per_posterior must give >= 1 chain per model")
+        self.counts = counts
+        self.D = max(m.D for m in models)
+        self.N = max(m.N for m in models)
+        dev = models[0].X.device
+        self.chain_posterior = torch.repeat_interleave(torch.arange(len(models), dtype=torch.int32),
+                                                       torch.tensor(counts)).to(dev)
+        self._table = _posterior_table(self.models, self.chain_posterior, dev)
+
+    @property
+    def dim(self):
+        return self.D
+
+    @property
+    def num_chains(self):
+        return int(sum(self.counts))
+
+    def data(self, chain_posterior=None):
+        return self._table.struct(self.chain_posterior)
+
+    def value_and_grad(self, position):
+        return logistic_value_and_grad(self, position)
+
+
+class _posterior_table:
+    def __init__(self, models, chain_posterior, dev):
+        self.x = torch.cat([m.X.reshape(-1) for m in models]).to(dev)
+        self.y = torch.cat([m.y for m in models]).to(dev)
+        xoff, yoff, ox, oy = [], [], 0, 0
+        for m in models:
+            xoff.append(ox)
+            yoff.append(oy)
+            ox += m.N * m.D
+            oy += m.N
+        self.N = torch.tensor([m.N for m in models], dtype=torch.int32, device=dev)
+        self.Dp = torch.tensor([m.D for m in models], dtype=torch.int32, device=dev)
+        self.xoff = torch.tensor(xoff, dtype=torch.int64, device=dev)
+        self.yoff = torch.tensor(yoff, dtype=torch.int64, device=dev)
+        self.P = len(models)
+        self.max_rows = max(m.N for m in models)
+        self.max_features = max(m.D for m in models)
+
+    def struct(self, chain_posterior):
+        return _lib.LogisticData(num_posteriors=self.P, max_rows=self.max_rows, max_features=self.max_features,
+                                 reserved=0, num_rows=self.N.data_ptr(), num_features=self.Dp.data_ptr(),
+                                 x_offset=self.xoff.data_ptr(), y_offset=self.yoff.data_ptr(),
+                                 chain_posterior=_lib.ptr(chain_posterior), x=self.x.data_ptr(), y=self.y.data_ptr())
+
+
+def logistic_params(model, C, L=0, T=1, step_size=0.0, max_iters=0, convergence_tol=1e-6, divergence_tol=1e10,
+                    divergence_threshold=1000.0, seed=0, counter=0):
+    return _lib.LogisticParams(num_chains=C, dim=model.dim, dtype=_lib.GJX_F64, num_integration_steps=int(L),
+                               max_iters=int(max_iters), num_transitions=int(T), flags=0, reserved=0,
+                               step_size=float(step_size), prior_var=model.prior_var,
+                               convergence_tol=float(convergence_tol), divergence_tol=float(divergence_tol),
+                               divergence_threshold=float(divergence_threshold), seed=int(seed),
+                               counter=int(counter))
+
+
+def logistic_value_and_grad(model, position):
+    if not position.is_cuda:
+        raise ValueError("positions must be on a HIP device (cuda:N)")
+    if position.dtype != torch.float64:
+        raise TypeError("the logistic kernels run in float64")
+    pos = position.contiguous()
+    squeeze = pos.dim() == 1
+    if squeeze:
+        pos = pos[None]
+    C, D = pos.shape
+    if D != model.dim:
+        raise ValueError(f"position has {D} coordinates, the model has {model.dim}")
+    if isinstance(model, LogisticBatch) and C != model.num_chains:
+        raise ValueError(f"the batch has {model.num_chains} chains, got {C} positions")
+    logp = torch.empty(C, dtype=torch.float64, device=pos.device)
+    grad = torch.zeros_like(pos)
+    rc = _lib.load().gjx_logistic_value_and_grad(logistic_params(model, C), model.data(), pos.data_ptr(),
+                                                 logp.data_ptr(), grad.data_ptr(), _lib.stream_handle(pos.device))
+    _lib.check(rc, "gjx_logistic_value_and_grad")
+    if squeeze:
+        return logp[0], grad[0]
+    return logp, grad

@@ -25,8 +25,8 @@ import torch
 
 from . import _lib
 from .base import SamplingAlgorithm
-from .metrics import MongeMetric
-from .models import NealFunnel, _dtype_code
+from .metrics import FisherMetric, MongeMetric
+from .models import LogisticBatch, LogisticRegression, NealFunnel, _dtype_code, logistic_params
 from .proposal import Proposal
 from .random import HostNoise, PRNGKey
 
@@ -100,21 +100,29 @@ def _as_batch(t):
 
 def init(position, logdensity_fn):
     """geomjax/rmhmc/rmhmc.py:96-98: value and gradient of the log density."""
-    if not isinstance(logdensity_fn, NealFunnel):
+    if not isinstance(logdensity_fn, (NealFunnel, LogisticRegression, LogisticBatch)):
         raise NotImplementedError(f"no HIP log density for {logdensity_fn!r}")
     logdensity, grad = logdensity_fn.value_and_grad(position)
     return RMHMCState(position, logdensity, grad)
 
 
-def _check_problem(logdensity_fn, metric_fn, position):
-    if not isinstance(logdensity_fn, NealFunnel):
-        raise NotImplementedError(f"no HIP RMHMC kernel for logdensity_fn={logdensity_fn!r}")
-    if not isinstance(metric_fn, MongeMetric):
-        raise NotImplementedError(f"no HIP RMHMC kernel for metric_fn={metric_fn!r}")
-    if position.shape[-1] != logdensity_fn.D:
-        raise ValueError(f"position has {position.shape[-1]} coordinates, the model has D={logdensity_fn.D}")
+def _problem(logdensity_fn, metric_fn, position):
+    """Which native kernel serves (logdensity_fn, metric_fn); validates shapes."""
     if not position.is_cuda:
         raise ValueError("RMHMC states must live on a HIP device (cuda:N)")
+    if isinstance(logdensity_fn, NealFunnel) and isinstance(metric_fn, MongeMetric):
+        if position.shape[-1] != logdensity_fn.D:
+            raise ValueError(f"position has {position.shape[-1]} coordinates, the model has D={logdensity_fn.D}")
+        return "funnel_monge"
+    if isinstance(logdensity_fn, (LogisticRegression, LogisticBatch)) and isinstance(metric_fn, FisherMetric):
+        if position.shape[-1] != logdensity_fn.dim:
+            raise ValueError(f"position has {position.shape[-1]} coordinates, the model has {logdensity_fn.dim}")
+        if position.dtype != torch.float64:
+            raise TypeError("the logistic RMHMC kernel runs in float64")
+        if isinstance(logdensity_fn, LogisticBatch) and position.shape[0] != logdensity_fn.num_chains:
+            raise ValueError(f"the batch has {logdensity_fn.num_chains} chains, got {position.shape[0]}")
+        return "logistic_fisher"
+    raise NotImplementedError(f"no HIP RMHMC kernel for logdensity_fn={logdensity_fn!r}, metric_fn={metric_fn!r}")
 
 
 def _params(C, D, dtype_code, step_size, L, solver, alpha2, sigma, divergence_threshold, T=1,
@@ -139,7 +147,7 @@ def build_kernel(integrator=implicit_midpoint, divergence_threshold: float = 100
 
     def kernel(rng_key, state, logdensity_fn, step_size, metric_fn, num_integration_steps):
         position, single = _as_batch(state.position)
-        _check_problem(logdensity_fn, metric_fn, position)
+        kind = _problem(logdensity_fn, metric_fn, position)
         position = position.contiguous()
         dev = position.device
         C, D = position.shape
@@ -174,10 +182,20 @@ def build_kernel(integrator=implicit_midpoint, divergence_threshold: float = 100
         io = _lib.RmhmcStepIO(position=position.data_ptr(), logdensity=logdensity.data_ptr(),
                               logdensity_grad=grad.data_ptr(), z=_lib.ptr(z), u=_lib.ptr(u),
                               **{k: v.data_ptr() for k, v in out.items()})
-        params = _params(C, D, _dtype_code(position), step_size, L, solver, metric_fn.alpha2, logdensity_fn.sigma,
-                         divergence_threshold, lanes_per_chain=lanes_per_chain, seed=seed, counter=counter)
-        rc = _lib.load().gjx_rmhmc_monge_funnel_step(params, io, _lib.stream_handle(dev))
-        _lib.check(rc, "gjx_rmhmc_monge_funnel_step")
+        if kind == "funnel_monge":
+            params = _params(C, D, _dtype_code(position), step_size, L, solver, metric_fn.alpha2,
+                             logdensity_fn.sigma, divergence_threshold, lanes_per_chain=lanes_per_chain, seed=seed,
+                             counter=counter)
+            rc = _lib.load().gjx_rmhmc_monge_funnel_step(params, io, _lib.stream_handle(dev))
+            _lib.check(rc, "gjx_rmhmc_monge_funnel_step")
+        else:
+            params = logistic_params(logdensity_fn, C, L=L, step_size=step_size, max_iters=solver["max_iters"],
+                                     convergence_tol=solver["convergence_tol"],
+                                     divergence_tol=solver["divergence_tol"],
+                                     divergence_threshold=divergence_threshold, seed=seed, counter=counter)
+            rc = _lib.load().gjx_rmhmc_fisher_logistic_step(params, logdensity_fn.data(), io,
+                                                            _lib.stream_handle(dev))
+            _lib.check(rc, "gjx_rmhmc_fisher_logistic_step")
 
         def sq(t):
             return t[0] if single else t
@@ -219,7 +237,7 @@ def sample(rng_key, state, logdensity_fn, step_size, metric_fn, num_integration_
     position, single = _as_batch(state.position)
     if single:
         raise ValueError("sample() runs a batch of chains: position must be [C, D]")
-    _check_problem(logdensity_fn, metric_fn, position)
+    kind = _problem(logdensity_fn, metric_fn, position)
     position = position.contiguous()
     dev = position.device
     C, D = position.shape
@@ -253,11 +271,19 @@ def sample(rng_key, state, logdensity_fn, step_size, metric_fn, num_integration_
                             final_logdensity_grad=final_g.data_ptr(), samples=_lib.ptr(samples),
                             acceptance_rate=acc.data_ptr(), num_accepted=n_acc.data_ptr(),
                             num_fevals=n_fev.data_ptr())
-    params = _params(C, D, _dtype_code(position), step_size, num_integration_steps, solver, metric_fn.alpha2,
-                     logdensity_fn.sigma, divergence_threshold, T=T, lanes_per_chain=lanes_per_chain, seed=seed,
-                     counter=counter)
-    rc = _lib.load().gjx_rmhmc_monge_funnel_sample(params, io, _lib.stream_handle(dev))
-    _lib.check(rc, "gjx_rmhmc_monge_funnel_sample")
+    if kind == "funnel_monge":
+        params = _params(C, D, _dtype_code(position), step_size, num_integration_steps, solver, metric_fn.alpha2,
+                         logdensity_fn.sigma, divergence_threshold, T=T, lanes_per_chain=lanes_per_chain, seed=seed,
+                         counter=counter)
+        rc = _lib.load().gjx_rmhmc_monge_funnel_sample(params, io, _lib.stream_handle(dev))
+        _lib.check(rc, "gjx_rmhmc_monge_funnel_sample")
+    else:
+        params = logistic_params(logdensity_fn, C, L=num_integration_steps, T=T, step_size=step_size,
+                                 max_iters=solver["max_iters"], convergence_tol=solver["convergence_tol"],
+                                 divergence_tol=solver["divergence_tol"], divergence_threshold=divergence_threshold,
+                                 seed=seed, counter=counter)
+        rc = _lib.load().gjx_rmhmc_fisher_logistic_sample(params, logdensity_fn.data(), io, _lib.stream_handle(dev))
+        _lib.check(rc, "gjx_rmhmc_fisher_logistic_sample")
     return SampleResult(RMHMCState(final_q, final_l, final_g), samples, acc, n_acc, n_fev)
 
 

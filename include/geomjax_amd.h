@@ -197,6 +197,60 @@ int gjx_lmcmonge_funnel_step(const gjx_funnel_params* params, const gjx_lmc_step
 /* T LMC-Monge transitions per chain in one launch. */
 int gjx_lmcmonge_funnel_sample(const gjx_funnel_params* params, const gjx_lmc_sample_io* io, void* stream);
 
+/* ---------------------------------------------------------------------------
+ * Bayesian logistic regression with the Fisher metric
+ *   l(q) = sum_n [y_n eta_n - log(1 + e^eta_n)] - |q|^2 / (2 prior_var)
+ *          - (D/2) log(2 pi prior_var),   eta = X q,
+ *   G(q) = X^T diag(s (1 - s)) X + I / prior_var,  s = sigmoid(eta)
+ * (the user-supplied logdensity_fn / metric_fn of the logistic
+ * configurations).  Several posteriors of different (N, D) may share one
+ * launch; each chain names its posterior.
+ * ------------------------------------------------------------------------- */
+typedef struct {
+  int32_t num_chains;            /* C */
+  int32_t dim;                   /* row stride of the [C, dim] state arrays, >= every D_p */
+  int32_t dtype;                 /* GJX_F64 */
+  int32_t num_integration_steps; /* L >= 0 */
+  int32_t max_iters;             /* fixed-point iteration cap */
+  int32_t num_transitions;       /* T >= 1 (sample entry point) */
+  int32_t flags;                 /* 0 */
+  int32_t reserved;
+  double step_size;
+  double prior_var;              /* 100 in the logistic configurations */
+  double convergence_tol;
+  double divergence_tol;
+  double divergence_threshold;
+  uint64_t seed;
+  uint64_t counter;
+} gjx_logistic_params;
+
+typedef struct {
+  int32_t num_posteriors;        /* P >= 1 */
+  int32_t max_rows;              /* host copy of max_p N_p */
+  int32_t max_features;          /* host copy of max_p D_p (<= 64 on the float64 path) */
+  int32_t reserved;
+  const int32_t* num_rows;       /* [P] device: N_p */
+  const int32_t* num_features;   /* [P] device: D_p */
+  const int64_t* x_offset;       /* [P] device: element offset of X_p ([N_p, D_p] row-major) in x */
+  const int64_t* y_offset;       /* [P] device: element offset of y_p in y */
+  const int32_t* chain_posterior;/* [C] device, or NULL: every chain on posterior 0 */
+  const void* x;                 /* float64 design matrices */
+  const void* y;                 /* float64 labels in {0, 1} */
+} gjx_logistic_data;
+
+/* One RMHMC transition per chain (same semantics and buffers as
+ * gjx_rmhmc_monge_funnel_step, with metric_fn = Fisher metric). */
+int gjx_rmhmc_fisher_logistic_step(const gjx_logistic_params* params, const gjx_logistic_data* data,
+                                   const gjx_rmhmc_step_io* io, void* stream);
+
+/* T RMHMC transitions per chain in one launch (one workgroup per chain). */
+int gjx_rmhmc_fisher_logistic_sample(const gjx_logistic_params* params, const gjx_logistic_data* data,
+                                     const gjx_rmhmc_sample_io* io, void* stream);
+
+/* value_and_grad of the logistic log density for C positions (init). */
+int gjx_logistic_value_and_grad(const gjx_logistic_params* params, const gjx_logistic_data* data,
+                                const void* position, double* logdensity, void* logdensity_grad, void* stream);
+
 /* Log density and its gradient of Neal's funnel for C positions: the
  * jax.value_and_grad(logdensity_fn)(position) of the reference's init
  * (geomjax/rmhmc/rmhmc.py:96-98, geomjax/lmcmonge/lmc.py:101-109) with the
