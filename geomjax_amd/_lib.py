@@ -115,6 +115,13 @@ class LogisticData(ctypes.Structure):
     ] + [(n, _vp) for n in ("num_rows", "num_features", "x_offset", "y_offset", "chain_posterior", "x", "y")]
 
 
+class TangentIO(ctypes.Structure):
+    _fields_ = [(n, _vp) for n in (
+        "position", "logdensity", "z", "objective", "acceptance", "sq_jump", "d_objective", "d_acceptance",
+        "d_sq_jump", "num_fevals",
+    )]
+
+
 LMC_EXACT_SHERMAN_MORRISON = 1
 LMC_EXACT_MOMENTUM = 2
 
@@ -124,6 +131,7 @@ _SIGNATURES = {
     "gjx_rmhmc_monge_funnel_sample": [ctypes.POINTER(FunnelParams), ctypes.POINTER(RmhmcSampleIO), _vp],
     "gjx_lmcmonge_funnel_step": [ctypes.POINTER(FunnelParams), ctypes.POINTER(LmcStepIO), _vp],
     "gjx_lmcmonge_funnel_sample": [ctypes.POINTER(FunnelParams), ctypes.POINTER(LmcSampleIO), _vp],
+    "gjx_rmhmc_monge_funnel_tangent": [ctypes.POINTER(FunnelParams), ctypes.POINTER(TangentIO), _vp],
     "gjx_funnel_value_and_grad": [_c_int32, _c_int32, _c_double, _c_int32, _vp, _vp, _vp, _vp],
     "gjx_rmhmc_fisher_logistic_step": [ctypes.POINTER(LogisticParams), ctypes.POINTER(LogisticData),
                                        ctypes.POINTER(RmhmcStepIO), _vp],

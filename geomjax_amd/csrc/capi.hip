@@ -162,6 +162,36 @@ int gjx_rmhmc_monge_funnel_sample(const gjx_funnel_params* params, const gjx_rmh
   return GJX_OK;
 }
 
+int gjx_rmhmc_monge_funnel_tangent(const gjx_funnel_params* params, const gjx_tangent_io* io, void* stream) {
+  GJX_REQUIRE(params != nullptr && io != nullptr, "params and io are required");
+  GJX_REQUIRE(params->dtype == GJX_F64, "the tangent kernel runs in float64");
+  GJX_REQUIRE(params->alpha2 > 0.0 && std::isfinite(params->alpha2), "alpha2 must be > 0 (log alpha2 is a parameter)");
+  GJX_REQUIRE(params->step_size > 0.0 && std::isfinite(params->step_size),
+              "step_size must be > 0 (log step_size is a parameter)");
+  gjx_funnel_params p = *params;
+  if (p.lanes_per_chain == 0) p.lanes_per_chain = 32;
+  GJX_REQUIRE(p.lanes_per_chain == 16 || p.lanes_per_chain == 32 || p.lanes_per_chain == 64,
+              "the tangent kernel takes lanes_per_chain 16, 32 or 64");
+  int lpc = 0, epl = 0;
+  int rc = check_funnel_params(&p, false, &lpc, &epl);
+  if (rc != GJX_OK) return rc;
+  if (epl > 8) return fail(GJX_EUNSUPPORTED, "dim %d too large for the tangent kernel at %d lanes", p.dim, lpc);
+  GJX_REQUIRE(io->position && io->logdensity, "position and logdensity are required");
+  GJX_REQUIRE(io->objective && io->acceptance && io->sq_jump && io->d_objective && io->d_acceptance &&
+                  io->d_sq_jump,
+              "every output except num_fevals is required");
+  gjx::FunnelArgs a = base_args(&p);
+  a.T = 1;
+  a.q_in = io->position;
+  a.logp_in = io->logdensity;
+  a.z = io->z;
+  gjx::FunnelGradOut o{io->objective, io->acceptance, io->sq_jump, io->d_objective, io->d_acceptance,
+                       io->d_sq_jump, io->num_fevals};
+  hipError_t e = gjx::launch_funnel_rmhmc_tangent(a, o, lpc, epl, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "gjx_rmhmc_monge_funnel_tangent launch");
+  return GJX_OK;
+}
+
 int gjx_funnel_value_and_grad(int32_t num_chains, int32_t dim, double sigma, int32_t dtype, const void* position,
                               double* logdensity, void* logdensity_grad, void* stream) {
   GJX_REQUIRE(num_chains > 0, "num_chains must be > 0 (got %d)", num_chains);

@@ -48,6 +48,20 @@ struct FunnelArgs {
   int64_t* num_fevals;
 };
 
+// Outputs of the tangent kernel (per chain; d_* are [C, 2]: d/d log eps, d/d log alpha2).
+struct FunnelGradOut {
+  double* objective;    // acceptance * squared jump
+  double* acceptance;   // min(1, e^dH)
+  double* sq_jump;      // |q_L - q_0|^2
+  double* d_objective;
+  double* d_acceptance;
+  double* d_sq_jump;
+  int64_t* num_fevals;
+};
+
+hipError_t launch_funnel_rmhmc_tangent(const FunnelArgs& a, const FunnelGradOut& o, int lpc, int epl,
+                                       hipStream_t stream);
+
 // Smallest compiled elements-per-lane that covers the D-1 "x" coordinates
 // with `lpc` lanes per chain, or -1 if none does.
 int funnel_pick_epl(int dim, int lpc);

@@ -251,6 +251,32 @@ int gjx_rmhmc_fisher_logistic_sample(const gjx_logistic_params* params, const gj
 int gjx_logistic_value_and_grad(const gjx_logistic_params* params, const gjx_logistic_data* data,
                                 const void* position, double* logdensity, void* logdensity_grad, void* stream);
 
+/* ---------------------------------------------------------------------------
+ * Derivatives for step-size / alpha2 tuning (no counterpart entry point in
+ * the reference; its analogue is jax.jacfwd of the proposed position with
+ * respect to alpha2, geomjax/adaptation/chees_adaptation_lmcmonge.py:475-519,
+ * consumed by the alpha2 gradient at :222-257).
+ * Per chain, one RMHMC-Monge transition from `position` with momentum
+ * chol(G(q0; alpha2)) z: acceptance p = min(1, e^dH), squared jump
+ * |q_L - q_0|^2, their product, and the exact derivatives of all three with
+ * respect to (log step_size, log alpha2) through the executed fixed-point
+ * iterates (forward-mode tangents).  float64 only; alpha2 > 0.
+ * ------------------------------------------------------------------------- */
+typedef struct {
+  const void* position;      /* [C, D] float64 */
+  const double* logdensity;  /* [C] */
+  const void* z;             /* [C, D] float64 standard normals, or NULL: device PRNG (seed, counter) */
+  double* objective;         /* [C] */
+  double* acceptance;        /* [C] */
+  double* sq_jump;           /* [C] */
+  double* d_objective;       /* [C, 2] */
+  double* d_acceptance;      /* [C, 2] */
+  double* d_sq_jump;         /* [C, 2] */
+  int64_t* num_fevals;       /* [C] optional */
+} gjx_tangent_io;
+
+int gjx_rmhmc_monge_funnel_tangent(const gjx_funnel_params* params, const gjx_tangent_io* io, void* stream);
+
 /* Log density and its gradient of Neal's funnel for C positions: the
  * jax.value_and_grad(logdensity_fn)(position) of the reference's init
  * (geomjax/rmhmc/rmhmc.py:96-98, geomjax/lmcmonge/lmc.py:101-109) with the

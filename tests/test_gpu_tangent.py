@@ -1,0 +1,93 @@
+"""GPU: derivatives of ESJD x acceptance w.r.t. (log step_size, log alpha2).
+
+The tangent kernel is checked (float64) against
+  * torch.autograd through the float64 oracle (dense metric, autodiff),
+    unrolled over the same executed fixed-point iterations: 1e-8 relative;
+  * central finite differences (h = 1e-5) of the primal HIP step kernel with
+    the iteration count fixed (convergence_tol = 0, so every step runs
+    max_iters iterations and the objective is smooth in the parameters):
+    1e-6 relative.
+"""
+import functools
+import math
+
+import pytest
+import torch
+
+import geomjax_amd as gj
+from geomjax_amd import random as gr
+from geomjax_amd.adjoint import ESJDAcceptance, esjd_acceptance
+from geomjax_amd.rmhmc import build_kernel, implicit_midpoint
+from oracle import models as om
+from oracle import rmhmc as orm
+from tests.helpers import rel_err, seeded_normal
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+
+
+def oracle_objective_grad(D, q0, z, eps, alpha2, L, max_iters, conv_tol=1e-6):
+    le = torch.tensor(math.log(eps), dtype=torch.float64, requires_grad=True)
+    la = torch.tensor(math.log(alpha2), dtype=torch.float64, requires_grad=True)
+    M = om.NealFunnel(D)
+    metric = om.monge_metric_fn(M.logp, torch.exp(la), D)
+    k = orm.build_kernel(max_iters=max_iters, convergence_tol=conv_tol)
+    u = torch.zeros(q0.shape[0], dtype=torch.float64)
+    _, info = k((z, u), orm.init(q0, M.logp), M.logp, torch.exp(le), metric, L)
+    jump = ((info.proposal.state.position - q0) ** 2).sum(-1)
+    obj = info.acceptance_rate * jump
+    g = [torch.autograd.grad(o, (le, la), retain_graph=True) for o in obj]
+    return obj.detach(), torch.stack([torch.stack(x) for x in g])
+
+
+@pytest.mark.parametrize("D,C,eps,alpha2,L,max_iters", [(10, 4, 0.05, 1.0, 8, 8), (30, 3, 0.03, 0.6, 6, 8)])
+def test_tangent_matches_autograd_through_oracle(D, C, eps, alpha2, L, max_iters):
+    q0, z = seeded_normal(90 + D, C, D) * 0.8, seeded_normal(91 + D, C, D)
+    obj_o, g_o = oracle_objective_grad(D, q0, z, eps, alpha2, L, max_iters)
+    model = gj.NealFunnel(D)
+    s = gj.rmhmc.init(q0.to(DEV), model)
+    r = esjd_acceptance(gr.HostNoise(z.to(DEV), torch.zeros(C, device=DEV)), s, model, eps, gj.MongeMetric(alpha2),
+                        L, integrator=functools.partial(implicit_midpoint, max_iters=max_iters))
+    torch.cuda.synchronize()
+    print("objective", r.objective.cpu().tolist(), "grad", r.d_objective.cpu().tolist(), "oracle", g_o.tolist())
+    assert rel_err(r.objective, obj_o) < 1e-10
+    assert rel_err(r.d_objective, g_o) < 1e-8
+
+
+def test_tangent_matches_central_differences():
+    D, C, eps, alpha2, L, max_iters, h = 20, 6, 0.04, 0.8, 8, 6, 1e-5
+    q0, z = seeded_normal(95, C, D) * 0.8, seeded_normal(96, C, D)
+    model = gj.NealFunnel(D)
+    integ = functools.partial(implicit_midpoint, max_iters=max_iters, convergence_tol=0.0)
+    s = gj.rmhmc.init(q0.to(DEV), model)
+    noise = gr.HostNoise(z.to(DEV), torch.zeros(C, device=DEV, dtype=torch.float64))
+    r = esjd_acceptance(noise, s, model, eps, gj.MongeMetric(alpha2), L, integrator=integ)
+    kern = build_kernel(integ)
+
+    def obj(le, la):
+        _, info = kern(noise, s, model, math.exp(le), gj.MongeMetric(math.exp(la)), L)
+        jump = ((info.proposal.state.position - s.position) ** 2).sum(-1)
+        return (info.acceptance_rate * jump).double()
+
+    le, la = math.log(eps), math.log(alpha2)
+    fd = torch.stack([(obj(le + h, la) - obj(le - h, la)) / (2 * h), (obj(le, la + h) - obj(le, la - h)) / (2 * h)],
+                     dim=1)
+    torch.cuda.synchronize()
+    print("tangent", r.d_objective.cpu().tolist(), "fd", fd.cpu().tolist())
+    assert rel_err(r.objective, obj(le, la)) < 1e-12
+    assert rel_err(r.d_objective, fd) < 1e-6
+
+
+def test_autograd_function():
+    D, C = 10, 64
+    model = gj.NealFunnel(D)
+    s = gj.rmhmc.init(seeded_normal(97, C, D).to(DEV), model)
+    key = gr.key(3)
+    le = torch.tensor(math.log(0.05), dtype=torch.float64, requires_grad=True)
+    la = torch.tensor(0.0, dtype=torch.float64, requires_grad=True)
+    val = ESJDAcceptance.apply(le, la, s, key, model, 8)
+    val.backward()
+    r = esjd_acceptance(key, s, model, 0.05, gj.MongeMetric(1.0), 8)
+    mean = r.d_objective.mean(0).cpu()
+    assert abs(le.grad.item() - mean[0].item()) <= 1e-12 * max(1.0, abs(mean[0].item()))
+    assert abs(la.grad.item() - mean[1].item()) <= 1e-12 * max(1.0, abs(mean[1].item()))
