@@ -7,6 +7,7 @@ from . import diagnostics, metrics, models, random, sampling
 from .base import SamplingAlgorithm
 from .metrics import FisherMetric, MongeMetric
 from .models import LogisticBatch, LogisticRegression, NealFunnel
+from .lmcmc import lmc
 from .lmcmonge import lmc as lmcmonge
 from .rmhmc import rmhmc
 
@@ -15,5 +16,5 @@ __version__ = "0.1.0"
 ess = diagnostics.effective_sample_size
 rhat = diagnostics.potential_scale_reduction
 
-__all__ = ["rmhmc", "lmcmonge", "SamplingAlgorithm", "NealFunnel", "LogisticRegression", "LogisticBatch", "MongeMetric", "FisherMetric", "metrics", "models", "random", "sampling",
+__all__ = ["rmhmc", "lmc", "lmcmonge", "SamplingAlgorithm", "NealFunnel", "LogisticRegression", "LogisticBatch", "MongeMetric", "FisherMetric", "metrics", "models", "random", "sampling",
            "diagnostics", "ess", "rhat", "__version__"]

@@ -122,6 +122,23 @@ class TangentIO(ctypes.Structure):
     )]
 
 
+class LmcmcStepIO(ctypes.Structure):
+    _fields_ = [(n, _vp) for n in (
+        "position", "logdensity", "logdensity_grad", "volume_adjustment", "z", "u",
+        "new_position", "new_logdensity", "new_logdensity_grad", "new_volume_adjustment",
+        "velocity", "acceptance_rate", "is_accepted", "is_divergent", "energy",
+        "proposal_position", "proposal_momentum", "proposal_velocity", "proposal_logdensity",
+        "proposal_logdensity_grad", "proposal_volume_adjustment", "proposal_weight", "proposal_sum_log_p_accept",
+    )]
+
+
+class LmcmcSampleIO(ctypes.Structure):
+    _fields_ = [(n, _vp) for n in (
+        "position", "logdensity", "volume_adjustment", "z", "u", "final_position", "final_logdensity",
+        "final_logdensity_grad", "final_volume_adjustment", "samples", "acceptance_rate", "num_accepted",
+    )]
+
+
 LMC_EXACT_SHERMAN_MORRISON = 1
 LMC_EXACT_MOMENTUM = 2
 
@@ -137,6 +154,10 @@ _SIGNATURES = {
                                        ctypes.POINTER(RmhmcStepIO), _vp],
     "gjx_rmhmc_fisher_logistic_sample": [ctypes.POINTER(LogisticParams), ctypes.POINTER(LogisticData),
                                          ctypes.POINTER(RmhmcSampleIO), _vp],
+    "gjx_lmc_fisher_logistic_step": [ctypes.POINTER(LogisticParams), ctypes.POINTER(LogisticData),
+                                     ctypes.POINTER(LmcmcStepIO), _vp],
+    "gjx_lmc_fisher_logistic_sample": [ctypes.POINTER(LogisticParams), ctypes.POINTER(LogisticData),
+                                       ctypes.POINTER(LmcmcSampleIO), _vp],
     "gjx_logistic_value_and_grad": [ctypes.POINTER(LogisticParams), ctypes.POINTER(LogisticData), _vp, _vp, _vp,
                                     _vp],
 }

@@ -416,6 +416,73 @@ int gjx_rmhmc_fisher_logistic_sample(const gjx_logistic_params* params, const gj
   return GJX_OK;
 }
 
+int gjx_lmc_fisher_logistic_step(const gjx_logistic_params* params, const gjx_logistic_data* data,
+                                 const gjx_lmcmc_step_io* io, void* stream) {
+  gjx::LogisticArgs a;
+  int rc = check_logistic(params, data, false, &a);
+  if (rc != GJX_OK) return rc;
+  GJX_REQUIRE(io != nullptr, "io is NULL");
+  GJX_REQUIRE(io->position && io->logdensity && io->volume_adjustment,
+              "position, logdensity and volume_adjustment are required");
+  GJX_REQUIRE(io->new_position && io->new_logdensity, "new_position and new_logdensity are required");
+  GJX_REQUIRE(!io->new_logdensity_grad || io->logdensity_grad,
+              "new_logdensity_grad needs logdensity_grad (copied through on rejection)");
+  GJX_REQUIRE((io->z == nullptr) == (io->u == nullptr), "z and u must both be given or both be NULL");
+  a.q_in = io->position;
+  a.logp_in = io->logdensity;
+  a.g_in = io->logdensity_grad;
+  a.J_in = io->volume_adjustment;
+  a.z = io->z;
+  a.u = io->u;
+  a.q_out = io->new_position;
+  a.logp_out = io->new_logdensity;
+  a.g_out = io->new_logdensity_grad;
+  a.J_out = io->new_volume_adjustment;
+  a.momentum = io->velocity;
+  a.acc_rate = io->acceptance_rate;
+  a.accepted = io->is_accepted;
+  a.divergent = io->is_divergent;
+  a.energy = io->energy;
+  a.prop_q = io->proposal_position;
+  a.prop_p = io->proposal_momentum;
+  a.prop_v = io->proposal_velocity;
+  a.prop_logp = io->proposal_logdensity;
+  a.prop_g = io->proposal_logdensity_grad;
+  a.prop_J = io->proposal_volume_adjustment;
+  a.prop_weight = io->proposal_weight;
+  a.prop_slpa = io->proposal_sum_log_p_accept;
+  hipError_t e = gjx::launch_logistic_lmc_f64(a, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "gjx_lmc_fisher_logistic_step launch");
+  return GJX_OK;
+}
+
+int gjx_lmc_fisher_logistic_sample(const gjx_logistic_params* params, const gjx_logistic_data* data,
+                                   const gjx_lmcmc_sample_io* io, void* stream) {
+  gjx::LogisticArgs a;
+  int rc = check_logistic(params, data, true, &a);
+  if (rc != GJX_OK) return rc;
+  GJX_REQUIRE(io != nullptr, "io is NULL");
+  GJX_REQUIRE(io->position && io->logdensity && io->volume_adjustment,
+              "position, logdensity and volume_adjustment are required");
+  GJX_REQUIRE(io->final_position && io->final_logdensity, "final_position and final_logdensity are required");
+  GJX_REQUIRE((io->z == nullptr) == (io->u == nullptr), "z and u must both be given or both be NULL");
+  a.q_in = io->position;
+  a.logp_in = io->logdensity;
+  a.J_in = io->volume_adjustment;
+  a.z = io->z;
+  a.u = io->u;
+  a.final_q = io->final_position;
+  a.final_logp = io->final_logdensity;
+  a.final_g = io->final_logdensity_grad;
+  a.final_J = io->final_volume_adjustment;
+  a.samples = io->samples;
+  a.sample_acc = io->acceptance_rate;
+  a.num_accepted = io->num_accepted;
+  hipError_t e = gjx::launch_logistic_lmc_f64(a, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "gjx_lmc_fisher_logistic_sample launch");
+  return GJX_OK;
+}
+
 int gjx_logistic_value_and_grad(const gjx_logistic_params* params, const gjx_logistic_data* data,
                                 const void* position, double* logdensity, void* logdensity_grad, void* stream) {
   gjx::LogisticArgs a;

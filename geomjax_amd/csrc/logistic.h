@@ -24,7 +24,11 @@ struct LogisticArgs {
   const void* g_in;
   const void* z;
   const double* u;
+  const double* J_in;  // LMC volume adjustment
   // single-transition outputs
+  double* J_out;
+  double* prop_J;
+  double* final_J;
   void* q_out;
   double* logp_out;
   void* g_out;
@@ -57,5 +61,6 @@ constexpr int kLogisticMaxD = 64;
 
 hipError_t launch_logistic_rmhmc_f64(const LogisticArgs& a, hipStream_t stream);
 hipError_t launch_logistic_value_and_grad(const LogisticArgs& a, hipStream_t stream);
+hipError_t launch_logistic_lmc_f64(const LogisticArgs& a, hipStream_t stream);
 
 }  // namespace gjx

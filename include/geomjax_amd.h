@@ -247,6 +247,63 @@ int gjx_rmhmc_fisher_logistic_step(const gjx_logistic_params* params, const gjx_
 int gjx_rmhmc_fisher_logistic_sample(const gjx_logistic_params* params, const gjx_logistic_data* data,
                                      const gjx_rmhmc_sample_io* io, void* stream);
 
+/* Buffers of one LMC transition with the general metric (lmcmc).  Inputs
+ * LMCState (geomjax/lmcmc/lmc.py:30-42); outputs the new LMCState and LMCInfo
+ * (:60-95) whose Proposal state is IntegratorState(position, momentum,
+ * velocity, logdensity, logdensity_grad, volume_adjustment)
+ * (geomjax/lmcmc/integrators.py:26-38).  Optional outputs may be NULL. */
+typedef struct {
+  const void* position;             /* [C, dim] */
+  const double* logdensity;         /* [C] */
+  const void* logdensity_grad;      /* [C, dim] */
+  const double* volume_adjustment;  /* [C] */
+  const void* z;                    /* [C, dim] standard normals, or NULL: device PRNG */
+  const double* u;                  /* [C], or NULL */
+  void* new_position;               /* [C, dim] */
+  double* new_logdensity;           /* [C] */
+  void* new_logdensity_grad;        /* [C, dim] optional */
+  double* new_volume_adjustment;    /* [C] optional */
+  void* velocity;                   /* [C, dim] optional: info.velocity (the v0 drawn) */
+  double* acceptance_rate;          /* [C] optional */
+  uint8_t* is_accepted;             /* [C] optional */
+  uint8_t* is_divergent;            /* [C] optional */
+  double* energy;                   /* [C] optional */
+  void* proposal_position;          /* [C, dim] optional */
+  void* proposal_momentum;          /* [C, dim] optional (flipped) */
+  void* proposal_velocity;          /* [C, dim] optional (flipped) */
+  double* proposal_logdensity;      /* [C] optional */
+  void* proposal_logdensity_grad;   /* [C, dim] optional */
+  double* proposal_volume_adjustment; /* [C] optional */
+  double* proposal_weight;          /* [C] optional */
+  double* proposal_sum_log_p_accept;/* [C] optional */
+} gjx_lmcmc_step_io;
+
+typedef struct {
+  const void* position;             /* [C, dim] */
+  const double* logdensity;         /* [C] */
+  const double* volume_adjustment;  /* [C] */
+  const void* z;                    /* [T, C, dim] or NULL */
+  const double* u;                  /* [T, C] or NULL */
+  void* final_position;             /* [C, dim] */
+  double* final_logdensity;         /* [C] */
+  void* final_logdensity_grad;      /* [C, dim] optional */
+  double* final_volume_adjustment;  /* [C] optional */
+  void* samples;                    /* [T, C, dim] optional */
+  float* acceptance_rate;           /* [T, C] optional */
+  int32_t* num_accepted;            /* [C] optional */
+} gjx_lmcmc_sample_io;
+
+/* One LMC transition per chain with metric_fn = Fisher metric: the reference
+ * kernel geomjax/lmcmc/lmc.py:135-180 with lan_integrator
+ * (geomjax/lmcmc/integrators.py:51-144) and gaussian_riemannian
+ * (geomjax/lmcmc/metrics.py:42-206). */
+int gjx_lmc_fisher_logistic_step(const gjx_logistic_params* params, const gjx_logistic_data* data,
+                                 const gjx_lmcmc_step_io* io, void* stream);
+
+/* T LMC transitions per chain in one launch. */
+int gjx_lmc_fisher_logistic_sample(const gjx_logistic_params* params, const gjx_logistic_data* data,
+                                   const gjx_lmcmc_sample_io* io, void* stream);
+
 /* value_and_grad of the logistic log density for C positions (init). */
 int gjx_logistic_value_and_grad(const gjx_logistic_params* params, const gjx_logistic_data* data,
                                 const void* position, double* logdensity, void* logdensity_grad, void* stream);
