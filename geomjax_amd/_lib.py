@@ -112,7 +112,8 @@ class LogisticData(ctypes.Structure):
         ("max_rows", _c_int32),
         ("max_features", _c_int32),
         ("reserved", _c_int32),
-    ] + [(n, _vp) for n in ("num_rows", "num_features", "x_offset", "y_offset", "chain_posterior", "x", "y")]
+    ] + [(n, _vp) for n in ("num_rows", "num_features", "x_offset", "y_offset", "chain_posterior", "x", "y",
+                            "packed", "workspace")] + [("workspace_bytes", ctypes.c_uint64)]
 
 
 class TangentIO(ctypes.Structure):
@@ -160,6 +161,10 @@ _SIGNATURES = {
                                        ctypes.POINTER(LmcmcSampleIO), _vp],
     "gjx_logistic_value_and_grad": [ctypes.POINTER(LogisticParams), ctypes.POINTER(LogisticData), _vp, _vp, _vp,
                                     _vp],
+    "gjx_logistic_pack_size": [_c_int32, _vp, _vp, ctypes.POINTER(ctypes.c_uint64)],
+    "gjx_logistic_pack": [ctypes.POINTER(LogisticData), _vp, _vp, _vp, ctypes.c_uint64, _vp],
+    "gjx_logistic_workspace_size": [ctypes.POINTER(LogisticParams), ctypes.POINTER(LogisticData),
+                                    ctypes.POINTER(ctypes.c_uint64)],
 }
 _STRING_FUNCS = ("gjx_version", "gjx_last_error")
 

@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "../../include/geomjax_amd.h"
 #include "funnel_lmc.h"
@@ -308,10 +309,14 @@ int gjx_lmcmonge_funnel_sample(const gjx_funnel_params* params, const gjx_lmc_sa
 
 namespace {
 
+void workspace_bytes(int C, int max_rows, uint64_t* bytes) {
+  *bytes = (uint64_t)C * 5 * (uint64_t)((max_rows + 31) / 32 * 32) * sizeof(float);
+}
+
 int check_logistic(const gjx_logistic_params* p, const gjx_logistic_data* d, bool sample, gjx::LogisticArgs* a) {
   GJX_REQUIRE(p != nullptr && d != nullptr, "params and data are required");
   GJX_REQUIRE(p->num_chains > 0, "num_chains must be > 0 (got %d)", p->num_chains);
-  GJX_REQUIRE(p->dtype == GJX_F64, "the logistic kernels take dtype GJX_F64 (got %d)", p->dtype);
+  GJX_REQUIRE(p->dtype == GJX_F64 || p->dtype == GJX_F32, "dtype must be GJX_F64 or GJX_F32 (got %d)", p->dtype);
   GJX_REQUIRE(p->num_integration_steps >= 0, "num_integration_steps must be >= 0");
   GJX_REQUIRE(p->max_iters >= 0, "max_iters must be >= 0");
   GJX_REQUIRE(std::isfinite(p->step_size), "step_size must be finite");
@@ -322,7 +327,7 @@ int check_logistic(const gjx_logistic_params* p, const gjx_logistic_data* d, boo
   GJX_REQUIRE(d->max_features >= 1 && d->max_features <= p->dim,
               "max_features must be in [1, dim] (got %d, dim %d)", d->max_features, p->dim);
   if (d->max_features > gjx::kLogisticMaxD)
-    return fail(GJX_EUNSUPPORTED, "D = %d exceeds the float64 kernel's maximum of %d features", d->max_features,
+    return fail(GJX_EUNSUPPORTED, "D = %d exceeds the kernels' maximum of %d features", d->max_features,
                 gjx::kLogisticMaxD);
   GJX_REQUIRE(d->num_rows && d->num_features && d->x_offset && d->y_offset && d->x && d->y,
               "data arrays are required");
@@ -347,6 +352,24 @@ int check_logistic(const gjx_logistic_params* p, const gjx_logistic_data* d, boo
   a->post_yoff = d->y_offset;
   a->X = static_cast<const double*>(d->x);
   a->y = static_cast<const double*>(d->y);
+  if (p->dtype == GJX_F32) {
+    GJX_REQUIRE(d->packed != nullptr, "the float32 path needs data->packed (gjx_logistic_pack)");
+    uint64_t need = 0;
+    workspace_bytes(p->num_chains, d->max_rows, &need);
+    GJX_REQUIRE(d->workspace != nullptr && d->workspace_bytes >= need,
+                "the float32 path needs a workspace of %llu bytes (got %llu)", (unsigned long long)need,
+                (unsigned long long)d->workspace_bytes);
+    a->packed_hdr = static_cast<const gjx::PackedPost*>(d->packed);
+    a->packed = static_cast<const float*>(d->packed);
+    a->work = static_cast<float*>(d->workspace);
+    a->work_rows = (d->max_rows + 31) / 32 * 32;
+  }
+  return GJX_OK;
+}
+
+// only the RMHMC entry points have a float32 path
+int require_f64(const gjx_logistic_params* p, const char* what) {
+  if (p != nullptr && p->dtype == GJX_F32) return fail(GJX_EUNSUPPORTED, "%s has no float32 path (use GJX_F64)", what);
   return GJX_OK;
 }
 
@@ -386,7 +409,8 @@ int gjx_rmhmc_fisher_logistic_step(const gjx_logistic_params* params, const gjx_
   a.fp_iters = io->fp_iters;
   a.fp_norm = io->fp_norm;
   a.fp_success = io->fp_success;
-  hipError_t e = gjx::launch_logistic_rmhmc_f64(a, (hipStream_t)stream);
+  hipError_t e = params->dtype == GJX_F32 ? gjx::launch_logistic_rmhmc_f32(a, data->max_features, (hipStream_t)stream)
+                                           : gjx::launch_logistic_rmhmc_f64(a, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "gjx_rmhmc_fisher_logistic_step launch");
   return GJX_OK;
 }
@@ -411,13 +435,15 @@ int gjx_rmhmc_fisher_logistic_sample(const gjx_logistic_params* params, const gj
   a.sample_acc = io->acceptance_rate;
   a.num_accepted = io->num_accepted;
   a.num_fevals = io->num_fevals;
-  hipError_t e = gjx::launch_logistic_rmhmc_f64(a, (hipStream_t)stream);
+  hipError_t e = params->dtype == GJX_F32 ? gjx::launch_logistic_rmhmc_f32(a, data->max_features, (hipStream_t)stream)
+                                           : gjx::launch_logistic_rmhmc_f64(a, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "gjx_rmhmc_fisher_logistic_sample launch");
   return GJX_OK;
 }
 
 int gjx_lmc_fisher_logistic_step(const gjx_logistic_params* params, const gjx_logistic_data* data,
                                  const gjx_lmcmc_step_io* io, void* stream) {
+  if (int rc0 = require_f64(params, "gjx_lmc_fisher_logistic_step")) return rc0;
   gjx::LogisticArgs a;
   int rc = check_logistic(params, data, false, &a);
   if (rc != GJX_OK) return rc;
@@ -458,6 +484,7 @@ int gjx_lmc_fisher_logistic_step(const gjx_logistic_params* params, const gjx_lo
 
 int gjx_lmc_fisher_logistic_sample(const gjx_logistic_params* params, const gjx_logistic_data* data,
                                    const gjx_lmcmc_sample_io* io, void* stream) {
+  if (int rc0 = require_f64(params, "gjx_lmc_fisher_logistic_sample")) return rc0;
   gjx::LogisticArgs a;
   int rc = check_logistic(params, data, true, &a);
   if (rc != GJX_OK) return rc;
@@ -485,6 +512,7 @@ int gjx_lmc_fisher_logistic_sample(const gjx_logistic_params* params, const gjx_
 
 int gjx_logistic_value_and_grad(const gjx_logistic_params* params, const gjx_logistic_data* data,
                                 const void* position, double* logdensity, void* logdensity_grad, void* stream) {
+  if (int rc0 = require_f64(params, "gjx_logistic_value_and_grad")) return rc0;
   gjx::LogisticArgs a;
   int rc = check_logistic(params, data, false, &a);
   if (rc != GJX_OK) return rc;
@@ -494,6 +522,55 @@ int gjx_logistic_value_and_grad(const gjx_logistic_params* params, const gjx_log
   a.g_out = logdensity_grad;
   hipError_t e = gjx::launch_logistic_value_and_grad(a, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "gjx_logistic_value_and_grad launch");
+  return GJX_OK;
+}
+
+int gjx_logistic_pack_size(int32_t num_posteriors, const int32_t* num_rows_host, const int32_t* num_features_host,
+                           uint64_t* bytes) {
+  GJX_REQUIRE(num_posteriors >= 1, "num_posteriors must be >= 1");
+  GJX_REQUIRE(num_rows_host && num_features_host && bytes, "num_rows_host, num_features_host and bytes are required");
+  for (int p = 0; p < num_posteriors; ++p) {
+    GJX_REQUIRE(num_rows_host[p] >= 1, "posterior %d has %d rows", p, num_rows_host[p]);
+    GJX_REQUIRE(num_features_host[p] >= 1, "posterior %d has %d features", p, num_features_host[p]);
+    if (num_features_host[p] > gjx::kLogisticMaxD)
+      return fail(GJX_EUNSUPPORTED, "posterior %d has D = %d > %d features", p, num_features_host[p],
+                  gjx::kLogisticMaxD);
+  }
+  *bytes = gjx::logistic_pack_layout(num_posteriors, num_rows_host, num_features_host, nullptr);
+  return GJX_OK;
+}
+
+int gjx_logistic_pack(const gjx_logistic_data* data, const int32_t* num_rows_host, const int32_t* num_features_host,
+                      void* packed, uint64_t packed_bytes, void* stream) {
+  GJX_REQUIRE(data != nullptr && packed != nullptr, "data and packed are required");
+  GJX_REQUIRE(data->num_rows && data->num_features && data->x_offset && data->y_offset && data->x && data->y,
+              "data arrays are required");
+  uint64_t need = 0;
+  int rc = gjx_logistic_pack_size(data->num_posteriors, num_rows_host, num_features_host, &need);
+  if (rc != GJX_OK) return rc;
+  GJX_REQUIRE(packed_bytes >= need, "packed buffer of %llu bytes is smaller than the %llu needed",
+              (unsigned long long)packed_bytes, (unsigned long long)need);
+  const int P = data->num_posteriors;
+  std::vector<gjx::PackedPost> hdr(P);
+  gjx::logistic_pack_layout(P, num_rows_host, num_features_host, hdr.data());
+  int max_npad = 0;
+  for (int p = 0; p < P; ++p) max_npad = hdr[p].npad > max_npad ? hdr[p].npad : max_npad;  // grid covers npad x 64
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e = hipMemcpyAsync(packed, hdr.data(), sizeof(gjx::PackedPost) * P, hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) return hip_fail(e, "gjx_logistic_pack header copy");
+  e = gjx::launch_logistic_pack(static_cast<const double*>(data->x), static_cast<const double*>(data->y),
+                                data->num_rows, data->num_features, data->x_offset, data->y_offset,
+                                static_cast<const gjx::PackedPost*>(packed), static_cast<float*>(packed), P, max_npad, s);
+  if (e != hipSuccess) return hip_fail(e, "gjx_logistic_pack launch");
+  e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hip_fail(e, "gjx_logistic_pack");
+  return GJX_OK;
+}
+
+int gjx_logistic_workspace_size(const gjx_logistic_params* params, const gjx_logistic_data* data, uint64_t* bytes) {
+  GJX_REQUIRE(params && data && bytes, "params, data and bytes are required");
+  GJX_REQUIRE(params->num_chains > 0 && data->max_rows >= 1, "num_chains and max_rows must be >= 1");
+  workspace_bytes(params->num_chains, data->max_rows, bytes);
   return GJX_OK;
 }
 

@@ -1,0 +1,770 @@
+// RMHMC with the Fisher metric on Bayesian logistic regression, tuned form:
+// float32 state, Gram and quadratic-form contractions on bf16 MFMA with float32
+// accumulation, float32 sweep inverse with one refinement step per solve,
+// float64 log-determinant sums, energies, accept test and fixed-point norm.
+//
+// Reference: geomjax/rmhmc/rmhmc.py:131-174,376-462 (kernel, proposal, accept),
+// geomjax/rmhmc/integrators.py:53-156 (implicit midpoint, fixed point),
+// geomjax/rmhmc/metrics.py:42-129 (kinetic energy -log N(p; 0, G));
+// the model/metric are the logistic configurations' user-supplied functions.
+// Closed forms as in the float64 kernel (w = s(1-s)(1-2s), r = y - s,
+// a_n = x_n^T G^-1 x_n, b = X u):
+//   dH/dp = u = G^-1 p,   dH/dq = X^T [ w (a - b^2) / 2 - r ] + q / prior_var.
+//
+// Layout.  One wave (64 lanes) per chain, no cross-wave synchronisation: each
+// chain runs its own fixed-point iteration count.  The design matrix is
+// packed once into two float32 fragment layouts (logistic_pack_kernel):
+//   Gram layout: for a 16-row K step, lane l holds X[k0 + (l/32)*8 + i][32b + l%32]
+//     (i < 8) -- the A and B operands of v_mfma_f32_32x32x16_bf16 for the Gram
+//     G = Y^T Y with Y = sqrt(lambda) X (both operands are the same registers);
+//   row layout: for a 32-row block, lane l holds X[r0 + l%32][16kc + (l/32)*8 + i]
+//     -- the B operand of V^T = G^-1 X^T and the rows for X u, X^T h.
+// Each lane loads 32 contiguous bytes per fragment.
+//
+// Precision of the contractions: every float32 operand v is split into
+// hi = bf16(v) and lo = bf16(v - hi); the products hi*hi + hi*lo + lo*hi
+// carry ~16 significant bits per term and accumulate in float32.
+//
+// a_n without a second layout: the output feature index m of V^T is mapped
+// through a fixed permutation phi (rows of the A operand G^-1 are permuted)
+// so that the accumulator slot (t, r) of lane l holds V[row l%32][f] for the
+// very feature f whose X value the lane holds in its row-layout fragment:
+// a_n is 32 FMAs per lane plus one exchange with lane l^32.
+//
+// eta = X q is never recomputed inside a trajectory: the fixed-point update
+// q' = q_0 + h u gives eta' = eta_0 + h (X u), and X u is formed anyway for
+// the kinetic term; at the end of every trajectory eta is recomputed from X
+// in float64 together with the log likelihood and gradient.  Per-chain
+// scratch (global, L2-resident): eta at the step start, eta at the current
+// iterate, sqrt(lambda) at the current iterate, eta of the chain state and of
+// the proposal.
+//
+// Sweep.  G (64 x 64) is held one column per lane in registers; the symmetric
+// sweep operator (Goodnight 1979) gives -G^-1 and the pivots d_k, whose logs
+// sum (float64) to log det G.  At pivot k the not-yet-swept column k is the
+// Schur complement, so the Cholesky factor column L[:, k] = A[:, k] / sqrt(d_k)
+// is available for the momentum draw p0 = L z at no extra cost.
+#include "common.h"
+#include "logistic.h"
+
+namespace gjx {
+namespace lfast {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x32 __attribute__((ext_vector_type(32)));
+
+constexpr int MAXD = 64;
+constexpr int LDG = MAXD + 4;  // LDS row stride of G (floats; rows stay 16-byte aligned)
+
+struct Smem {
+  float G[MAXD * LDG];  // Gram G, then G^-1; scratch of the transpose-reductions
+  float bc[2][MAXD];    // sweep: pivot column broadcast (double-buffered)
+  float dk[MAXD];       // sweep pivots
+  float q0[MAXD], p0[MAXD], q[MAXD], p[MAXD], u[MAXD], dH[MAXD], z[MAXD], g[MAXD], gs[MAXD], r[MAXD];
+};
+
+__device__ __forceinline__ void load8(const float* p, float (&v)[8]) {
+  const float4 a = reinterpret_cast<const float4*>(p)[0];
+  const float4 b = reinterpret_cast<const float4*>(p)[1];
+  v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+  v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+}
+
+// v = hi + lo + O(2^-17 |v|), hi and lo in bf16 (round to nearest even)
+__device__ __forceinline__ void split8(const float (&v)[8], bf16x8& hi, bf16x8& lo) {
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const __bf16 h = (__bf16)v[i];
+    hi[i] = h;
+    lo[i] = (__bf16)(v[i] - (float)h);
+  }
+}
+
+// acc += (ah + al)(bh + bl) without the lo*lo term
+__device__ __forceinline__ f32x16 mfma3(const bf16x8& ah, const bf16x8& al, const bf16x8& bh, const bf16x8& bl,
+                                        f32x16 acc) {
+  acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc, 0, 0, 0);
+  return acc;
+}
+
+__device__ __forceinline__ float sigmoidf(float x) { return 1.0f / (1.0f + expf(-x)); }
+__device__ __forceinline__ double softplus(double x) { return fmax(x, 0.0) + log1p(exp(-fabs(x))); }
+
+// phi: accumulator feature slot m = 32t + 8a + 4h + c  ->  feature 16(j/8) + 8h + j%8, j = 16t + 4a + c
+__device__ __forceinline__ int phi(int m) {
+  const int t = m >> 5, a = (m >> 3) & 3, h = (m >> 2) & 1, c = m & 3;
+  const int j = 16 * t + 4 * a + c;
+  return (j >> 3) * 16 + h * 8 + (j & 7);
+}
+
+// ---------------------------------------------------------------------------
+// Gram pass: S.G = X^T diag(lambda) X + I / prior_var for the first D
+// features, identity on the padding, from sqrt(lambda) in `sl`.
+// ---------------------------------------------------------------------------
+template <int NB>
+__device__ __forceinline__ void gram_pass(Smem& S, const float* __restrict__ Xg, const float* __restrict__ sl, int npad, int D,
+                          float inv_prior) {
+  constexpr int NTL = NB * (NB + 1) / 2;
+  const int l = threadIdx.x;
+  f32x16 acc[NTL];
+#pragma unroll
+  for (int t = 0; t < NTL; ++t) acc[t] = f32x16{0};
+  const float* slp = sl + (l / 32) * 8;
+  const float* xp = Xg + (size_t)l * 8;
+  const int nkb = npad / 16;
+  for (int kb = 0; kb < nkb; ++kb) {
+    float s8[8];
+    load8(slp + kb * 16, s8);
+    bf16x8 hi[NB], lo[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      float x8[8];
+      load8(xp + (size_t)(kb * NB + b) * 512, x8);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) x8[i] *= s8[i];
+      split8(x8, hi[b], lo[b]);
+    }
+    int t = 0;
+#pragma unroll
+    for (int bi = 0; bi < NB; ++bi)
+#pragma unroll
+      for (int bj = 0; bj <= bi; ++bj) {
+        acc[t] = mfma3(hi[bi], lo[bi], hi[bj], lo[bj], acc[t]);
+        ++t;
+      }
+  }
+  __syncthreads();
+  int t = 0;
+#pragma unroll
+  for (int bi = 0; bi < NB; ++bi)
+#pragma unroll
+    for (int bj = 0; bj <= bi; ++bj) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int i = bi * 32 + (r / 4) * 8 + (l / 32) * 4 + (r % 4);
+        const int j = bj * 32 + (l % 32);
+        // a diagonal tile holds (i, j) and (j, i) rounded differently: keep the lower one
+        if (bi == bj && i < j) continue;
+        float v = acc[t][r];
+        if (i == j) v = i < D ? v + inv_prior : 1.0f;
+        S.G[i * LDG + j] = v;
+        S.G[j * LDG + i] = v;
+      }
+      ++t;
+    }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Sweep of the DP x DP matrix in S.G (left intact): lane jl = l % DP holds
+// column jl; on return A = -G^-1 (column jl) and S.dk the pivots.  With
+// want_chol, p0j = (L z)_jl with L = chol(G) and z in S.z.
+// ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Sweep of the DP x DP matrix in S.G (left intact).  Lane jl = l % DP holds
+// column jl in registers A[b][i] = A[32b + i][jl]; the pivot loop is rolled
+// and reads/writes A[k] with a wave-uniform k (register-relative moves), the
+// update of the other entries is unrolled.  On return A = -G^-1 and S.dk the
+// pivots.  With want_chol, p0j = (L z)_jl with L = chol(G) and z in S.z.
+// ---------------------------------------------------------------------------
+template <int NB>
+__device__ __forceinline__ float col_get(const f32x32 (&A)[NB], int k) {
+  if constexpr (NB == 1) {
+    return A[0][k];
+  } else {
+    float v;
+    if (k < 32)
+      v = A[0][k];
+    else
+      v = A[1][k - 32];
+    return v;
+  }
+}
+template <int NB>
+__device__ __forceinline__ void col_set(f32x32 (&A)[NB], int k, float v) {
+  if constexpr (NB == 1) {
+    A[0][k] = v;
+  } else {
+    if (k < 32)
+      A[0][k] = v;
+    else
+      A[1][k - 32] = v;
+  }
+}
+
+template <int NB>
+__device__ __forceinline__ void sweep(Smem& S, f32x32 (&A)[NB], int D, bool want_chol, float& p0j) {
+  constexpr int DP = 32 * NB;
+  const int jl = threadIdx.x % DP;
+#pragma unroll
+  for (int b = 0; b < NB; ++b)
+#pragma unroll
+    for (int i = 0; i < 32; ++i) A[b][i] = S.G[(32 * b + i) * LDG + jl];
+  p0j = 0.0f;
+  // padding pivots (k >= D) are 1 with zero off-diagonals: nothing to do
+  for (int k = 0; k < D; ++k) {
+    const float ak = col_get<NB>(A, k);  // A[k][jl] = A[jl][k]
+    float* bc = S.bc[k & 1];
+    bc[jl] = ak;
+    __syncthreads();
+    float col[DP];
+#pragma unroll
+    for (int i = 0; i < DP; i += 4) {
+      const float4 v = *reinterpret_cast<const float4*>(bc + i);
+      col[i] = v.x; col[i + 1] = v.y; col[i + 2] = v.z; col[i + 3] = v.w;
+    }
+    const float d = bc[k];
+    const float rd = 1.0f / d;
+    if (want_chol && jl >= k) p0j = fmaf(ak / sqrtf(d), S.z[k], p0j);
+    // lane k's own column equals col, so f = 1 - 1/d turns its update into A[i][k] / d;
+    // entry k of every column is overwritten below
+    const float f = (jl == k) ? 1.0f - rd : ak * rd;
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int i = 0; i < 32; ++i) A[b][i] = fmaf(-col[32 * b + i], f, A[b][i]);
+    col_set<NB>(A, k, (jl == k) ? -rd : ak * rd);
+    if (threadIdx.x == 0) S.dk[k] = d;
+  }
+  // -G^-1 on the padding block is -I
+  for (int k = D; k < DP; ++k) col_set<NB>(A, k, jl == k ? -1.0f : 0.0f);
+  if (threadIdx.x < DP && threadIdx.x >= D) S.dk[threadIdx.x] = 1.0f;
+  __syncthreads();
+}
+
+// log det G = sum log d_k in float64 (padding pivots are 1)
+__device__ __forceinline__ double logdet_from_pivots(const Smem& S, int D) {
+  const int l = threadIdx.x;
+  double v = l < D ? log((double)S.dk[l]) : 0.0;
+  return group_sum<64>(v);
+}
+
+// S.u = G^-1 S.p with one refinement step against the Gram still in S.G.
+template <int NB>
+__device__ __forceinline__ void ginv_apply(Smem& S, const f32x32 (&A)[NB]) {
+  constexpr int DP = 32 * NB;
+  const int l = threadIdx.x, jl = l % DP;
+  float uj = 0.0f;
+#pragma unroll
+  for (int b = 0; b < NB; ++b)
+#pragma unroll
+    for (int i = 0; i < 32; ++i) uj = fmaf(-A[b][i], S.p[32 * b + i], uj);
+  if (l < DP) S.u[jl] = uj;
+  __syncthreads();
+  float gu = 0.0f;
+#pragma unroll
+  for (int i = 0; i < DP; ++i) gu = fmaf(S.G[i * LDG + jl], S.u[i], gu);
+  if (l < DP) S.r[jl] = S.p[jl] - gu;
+  __syncthreads();
+  float du = 0.0f;
+#pragma unroll
+  for (int b = 0; b < NB; ++b)
+#pragma unroll
+    for (int i = 0; i < 32; ++i) du = fmaf(-A[b][i], S.r[32 * b + i], du);
+  __syncthreads();
+  if (l < DP) S.u[jl] = uj + du;
+  __syncthreads();
+}
+
+// S.G = G^-1 (overwrites the Gram)
+template <int NB>
+__device__ __forceinline__ void store_ginv(Smem& S, const f32x32 (&A)[NB]) {
+  const int l = threadIdx.x;
+  if (l < 32 * NB) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int i = 0; i < 32; ++i) S.G[(32 * b + i) * LDG + l] = -A[b][i];
+  }
+  __syncthreads();
+}
+
+// G (in S.G) -> pivots in S.dk, S.u = G^-1 S.p (refined), and with
+// keep_inverse S.G = G^-1.  With want_chol the momentum p0 = chol(G) S.z is
+// drawn first into S.p0 and S.p.
+template <int NB>
+__device__ __forceinline__ void invert(Smem& S, int D, bool want_chol, bool keep_inverse) {
+  f32x32 A[NB];
+  float p0j;
+  sweep<NB>(S, A, D, want_chol, p0j);
+  if (want_chol) {
+    const int l = threadIdx.x;
+    const float pv = l < D ? p0j : 0.0f;
+    S.p0[l] = pv;
+    S.p[l] = pv;
+    __syncthreads();
+  }
+  ginv_apply<NB>(S, A);
+  if (keep_inverse) store_ginv<NB>(S, A);
+}
+
+// Sum the per-lane feature partials of a row pass over the 32 lanes of each
+// half wave: lane f < DP receives the total for feature f.
+template <int NB>
+__device__ __forceinline__ float reduce_features(Smem& S, const float (&acc)[2 * NB][8]) {
+  constexpr int KC = 2 * NB, RS = KC * 8 + 1;
+  const int l = threadIdx.x;
+  float* R = S.G;
+  __syncthreads();
+#pragma unroll
+  for (int kc = 0; kc < KC; ++kc)
+#pragma unroll
+    for (int i = 0; i < 8; ++i) R[l * RS + kc * 8 + i] = acc[kc][i];
+  __syncthreads();
+  float s = 0.0f;
+  if (l < 32 * NB) {
+    const int kc = l / 16, h = (l / 8) % 2, i = l % 8;
+    for (int j = 0; j < 32; ++j) s += R[(h * 32 + j) * RS + kc * 8 + i];
+  }
+  __syncthreads();
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// One evaluation of the vector field at (S.q, S.p) with S.u = G^-1 p and
+// G^-1 in S.G: S.dH = dH/dq.  Also advances the scratch: E1 <- eta of the
+// next iterate (base + h X u, base = E0 for a fixed-point iterate, E1 for
+// the explicit half step, which also sets E0), SL <- sqrt(lambda) there.
+// ---------------------------------------------------------------------------
+template <int NB>
+__device__ __forceinline__ void field_pass(Smem& S, const float* __restrict__ Xr, const float* __restrict__ yf, float* E0,
+                           float* E1, float* SL, int npad, int D, float hs, float inv_prior, bool explicit_step) {
+  constexpr int KC = 2 * NB;
+  const int l = threadIdx.x, h = l / 32;
+  bf16x8 Ah[NB][KC], Al[NB][KC];
+#pragma unroll
+  for (int t = 0; t < NB; ++t) {
+    const int row = phi(t * 32 + (l % 32));
+#pragma unroll
+    for (int kc = 0; kc < KC; ++kc) {
+      float v[8];
+      load8(S.G + row * LDG + kc * 16 + h * 8, v);
+      split8(v, Ah[t][kc], Al[t][kc]);
+    }
+  }
+  float uv[KC][8], dacc[KC][8];
+#pragma unroll
+  for (int kc = 0; kc < KC; ++kc)
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      uv[kc][i] = S.u[kc * 16 + h * 8 + i];
+      dacc[kc][i] = 0.0f;
+    }
+  const float* xp = Xr + (size_t)l * 8;
+  const int nrb = npad / 32;
+  for (int rb = 0; rb < nrb; ++rb) {
+    float x[KC][8];
+#pragma unroll
+    for (int kc = 0; kc < KC; ++kc) load8(xp + (size_t)(rb * KC + kc) * 512, x[kc]);
+    const int row = rb * 32 + (l % 32);
+    const float eta = E1[row];
+    const float base = explicit_step ? eta : E0[row];
+    const float yv = yf[row];
+    float bp = 0.0f;
+#pragma unroll
+    for (int kc = 0; kc < KC; ++kc)
+#pragma unroll
+      for (int i = 0; i < 8; ++i) bp = fmaf(x[kc][i], uv[kc][i], bp);
+    const float b = bp + __shfl_xor(bp, 32, 64);
+    bf16x8 xh[KC], xl[KC];
+#pragma unroll
+    for (int kc = 0; kc < KC; ++kc) split8(x[kc], xh[kc], xl[kc]);
+    f32x16 acc[NB];
+#pragma unroll
+    for (int t = 0; t < NB; ++t) {
+      acc[t] = f32x16{0};
+#pragma unroll
+      for (int kc = 0; kc < KC; ++kc) acc[t] = mfma3(Ah[t][kc], Al[t][kc], xh[kc], xl[kc], acc[t]);
+    }
+    float ap = 0.0f;
+#pragma unroll
+    for (int t = 0; t < NB; ++t)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int j = 16 * t + r;
+        ap = fmaf(acc[t][r], x[j / 8][j % 8], ap);
+      }
+    const float an = ap + __shfl_xor(ap, 32, 64);
+    const float s = sigmoidf(eta);
+    const float w = s * (1.0f - s) * (1.0f - 2.0f * s);
+    const float hv = 0.5f * w * (an - b * b) - (yv - s);
+#pragma unroll
+    for (int kc = 0; kc < KC; ++kc)
+#pragma unroll
+      for (int i = 0; i < 8; ++i) dacc[kc][i] = fmaf(hv, x[kc][i], dacc[kc][i]);
+    if (h == 0) {
+      const float en = fmaf(hs, b, base);
+      const float sn = sigmoidf(en);
+      E1[row] = en;
+      if (explicit_step) E0[row] = en;
+      SL[row] = sqrtf(sn * (1.0f - sn));
+    }
+  }
+  const float tot = reduce_features<NB>(S, dacc);
+  if (l < MAXD) S.dH[l] = l < D ? tot + S.q[l] * inv_prior : 0.0f;
+  __syncthreads();
+}
+
+// eta = X q (float64 accumulation) into Eout, sqrt(lambda) into SL; with
+// want_grad also the log likelihood (float64, returned) and
+// S.g = X^T (y - s) - q / prior_var.
+template <int NB>
+__device__ __forceinline__ double fresh_pass(Smem& S, const float* __restrict__ Xr, const float* __restrict__ yf, int N, int npad,
+                             int D, float* Eout, float* SL, bool want_grad, float inv_prior) {
+  constexpr int KC = 2 * NB;
+  const int l = threadIdx.x, h = l / 32;
+  double qv[KC][8];
+  float gacc[KC][8];
+#pragma unroll
+  for (int kc = 0; kc < KC; ++kc)
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      qv[kc][i] = (double)S.q[kc * 16 + h * 8 + i];
+      gacc[kc][i] = 0.0f;
+    }
+  double ll = 0.0;
+  const float* xp = Xr + (size_t)l * 8;
+  const int nrb = npad / 32;
+  for (int rb = 0; rb < nrb; ++rb) {
+    float x[KC][8];
+#pragma unroll
+    for (int kc = 0; kc < KC; ++kc) load8(xp + (size_t)(rb * KC + kc) * 512, x[kc]);
+    double ep = 0.0;
+#pragma unroll
+    for (int kc = 0; kc < KC; ++kc)
+#pragma unroll
+      for (int i = 0; i < 8; ++i) ep = fma((double)x[kc][i], qv[kc][i], ep);
+    const double eta = ep + __shfl_xor(ep, 32, 64);
+    const int row = rb * 32 + (l % 32);
+    const double s = 1.0 / (1.0 + exp(-eta));
+    if (want_grad) {
+      const bool ok = row < N;
+      const double yv = (double)yf[row];
+      if (ok && h == 0) ll += yv * eta - softplus(eta);
+      const float rr = ok ? (float)(yv - s) : 0.0f;
+#pragma unroll
+      for (int kc = 0; kc < KC; ++kc)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) gacc[kc][i] = fmaf(rr, x[kc][i], gacc[kc][i]);
+    }
+    if (h == 0) {
+      Eout[row] = (float)eta;
+      SL[row] = (float)sqrt(s * (1.0 - s));
+    }
+  }
+  if (want_grad) {
+    const float tot = reduce_features<NB>(S, gacc);
+    if (l < MAXD) S.g[l] = l < D ? tot - S.q[l] * inv_prior : 0.0f;
+  }
+  __syncthreads();
+  return group_sum<64>(ll);
+}
+
+// E0 = E1 = ES, SL = sqrt(lambda(ES))
+__device__ __forceinline__ void prep_pass(const float* ES, float* E0, float* E1, float* SL, int npad) {
+  for (int row = threadIdx.x; row < npad; row += 64) {
+    const float e = ES[row];
+    const float s = sigmoidf(e);
+    E0[row] = e;
+    E1[row] = e;
+    SL[row] = sqrtf(s * (1.0f - s));
+  }
+  __syncthreads();
+}
+
+enum Stage { kInit, kNoise, kTraj, kEnd };
+
+// All transitions of one chain.  The trajectory is a flat loop of vector-field
+// evaluations with a single site for each pass (Gram, sweep, field, fresh),
+// which keeps one copy of each unrolled pass in the kernel.
+template <int NB, bool STEP>
+__device__ __forceinline__ void run_chain(const LogisticArgs& a, Smem& S, int c, int post) {
+  const int l = threadIdx.x;
+  const PackedPost pp = a.packed_hdr[post];
+  const int N = a.post_N[post], D = a.post_D[post];
+  const float* Xg = a.packed + pp.xg;
+  const float* Xr = a.packed + pp.xr;
+  const float* yf = a.packed + pp.yf;
+  const int npad = pp.npad;
+  const int64_t ns = a.work_rows;
+  float* const W = a.work + (size_t)c * 5 * ns;
+  float* const E0 = W;
+  float* const E1 = W + ns;
+  float* const SL = W + 2 * ns;
+  float* const ESa = W + 3 * ns;
+  float* const ESb = W + 4 * ns;
+  int cur = 0;  // eta of the chain state lives in cur ? ESb : ESa
+  const int Dp = a.Dmax;
+  const float hs = (float)(0.5 * a.eps);
+  const float inv_prior = (float)(1.0 / a.prior_var);
+  const double prior = a.prior_var;
+  const double lp_const = -0.5 * (double)D * log(2.0 * M_PI * prior);
+  const double ke_const = 0.5 * (double)D * kLog2Pi;
+  const int T_steps = STEP ? 1 : a.T;
+
+  {
+    const float qv = l < D ? static_cast<const float*>(a.q_in)[(size_t)c * Dp + l] : 0.0f;
+    S.q0[l] = qv;
+    S.q[l] = qv;
+  }
+  double logp = a.logp_in[c];
+  int n_acc = 0;
+  int64_t n_fev = 0;
+  __syncthreads();
+
+  int stage = kInit, tr = 0, s = 0, mode = 0, n = 0;
+  bool have_ginv = false;
+  float q0v = 0.0f, p0v = 0.0f;
+  double uacc = 0.0, H0 = 0.0, loglik = 0.0, norm = 0.0;
+  for (;;) {
+    // ---- eta, log likelihood, gradient from X (initial state / end of trajectory) ----
+    if (stage == kInit || stage == kEnd) {
+      const bool end = stage == kEnd;
+      float* Eout = (end ? 1 - cur : cur) ? ESb : ESa;
+      loglik = fresh_pass<NB>(S, Xr, yf, N, npad, D, Eout, SL, end || !STEP, inv_prior);
+      if (!end) {
+        if (!STEP) S.gs[l] = S.g[l];
+        stage = kNoise;
+      }
+    }
+    // ---- noise, chain state, scratch ----
+    if (stage == kNoise) {
+      const uint64_t tt = a.counter + (uint64_t)tr;
+      if (a.z != nullptr) {
+        S.z[l] = l < D ? static_cast<const float*>(a.z)[((size_t)tr * a.C + c) * Dp + l] : 0.0f;
+        uacc = a.u[(size_t)tr * a.C + c];
+      } else {
+        if (l < MAXD / 2) {
+          u32x4 r = philox4x32_10(u32x4{(uint32_t)c, (uint32_t)tt, kStreamMomentum + (uint32_t)(2 * l),
+                                        (uint32_t)(tt >> 32)}, (uint32_t)a.seed, (uint32_t)(a.seed >> 32));
+          double n0, n1;
+          normal2(r, n0, n1);
+          S.z[2 * l] = 2 * l < D ? (float)n0 : 0.0f;
+          S.z[2 * l + 1] = 2 * l + 1 < D ? (float)n1 : 0.0f;
+        }
+        u32x4 r = philox4x32_10(u32x4{(uint32_t)c, (uint32_t)tt, kStreamAccept, (uint32_t)(tt >> 32)},
+                                (uint32_t)a.seed, (uint32_t)(a.seed >> 32));
+        uacc = u01_f64(r.x, r.y) - 0.5 * (1.0 / 9007199254740992.0);
+      }
+      S.q[l] = S.q0[l];
+      __syncthreads();
+      prep_pass(cur ? ESb : ESa, E0, E1, SL, npad);
+    }
+    // ---- metric at S.q: G, its sweep, u = G^-1 p (momentum draw first at kNoise) ----
+    if (stage != kTraj || !have_ginv) {
+      gram_pass<NB>(S, Xg, SL, npad, D, inv_prior);
+      invert<NB>(S, D, stage == kNoise, stage != kEnd);
+    }
+    if (stage == kNoise) {
+      if (STEP && a.momentum != nullptr && l < D) static_cast<float*>(a.momentum)[(size_t)c * Dp + l] = S.p0[l];
+      const double logdet = logdet_from_pivots(S, D);
+      const double pu = group_sum<64>(l < D ? (double)S.p[l] * (double)S.u[l] : 0.0);
+      H0 = -logp + 0.5 * pu + 0.5 * logdet + ke_const;
+      have_ginv = true;  // S.G = G(S.q)^-1 and S.u = G^-1 S.p
+      s = 0;
+      mode = 0;
+      stage = a.L > 0 ? kTraj : kEnd;
+      if (stage == kEnd) continue;
+    }
+    // ---- energy, accept, outputs ----
+    if (stage == kEnd) {
+      const double logdet = logdet_from_pivots(S, D);
+      const double qq = group_sum<64>(l < D ? (double)S.q[l] * (double)S.q[l] : 0.0);
+      const double pu = group_sum<64>(l < D ? (double)S.p[l] * (double)S.u[l] : 0.0);
+      const double logp1 = loglik - 0.5 * qq / prior + lp_const;
+      const double H1 = -logp1 + 0.5 * pu + 0.5 * logdet + ke_const;
+      double dHe = H0 - H1;
+      if (isnan(dHe)) dHe = -INFINITY;
+      const double p_acc = fmin(exp(dHe), 1.0);
+      const bool accept = uacc < p_acc;
+      if (STEP) {
+        if (l == 0) {
+          if (a.acc_rate) a.acc_rate[c] = p_acc;
+          if (a.accepted) a.accepted[c] = (uint8_t)accept;
+          if (a.divergent) a.divergent[c] = (uint8_t)(-dHe > a.div_threshold);
+          if (a.energy) a.energy[c] = H1;
+          if (a.prop_logp) a.prop_logp[c] = logp1;
+          if (a.prop_weight) a.prop_weight[c] = dHe;
+          if (a.prop_slpa) a.prop_slpa[c] = fmin(dHe, 0.0);
+          a.logp_out[c] = accept ? logp1 : logp;
+        }
+        if (l < D) {
+          const size_t i = (size_t)c * Dp + l;
+          if (a.prop_q) static_cast<float*>(a.prop_q)[i] = S.q[l];
+          if (a.prop_p) static_cast<float*>(a.prop_p)[i] = -S.p[l];
+          if (a.prop_v) static_cast<float*>(a.prop_v)[i] = -S.u[l];
+          if (a.prop_g) static_cast<float*>(a.prop_g)[i] = S.g[l];
+          static_cast<float*>(a.q_out)[i] = accept ? S.q[l] : S.q0[l];
+          if (a.g_out)
+            static_cast<float*>(a.g_out)[i] = accept ? S.g[l] : static_cast<const float*>(a.g_in)[i];
+        }
+      }
+      if (accept) {
+        S.q0[l] = S.q[l];
+        S.gs[l] = S.g[l];
+        logp = logp1;
+        cur = 1 - cur;
+        ++n_acc;
+      }
+      __syncthreads();
+      if (!STEP) {
+        if (a.samples != nullptr && l < D)
+          static_cast<float*>(a.samples)[((size_t)tr * a.C + c) * Dp + l] = S.q0[l];
+        if (a.sample_acc != nullptr && l == 0) a.sample_acc[(size_t)tr * a.C + c] = (float)p_acc;
+      }
+      if (++tr == T_steps) break;
+      stage = kNoise;
+      continue;
+    }
+    // ---- stage kTraj: one evaluation of the vector field ----
+    if (mode == 0) {  // z0 of step s (integrators.py:84 evaluates once outside the loop)
+      q0v = S.q[l];
+      p0v = S.p[l];
+    }
+    field_pass<NB>(S, Xr, yf, E0, E1, SL, npad, D, hs, inv_prior, mode == 2);
+    have_ginv = false;
+    ++n_fev;
+    if (mode == 2) {  // explicit half step from the last iterate (integrators.py:146-148)
+      if (l < D) {
+        S.q[l] = fmaf(hs, S.u[l], S.q[l]);
+        S.p[l] = fmaf(-hs, S.dH[l], S.p[l]);
+      }
+      __syncthreads();
+      mode = 0;
+      if (++s == a.L) stage = kEnd;
+      continue;
+    }
+    {
+      float dq = 0.0f;
+      if (l < D) {
+        const float qn = fmaf(hs, S.u[l], q0v);
+        const float pn = fmaf(-hs, S.dH[l], p0v);
+        dq = nan_max(fabsf(qn - S.q[l]), fabsf(pn - S.p[l]));
+        S.q[l] = qn;
+        S.p[l] = pn;
+      }
+      norm = (double)group_reduce<64, NanMax>(dq);
+      __syncthreads();
+    }
+    if (mode == 0) {
+      n = 0;
+      mode = 1;
+    } else {
+      ++n;
+    }
+    if (!(n < a.max_iters && isfinite(norm) && norm < a.div_tol && norm > a.conv_tol)) {
+      if (STEP && l == 0) {
+        const size_t idx = (size_t)c * a.L + s;
+        if (a.fp_iters) a.fp_iters[idx] = n;
+        if (a.fp_norm) a.fp_norm[idx] = norm;
+        if (a.fp_success) a.fp_success[idx] = (uint8_t)(isfinite(norm) && norm <= a.conv_tol);
+      }
+      mode = 2;
+    }
+  }
+
+  if (!STEP) {
+    if (l < D) {
+      static_cast<float*>(a.final_q)[(size_t)c * Dp + l] = S.q0[l];
+      if (a.final_g) static_cast<float*>(a.final_g)[(size_t)c * Dp + l] = S.gs[l];
+    }
+    if (l == 0) {
+      a.final_logp[c] = logp;
+      if (a.num_accepted) a.num_accepted[c] = n_acc;
+      if (a.num_fevals) a.num_fevals[c] = n_fev;
+    }
+  }
+}
+
+// One wave per chain; one instantiation per tile width NB (chains of the
+// other width leave at once).  Workgroups are dealt to the 8 XCDs round
+// robin, so block b runs chain (b % 8) * (blocks / 8) + b / 8: each XCD takes
+// a contiguous range of chains -- for a multi-posterior batch, a few
+// posteriors, whose packed X then stays in that XCD's L2.
+template <int NB, bool STEP>
+__global__ void __launch_bounds__(64) logistic_rmhmc_f32_kernel(const LogisticArgs a) {
+  __shared__ Smem S;
+  const int per_xcd = gridDim.x / 8;
+  const int c = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+  if (c >= a.C) return;
+  const int post = a.chain_post ? a.chain_post[c] : 0;
+  if (post < 0 || post >= a.P) return;  // invalid posterior index: chain left untouched
+  if (a.packed_hdr[post].nb != NB) return;
+  run_chain<NB, STEP>(a, S, c, post);
+}
+
+// Pack posterior p (blockIdx.y) into the two fragment layouts and float labels.
+__global__ void logistic_pack_kernel(const double* __restrict__ X, const double* __restrict__ y,
+                                     const int32_t* __restrict__ Nv, const int32_t* __restrict__ Dv,
+                                     const int64_t* __restrict__ xoff, const int64_t* __restrict__ yoff,
+                                     const PackedPost* __restrict__ hdr, float* __restrict__ packed) {
+  const int p = blockIdx.y;
+  const PackedPost pp = hdr[p];
+  const int N = Nv[p], D = Dv[p];
+  const int DP = 32 * pp.nb, KC = 2 * pp.nb;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)pp.npad * DP) return;
+  const int n = (int)(e / DP), f = (int)(e % DP);
+  const float v = (n < N && f < D) ? (float)X[xoff[p] + (int64_t)n * D + f] : 0.0f;
+  {
+    const int kb = n / 16, w = n % 16, b = f / 32;
+    const int lane = (w / 8) * 32 + f % 32, i = w % 8;
+    packed[pp.xg + (((int64_t)kb * pp.nb + b) * 64 + lane) * 8 + i] = v;
+  }
+  {
+    const int rb = n / 32, kc = f / 16;
+    const int lane = ((f / 8) % 2) * 32 + n % 32, i = f % 8;
+    packed[pp.xr + (((int64_t)rb * KC + kc) * 64 + lane) * 8 + i] = v;
+  }
+  if (f == 0) packed[pp.yf + n] = n < N ? (float)y[yoff[p] + n] : 0.0f;
+}
+
+}  // namespace lfast
+
+uint64_t logistic_pack_layout(int P, const int32_t* N_host, const int32_t* D_host, PackedPost* hdr_out) {
+  const uint64_t hdr_floats = (((uint64_t)P * sizeof(PackedPost) + 255) / 256) * 64;
+  uint64_t off = hdr_floats;
+  for (int p = 0; p < P; ++p) {
+    PackedPost h;
+    h.npad = (N_host[p] + 31) / 32 * 32;
+    h.nb = D_host[p] <= 32 ? 1 : 2;
+    const uint64_t xs = (uint64_t)h.npad * 32 * h.nb;
+    h.xg = (int64_t)off;
+    off += xs;
+    h.xr = (int64_t)off;
+    off += xs;
+    h.yf = (int64_t)off;
+    off += ((uint64_t)h.npad + 63) / 64 * 64;
+    if (hdr_out) hdr_out[p] = h;
+  }
+  return off * sizeof(float);
+}
+
+hipError_t launch_logistic_pack(const double* X, const double* y, const int32_t* N, const int32_t* D,
+                                const int64_t* xoff, const int64_t* yoff, const PackedPost* hdr, float* packed,
+                                int P, int max_npad, hipStream_t stream) {
+  const int64_t elems = (int64_t)max_npad * 64;
+  const dim3 grid((unsigned)((elems + 255) / 256), (unsigned)P);
+  hipLaunchKernelGGL(lfast::logistic_pack_kernel, grid, dim3(256), 0, stream, X, y, N, D, xoff, yoff, hdr, packed);
+  return hipGetLastError();
+}
+
+hipError_t launch_logistic_rmhmc_f32(const LogisticArgs& a, int max_features, hipStream_t stream) {
+  const int blocks = (a.C + 7) / 8 * 8;
+  // every posterior has D <= max_features: skip the wide kernel when all are narrow
+  if (a.q_out != nullptr) {
+    hipLaunchKernelGGL((lfast::logistic_rmhmc_f32_kernel<1, true>), dim3(blocks), dim3(64), 0, stream, a);
+    if (max_features > 32)
+      hipLaunchKernelGGL((lfast::logistic_rmhmc_f32_kernel<2, true>), dim3(blocks), dim3(64), 0, stream, a);
+  } else {
+    hipLaunchKernelGGL((lfast::logistic_rmhmc_f32_kernel<1, false>), dim3(blocks), dim3(64), 0, stream, a);
+    if (max_features > 32)
+      hipLaunchKernelGGL((lfast::logistic_rmhmc_f32_kernel<2, false>), dim3(blocks), dim3(64), 0, stream, a);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace gjx

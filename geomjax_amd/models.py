@@ -6,6 +6,8 @@ A GPU kernel cannot trace a Python callable, so the boundary accepts a closed
 set of model descriptors whose log density, gradient and Hessian-vector
 products are written in closed form inside the kernels.
 """
+import ctypes
+
 import torch
 
 from . import _lib
@@ -95,8 +97,8 @@ class LogisticRegression:
     def dim(self):
         return self.D
 
-    def data(self, chain_posterior=None):
-        return self._table.struct(chain_posterior)
+    def data(self, chain_posterior=None, *, dtype=torch.float64, num_chains=None):
+        return self._table.struct(chain_posterior, dtype=dtype, num_chains=num_chains)
 
     def value_and_grad(self, position):
         return logistic_value_and_grad(self, position)
@@ -144,8 +146,8 @@ class LogisticBatch:
     def num_chains(self):
         return int(sum(self.counts))
 
-    def data(self, chain_posterior=None):
-        return self._table.struct(self.chain_posterior)
+    def data(self, chain_posterior=None, *, dtype=torch.float64, num_chains=None):
+        return self._table.struct(self.chain_posterior, dtype=dtype, num_chains=num_chains)
 
     def value_and_grad(self, position):
         return logistic_value_and_grad(self, position)
@@ -168,17 +170,55 @@ class _posterior_table:
         self.P = len(models)
         self.max_rows = max(m.N for m in models)
         self.max_features = max(m.D for m in models)
+        self._host_N = [m.N for m in models]
+        self._host_D = [m.D for m in models]
+        self._packed = None     # float32 fragment table (built on first float32 use)
+        self._workspace = None  # per-chain scratch of the float32 path
 
-    def struct(self, chain_posterior):
+    def _base(self, chain_posterior):
         return _lib.LogisticData(num_posteriors=self.P, max_rows=self.max_rows, max_features=self.max_features,
                                  reserved=0, num_rows=self.N.data_ptr(), num_features=self.Dp.data_ptr(),
                                  x_offset=self.xoff.data_ptr(), y_offset=self.yoff.data_ptr(),
-                                 chain_posterior=_lib.ptr(chain_posterior), x=self.x.data_ptr(), y=self.y.data_ptr())
+                                 chain_posterior=_lib.ptr(chain_posterior), x=self.x.data_ptr(), y=self.y.data_ptr(),
+                                 packed=None, workspace=None, workspace_bytes=0)
+
+    def packed(self):
+        """The device table of X in the float32 kernel's fragment layouts (gjx_logistic_pack)."""
+        if self._packed is None:
+            lib = _lib.load()
+            n = (ctypes.c_int32 * self.P)(*self._host_N)
+            d = (ctypes.c_int32 * self.P)(*self._host_D)
+            nbytes = ctypes.c_uint64()
+            _lib.check(lib.gjx_logistic_pack_size(self.P, n, d, ctypes.byref(nbytes)), "gjx_logistic_pack_size")
+            buf = torch.empty(int(nbytes.value), dtype=torch.uint8, device=self.x.device)
+            rc = lib.gjx_logistic_pack(self._base(None), n, d, buf.data_ptr(), nbytes.value,
+                                       _lib.stream_handle(self.x.device))
+            _lib.check(rc, "gjx_logistic_pack")
+            self._packed = buf
+        return self._packed
+
+    def workspace(self, num_chains):
+        need = num_chains * 5 * ((self.max_rows + 31) // 32 * 32) * 4
+        if self._workspace is None or self._workspace.numel() < need:
+            self._workspace = torch.empty(need, dtype=torch.uint8, device=self.x.device)
+        return self._workspace
+
+    def struct(self, chain_posterior, dtype=torch.float64, num_chains=None):
+        st = self._base(chain_posterior)
+        if dtype == torch.float32:
+            if num_chains is None:
+                raise ValueError("the float32 path needs num_chains to size its workspace")
+            ws = self.workspace(num_chains)
+            st.packed = self.packed().data_ptr()
+            st.workspace = ws.data_ptr()
+            st.workspace_bytes = ws.numel()
+        return st
 
 
 def logistic_params(model, C, L=0, T=1, step_size=0.0, max_iters=0, convergence_tol=1e-6, divergence_tol=1e10,
-                    divergence_threshold=1000.0, seed=0, counter=0):
-    return _lib.LogisticParams(num_chains=C, dim=model.dim, dtype=_lib.GJX_F64, num_integration_steps=int(L),
+                    divergence_threshold=1000.0, seed=0, counter=0, dtype=torch.float64):
+    code = _lib.GJX_F32 if dtype == torch.float32 else _lib.GJX_F64
+    return _lib.LogisticParams(num_chains=C, dim=model.dim, dtype=code, num_integration_steps=int(L),
                                max_iters=int(max_iters), num_transitions=int(T), flags=0, reserved=0,
                                step_size=float(step_size), prior_var=model.prior_var,
                                convergence_tol=float(convergence_tol), divergence_tol=float(divergence_tol),
@@ -187,10 +227,17 @@ def logistic_params(model, C, L=0, T=1, step_size=0.0, max_iters=0, convergence_
 
 
 def logistic_value_and_grad(model, position):
+    """Log density (float64) and gradient (in the position's dtype) of C positions.
+
+    float32 positions are evaluated by the float64 kernel on a widened copy
+    (this is the reference's init, not the sampling loop)."""
     if not position.is_cuda:
         raise ValueError("positions must be on a HIP device (cuda:N)")
-    if position.dtype != torch.float64:
-        raise TypeError("the logistic kernels run in float64")
+    if position.dtype not in (torch.float64, torch.float32):
+        raise TypeError("logistic positions must be float64 or float32")
+    if position.dtype == torch.float32:
+        logp, grad = logistic_value_and_grad(model, position.double())
+        return logp, grad.float()
     pos = position.contiguous()
     squeeze = pos.dim() == 1
     if squeeze:

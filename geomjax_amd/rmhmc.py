@@ -117,8 +117,8 @@ def _problem(logdensity_fn, metric_fn, position):
     if isinstance(logdensity_fn, (LogisticRegression, LogisticBatch)) and isinstance(metric_fn, FisherMetric):
         if position.shape[-1] != logdensity_fn.dim:
             raise ValueError(f"position has {position.shape[-1]} coordinates, the model has {logdensity_fn.dim}")
-        if position.dtype != torch.float64:
-            raise TypeError("the logistic RMHMC kernel runs in float64")
+        if position.dtype not in (torch.float64, torch.float32):
+            raise TypeError("logistic RMHMC states are float64 (plain) or float32 (tuned)")
         if isinstance(logdensity_fn, LogisticBatch) and position.shape[0] != logdensity_fn.num_chains:
             raise ValueError(f"the batch has {logdensity_fn.num_chains} chains, got {position.shape[0]}")
         return "logistic_fisher"
@@ -192,9 +192,10 @@ def build_kernel(integrator=implicit_midpoint, divergence_threshold: float = 100
             params = logistic_params(logdensity_fn, C, L=L, step_size=step_size, max_iters=solver["max_iters"],
                                      convergence_tol=solver["convergence_tol"],
                                      divergence_tol=solver["divergence_tol"],
-                                     divergence_threshold=divergence_threshold, seed=seed, counter=counter)
-            rc = _lib.load().gjx_rmhmc_fisher_logistic_step(params, logdensity_fn.data(), io,
-                                                            _lib.stream_handle(dev))
+                                     divergence_threshold=divergence_threshold, seed=seed, counter=counter,
+                                     dtype=position.dtype)
+            data = logdensity_fn.data(dtype=position.dtype, num_chains=C)
+            rc = _lib.load().gjx_rmhmc_fisher_logistic_step(params, data, io, _lib.stream_handle(dev))
             _lib.check(rc, "gjx_rmhmc_fisher_logistic_step")
 
         def sq(t):
@@ -281,8 +282,9 @@ def sample(rng_key, state, logdensity_fn, step_size, metric_fn, num_integration_
         params = logistic_params(logdensity_fn, C, L=num_integration_steps, T=T, step_size=step_size,
                                  max_iters=solver["max_iters"], convergence_tol=solver["convergence_tol"],
                                  divergence_tol=solver["divergence_tol"], divergence_threshold=divergence_threshold,
-                                 seed=seed, counter=counter)
-        rc = _lib.load().gjx_rmhmc_fisher_logistic_sample(params, logdensity_fn.data(), io, _lib.stream_handle(dev))
+                                 seed=seed, counter=counter, dtype=position.dtype)
+        data = logdensity_fn.data(dtype=position.dtype, num_chains=C)
+        rc = _lib.load().gjx_rmhmc_fisher_logistic_sample(params, data, io, _lib.stream_handle(dev))
         _lib.check(rc, "gjx_rmhmc_fisher_logistic_sample")
     return SampleResult(RMHMCState(final_q, final_l, final_g), samples, acc, n_acc, n_fev)
 

@@ -209,7 +209,7 @@ int gjx_lmcmonge_funnel_sample(const gjx_funnel_params* params, const gjx_lmc_sa
 typedef struct {
   int32_t num_chains;            /* C */
   int32_t dim;                   /* row stride of the [C, dim] state arrays, >= every D_p */
-  int32_t dtype;                 /* GJX_F64 */
+  int32_t dtype;                 /* GJX_F64, or GJX_F32 for the tuned RMHMC path (see below) */
   int32_t num_integration_steps; /* L >= 0 */
   int32_t max_iters;             /* fixed-point iteration cap */
   int32_t num_transitions;       /* T >= 1 (sample entry point) */
@@ -236,7 +236,36 @@ typedef struct {
   const int32_t* chain_posterior;/* [C] device, or NULL: every chain on posterior 0 */
   const void* x;                 /* float64 design matrices */
   const void* y;                 /* float64 labels in {0, 1} */
+  /* GJX_F32 only (NULL / 0 otherwise): */
+  const void* packed;            /* the table built by gjx_logistic_pack */
+  void* workspace;               /* per-chain scratch, gjx_logistic_workspace_size bytes */
+  uint64_t workspace_bytes;
 } gjx_logistic_data;
+
+/* Tuned RMHMC path (params->dtype == GJX_F32).  State arrays (positions,
+ * momenta, velocities, gradients, samples, z) are float32; per-chain scalars
+ * stay float64.  Gram and quadratic-form contractions run on bf16 MFMA with
+ * each float32 operand split into two bf16 terms (about 16 significant bits
+ * per product) and float32 accumulation; the metric inverse is a float32
+ * sweep with one refinement step per solve; log det G is summed, and the
+ * energies, energy difference, accept test and fixed-point norm are kept, in
+ * float64.  D <= 64 per posterior.  The design matrices must first be packed
+ * (once per data set) into a device table: */
+
+/* Bytes of the packed table for P posteriors of the given (host) sizes. */
+int gjx_logistic_pack_size(int32_t num_posteriors, const int32_t* num_rows_host, const int32_t* num_features_host,
+                           uint64_t* bytes);
+
+/* Fill `packed` (device, packed_bytes >= gjx_logistic_pack_size) from
+ * data->x / data->y.  Set-up call: returns after the table is written
+ * (synchronises `stream`). */
+int gjx_logistic_pack(const gjx_logistic_data* data, const int32_t* num_rows_host, const int32_t* num_features_host,
+                      void* packed, uint64_t packed_bytes, void* stream);
+
+/* Bytes of the per-chain scratch the tuned path needs for params->num_chains
+ * chains and data->max_rows rows (5 float arrays of max_rows rounded up to 32
+ * per chain).  Calls that overlap in time need separate workspaces. */
+int gjx_logistic_workspace_size(const gjx_logistic_params* params, const gjx_logistic_data* data, uint64_t* bytes);
 
 /* One RMHMC transition per chain (same semantics and buffers as
  * gjx_rmhmc_monge_funnel_step, with metric_fn = Fisher metric). */

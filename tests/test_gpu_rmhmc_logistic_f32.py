@@ -1,0 +1,163 @@
+"""GPU: the tuned RMHMC Fisher-logistic path (float32 state, bf16 MFMA
+contractions with split operands, float32 sweep + refinement, float64 energies)
+against float64 on the same q0, z, u.
+
+The reference for small shapes is the float64 oracle (dense Fisher metric,
+Cholesky inside the Gaussian log-density, autodiff); for the 2048 x 64 shape
+and the multi-posterior batch it is the float64 HIP kernel, itself pinned to
+the oracle at 1e-10.  Stated tolerances of the tuned path:
+  * end state (new position, proposal position): 1e-5 norm-wise relative;
+  * momentum p0 = chol(G) z: 1e-5; velocity and gradient at the proposal: 1e-4;
+  * energy and energy difference dH: 1e-3 absolute;
+  * accept decisions identical except where |log u - dH| < 1e-3.
+"""
+import functools
+
+import pytest
+import torch
+
+import geomjax_amd as gj
+from geomjax_amd import random as gr
+from geomjax_amd.rmhmc import build_kernel, implicit_midpoint
+from geomjax_amd.synthetic import make_logistic
+from oracle import models as om
+from oracle import rmhmc as orm
+from tests.helpers import rel_err, seeded_normal, seeded_uniform
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+TOL_STATE, TOL_MOM, TOL_VEL, TOL_H = 1e-5, 1e-5, 1e-4, 1e-3
+
+
+def _check_step(info, s, ref_info, ref_state, u, q_prev, label):
+    """Compare a float32 step (info, s) with a float64 step (ref_info, ref_state)."""
+    w64 = ref_info.proposal.weight.double().cpu()
+    ties = (torch.log(u.double().cpu()) - w64).abs() < TOL_H
+    keep = ~ties
+    errs = {
+        "momentum": rel_err(info.momentum.double(), ref_info.momentum.double()),
+        "proposal.position": rel_err(info.proposal.state.position.double(), ref_info.proposal.state.position.double()),
+        "proposal.velocity": rel_err(info.proposal.state.velocity.double(), ref_info.proposal.state.velocity.double()),
+        "proposal.logdensity_grad": rel_err(info.proposal.state.logdensity_grad.double(),
+                                            ref_info.proposal.state.logdensity_grad.double()),
+        "energy": (info.energy.cpu() - ref_info.energy.cpu()).abs().max().item(),
+        "dH": (info.proposal.weight.cpu() - w64).abs().max().item(),
+        "position": rel_err(s.position.double().cpu()[keep], ref_state.position.double().cpu()[keep]),
+    }
+    print(f"  {label}: ties {int(ties.sum())}, accepted {int(info.is_accepted.sum())}, "
+          + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()), flush=True)
+    assert torch.equal(info.is_accepted.cpu()[keep], ref_info.is_accepted.cpu()[keep])
+    assert errs["momentum"] < TOL_MOM
+    assert errs["proposal.position"] < TOL_STATE
+    assert errs["position"] < TOL_STATE
+    assert errs["proposal.velocity"] < TOL_VEL
+    assert errs["proposal.logdensity_grad"] < TOL_VEL
+    assert errs["energy"] < TOL_H
+    assert errs["dH"] < TOL_H
+
+
+@pytest.mark.parametrize("N,D,C,eps,L,max_iters,seed", [
+    (256, 8, 3, 0.1, 10, 6, 0),
+    (300, 13, 2, 0.1, 6, 6, 1),    # ragged: N not a multiple of 32
+    (1000, 40, 2, 0.1, 5, 6, 3),   # two 32-feature blocks with padding
+])
+def test_step_f32_matches_oracle(N, D, C, eps, L, max_iters, seed):
+    print(f"case N={N} D={D} C={C} eps={eps} L={L} max_iters={max_iters}", flush=True)
+    X, y, _ = make_logistic(N, D, seed)
+    M = om.LogisticRegression(X, y, 100.0)
+    q0 = seeded_normal(seed, C, D)
+    z, u = seeded_normal(500 + seed, C, D), seeded_uniform(600 + seed, C)
+    so = orm.init(q0, M.logp)
+    so1, io = orm.build_kernel(max_iters=max_iters)((z, u), so, M.logp, eps, M.fisher_metric_fn, L)
+    model = gj.LogisticRegression(X, y, 100.0)
+    s = gj.rmhmc.init(q0.float().to(DEV), model)
+    kern = build_kernel(functools.partial(implicit_midpoint, max_iters=max_iters))
+    s1, info = kern(gr.HostNoise(z.float().to(DEV), u.to(DEV)), s, model, eps, gj.FisherMetric(), L)
+    torch.cuda.synchronize()
+    assert s1.position.dtype == torch.float32
+    _check_step(info, s1, io, so1, u, q0, "f32 vs oracle")
+
+
+def test_step_f32_matches_f64_kernel_2048x64():
+    N, D, C, eps, L, max_iters = 2048, 64, 16, 0.1, 10, 6
+    X, y, _ = make_logistic(N, D, 2)
+    model = gj.LogisticRegression(X, y, 100.0)
+    q0 = seeded_normal(11, C, D).to(DEV)
+    integ = functools.partial(implicit_midpoint, max_iters=max_iters)
+    kern = build_kernel(integ)
+    s64 = gj.rmhmc.init(q0, model)
+    s32 = gj.rmhmc.init(q0.float(), model)
+    for t in range(2):
+        print(f"case N={N} D={D} C={C} transition {t}", flush=True)
+        z, u = seeded_normal(700 + t, C, D).to(DEV), seeded_uniform(800 + t, C).to(DEV)
+        s64n, i64 = kern(gr.HostNoise(z, u), s64, model, eps, gj.FisherMetric(), L)
+        s32n, i32 = kern(gr.HostNoise(z.float(), u), s32, model, eps, gj.FisherMetric(), L)
+        torch.cuda.synchronize()
+        _check_step(i32, s32n, i64, s64n, u, q0, f"2048x64 t={t}")
+        # continue both from the float64 state so the comparison stays one transition deep
+        s64 = s64n
+        s32 = gj.rmhmc.init(s64n.position.float(), model)
+
+
+def test_batch_f32_matches_f64_kernel():
+    shapes = [(200, 8), (700, 24), (1500, 40), (2048, 64), (333, 32)]
+    models = [gj.LogisticRegression(*make_logistic(n, d, 20 + i)[:2], 100.0) for i, (n, d) in enumerate(shapes)]
+    batch = gj.LogisticBatch(models, 3)
+    C, Dm, eps, L = batch.num_chains, batch.dim, 0.08, 6
+    print(f"case batch {shapes} x 3 chains", flush=True)
+    q0 = torch.zeros(C, Dm, dtype=torch.float64)
+    for c in range(C):
+        d = shapes[c // 3][1]
+        q0[c, :d] = seeded_normal(900 + c, d)
+    z = torch.zeros(C, Dm, dtype=torch.float64)
+    for c in range(C):
+        d = shapes[c // 3][1]
+        z[c, :d] = seeded_normal(950 + c, d)
+    u = seeded_uniform(990, C)
+    kern = build_kernel(functools.partial(implicit_midpoint, max_iters=6))
+    s64 = gj.rmhmc.init(q0.to(DEV), batch)
+    s32 = gj.rmhmc.init(q0.float().to(DEV), batch)
+    s64n, i64 = kern(gr.HostNoise(z.to(DEV), u.to(DEV)), s64, batch, eps, gj.FisherMetric(), L)
+    s32n, i32 = kern(gr.HostNoise(z.float().to(DEV), u.to(DEV)), s32, batch, eps, gj.FisherMetric(), L)
+    torch.cuda.synchronize()
+    _check_step(i32, s32n, i64, s64n, u, q0, "batch")
+    # coordinates past a chain's own D stay zero
+    for c in range(C):
+        d = shapes[c // 3][1]
+        assert torch.all(s32n.position[c, d:] == 0)
+
+
+def test_sample_kernel_matches_step_kernel_f32():
+    from geomjax_amd.sampling import run_chains
+
+    N, D, C, eps, L, T = 512, 40, 24, 0.1, 5, 3
+    X, y, _ = make_logistic(N, D, 7)
+    model = gj.LogisticRegression(X, y, 100.0)
+    alg = gj.rmhmc(model, eps, gj.FisherMetric(), L, integrator=functools.partial(implicit_midpoint, max_iters=6))
+    q0 = seeded_normal(71, C, D).float().to(DEV)
+    zs, us = seeded_normal(72, T, C, D).float().to(DEV), seeded_uniform(73, T, C).to(DEV)
+    s = alg.init(q0)
+    res = run_chains(alg, s, T, gr.HostNoise(zs, us))
+    for t in range(T):
+        s, _ = alg.step(gr.HostNoise(zs[t], us[t]), s)
+        assert rel_err(res.samples[t], s.position) <= 1e-6
+    torch.cuda.synchronize()
+    assert rel_err(res.state.position, s.position) <= 1e-6
+    assert rel_err(res.state.logdensity, s.logdensity) <= 1e-9
+    assert rel_err(res.state.logdensity_grad, s.logdensity_grad) <= 1e-5
+    assert int(res.num_fevals.min()) >= T * L * 2
+
+
+def test_device_prng_sampling_f32():
+    N, D, C, eps, L, T = 2048, 64, 64, 0.1, 10, 4
+    X, y, _ = make_logistic(N, D, 2)
+    model = gj.LogisticRegression(X, y, 100.0)
+    alg = gj.rmhmc(model, eps, gj.FisherMetric(), L, integrator=functools.partial(implicit_midpoint, max_iters=6))
+    s = alg.init(seeded_normal(5, C, D).float().to(DEV))
+    res = alg.sample_fn(gr.key(3), s, T)
+    torch.cuda.synchronize()
+    acc = res.acceptance_rate.mean().item()
+    print(f"  acceptance {acc:.3f}, fevals/chain/transition {res.num_fevals.double().mean().item() / T:.1f}")
+    assert torch.isfinite(res.samples).all()
+    assert 0.5 < acc <= 1.0
+    assert int(res.num_fevals.max()) <= T * L * 8
