@@ -1,31 +1,36 @@
 #!/usr/bin/env python
 """Benchmark of the Riemannian integrator hot path on one MI355X.
 
-Workload: Neal's funnel D=100 (1 log-scale + 99 conditionals), RMHMC with the
-Monge metric G = I + alpha2 grad l grad l^T (alpha2 = 1), step 0.03, 32
-implicit-midpoint steps per transition, fixed point capped at 8 iterations,
-8192 chains, initial positions N(0, I) with seed 1.
+Headline workload (the metric's "4096 chains, tuned vs plain"): Bayesian
+logistic regression with the Fisher metric G = X^T diag(s(1-s)) X + I/100,
+X [2048, 64] iid N(0, 1/64), beta* ~ N(0, I), y ~ Bernoulli(sigmoid(X beta*)),
+data seed 2; RMHMC with the implicit midpoint, step 0.1, 10 steps, fixed point
+capped at 6 iterations; 4096 chains started at N(0, I) (seed 1).
 
-A "step" is one transition of all 8192 chains (32 integrator steps each).
+A "step" is one transition of all 4096 chains (10 integrator steps each).
 
-* tuned form (the reported ``value``): FP32 state with FP64 energy /
-  convergence test, autotuned lanes-per-chain, persistent launches of 100
-  transitions each with the chain state held in registers, device Philox
-  noise.
-* plain form: FP64 kernels, library-default launch shape, one launch per
-  transition from the host.
+* tuned form (the reported ``value``): float32 state, Gram and quadratic-form
+  contractions on bf16 MFMA (split operands, float32 accumulation), float32
+  sweep inverse with one refinement step, float64 log-det / energies /
+  accept test / fixed-point norm; persistent launches of several transitions
+  with device Philox noise.
+* plain form: the float64 kernel, one launch per transition from the host
+  (the reference's per-step API).  It starts from the tuned form's warmed-up
+  chain state, so both time stationary sampling.
 
-``value`` = chains * 32 * steps / wall seconds of the timed region (inputs
-already resident in HBM).  min-ESS/s uses the multi-chain ESS of
-geomjax/diagnostics.py:78-209 over the timed samples.  The CPU baseline is the
-float64 oracle restatement (dense metric + autodiff as the reference) timed on
-this host's cores on a bounded sample.
+``value`` = chains * 10 * steps / wall seconds of the timed region (inputs
+resident in HBM).  min-ESS/s uses the multi-chain ESS of
+geomjax/diagnostics.py:78-209 over the timed samples.  Extra fields report the
+same comparison at 1024 chains and for Neal's funnel D=100 (RMHMC with the
+Monge metric, 8192 chains).  The CPU baseline is the float64 oracle
+restatement (dense metric + autodiff, as the reference) timed on this host's
+cores on a bounded sample of the headline workload.
 
 Prints ONE JSON line on stdout; progress goes to stderr.
 """
 import argparse
+import functools
 import json
-import math
 import os
 import sys
 import time
@@ -35,88 +40,165 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 METRIC = "min-ESS/sec & leapfrog-steps/sec, 4096 chains, tuned vs plain GPU form"
-D, CHAINS, EPS, L, MAX_ITERS, ALPHA2, SIGMA, SEED = 100, 8192, 0.03, 32, 8, 1.0, 3.0, 1
-CHUNK = 100  # transitions per persistent launch
+# logistic regression (headline)
+N_ROWS, D_LOG, CHAINS, EPS_LOG, L_LOG, MAXIT_LOG, PRIOR, DATA_SEED, INIT_SEED = 2048, 64, 4096, 0.1, 10, 6, 100.0, 2, 1
+CHUNK_LOG = 10  # transitions per persistent launch (tuned form)
+# Neal's funnel (secondary)
+D_F, CHAINS_F, EPS_F, L_F, MAXIT_F, ALPHA2, SIGMA, SEED_F = 100, 8192, 0.03, 32, 8, 1.0, 3.0, 1
+CHUNK_F = 100
+BF16_PEAK = 2.5e15
 FP32_PEAK = 157.3e12
-FP64_PEAK = 78.6e12
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def mfma_flops_logistic(num_fevals_total):
+    """Algorithmic MFMA work of RMHMC-Fisher on logistic regression: per evaluation
+    of the vector field the Gram X^T Lambda X (2 N D^2) and X G^-1 for the trace
+    term (2 N D^2)."""
+    return num_fevals_total * 4 * N_ROWS * D_LOG * D_LOG
+
+
 def flops_rmhmc_monge_funnel(num_fevals_total, chains, transitions):
-    """Algorithmic flops of RMHMC-Monge on the funnel (transcendentals count 1).
-
+    """Algorithmic flops of RMHMC-Monge on the funnel (transcendentals count 1):
     per evaluation of the vector field 33 D + 25 flops + 1 exp; per chain-step
-    11 D; per transition two energies (8 D + 2 log) and the rank-one momentum
-    draw (10 D).
-    """
-    return (num_fevals_total * (33 * D + 26) + chains * transitions * L * 11 * D
-            + chains * transitions * (18 * D + 2))
+    11 D; per transition two energies (8 D + 2 log) and the rank-one draw (10 D)."""
+    return (num_fevals_total * (33 * D_F + 26) + chains * transitions * L_F * 11 * D_F
+            + chains * transitions * (18 * D_F + 2))
 
 
-def min_ess(samples_list):
+def min_ess(x):
     """min over coordinates of the multi-chain ESS of [T, C, D] samples (chunked over D)."""
     import torch
 
     from geomjax_amd.diagnostics import effective_sample_size
 
-    x = torch.cat(samples_list, dim=0) if isinstance(samples_list, list) else samples_list
     out = []
-    for d0 in range(0, x.shape[2], 10):
-        out.append(effective_sample_size(x[:, :, d0:d0 + 10], chain_axis=1, sample_axis=0))
+    for d0 in range(0, x.shape[2], 8):
+        out.append(effective_sample_size(x[:, :, d0:d0 + 8], chain_axis=1, sample_axis=0))
     ess = torch.cat([o.reshape(-1) for o in out])
     return float(ess.min().item()), int(ess.argmin().item())
 
 
 def cpu_baseline(seconds_budget=30.0):
-    """Oracle (float64 dense + autodiff restatement) on this host's cores."""
+    """Oracle (float64 dense Fisher metric + autodiff restatement) on this host's cores."""
     import torch
 
+    from geomjax_amd.synthetic import make_logistic
     from oracle import models as om
     from oracle import rmhmc as orm
 
     threads = len(os.sched_getaffinity(0))
     torch.set_num_threads(threads)
-    M = om.NealFunnel(D, sigma=SIGMA)
-    kern = orm.build_kernel(max_iters=MAX_ITERS)
-    metric = om.monge_metric_fn(M.logp, ALPHA2, D)
-    g = torch.Generator().manual_seed(SEED)
+    X, y, _ = make_logistic(N_ROWS, D_LOG, DATA_SEED)
+    M = om.LogisticRegression(X, y, PRIOR)
+    kern = orm.build_kernel(max_iters=MAXIT_LOG)
+    g = torch.Generator().manual_seed(INIT_SEED)
     C = 64
-    q0 = torch.randn(C, D, generator=g, dtype=torch.float64)
-    s = orm.init(q0, M.logp)
+    s = orm.init(torch.randn(C, D_LOG, generator=g, dtype=torch.float64), M.logp)
     # untimed warm-up on a few chains
-    kern((torch.randn(4, D, generator=g, dtype=torch.float64), torch.rand(4, generator=g, dtype=torch.float64)),
-         orm.init(q0[:4], M.logp), M.logp, EPS, metric, L)
+    kern((torch.randn(2, D_LOG, generator=g, dtype=torch.float64), torch.rand(2, generator=g, dtype=torch.float64)),
+         orm.init(s.position[:2], M.logp), M.logp, EPS_LOG, M.fisher_metric_fn, L_LOG)
     n_tr, t_used = 0, 0.0
     while True:
-        z = torch.randn(C, D, generator=g, dtype=torch.float64)
+        z = torch.randn(C, D_LOG, generator=g, dtype=torch.float64)
         u = torch.rand(C, generator=g, dtype=torch.float64)
         t0 = time.perf_counter()
-        s, _ = kern((z, u), s, M.logp, EPS, metric, L)
+        s, _ = kern((z, u), s, M.logp, EPS_LOG, M.fisher_metric_fn, L_LOG)
         t_used += time.perf_counter() - t0
         n_tr += 1
-        if t_used > seconds_budget / 2 or n_tr >= 4:
+        if t_used > seconds_budget / 2 or n_tr >= 3:
             break
-    return {"value": C * L * n_tr / t_used, "unit": "leapfrog-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{C} chains x {n_tr} transitions of the same workload (float64 dense-metric autodiff "
-                      f"restatement, {t_used:.1f} s)"}
+    return {"value": C * L_LOG * n_tr / t_used, "unit": "leapfrog-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{C} chains x {n_tr} transitions of the headline logistic workload (float64 dense "
+                      f"Fisher metric, Cholesky + autodiff restatement, {t_used:.1f} s)"}
 
 
-def main():
-    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000, help="timed transitions (multiple of 100)")
-    ap.add_argument("--warmup", type=int, default=100, help="untimed burn-in transitions")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-plain", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU only (got --gpus {args.gpus}): the chains of this workload are "
-              "independent, so more GPUs would each run their own share; see DESIGN.md.", file=sys.stderr)
-        sys.exit(2)
+def logistic_forms(chains, K, W, dev, plain=True):
+    """Tuned and plain RMHMC-Fisher on the 2048 x 64 logistic posterior with `chains` chains."""
+    import torch
 
+    import geomjax_amd as gj
+    from geomjax_amd import random as gr
+    from geomjax_amd.rmhmc import build_kernel, implicit_midpoint, sample
+    from geomjax_amd.synthetic import make_logistic
+
+    X, y, _ = make_logistic(N_ROWS, D_LOG, DATA_SEED)
+    model, metric = gj.LogisticRegression(X, y, PRIOR, device=dev), gj.FisherMetric()
+    integ = functools.partial(implicit_midpoint, max_iters=MAXIT_LOG)
+    q_init = torch.randn(chains, D_LOG, generator=torch.Generator().manual_seed(INIT_SEED), dtype=torch.float64)
+    key = gr.key(INIT_SEED)
+    K = max(CHUNK_LOG, (K // CHUNK_LOG) * CHUNK_LOG)
+
+    # ---- tuned ----
+    state = gj.rmhmc.init(q_init.to(dev, torch.float32), model)
+    counter = 10_000
+    for _ in range(max(1, W // CHUNK_LOG)):
+        r = sample(gr.PRNGKey(key.seed, counter), state, model, EPS_LOG, metric, L_LOG, CHUNK_LOG,
+                   integrator=integ, store_samples=False)
+        state, counter = r.state, counter + CHUNK_LOG
+    warmed = state
+    samples = torch.empty(K, chains, D_LOG, dtype=torch.float32, device=dev)
+    fevs, accs = [], []
+    stream = torch.cuda.current_stream(dev)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for c in range(K // CHUNK_LOG):
+        r = sample(gr.PRNGKey(key.seed, counter), state, model, EPS_LOG, metric, L_LOG, CHUNK_LOG,
+                   integrator=integ, samples_out=samples[c * CHUNK_LOG:(c + 1) * CHUNK_LOG])
+        state, counter = r.state, counter + CHUNK_LOG
+        fevs.append(r.num_fevals)
+        accs.append(r.acceptance_rate)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_s = ev0.elapsed_time(ev1) / 1e3
+    fev_total = int(torch.stack(fevs).sum().item())
+    ess, ess_arg = min_ess(samples)
+    del samples
+    n_launch = K // CHUNK_LOG
+    tuned = {"value": chains * L_LOG * K / wall, "unit": "leapfrog-steps/s", "min_ess_per_sec": ess / wall,
+             "min_ess": ess, "min_ess_coord": ess_arg, "ms_per_step": wall / K * 1e3,
+             "acceptance_rate": float(torch.stack(accs).mean().item()),
+             "fevals_per_chain_transition": fev_total / chains / K, "dtype": "f32 state, bf16x3 MFMA, f64 energies",
+             "launches": n_launch, "avg_launch_ms": kernel_s / n_launch * 1e3,
+             "mfma_flops_per_launch": mfma_flops_logistic(fev_total) / n_launch, "steps": K}
+    log(f"logistic tuned, {chains} chains: {tuned['value']:.3e} leapfrog-steps/s, {tuned['ms_per_step']:.2f} ms/step, "
+        f"min-ESS/s {tuned['min_ess_per_sec']:.3e}, acc {tuned['acceptance_rate']:.3f}, "
+        f"fevals/chain/tr {tuned['fevals_per_chain_transition']:.1f}")
+
+    res = {"tuned": tuned, "plain": None}
+    if plain:
+        kern = build_kernel(integ)
+        s64 = gj.rmhmc.init(warmed.position.double(), model)
+        s64, _ = kern(gr.PRNGKey(key.seed + 1, 0), s64, model, EPS_LOG, metric, L_LOG)  # untimed
+        psamples = torch.empty(K, chains, D_LOG, dtype=torch.float64, device=dev)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for t in range(K):
+            s64, _ = kern(gr.PRNGKey(key.seed + 1, 1 + t), s64, model, EPS_LOG, metric, L_LOG)
+            psamples[t].copy_(s64.position)
+        torch.cuda.synchronize()
+        pwall = time.perf_counter() - t0
+        pess, _ = min_ess(psamples)
+        del psamples
+        res["plain"] = {"value": chains * L_LOG * K / pwall, "unit": "leapfrog-steps/s",
+                        "min_ess_per_sec": pess / pwall, "min_ess": pess, "ms_per_step": pwall / K * 1e3,
+                        "dtype": "f64", "steps": K,
+                        "form": "float64 kernel, one launch per transition"}
+        res["tuned_vs_plain"] = {"leapfrog_steps_per_sec": tuned["value"] / res["plain"]["value"],
+                                 "min_ess_per_sec": tuned["min_ess_per_sec"] / res["plain"]["min_ess_per_sec"]}
+        log(f"logistic plain, {chains} chains: {res['plain']['value']:.3e} leapfrog-steps/s, "
+            f"min-ESS/s {res['plain']['min_ess_per_sec']:.3e}; tuned/plain {res['tuned_vs_plain']}")
+    return res
+
+
+def funnel_forms(K, W, dev, plain=True):
+    """Tuned and plain RMHMC-Monge on Neal's funnel D=100 with 8192 chains."""
     import torch
 
     import geomjax_amd as gj
@@ -124,82 +206,107 @@ def main():
     from geomjax_amd import random as gr
     from geomjax_amd.rmhmc import build_kernel, implicit_midpoint, sample
 
-    dev = torch.device("cuda:0")
-    torch.cuda.set_device(dev)
-    K = max(CHUNK, (args.steps // CHUNK) * CHUNK)
-    W = max(0, args.warmup)
-    model, metric = gj.NealFunnel(D, sigma=SIGMA), gj.MongeMetric(ALPHA2)
-    integ = __import__("functools").partial(implicit_midpoint, max_iters=MAX_ITERS)
-    q_init = torch.randn(CHAINS, D, generator=torch.Generator().manual_seed(SEED), dtype=torch.float64)
-
-    # ------------------------------------------------------------------ tuned
+    K = max(CHUNK_F, (K // CHUNK_F) * CHUNK_F)
+    model, metric = gj.NealFunnel(D_F, sigma=SIGMA), gj.MongeMetric(ALPHA2)
+    integ = functools.partial(implicit_midpoint, max_iters=MAXIT_F)
+    q_init = torch.randn(CHAINS_F, D_F, generator=torch.Generator().manual_seed(SEED_F), dtype=torch.float64)
+    key = gr.key(SEED_F)
     s32 = gj.rmhmc.init(q_init.to(dev, torch.float32), model)
-    key = gr.key(SEED)
 
-    def run(lpc, state=s32, k=key, T=CHUNK, out=None):
-        return sample(k, state, model, EPS, metric, L, T, integrator=integ, lanes_per_chain=lpc,
+    def run(lpc, state=s32, k=key, T=CHUNK_F, out=None):
+        return sample(k, state, model, EPS_F, metric, L_F, T, integrator=integ, lanes_per_chain=lpc,
                       store_samples=out is not None, samples_out=out)
 
-    lpc, timings = autotune.tune("rmhmc_monge_funnel", D, CHAINS, torch.float32, lambda l: run(l), repeats=2)
-    log(f"autotune lanes_per_chain: {lpc}  ({ {k: round(v * 1e3, 2) for k, v in timings.items()} } ms / {CHUNK} tr)")
-    counter = 10_000
-    state = s32
-    for _ in range(max(1, W // CHUNK)):
+    lpc, timings = autotune.tune("rmhmc_monge_funnel", D_F, CHAINS_F, torch.float32, lambda l: run(l), repeats=2)
+    log(f"funnel autotune lanes_per_chain: {lpc} ({ {k: round(v * 1e3, 2) for k, v in timings.items()} } ms)")
+    counter, state = 10_000, s32
+    for _ in range(max(1, W // CHUNK_F)):
         r = run(lpc, state, gr.PRNGKey(key.seed, counter))
-        state, counter = r.state, counter + CHUNK
-    samples = torch.empty(K, CHAINS, D, dtype=torch.float32, device=dev)
-    fev_list, accs = [], []
+        state, counter = r.state, counter + CHUNK_F
+    samples = torch.empty(K, CHAINS_F, D_F, dtype=torch.float32, device=dev)
+    fevs, accs = [], []
     stream = torch.cuda.current_stream(dev)
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     ev0.record(stream)
-    for c in range(K // CHUNK):
-        r = run(lpc, state, gr.PRNGKey(key.seed, counter), out=samples[c * CHUNK:(c + 1) * CHUNK])
-        state, counter = r.state, counter + CHUNK
-        fev_list.append(r.num_fevals)
+    for c in range(K // CHUNK_F):
+        r = run(lpc, state, gr.PRNGKey(key.seed, counter), out=samples[c * CHUNK_F:(c + 1) * CHUNK_F])
+        state, counter = r.state, counter + CHUNK_F
+        fevs.append(r.num_fevals)
         accs.append(r.acceptance_rate)
     ev1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     kernel_s = ev0.elapsed_time(ev1) / 1e3
-    n_launch = K // CHUNK
-    value = CHAINS * L * K / wall
-    acc_mean = float(torch.stack(accs).mean().item())
-    fev_total = int(torch.stack(fev_list).sum().item())
-    ess_min, ess_arg = min_ess(samples)
-    log(f"tuned: {K} transitions in {wall:.3f} s ({value:.3e} leapfrog-steps/s), kernel {kernel_s:.3f} s, "
-        f"acc {acc_mean:.3f}, fevals/chain/transition {fev_total / CHAINS / K:.2f}, min-ESS {ess_min:.0f} "
-        f"(coord {ess_arg})")
-    flops = flops_rmhmc_monge_funnel(fev_total, CHAINS, K)
-    achieved = flops / n_launch / (kernel_s / n_launch)
-    roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": FP32_PEAK / 1e12, "unit": "TFLOP/s",
-                "frac": achieved / FP32_PEAK, "traffic": None,
-                "kernel": f"funnel_rmhmc_kernel<float,{lpc},*,false>", "launches": n_launch,
-                "avg_launch_ms": kernel_s / n_launch * 1e3,
-                "algorithmic_flops_per_launch": flops / n_launch}
+    fev_total = int(torch.stack(fevs).sum().item())
+    ess, _ = min_ess(samples)
     del samples
-
-    # ------------------------------------------------------------------ plain
-    plain = None
-    if not args.no_plain:
-        s64 = gj.rmhmc.init(q_init.to(dev), model)
+    n_launch = K // CHUNK_F
+    flops = flops_rmhmc_monge_funnel(fev_total, CHAINS_F, K)
+    tuned = {"value": CHAINS_F * L_F * K / wall, "unit": "leapfrog-steps/s", "min_ess_per_sec": ess / wall,
+             "min_ess": ess, "acceptance_rate": float(torch.stack(accs).mean().item()), "lanes_per_chain": lpc,
+             "kernel": f"funnel_rmhmc_kernel<float,{lpc},*,false>", "avg_launch_ms": kernel_s / n_launch * 1e3,
+             "achieved_tflops": flops / kernel_s / 1e12, "frac_fp32_peak": flops / kernel_s / FP32_PEAK,
+             "steps": K}
+    log(f"funnel tuned: {tuned['value']:.3e} leapfrog-steps/s, min-ESS/s {tuned['min_ess_per_sec']:.3e}")
+    res = {"tuned": tuned, "plain": None}
+    if plain:
+        s64 = gj.rmhmc.init(state.position.double(), model)
         kern = build_kernel(integ)
-        for t in range(min(W, 20)):
-            s64, _ = kern(gr.PRNGKey(key.seed + 1, t), s64, model, EPS, metric, L)
-        psamples = torch.empty(K, CHAINS, D, dtype=torch.float64, device=dev)
+        s64, _ = kern(gr.PRNGKey(key.seed + 1, 0), s64, model, EPS_F, metric, L_F)
+        psamples = torch.empty(K, CHAINS_F, D_F, dtype=torch.float64, device=dev)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for t in range(K):
-            s64, info = kern(gr.PRNGKey(key.seed + 1, 1000 + t), s64, model, EPS, metric, L)
+            s64, _ = kern(gr.PRNGKey(key.seed + 1, 1 + t), s64, model, EPS_F, metric, L_F)
             psamples[t].copy_(s64.position)
         torch.cuda.synchronize()
         pwall = time.perf_counter() - t0
         pess, _ = min_ess(psamples)
         del psamples
-        plain = {"value": CHAINS * L * K / pwall, "unit": "leapfrog-steps/s", "min_ess_per_sec": pess / pwall,
-                 "min_ess": pess, "dtype": "f64", "form": "FP64 kernels, default launch shape, one launch per transition"}
-        log(f"plain: {plain['value']:.3e} leapfrog-steps/s, min-ESS/s {plain['min_ess_per_sec']:.3e}")
+        res["plain"] = {"value": CHAINS_F * L_F * K / pwall, "unit": "leapfrog-steps/s",
+                        "min_ess_per_sec": pess / pwall, "min_ess": pess, "dtype": "f64",
+                        "form": "float64 kernel, one launch per transition"}
+        res["tuned_vs_plain"] = {"leapfrog_steps_per_sec": tuned["value"] / res["plain"]["value"],
+                                 "min_ess_per_sec": tuned["min_ess_per_sec"] / res["plain"]["min_ess_per_sec"]}
+        log(f"funnel plain: {res['plain']['value']:.3e} leapfrog-steps/s; tuned/plain {res['tuned_vs_plain']}")
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50, help="timed transitions of the headline workload")
+    ap.add_argument("--warmup", type=int, default=20, help="untimed transitions before timing")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the 1024-chain and funnel comparisons")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}): the chains of this workload are "
+              "independent, so more GPUs would each run their own share of them.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    K, W = max(1, args.steps), max(0, args.warmup)
+
+    head = logistic_forms(CHAINS, K, W, dev, plain=not args.no_plain)
+    tuned = head["tuned"]
+    achieved = tuned["mfma_flops_per_launch"] / (tuned["avg_launch_ms"] / 1e3)
+    roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": BF16_PEAK / 1e12, "unit": "TFLOP/s",
+                "frac": achieved / BF16_PEAK, "traffic": None,
+                "kernel": "logistic_rmhmc_f32_kernel<2,false>", "launches": tuned["launches"],
+                "avg_launch_ms": tuned["avg_launch_ms"],
+                "algorithmic_flops_per_launch": tuned["mfma_flops_per_launch"]}
+
+    extra = {}
+    if not args.no_extra:
+        extra["logistic_1024_chains"] = logistic_forms(1024, K, W, dev, plain=not args.no_plain)
+        extra["funnel_d100_8192_chains"] = funnel_forms(300, 100, dev, plain=not args.no_plain)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -209,30 +316,30 @@ def main():
 
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": tuned["value"],
         "unit": "leapfrog-steps/s",
         "n_gpus": 1,
-        "steps": K,
+        "steps": tuned["steps"],
         "warmup": W,
-        "ms_per_step": wall / K * 1e3,
+        "ms_per_step": tuned["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "Neal's funnel D=100, RMHMC implicit midpoint, Monge metric alpha2=1",
-                   "chains": CHAINS, "dim": D, "step_size": EPS, "num_integration_steps": L,
-                   "max_iters": MAX_ITERS, "init": "N(0, I) seed 1", "transitions_per_launch": CHUNK,
-                   "lanes_per_chain": lpc},
-        "min_ess_per_sec": ess_min / wall,
-        "min_ess": ess_min,
-        "acceptance_rate": acc_mean,
-        "plain": plain,
-        "tuned_vs_plain": None if plain is None else {
-            "leapfrog_steps_per_sec": value / plain["value"],
-            "min_ess_per_sec": (ess_min / wall) / plain["min_ess_per_sec"]},
+        "config": {"workload": "Bayesian logistic regression RMHMC, Fisher metric, X 2048x64 iid N(0,1/64), "
+                               "prior N(0,100 I), data seed 2",
+                   "chains": CHAINS, "dim": D_LOG, "rows": N_ROWS, "step_size": EPS_LOG,
+                   "num_integration_steps": L_LOG, "max_iters": MAXIT_LOG, "init": "N(0, I) seed 1",
+                   "transitions_per_launch": CHUNK_LOG},
+        "min_ess_per_sec": tuned["min_ess_per_sec"],
+        "min_ess": tuned["min_ess"],
+        "acceptance_rate": tuned["acceptance_rate"],
+        "plain": head["plain"],
+        "tuned_vs_plain": head.get("tuned_vs_plain"),
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "extra": extra,
     }
     print(json.dumps(out), flush=True)
 

@@ -754,15 +754,16 @@ hipError_t launch_logistic_pack(const double* X, const double* y, const int32_t*
 
 hipError_t launch_logistic_rmhmc_f32(const LogisticArgs& a, int max_features, hipStream_t stream) {
   const int blocks = (a.C + 7) / 8 * 8;
-  // every posterior has D <= max_features: skip the wide kernel when all are narrow
+  // one launch per tile width present: a single posterior needs only its own;
+  // a batch runs the narrow kernel, then (if any D > 32) the wide one
+  const bool narrow = a.P > 1 || max_features <= 32;
+  const bool wide = max_features > 32;
   if (a.q_out != nullptr) {
-    hipLaunchKernelGGL((lfast::logistic_rmhmc_f32_kernel<1, true>), dim3(blocks), dim3(64), 0, stream, a);
-    if (max_features > 32)
-      hipLaunchKernelGGL((lfast::logistic_rmhmc_f32_kernel<2, true>), dim3(blocks), dim3(64), 0, stream, a);
+    if (narrow) hipLaunchKernelGGL((lfast::logistic_rmhmc_f32_kernel<1, true>), dim3(blocks), dim3(64), 0, stream, a);
+    if (wide) hipLaunchKernelGGL((lfast::logistic_rmhmc_f32_kernel<2, true>), dim3(blocks), dim3(64), 0, stream, a);
   } else {
-    hipLaunchKernelGGL((lfast::logistic_rmhmc_f32_kernel<1, false>), dim3(blocks), dim3(64), 0, stream, a);
-    if (max_features > 32)
-      hipLaunchKernelGGL((lfast::logistic_rmhmc_f32_kernel<2, false>), dim3(blocks), dim3(64), 0, stream, a);
+    if (narrow) hipLaunchKernelGGL((lfast::logistic_rmhmc_f32_kernel<1, false>), dim3(blocks), dim3(64), 0, stream, a);
+    if (wide) hipLaunchKernelGGL((lfast::logistic_rmhmc_f32_kernel<2, false>), dim3(blocks), dim3(64), 0, stream, a);
   }
   return hipGetLastError();
 }
