@@ -53,3 +53,49 @@ def test_invalid_arguments_are_rejected_without_a_gpu():
     p.alpha2 = 1.0
     assert lib.gjx_rmhmc_monge_funnel_step(p, io, None) == -1  # buffers missing
     assert "position" in _lib.last_error()
+
+
+def _pack_size_numpy(N, D):
+    """Bytes of the packed float32 table: 256-byte header of 32-byte entries, then per posterior
+    two fragment layouts of npad x 32nb floats and npad (rounded up to 64) labels."""
+    P = len(N)
+    off = -(-P * 32 // 256) * 64
+    for n, d in zip(N, D):
+        npad, nb = -(-n // 32) * 32, 1 if d <= 32 else 2
+        off += 2 * npad * 32 * nb + -(-npad // 64) * 64
+    return off * 4
+
+
+def test_logistic_pack_and_workspace_sizes():
+    import ctypes
+
+    lib = _lib.load()
+    for N, D in [([2048], [64]), ([200, 4096, 333], [8, 64, 33]), ([31], [1])]:
+        n = (ctypes.c_int32 * len(N))(*N)
+        d = (ctypes.c_int32 * len(D))(*D)
+        out = ctypes.c_uint64()
+        assert lib.gjx_logistic_pack_size(len(N), n, d, ctypes.byref(out)) == 0
+        assert out.value == _pack_size_numpy(N, D)
+    n, d = (ctypes.c_int32 * 1)(100), (ctypes.c_int32 * 1)(65)
+    assert lib.gjx_logistic_pack_size(1, n, d, ctypes.byref(out)) == -2
+    p = _lib.LogisticParams(num_chains=7, dim=64, dtype=_lib.GJX_F32)
+    data = _lib.LogisticData(num_posteriors=1, max_rows=2000, max_features=64)
+    assert lib.gjx_logistic_workspace_size(p, data, ctypes.byref(out)) == 0
+    assert out.value == 7 * 5 * 2016 * 4
+
+
+def test_logistic_float32_requires_packed_table_and_workspace():
+    lib = _lib.load()
+    p = _lib.LogisticParams(num_chains=4, dim=8, dtype=_lib.GJX_F32, num_integration_steps=2, max_iters=2,
+                            num_transitions=1, step_size=0.1, prior_var=100.0)
+    data = _lib.LogisticData(num_posteriors=1, max_rows=16, max_features=8, num_rows=1, num_features=1,
+                             x_offset=1, y_offset=1, x=1, y=1)
+    io = _lib.RmhmcStepIO()
+    assert lib.gjx_rmhmc_fisher_logistic_step(p, data, io, None) == -1
+    assert "packed" in _lib.last_error()
+    data.packed, data.workspace, data.workspace_bytes = 1, 1, 10
+    assert lib.gjx_rmhmc_fisher_logistic_step(p, data, io, None) == -1
+    assert "workspace" in _lib.last_error()
+    # entry points without a float32 path say so
+    assert lib.gjx_lmc_fisher_logistic_step(p, data, _lib.LmcmcStepIO(), None) == -2
+    assert "float32" in _lib.last_error()

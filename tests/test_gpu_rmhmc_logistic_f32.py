@@ -161,3 +161,46 @@ def test_device_prng_sampling_f32():
     assert torch.isfinite(res.samples).all()
     assert 0.5 < acc <= 1.0
     assert int(res.num_fevals.max()) <= T * L * 8
+
+
+def _layouts_numpy(X, y):
+    """The two fragment layouts and labels of one posterior, restated with numpy."""
+    import numpy as np
+
+    N, D = X.shape
+    npad, nb = -(-N // 32) * 32, 1 if D <= 32 else 2
+    Xp = np.zeros((npad, 32 * nb))
+    Xp[:N, :D] = X
+    lane = np.arange(64)
+    xg = np.empty((npad // 16, nb, 64, 8))
+    for kb in range(npad // 16):
+        for b in range(nb):
+            for i in range(8):
+                xg[kb, b, :, i] = Xp[kb * 16 + (lane // 32) * 8 + i, b * 32 + lane % 32]
+    xr = np.empty((npad // 32, 2 * nb, 64, 8))
+    for rb in range(npad // 32):
+        for kc in range(2 * nb):
+            for i in range(8):
+                xr[rb, kc, :, i] = Xp[rb * 32 + lane % 32, kc * 16 + (lane // 32) * 8 + i]
+    yf = np.zeros(npad)
+    yf[:N] = y
+    return xg.reshape(-1), xr.reshape(-1), yf
+
+
+def test_pack_layout_matches_numpy_restatement():
+    import numpy as np
+
+    shapes = [(70, 5), (64, 32), (333, 47)]
+    data = [make_logistic(n, d, 40 + i)[:2] for i, (n, d) in enumerate(shapes)]
+    batch = gj.LogisticBatch([gj.LogisticRegression(X, y, 100.0) for X, y in data], 1)
+    buf = batch._table.packed().cpu().numpy()
+    hdr = np.frombuffer(buf[:32 * len(shapes)].tobytes(), dtype=np.dtype(
+        [("xg", "<i8"), ("xr", "<i8"), ("yf", "<i8"), ("npad", "<i4"), ("nb", "<i4")]))
+    flt = np.frombuffer(buf.tobytes(), dtype=np.float32)
+    for p, (X, y) in enumerate(data):
+        xg, xr, yf = _layouts_numpy(X, y)
+        h = hdr[p]
+        assert h["npad"] == -(-X.shape[0] // 32) * 32 and h["nb"] == (1 if X.shape[1] <= 32 else 2)
+        np.testing.assert_array_equal(flt[h["xg"]:h["xg"] + xg.size], xg.astype(np.float32))
+        np.testing.assert_array_equal(flt[h["xr"]:h["xr"] + xr.size], xr.astype(np.float32))
+        np.testing.assert_array_equal(flt[h["yf"]:h["yf"] + yf.size], yf.astype(np.float32))
