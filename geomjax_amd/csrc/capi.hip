@@ -310,7 +310,7 @@ int gjx_lmcmonge_funnel_sample(const gjx_funnel_params* params, const gjx_lmc_sa
 namespace {
 
 void workspace_bytes(int C, int max_rows, uint64_t* bytes) {
-  *bytes = (uint64_t)C * 5 * (uint64_t)((max_rows + 31) / 32 * 32) * sizeof(float);
+  *bytes = (uint64_t)C * 5 * (uint64_t)((max_rows + 63) / 64 * 64) * sizeof(float);
 }
 
 int check_logistic(const gjx_logistic_params* p, const gjx_logistic_data* d, bool sample, gjx::LogisticArgs* a) {
@@ -362,7 +362,7 @@ int check_logistic(const gjx_logistic_params* p, const gjx_logistic_data* d, boo
     a->packed_hdr = static_cast<const gjx::PackedPost*>(d->packed);
     a->packed = static_cast<const float*>(d->packed);
     a->work = static_cast<float*>(d->workspace);
-    a->work_rows = (d->max_rows + 31) / 32 * 32;
+    a->work_rows = (d->max_rows + 63) / 64 * 64;
   }
   return GJX_OK;
 }

@@ -11,8 +11,9 @@ namespace gjx {
 struct PackedPost {
   int64_t xg;    // Gram layout: [npad/16][nb][64 lanes][8]
   int64_t xr;    // row layout:  [npad/32][2 nb][64 lanes][8]
+  int64_t xs;    // row layout split into bf16: [npad/32][2 nb][64 lanes][8 hi, 8 lo]
   int64_t yf;    // labels [npad]
-  int32_t npad;  // rows rounded up to 32 (padding rows are zero)
+  int32_t npad;  // rows rounded up to 64 (padding rows are zero)
   int32_t nb;    // 32-feature blocks: 1 (D <= 32) or 2 (D <= 64)
 };
 

@@ -55,13 +55,16 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef float f32x32 __attribute__((ext_vector_type(32)));
 
 constexpr int MAXD = 64;
-constexpr int LDG = MAXD + 4;  // LDS row stride of G (floats; rows stay 16-byte aligned)
+constexpr int LDG = MAXD + 4;  // LDS row stride of G^-1 (floats; rows stay 16-byte aligned)
+constexpr int LDW = 33;        // padded row stride of the 32 x 32 block W
 
 struct Smem {
-  float G[MAXD * LDG];  // Gram G, then G^-1; scratch of the transpose-reductions
-  float bc[2][MAXD];    // sweep: pivot column broadcast (double-buffered)
-  float dk[MAXD];       // sweep pivots
-  float q0[MAXD], p0[MAXD], q[MAXD], p[MAXD], u[MAXD], dH[MAXD], z[MAXD], g[MAXD], gs[MAXD], r[MAXD];
+  float G[MAXD * MAXD];  // Gram G (row stride 64), kept for the refinement step
+  float Gi[MAXD * LDG];  // G^-1 and the blocks of its computation; scratch of the transpose-reductions
+  float W[32 * LDW];     // W = A^-1 B of the blocked inverse
+  float bc[2][2][32];    // 32 x 32 sweep: column broadcast [pivot parity][half][row]
+  float dk[MAXD];        // pivots
+  float q0[MAXD], p0[MAXD], q[MAXD], p[MAXD], u[MAXD], dH[MAXD], z[MAXD], g[MAXD];
 };
 
 __device__ __forceinline__ void load8(const float* p, float (&v)[8]) {
@@ -90,8 +93,21 @@ __device__ __forceinline__ f32x16 mfma3(const bf16x8& ah, const bf16x8& al, cons
   return acc;
 }
 
-__device__ __forceinline__ float sigmoidf(float x) { return 1.0f / (1.0f + expf(-x)); }
+// sigmoid with the hardware exp2 and reciprocal (a few ulp: the state is float32)
+__device__ __forceinline__ float sigmoidf(float x) { return __builtin_amdgcn_rcpf(1.0f + __expf(-x)); }
+__device__ __forceinline__ float sqrt_fast(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ double softplus(double x) { return fmax(x, 0.0) + log1p(exp(-fabs(x))); }
+
+// Ask the scheduler to issue the region as NM groups of (1 MFMA, NV VALU), so
+// that independent VALU work runs while each MFMA executes.
+template <int NM, int NV>
+__device__ __forceinline__ void interleave_mfma_valu() {
+#pragma unroll
+  for (int i = 0; i < NM; ++i) {
+    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+    __builtin_amdgcn_sched_group_barrier(0x002, NV, 0);
+  }
+}
 
 // phi: accumulator feature slot m = 32t + 8a + 4h + c  ->  feature 16(j/8) + 8h + j%8, j = 16t + 4a + c
 __device__ __forceinline__ int phi(int m) {
@@ -114,19 +130,28 @@ __device__ __forceinline__ void gram_pass(Smem& S, const float* __restrict__ Xg,
   for (int t = 0; t < NTL; ++t) acc[t] = f32x16{0};
   const float* slp = sl + (l / 32) * 8;
   const float* xp = Xg + (size_t)l * 8;
-  const int nkb = npad / 16;
-  for (int kb = 0; kb < nkb; ++kb) {
-    float s8[8];
-    load8(slp + kb * 16, s8);
-    bf16x8 hi[NB], lo[NB];
+  const int nkb = npad / 16;  // even: npad is a multiple of 32
+  // fragments of K step kb: sqrt(lambda) of the lane's 8 rows and X of its NB feature blocks
+  struct Frag {
+    float s[8];
+    float x[NB][8];
+  };
+  auto fetch = [&](Frag& f, int kb) {
+    kb = kb < nkb ? kb : nkb - 1;  // the pipeline's tail re-reads the last step (in bounds, unused)
+    load8(slp + kb * 16, f.s);
+#pragma unroll
+    for (int b = 0; b < NB; ++b) load8(xp + (size_t)(kb * NB + b) * 512, f.x[b]);
+  };
+  auto split_frag = [&](const Frag& f, bf16x8 (&hi)[NB], bf16x8 (&lo)[NB]) {
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-      float x8[8];
-      load8(xp + (size_t)(kb * NB + b) * 512, x8);
+      float y8[8];
 #pragma unroll
-      for (int i = 0; i < 8; ++i) x8[i] *= s8[i];
-      split8(x8, hi[b], lo[b]);
+      for (int i = 0; i < 8; ++i) y8[i] = f.x[b][i] * f.s[i];
+      split8(y8, hi[b], lo[b]);
     }
+  };
+  auto tiles = [&](const bf16x8 (&hi)[NB], const bf16x8 (&lo)[NB]) {
     int t = 0;
 #pragma unroll
     for (int bi = 0; bi < NB; ++bi)
@@ -135,6 +160,23 @@ __device__ __forceinline__ void gram_pass(Smem& S, const float* __restrict__ Xg,
         acc[t] = mfma3(hi[bi], lo[bi], hi[bj], lo[bj], acc[t]);
         ++t;
       }
+  };
+  // Two-stage pipeline: the MFMAs of step k run while the operands of step
+  // k+1 are scaled and split on VALU; loads run two steps ahead.
+  Frag fa, fb;
+  bf16x8 ha[NB], la[NB], hb[NB], lb[NB];
+  fetch(fa, 0);
+  fetch(fb, 1);
+  split_frag(fa, ha, la);
+  for (int kb = 0; kb < nkb; kb += 2) {
+    fetch(fa, kb + 2);
+    tiles(ha, la);
+    split_frag(fb, hb, lb);
+    interleave_mfma_valu<3 * NTL, 64 / (3 * NTL) + 1>();
+    fetch(fb, kb + 3);
+    tiles(hb, lb);
+    split_frag(fa, ha, la);
+    interleave_mfma_valu<3 * NTL, 64 / (3 * NTL) + 1>();
   }
   __syncthreads();
   int t = 0;
@@ -150,19 +192,14 @@ __device__ __forceinline__ void gram_pass(Smem& S, const float* __restrict__ Xg,
         if (bi == bj && i < j) continue;
         float v = acc[t][r];
         if (i == j) v = i < D ? v + inv_prior : 1.0f;
-        S.G[i * LDG + j] = v;
-        S.G[j * LDG + i] = v;
+        S.G[i * MAXD + j] = v;
+        S.G[j * MAXD + i] = v;
       }
       ++t;
     }
   __syncthreads();
 }
 
-// ---------------------------------------------------------------------------
-// Sweep of the DP x DP matrix in S.G (left intact): lane jl = l % DP holds
-// column jl; on return A = -G^-1 (column jl) and S.dk the pivots.  With
-// want_chol, p0j = (L z)_jl with L = chol(G) and z in S.z.
-// ---------------------------------------------------------------------------
 // ---------------------------------------------------------------------------
 // Sweep of the DP x DP matrix in S.G (left intact).  Lane jl = l % DP holds
 // column jl in registers A[b][i] = A[32b + i][jl]; the pivot loop is rolled
@@ -202,12 +239,15 @@ __device__ __forceinline__ void sweep(Smem& S, f32x32 (&A)[NB], int D, bool want
 #pragma unroll
   for (int b = 0; b < NB; ++b)
 #pragma unroll
-    for (int i = 0; i < 32; ++i) A[b][i] = S.G[(32 * b + i) * LDG + jl];
+    for (int i = 0; i < 32; ++i) A[b][i] = S.G[(32 * b + i) * MAXD + jl];
   p0j = 0.0f;
-  // padding pivots (k >= D) are 1 with zero off-diagonals: nothing to do
+  // Pivot k needs column k in every lane: entry i is lane i's register k
+  // (A[k][i] = A[i][k]), broadcast through LDS.  Padding pivots (k >= D) are
+  // 1 with zero off-diagonals: nothing to do.
+  float(*bcast)[64] = reinterpret_cast<float(*)[64]>(&S.bc[0][0][0]);
   for (int k = 0; k < D; ++k) {
     const float ak = col_get<NB>(A, k);  // A[k][jl] = A[jl][k]
-    float* bc = S.bc[k & 1];
+    float* bc = bcast[k & 1];
     bc[jl] = ak;
     __syncthreads();
     float col[DP];
@@ -217,10 +257,9 @@ __device__ __forceinline__ void sweep(Smem& S, f32x32 (&A)[NB], int D, bool want
       col[i] = v.x; col[i + 1] = v.y; col[i + 2] = v.z; col[i + 3] = v.w;
     }
     const float d = bc[k];
-    const float rd = 1.0f / d;
+    const float rd = __builtin_amdgcn_rcpf(d);  // 1 ulp, as the float32 sweep itself
     if (want_chol && jl >= k) p0j = fmaf(ak / sqrtf(d), S.z[k], p0j);
-    // lane k's own column equals col, so f = 1 - 1/d turns its update into A[i][k] / d;
-    // entry k of every column is overwritten below
+    // lane k's own column equals col, so f = 1 - 1/d turns its update into A[i][k] / d
     const float f = (jl == k) ? 1.0f - rd : ak * rd;
 #pragma unroll
     for (int b = 0; b < NB; ++b)
@@ -256,20 +295,20 @@ __device__ __forceinline__ void ginv_apply(Smem& S, const f32x32 (&A)[NB]) {
   __syncthreads();
   float gu = 0.0f;
 #pragma unroll
-  for (int i = 0; i < DP; ++i) gu = fmaf(S.G[i * LDG + jl], S.u[i], gu);
-  if (l < DP) S.r[jl] = S.p[jl] - gu;
+  for (int i = 0; i < DP; ++i) gu = fmaf(S.G[i * MAXD + jl], S.u[i], gu);
+  if (l < DP) S.dH[jl] = S.p[jl] - gu;  // residual (dH is free here)
   __syncthreads();
   float du = 0.0f;
 #pragma unroll
   for (int b = 0; b < NB; ++b)
 #pragma unroll
-    for (int i = 0; i < 32; ++i) du = fmaf(-A[b][i], S.r[32 * b + i], du);
+    for (int i = 0; i < 32; ++i) du = fmaf(-A[b][i], S.dH[32 * b + i], du);
   __syncthreads();
   if (l < DP) S.u[jl] = uj + du;
   __syncthreads();
 }
 
-// S.G = G^-1 (overwrites the Gram)
+// S.Gi = G^-1
 template <int NB>
 __device__ __forceinline__ void store_ginv(Smem& S, const f32x32 (&A)[NB]) {
   const int l = threadIdx.x;
@@ -277,13 +316,13 @@ __device__ __forceinline__ void store_ginv(Smem& S, const f32x32 (&A)[NB]) {
 #pragma unroll
     for (int b = 0; b < NB; ++b)
 #pragma unroll
-      for (int i = 0; i < 32; ++i) S.G[(32 * b + i) * LDG + l] = -A[b][i];
+      for (int i = 0; i < 32; ++i) S.Gi[(32 * b + i) * LDG + l] = -A[b][i];
   }
   __syncthreads();
 }
 
 // G (in S.G) -> pivots in S.dk, S.u = G^-1 S.p (refined), and with
-// keep_inverse S.G = G^-1.  With want_chol the momentum p0 = chol(G) S.z is
+// keep_inverse S.Gi = G^-1.  With want_chol the momentum p0 = chol(G) S.z is
 // drawn first into S.p0 and S.p.
 template <int NB>
 __device__ __forceinline__ void invert(Smem& S, int D, bool want_chol, bool keep_inverse) {
@@ -301,13 +340,155 @@ __device__ __forceinline__ void invert(Smem& S, int D, bool want_chol, bool keep
   if (keep_inverse) store_ginv<NB>(S, A);
 }
 
+// ---------------------------------------------------------------------------
+// Blocked inverse for the vector-field evaluations.  With G = [A B; B^T C]
+// (32 x 32 blocks): A^-1 by a 32 x 32 sweep, W = A^-1 B, the Schur complement
+// S = C - B^T W, S^-1 by a second sweep, Z = W S^-1 and
+//   G^-1 = [A^-1 + Z W^T, -Z; -Z^T, S^-1],   log det G = log det A + log det S.
+// The four 32 x 32 x 32 products run on v_mfma_f32_32x32x2f32 (float32
+// operands and accumulation, as the sweep).  The 32 x 32 sweep keeps half a
+// column per lane (16 FMAs per pivot).
+// ---------------------------------------------------------------------------
+
+// -X^-1 of the 32 x 32 symmetric block at src (row stride ld), written as X^-1
+// to dst (row stride dld; dst may equal src); the first nk pivots are real,
+// the rest are identity padding; pivots to S.dk[dk0 + k].  Lane l holds rows
+// [16 (l/32), +16) of column l % 32.
+__device__ __forceinline__ void sweep32(Smem& S, const float* src, int ld, float* dst, int dld, int nk, int dk0) {
+  const int l = threadIdx.x, j = l % 32, hf = l / 32;
+  f32x16 A;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) A[r] = src[(16 * hf + r) * ld + j];
+  __syncthreads();  // src may alias dst
+  for (int k = 0; k < nk; ++k) {
+    const int ho = k >> 4, kr = k & 15;
+    // column k: entry j is A[k][j] = A[j][k], register kr of the half that holds row k
+    const float mine = A[kr];
+    S.bc[k & 1][hf][j] = mine;
+    __syncthreads();
+    const float* bc = S.bc[k & 1][ho];
+    float col[16];
+#pragma unroll
+    for (int i = 0; i < 16; i += 4) {
+      const float4 v = *reinterpret_cast<const float4*>(bc + 16 * hf + i);
+      col[i] = v.x; col[i + 1] = v.y; col[i + 2] = v.z; col[i + 3] = v.w;
+    }
+    const float ak = bc[j];
+    const float d = bc[k];
+    const float rd = __builtin_amdgcn_rcpf(d);  // 1 ulp, as the float32 sweep itself
+    // column k's own lanes hold col: f = 1 - 1/d turns their update into A[i][k] / d
+    const float f = (j == k) ? 1.0f - rd : ak * rd;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) A[r] = fmaf(-col[r], f, A[r]);
+    const float keep = A[kr];
+    A[kr] = (hf == ho) ? ((j == k) ? -rd : ak * rd) : keep;
+    if (l == 0) S.dk[dk0 + k] = d;
+  }
+  for (int k = nk; k < 32; ++k) {  // padding pivots: -1 on the diagonal
+    const float keep = A[k & 15];
+    A[k & 15] = (hf == (k >> 4) && j == k) ? -1.0f : keep;
+  }
+  if (l >= nk && l < 32) S.dk[dk0 + l] = 1.0f;
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 16; ++r) dst[(16 * hf + r) * dld + j] = -A[r];
+  __syncthreads();
+}
+
+// acc + X Y for 32 x 32 operands read through xa(m, k) = X[m][k], yb(k, n) = Y[k][n];
+// result slot r of lane l is C[(r/4)*8 + (l/32)*4 + r%4][l%32]
+template <class FX, class FY>
+__device__ __forceinline__ f32x16 gemm32(FX xa, FY yb, f32x16 acc) {
+  const int l = threadIdx.x, m = l % 32, kh = l / 32;
+#pragma unroll
+  for (int ks = 0; ks < 16; ++ks) {
+    const int k = 2 * ks + kh;
+    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(xa(m, k), yb(k, m), acc, 0, 0, 0);
+  }
+  return acc;
+}
+__device__ __forceinline__ int crow(int r) { return (r / 4) * 8 + ((int)threadIdx.x / 32) * 4 + (r % 4); }
+
+// S.u = G^-1 S.p with G^-1 in S.Gi, one refinement step against the Gram in S.G.
+template <int NB>
+__device__ __forceinline__ void solve_refined(Smem& S) {
+  constexpr int DP = 32 * NB;
+  const int l = threadIdx.x, jl = l % DP;
+  float uj = 0.0f;
+#pragma unroll
+  for (int i = 0; i < DP; ++i) uj = fmaf(S.Gi[i * LDG + jl], S.p[i], uj);
+  if (l < DP) S.u[jl] = uj;
+  __syncthreads();
+  float gu = 0.0f;
+#pragma unroll
+  for (int i = 0; i < DP; ++i) gu = fmaf(S.G[i * MAXD + jl], S.u[i], gu);
+  if (l < DP) S.dH[jl] = S.p[jl] - gu;  // residual (dH is free here)
+  __syncthreads();
+  float du = 0.0f;
+#pragma unroll
+  for (int i = 0; i < DP; ++i) du = fmaf(S.Gi[i * LDG + jl], S.dH[i], du);
+  __syncthreads();
+  if (l < DP) S.u[jl] = uj + du;
+  __syncthreads();
+}
+
+// G (in S.G) -> S.Gi = G^-1, pivots in S.dk, S.u = G^-1 S.p (refined).
+template <int NB>
+__device__ __forceinline__ void invert_blocked(Smem& S, int D) {
+  float* Gi = S.Gi;
+  const float* G = S.G;
+  if constexpr (NB == 1) {
+    sweep32(S, G, MAXD, Gi, LDG, D, 0);
+  } else {
+    const int l = threadIdx.x, c = l % 32;
+    sweep32(S, G, MAXD, Gi, LDG, 32, 0);  // Gi_TL = A^-1
+    // W = A^-1 B (A^-1 symmetric: read it transposed, conflict-free)
+    f32x16 acc = gemm32([&](int m, int k) { return Gi[k * LDG + m]; },
+                        [&](int k, int n) { return G[k * MAXD + 32 + n]; }, f32x16{0});
+#pragma unroll
+    for (int r = 0; r < 16; ++r) S.W[crow(r) * LDW + c] = acc[r];
+    __syncthreads();
+    // S = C - B^T W  into Gi_BR
+    acc = gemm32([&](int m, int k) { return G[k * MAXD + 32 + m]; },
+                 [&](int k, int n) { return S.W[k * LDW + n]; }, f32x16{0});
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int i = crow(r);
+      Gi[(32 + i) * LDG + 32 + c] = G[(32 + i) * MAXD + 32 + c] - acc[r];
+    }
+    __syncthreads();
+    sweep32(S, Gi + 32 * LDG + 32, LDG, Gi + 32 * LDG + 32, LDG, D - 32, 32);  // Gi_BR = S^-1
+    // Z = W S^-1 -> Gi_TR = -Z, Gi_BL = -Z^T
+    acc = gemm32([&](int m, int k) { return S.W[m * LDW + k]; },
+                 [&](int k, int n) { return Gi[(32 + k) * LDG + 32 + n]; }, f32x16{0});
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int i = crow(r);
+      Gi[i * LDG + 32 + c] = -acc[r];
+      Gi[(32 + c) * LDG + i] = -acc[r];
+    }
+    __syncthreads();
+    // Gi_TL = A^-1 + Z W^T   (Z[m][k] = -Gi_BL[k][m], W^T[k][n] = W[n][k])
+    acc = gemm32([&](int m, int k) { return -Gi[(32 + k) * LDG + m]; },
+                 [&](int k, int n) { return S.W[n * LDW + k]; }, f32x16{0});
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int i = crow(r);
+      Gi[i * LDG + c] += acc[r];
+    }
+    __syncthreads();
+  }
+  solve_refined<NB>(S);
+}
+
 // Sum the per-lane feature partials of a row pass over the 32 lanes of each
 // half wave: lane f < DP receives the total for feature f.
 template <int NB>
 __device__ __forceinline__ float reduce_features(Smem& S, const float (&acc)[2 * NB][8]) {
   constexpr int KC = 2 * NB, RS = KC * 8 + 1;
   const int l = threadIdx.x;
-  float* R = S.G;
+  float* R = S.Gi;
   __syncthreads();
 #pragma unroll
   for (int kc = 0; kc < KC; ++kc)
@@ -325,12 +506,13 @@ __device__ __forceinline__ float reduce_features(Smem& S, const float (&acc)[2 *
 
 // ---------------------------------------------------------------------------
 // One evaluation of the vector field at (S.q, S.p) with S.u = G^-1 p and
-// G^-1 in S.G: S.dH = dH/dq.  Also advances the scratch: E1 <- eta of the
+// G^-1 in S.Gi: S.dH = dH/dq.  Also advances the scratch: E1 <- eta of the
 // next iterate (base + h X u, base = E0 for a fixed-point iterate, E1 for
 // the explicit half step, which also sets E0), SL <- sqrt(lambda) there.
 // ---------------------------------------------------------------------------
 template <int NB>
-__device__ __forceinline__ void field_pass(Smem& S, const float* __restrict__ Xr, const float* __restrict__ yf, float* E0,
+__device__ __forceinline__ void field_pass(Smem& S, const float* __restrict__ Xr, const float* __restrict__ Xs,
+                                           const float* __restrict__ yf, float* E0,
                            float* E1, float* SL, int npad, int D, float hs, float inv_prior, bool explicit_step) {
   constexpr int KC = 2 * NB;
   const int l = threadIdx.x, h = l / 32;
@@ -341,67 +523,122 @@ __device__ __forceinline__ void field_pass(Smem& S, const float* __restrict__ Xr
 #pragma unroll
     for (int kc = 0; kc < KC; ++kc) {
       float v[8];
-      load8(S.G + row * LDG + kc * 16 + h * 8, v);
+      load8(S.Gi + row * LDG + kc * 16 + h * 8, v);
       split8(v, Ah[t][kc], Al[t][kc]);
     }
   }
-  float uv[KC][8], dacc[KC][8];
+  float dacc[KC][8];
 #pragma unroll
   for (int kc = 0; kc < KC; ++kc)
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      uv[kc][i] = S.u[kc * 16 + h * 8 + i];
-      dacc[kc][i] = 0.0f;
-    }
+    for (int i = 0; i < 8; ++i) dacc[kc][i] = 0.0f;
   const float* xp = Xr + (size_t)l * 8;
-  const int nrb = npad / 32;
-  for (int rb = 0; rb < nrb; ++rb) {
-    float x[KC][8];
-#pragma unroll
-    for (int kc = 0; kc < KC; ++kc) load8(xp + (size_t)(rb * KC + kc) * 512, x[kc]);
-    const int row = rb * 32 + (l % 32);
-    const float eta = E1[row];
-    const float base = explicit_step ? eta : E0[row];
-    const float yv = yf[row];
-    float bp = 0.0f;
-#pragma unroll
-    for (int kc = 0; kc < KC; ++kc)
-#pragma unroll
-      for (int i = 0; i < 8; ++i) bp = fmaf(x[kc][i], uv[kc][i], bp);
-    const float b = bp + __shfl_xor(bp, 32, 64);
+  const uint4* sp = reinterpret_cast<const uint4*>(Xs) + (size_t)l * 2;
+  const int nrb = npad / 32;  // even: npad is a multiple of 64
+  // One 32-row block: its MFMA operand (X split into bf16 hi/lo), and the
+  // float32 X fragment, eta, base and label of the lane's row with the block's
+  // b = X u and V^T accumulators.
+  struct XF {
     bf16x8 xh[KC], xl[KC];
-#pragma unroll
-    for (int kc = 0; kc < KC; ++kc) split8(x[kc], xh[kc], xl[kc]);
+  };
+  struct Blk {
+    float x[KC][8];
+    float eta, base, yv, b;
     f32x16 acc[NB];
+  };
+  // the pipeline's tail re-reads the last block (in bounds; its results are unused)
+  auto clamp_rb = [&](int rb) { return rb < nrb ? rb : nrb - 1; };
+  auto fetch_xf = [&](XF& f, int rb) {
+    rb = clamp_rb(rb);
+#pragma unroll
+    for (int kc = 0; kc < KC; ++kc) {
+      const uint4* q = sp + (size_t)(rb * KC + kc) * 128;
+      f.xh[kc] = __builtin_bit_cast(bf16x8, q[0]);
+      f.xl[kc] = __builtin_bit_cast(bf16x8, q[1]);
+    }
+  };
+  auto fetch_blk = [&](Blk& B, int rb) {
+    rb = clamp_rb(rb);
+#pragma unroll
+    for (int kc = 0; kc < KC; ++kc) load8(xp + (size_t)(rb * KC + kc) * 512, B.x[kc]);
+    const int row = rb * 32 + (l % 32);
+    B.eta = E1[row];
+    B.base = E0[row];
+    B.yv = yf[row];
+  };
+  auto mfma_blk = [&](const XF& f, Blk& B) {
 #pragma unroll
     for (int t = 0; t < NB; ++t) {
-      acc[t] = f32x16{0};
+      B.acc[t] = f32x16{0};
 #pragma unroll
-      for (int kc = 0; kc < KC; ++kc) acc[t] = mfma3(Ah[t][kc], Al[t][kc], xh[kc], xl[kc], acc[t]);
+      for (int kc = 0; kc < KC; ++kc) B.acc[t] = mfma3(Ah[t][kc], Al[t][kc], f.xh[kc], f.xl[kc], B.acc[t]);
     }
+  };
+  // b = X u; u is read from LDS per block (holding it in registers for the
+  // whole pass would cost 32 of them at the pass's peak)
+  auto bdot = [&](Blk& B) {
+    float bp = 0.0f;
+#pragma unroll
+    for (int kc = 0; kc < KC; ++kc) {
+      float u8[8];
+      load8(S.u + kc * 16 + h * 8, u8);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) bp = fmaf(B.x[kc][i], u8[i], bp);
+    }
+    B.b = bp + __shfl_xor(bp, 32, 64);
+  };
+  // a_n, h_n, X^T h and the next iterate's eta / sqrt(lambda) of block rb;
+  // both lanes of a row store the same values
+  auto post = [&](const Blk& B, int rb) {
+    const int row = rb * 32 + (l % 32);
     float ap = 0.0f;
 #pragma unroll
     for (int t = 0; t < NB; ++t)
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const int j = 16 * t + r;
-        ap = fmaf(acc[t][r], x[j / 8][j % 8], ap);
+        ap = fmaf(B.acc[t][r], B.x[j / 8][j % 8], ap);
       }
     const float an = ap + __shfl_xor(ap, 32, 64);
-    const float s = sigmoidf(eta);
+    const float s = sigmoidf(B.eta);
     const float w = s * (1.0f - s) * (1.0f - 2.0f * s);
-    const float hv = 0.5f * w * (an - b * b) - (yv - s);
+    const float hv = 0.5f * w * (an - B.b * B.b) - (B.yv - s);
 #pragma unroll
     for (int kc = 0; kc < KC; ++kc)
 #pragma unroll
-      for (int i = 0; i < 8; ++i) dacc[kc][i] = fmaf(hv, x[kc][i], dacc[kc][i]);
-    if (h == 0) {
-      const float en = fmaf(hs, b, base);
-      const float sn = sigmoidf(en);
-      E1[row] = en;
-      if (explicit_step) E0[row] = en;
-      SL[row] = sqrtf(sn * (1.0f - sn));
-    }
+      for (int i = 0; i < 8; ++i) dacc[kc][i] = fmaf(hv, B.x[kc][i], dacc[kc][i]);
+    const float en = fmaf(hs, B.b, explicit_step ? B.eta : B.base);
+    const float sn = sigmoidf(en);
+    E1[row] = en;
+    E0[row] = explicit_step ? en : B.base;
+    SL[row] = sqrt_fast(sn * (1.0f - sn));
+  };
+  // Two-stage pipeline: the MFMAs of block r+1 run while block r is finished
+  // on VALU; operand loads run one to two blocks ahead.
+  constexpr int NMF = 3 * NB * KC;  // MFMAs per block
+  XF f0, f1;
+  Blk b0, b1;
+  fetch_xf(f0, 0);
+  fetch_blk(b0, 0);
+  fetch_xf(f1, 1);
+  fetch_blk(b1, 1);
+  mfma_blk(f0, b0);
+  bdot(b0);
+  fetch_xf(f0, 2);
+  for (int rb = 0; rb < nrb; rb += 2) {
+    asm volatile("" ::: "memory");  // keep the per-block LDS reads of u inside the loop
+    mfma_blk(f1, b1);  // block rb + 1
+    post(b0, rb);
+    bdot(b1);
+    interleave_mfma_valu<NMF, 160 / NMF + 1>();
+    fetch_blk(b0, rb + 2);
+    fetch_xf(f1, rb + 3);
+    mfma_blk(f0, b0);  // block rb + 2 (past the end: unused)
+    post(b1, rb + 1);
+    bdot(b0);
+    interleave_mfma_valu<NMF, 160 / NMF + 1>();
+    fetch_blk(b1, rb + 3);
+    fetch_xf(f0, rb + 4);
   }
   const float tot = reduce_features<NB>(S, dacc);
   if (l < MAXD) S.dH[l] = l < D ? tot + S.q[l] * inv_prior : 0.0f;
@@ -470,7 +707,7 @@ __device__ __forceinline__ void prep_pass(const float* ES, float* E0, float* E1,
     const float s = sigmoidf(e);
     E0[row] = e;
     E1[row] = e;
-    SL[row] = sqrtf(s * (1.0f - s));
+    SL[row] = sqrt_fast(s * (1.0f - s));
   }
   __syncthreads();
 }
@@ -487,6 +724,7 @@ __device__ __forceinline__ void run_chain(const LogisticArgs& a, Smem& S, int c,
   const int N = a.post_N[post], D = a.post_D[post];
   const float* Xg = a.packed + pp.xg;
   const float* Xr = a.packed + pp.xr;
+  const float* Xs = a.packed + pp.xs;
   const float* yf = a.packed + pp.yf;
   const int npad = pp.npad;
   const int64_t ns = a.work_rows;
@@ -526,7 +764,8 @@ __device__ __forceinline__ void run_chain(const LogisticArgs& a, Smem& S, int c,
       float* Eout = (end ? 1 - cur : cur) ? ESb : ESa;
       loglik = fresh_pass<NB>(S, Xr, yf, N, npad, D, Eout, SL, end || !STEP, inv_prior);
       if (!end) {
-        if (!STEP) S.gs[l] = S.g[l];
+        // the state's gradient goes straight to final_g (rewritten on every acceptance)
+        if (!STEP && a.final_g != nullptr && l < D) static_cast<float*>(a.final_g)[(size_t)c * Dp + l] = S.g[l];
         stage = kNoise;
       }
     }
@@ -556,14 +795,17 @@ __device__ __forceinline__ void run_chain(const LogisticArgs& a, Smem& S, int c,
     // ---- metric at S.q: G, its sweep, u = G^-1 p (momentum draw first at kNoise) ----
     if (stage != kTraj || !have_ginv) {
       gram_pass<NB>(S, Xg, SL, npad, D, inv_prior);
-      invert<NB>(S, D, stage == kNoise, stage != kEnd);
+      if (stage == kNoise)
+        invert<NB>(S, D, true, true);  // full sweep: the momentum draw needs chol(G)
+      else
+        invert_blocked<NB>(S, D);
     }
     if (stage == kNoise) {
       if (STEP && a.momentum != nullptr && l < D) static_cast<float*>(a.momentum)[(size_t)c * Dp + l] = S.p0[l];
       const double logdet = logdet_from_pivots(S, D);
       const double pu = group_sum<64>(l < D ? (double)S.p[l] * (double)S.u[l] : 0.0);
       H0 = -logp + 0.5 * pu + 0.5 * logdet + ke_const;
-      have_ginv = true;  // S.G = G(S.q)^-1 and S.u = G^-1 S.p
+      have_ginv = true;  // S.Gi = G(S.q)^-1 and S.u = G^-1 S.p
       s = 0;
       mode = 0;
       stage = a.L > 0 ? kTraj : kEnd;
@@ -604,7 +846,7 @@ __device__ __forceinline__ void run_chain(const LogisticArgs& a, Smem& S, int c,
       }
       if (accept) {
         S.q0[l] = S.q[l];
-        S.gs[l] = S.g[l];
+        if (!STEP && a.final_g != nullptr && l < D) static_cast<float*>(a.final_g)[(size_t)c * Dp + l] = S.g[l];
         logp = logp1;
         cur = 1 - cur;
         ++n_acc;
@@ -624,7 +866,7 @@ __device__ __forceinline__ void run_chain(const LogisticArgs& a, Smem& S, int c,
       q0v = S.q[l];
       p0v = S.p[l];
     }
-    field_pass<NB>(S, Xr, yf, E0, E1, SL, npad, D, hs, inv_prior, mode == 2);
+    field_pass<NB>(S, Xr, Xs, yf, E0, E1, SL, npad, D, hs, inv_prior, mode == 2);
     have_ginv = false;
     ++n_fev;
     if (mode == 2) {  // explicit half step from the last iterate (integrators.py:146-148)
@@ -669,7 +911,6 @@ __device__ __forceinline__ void run_chain(const LogisticArgs& a, Smem& S, int c,
   if (!STEP) {
     if (l < D) {
       static_cast<float*>(a.final_q)[(size_t)c * Dp + l] = S.q0[l];
-      if (a.final_g) static_cast<float*>(a.final_g)[(size_t)c * Dp + l] = S.gs[l];
     }
     if (l == 0) {
       a.final_logp[c] = logp;
@@ -717,7 +958,14 @@ __global__ void logistic_pack_kernel(const double* __restrict__ X, const double*
   {
     const int rb = n / 32, kc = f / 16;
     const int lane = ((f / 8) % 2) * 32 + n % 32, i = f % 8;
-    packed[pp.xr + (((int64_t)rb * KC + kc) * 64 + lane) * 8 + i] = v;
+    const int64_t frag = ((int64_t)rb * KC + kc) * 64 + lane;
+    packed[pp.xr + frag * 8 + i] = v;
+    // the same split as split8: hi = bf16(v), lo = bf16(v - hi)
+    const __bf16 hi = (__bf16)v;
+    const __bf16 lo = (__bf16)(v - (float)hi);
+    __bf16* xs = reinterpret_cast<__bf16*>(packed + pp.xs);
+    xs[frag * 16 + i] = hi;
+    xs[frag * 16 + 8 + i] = lo;
   }
   if (f == 0) packed[pp.yf + n] = n < N ? (float)y[yoff[p] + n] : 0.0f;
 }
@@ -729,13 +977,15 @@ uint64_t logistic_pack_layout(int P, const int32_t* N_host, const int32_t* D_hos
   uint64_t off = hdr_floats;
   for (int p = 0; p < P; ++p) {
     PackedPost h;
-    h.npad = (N_host[p] + 31) / 32 * 32;
+    h.npad = (N_host[p] + 63) / 64 * 64;
     h.nb = D_host[p] <= 32 ? 1 : 2;
-    const uint64_t xs = (uint64_t)h.npad * 32 * h.nb;
+    const uint64_t sz = (uint64_t)h.npad * 32 * h.nb;  // floats of one layout
     h.xg = (int64_t)off;
-    off += xs;
+    off += sz;
     h.xr = (int64_t)off;
-    off += xs;
+    off += sz;
+    h.xs = (int64_t)off;  // bf16 pairs: as many bytes as the float layout
+    off += sz;
     h.yf = (int64_t)off;
     off += ((uint64_t)h.npad + 63) / 64 * 64;
     if (hdr_out) hdr_out[p] = h;

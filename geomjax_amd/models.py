@@ -198,7 +198,7 @@ class _posterior_table:
         return self._packed
 
     def workspace(self, num_chains):
-        need = num_chains * 5 * ((self.max_rows + 31) // 32 * 32) * 4
+        need = num_chains * 5 * ((self.max_rows + 63) // 64 * 64) * 4
         if self._workspace is None or self._workspace.numel() < need:
             self._workspace = torch.empty(need, dtype=torch.uint8, device=self.x.device)
         return self._workspace

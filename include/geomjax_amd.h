@@ -263,7 +263,7 @@ int gjx_logistic_pack(const gjx_logistic_data* data, const int32_t* num_rows_hos
                       void* packed, uint64_t packed_bytes, void* stream);
 
 /* Bytes of the per-chain scratch the tuned path needs for params->num_chains
- * chains and data->max_rows rows (5 float arrays of max_rows rounded up to 32
+ * chains and data->max_rows rows (5 float arrays of max_rows rounded up to 64
  * per chain).  Calls that overlap in time need separate workspaces. */
 int gjx_logistic_workspace_size(const gjx_logistic_params* params, const gjx_logistic_data* data, uint64_t* bytes);
 

@@ -56,13 +56,14 @@ def test_invalid_arguments_are_rejected_without_a_gpu():
 
 
 def _pack_size_numpy(N, D):
-    """Bytes of the packed float32 table: 256-byte header of 32-byte entries, then per posterior
-    two fragment layouts of npad x 32nb floats and npad (rounded up to 64) labels."""
+    """Bytes of the packed float32 table: header of 40-byte entries rounded up to 256 bytes, then
+    per posterior three fragment layouts of npad x 32nb words (Gram, rows, rows split into bf16
+    pairs) and npad (rounded up to 64) labels."""
     P = len(N)
-    off = -(-P * 32 // 256) * 64
+    off = -(-P * 40 // 256) * 64
     for n, d in zip(N, D):
-        npad, nb = -(-n // 32) * 32, 1 if d <= 32 else 2
-        off += 2 * npad * 32 * nb + -(-npad // 64) * 64
+        npad, nb = -(-n // 64) * 64, 1 if d <= 32 else 2
+        off += 3 * npad * 32 * nb + -(-npad // 64) * 64
     return off * 4
 
 
@@ -81,7 +82,7 @@ def test_logistic_pack_and_workspace_sizes():
     p = _lib.LogisticParams(num_chains=7, dim=64, dtype=_lib.GJX_F32)
     data = _lib.LogisticData(num_posteriors=1, max_rows=2000, max_features=64)
     assert lib.gjx_logistic_workspace_size(p, data, ctypes.byref(out)) == 0
-    assert out.value == 7 * 5 * 2016 * 4
+    assert out.value == 7 * 5 * 2048 * 4
 
 
 def test_logistic_float32_requires_packed_table_and_workspace():

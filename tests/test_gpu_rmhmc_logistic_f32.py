@@ -168,7 +168,7 @@ def _layouts_numpy(X, y):
     import numpy as np
 
     N, D = X.shape
-    npad, nb = -(-N // 32) * 32, 1 if D <= 32 else 2
+    npad, nb = -(-N // 64) * 64, 1 if D <= 32 else 2
     Xp = np.zeros((npad, 32 * nb))
     Xp[:N, :D] = X
     lane = np.arange(64)
@@ -194,13 +194,21 @@ def test_pack_layout_matches_numpy_restatement():
     data = [make_logistic(n, d, 40 + i)[:2] for i, (n, d) in enumerate(shapes)]
     batch = gj.LogisticBatch([gj.LogisticRegression(X, y, 100.0) for X, y in data], 1)
     buf = batch._table.packed().cpu().numpy()
-    hdr = np.frombuffer(buf[:32 * len(shapes)].tobytes(), dtype=np.dtype(
-        [("xg", "<i8"), ("xr", "<i8"), ("yf", "<i8"), ("npad", "<i4"), ("nb", "<i4")]))
+    hdr = np.frombuffer(buf[:40 * len(shapes)].tobytes(), dtype=np.dtype(
+        [("xg", "<i8"), ("xr", "<i8"), ("xs", "<i8"), ("yf", "<i8"), ("npad", "<i4"), ("nb", "<i4")]))
     flt = np.frombuffer(buf.tobytes(), dtype=np.float32)
+    u16 = np.frombuffer(buf.tobytes(), dtype=np.uint16)
     for p, (X, y) in enumerate(data):
         xg, xr, yf = _layouts_numpy(X, y)
         h = hdr[p]
-        assert h["npad"] == -(-X.shape[0] // 32) * 32 and h["nb"] == (1 if X.shape[1] <= 32 else 2)
+        assert h["npad"] == -(-X.shape[0] // 64) * 64 and h["nb"] == (1 if X.shape[1] <= 32 else 2)
         np.testing.assert_array_equal(flt[h["xg"]:h["xg"] + xg.size], xg.astype(np.float32))
         np.testing.assert_array_equal(flt[h["xr"]:h["xr"] + xr.size], xr.astype(np.float32))
         np.testing.assert_array_equal(flt[h["yf"]:h["yf"] + yf.size], yf.astype(np.float32))
+        # split layout: per lane fragment 8 bf16 hi then 8 bf16 lo with hi + lo ~ x (|x - hi - lo| <= 2^-17 |x|)
+        pairs = u16[2 * h["xs"]:2 * h["xs"] + 2 * xr.size].reshape(-1, 2, 8)
+        bf = lambda b: (b.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+        hi, lo = bf(pairs[:, 0, :]).reshape(-1), bf(pairs[:, 1, :]).reshape(-1)
+        x32 = xr.astype(np.float32).astype(np.float64)
+        assert np.all(np.abs(x32 - hi - lo) <= 2.0 ** -17 * np.abs(x32))
+        assert np.all(np.abs(lo) <= 2.0 ** -8 * np.abs(hi) + 1e-45)
