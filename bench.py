@@ -21,7 +21,8 @@ A "step" is one transition of all 4096 chains (10 integrator steps each).
 ``value`` = chains * 10 * steps / wall seconds of the timed region (inputs
 resident in HBM).  min-ESS/s uses the multi-chain ESS of
 geomjax/diagnostics.py:78-209 over the timed samples.  Extra fields report the
-same comparison at 1024 chains and for Neal's funnel D=100 (RMHMC with the
+same comparison at 1024 chains, for 256 logistic posteriors of uneven size in
+one launch (8192 and 4096 chains), and for Neal's funnel D=100 (RMHMC with the
 Monge metric, 8192 chains).  The CPU baseline is the float64 oracle
 restatement (dense metric + autodiff, as the reference) timed on this host's
 cores on a bounded sample of the headline workload.
@@ -197,6 +198,101 @@ def logistic_forms(chains, K, W, dev, plain=True):
     return res
 
 
+def batch_min_ess(samples, counts, dims):
+    """min over posteriors and their coordinates of the multi-chain ESS (chains of one posterior)."""
+    import torch
+
+    from geomjax_amd.diagnostics import effective_sample_size
+
+    out, c0 = [], 0
+    for n, d in zip(counts, dims):
+        ess = effective_sample_size(samples[:, c0:c0 + n, :d], chain_axis=1, sample_axis=0)
+        out.append(ess.min())
+        c0 += n
+    return float(torch.stack(out).min().item())
+
+
+def uneven_batch_forms(chains_per_posterior, K, W, dev, plain=True):
+    """Tuned and plain RMHMC-Fisher on 256 logistic posteriors of uneven size in one launch.
+
+    Posteriors as the uneven-batch configuration generates them (N log-uniform in
+    [256, 4096], data seed 3) with D drawn from {8, 16, 32, 64}: the kernels serve
+    D <= 64 (a posterior with D = 128 needs a multi-wave chain tile)."""
+    import torch
+
+    import geomjax_amd as gj
+    from geomjax_amd import random as gr
+    from geomjax_amd.rmhmc import build_kernel, implicit_midpoint, sample
+    from geomjax_amd.synthetic import make_uneven_batch
+
+    data = make_uneven_batch(P=256, seed=3, dims=(8, 16, 32, 64))
+    models = [gj.LogisticRegression(X, y, PRIOR, device=dev) for X, y, _ in data]
+    batch = gj.LogisticBatch(models, chains_per_posterior)
+    counts, dims = batch.counts, [m.D for m in models]
+    C, Dm = batch.num_chains, batch.dim
+    integ = functools.partial(implicit_midpoint, max_iters=MAXIT_LOG)
+    metric = gj.FisherMetric()
+    q_init = torch.zeros(C, Dm, dtype=torch.float64)
+    g = torch.Generator().manual_seed(INIT_SEED)
+    c0 = 0
+    for n, d in zip(counts, dims):
+        q_init[c0:c0 + n, :d] = torch.randn(n, d, generator=g, dtype=torch.float64)
+        c0 += n
+    key = gr.key(INIT_SEED)
+    K = max(CHUNK_LOG, (K // CHUNK_LOG) * CHUNK_LOG)
+    work = sum(n * m.N * m.D * m.D for n, m in zip(counts, models))
+    log(f"uneven batch: {len(models)} posteriors, {C} chains, sum chains*N*D^2 = {work:.3e}")
+
+    state = gj.rmhmc.init(q_init.to(dev, torch.float32), batch)
+    counter = 10_000
+    for _ in range(max(1, W // CHUNK_LOG)):
+        r = sample(gr.PRNGKey(key.seed, counter), state, batch, EPS_LOG, metric, L_LOG, CHUNK_LOG,
+                   integrator=integ, store_samples=False)
+        state, counter = r.state, counter + CHUNK_LOG
+    warmed = state
+    samples = torch.empty(K, C, Dm, dtype=torch.float32, device=dev)
+    fevs = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for c in range(K // CHUNK_LOG):
+        r = sample(gr.PRNGKey(key.seed, counter), state, batch, EPS_LOG, metric, L_LOG, CHUNK_LOG,
+                   integrator=integ, samples_out=samples[c * CHUNK_LOG:(c + 1) * CHUNK_LOG])
+        state, counter = r.state, counter + CHUNK_LOG
+        fevs.append(r.num_fevals)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ess = batch_min_ess(samples, counts, dims)
+    del samples
+    tuned = {"value": C * L_LOG * K / wall, "unit": "leapfrog-steps/s", "min_ess_per_sec": ess / wall,
+             "min_ess": ess, "ms_per_step": wall / K * 1e3, "steps": K,
+             "fevals_per_chain_transition": float(torch.stack(fevs).double().sum().item()) / C / K}
+    log(f"uneven batch tuned: {tuned['value']:.3e} leapfrog-steps/s, {tuned['ms_per_step']:.2f} ms/step, "
+        f"min-ESS/s {tuned['min_ess_per_sec']:.3e}")
+    res = {"posteriors": len(models), "chains": C, "dims": "D in {8, 16, 32, 64}, N in [256, 4096]",
+           "tuned": tuned, "plain": None}
+    if plain:
+        kern = build_kernel(integ)
+        s64 = gj.rmhmc.init(warmed.position.double(), batch)
+        s64, _ = kern(gr.PRNGKey(key.seed + 1, 0), s64, batch, EPS_LOG, metric, L_LOG)
+        psamples = torch.empty(K, C, Dm, dtype=torch.float64, device=dev)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for t in range(K):
+            s64, _ = kern(gr.PRNGKey(key.seed + 1, 1 + t), s64, batch, EPS_LOG, metric, L_LOG)
+            psamples[t].copy_(s64.position)
+        torch.cuda.synchronize()
+        pwall = time.perf_counter() - t0
+        pess = batch_min_ess(psamples, counts, dims)
+        del psamples
+        res["plain"] = {"value": C * L_LOG * K / pwall, "unit": "leapfrog-steps/s", "min_ess_per_sec": pess / pwall,
+                        "min_ess": pess, "ms_per_step": pwall / K * 1e3, "dtype": "f64", "steps": K,
+                        "form": "float64 kernel, one launch per transition"}
+        res["tuned_vs_plain"] = {"leapfrog_steps_per_sec": tuned["value"] / res["plain"]["value"],
+                                 "min_ess_per_sec": tuned["min_ess_per_sec"] / res["plain"]["min_ess_per_sec"]}
+        log(f"uneven batch plain: {res['plain']['value']:.3e} leapfrog-steps/s; tuned/plain {res['tuned_vs_plain']}")
+    return res
+
+
 def funnel_forms(K, W, dev, plain=True):
     """Tuned and plain RMHMC-Monge on Neal's funnel D=100 with 8192 chains."""
     import torch
@@ -306,6 +402,8 @@ def main():
     extra = {}
     if not args.no_extra:
         extra["logistic_1024_chains"] = logistic_forms(1024, K, W, dev, plain=not args.no_plain)
+        extra["uneven_batch_8192_chains"] = uneven_batch_forms(32, K, W, dev, plain=not args.no_plain)
+        extra["uneven_batch_4096_chains"] = uneven_batch_forms(16, K, W, dev, plain=not args.no_plain)
         extra["funnel_d100_8192_chains"] = funnel_forms(300, 100, dev, plain=not args.no_plain)
 
     cpu = None

@@ -563,7 +563,7 @@ __device__ __forceinline__ void field_pass(Smem& S, const float* __restrict__ Xr
     for (int kc = 0; kc < KC; ++kc) load8(xp + (size_t)(rb * KC + kc) * 512, B.x[kc]);
     const int row = rb * 32 + (l % 32);
     B.eta = E1[row];
-    B.base = E0[row];
+    B.base = explicit_step ? 0.0f : E0[row];
     B.yv = yf[row];
   };
   auto mfma_blk = [&](const XF& f, Blk& B) {
@@ -587,6 +587,7 @@ __device__ __forceinline__ void field_pass(Smem& S, const float* __restrict__ Xr
     }
     B.b = bp + __shfl_xor(bp, 32, 64);
   };
+  float* const e0_dst = explicit_step ? E0 : E1;
   // a_n, h_n, X^T h and the next iterate's eta / sqrt(lambda) of block rb;
   // both lanes of a row store the same values
   auto post = [&](const Blk& B, int rb) {
@@ -610,7 +611,7 @@ __device__ __forceinline__ void field_pass(Smem& S, const float* __restrict__ Xr
     const float en = fmaf(hs, B.b, explicit_step ? B.eta : B.base);
     const float sn = sigmoidf(en);
     E1[row] = en;
-    E0[row] = explicit_step ? en : B.base;
+    e0_dst[row] = en;  // the explicit step moves the step start; iterates rewrite E1 (no E0 traffic)
     SL[row] = sqrt_fast(sn * (1.0f - sn));
   };
   // Two-stage pipeline: the MFMAs of block r+1 run while block r is finished
