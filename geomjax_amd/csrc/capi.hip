@@ -313,7 +313,8 @@ void workspace_bytes(int C, int max_rows, uint64_t* bytes) {
   *bytes = (uint64_t)C * 5 * (uint64_t)((max_rows + 63) / 64 * 64) * sizeof(float);
 }
 
-int check_logistic(const gjx_logistic_params* p, const gjx_logistic_data* d, bool sample, gjx::LogisticArgs* a) {
+int check_logistic(const gjx_logistic_params* p, const gjx_logistic_data* d, bool sample, gjx::LogisticArgs* a,
+                   int max_d) {
   GJX_REQUIRE(p != nullptr && d != nullptr, "params and data are required");
   GJX_REQUIRE(p->num_chains > 0, "num_chains must be > 0 (got %d)", p->num_chains);
   GJX_REQUIRE(p->dtype == GJX_F64 || p->dtype == GJX_F32, "dtype must be GJX_F64 or GJX_F32 (got %d)", p->dtype);
@@ -326,9 +327,8 @@ int check_logistic(const gjx_logistic_params* p, const gjx_logistic_data* d, boo
   GJX_REQUIRE(d->max_rows >= 1, "max_rows must be >= 1");
   GJX_REQUIRE(d->max_features >= 1 && d->max_features <= p->dim,
               "max_features must be in [1, dim] (got %d, dim %d)", d->max_features, p->dim);
-  if (d->max_features > gjx::kLogisticMaxD)
-    return fail(GJX_EUNSUPPORTED, "D = %d exceeds the kernels' maximum of %d features", d->max_features,
-                gjx::kLogisticMaxD);
+  if (d->max_features > max_d)
+    return fail(GJX_EUNSUPPORTED, "D = %d exceeds this kernel's maximum of %d features", d->max_features, max_d);
   GJX_REQUIRE(d->num_rows && d->num_features && d->x_offset && d->y_offset && d->x && d->y,
               "data arrays are required");
   std::memset(a, 0, sizeof(*a));
@@ -338,6 +338,7 @@ int check_logistic(const gjx_logistic_params* p, const gjx_logistic_data* d, boo
   a->max_iters = p->max_iters;
   a->Dmax = p->dim;
   a->P = d->num_posteriors;
+  a->max_features = d->max_features;
   a->eps = p->step_size;
   a->prior_var = p->prior_var;
   a->conv_tol = p->convergence_tol;
@@ -367,6 +368,11 @@ int check_logistic(const gjx_logistic_params* p, const gjx_logistic_data* d, boo
   return GJX_OK;
 }
 
+// the float64 RMHMC kernel serves D <= 128, the float32 one D <= 64
+int rmhmc_max_d(const gjx_logistic_params* p) {
+  return (p != nullptr && p->dtype == GJX_F32) ? gjx::kLogisticMaxD : gjx::kLogisticMaxDF64;
+}
+
 // only the RMHMC entry points have a float32 path
 int require_f64(const gjx_logistic_params* p, const char* what) {
   if (p != nullptr && p->dtype == GJX_F32) return fail(GJX_EUNSUPPORTED, "%s has no float32 path (use GJX_F64)", what);
@@ -378,7 +384,7 @@ int require_f64(const gjx_logistic_params* p, const char* what) {
 int gjx_rmhmc_fisher_logistic_step(const gjx_logistic_params* params, const gjx_logistic_data* data,
                                    const gjx_rmhmc_step_io* io, void* stream) {
   gjx::LogisticArgs a;
-  int rc = check_logistic(params, data, false, &a);
+  int rc = check_logistic(params, data, false, &a, rmhmc_max_d(params));
   if (rc != GJX_OK) return rc;
   GJX_REQUIRE(io != nullptr, "io is NULL");
   GJX_REQUIRE(io->position && io->logdensity, "position and logdensity are required");
@@ -418,7 +424,7 @@ int gjx_rmhmc_fisher_logistic_step(const gjx_logistic_params* params, const gjx_
 int gjx_rmhmc_fisher_logistic_sample(const gjx_logistic_params* params, const gjx_logistic_data* data,
                                      const gjx_rmhmc_sample_io* io, void* stream) {
   gjx::LogisticArgs a;
-  int rc = check_logistic(params, data, true, &a);
+  int rc = check_logistic(params, data, true, &a, rmhmc_max_d(params));
   if (rc != GJX_OK) return rc;
   GJX_REQUIRE(io != nullptr, "io is NULL");
   GJX_REQUIRE(io->position && io->logdensity, "position and logdensity are required");
@@ -445,7 +451,7 @@ int gjx_lmc_fisher_logistic_step(const gjx_logistic_params* params, const gjx_lo
                                  const gjx_lmcmc_step_io* io, void* stream) {
   if (int rc0 = require_f64(params, "gjx_lmc_fisher_logistic_step")) return rc0;
   gjx::LogisticArgs a;
-  int rc = check_logistic(params, data, false, &a);
+  int rc = check_logistic(params, data, false, &a, gjx::kLogisticMaxD);
   if (rc != GJX_OK) return rc;
   GJX_REQUIRE(io != nullptr, "io is NULL");
   GJX_REQUIRE(io->position && io->logdensity && io->volume_adjustment,
@@ -486,7 +492,7 @@ int gjx_lmc_fisher_logistic_sample(const gjx_logistic_params* params, const gjx_
                                    const gjx_lmcmc_sample_io* io, void* stream) {
   if (int rc0 = require_f64(params, "gjx_lmc_fisher_logistic_sample")) return rc0;
   gjx::LogisticArgs a;
-  int rc = check_logistic(params, data, true, &a);
+  int rc = check_logistic(params, data, true, &a, gjx::kLogisticMaxD);
   if (rc != GJX_OK) return rc;
   GJX_REQUIRE(io != nullptr, "io is NULL");
   GJX_REQUIRE(io->position && io->logdensity && io->volume_adjustment,
@@ -514,7 +520,7 @@ int gjx_logistic_value_and_grad(const gjx_logistic_params* params, const gjx_log
                                 const void* position, double* logdensity, void* logdensity_grad, void* stream) {
   if (int rc0 = require_f64(params, "gjx_logistic_value_and_grad")) return rc0;
   gjx::LogisticArgs a;
-  int rc = check_logistic(params, data, false, &a);
+  int rc = check_logistic(params, data, false, &a, gjx::kLogisticMaxDF64);
   if (rc != GJX_OK) return rc;
   GJX_REQUIRE(position && logdensity && logdensity_grad, "position, logdensity and logdensity_grad are required");
   a.q_in = position;

@@ -19,6 +19,7 @@ struct PackedPost {
 
 struct LogisticArgs {
   int C, L, T, max_iters, Dmax, P;
+  int max_features;  // host copy of max_p D_p
   double eps, prior_var, conv_tol, div_tol, div_threshold;
   uint64_t seed, counter;
   // data: posteriors p = 0..P-1 with X_p [N_p, D_p] row-major at X + post_xoff[p]
@@ -74,7 +75,37 @@ struct LogisticArgs {
   int64_t work_rows;
 };
 
-constexpr int kLogisticMaxD = 64;
+// Coordinates [D, Dmax) of chain c's rows (a posterior narrower than the
+// state arrays) are written as zeros in every per-chain row output the call
+// fills, so outputs never depend on what the caller's buffers held.  Lanes
+// lane, lane + nlanes, ... of the chain's workgroup share the work.
+template <class T, bool STEP>
+__device__ __forceinline__ void zero_row_tails(const LogisticArgs& a, int c, int D, int lane, int nlanes) {
+  const int Dp = a.Dmax;
+  if (D >= Dp) return;
+  const size_t row = (size_t)c * Dp;
+  auto zero = [&](void* base, size_t r) {
+    if (base == nullptr) return;
+    for (int j = D + lane; j < Dp; j += nlanes) static_cast<T*>(base)[r + j] = T(0);
+  };
+  if (STEP) {
+    zero(a.q_out, row);
+    zero(a.g_out, row);
+    zero(a.momentum, row);
+    zero(a.prop_q, row);
+    zero(a.prop_p, row);
+    zero(a.prop_v, row);
+    zero(a.prop_g, row);
+  } else {
+    zero(a.final_q, row);
+    zero(a.final_g, row);
+    if (a.samples != nullptr)
+      for (int tr = 0; tr < a.T; ++tr) zero(a.samples, ((size_t)tr * a.C + c) * Dp);
+  }
+}
+
+constexpr int kLogisticMaxD = 64;       // LMC and the float32 RMHMC kernels
+constexpr int kLogisticMaxDF64 = 128;   // float64 RMHMC kernel and value_and_grad
 
 hipError_t launch_logistic_rmhmc_f64(const LogisticArgs& a, hipStream_t stream);
 hipError_t launch_logistic_value_and_grad(const LogisticArgs& a, hipStream_t stream);

@@ -919,6 +919,7 @@ __device__ __forceinline__ void run_chain(const LogisticArgs& a, Smem& S, int c,
       if (a.num_fevals) a.num_fevals[c] = n_fev;
     }
   }
+  zero_row_tails<float, STEP>(a, c, D, l, 64);
 }
 
 // One wave per chain; one instantiation per tile width NB (chains of the

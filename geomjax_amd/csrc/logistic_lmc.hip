@@ -26,10 +26,13 @@ namespace gjx {
 
 using namespace logistic;
 
+using C64 = Cfg<64>;  // the LMC kernel serves D <= 64
+constexpr int MAXD = C64::MAXD, LDX = C64::LDX;
+
 template <bool STEP>
 __global__ void __launch_bounds__(NT, 2) logistic_lmc_f64_kernel(const LogisticArgs a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  Smem& S = *reinterpret_cast<Smem*>(smem_raw);
+  Smem<C64>& S = *reinterpret_cast<Smem<C64>*>(smem_raw);
   const int t = threadIdx.x;
   const int c = blockIdx.x;
   if (c >= a.C) return;
@@ -59,7 +62,7 @@ __global__ void __launch_bounds__(NT, 2) logistic_lmc_f64_kernel(const LogisticA
 
   // position quantities at S.q: S.Gi = G^-1, S.g = grad l, S.dH = grad log det G;
   // returns log det G and (through loglik) the log-likelihood sum
-  auto position_pass = [&](bool chol_draw, double& loglik) -> double {
+  auto position_pass = [&](bool chol_draw, double& loglik) __attribute__((always_inline)) -> double {
     pass_gram(S, X, y, N, D, prior, nullptr, 0.0, true, false, loglik);
     load_block(S, A);
     __syncthreads();
@@ -84,7 +87,7 @@ __global__ void __launch_bounds__(NT, 2) logistic_lmc_f64_kernel(const LogisticA
   };
 
   // one half step (integrators.py:61-91) with velocity S.p; accumulates dJ
-  auto half_step = [&](double& dJ) {
+  auto half_step = [&](double& dJ) __attribute__((always_inline)) {
     double dd = 0.0;
     // S.vt = grad log det G and S.u = grad l survive the two Gram passes below
     if (t < D) {
@@ -219,10 +222,11 @@ __global__ void __launch_bounds__(NT, 2) logistic_lmc_f64_kernel(const LogisticA
       if (a.num_accepted) a.num_accepted[c] = n_acc;
     }
   }
+  zero_row_tails<double, STEP>(a, c, D, t, NT);
 }
 
 hipError_t launch_logistic_lmc_f64(const LogisticArgs& a, hipStream_t stream) {
-  const size_t smem = sizeof(Smem);
+  const size_t smem = sizeof(Smem<C64>);
   if (a.q_out != nullptr) {
     (void)hipFuncSetAttribute((const void*)logistic_lmc_f64_kernel<true>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);

@@ -16,17 +16,19 @@
 // so dH/dq = X^T [ w (a - b^2) / 2 - r ] + q / prior_var.
 //
 // One workgroup (256 lanes, 4 waves) owns one chain for the whole launch, so
-// every chain runs its own fixed-point iteration count.  X streams through LDS
-// in chunks of 32 rows.  Per evaluation:
+// every chain runs its own fixed-point iteration count.  Two instantiations by
+// feature capacity: 64 (X in LDS chunks of 32 rows, 10 lower 16x16 Gram tiles,
+// 4x4 inverse entries per lane) and 128 (chunks of 16 rows, 36 tiles, 8x8
+// entries per lane; G^-1 alone is 129 KiB of LDS).  Per evaluation:
 //   pass 1: eta = X q, lambda, and the Gram X^T diag(lambda) X on MFMA
-//           (10 lower 16x16 tiles of the 64x64 matrix, tiles owned by waves);
+//           (lower tiles owned by waves);
 //   sweep : G^-1 by the symmetric sweep operator with G held in registers
-//           (4x4 entries per lane, one barrier per pivot), log det G = sum of
-//           the pivots' logs;
+//           (one barrier per pivot), log det G = sum of the pivots' logs;
 //   pass 2: eta, s, w, r, b = X u on VALU, V = X_chunk G^-1 on MFMA,
 //           a_n = x_n . V_n, h_n, and X^T h.
-// The momentum draw p0 = chol(G) z uses a right-looking Cholesky in LDS
-// (once per transition).
+// The momentum draw p0 = chol(G) z comes out of the first sweep of each
+// transition: before pivot k is eliminated, row k of the partly swept matrix
+// holds column k of the Schur complement, i.e. sqrt(d_k) L[:, k].
 #include "common.h"
 #include "logistic.h"
 #include "logistic_common.h"
@@ -35,16 +37,21 @@ namespace gjx {
 
 using namespace logistic;
 
-template <bool STEP>
-__global__ void __launch_bounds__(NT, 2) logistic_rmhmc_f64_kernel(const LogisticArgs a) {
+// MD: feature capacity (64 or 128); chains whose posterior has D outside
+// (MD/2, MD] (or D <= 64 for MD = 64) belong to the other instantiation.
+template <int MD, bool STEP>
+__global__ void __launch_bounds__(NT, MD == 64 ? 2 : 1) logistic_rmhmc_f64_kernel(const LogisticArgs a) {
+  using C = Cfg<MD>;
+  constexpr int MAXD = C::MAXD, BS = C::BS;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  Smem& S = *reinterpret_cast<Smem*>(smem_raw);
+  Smem<C>& S = *reinterpret_cast<Smem<C>*>(smem_raw);
   const int t = threadIdx.x;
   const int c = blockIdx.x;
   if (c >= a.C) return;  // whole workgroups leave together
   const int post = a.chain_post ? a.chain_post[c] : 0;
   if (post < 0 || post >= a.P) return;  // invalid posterior index: chain left untouched
   const int N = a.post_N[post], D = a.post_D[post];
+  if (D > MD || (MD == 128 && D <= 64)) return;  // served by the other capacity
   const double* X = a.X + a.post_xoff[post];
   const double* y = a.y + a.post_yoff[post];
   const int Dp = a.Dmax;  // row stride of the chain-major state arrays
@@ -93,20 +100,14 @@ __global__ void __launch_bounds__(NT, 2) logistic_rmhmc_f64_kernel(const Logisti
     // ---- momentum p0 = chol(G(q0)) z and H0 ----
     double dummy = 0.0;
     pass1(S, X, y, N, D, prior, false, dummy);
-    double A[4][4];
+    double A[BS][BS];
     load_block(S, A);
     __syncthreads();
-    cholesky(S, D);
-    if (t < MAXD) {
-      double s = 0.0;
-      for (int k = 0; k <= t && k < D; ++k) s += S.Gi[t * LDX + k] * S.z[k];
-      S.p0[t] = t < D ? s : 0.0;
-      S.p[t] = S.p0[t];
-    }
+    // the sweep also draws p0 = chol(G(q0)) z from its Schur-complement columns
+    double logdet = sweep(S, A, D, S.z, S.p0);
     __syncthreads();
+    if (t < MAXD) S.p[t] = S.p0[t];
     if (STEP && a.momentum != nullptr && t < D) static_cast<double*>(a.momentum)[(size_t)c * Dp + t] = S.p0[t];
-    double logdet = sweep(S, A, D);
-    __syncthreads();
     store_neg_block(S, A);
     __syncthreads();
     ginv_times(S, S.p, S.u, D);
@@ -118,7 +119,7 @@ __global__ void __launch_bounds__(NT, 2) logistic_rmhmc_f64_kernel(const Logisti
     bool have_ginv = true;  // S.Gi = G(S.q)^-1 and S.u = G^-1 S.p
 
     // one evaluation of the vector field at (S.q, S.p): S.u = dH/dp, S.dH = dH/dq
-    auto feval = [&]() {
+    auto feval = [&]() __attribute__((always_inline)) {
       if (!have_ginv) {
         double dd = 0.0;
         pass1(S, X, y, N, D, prior, false, dd);
@@ -250,19 +251,24 @@ __global__ void __launch_bounds__(NT, 2) logistic_rmhmc_f64_kernel(const Logisti
       if (a.num_fevals) a.num_fevals[c] = n_fev;
     }
   }
+  zero_row_tails<double, STEP>(a, c, D, t, NT);
 }
 
 // value_and_grad of the logistic log density (the reference's init,
 // geomjax/rmhmc/rmhmc.py:96-98): one workgroup per chain, pass 1 with the
 // likelihood and gradient accumulations.
+template <int MD>
 __global__ void __launch_bounds__(NT) logistic_value_and_grad_kernel(const LogisticArgs a) {
+  using C = Cfg<MD>;
+  constexpr int MAXD = C::MAXD;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  Smem& S = *reinterpret_cast<Smem*>(smem_raw);
+  Smem<C>& S = *reinterpret_cast<Smem<C>*>(smem_raw);
   const int t = threadIdx.x, c = blockIdx.x;
   if (c >= a.C) return;
   const int post = a.chain_post ? a.chain_post[c] : 0;
   if (post < 0 || post >= a.P) return;
   const int N = a.post_N[post], D = a.post_D[post];
+  if (D > MD || (MD == 128 && D <= 64)) return;
   const double* X = a.X + a.post_xoff[post];
   const double* y = a.y + a.post_yoff[post];
   if (t < MAXD) S.q[t] = t < D ? static_cast<const double*>(a.q_in)[(size_t)c * a.Dmax + t] : 0.0;
@@ -273,26 +279,42 @@ __global__ void __launch_bounds__(NT) logistic_value_and_grad_kernel(const Logis
   if (t == 0 && a.logp_out != nullptr)
     a.logp_out[c] = ll - 0.5 * qq / a.prior_var - 0.5 * (double)D * log(2.0 * M_PI * a.prior_var);
   if (t < D) static_cast<double*>(a.g_out)[(size_t)c * a.Dmax + t] = S.g[t];
+  for (int j = D + t; j < a.Dmax; j += NT) static_cast<double*>(a.g_out)[(size_t)c * a.Dmax + j] = 0.0;
 }
 
-hipError_t launch_logistic_value_and_grad(const LogisticArgs& a, hipStream_t stream) {
-  const size_t smem = sizeof(Smem);
-  (void)hipFuncSetAttribute((const void*)logistic_value_and_grad_kernel,
+template <int MD>
+static void launch_vg(const LogisticArgs& a, hipStream_t stream) {
+  const size_t smem = sizeof(Smem<Cfg<MD>>);
+  (void)hipFuncSetAttribute((const void*)logistic_value_and_grad_kernel<MD>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
-  hipLaunchKernelGGL(logistic_value_and_grad_kernel, dim3(a.C), dim3(NT), smem, stream, a);
+  hipLaunchKernelGGL(logistic_value_and_grad_kernel<MD>, dim3(a.C), dim3(NT), smem, stream, a);
+}
+
+// max_features <= 64: the 64-capacity kernels only; otherwise both (each
+// chain runs in the one whose capacity fits its posterior).
+hipError_t launch_logistic_value_and_grad(const LogisticArgs& a, hipStream_t stream) {
+  launch_vg<64>(a, stream);
+  if (a.Dmax > 64 && a.max_features > 64) launch_vg<128>(a, stream);
   return hipGetLastError();
 }
 
+template <int MD, bool STEP>
+static void launch_rmhmc(const LogisticArgs& a, hipStream_t stream) {
+  const size_t smem = sizeof(Smem<Cfg<MD>>);
+  (void)hipFuncSetAttribute((const void*)logistic_rmhmc_f64_kernel<MD, STEP>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+  hipLaunchKernelGGL((logistic_rmhmc_f64_kernel<MD, STEP>), dim3(a.C), dim3(NT), smem, stream, a);
+}
+
 hipError_t launch_logistic_rmhmc_f64(const LogisticArgs& a, hipStream_t stream) {
-  const size_t smem = sizeof(Smem);
+  const bool wide = a.max_features > 64;
+  const bool narrow = a.P > 1 || !wide;
   if (a.q_out != nullptr) {
-    (void)hipFuncSetAttribute((const void*)logistic_rmhmc_f64_kernel<true>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
-    hipLaunchKernelGGL(logistic_rmhmc_f64_kernel<true>, dim3(a.C), dim3(NT), smem, stream, a);
+    if (narrow) launch_rmhmc<64, true>(a, stream);
+    if (wide) launch_rmhmc<128, true>(a, stream);
   } else {
-    (void)hipFuncSetAttribute((const void*)logistic_rmhmc_f64_kernel<false>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
-    hipLaunchKernelGGL(logistic_rmhmc_f64_kernel<false>, dim3(a.C), dim3(NT), smem, stream, a);
+    if (narrow) launch_rmhmc<64, false>(a, stream);
+    if (wide) launch_rmhmc<128, false>(a, stream);
     if (a.final_g != nullptr) {
       // gradient at the final positions: value_and_grad on them (log density already carried)
       hipError_t e = hipGetLastError();

@@ -100,3 +100,23 @@ def test_logistic_float32_requires_packed_table_and_workspace():
     # entry points without a float32 path say so
     assert lib.gjx_lmc_fisher_logistic_step(p, data, _lib.LmcmcStepIO(), None) == -2
     assert "float32" in _lib.last_error()
+
+
+def test_logistic_feature_capacity_per_path():
+    """float64 RMHMC takes up to 128 features, float32 RMHMC and LMC up to 64."""
+    lib = _lib.load()
+
+    def step(fn, io, dtype, D):
+        p = _lib.LogisticParams(num_chains=4, dim=D, dtype=dtype, num_integration_steps=2, max_iters=2,
+                                num_transitions=1, step_size=0.1, prior_var=100.0)
+        data = _lib.LogisticData(num_posteriors=1, max_rows=16, max_features=D)  # data arrays missing
+        return fn(p, data, io, None)
+
+    rm, lmc = lib.gjx_rmhmc_fisher_logistic_step, lib.gjx_lmc_fisher_logistic_step
+    rio, lio = _lib.RmhmcStepIO(), _lib.LmcmcStepIO()
+    assert step(rm, rio, _lib.GJX_F64, 128) == -1 and "data arrays" in _lib.last_error()
+    assert step(rm, rio, _lib.GJX_F64, 129) == -2
+    assert step(rm, rio, _lib.GJX_F32, 64) == -1
+    assert step(rm, rio, _lib.GJX_F32, 65) == -2
+    assert step(lmc, lio, _lib.GJX_F64, 64) == -1
+    assert step(lmc, lio, _lib.GJX_F64, 65) == -2

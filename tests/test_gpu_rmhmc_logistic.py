@@ -28,6 +28,8 @@ DEV = "cuda:0"
     (256, 8, 3, 0.1, 10, 6, 2, 0),
     (300, 13, 2, 0.1, 6, 6, 1, 1),     # ragged: N not a multiple of 32, D not of 16
     (2048, 64, 2, 0.1, 10, 6, 1, 2),   # the 2048 x 64 logistic configuration's shape
+    (500, 100, 2, 0.05, 4, 6, 1, 3),   # wide tiling (65..128 features), ragged D
+    (1024, 128, 2, 0.05, 4, 6, 1, 4),  # wide tiling at its capacity
 ])
 def test_step_f64_matches_oracle(N, D, C, eps, L, max_iters, T, seed):
     print(f"case N={N} D={D} C={C} eps={eps} L={L} max_iters={max_iters} T={T}", flush=True)
@@ -88,3 +90,40 @@ def test_sample_kernel_matches_step_kernel():
     assert rel_err(res.state.logdensity, s.logdensity) <= 1e-12
     assert rel_err(res.state.logdensity_grad, s.logdensity_grad) <= 1e-12
     assert int(res.num_fevals.min()) >= T * L * 2
+
+
+def test_batch_mixed_widths_matches_single_posteriors():
+    """Posteriors of 8..128 features in one call: the narrow (<=64) and wide
+    (65..128) tilings each take their own chains; every chain must equal the
+    same chain run on its posterior alone."""
+    shapes = [(200, 8), (700, 24), (333, 100), (1500, 40), (600, 128)]
+    Cp, eps, L, T = 2, 0.05, 4, 2
+    integ = functools.partial(implicit_midpoint, max_iters=6)
+    models = [gj.LogisticRegression(*make_logistic(n, d, 40 + i)[:2], 100.0) for i, (n, d) in enumerate(shapes)]
+    batch = gj.LogisticBatch(models, chains_per_posterior=Cp)
+    Dmax = max(d for _, d in shapes)
+    C = Cp * len(shapes)
+    q0 = torch.zeros(C, Dmax, dtype=torch.float64)
+    for i, (_, d) in enumerate(shapes):
+        q0[i * Cp:(i + 1) * Cp, :d] = 0.1 * seeded_normal(80 + i, Cp, d)
+    zs = seeded_normal(90, T, C, Dmax)
+    us = seeded_uniform(91, T, C)
+    for i, (_, d) in enumerate(shapes):
+        zs[:, i * Cp:(i + 1) * Cp, d:] = 0.0
+    alg = gj.rmhmc(batch, eps, gj.FisherMetric(), L, integrator=integ)
+    s = alg.init(q0.to(DEV))
+    for t in range(T):
+        s, _ = alg.step(gr.HostNoise(zs[t].to(DEV), us[t].to(DEV)), s)
+    torch.cuda.synchronize()
+    for i, (m, (_, d)) in enumerate(zip(models, shapes)):
+        rows = slice(i * Cp, (i + 1) * Cp)
+        print(f"posterior {i}: D={d}", flush=True)
+        a1 = gj.rmhmc(m, eps, gj.FisherMetric(), L, integrator=integ)
+        s1 = a1.init(q0[rows, :d].contiguous().to(DEV))
+        for t in range(T):
+            s1, _ = a1.step(gr.HostNoise(zs[t, rows, :d].contiguous().to(DEV), us[t, rows].to(DEV)), s1)
+        torch.cuda.synchronize()
+        assert rel_err(s.position[rows, :d], s1.position) <= 1e-12
+        assert rel_err(s.logdensity[rows], s1.logdensity) <= 1e-12
+        if d < Dmax:
+            assert float(s.position[rows, d:].abs().max()) == 0.0
