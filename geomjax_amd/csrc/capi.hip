@@ -368,10 +368,8 @@ int check_logistic(const gjx_logistic_params* p, const gjx_logistic_data* d, boo
   return GJX_OK;
 }
 
-// the float64 RMHMC kernel serves D <= 128, the float32 one D <= 64
-int rmhmc_max_d(const gjx_logistic_params* p) {
-  return (p != nullptr && p->dtype == GJX_F32) ? gjx::kLogisticMaxD : gjx::kLogisticMaxDF64;
-}
+// both RMHMC kernels (float64 and float32) serve D <= 128
+int rmhmc_max_d(const gjx_logistic_params*) { return gjx::kLogisticMaxD; }
 
 // only the RMHMC entry points have a float32 path
 int require_f64(const gjx_logistic_params* p, const char* what) {
@@ -451,7 +449,7 @@ int gjx_lmc_fisher_logistic_step(const gjx_logistic_params* params, const gjx_lo
                                  const gjx_lmcmc_step_io* io, void* stream) {
   if (int rc0 = require_f64(params, "gjx_lmc_fisher_logistic_step")) return rc0;
   gjx::LogisticArgs a;
-  int rc = check_logistic(params, data, false, &a, gjx::kLogisticMaxD);
+  int rc = check_logistic(params, data, false, &a, gjx::kLogisticMaxDLmc);
   if (rc != GJX_OK) return rc;
   GJX_REQUIRE(io != nullptr, "io is NULL");
   GJX_REQUIRE(io->position && io->logdensity && io->volume_adjustment,
@@ -492,7 +490,7 @@ int gjx_lmc_fisher_logistic_sample(const gjx_logistic_params* params, const gjx_
                                    const gjx_lmcmc_sample_io* io, void* stream) {
   if (int rc0 = require_f64(params, "gjx_lmc_fisher_logistic_sample")) return rc0;
   gjx::LogisticArgs a;
-  int rc = check_logistic(params, data, true, &a, gjx::kLogisticMaxD);
+  int rc = check_logistic(params, data, true, &a, gjx::kLogisticMaxDLmc);
   if (rc != GJX_OK) return rc;
   GJX_REQUIRE(io != nullptr, "io is NULL");
   GJX_REQUIRE(io->position && io->logdensity && io->volume_adjustment,
@@ -520,7 +518,7 @@ int gjx_logistic_value_and_grad(const gjx_logistic_params* params, const gjx_log
                                 const void* position, double* logdensity, void* logdensity_grad, void* stream) {
   if (int rc0 = require_f64(params, "gjx_logistic_value_and_grad")) return rc0;
   gjx::LogisticArgs a;
-  int rc = check_logistic(params, data, false, &a, gjx::kLogisticMaxDF64);
+  int rc = check_logistic(params, data, false, &a, gjx::kLogisticMaxD);
   if (rc != GJX_OK) return rc;
   GJX_REQUIRE(position && logdensity && logdensity_grad, "position, logdensity and logdensity_grad are required");
   a.q_in = position;

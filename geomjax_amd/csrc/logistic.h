@@ -14,7 +14,7 @@ struct PackedPost {
   int64_t xs;    // row layout split into bf16: [npad/32][2 nb][64 lanes][8 hi, 8 lo]
   int64_t yf;    // labels [npad]
   int32_t npad;  // rows rounded up to 64 (padding rows are zero)
-  int32_t nb;    // 32-feature blocks: 1 (D <= 32) or 2 (D <= 64)
+  int32_t nb;    // 32-feature blocks: 1 (D <= 32), 2 (D <= 64) or 4 (D <= 128)
 };
 
 struct LogisticArgs {
@@ -104,13 +104,14 @@ __device__ __forceinline__ void zero_row_tails(const LogisticArgs& a, int c, int
   }
 }
 
-constexpr int kLogisticMaxD = 64;       // LMC and the float32 RMHMC kernels
-constexpr int kLogisticMaxDF64 = 128;   // float64 RMHMC kernel and value_and_grad
+constexpr int kLogisticMaxDLmc = 64;   // LMC kernel
+constexpr int kLogisticMaxD = 128;     // RMHMC kernels (float64 and float32), value_and_grad, packing
 
 hipError_t launch_logistic_rmhmc_f64(const LogisticArgs& a, hipStream_t stream);
 hipError_t launch_logistic_value_and_grad(const LogisticArgs& a, hipStream_t stream);
 hipError_t launch_logistic_lmc_f64(const LogisticArgs& a, hipStream_t stream);
 hipError_t launch_logistic_rmhmc_f32(const LogisticArgs& a, int max_features, hipStream_t stream);
+hipError_t launch_logistic_rmhmc_f32_wide(const LogisticArgs& a, hipStream_t stream);
 hipError_t launch_logistic_pack(const double* X, const double* y, const int32_t* N, const int32_t* D,
                                 const int64_t* xoff, const int64_t* yoff, const PackedPost* hdr, float* packed,
                                 int P, int max_npad, hipStream_t stream);

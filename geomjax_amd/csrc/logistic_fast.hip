@@ -46,13 +46,12 @@
 // is available for the momentum draw p0 = L z at no extra cost.
 #include "common.h"
 #include "logistic.h"
+#include "logistic_f32.h"
 
 namespace gjx {
 namespace lfast {
 
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x16 __attribute__((ext_vector_type(16)));
-typedef float f32x32 __attribute__((ext_vector_type(32)));
+using namespace lf32;
 
 constexpr int MAXD = 64;
 constexpr int LDG = MAXD + 4;  // LDS row stride of G^-1 (floats; rows stay 16-byte aligned)
@@ -66,55 +65,6 @@ struct Smem {
   float dk[MAXD];        // pivots
   float q0[MAXD], p0[MAXD], q[MAXD], p[MAXD], u[MAXD], dH[MAXD], z[MAXD], g[MAXD];
 };
-
-__device__ __forceinline__ void load8(const float* p, float (&v)[8]) {
-  const float4 a = reinterpret_cast<const float4*>(p)[0];
-  const float4 b = reinterpret_cast<const float4*>(p)[1];
-  v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
-  v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
-}
-
-// v = hi + lo + O(2^-17 |v|), hi and lo in bf16 (round to nearest even)
-__device__ __forceinline__ void split8(const float (&v)[8], bf16x8& hi, bf16x8& lo) {
-#pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    const __bf16 h = (__bf16)v[i];
-    hi[i] = h;
-    lo[i] = (__bf16)(v[i] - (float)h);
-  }
-}
-
-// acc += (ah + al)(bh + bl) without the lo*lo term
-__device__ __forceinline__ f32x16 mfma3(const bf16x8& ah, const bf16x8& al, const bf16x8& bh, const bf16x8& bl,
-                                        f32x16 acc) {
-  acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc, 0, 0, 0);
-  acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc, 0, 0, 0);
-  acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc, 0, 0, 0);
-  return acc;
-}
-
-// sigmoid with the hardware exp2 and reciprocal (a few ulp: the state is float32)
-__device__ __forceinline__ float sigmoidf(float x) { return __builtin_amdgcn_rcpf(1.0f + __expf(-x)); }
-__device__ __forceinline__ float sqrt_fast(float x) { return __builtin_amdgcn_sqrtf(x); }
-__device__ __forceinline__ double softplus(double x) { return fmax(x, 0.0) + log1p(exp(-fabs(x))); }
-
-// Ask the scheduler to issue the region as NM groups of (1 MFMA, NV VALU), so
-// that independent VALU work runs while each MFMA executes.
-template <int NM, int NV>
-__device__ __forceinline__ void interleave_mfma_valu() {
-#pragma unroll
-  for (int i = 0; i < NM; ++i) {
-    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-    __builtin_amdgcn_sched_group_barrier(0x002, NV, 0);
-  }
-}
-
-// phi: accumulator feature slot m = 32t + 8a + 4h + c  ->  feature 16(j/8) + 8h + j%8, j = 16t + 4a + c
-__device__ __forceinline__ int phi(int m) {
-  const int t = m >> 5, a = (m >> 3) & 3, h = (m >> 2) & 1, c = m & 3;
-  const int j = 16 * t + 4 * a + c;
-  return (j >> 3) * 16 + h * 8 + (j & 7);
-}
 
 // ---------------------------------------------------------------------------
 // Gram pass: S.G = X^T diag(lambda) X + I / prior_var for the first D
@@ -980,7 +930,7 @@ uint64_t logistic_pack_layout(int P, const int32_t* N_host, const int32_t* D_hos
   for (int p = 0; p < P; ++p) {
     PackedPost h;
     h.npad = (N_host[p] + 63) / 64 * 64;
-    h.nb = D_host[p] <= 32 ? 1 : 2;
+    h.nb = D_host[p] <= 32 ? 1 : (D_host[p] <= 64 ? 2 : 4);
     const uint64_t sz = (uint64_t)h.npad * 32 * h.nb;  // floats of one layout
     h.xg = (int64_t)off;
     off += sz;
@@ -998,7 +948,7 @@ uint64_t logistic_pack_layout(int P, const int32_t* N_host, const int32_t* D_hos
 hipError_t launch_logistic_pack(const double* X, const double* y, const int32_t* N, const int32_t* D,
                                 const int64_t* xoff, const int64_t* yoff, const PackedPost* hdr, float* packed,
                                 int P, int max_npad, hipStream_t stream) {
-  const int64_t elems = (int64_t)max_npad * 64;
+  const int64_t elems = (int64_t)max_npad * 128;  // npad x 32 nb elements per posterior, nb <= 4
   const dim3 grid((unsigned)((elems + 255) / 256), (unsigned)P);
   hipLaunchKernelGGL(lfast::logistic_pack_kernel, grid, dim3(256), 0, stream, X, y, N, D, xoff, yoff, hdr, packed);
   return hipGetLastError();
@@ -1006,16 +956,23 @@ hipError_t launch_logistic_pack(const double* X, const double* y, const int32_t*
 
 hipError_t launch_logistic_rmhmc_f32(const LogisticArgs& a, int max_features, hipStream_t stream) {
   const int blocks = (a.C + 7) / 8 * 8;
-  // one launch per tile width present: a single posterior needs only its own;
-  // a batch runs the narrow kernel, then (if any D > 32) the wide one
-  const bool narrow = a.P > 1 || max_features <= 32;
-  const bool wide = max_features > 32;
+  // one launch per packed width present: a single posterior needs only its
+  // own; a batch runs every width up to its widest posterior's (chains of
+  // other widths leave at once)
+  const bool nb1 = max_features <= 32 || a.P > 1;
+  const bool nb2 = max_features > 32 && (max_features <= 64 || a.P > 1);
+  const bool nb4 = max_features > 64;
   if (a.q_out != nullptr) {
-    if (narrow) hipLaunchKernelGGL((lfast::logistic_rmhmc_f32_kernel<1, true>), dim3(blocks), dim3(64), 0, stream, a);
-    if (wide) hipLaunchKernelGGL((lfast::logistic_rmhmc_f32_kernel<2, true>), dim3(blocks), dim3(64), 0, stream, a);
+    if (nb1) hipLaunchKernelGGL((lfast::logistic_rmhmc_f32_kernel<1, true>), dim3(blocks), dim3(64), 0, stream, a);
+    if (nb2) hipLaunchKernelGGL((lfast::logistic_rmhmc_f32_kernel<2, true>), dim3(blocks), dim3(64), 0, stream, a);
   } else {
-    if (narrow) hipLaunchKernelGGL((lfast::logistic_rmhmc_f32_kernel<1, false>), dim3(blocks), dim3(64), 0, stream, a);
-    if (wide) hipLaunchKernelGGL((lfast::logistic_rmhmc_f32_kernel<2, false>), dim3(blocks), dim3(64), 0, stream, a);
+    if (nb1) hipLaunchKernelGGL((lfast::logistic_rmhmc_f32_kernel<1, false>), dim3(blocks), dim3(64), 0, stream, a);
+    if (nb2) hipLaunchKernelGGL((lfast::logistic_rmhmc_f32_kernel<2, false>), dim3(blocks), dim3(64), 0, stream, a);
+  }
+  if (nb4) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_logistic_rmhmc_f32_wide(a, stream);
   }
   return hipGetLastError();
 }

@@ -227,7 +227,7 @@ typedef struct {
 typedef struct {
   int32_t num_posteriors;        /* P >= 1 */
   int32_t max_rows;              /* host copy of max_p N_p */
-  int32_t max_features;          /* host copy of max_p D_p (<= 64 on the float64 path) */
+  int32_t max_features;          /* host copy of max_p D_p (<= 128 for RMHMC, <= 64 for LMC) */
   int32_t reserved;
   const int32_t* num_rows;       /* [P] device: N_p */
   const int32_t* num_features;   /* [P] device: D_p */
@@ -249,8 +249,9 @@ typedef struct {
  * per product) and float32 accumulation; the metric inverse is a float32
  * sweep with one refinement step per solve; log det G is summed, and the
  * energies, energy difference, accept test and fixed-point norm are kept, in
- * float64.  D <= 64 per posterior.  The design matrices must first be packed
- * (once per data set) into a device table: */
+ * float64.  D <= 128 per posterior (one wave per chain up to 64 features,
+ * four waves above).  The design matrices must first be packed (once per
+ * data set) into a device table: */
 
 /* Bytes of the packed table for P posteriors of the given (host) sizes. */
 int gjx_logistic_pack_size(int32_t num_posteriors, const int32_t* num_rows_host, const int32_t* num_features_host,

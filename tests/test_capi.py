@@ -62,7 +62,7 @@ def _pack_size_numpy(N, D):
     P = len(N)
     off = -(-P * 40 // 256) * 64
     for n, d in zip(N, D):
-        npad, nb = -(-n // 64) * 64, 1 if d <= 32 else 2
+        npad, nb = -(-n // 64) * 64, 1 if d <= 32 else (2 if d <= 64 else 4)
         off += 3 * npad * 32 * nb + -(-npad // 64) * 64
     return off * 4
 
@@ -71,13 +71,13 @@ def test_logistic_pack_and_workspace_sizes():
     import ctypes
 
     lib = _lib.load()
-    for N, D in [([2048], [64]), ([200, 4096, 333], [8, 64, 33]), ([31], [1])]:
+    for N, D in [([2048], [64]), ([200, 4096, 333], [8, 64, 33]), ([31], [1]), ([500, 77], [128, 65])]:
         n = (ctypes.c_int32 * len(N))(*N)
         d = (ctypes.c_int32 * len(D))(*D)
         out = ctypes.c_uint64()
         assert lib.gjx_logistic_pack_size(len(N), n, d, ctypes.byref(out)) == 0
         assert out.value == _pack_size_numpy(N, D)
-    n, d = (ctypes.c_int32 * 1)(100), (ctypes.c_int32 * 1)(65)
+    n, d = (ctypes.c_int32 * 1)(100), (ctypes.c_int32 * 1)(129)
     assert lib.gjx_logistic_pack_size(1, n, d, ctypes.byref(out)) == -2
     p = _lib.LogisticParams(num_chains=7, dim=64, dtype=_lib.GJX_F32)
     data = _lib.LogisticData(num_posteriors=1, max_rows=2000, max_features=64)
@@ -103,7 +103,7 @@ def test_logistic_float32_requires_packed_table_and_workspace():
 
 
 def test_logistic_feature_capacity_per_path():
-    """float64 RMHMC takes up to 128 features, float32 RMHMC and LMC up to 64."""
+    """RMHMC (float64 and float32) takes up to 128 features, LMC up to 64."""
     lib = _lib.load()
 
     def step(fn, io, dtype, D):
@@ -116,7 +116,7 @@ def test_logistic_feature_capacity_per_path():
     rio, lio = _lib.RmhmcStepIO(), _lib.LmcmcStepIO()
     assert step(rm, rio, _lib.GJX_F64, 128) == -1 and "data arrays" in _lib.last_error()
     assert step(rm, rio, _lib.GJX_F64, 129) == -2
-    assert step(rm, rio, _lib.GJX_F32, 64) == -1
-    assert step(rm, rio, _lib.GJX_F32, 65) == -2
+    assert step(rm, rio, _lib.GJX_F32, 128) == -1
+    assert step(rm, rio, _lib.GJX_F32, 129) == -2
     assert step(lmc, lio, _lib.GJX_F64, 64) == -1
     assert step(lmc, lio, _lib.GJX_F64, 65) == -2

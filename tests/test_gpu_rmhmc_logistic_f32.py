@@ -99,8 +99,34 @@ def test_step_f32_matches_f64_kernel_2048x64():
         s32 = gj.rmhmc.init(s64n.position.float(), model)
 
 
-def test_batch_f32_matches_f64_kernel():
-    shapes = [(200, 8), (700, 24), (1500, 40), (2048, 64), (333, 32)]
+@pytest.mark.parametrize("N,D,C,seed", [
+    (500, 100, 4, 5),    # four 32-feature blocks with padding
+    (1024, 128, 4, 6),   # at the capacity of the four-wave kernel
+])
+def test_step_f32_wide_matches_f64_kernel(N, D, C, seed):
+    eps, L, max_iters = 0.05, 6, 6
+    X, y, _ = make_logistic(N, D, seed)
+    model = gj.LogisticRegression(X, y, 100.0)
+    q0 = (0.5 * seeded_normal(30 + seed, C, D)).to(DEV)
+    kern = build_kernel(functools.partial(implicit_midpoint, max_iters=max_iters))
+    s64 = gj.rmhmc.init(q0, model)
+    s32 = gj.rmhmc.init(q0.float(), model)
+    for t in range(2):
+        print(f"case N={N} D={D} C={C} transition {t}", flush=True)
+        z, u = seeded_normal(40 + 10 * seed + t, C, D).to(DEV), seeded_uniform(60 + 10 * seed + t, C).to(DEV)
+        s64n, i64 = kern(gr.HostNoise(z, u), s64, model, eps, gj.FisherMetric(), L)
+        s32n, i32 = kern(gr.HostNoise(z.float(), u), s32, model, eps, gj.FisherMetric(), L)
+        torch.cuda.synchronize()
+        _check_step(i32, s32n, i64, s64n, u, q0, f"{N}x{D} t={t}")
+        s64 = s64n
+        s32 = gj.rmhmc.init(s64n.position.float(), model)
+
+
+@pytest.mark.parametrize("shapes", [
+    [(200, 8), (700, 24), (1500, 40), (2048, 64), (333, 32)],
+    [(200, 8), (600, 128), (1500, 40), (400, 72), (333, 32), (2048, 100)],  # all three packed widths
+])
+def test_batch_f32_matches_f64_kernel(shapes):
     models = [gj.LogisticRegression(*make_logistic(n, d, 20 + i)[:2], 100.0) for i, (n, d) in enumerate(shapes)]
     batch = gj.LogisticBatch(models, 3)
     C, Dm, eps, L = batch.num_chains, batch.dim, 0.08, 6
@@ -127,10 +153,11 @@ def test_batch_f32_matches_f64_kernel():
         assert torch.all(s32n.position[c, d:] == 0)
 
 
-def test_sample_kernel_matches_step_kernel_f32():
+@pytest.mark.parametrize("N,D", [(512, 40), (384, 100)])
+def test_sample_kernel_matches_step_kernel_f32(N, D):
     from geomjax_amd.sampling import run_chains
 
-    N, D, C, eps, L, T = 512, 40, 24, 0.1, 5, 3
+    C, eps, L, T = 24, 0.1, 5, 3
     X, y, _ = make_logistic(N, D, 7)
     model = gj.LogisticRegression(X, y, 100.0)
     alg = gj.rmhmc(model, eps, gj.FisherMetric(), L, integrator=functools.partial(implicit_midpoint, max_iters=6))
@@ -168,7 +195,7 @@ def _layouts_numpy(X, y):
     import numpy as np
 
     N, D = X.shape
-    npad, nb = -(-N // 64) * 64, 1 if D <= 32 else 2
+    npad, nb = -(-N // 64) * 64, 1 if D <= 32 else (2 if D <= 64 else 4)
     Xp = np.zeros((npad, 32 * nb))
     Xp[:N, :D] = X
     lane = np.arange(64)
@@ -190,7 +217,7 @@ def _layouts_numpy(X, y):
 def test_pack_layout_matches_numpy_restatement():
     import numpy as np
 
-    shapes = [(70, 5), (64, 32), (333, 47)]
+    shapes = [(70, 5), (64, 32), (333, 47), (100, 90)]
     data = [make_logistic(n, d, 40 + i)[:2] for i, (n, d) in enumerate(shapes)]
     batch = gj.LogisticBatch([gj.LogisticRegression(X, y, 100.0) for X, y in data], 1)
     buf = batch._table.packed().cpu().numpy()
@@ -201,7 +228,8 @@ def test_pack_layout_matches_numpy_restatement():
     for p, (X, y) in enumerate(data):
         xg, xr, yf = _layouts_numpy(X, y)
         h = hdr[p]
-        assert h["npad"] == -(-X.shape[0] // 64) * 64 and h["nb"] == (1 if X.shape[1] <= 32 else 2)
+        d = X.shape[1]
+        assert h["npad"] == -(-X.shape[0] // 64) * 64 and h["nb"] == (1 if d <= 32 else (2 if d <= 64 else 4))
         np.testing.assert_array_equal(flt[h["xg"]:h["xg"] + xg.size], xg.astype(np.float32))
         np.testing.assert_array_equal(flt[h["xr"]:h["xr"] + xr.size], xr.astype(np.float32))
         np.testing.assert_array_equal(flt[h["yf"]:h["yf"] + yf.size], yf.astype(np.float32))
