@@ -215,9 +215,9 @@ def batch_min_ess(samples, counts, dims):
 def uneven_batch_forms(chains_per_posterior, K, W, dev, plain=True):
     """Tuned and plain RMHMC-Fisher on 256 logistic posteriors of uneven size in one launch.
 
-    Posteriors as the uneven-batch configuration generates them (N log-uniform in
-    [256, 4096], data seed 3) with D drawn from {8, 16, 32, 64}: the kernels serve
-    D <= 64 (a posterior with D = 128 needs a multi-wave chain tile)."""
+    Posteriors as the uneven-batch configuration generates them: D drawn from
+    {8, 16, 32, 64, 128}, N log-uniform in [256, 4096], data seed 3.  Chains of
+    posteriors up to 64 features run one wave each, wider ones four waves."""
     import torch
 
     import geomjax_amd as gj
@@ -225,7 +225,7 @@ def uneven_batch_forms(chains_per_posterior, K, W, dev, plain=True):
     from geomjax_amd.rmhmc import build_kernel, implicit_midpoint, sample
     from geomjax_amd.synthetic import make_uneven_batch
 
-    data = make_uneven_batch(P=256, seed=3, dims=(8, 16, 32, 64))
+    data = make_uneven_batch(P=256, seed=3, dims=(8, 16, 32, 64, 128))
     models = [gj.LogisticRegression(X, y, PRIOR, device=dev) for X, y, _ in data]
     batch = gj.LogisticBatch(models, chains_per_posterior)
     counts, dims = batch.counts, [m.D for m in models]
@@ -268,7 +268,7 @@ def uneven_batch_forms(chains_per_posterior, K, W, dev, plain=True):
              "fevals_per_chain_transition": float(torch.stack(fevs).double().sum().item()) / C / K}
     log(f"uneven batch tuned: {tuned['value']:.3e} leapfrog-steps/s, {tuned['ms_per_step']:.2f} ms/step, "
         f"min-ESS/s {tuned['min_ess_per_sec']:.3e}")
-    res = {"posteriors": len(models), "chains": C, "dims": "D in {8, 16, 32, 64}, N in [256, 4096]",
+    res = {"posteriors": len(models), "chains": C, "dims": "D in {8, 16, 32, 64, 128}, N in [256, 4096]",
            "tuned": tuned, "plain": None}
     if plain:
         kern = build_kernel(integ)

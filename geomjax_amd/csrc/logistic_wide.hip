@@ -16,7 +16,7 @@
 // four 32-feature blocks (eight 16-feature chunks).
 //   Gram pass: the 10 lower 32 x 32 tiles are split over the waves (3, 3, 2,
 //     2), each wave streaming only the feature blocks its tiles touch.
-//   Sweep: lane l holds rows [64 (l/128), +64) of column l % 128; one barrier
+//   Sweep: each lane holds an 8 x 8 block of G (16 x 16 lanes); one barrier
 //     per pivot.  Its Schur-complement columns give the momentum draw.
 //   Field pass: wave w computes the 32 output features phi(32 w + .) of
 //     V^T = G^-1 X^T for each 32-row block, whose accumulator slots pair with
@@ -181,61 +181,76 @@ __device__ __forceinline__ void gram_pass(Smem& S, const float* __restrict__ Xg,
 }
 
 // ---------------------------------------------------------------------------
-// Sweep of S.G (left intact) into registers: A = -G^-1, pivots in S.dk.
-// Lane l holds A[64 hh + 32 b + i][jl] in A[b][i] (jl = l % 128, hh = l / 128).
-// Row k of the matrix is column k by symmetry: the lanes of half k / 64 hold
-// it in register k % 64.  With want_chol, p0j = (L z)_jl, L = chol(G).
+// Sweep of S.G (left intact): lane t holds the 8 x 8 block of rows
+// [8 (t/16), +8) and columns [8 (t%16), +8) in registers.  Pivots run in
+// groups of 8 with the pivot inside a group unrolled, so row k = 8 kk + a0 is
+// register row a0 of the 16 lanes t / 16 == kk, which broadcast it through
+// LDS (one barrier per pivot); by symmetry it is also column k.  On return
+// S.Gi = G^-1, S.dk the pivots; with want_chol S.p0 = S.p = chol(G) S.z, from
+// the Schur-complement column (row k before its elimination) of each pivot.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float col_get(const f32x32 (&A)[2], int k) {
-  float v;
-  if (k < 32)
-    v = A[0][k];
-  else
-    v = A[1][k - 32];
-  return v;
-}
-__device__ __forceinline__ void col_set(f32x32 (&A)[2], int k, float v) {
-  if (k < 32)
-    A[0][k] = v;
-  else
-    A[1][k - 32] = v;
-}
-
-__device__ __forceinline__ void sweep(Smem& S, f32x32 (&A)[2], int D, bool want_chol, float& p0j) {
-  const int l = threadIdx.x, jl = l % MAXD, hh = l / MAXD;
+__device__ __forceinline__ void sweep(Smem& S, int D, bool want_chol) {
+  const int t = threadIdx.x;
+  const int i0 = 8 * (t / 16), j0 = 8 * (t % 16);
+  float A[8][8];
 #pragma unroll
-  for (int b = 0; b < 2; ++b)
+  for (int a = 0; a < 8; ++a) {
+    float v[8];
+    load8(S.G + (i0 + a) * MAXD + j0, v);
 #pragma unroll
-    for (int i = 0; i < 32; ++i) A[b][i] = S.G[(64 * hh + 32 * b + i) * MAXD + jl];
-  p0j = 0.0f;
-  for (int k = 0; k < D; ++k) {
-    const int kh = k / 64, kr = k % 64;
-    float* bc = S.bc[k & 1];
-    if (hh == kh) bc[jl] = col_get(A, kr);  // A[k][jl] = A[jl][k]
-    __syncthreads();
-    float col[64];
-#pragma unroll
-    for (int i = 0; i < 64; i += 4) {
-      const float4 v = *reinterpret_cast<const float4*>(bc + 64 * hh + i);
-      col[i] = v.x; col[i + 1] = v.y; col[i + 2] = v.z; col[i + 3] = v.w;
-    }
-    const float d = bc[k];
-    const float ak = bc[jl];
-    const float rd = __builtin_amdgcn_rcpf(d);  // 1 ulp, as the float32 sweep itself
-    if (want_chol && hh == 0 && jl >= k) p0j = fmaf(ak / sqrtf(d), S.z[k], p0j);
-    // column k's own lanes hold col: f = 1 - 1/d turns their update into A[i][k] / d
-    const float f = (jl == k) ? 1.0f - rd : ak * rd;
-#pragma unroll
-    for (int b = 0; b < 2; ++b)
-#pragma unroll
-      for (int i = 0; i < 32; ++i) A[b][i] = fmaf(-col[32 * b + i], f, A[b][i]);
-    if (hh == kh) col_set(A, kr, (jl == k) ? -rd : ak * rd);
-    if (l == 0) S.dk[k] = d;
+    for (int b = 0; b < 8; ++b) A[a][b] = v[b];
   }
-  // -G^-1 on the padding block is -I
-  for (int k = D; k < MAXD; ++k)
-    if (hh == k / 64) col_set(A, k % 64, jl == k ? -1.0f : 0.0f);
-  if (l < MAXD && l >= D) S.dk[l] = 1.0f;
+  float pacc = 0.0f;
+  for (int kk = 0; kk * 8 < D; ++kk) {
+#pragma unroll
+    for (int a0 = 0; a0 < 8; ++a0) {
+      const int k = kk * 8 + a0;
+      if (k >= D) break;
+      float* buf = S.bc[k & 1];
+      if (t / 16 == kk) {
+        *reinterpret_cast<float4*>(buf + j0) = float4{A[a0][0], A[a0][1], A[a0][2], A[a0][3]};
+        *reinterpret_cast<float4*>(buf + j0 + 4) = float4{A[a0][4], A[a0][5], A[a0][6], A[a0][7]};
+      }
+      __syncthreads();
+      const float d = buf[k];
+      const float rd = __builtin_amdgcn_rcpf(d);  // 1 ulp, as the float32 sweep itself
+      if (want_chol && t >= k && t < D) pacc = fmaf(buf[t] / sqrtf(d), S.z[k], pacc);
+      float col[8], bjr[8];
+      load8(buf + i0, col);  // A[i][k] = A[k][i]
+      load8(buf + j0, bjr);
+#pragma unroll
+      for (int b = 0; b < 8; ++b) bjr[b] *= rd;
+      // a_ij - a_ik a_kj / d everywhere, then the pivot row and column (register
+      // row / column a0 of the lanes that hold them) become a_kj / d and
+      // a_ik / d, and the pivot -1/d
+#pragma unroll
+      for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int b = 0; b < 8; ++b) A[a][b] = fmaf(-col[a], bjr[b], A[a][b]);
+      const bool prow = t / 16 == kk, pcol = t % 16 == kk;
+#pragma unroll
+      for (int b = 0; b < 8; ++b) A[a0][b] = prow ? bjr[b] : A[a0][b];
+#pragma unroll
+      for (int a = 0; a < 8; ++a) A[a][a0] = pcol ? col[a] * rd : A[a][a0];
+      if (prow && pcol) A[a0][a0] = -rd;
+      if (t == 0) S.dk[k] = d;
+    }
+  }
+  // G^-1 = -A; the padding block of G is I, so its inverse is I
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    float v[8];
+#pragma unroll
+    for (int b = 0; b < 8; ++b) v[b] = (i0 + a == j0 + b && i0 + a >= D) ? 1.0f : -A[a][b];
+    *reinterpret_cast<float4*>(S.Gi + (i0 + a) * LDG + j0) = float4{v[0], v[1], v[2], v[3]};
+    *reinterpret_cast<float4*>(S.Gi + (i0 + a) * LDG + j0 + 4) = float4{v[4], v[5], v[6], v[7]};
+  }
+  if (t < MAXD && t >= D) S.dk[t] = 1.0f;
+  if (want_chol && t < MAXD) {
+    const float pv = t < D ? pacc : 0.0f;
+    S.p0[t] = pv;
+    S.p[t] = pv;
+  }
   __syncthreads();
 }
 
@@ -245,60 +260,34 @@ __device__ __forceinline__ double logdet_from_pivots(Smem& S, int D) {
   return block_sum(S, l < D ? log((double)S.dk[l]) : 0.0);
 }
 
-// out[jl] = sum over the 128 rows of -A[i][jl] v[i]: each half of the lanes
-// sums its 64 rows, the halves meet in LDS.  Returns the total on lanes < 128.
-__device__ __forceinline__ float neg_a_times(Smem& S, const f32x32 (&A)[2], const float* v) {
+// (M v)_j for symmetric M (row stride ld) on lanes j < 128: each half of the
+// lanes sums 64 rows of column j, the halves meet in LDS.
+__device__ __forceinline__ float sym_times(Smem& S, const float* M, int ld, const float* v) {
   const int l = threadIdx.x, jl = l % MAXD, hh = l / MAXD;
   float s = 0.0f;
-#pragma unroll
-  for (int b = 0; b < 2; ++b)
-#pragma unroll
-    for (int i = 0; i < 32; ++i) s = fmaf(-A[b][i], v[64 * hh + 32 * b + i], s);
+#pragma unroll 16
+  for (int i = 0; i < 64; ++i) s = fmaf(M[(64 * hh + i) * ld + jl], v[64 * hh + i], s);
   S.half[hh][jl] = s;
   __syncthreads();
-  const float t = l < MAXD ? S.half[0][l] + S.half[1][l] : 0.0f;
+  const float r = l < MAXD ? S.half[0][l] + S.half[1][l] : 0.0f;
   __syncthreads();
-  return t;
+  return r;
 }
 
-// S.u = G^-1 S.p with one refinement step against the Gram still in S.G.
-__device__ __forceinline__ void ginv_apply(Smem& S, const f32x32 (&A)[2]) {
-  const int l = threadIdx.x, jl = l % MAXD, hh = l / MAXD;
-  const float u0 = neg_a_times(S, A, S.p);
+// G (in S.G) -> pivots in S.dk, S.Gi = G^-1, S.u = G^-1 S.p with one
+// refinement step against G.  With want_chol the momentum p0 = chol(G) S.z is
+// drawn first into S.p0 and S.p.
+__device__ __forceinline__ void invert(Smem& S, int D, bool want_chol) {
+  const int l = threadIdx.x;
+  sweep(S, D, want_chol);
+  const float u0 = sym_times(S, S.Gi, LDG, S.p);
   if (l < MAXD) S.u[l] = u0;
   __syncthreads();
-  float gu = 0.0f;
-#pragma unroll 16
-  for (int i = 0; i < 64; ++i) gu = fmaf(S.G[(64 * hh + i) * MAXD + jl], S.u[64 * hh + i], gu);
-  S.half[hh][jl] = gu;
+  const float gu = sym_times(S, S.G, MAXD, S.u);
+  if (l < MAXD) S.dH[l] = S.p[l] - gu;  // residual (dH is free here)
   __syncthreads();
-  if (l < MAXD) S.dH[l] = S.p[l] - (S.half[0][l] + S.half[1][l]);  // residual (dH is free here)
-  __syncthreads();
-  const float du = neg_a_times(S, A, S.dH);
+  const float du = sym_times(S, S.Gi, LDG, S.dH);
   if (l < MAXD) S.u[l] = u0 + du;
-  __syncthreads();
-}
-
-// G (in S.G) -> pivots in S.dk, S.Gi = G^-1, S.u = G^-1 S.p (refined).  With
-// want_chol the momentum p0 = chol(G) S.z is drawn first into S.p0 and S.p.
-__device__ __forceinline__ void invert(Smem& S, int D, bool want_chol) {
-  const int l = threadIdx.x, jl = l % MAXD, hh = l / MAXD;
-  f32x32 A[2];
-  float p0j;
-  sweep(S, A, D, want_chol, p0j);
-  if (want_chol) {
-    if (l < MAXD) {
-      const float pv = l < D ? p0j : 0.0f;
-      S.p0[l] = pv;
-      S.p[l] = pv;
-    }
-    __syncthreads();
-  }
-  ginv_apply(S, A);
-#pragma unroll
-  for (int b = 0; b < 2; ++b)
-#pragma unroll
-    for (int i = 0; i < 32; ++i) S.Gi[(64 * hh + 32 * b + i) * LDG + jl] = -A[b][i];
   __syncthreads();
 }
 
@@ -333,11 +322,6 @@ __device__ __forceinline__ void field_pass(Smem& S, const float* __restrict__ Xr
   const uint4* sp = reinterpret_cast<const uint4*>(Xs) + (size_t)l * 2;
   const int nrb = npad / 32;
   float* const e0_dst = explicit_step ? E0 : E1;
-  // this wave's u chunks, in registers for the whole pass
-  float u8[2][8];
-#pragma unroll
-  for (int c = 0; c < 2; ++c) load8(S.u + (2 * w + c) * 16 + h * 8, u8[c]);
-
   bf16x8 xh[KC], xl[KC];
   float xa[2][8];
   float eta = 0.0f, base = 0.0f, yv = 0.0f;
@@ -359,15 +343,19 @@ __device__ __forceinline__ void field_pass(Smem& S, const float* __restrict__ Xr
   };
   fetch(0);
   for (int rb = 0; rb < nrb; ++rb) {
+    asm volatile("" ::: "memory");  // u is re-read from LDS per block (registers are scarce here)
     const int par = rb & 1;
     f32x16 acc = f32x16{0};
 #pragma unroll
     for (int kc = 0; kc < KC; ++kc) acc = mfma3(Ah[kc], Al[kc], xh[kc], xl[kc], acc);
     float bp = 0.0f;
 #pragma unroll
-    for (int c = 0; c < 2; ++c)
+    for (int c = 0; c < 2; ++c) {
+      float u8[8];
+      load8(S.u + (2 * w + c) * 16 + h * 8, u8);
 #pragma unroll
-      for (int i = 0; i < 8; ++i) bp = fmaf(xa[c][i], u8[c][i], bp);
+      for (int i = 0; i < 8; ++i) bp = fmaf(xa[c][i], u8[i], bp);
+    }
     interleave_mfma_valu<3 * KC, 2>();
     float ap = 0.0f;
 #pragma unroll
@@ -443,14 +431,18 @@ __device__ __forceinline__ double fresh_pass(Smem& S, const float* __restrict__ 
   const float* xp = Xr + (size_t)l * 8;
   const int nrb = npad / 32;
   for (int rb = w; rb < nrb; rb += NW) {
+    asm volatile("" ::: "memory");  // q is re-read from LDS per block (hoisting it costs 64 registers)
     float x[KC][8];
 #pragma unroll
     for (int kc = 0; kc < KC; ++kc) load8(xp + (size_t)(rb * KC + kc) * 512, x[kc]);
     double ep = 0.0;
 #pragma unroll
-    for (int kc = 0; kc < KC; ++kc)
+    for (int kc = 0; kc < KC; ++kc) {
+      float q8[8];
+      load8(S.q + kc * 16 + h * 8, q8);
 #pragma unroll
-      for (int i = 0; i < 8; ++i) ep = fma((double)x[kc][i], (double)S.q[kc * 16 + h * 8 + i], ep);
+      for (int i = 0; i < 8; ++i) ep = fma((double)x[kc][i], (double)q8[i], ep);
+    }
     const double eta = ep + __shfl_xor(ep, 32, 64);
     const int row = rb * 32 + (l % 32);
     const double s = 1.0 / (1.0 + exp(-eta));
