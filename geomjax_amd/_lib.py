@@ -161,6 +161,8 @@ _SIGNATURES = {
                                        ctypes.POINTER(LmcmcSampleIO), _vp],
     "gjx_logistic_value_and_grad": [ctypes.POINTER(LogisticParams), ctypes.POINTER(LogisticData), _vp, _vp, _vp,
                                     _vp],
+    "gjx_rmhmc_fisher_logistic_tangent": [ctypes.POINTER(LogisticParams), ctypes.POINTER(LogisticData),
+                                          ctypes.POINTER(TangentIO), _vp],
     "gjx_logistic_pack_size": [_c_int32, _vp, _vp, ctypes.POINTER(ctypes.c_uint64)],
     "gjx_logistic_pack": [ctypes.POINTER(LogisticData), _vp, _vp, _vp, ctypes.c_uint64, _vp],
     "gjx_logistic_workspace_size": [ctypes.POINTER(LogisticParams), ctypes.POINTER(LogisticData),

@@ -1,4 +1,4 @@
-"""Gradients for tuning the step size and alpha2 of RMHMC-Monge by gradient.
+"""Gradients for tuning the step size (and alpha2 of the Monge metric) of RMHMC by gradient.
 
 The quantity tuned is the expected squared jump distance times acceptance,
 mean over chains of  p_c * |q_L,c - q_0,c|^2  with p_c = min(1, e^dH_c), the
@@ -14,6 +14,10 @@ parameters forward tangents cost about two primal transitions and keep no
 trajectory in memory, so no checkpointing is needed; the momentum is
 reparameterised as chol(G(q0; alpha2)) z with z held fixed.
 
+Two kernels: RMHMC-Monge on Neal's funnel (derivatives in log step size and
+log alpha2) and RMHMC-Fisher on logistic regression (log step size; the
+Fisher metric has no alpha2, its derivative column is zero).
+
 ``ESJDAcceptance`` exposes it as a ``torch.autograd.Function`` of
 (log_step_size, log_alpha2); ``adapt`` runs a small Adam loop on them.
 """
@@ -23,8 +27,8 @@ from typing import NamedTuple
 import torch
 
 from . import _lib
-from .metrics import MongeMetric
-from .models import NealFunnel
+from .metrics import FisherMetric, MongeMetric
+from .models import LogisticBatch, LogisticRegression, NealFunnel, logistic_params
 from .random import HostNoise, PRNGKey
 from .rmhmc import _params, _solver_kwargs, implicit_midpoint
 
@@ -41,9 +45,14 @@ class TangentResult(NamedTuple):
 
 def esjd_acceptance(rng_key, state, logdensity_fn, step_size, metric_fn, num_integration_steps, *,
                     integrator=implicit_midpoint, lanes_per_chain=0) -> TangentResult:
-    """Per-chain objective and its derivatives w.r.t. (log step_size, log alpha2)."""
-    if not isinstance(logdensity_fn, NealFunnel) or not isinstance(metric_fn, MongeMetric):
-        raise NotImplementedError("the tangent kernel serves RMHMC with the Monge metric on the funnel")
+    """Per-chain objective and its derivatives w.r.t. (log step_size, log alpha2).
+
+    Funnel + MongeMetric: both columns.  LogisticRegression / LogisticBatch +
+    FisherMetric: column 0 (column 1 is zero)."""
+    funnel = isinstance(logdensity_fn, NealFunnel) and isinstance(metric_fn, MongeMetric)
+    logistic = isinstance(logdensity_fn, (LogisticRegression, LogisticBatch)) and isinstance(metric_fn, FisherMetric)
+    if not (funnel or logistic):
+        raise NotImplementedError("tangent kernels: RMHMC-Monge on the funnel, RMHMC-Fisher on logistic regression")
     solver = _solver_kwargs(integrator)
     q = state.position
     if q.dim() != 2 or q.dtype != torch.float64 or not q.is_cuda:
@@ -66,11 +75,25 @@ def esjd_acceptance(rng_key, state, logdensity_fn, step_size, metric_fn, num_int
                         sq_jump=out.sq_jump.data_ptr(), d_objective=out.d_objective.data_ptr(),
                         d_acceptance=out.d_acceptance.data_ptr(), d_sq_jump=out.d_sq_jump.data_ptr(),
                         num_fevals=out.num_fevals.data_ptr())
-    params = _params(C, D, _lib.GJX_F64, step_size, num_integration_steps, solver, metric_fn.alpha2,
-                     logdensity_fn.sigma, 1000.0, lanes_per_chain=lanes_per_chain, seed=seed, counter=counter)
-    rc = _lib.load().gjx_rmhmc_monge_funnel_tangent(params, io, _lib.stream_handle(dev))
-    _lib.check(rc, "gjx_rmhmc_monge_funnel_tangent")
+    if funnel:
+        params = _params(C, D, _lib.GJX_F64, step_size, num_integration_steps, solver, metric_fn.alpha2,
+                         logdensity_fn.sigma, 1000.0, lanes_per_chain=lanes_per_chain, seed=seed, counter=counter)
+        rc = _lib.load().gjx_rmhmc_monge_funnel_tangent(params, io, _lib.stream_handle(dev))
+        _lib.check(rc, "gjx_rmhmc_monge_funnel_tangent")
+    else:
+        if D != logdensity_fn.dim:
+            raise ValueError(f"positions have {D} coordinates, the model {logdensity_fn.dim}")
+        params = logistic_params(logdensity_fn, C, L=num_integration_steps, step_size=step_size,
+                                 max_iters=solver["max_iters"], convergence_tol=solver["convergence_tol"],
+                                 divergence_tol=solver["divergence_tol"], seed=seed, counter=counter)
+        data = logdensity_fn.data(dtype=torch.float64, num_chains=C)
+        rc = _lib.load().gjx_rmhmc_fisher_logistic_tangent(params, data, io, _lib.stream_handle(dev))
+        _lib.check(rc, "gjx_rmhmc_fisher_logistic_tangent")
     return out
+
+
+def _metric_for(logdensity_fn, alpha2):
+    return FisherMetric() if isinstance(logdensity_fn, (LogisticRegression, LogisticBatch)) else MongeMetric(alpha2)
 
 
 class ESJDAcceptance(torch.autograd.Function):
@@ -86,8 +109,8 @@ class ESJDAcceptance(torch.autograd.Function):
                 integrator=implicit_midpoint):
         eps = math.exp(float(log_step_size))
         a2 = math.exp(float(log_alpha2))
-        r = esjd_acceptance(rng_key, state, logdensity_fn, eps, MongeMetric(a2), num_integration_steps,
-                            integrator=integrator)
+        r = esjd_acceptance(rng_key, state, logdensity_fn, eps, _metric_for(logdensity_fn, a2),
+                            num_integration_steps, integrator=integrator)
         grad = r.d_objective.mean(dim=0)
         ctx.save_for_backward(grad.to(log_step_size.device))
         return r.objective.mean().to(log_step_size.device)
@@ -117,8 +140,8 @@ def adapt(rng_key, state, logdensity_fn, num_integration_steps, *, step_size=0.1
     key = rng_key
     for it in range(1, num_steps + 1):
         eps, a2 = math.exp(theta[0]), math.exp(theta[1])
-        r = esjd_acceptance(PRNGKey(key.seed, key.counter + it), state, logdensity_fn, eps, MongeMetric(a2),
-                            num_integration_steps, integrator=integrator)
+        r = esjd_acceptance(PRNGKey(key.seed, key.counter + it), state, logdensity_fn, eps,
+                            _metric_for(logdensity_fn, a2), num_integration_steps, integrator=integrator)
         g = r.d_objective.mean(dim=0).cpu()
         if not torch.all(torch.isfinite(g)):
             g = torch.zeros(2, dtype=torch.float64)
@@ -126,6 +149,6 @@ def adapt(rng_key, state, logdensity_fn, num_integration_steps, *, step_size=0.1
         v = b2 * v + (1 - b2) * g * g
         theta = theta + learning_rate * (m / (1 - b1 ** it)) / (torch.sqrt(v / (1 - b2 ** it)) + tiny)
         hist.append((eps, a2, float(r.objective.mean()), float(r.acceptance.mean())))
-        alg = rmhmc(logdensity_fn, eps, MongeMetric(a2), num_integration_steps, integrator=integrator)
+        alg = rmhmc(logdensity_fn, eps, _metric_for(logdensity_fn, a2), num_integration_steps, integrator=integrator)
         state, _ = alg.step(PRNGKey(key.seed ^ 0x5DEECE66D, key.counter + it), state)
     return math.exp(theta[0]), math.exp(theta[1]), state, hist

@@ -529,6 +529,28 @@ int gjx_logistic_value_and_grad(const gjx_logistic_params* params, const gjx_log
   return GJX_OK;
 }
 
+int gjx_rmhmc_fisher_logistic_tangent(const gjx_logistic_params* params, const gjx_logistic_data* data,
+                                      const gjx_tangent_io* io, void* stream) {
+  if (int rc0 = require_f64(params, "gjx_rmhmc_fisher_logistic_tangent")) return rc0;
+  gjx::LogisticArgs a;
+  int rc = check_logistic(params, data, false, &a, gjx::kLogisticMaxDLmc);
+  if (rc != GJX_OK) return rc;
+  GJX_REQUIRE(params->step_size > 0.0, "step_size must be > 0 (log step_size is the parameter)");
+  GJX_REQUIRE(io != nullptr, "io is NULL");
+  GJX_REQUIRE(io->position && io->logdensity, "position and logdensity are required");
+  GJX_REQUIRE(io->objective && io->acceptance && io->sq_jump && io->d_objective && io->d_acceptance &&
+                  io->d_sq_jump,
+              "every output except num_fevals is required");
+  a.q_in = io->position;
+  a.logp_in = io->logdensity;
+  a.z = io->z;
+  gjx::TangentOut o{io->objective, io->acceptance, io->sq_jump, io->d_objective, io->d_acceptance,
+                    io->d_sq_jump, io->num_fevals};
+  hipError_t e = gjx::launch_logistic_rmhmc_tangent(a, o, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "gjx_rmhmc_fisher_logistic_tangent launch");
+  return GJX_OK;
+}
+
 int gjx_logistic_pack_size(int32_t num_posteriors, const int32_t* num_rows_host, const int32_t* num_features_host,
                            uint64_t* bytes) {
   GJX_REQUIRE(num_posteriors >= 1, "num_posteriors must be >= 1");

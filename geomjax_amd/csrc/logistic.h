@@ -107,7 +107,19 @@ __device__ __forceinline__ void zero_row_tails(const LogisticArgs& a, int c, int
 constexpr int kLogisticMaxDLmc = 64;   // LMC kernel
 constexpr int kLogisticMaxD = 128;     // RMHMC kernels (float64 and float32), value_and_grad, packing
 
+// per-chain outputs of the tangent kernel ([C] values, [C, 2] derivatives)
+struct TangentOut {
+  double* objective;    // acceptance * squared jump
+  double* acceptance;   // min(1, e^dH)
+  double* sq_jump;      // |q_L - q_0|^2
+  double* d_objective;
+  double* d_acceptance;
+  double* d_sq_jump;
+  int64_t* num_fevals;
+};
+
 hipError_t launch_logistic_rmhmc_f64(const LogisticArgs& a, hipStream_t stream);
+hipError_t launch_logistic_rmhmc_tangent(const LogisticArgs& a, const TangentOut& o, hipStream_t stream);
 hipError_t launch_logistic_value_and_grad(const LogisticArgs& a, hipStream_t stream);
 hipError_t launch_logistic_lmc_f64(const LogisticArgs& a, hipStream_t stream);
 hipError_t launch_logistic_rmhmc_f32(const LogisticArgs& a, int max_features, hipStream_t stream);

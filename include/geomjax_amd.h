@@ -364,6 +364,17 @@ typedef struct {
 
 int gjx_rmhmc_monge_funnel_tangent(const gjx_funnel_params* params, const gjx_tangent_io* io, void* stream);
 
+/* Derivatives for step-size tuning of RMHMC with the Fisher metric (the
+ * quantities of gjx_rmhmc_monge_funnel_tangent; the analogue in the
+ * reference is jax.jacfwd through its integrator,
+ * geomjax/adaptation/chees_adaptation_lmcmonge.py:497-519).  One transition
+ * per chain from io->position with momentum chol(G(q0)) z; column 0 of the
+ * [C, 2] derivative arrays is d/d log step_size, column 1 is zero (the
+ * Fisher metric has no alpha2).  float64, D <= 64, io->z as for the step
+ * entry point (NULL: device PRNG at params->counter). */
+int gjx_rmhmc_fisher_logistic_tangent(const gjx_logistic_params* params, const gjx_logistic_data* data,
+                                      const gjx_tangent_io* io, void* stream);
+
 /* Log density and its gradient of Neal's funnel for C positions: the
  * jax.value_and_grad(logdensity_fn)(position) of the reference's init
  * (geomjax/rmhmc/rmhmc.py:96-98, geomjax/lmcmonge/lmc.py:101-109) with the

@@ -91,3 +91,89 @@ def test_autograd_function():
     mean = r.d_objective.mean(0).cpu()
     assert abs(le.grad.item() - mean[0].item()) <= 1e-12 * max(1.0, abs(mean[0].item()))
     assert abs(la.grad.item() - mean[1].item()) <= 1e-12 * max(1.0, abs(mean[1].item()))
+
+
+def oracle_logistic_objective_grad(X, y, q0, z, eps, L, max_iters):
+    le = torch.tensor(math.log(eps), dtype=torch.float64, requires_grad=True)
+    M = om.LogisticRegression(X, y, 100.0)
+    k = orm.build_kernel(max_iters=max_iters)
+    u = torch.zeros(q0.shape[0], dtype=torch.float64)
+    _, info = k((z, u), orm.init(q0, M.logp), M.logp, torch.exp(le), M.fisher_metric_fn, L)
+    jump = ((info.proposal.state.position - q0) ** 2).sum(-1)
+    obj = info.acceptance_rate * jump
+    g = [torch.autograd.grad(o, (le,), retain_graph=True)[0] for o in obj]
+    return obj.detach(), torch.stack(g)
+
+
+@pytest.mark.parametrize("N,D,C,eps,L,max_iters,seed", [
+    (256, 8, 3, 0.1, 6, 6, 0),
+    (300, 13, 2, 0.15, 5, 6, 1),    # ragged N and D
+    (1024, 64, 2, 0.1, 4, 6, 2),    # at the tangent kernel's capacity
+])
+def test_logistic_tangent_matches_autograd_through_oracle(N, D, C, eps, L, max_iters, seed):
+    from geomjax_amd.synthetic import make_logistic
+
+    print(f"case N={N} D={D} C={C} eps={eps} L={L}", flush=True)
+    X, y, _ = make_logistic(N, D, seed)
+    q0, z = seeded_normal(120 + seed, C, D) * 0.5, seeded_normal(121 + seed, C, D)
+    obj_o, g_o = oracle_logistic_objective_grad(X, y, q0, z, eps, L, max_iters)
+    model = gj.LogisticRegression(X, y, 100.0)
+    s = gj.rmhmc.init(q0.to(DEV), model)
+    r = esjd_acceptance(gr.HostNoise(z.to(DEV), torch.zeros(C, device=DEV)), s, model, eps, gj.FisherMetric(), L,
+                        integrator=functools.partial(implicit_midpoint, max_iters=max_iters))
+    torch.cuda.synchronize()
+    print("  objective", r.objective.cpu().tolist(), "d/dlog eps", r.d_objective[:, 0].cpu().tolist(),
+          "oracle", g_o.tolist(), flush=True)
+    assert rel_err(r.objective, obj_o) < 1e-10
+    assert rel_err(r.d_objective[:, 0], g_o) < 1e-8
+    assert float(r.d_objective[:, 1].abs().max()) == 0.0
+    assert rel_err(r.d_objective[:, 0], r.d_acceptance[:, 0] * r.sq_jump + r.acceptance * r.d_sq_jump[:, 0]) < 1e-12
+
+
+def test_logistic_tangent_matches_central_differences():
+    from geomjax_amd.synthetic import make_logistic
+
+    N, D, C, eps, L, max_iters, h = 512, 24, 4, 0.12, 6, 5, 1e-5
+    X, y, _ = make_logistic(N, D, 5)
+    q0, z = seeded_normal(130, C, D) * 0.5, seeded_normal(131, C, D)
+    model = gj.LogisticRegression(X, y, 100.0)
+    integ = functools.partial(implicit_midpoint, max_iters=max_iters, convergence_tol=0.0)
+    s = gj.rmhmc.init(q0.to(DEV), model)
+    noise = gr.HostNoise(z.to(DEV), torch.zeros(C, device=DEV, dtype=torch.float64))
+    r = esjd_acceptance(noise, s, model, eps, gj.FisherMetric(), L, integrator=integ)
+    kern = build_kernel(integ)
+
+    def obj(le):
+        _, info = kern(noise, s, model, math.exp(le), gj.FisherMetric(), L)
+        jump = ((info.proposal.state.position - s.position) ** 2).sum(-1)
+        return (info.acceptance_rate * jump).double()
+
+    le = math.log(eps)
+    fd = (obj(le + h) - obj(le - h)) / (2 * h)
+    torch.cuda.synchronize()
+    print("tangent", r.d_objective[:, 0].cpu().tolist(), "fd", fd.cpu().tolist())
+    assert rel_err(r.objective, obj(le)) < 1e-12
+    assert rel_err(r.d_objective[:, 0], fd) < 1e-6
+
+
+def test_logistic_autograd_function_and_adapt():
+    from geomjax_amd.adjoint import adapt
+    from geomjax_amd.synthetic import make_logistic
+
+    N, D, C = 512, 16, 32
+    X, y, _ = make_logistic(N, D, 6)
+    model = gj.LogisticRegression(X, y, 100.0)
+    s = gj.rmhmc.init(seeded_normal(140, C, D).to(DEV) * 0.3, model)
+    key = gr.key(4)
+    le = torch.tensor(math.log(0.1), dtype=torch.float64, requires_grad=True)
+    la = torch.tensor(0.0, dtype=torch.float64, requires_grad=True)
+    val = ESJDAcceptance.apply(le, la, s, key, model, 6)
+    val.backward()
+    r = esjd_acceptance(key, s, model, 0.1, gj.FisherMetric(), 6)
+    mean = r.d_objective.mean(0).cpu()
+    assert abs(le.grad.item() - mean[0].item()) <= 1e-12 * max(1.0, abs(mean[0].item()))
+    assert la.grad.item() == 0.0
+    eps, a2, s2, hist = adapt(key, s, model, 6, step_size=0.02, num_steps=15, learning_rate=0.1)
+    print("adapted step size", eps, "history", [round(h[2], 4) for h in hist])
+    assert a2 == 1.0 and math.isfinite(eps) and eps > 0.02
+    assert torch.isfinite(s2.position).all()
