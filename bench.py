@@ -22,8 +22,9 @@ A "step" is one transition of all 4096 chains (10 integrator steps each).
 resident in HBM).  min-ESS/s uses the multi-chain ESS of
 geomjax/diagnostics.py:78-209 over the timed samples.  Extra fields report the
 same comparison at 1024 chains, for 256 logistic posteriors of uneven size in
-one launch (8192 and 4096 chains), and for Neal's funnel D=100 (RMHMC with the
-Monge metric, 8192 chains).  The CPU baseline is the float64 oracle
+one launch (8192 and 4096 chains), for Neal's funnel D=100 (RMHMC with the
+Monge metric, 8192 chains), and the cost of the tuning derivatives (tangent
+launch vs primal transition).  The CPU baseline is the float64 oracle
 restatement (dense metric + autodiff, as the reference) timed on this host's
 cores on a bounded sample of the headline workload.
 
@@ -53,6 +54,24 @@ FP32_PEAK = 157.3e12
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "logistic_rmhmc_f32_kernel_logistic_2048x64_4096chains.md")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the headline kernel, as measured by the FETCH_SIZE and
+    WRITE_SIZE counters (calibrated on a copy kernel) in the committed profile summary;
+    None if the summary is absent.  The counters need a profiler run of their own, so
+    this figure is not re-measured by a plain bench run."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def mfma_flops_logistic(num_fevals_total):
@@ -370,6 +389,51 @@ def funnel_forms(K, W, dev, plain=True):
     return res
 
 
+def tangent_forms(dev, repeats=3):
+    """Cost of the derivative of acceptance x squared jump w.r.t. (log eps, log alpha2):
+    one tangent launch (one transition carried with its forward derivative) against one
+    primal float64 transition, on the tuning workloads: funnel D=100 x 4096 chains and
+    logistic 2048 x 64 x 1024 chains, 32 integrator steps, as many fixed-point
+    iterations as the solver runs."""
+    import torch
+
+    import geomjax_amd as gj
+    from geomjax_amd import random as gr
+    from geomjax_amd.adjoint import esjd_acceptance
+    from geomjax_amd.rmhmc import build_kernel, implicit_midpoint
+    from geomjax_amd.synthetic import make_logistic
+
+    out = {}
+    cases = [("funnel_d100_4096_chains", gj.NealFunnel(D_F, sigma=SIGMA), gj.MongeMetric(ALPHA2), 4096, D_F, 0.03,
+              MAXIT_F)]
+    X, y, _ = make_logistic(N_ROWS, D_LOG, DATA_SEED)
+    cases.append(("logistic_2048x64_1024_chains", gj.LogisticRegression(X, y, PRIOR, device=dev), gj.FisherMetric(),
+                  1024, D_LOG, EPS_LOG, MAXIT_LOG))
+    for name, model, metric, C, D, eps, maxit in cases:
+        integ = functools.partial(implicit_midpoint, max_iters=maxit)
+        q = 0.5 * torch.randn(C, D, generator=torch.Generator().manual_seed(7), dtype=torch.float64)
+        s = gj.rmhmc.init(q.to(dev), model)
+        kern = build_kernel(integ)
+        esjd_acceptance(gr.PRNGKey(1, 0), s, model, eps, metric, 32, integrator=integ)
+        kern(gr.PRNGKey(1, 0), s, model, eps, metric, 32)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(repeats):
+            r = esjd_acceptance(gr.PRNGKey(2, i), s, model, eps, metric, 32, integrator=integ)
+        torch.cuda.synchronize()
+        t_tan = (time.perf_counter() - t0) / repeats
+        t0 = time.perf_counter()
+        for i in range(repeats):
+            kern(gr.PRNGKey(2, i), s, model, eps, metric, 32)
+        torch.cuda.synchronize()
+        t_pri = (time.perf_counter() - t0) / repeats
+        out[name] = {"tangent_ms": t_tan * 1e3, "primal_f64_transition_ms": t_pri * 1e3,
+                     "tangent_over_primal": t_tan / t_pri, "num_integration_steps": 32,
+                     "mean_objective": float(r.objective.mean()), "mean_d_objective": r.d_objective.mean(0).tolist()}
+        log(f"tangent {name}: {t_tan * 1e3:.2f} ms vs primal {t_pri * 1e3:.2f} ms ({t_tan / t_pri:.2f}x)")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -378,6 +442,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the 1024-chain and funnel comparisons")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 256 MiB three times (reference kernel for the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}): the chains of this workload are "
@@ -388,13 +454,22 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.copy_calibration:
+        x = torch.ones(64 * 1024 * 1024, device=dev)
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        log(f"copied {x.numel() * 4} bytes x 3")
+        return
     K, W = max(1, args.steps), max(0, args.warmup)
 
     head = logistic_forms(CHAINS, K, W, dev, plain=not args.no_plain)
     tuned = head["tuned"]
     achieved = tuned["mfma_flops_per_launch"] / (tuned["avg_launch_ms"] / 1e3)
     roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": BF16_PEAK / 1e12, "unit": "TFLOP/s",
-                "frac": achieved / BF16_PEAK, "traffic": None,
+                "frac": achieved / BF16_PEAK, "traffic": measured_traffic(),
+                "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT),
                 "kernel": "logistic_rmhmc_f32_kernel<2,false>", "launches": tuned["launches"],
                 "avg_launch_ms": tuned["avg_launch_ms"],
                 "algorithmic_flops_per_launch": tuned["mfma_flops_per_launch"]}
@@ -405,6 +480,7 @@ def main():
         extra["uneven_batch_8192_chains"] = uneven_batch_forms(32, K, W, dev, plain=not args.no_plain)
         extra["uneven_batch_4096_chains"] = uneven_batch_forms(16, K, W, dev, plain=not args.no_plain)
         extra["funnel_d100_8192_chains"] = funnel_forms(300, 100, dev, plain=not args.no_plain)
+        extra["tangent"] = tangent_forms(dev)
 
     cpu = None
     if not args.no_cpu_baseline:
