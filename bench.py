@@ -22,8 +22,8 @@ A "step" is one transition of all 4096 chains (10 integrator steps each).
 resident in HBM).  min-ESS/s uses the multi-chain ESS of
 geomjax/diagnostics.py:78-209 over the timed samples.  Extra fields report the
 same comparison at 1024 chains, for 256 logistic posteriors of uneven size in
-one launch (8192 and 4096 chains), for Neal's funnel D=100 (RMHMC with the
-Monge metric, 8192 chains), and the cost of the tuning derivatives (tangent
+one launch (8192 and 4096 chains), for Neal's funnel D=100 (RMHMC and LMC with
+the Monge metric, 8192 chains), and the cost of the tuning derivatives (tangent
 launch vs primal transition).  The CPU baseline is the float64 oracle
 restatement (dense metric + autodiff, as the reference) timed on this host's
 cores on a bounded sample of the headline workload.
@@ -389,6 +389,76 @@ def funnel_forms(K, W, dev, plain=True):
     return res
 
 
+def funnel_lmc_forms(K, W, dev, plain=True):
+    """Tuned and plain LMC-Monge (lmcmonge) on Neal's funnel D=100 with 8192 chains:
+    inverse mass = ones (M = I), alpha2 = 1, step 0.03, 32 integrator steps."""
+    import torch
+
+    import geomjax_amd as gj
+    from geomjax_amd import autotune
+    from geomjax_amd import random as gr
+    from geomjax_amd.lmcmonge import build_kernel as lmc_build_kernel
+    from geomjax_amd.lmcmonge import init as lmc_init
+    from geomjax_amd.lmcmonge import sample as lmc_sample
+
+    K = max(CHUNK_F, (K // CHUNK_F) * CHUNK_F)
+    model = gj.NealFunnel(D_F, sigma=SIGMA)
+    q_init = torch.randn(CHAINS_F, D_F, generator=torch.Generator().manual_seed(SEED_F), dtype=torch.float64)
+    key = gr.key(SEED_F)
+    s32 = lmc_init(q_init.to(dev, torch.float32), model)
+    im32 = torch.ones(D_F, dtype=torch.float32, device=dev)
+
+    def run(lpc, state=s32, k=key, T=CHUNK_F, out=None):
+        return lmc_sample(k, state, model, EPS_F, im32, L_F, ALPHA2, T, lanes_per_chain=lpc,
+                               store_samples=out is not None, samples_out=out)
+
+    lpc, timings = autotune.tune("lmcmonge_funnel", D_F, CHAINS_F, torch.float32, lambda l: run(l), repeats=2)
+    log(f"funnel lmc autotune lanes_per_chain: {lpc} ({ {k: round(v * 1e3, 2) for k, v in timings.items()} } ms)")
+    counter, state = 10_000, s32
+    for _ in range(max(1, W // CHUNK_F)):
+        r = run(lpc, state, gr.PRNGKey(key.seed, counter))
+        state, counter = r.state, counter + CHUNK_F
+    samples = torch.empty(K, CHAINS_F, D_F, dtype=torch.float32, device=dev)
+    accs = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for c in range(K // CHUNK_F):
+        r = run(lpc, state, gr.PRNGKey(key.seed, counter), out=samples[c * CHUNK_F:(c + 1) * CHUNK_F])
+        state, counter = r.state, counter + CHUNK_F
+        accs.append(r.acceptance_rate)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ess, _ = min_ess(samples)
+    del samples
+    tuned = {"value": CHAINS_F * L_F * K / wall, "unit": "leapfrog-steps/s", "min_ess_per_sec": ess / wall,
+             "min_ess": ess, "acceptance_rate": float(torch.stack(accs).mean().item()), "lanes_per_chain": lpc,
+             "steps": K, "ms_per_launch": wall / (K // CHUNK_F) * 1e3}
+    log(f"funnel lmc tuned: {tuned['value']:.3e} leapfrog-steps/s, min-ESS/s {tuned['min_ess_per_sec']:.3e}")
+    res = {"tuned": tuned, "plain": None}
+    if plain:
+        s64 = lmc_init(state.position.double(), model)
+        im64 = torch.ones(D_F, dtype=torch.float64, device=dev)
+        kern = lmc_build_kernel()
+        s64, _ = kern(gr.PRNGKey(key.seed + 1, 0), s64, model, EPS_F, im64, L_F, ALPHA2)
+        psamples = torch.empty(K, CHAINS_F, D_F, dtype=torch.float64, device=dev)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for t in range(K):
+            s64, _ = kern(gr.PRNGKey(key.seed + 1, 1 + t), s64, model, EPS_F, im64, L_F, ALPHA2)
+            psamples[t].copy_(s64.position)
+        torch.cuda.synchronize()
+        pwall = time.perf_counter() - t0
+        pess, _ = min_ess(psamples)
+        del psamples
+        res["plain"] = {"value": CHAINS_F * L_F * K / pwall, "unit": "leapfrog-steps/s",
+                        "min_ess_per_sec": pess / pwall, "min_ess": pess, "dtype": "f64",
+                        "form": "float64 kernel, one launch per transition"}
+        res["tuned_vs_plain"] = {"leapfrog_steps_per_sec": tuned["value"] / res["plain"]["value"],
+                                 "min_ess_per_sec": tuned["min_ess_per_sec"] / res["plain"]["min_ess_per_sec"]}
+        log(f"funnel lmc plain: {res['plain']['value']:.3e} leapfrog-steps/s; tuned/plain {res['tuned_vs_plain']}")
+    return res
+
+
 def tangent_forms(dev, repeats=3):
     """Cost of the derivative of acceptance x squared jump w.r.t. (log eps, log alpha2):
     one tangent launch (one transition carried with its forward derivative) against one
@@ -480,6 +550,7 @@ def main():
         extra["uneven_batch_8192_chains"] = uneven_batch_forms(32, K, W, dev, plain=not args.no_plain)
         extra["uneven_batch_4096_chains"] = uneven_batch_forms(16, K, W, dev, plain=not args.no_plain)
         extra["funnel_d100_8192_chains"] = funnel_forms(300, 100, dev, plain=not args.no_plain)
+        extra["funnel_lmcmonge_d100_8192_chains"] = funnel_lmc_forms(300, 100, dev, plain=not args.no_plain)
         extra["tangent"] = tangent_forms(dev)
 
     cpu = None
