@@ -174,6 +174,7 @@ class _posterior_table:
         self._host_D = [m.D for m in models]
         self._packed = None     # float32 fragment table (built on first float32 use)
         self._workspace = None  # per-chain scratch of the float32 path
+        self._retired = []      # outgrown scratch buffers: a captured HIP graph may still point at them
 
     def _base(self, chain_posterior):
         return _lib.LogisticData(num_posteriors=self.P, max_rows=self.max_rows, max_features=self.max_features,
@@ -200,6 +201,8 @@ class _posterior_table:
     def workspace(self, num_chains):
         need = num_chains * 5 * ((self.max_rows + 63) // 64 * 64) * 4
         if self._workspace is None or self._workspace.numel() < need:
+            if self._workspace is not None:
+                self._retired.append(self._workspace)
             self._workspace = torch.empty(need, dtype=torch.uint8, device=self.x.device)
         return self._workspace
 
