@@ -9,12 +9,12 @@ from .metrics import FisherMetric, MongeMetric
 from .models import LogisticBatch, LogisticRegression, NealFunnel
 from .lmcmc import lmc
 from .lmcmonge import lmc as lmcmonge
-from .rmhmc import rmhmc
+from .rmhmc import dynamic_rmhmc, rmhmc
 
 __version__ = "0.1.0"
 
 ess = diagnostics.effective_sample_size
 rhat = diagnostics.potential_scale_reduction
 
-__all__ = ["rmhmc", "lmc", "lmcmonge", "SamplingAlgorithm", "NealFunnel", "LogisticRegression", "LogisticBatch", "MongeMetric", "FisherMetric", "metrics", "models", "random", "sampling",
+__all__ = ["rmhmc", "dynamic_rmhmc", "lmc", "lmcmonge", "SamplingAlgorithm", "NealFunnel", "LogisticRegression", "LogisticBatch", "MongeMetric", "FisherMetric", "metrics", "models", "random", "sampling",
            "diagnostics", "ess", "rhat", "__version__"]

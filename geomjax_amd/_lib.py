@@ -54,7 +54,7 @@ class RmhmcStepIO(ctypes.Structure):
         "momentum", "acceptance_rate", "is_accepted", "is_divergent", "energy",
         "proposal_position", "proposal_momentum", "proposal_velocity", "proposal_logdensity",
         "proposal_logdensity_grad", "proposal_weight", "proposal_sum_log_p_accept",
-        "fp_iters", "fp_norm", "fp_success",
+        "fp_iters", "fp_norm", "fp_success", "num_integration_steps",
     )]
 
 
@@ -62,7 +62,7 @@ class RmhmcSampleIO(ctypes.Structure):
     _fields_ = [(n, _vp) for n in (
         "position", "logdensity", "z", "u",
         "final_position", "final_logdensity", "final_logdensity_grad",
-        "samples", "acceptance_rate", "num_accepted", "num_fevals",
+        "samples", "acceptance_rate", "num_accepted", "num_fevals", "num_integration_steps",
     )]
 
 

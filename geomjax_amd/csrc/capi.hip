@@ -131,6 +131,7 @@ int gjx_rmhmc_monge_funnel_step(const gjx_funnel_params* params, const gjx_rmhmc
   a.fp_iters = io->fp_iters;
   a.fp_norm = io->fp_norm;
   a.fp_success = io->fp_success;
+  a.Lc = io->num_integration_steps;
   hipError_t e = gjx::launch_funnel_rmhmc(a, params->dtype, lpc, epl, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "gjx_rmhmc_monge_funnel_step launch");
   return GJX_OK;
@@ -158,6 +159,7 @@ int gjx_rmhmc_monge_funnel_sample(const gjx_funnel_params* params, const gjx_rmh
   a.sample_acc = io->acceptance_rate;
   a.num_accepted = io->num_accepted;
   a.num_fevals = io->num_fevals;
+  a.Lc = io->num_integration_steps;
   hipError_t e = gjx::launch_funnel_rmhmc(a, params->dtype, lpc, epl, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "gjx_rmhmc_monge_funnel_sample launch");
   return GJX_OK;
@@ -413,6 +415,7 @@ int gjx_rmhmc_fisher_logistic_step(const gjx_logistic_params* params, const gjx_
   a.fp_iters = io->fp_iters;
   a.fp_norm = io->fp_norm;
   a.fp_success = io->fp_success;
+  a.Lc = io->num_integration_steps;
   hipError_t e = params->dtype == GJX_F32 ? gjx::launch_logistic_rmhmc_f32(a, data->max_features, (hipStream_t)stream)
                                            : gjx::launch_logistic_rmhmc_f64(a, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "gjx_rmhmc_fisher_logistic_step launch");
@@ -439,6 +442,7 @@ int gjx_rmhmc_fisher_logistic_sample(const gjx_logistic_params* params, const gj
   a.sample_acc = io->acceptance_rate;
   a.num_accepted = io->num_accepted;
   a.num_fevals = io->num_fevals;
+  a.Lc = io->num_integration_steps;
   hipError_t e = params->dtype == GJX_F32 ? gjx::launch_logistic_rmhmc_f32(a, data->max_features, (hipStream_t)stream)
                                            : gjx::launch_logistic_rmhmc_f64(a, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "gjx_rmhmc_fisher_logistic_sample launch");

@@ -79,6 +79,14 @@ __device__ __forceinline__ T group_exclusive_scan(T v, int lane_in_group) {
   return inc - v;
 }
 
+// Number of integration steps of chain c in transition tr: the per-chain
+// value of the dynamic kernel ([T, C], clamped to [0, L]), or L for all.
+__device__ __forceinline__ int chain_steps(const int32_t* Lc, int L, int C, int tr, int c) {
+  if (Lc == nullptr) return L;
+  const int v = Lc[(size_t)tr * C + c];
+  return v < 0 ? 0 : (v > L ? L : v);
+}
+
 // ---------------------------------------------------------------------------
 // Counter-based PRNG: Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel
 // random numbers: as easy as 1, 2, 3", SC'11), written from the published

@@ -19,6 +19,7 @@ struct FunnelArgs {
   const void* g_in;
   const void* z;    // [T, C, D] or null (device PRNG)
   const double* u;  // [T, C] or null (device PRNG)
+  const int32_t* Lc;  // [T, C] per-chain integration steps (dynamic kernel) or null: L for all
   // single-transition outputs
   void* q_out;
   double* logp_out;

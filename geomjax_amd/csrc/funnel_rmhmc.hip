@@ -181,7 +181,11 @@ __global__ void __launch_bounds__(256) funnel_rmhmc_kernel(const FunnelArgs a) {
       xc[k] = x[k];
       pc[k] = px[k];
     }
+    // chains of the dynamic kernel stop after their own number of steps
+    const int Ls = live ? chain_steps(a.Lc, a.L, a.C, STEP ? 0 : t, (int)c) : 0;
     for (int s = 0; s < a.L; ++s) {
+      const bool on = s < Ls;
+      if (!__any(on)) break;
       // z0 = (x0, p0) of this step; every iterate is z0 + h F(previous iterate)
       R x0[EPL], p0[EPL];
 #pragma unroll
@@ -192,7 +196,7 @@ __global__ void __launch_bounds__(256) funnel_rmhmc_kernel(const FunnelArgs a) {
       const R v0 = vc, pv0 = pvc;
       double norm = 0.0;
       int n = -1;  // the first evaluation (integrators.py:84) is outside the loop count
-      bool active = true;
+      bool active = on;
       while (true) {
         const FieldCoef<R> f = field_coef<R>(vc, pvc, sx, xpx, a2, inv_s2, halfDx);
         R s1 = R(0), s2 = R(0), mx = R(0);
@@ -222,10 +226,10 @@ __global__ void __launch_bounds__(256) funnel_rmhmc_kernel(const FunnelArgs a) {
           ++n_fev;
         }
         // cond_fn (integrators.py:67-74)
-        active = live && n < a.max_iters && isfinite(norm) && norm < a.div_tol && norm > a.conv_tol;
+        active = on && n < a.max_iters && isfinite(norm) && norm < a.div_tol && norm > a.conv_tol;
         if (!__any(active)) break;
       }
-      if (STEP && live && lg == 0) {
+      if (STEP && on && lg == 0) {
         const size_t idx = c * a.L + s;
         if (a.fp_iters) a.fp_iters[idx] = n;
         if (a.fp_norm) a.fp_norm[idx] = norm;
@@ -238,16 +242,18 @@ __global__ void __launch_bounds__(256) funnel_rmhmc_kernel(const FunnelArgs a) {
 #pragma unroll
         for (int k = 0; k < EPL; ++k) {
           const R xo = xc[k], po = pc[k];
-          xc[k] = xo + h * (po + f.ce * xo);
-          pc[k] = po - h * (f.A * xo + f.B * po);
+          const R xn = xo + h * (po + f.ce * xo);
+          const R pn = po - h * (f.A * xo + f.B * po);
+          xc[k] = on ? xn : xo;
+          pc[k] = on ? pn : po;
           s1 += xc[k] * xc[k];
           s2 += xc[k] * pc[k];
         }
-        vc = vc + h * f.uv;
-        pvc = pvc - h * f.dHv;
+        vc = on ? vc + h * f.uv : vc;
+        pvc = on ? pvc - h * f.dHv : pvc;
         sx = group_sum<LPC>(s1);
         xpx = group_sum<LPC>(s2);
-        n_fev += 1;
+        n_fev += on ? 1 : 0;
       }
     }
 

@@ -37,6 +37,7 @@ struct LogisticArgs {
   const void* z;
   const double* u;
   const double* J_in;  // LMC volume adjustment
+  const int32_t* Lc;   // [T, C] per-chain integration steps (dynamic kernel) or null: L for all
   // single-transition outputs
   double* J_out;
   double* prop_J;

@@ -137,7 +137,8 @@ __global__ void __launch_bounds__(NT, MD == 64 ? 2 : 1) logistic_rmhmc_f64_kerne
     };
 
     // ---- trajectory ----
-    for (int s = 0; s < a.L; ++s) {
+    const int Ls = chain_steps(a.Lc, a.L, a.C, STEP ? 0 : tr, c);
+    for (int s = 0; s < Ls; ++s) {
       // z0 of this step is (S.q, S.p), kept in registers of lanes t < D
       double q0v = 0.0, p0v = 0.0;
       if (t < D) {

@@ -534,7 +534,7 @@ __device__ __forceinline__ void run_chain(const LogisticArgs& a, Smem& S, int c,
   int64_t n_fev = 0;
   __syncthreads();
 
-  int stage = kInit, tr = 0, s = 0, mode = 0, n = 0;
+  int stage = kInit, tr = 0, s = 0, mode = 0, n = 0, Lcur = 0;
   bool have_ginv = false;
   float q0v = 0.0f, p0v = 0.0f;
   double uacc = 0.0, H0 = 0.0, loglik = 0.0, norm = 0.0;
@@ -586,7 +586,8 @@ __device__ __forceinline__ void run_chain(const LogisticArgs& a, Smem& S, int c,
       have_ginv = true;  // S.Gi = G(S.q)^-1 and S.u = G^-1 S.p
       s = 0;
       mode = 0;
-      stage = a.L > 0 ? kTraj : kEnd;
+      Lcur = chain_steps(a.Lc, a.L, a.C, STEP ? 0 : tr, c);
+      stage = Lcur > 0 ? kTraj : kEnd;
       if (stage == kEnd) continue;
     }
     // ---- energy, accept, outputs ----
@@ -654,7 +655,7 @@ __device__ __forceinline__ void run_chain(const LogisticArgs& a, Smem& S, int c,
       }
       __syncthreads();
       mode = 0;
-      if (++s == a.L) stage = kEnd;
+      if (++s == Lcur) stage = kEnd;
       continue;
     }
     {

@@ -98,6 +98,21 @@ def _as_batch(t):
     return (t[None], True) if t.dim() == 1 else (t, False)
 
 
+def _steps(num_integration_steps, shape, dev, max_num_integration_steps=None):
+    """(L, per-chain tensor or None): an int gives every chain L steps; an integer
+    tensor of ``shape`` gives each chain (and transition) its own count, the
+    dynamic kernel's form (geomjax/rmhmc/rmhmc.py:179-244), with L its maximum
+    (``max_num_integration_steps`` if given, which avoids reading the tensor
+    back to the host)."""
+    if isinstance(num_integration_steps, torch.Tensor):
+        t = num_integration_steps.to(device=dev, dtype=torch.int32).reshape(shape).contiguous()
+        L = int(max_num_integration_steps) if max_num_integration_steps is not None else int(t.max().item())
+        if L < 0:
+            raise ValueError("num_integration_steps must be >= 0")
+        return max(L, 0), t
+    return int(num_integration_steps), None
+
+
 def init(position, logdensity_fn):
     """geomjax/rmhmc/rmhmc.py:96-98: value and gradient of the log density."""
     if not isinstance(logdensity_fn, (NealFunnel, LogisticRegression, LogisticBatch)):
@@ -145,7 +160,8 @@ def build_kernel(integrator=implicit_midpoint, divergence_threshold: float = 100
     """
     solver = _solver_kwargs(integrator)
 
-    def kernel(rng_key, state, logdensity_fn, step_size, metric_fn, num_integration_steps):
+    def kernel(rng_key, state, logdensity_fn, step_size, metric_fn, num_integration_steps, *,
+               max_num_integration_steps=None):
         position, single = _as_batch(state.position)
         kind = _problem(logdensity_fn, metric_fn, position)
         position = position.contiguous()
@@ -153,7 +169,7 @@ def build_kernel(integrator=implicit_midpoint, divergence_threshold: float = 100
         C, D = position.shape
         logdensity = state.logdensity.reshape(C).to(torch.float64).contiguous()
         grad = state.logdensity_grad.reshape(C, D).to(position.dtype).contiguous()
-        L = int(num_integration_steps)
+        L, steps = _steps(num_integration_steps, (C,), dev, max_num_integration_steps)
         if isinstance(rng_key, HostNoise):
             z = rng_key.z.reshape(C, D).to(device=dev, dtype=position.dtype).contiguous()
             u = rng_key.u.reshape(C).to(device=dev, dtype=torch.float64).contiguous()
@@ -181,6 +197,7 @@ def build_kernel(integrator=implicit_midpoint, divergence_threshold: float = 100
         )
         io = _lib.RmhmcStepIO(position=position.data_ptr(), logdensity=logdensity.data_ptr(),
                               logdensity_grad=grad.data_ptr(), z=_lib.ptr(z), u=_lib.ptr(u),
+                              num_integration_steps=_lib.ptr(steps),
                               **{k: v.data_ptr() for k, v in out.items()})
         if kind == "funnel_monge":
             params = _params(C, D, _dtype_code(position), step_size, L, solver, metric_fn.alpha2,
@@ -208,7 +225,8 @@ def build_kernel(integrator=implicit_midpoint, divergence_threshold: float = 100
         proposal = Proposal(pstate, sq(out["energy"]), sq(out["proposal_weight"]),
                             sq(out["proposal_sum_log_p_accept"]))
         info = RMHMCInfo(sq(out["momentum"]), sq(out["acceptance_rate"]), sq(out["is_accepted"].bool()),
-                         sq(out["is_divergent"].bool()), sq(out["energy"]), proposal, L)
+                         sq(out["is_divergent"].bool()), sq(out["energy"]), proposal,
+                         L if steps is None else sq(steps))
         if return_fixed_point_info:
             fp = FixedPointIterationInfo(sq(out["fp_success"][:, :L].bool()), sq(out["fp_norm"][:, :L]),
                                          sq(out["fp_iters"][:, :L]))
@@ -222,7 +240,7 @@ def build_kernel(integrator=implicit_midpoint, divergence_threshold: float = 100
 
 def sample(rng_key, state, logdensity_fn, step_size, metric_fn, num_integration_steps, num_samples, *,
            integrator=implicit_midpoint, divergence_threshold=1000, store_samples=True, lanes_per_chain=0,
-           samples_out=None):
+           samples_out=None, max_num_integration_steps=None):
     """``num_samples`` transitions of every chain in ONE persistent launch.
 
     The reference runs this loop as ``jax.lax.scan`` over ``jax.vmap(kernel)``
@@ -230,7 +248,9 @@ def sample(rng_key, state, logdensity_fn, step_size, metric_fn, num_integration_
     across transitions.  ``rng_key`` is a PRNGKey (device Philox; transition t
     uses counter ``key.counter + t``) or HostNoise with z [T, C, D], u [T, C].
     ``samples_out`` (a contiguous [T, C, D] tensor of the state dtype) receives
-    the samples in place of a new allocation.  Returns ``SampleResult``.
+    the samples in place of a new allocation.  ``num_integration_steps`` is an
+    int or, for the dynamic kernel, an integer tensor [T, C] of per-transition,
+    per-chain counts.  Returns ``SampleResult``.
     """
     from .sampling import SampleResult
 
@@ -246,6 +266,7 @@ def sample(rng_key, state, logdensity_fn, step_size, metric_fn, num_integration_
     if T < 1:
         raise ValueError("num_samples must be >= 1")
     logdensity = state.logdensity.reshape(C).to(torch.float64).contiguous()
+    L, steps = _steps(num_integration_steps, (T, C), dev, max_num_integration_steps)
     if isinstance(rng_key, HostNoise):
         z = rng_key.z.reshape(T, C, D).to(device=dev, dtype=position.dtype).contiguous()
         u = rng_key.u.reshape(T, C).to(device=dev, dtype=torch.float64).contiguous()
@@ -271,15 +292,15 @@ def sample(rng_key, state, logdensity_fn, step_size, metric_fn, num_integration_
                             u=_lib.ptr(u), final_position=final_q.data_ptr(), final_logdensity=final_l.data_ptr(),
                             final_logdensity_grad=final_g.data_ptr(), samples=_lib.ptr(samples),
                             acceptance_rate=acc.data_ptr(), num_accepted=n_acc.data_ptr(),
-                            num_fevals=n_fev.data_ptr())
+                            num_fevals=n_fev.data_ptr(), num_integration_steps=_lib.ptr(steps))
     if kind == "funnel_monge":
-        params = _params(C, D, _dtype_code(position), step_size, num_integration_steps, solver, metric_fn.alpha2,
+        params = _params(C, D, _dtype_code(position), step_size, L, solver, metric_fn.alpha2,
                          logdensity_fn.sigma, divergence_threshold, T=T, lanes_per_chain=lanes_per_chain, seed=seed,
                          counter=counter)
         rc = _lib.load().gjx_rmhmc_monge_funnel_sample(params, io, _lib.stream_handle(dev))
         _lib.check(rc, "gjx_rmhmc_monge_funnel_sample")
     else:
-        params = logistic_params(logdensity_fn, C, L=num_integration_steps, T=T, step_size=step_size,
+        params = logistic_params(logdensity_fn, C, L=L, T=T, step_size=step_size,
                                  max_iters=solver["max_iters"], convergence_tol=solver["convergence_tol"],
                                  divergence_tol=solver["divergence_tol"], divergence_threshold=divergence_threshold,
                                  seed=seed, counter=counter, dtype=position.dtype)
@@ -315,5 +336,101 @@ class rmhmc:
         def sample_fn(rng_key, state, num_samples, **kw):
             return sample(rng_key, state, logdensity_fn, step_size, metric_fn, num_integration_steps, num_samples,
                           integrator=integrator, divergence_threshold=divergence_threshold, **kw)
+
+        return BoundAlgorithm(init_fn, step_fn, sample_fn)
+
+
+# ---------------------------------------------------------------------------
+# Dynamic RMHMC (geomjax/rmhmc/rmhmc.py:179-244, 314-373)
+# ---------------------------------------------------------------------------
+
+
+class DynamicRMHMCState(NamedTuple):  # geomjax/rmhmc/rmhmc.py:44-55 (same fields and order)
+    position: torch.Tensor
+    logdensity: torch.Tensor
+    logdensity_grad: torch.Tensor
+    random_generator_arg: torch.Tensor  # [C] int64, one per chain
+
+
+def _next_random_arg(arg):
+    """Default successor of a chain's random argument (the reference splits a
+    JAX key; here the argument is an int64 counter per chain)."""
+    return arg + 1
+
+
+def _integration_steps(arg):
+    """Default number of steps: uniform on {1, ..., 9} (the reference's
+    ``jax.random.randint(key, (), 1, 10)``), from a multiplicative hash of the
+    chain's counter (Knuth's 2654435761, low 32 bits), so it needs no host
+    round trip."""
+    h = (arg.to(torch.int64) * 2654435761) & 0xFFFFFFFF
+    return (1 + h % 9).to(torch.int32)
+
+
+def init_dynamic(position, logdensity_fn, random_generator_arg):
+    """geomjax/rmhmc/rmhmc.py:99-106; ``random_generator_arg`` is a scalar or a [C]
+    int64 tensor (give each chain its own, e.g. ``torch.arange(C)``)."""
+    s = init(position, logdensity_fn)
+    arg = torch.as_tensor(random_generator_arg, dtype=torch.int64, device=s.position.device)
+    C = s.position.shape[0] if s.position.dim() == 2 else 1
+    return DynamicRMHMCState(s.position, s.logdensity, s.logdensity_grad, arg.expand(C).clone())
+
+
+def build_dynamic_kernel(integrator=implicit_midpoint, divergence_threshold: float = 1000,
+                         next_random_arg_fn=_next_random_arg, integration_steps_fn=_integration_steps,
+                         max_num_integration_steps=9, **kernel_kwargs):
+    """geomjax/rmhmc/rmhmc.py:179-244: the RMHMC kernel with a number of
+    integration steps drawn per chain from its ``random_generator_arg``.
+
+    ``integration_steps_fn(arg, **kw)`` maps the [C] int64 arguments to [C]
+    step counts (a torch function on the device); ``max_num_integration_steps``
+    bounds them (values above are clamped by the kernel; None reads the
+    maximum back from the device)."""
+    base = build_kernel(integrator, divergence_threshold, **kernel_kwargs)
+
+    def kernel(rng_key, state, logdensity_fn, step_size, metric_fn, **integration_steps_kwargs):
+        steps = integration_steps_fn(state.random_generator_arg, **integration_steps_kwargs)
+        new, info = base(rng_key, RMHMCState(state.position, state.logdensity, state.logdensity_grad),
+                         logdensity_fn, step_size, metric_fn, steps,
+                         max_num_integration_steps=max_num_integration_steps)
+        return (DynamicRMHMCState(new.position, new.logdensity, new.logdensity_grad,
+                                  next_random_arg_fn(state.random_generator_arg)), info)
+
+    return kernel
+
+
+class dynamic_rmhmc:
+    """geomjax/rmhmc/rmhmc.py:314-373."""
+
+    init = staticmethod(init_dynamic)
+    build_kernel = staticmethod(build_dynamic_kernel)
+
+    def __new__(cls, logdensity_fn, step_size, metric_fn, *, divergence_threshold: int = 1000,
+                integrator=implicit_midpoint, next_random_arg_fn=_next_random_arg,
+                integration_steps_fn=_integration_steps, max_num_integration_steps=9):
+        from .sampling import BoundAlgorithm
+
+        kernel = cls.build_kernel(integrator, divergence_threshold, next_random_arg_fn, integration_steps_fn,
+                                  max_num_integration_steps)
+
+        def init_fn(position, random_generator_arg):
+            return cls.init(position, logdensity_fn, random_generator_arg)
+
+        def step_fn(rng_key, state):
+            return kernel(rng_key, state, logdensity_fn, step_size, metric_fn)
+
+        def sample_fn(rng_key, state, num_samples, **kw):
+            """``num_samples`` transitions in one persistent launch; the step counts
+            of all transitions are drawn up front from the successive arguments."""
+            args, a = [], state.random_generator_arg
+            for _ in range(int(num_samples)):
+                args.append(integration_steps_fn(a))
+                a = next_random_arg_fn(a)
+            r = sample(rng_key, RMHMCState(state.position, state.logdensity, state.logdensity_grad),
+                       logdensity_fn, step_size, metric_fn, torch.stack(args), num_samples, integrator=integrator,
+                       divergence_threshold=divergence_threshold,
+                       max_num_integration_steps=max_num_integration_steps, **kw)
+            return r._replace(state=DynamicRMHMCState(r.state.position, r.state.logdensity,
+                                                      r.state.logdensity_grad, a))
 
         return BoundAlgorithm(init_fn, step_fn, sample_fn)

@@ -101,6 +101,10 @@ typedef struct {
   int32_t* fp_iters;             /* [C, L] optional: FixedPointIterationInfo.iters */
   double* fp_norm;               /* [C, L] optional: .norm */
   uint8_t* fp_success;           /* [C, L] optional: .success */
+  /* [C] optional: per-chain number of integration steps (the dynamic kernel,
+   * geomjax/rmhmc/rmhmc.py:179-244); each value is clamped to [0, L] with
+   * L = params->num_integration_steps, which also stays the row stride of fp_*. */
+  const int32_t* num_integration_steps;
 } gjx_rmhmc_step_io;
 
 /* Buffers of T consecutive transitions in one persistent launch
@@ -118,6 +122,9 @@ typedef struct {
   float* acceptance_rate;        /* [T, C] optional */
   int32_t* num_accepted;         /* [C] optional: accepted transitions */
   int64_t* num_fevals;           /* [C] optional: evaluations of the vector field executed */
+  /* [T, C] optional: per-transition, per-chain number of integration steps
+   * (dynamic kernel), clamped to [0, params->num_integration_steps]. */
+  const int32_t* num_integration_steps;
 } gjx_rmhmc_sample_io;
 
 /* One RMHMC transition per chain.  Replaces the reference kernel
