@@ -1,0 +1,260 @@
+"""ChEES adaptation of the step size and trajectory length of jittered RMHMC.
+
+Reference: geomjax/adaptation/chees_adaptation_riemanian.py (the driver,
+``dynamics="rmhmc"``), geomjax/optimizers/dual_averaging.py (step size), and
+the optax Adam its callers pass for the trajectory length.  Every iteration
+runs one RMHMC transition of all chains with the HIP kernel (the same number
+of integration steps for every chain: ceil(jitter * trajectory_length /
+step_size), the jitter a Halton sequence shared by the chains), then updates
+the parameters from cross-chain statistics computed on the device:
+
+* step size: dual averaging towards a harmonic-mean acceptance of 0.651
+  (non-divergent chains);
+* trajectory length: one Adam step on log T with the ChEES trajectory
+  gradient  jitter * T * (|q_L - m_L|^2 - |q_0 - m_0|^2) * ((q_L - m_L) . v_L),
+  acceptance-weighted over non-divergent chains, where m are the cross-chain
+  means and v_L the proposal's velocity;
+* both parameters are reported as moving averages of their logs with weight
+  step ** -decay_rate.
+
+Like the reference, the Adam update is applied to the gradient as computed
+(``optim.update(trajectory_gradient, ...)``, chees_adaptation_riemanian.py:196),
+which descends the ChEES criterion; ``reference_faithful=False`` negates it so
+that Adam ascends the criterion, as the ChEES method intends.
+"""
+import math
+from typing import NamedTuple
+
+import torch
+
+from .random import PRNGKey
+from .rmhmc import DynamicRMHMCState, build_kernel, dynamic_rmhmc, implicit_midpoint, init
+
+OPTIMAL_TARGET_ACCEPTANCE_RATE = 0.651
+
+
+def _exp(x):
+    """exp of a Python float that overflows to inf (and keeps nan) instead of raising."""
+    return math.exp(x) if x < 709.0 or x != x else math.inf
+
+
+# ---------------------------------------------------------------------------
+# Dual averaging (geomjax/optimizers/dual_averaging.py; Nesterov 2009, with the
+# parameterisation of Hoffman & Gelman 2014)
+# ---------------------------------------------------------------------------
+
+
+class DualAveragingState(NamedTuple):
+    log_x: float
+    log_x_avg: float
+    step: int
+    avg_error: float
+    mu: float
+
+
+def dual_averaging(t0: int = 10, gamma: float = 0.05, kappa: float = 0.75):
+    """(init, update, final) of the dual averaging scheme (dual_averaging.py:53-129)."""
+
+    def init_fn(x_init: float) -> DualAveragingState:
+        return DualAveragingState(math.log(x_init), 0.0, 1, 0.0, math.log(10.0 * x_init))
+
+    def update_fn(da: DualAveragingState, gradient: float) -> DualAveragingState:
+        # dual_averaging.py:116-123 (the average uses the log_x before this update)
+        reg_step = da.step + t0
+        eta_t = da.step ** (-kappa)
+        avg_error = (1.0 - 1.0 / reg_step) * da.avg_error + gradient / reg_step
+        log_x = da.mu - (math.sqrt(da.step) / gamma) * avg_error
+        log_x_avg = eta_t * da.log_x + (1.0 - eta_t) * da.log_x_avg
+        return DualAveragingState(log_x, log_x_avg, da.step + 1, avg_error, da.mu)
+
+    def final_fn(da: DualAveragingState) -> float:
+        return math.exp(da.log_x_avg)
+
+    return init_fn, update_fn, final_fn
+
+
+# ---------------------------------------------------------------------------
+# Adam on a scalar (optax.adam: scale_by_adam then scale(-learning_rate))
+# ---------------------------------------------------------------------------
+
+
+class AdamState(NamedTuple):
+    count: int
+    mu: float
+    nu: float
+
+
+class adam:
+    """``init(params)``, ``update(grad, state, params) -> (updates, state)`` with the
+    optax conventions: bias-corrected moments, eps added outside the square root."""
+
+    def __init__(self, learning_rate: float, b1: float = 0.9, b2: float = 0.999, eps: float = 1e-8):
+        self.lr, self.b1, self.b2, self.eps = float(learning_rate), b1, b2, eps
+
+    def init(self, params):
+        return AdamState(0, 0.0, 0.0)
+
+    def update(self, grad, state, params=None):
+        g = float(grad)
+        mu = self.b1 * state.mu + (1.0 - self.b1) * g
+        nu = self.b2 * state.nu + (1.0 - self.b2) * g * g
+        count = state.count + 1
+        mu_hat = mu / (1.0 - self.b1 ** count)
+        nu_hat = nu / (1.0 - self.b2 ** count)
+        return -self.lr * mu_hat / (math.sqrt(nu_hat) + self.eps), AdamState(count, mu, nu)
+
+
+# ---------------------------------------------------------------------------
+# Halton jitter (chees_adaptation_riemanian.py:476-479): the base-2 van der
+# Corput value of i + 1 over max_bits bits
+# ---------------------------------------------------------------------------
+
+
+def halton(i, max_bits=10):
+    """Scalar (int) or tensor (int64) argument; same type of result."""
+    if isinstance(i, torch.Tensor):
+        bits = torch.arange(int(max_bits), device=i.device, dtype=torch.int64)
+        masks = torch.pow(2, bits)
+        digits = torch.remainder(torch.div(i.to(torch.int64)[..., None] + 1, masks, rounding_mode="floor"), 2)
+        return (digits.double() * (0.5 / masks.double())).sum(-1)
+    return sum((((int(i) + 1) // (1 << b)) % 2) * (0.5 / (1 << b)) for b in range(int(max_bits)))
+
+
+class ChEESAdaptationState(NamedTuple):  # chees_adaptation_riemanian.py:21-56 (same fields)
+    step_size: float
+    log_step_size_moving_average: float
+    trajectory_length: float
+    log_trajectory_length_moving_average: float
+    da_state: DualAveragingState
+    optim_state: AdamState
+    random_generator_arg: int
+    step: int
+
+
+def chees_trajectory_gradient(jitter_value, trajectory_length, proposed_positions, proposed_velocities,
+                              initial_positions, acceptance_probabilities, is_divergent):
+    """Acceptance-weighted mean over non-divergent chains of
+    jitter * T * (|q_L - m_L|^2 - |q_0 - m_0|^2) * ((q_L - m_L) . v_L), with m the
+    nan-means over chains (chees_adaptation_riemanian.py:157-189): the derivative in
+    log T of (|q_L - m_L|^2 - |q_0 - m_0|^2)^2 / 4 when q_L moves with velocity
+    jitter * v_L per unit of T and the means are held fixed."""
+    ok = ~is_divergent.bool()
+    acc = acceptance_probabilities.double()
+    qL = proposed_positions.double()
+    q0 = initial_positions.double()
+    pc = qL - torch.nanmean(qL, dim=0)
+    ic = q0 - torch.nanmean(q0, dim=0)
+    v = proposed_velocities.double()
+    grads = jitter_value * trajectory_length * ((pc * pc).sum(-1) - (ic * ic).sum(-1)) * (pc * v).sum(-1)
+    return float(((acc * grads) * ok).sum() / (acc * ok).sum())
+
+
+def chees_base(jitter_generator, next_random_arg_fn, optim, target_acceptance_rate, decay_rate,
+               reference_faithful=True):
+    """(init, update) of the adaptation state (chees_adaptation_riemanian.py:59-268)."""
+    sign = 1.0 if reference_faithful else -1.0
+    da_init, da_update, _ = dual_averaging()
+
+    def init_fn(random_generator_arg, step_size):
+        return ChEESAdaptationState(step_size, 0.0, step_size, 0.0, da_init(step_size), optim.init(step_size),
+                                    random_generator_arg, 1)
+
+    def update_fn(st: ChEESAdaptationState, proposed_positions, proposed_velocities, initial_positions,
+                  acceptance_probabilities, is_divergent):
+        ok = ~is_divergent.bool()
+        acc = acceptance_probabilities.double()
+        # harmonic mean acceptance over non-divergent chains (:142-144)
+        harmonic = 1.0 / float((1.0 / acc[ok]).mean()) if bool(ok.any()) else float("nan")
+        da_ = da_update(st.da_state, target_acceptance_rate - harmonic)
+        step_size_ = _exp(da_.log_x)
+        if math.isfinite(step_size_):
+            new_step_size, new_da, new_log_step = step_size_, da_, da_.log_x
+        else:
+            new_step_size, new_da, new_log_step = st.step_size, st.da_state, st.da_state.log_x
+        w = st.step ** (-decay_rate)
+        new_log_step_ma = (1.0 - w) * st.log_step_size_moving_average + w * new_log_step
+        traj_grad = chees_trajectory_gradient(jitter_generator(st.random_generator_arg), st.trajectory_length,
+                                              proposed_positions, proposed_velocities, initial_positions,
+                                              acceptance_probabilities, is_divergent)
+        log_T = math.log(st.trajectory_length)
+        upd, optim_ = optim.update(sign * traj_grad, st.optim_state, log_T)
+        log_T_ = log_T + upd
+        if math.isfinite(log_T_):
+            new_log_T, new_optim = log_T_, optim_
+        else:
+            new_log_T, new_optim = log_T, st.optim_state
+        new_log_T_ma = (1.0 - w) * st.log_trajectory_length_moving_average + w * new_log_T
+        return ChEESAdaptationState(new_step_size, new_log_step_ma, _exp(new_log_T_ma), new_log_T_ma, new_da,
+                                    new_optim, next_random_arg_fn(st.random_generator_arg), st.step + 1)
+
+    return init_fn, update_fn
+
+
+class AdaptationResults(NamedTuple):  # geomjax/adaptation/base.py:19-21
+    state: DynamicRMHMCState
+    parameters: dict
+
+
+class chees_adaptation:
+    """geomjax/adaptation/chees_adaptation_riemanian.py:271-473 with ``dynamics="rmhmc"``.
+
+    ``run(rng_key, positions, step_size, optim, num_steps, max_sampling_steps=1000)``
+    returns ``(AdaptationResults(last_states, parameters), history)``; the
+    parameters build ``geomjax_amd.dynamic_rmhmc(logdensity_fn, **parameters)``.
+    ``history`` holds per-iteration scalars (step size, trajectory length, number
+    of integration steps, mean acceptance) rather than every chain state.
+    """
+
+    def __init__(self, logdensity_fn, metric_fn, num_chains, *, jitter_amount=1.0,
+                 target_acceptance_rate=OPTIMAL_TARGET_ACCEPTANCE_RATE, decay_rate=0.5,
+                 integrator=implicit_midpoint, reference_faithful=True):
+        self.logdensity_fn, self.metric_fn, self.num_chains = logdensity_fn, metric_fn, int(num_chains)
+        self.jitter_amount, self.target, self.decay_rate = jitter_amount, target_acceptance_rate, decay_rate
+        self.integrator, self.reference_faithful = integrator, reference_faithful
+
+    def run(self, rng_key: PRNGKey, positions, step_size, optim, num_steps=1000, *, max_sampling_steps=1000):
+        if positions.shape[0] != self.num_chains:
+            raise ValueError("initial positions' leading dimension must be num_chains")
+        max_bits = math.ceil(math.log2(num_steps + max_sampling_steps))
+        ja = self.jitter_amount
+
+        def jitter(i):
+            return halton(i, max_bits) * ja + (1.0 - ja)
+
+        def next_arg(i):
+            return i + 1
+
+        init_fn, update_fn = chees_base(jitter, next_arg, optim, self.target, self.decay_rate,
+                                        self.reference_faithful)
+        kernel = build_kernel(self.integrator)
+        states = init(positions, self.logdensity_fn)
+        ast = init_fn(0, float(step_size))
+        history = []
+        for it in range(int(num_steps)):
+            # integration_steps_fn(arg, trajectory_length_adjusted) (:386-390), shared by the chains
+            L = int(math.ceil(jitter(ast.random_generator_arg) * ast.trajectory_length / ast.step_size))
+            new_states, info = kernel(PRNGKey(rng_key.seed, rng_key.counter + it), states, self.logdensity_fn,
+                                      ast.step_size, self.metric_fn, L)
+            ast_new = update_fn(ast, info.proposal.state.position, info.proposal.state.velocity, states.position,
+                                info.acceptance_rate, info.is_divergent)
+            history.append((ast.step_size, ast.trajectory_length, L, float(info.acceptance_rate.mean())))
+            states, ast = new_states, ast_new
+
+        traj_adj = math.exp(ast.log_trajectory_length_moving_average - ast.log_step_size_moving_average)
+
+        def integration_steps_fn(arg):
+            return torch.ceil(jitter(arg) * traj_adj).to(torch.int32)
+
+        parameters = {"step_size": math.exp(ast.log_step_size_moving_average), "metric_fn": self.metric_fn,
+                      "next_random_arg_fn": next_arg, "integration_steps_fn": integration_steps_fn,
+                      "max_num_integration_steps": int(math.ceil(traj_adj)), "integrator": self.integrator}
+        C = states.position.shape[0]
+        last = DynamicRMHMCState(states.position, states.logdensity, states.logdensity_grad,
+                                 torch.full((C,), ast.random_generator_arg, dtype=torch.int64,
+                                            device=states.position.device))
+        return AdaptationResults(last, parameters), history
+
+
+__all__ = ["chees_adaptation", "chees_base", "chees_trajectory_gradient", "dual_averaging", "adam", "halton",
+           "dynamic_rmhmc",
+           "OPTIMAL_TARGET_ACCEPTANCE_RATE"]
