@@ -134,10 +134,20 @@ __global__ void __launch_bounds__(256) funnel_rmhmc_tangent_kernel(const FunnelA
     const RD e = exp(-vv);
     const RD gv = -vv * inv_s2 - halfDx + RD(0.5) * e * Sx;
     const RD gp = -e * Xp + gv * pvv;
-    const RD L = RD(1.0) + a2 * (e * e * Sx + gv * gv);
+    const RD gg = e * e * Sx + gv * gv;
+    const RD cp = monge_perp_coef(gp, gg);
+    RD d4(0.0);
+#pragma unroll
+    for (int k = 0; k < EPL; ++k) {
+      const RD r = pp[k] + cp * e * xx[k];
+      d4 += r * r;
+    }
+    const RD rv = pvv - cp * gv;
+    const RD perp2 = group_sum<LPC>(d4) + rv * rv;
     logp_out = RD(-0.5) * vv * vv * inv_s2 - RD(0.5) * e * Sx - RD(0.5 * Dx) * vv +
                RD(-log(a.sigma) - 0.5 * kLog2Pi - 0.5 * (double)Dx * kLog2Pi);
-    return RD(0.5) * (P2 - a2 * gp * gp / L) + RD(0.5) * log(L) + RD(0.5 * (double)D * kLog2Pi);
+    return RD(0.5) * monge_quad(P2, perp2, gg, a2) + RD(0.5) * log(RD(1.0) + a2 * gg) +
+           RD(0.5 * (double)D * kLog2Pi);
   };
   RD lp0;
   const RD T0 = energy(x, px, v, pv, lp0);

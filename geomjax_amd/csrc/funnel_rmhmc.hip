@@ -168,8 +168,18 @@ __global__ void __launch_bounds__(256) funnel_rmhmc_kernel(const FunnelArgs a) {
       const double e = exp(-(double)v);
       const double gv = -(double)v / (a.sigma * a.sigma) - 0.5 * Dx + 0.5 * e * (double)sx;
       const double gp = -e * (double)xpx + gv * (double)pv;
-      const double L = 1.0 + a.alpha2 * (e * e * (double)sx + gv * gv);
-      const double Tk = 0.5 * (pp - a.alpha2 * gp * gp / L) + 0.5 * log(L) + 0.5 * (double)D * kLog2Pi;
+      const double gg = e * e * (double)sx + gv * gv;
+      const double cp = monge_perp_coef(gp, gg);
+      double a4 = 0.0;
+#pragma unroll
+      for (int k = 0; k < EPL; ++k) {
+        const double r = (double)px[k] + cp * e * (double)x[k];
+        a4 += r * r;
+      }
+      const double rv = (double)pv - cp * gv;
+      const double perp2 = group_sum<LPC>(a4) + rv * rv;
+      const double Tk = 0.5 * monge_quad(pp, perp2, gg, a.alpha2) + 0.5 * log(1.0 + a.alpha2 * gg) +
+                        0.5 * (double)D * kLog2Pi;
       H0 = -logp + Tk;
     }
 
@@ -277,11 +287,21 @@ __global__ void __launch_bounds__(256) funnel_rmhmc_kernel(const FunnelArgs a) {
       const double e = exp(-vd);
       const double gv = -vd / s2 - 0.5 * Dx + 0.5 * e * Sx;
       const double gp = -e * Xp + gv * (double)pvc;
-      const double L = 1.0 + a.alpha2 * (e * e * Sx + gv * gv);
+      const double gg = e * e * Sx + gv * gv;
+      const double L = 1.0 + a.alpha2 * gg;
+      const double cp = monge_perp_coef(gp, gg);
+      double d4 = 0.0;
+#pragma unroll
+      for (int k = 0; k < EPL; ++k) {
+        const double r = (double)pc[k] + cp * e * (double)xc[k];
+        d4 += r * r;
+      }
+      const double rv = (double)pvc - cp * gv;
+      const double perp2 = group_sum<LPC>(d4) + rv * rv;
       // funnel log density (examples/funnel/main.py:34-39)
       logp1 = -0.5 * vd * vd / s2 - log(a.sigma) - 0.5 * kLog2Pi - 0.5 * e * Sx - 0.5 * (double)Dx * vd -
               0.5 * (double)Dx * kLog2Pi;
-      const double Tk = 0.5 * (pp - a.alpha2 * gp * gp / L) + 0.5 * log(L) + 0.5 * (double)D * kLog2Pi;
+      const double Tk = 0.5 * monge_quad(pp, perp2, gg, a.alpha2) + 0.5 * log(L) + 0.5 * (double)D * kLog2Pi;
       H1 = -logp1 + Tk;
       e1 = R(e);
       gv1 = R(gv);
