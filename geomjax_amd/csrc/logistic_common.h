@@ -328,6 +328,9 @@ __device__ double cholesky(Smem<C>& S, int D) {
 }
 
 // u = G^-1 p with G^-1 in S.Gi (vector p in LDS).
+// Entries past D are written as zeros: the passes over X multiply whole padded
+// rows by this vector, and 0 times a stale LDS value left by an earlier kernel
+// (possibly Inf or NaN) would not be 0.
 template <class C>
 __device__ __forceinline__ void ginv_times(Smem<C>& S, const double* p, double* out, int D) {
   const int t = threadIdx.x;
@@ -335,6 +338,8 @@ __device__ __forceinline__ void ginv_times(Smem<C>& S, const double* p, double* 
     double s = 0.0;
     for (int j = 0; j < D; ++j) s += S.Gi[t * C::LDX + j] * p[j];
     out[t] = s;
+  } else if (t < C::MAXD) {
+    out[t] = 0.0;
   }
   __syncthreads();
 }

@@ -277,6 +277,11 @@ __device__ __forceinline__ void invert_solve(Smem& S, int D, double& logdet, dou
     }
     S.u[t] = s;
     S.tu[t] = ts;
+  } else if (t < MAXD) {
+    // the quadratic-form pass multiplies whole zero-padded rows of X by u and
+    // u': entries past D must be 0, not stale LDS (Inf or NaN times 0 is NaN)
+    S.u[t] = 0.0;
+    S.tu[t] = 0.0;
   }
   __syncthreads();
 }
