@@ -149,6 +149,16 @@ def chees_trajectory_gradient(jitter_value, trajectory_length, proposed_position
     return float(((acc * grads) * ok).sum() / (acc * ok).sum())
 
 
+def harmonic_acceptance(acceptance_probabilities, is_divergent):
+    """Harmonic mean of the acceptance probabilities of the non-divergent chains
+    (geomjax/adaptation/chees_adaptation_riemanian.py:142-144): the statistic
+    the step size's dual averaging drives to the target; NaN if every chain
+    diverged."""
+    ok = ~is_divergent.bool()
+    acc = acceptance_probabilities.double()
+    return 1.0 / float((1.0 / acc[ok]).mean()) if bool(ok.any()) else float("nan")
+
+
 def chees_base(jitter_generator, next_random_arg_fn, optim, target_acceptance_rate, decay_rate,
                reference_faithful=True):
     """(init, update) of the adaptation state (chees_adaptation_riemanian.py:59-268)."""
@@ -161,10 +171,7 @@ def chees_base(jitter_generator, next_random_arg_fn, optim, target_acceptance_ra
 
     def update_fn(st: ChEESAdaptationState, proposed_positions, proposed_velocities, initial_positions,
                   acceptance_probabilities, is_divergent):
-        ok = ~is_divergent.bool()
-        acc = acceptance_probabilities.double()
-        # harmonic mean acceptance over non-divergent chains (:142-144)
-        harmonic = 1.0 / float((1.0 / acc[ok]).mean()) if bool(ok.any()) else float("nan")
+        harmonic = harmonic_acceptance(acceptance_probabilities, is_divergent)
         da_ = da_update(st.da_state, target_acceptance_rate - harmonic)
         step_size_ = _exp(da_.log_x)
         if math.isfinite(step_size_):
@@ -202,7 +209,8 @@ class chees_adaptation:
     returns ``(AdaptationResults(last_states, parameters), history)``; the
     parameters build ``geomjax_amd.dynamic_rmhmc(logdensity_fn, **parameters)``.
     ``history`` holds per-iteration scalars (step size, trajectory length, number
-    of integration steps, mean acceptance) rather than every chain state.
+    of integration steps, mean acceptance, harmonic-mean acceptance of the
+    non-divergent chains) rather than every chain state.
     """
 
     def __init__(self, logdensity_fn, metric_fn, num_chains, *, jitter_amount=1.0,
@@ -237,7 +245,8 @@ class chees_adaptation:
                                       ast.step_size, self.metric_fn, L)
             ast_new = update_fn(ast, info.proposal.state.position, info.proposal.state.velocity, states.position,
                                 info.acceptance_rate, info.is_divergent)
-            history.append((ast.step_size, ast.trajectory_length, L, float(info.acceptance_rate.mean())))
+            history.append((ast.step_size, ast.trajectory_length, L, float(info.acceptance_rate.mean()),
+                            harmonic_acceptance(info.acceptance_rate, info.is_divergent)))
             states, ast = new_states, ast_new
 
         traj_adj = math.exp(ast.log_trajectory_length_moving_average - ast.log_step_size_moving_average)

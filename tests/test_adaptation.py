@@ -14,7 +14,7 @@ import math
 
 import torch
 
-from geomjax_amd.adaptation import adam, chees_trajectory_gradient, dual_averaging, halton
+from geomjax_amd.adaptation import adam, chees_trajectory_gradient, dual_averaging, halton, harmonic_acceptance
 
 
 def test_halton_is_van_der_corput_base_2():
@@ -35,6 +35,16 @@ def test_adam_matches_torch_optim_adam():
         ((p - 3.0) ** 2).backward()
         ref.step()
         assert abs(x - float(p)) <= 1e-12 * max(1.0, abs(x))
+
+
+def test_harmonic_acceptance_skips_divergent_chains():
+    acc = torch.tensor([0.5, 1.0, 0.25, 0.9])
+    div = torch.tensor([False, False, False, True])
+    # 3 / (1/0.5 + 1/1 + 1/0.25); the divergent chain's 0.9 is left out
+    assert math.isclose(harmonic_acceptance(acc, div), 3.0 / 7.0, rel_tol=1e-15)
+    assert math.isnan(harmonic_acceptance(acc, torch.ones(4, dtype=torch.bool)))
+    # one chain that never accepts pulls the harmonic mean to 0
+    assert harmonic_acceptance(torch.tensor([0.0, 1.0]), torch.zeros(2, dtype=torch.bool)) == 0.0
 
 
 def test_dual_averaging_finds_the_target_acceptance():
