@@ -10,11 +10,12 @@ from .models import LogisticBatch, LogisticRegression, NealFunnel
 from .lmcmc import lmc
 from .lmcmonge import lmc as lmcmonge
 from .rmhmc import dynamic_rmhmc, rmhmc
+from .nuts import nuts
 
 __version__ = "0.1.0"
 
 ess = diagnostics.effective_sample_size
 rhat = diagnostics.potential_scale_reduction
 
-__all__ = ["rmhmc", "dynamic_rmhmc", "lmc", "lmcmonge", "SamplingAlgorithm", "NealFunnel", "LogisticRegression", "LogisticBatch", "MongeMetric", "FisherMetric", "metrics", "models", "random", "sampling",
+__all__ = ["rmhmc", "dynamic_rmhmc", "nuts", "lmc", "lmcmonge", "SamplingAlgorithm", "NealFunnel", "LogisticRegression", "LogisticBatch", "MongeMetric", "FisherMetric", "metrics", "models", "random", "sampling",
            "diagnostics", "ess", "rhat", "__version__"]

@@ -58,6 +58,15 @@ class RmhmcStepIO(ctypes.Structure):
     )]
 
 
+class NutsIO(ctypes.Structure):
+    _fields_ = [(n, _vp) for n in (
+        "position", "logdensity", "z", "u_direction", "u_merge", "u_step",
+        "new_position", "new_logdensity", "new_logdensity_grad", "momentum", "energy", "acceptance_rate",
+        "is_divergent", "is_turning", "num_trajectory_expansions", "num_integration_steps",
+        "leftmost_position", "rightmost_position",
+    )]
+
+
 class RmhmcSampleIO(ctypes.Structure):
     _fields_ = [(n, _vp) for n in (
         "position", "logdensity", "z", "u",
@@ -150,6 +159,7 @@ _SIGNATURES = {
     "gjx_lmcmonge_funnel_step": [ctypes.POINTER(FunnelParams), ctypes.POINTER(LmcStepIO), _vp],
     "gjx_lmcmonge_funnel_sample": [ctypes.POINTER(FunnelParams), ctypes.POINTER(LmcSampleIO), _vp],
     "gjx_rmhmc_monge_funnel_tangent": [ctypes.POINTER(FunnelParams), ctypes.POINTER(TangentIO), _vp],
+    "gjx_rmhmc_monge_funnel_nuts": [ctypes.POINTER(FunnelParams), _c_int32, ctypes.POINTER(NutsIO), _vp],
     "gjx_funnel_value_and_grad": [_c_int32, _c_int32, _c_double, _c_int32, _vp, _vp, _vp, _vp],
     "gjx_rmhmc_fisher_logistic_step": [ctypes.POINTER(LogisticParams), ctypes.POINTER(LogisticData),
                                        ctypes.POINTER(RmhmcStepIO), _vp],

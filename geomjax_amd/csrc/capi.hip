@@ -195,6 +195,53 @@ int gjx_rmhmc_monge_funnel_tangent(const gjx_funnel_params* params, const gjx_ta
   return GJX_OK;
 }
 
+int gjx_rmhmc_monge_funnel_nuts(const gjx_funnel_params* params, int32_t max_num_doublings, const gjx_nuts_io* io,
+                                void* stream) {
+  GJX_REQUIRE(params != nullptr && io != nullptr, "params and io are required");
+  GJX_REQUIRE(params->dtype == GJX_F64, "the NUTS kernel runs in float64");
+  GJX_REQUIRE(params->num_chains > 0, "num_chains must be > 0 (got %d)", params->num_chains);
+  GJX_REQUIRE(params->dim >= 2, "the funnel needs dim >= 2 (got %d)", params->dim);
+  if (params->dim - 1 > 256) return fail(GJX_EUNSUPPORTED, "dim %d > 257 for the NUTS kernel", params->dim);
+  GJX_REQUIRE(max_num_doublings >= 1 && max_num_doublings <= 12, "max_num_doublings must be in [1, 12] (got %d)",
+              max_num_doublings);
+  GJX_REQUIRE(params->max_iters >= 0, "max_iters must be >= 0");
+  GJX_REQUIRE(std::isfinite(params->step_size), "step_size must be finite");
+  GJX_REQUIRE(std::isfinite(params->alpha2) && params->alpha2 >= 0.0, "alpha2 must be finite and >= 0");
+  GJX_REQUIRE(std::isfinite(params->sigma) && params->sigma > 0.0, "sigma must be finite and > 0");
+  GJX_REQUIRE(io->position && io->logdensity, "position and logdensity are required");
+  GJX_REQUIRE(io->new_position && io->new_logdensity, "new_position and new_logdensity are required");
+  const bool host = io->z != nullptr;
+  GJX_REQUIRE((io->u_direction != nullptr) == host && (io->u_merge != nullptr) == host &&
+                  (io->u_step != nullptr) == host,
+              "z, u_direction, u_merge and u_step must all be given or all be NULL");
+  gjx::FunnelArgs a = base_args(params);
+  a.T = 1;
+  a.q_in = io->position;
+  a.logp_in = io->logdensity;
+  a.z = io->z;
+  a.q_out = io->new_position;
+  a.logp_out = io->new_logdensity;
+  a.g_out = io->new_logdensity_grad;
+  a.momentum = io->momentum;
+  gjx::NutsArgs n;
+  std::memset(&n, 0, sizeof(n));
+  n.max_doublings = max_num_doublings;
+  n.u_dir = io->u_direction;
+  n.u_merge = io->u_merge;
+  n.u_step = io->u_step;
+  n.energy = io->energy;
+  n.acceptance_rate = io->acceptance_rate;
+  n.is_divergent = io->is_divergent;
+  n.is_turning = io->is_turning;
+  n.num_expansions = io->num_trajectory_expansions;
+  n.num_steps = io->num_integration_steps;
+  n.left_position = io->leftmost_position;
+  n.right_position = io->rightmost_position;
+  hipError_t e = gjx::launch_funnel_nuts(a, n, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "gjx_rmhmc_monge_funnel_nuts launch");
+  return GJX_OK;
+}
+
 int gjx_funnel_value_and_grad(int32_t num_chains, int32_t dim, double sigma, int32_t dtype, const void* position,
                               double* logdensity, void* logdensity_grad, void* stream) {
   GJX_REQUIRE(num_chains > 0, "num_chains must be > 0 (got %d)", num_chains);

@@ -143,7 +143,10 @@ __device__ __forceinline__ void normal2(u32x4 r, float& n0, float& n1) {
 enum : uint32_t {
   kStreamMomentum = 0x10000000u,   // + element index
   kStreamMomentumV = 0x20000000u,  // replicated last coordinate
-  kStreamAccept = 0x30000000u
+  kStreamAccept = 0x30000000u,
+  kStreamNutsDirection = 0x40000000u,  // + expansion index
+  kStreamNutsMerge = 0x50000000u,      // + expansion index
+  kStreamNutsStep = 0x60000000u        // + integration step index within the transition
 };
 
 template <class T> struct is_f64 { static constexpr bool value = false; };

@@ -1,0 +1,434 @@
+// NUTS on Neal's funnel with the Monge metric and the RMHMC implicit-midpoint
+// integrator, float64.
+//
+// Reference: geomjax/rmhmc/nuts.py:78-161,253-348 (kernel, iterative proposal),
+// geomjax/mcmc/trajectory.py:149-317 (dynamic_progressive_integration: a
+// subtree of up to 2^e steps in one direction, progressive uniform sampling,
+// stopping at divergence or an iterative U-turn), :517-668
+// (dynamic_multiplicative_expansion: random direction, biased merge, U-turn
+// of the merged trajectory), geomjax/mcmc/termination.py:30-119 (numpyro
+// checkpoints), geomjax/rmhmc/metrics.py:104-108 (criterion "euc": the
+// velocity sum against the end velocities).
+//
+// One wave per chain: every branch of the tree building is wave-uniform.  Lane
+// l holds x coordinates [l EPL, (l+1) EPL) of the D-1 "x" part; the log-scale
+// coordinate v is replicated.  The integrator, energies and velocities are the
+// closed forms of funnel_rmhmc.hip.  The U-turn checkpoints (velocity and
+// velocity sum at up to M leaves) live in LDS.  The trajectory keeps its two
+// ends, its momentum and velocity sums and its length; the subtree being built
+// keeps its last state, sums and length; proposals keep their position,
+// energy, weight and accumulated acceptance.
+#include "common.h"
+#include "funnel_field.h"
+#include "funnel_rmhmc.h"
+
+#include <math.h>
+
+namespace gjx {
+namespace nuts {
+
+template <int EPL>
+struct FState {
+  double x[EPL], px[EPL], ux[EPL];
+  double v, pv, uv;
+  double sx, xpx;  // |x|^2 and x . p_x (group totals)
+};
+
+template <int EPL>
+struct Prop {
+  double x[EPL];
+  double v;
+  double energy, weight, slpa;
+};
+
+__device__ __forceinline__ double logaddexp(double a, double b) {
+  const double m = fmax(a, b);
+  if (m == -INFINITY) return -INFINITY;
+  return m + log(exp(a - m) + exp(b - m));
+}
+
+}  // namespace nuts
+
+using namespace nuts;
+
+template <int EPL>
+__global__ void __launch_bounds__(64) funnel_nuts_kernel(const FunnelArgs a, const NutsArgs nu) {
+  extern __shared__ __align__(16) double ck[];  // [2][M][64 * EPL]: checkpoint velocities, velocity sums
+  const int c = blockIdx.x;
+  if (c >= a.C) return;
+  const int l = threadIdx.x;
+  const int D = a.D, Dx = a.D - 1;
+  const int j0 = l * EPL;
+  const int M = nu.max_doublings;
+  const int S = (1 << M) - 1;
+  const double a2 = a.alpha2;
+  const double alpha = sqrt(a.alpha2);
+  const double inv_s2 = 1.0 / (a.sigma * a.sigma);
+  const double halfDx = 0.5 * Dx;
+  const uint64_t tt = a.counter;
+  const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
+  const size_t base = (size_t)c * D;
+  auto on = [&](int k) { return j0 + k < Dx; };
+  auto uniform = [&](const double* host, size_t idx, uint32_t stream, uint32_t i) -> double {
+    if (host != nullptr) return host[idx];
+    u32x4 r = philox4x32_10(u32x4{(uint32_t)c, (uint32_t)tt, stream + i, (uint32_t)(tt >> 32)}, k0, k1);
+    return u01_f64(r.x, r.y) - 0.5 * (1.0 / 9007199254740992.0);
+  };
+
+  for (int i = l; i < 2 * M * 64 * EPL + 2 * M; i += 64) ck[i] = 0.0;
+  __syncthreads();
+
+  // ---- chain state and momentum p0 = chol(G) z (funnel_rmhmc.hip) ----
+  FState<EPL> s0;
+  double z[EPL], zv;
+#pragma unroll
+  for (int k = 0; k < EPL; ++k) s0.x[k] = on(k) ? static_cast<const double*>(a.q_in)[base + j0 + k] : 0.0;
+  s0.v = static_cast<const double*>(a.q_in)[base + Dx];
+  const double logp0 = a.logp_in[c];
+  if (a.z != nullptr) {
+#pragma unroll
+    for (int k = 0; k < EPL; ++k) z[k] = on(k) ? static_cast<const double*>(a.z)[base + j0 + k] : 0.0;
+    zv = static_cast<const double*>(a.z)[base + Dx];
+  } else {
+#pragma unroll
+    for (int k = 0; k < EPL; k += 2) {
+      u32x4 r = philox4x32_10(
+          u32x4{(uint32_t)c, (uint32_t)tt, kStreamMomentum + (uint32_t)(j0 + k), (uint32_t)(tt >> 32)}, k0, k1);
+      double n0, n1;
+      normal2(r, n0, n1);
+      z[k] = on(k) ? n0 : 0.0;
+      if (k + 1 < EPL) z[k + 1] = on(k + 1) ? n1 : 0.0;
+    }
+    u32x4 r = philox4x32_10(u32x4{(uint32_t)c, (uint32_t)tt, kStreamMomentumV, (uint32_t)(tt >> 32)}, k0, k1);
+    double n0, n1;
+    normal2(r, n0, n1);
+    zv = n0;
+  }
+  {
+    double loc_inc[EPL], acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < EPL; ++k) {
+      acc += s0.x[k] * s0.x[k];
+      loc_inc[k] = acc;
+    }
+    s0.sx = group_sum<64>(acc);
+    const double off1 = group_exclusive_scan<64>(acc, l);
+    const double e = exp(-s0.v);
+    const double a2e2 = a2 * e * e;
+    const double gv = -s0.v * inv_s2 - halfDx + 0.5 * e * s0.sx;
+    double cj[EPL], acc2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < EPL; ++k) {
+      const double tj = 1.0 + a2e2 * (off1 + loc_inc[k]);
+      const double tjm = 1.0 + a2e2 * (off1 + loc_inc[k] - s0.x[k] * s0.x[k]);
+      const double aj = -alpha * e * s0.x[k];
+      cj[k] = aj * z[k] / sqrt(tj * tjm);
+      s0.px[k] = sqrt(tj / tjm) * z[k];
+      acc2 += cj[k];
+    }
+    const double off2 = group_exclusive_scan<64>(acc2, l);
+    const double tot2 = group_sum<64>(acc2);
+    double run = off2;
+#pragma unroll
+    for (int k = 0; k < EPL; ++k) {
+      const double aj = -alpha * e * s0.x[k];
+      s0.px[k] = on(k) ? s0.px[k] + aj * run : 0.0;
+      run += cj[k];
+    }
+    const double txe = 1.0 + a2e2 * s0.sx;
+    const double tv = txe + a2 * gv * gv;
+    s0.pv = sqrt(tv / txe) * zv + alpha * gv * tot2;
+  }
+  if (a.momentum != nullptr) {
+#pragma unroll
+    for (int k = 0; k < EPL; ++k)
+      if (on(k)) static_cast<double*>(a.momentum)[base + j0 + k] = s0.px[k];
+    if (l == 0) static_cast<double*>(a.momentum)[base + Dx] = s0.pv;
+  }
+
+  // velocity u = G^-1 p at a state whose sums sx, xpx are current
+  auto set_velocity = [&](FState<EPL>& s) {
+    const FieldCoef<double> f = field_coef<double>(s.v, s.pv, s.sx, s.xpx, a2, inv_s2, halfDx);
+#pragma unroll
+    for (int k = 0; k < EPL; ++k) s.ux[k] = s.px[k] + f.ce * s.x[k];
+    s.uv = f.uv;
+  };
+  // kinetic energy T(q, p) = -log N(p; 0, G(q)) (geomjax/rmhmc/metrics.py:66-69) and
+  // the log density (examples/funnel/main.py:34-39) at a state with current sums
+  auto kinetic = [&](const FState<EPL>& s) -> double {
+    double d3 = 0.0;
+#pragma unroll
+    for (int k = 0; k < EPL; ++k) d3 += s.px[k] * s.px[k];
+    const double pp = group_sum<64>(d3) + s.pv * s.pv;
+    const double e = exp(-s.v);
+    const double gv = -s.v * inv_s2 - halfDx + 0.5 * e * s.sx;
+    const double gp = -e * s.xpx + gv * s.pv;
+    const double gg = e * e * s.sx + gv * gv;
+    const double cp = monge_perp_coef(gp, gg);
+    double d4 = 0.0;
+#pragma unroll
+    for (int k = 0; k < EPL; ++k) {
+      const double r = s.px[k] + cp * e * s.x[k];
+      d4 += r * r;
+    }
+    const double rv = s.pv - cp * gv;
+    const double perp2 = group_sum<64>(d4) + rv * rv;
+    return 0.5 * monge_quad(pp, perp2, gg, a2) + 0.5 * log(1.0 + a2 * gg) + 0.5 * (double)D * kLog2Pi;
+  };
+  auto logdensity = [&](double v, double sx) -> double {
+    return -0.5 * v * v * inv_s2 - log(a.sigma) - 0.5 * kLog2Pi - 0.5 * exp(-v) * sx - 0.5 * (double)Dx * v -
+           0.5 * (double)Dx * kLog2Pi;
+  };
+  auto dot_u = [&](const double (&ux)[EPL], double uv, const double (&wx)[EPL], double wv) -> double {
+    double d = 0.0;
+#pragma unroll
+    for (int k = 0; k < EPL; ++k) d += ux[k] * wx[k];
+    return group_sum<64>(d) + uv * wv;
+  };
+  // one implicit-midpoint step of size hs = direction * eps (integrators.py:92-156)
+  auto one_step = [&](FState<EPL>& s, double hstep) {
+    const double h = 0.5 * hstep;
+    double x0[EPL], p0[EPL];
+#pragma unroll
+    for (int k = 0; k < EPL; ++k) {
+      x0[k] = s.x[k];
+      p0[k] = s.px[k];
+    }
+    const double v0 = s.v, pv0 = s.pv;
+    int n = -1;
+    double norm = 0.0;
+    while (true) {
+      const FieldCoef<double> f = field_coef<double>(s.v, s.pv, s.sx, s.xpx, a2, inv_s2, halfDx);
+      double s1 = 0.0, s2 = 0.0, mx = 0.0;
+#pragma unroll
+      for (int k = 0; k < EPL; ++k) {
+        const double xn = x0[k] + h * (s.px[k] + f.ce * s.x[k]);
+        const double pn = p0[k] - h * (f.A * s.x[k] + f.B * s.px[k]);
+        mx = nan_max(mx, nan_max(fabs(xn - s.x[k]), fabs(pn - s.px[k])));
+        s1 += xn * xn;
+        s2 += xn * pn;
+        s.x[k] = xn;
+        s.px[k] = pn;
+      }
+      const double vn = v0 + h * f.uv;
+      const double pvn = pv0 - h * f.dHv;
+      mx = group_reduce<64, NanMax>(mx);
+      mx = nan_max(mx, nan_max(fabs(vn - s.v), fabs(pvn - s.pv)));
+      s.v = vn;
+      s.pv = pvn;
+      s.sx = group_sum<64>(s1);
+      s.xpx = group_sum<64>(s2);
+      norm = mx;
+      ++n;
+      if (!(n < a.max_iters && isfinite(norm) && norm < a.div_tol && norm > a.conv_tol)) break;
+    }
+    const FieldCoef<double> f = field_coef<double>(s.v, s.pv, s.sx, s.xpx, a2, inv_s2, halfDx);
+    double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < EPL; ++k) {
+      const double xo = s.x[k], po = s.px[k];
+      s.x[k] = xo + h * (po + f.ce * xo);
+      s.px[k] = po - h * (f.A * xo + f.B * po);
+      s1 += s.x[k] * s.x[k];
+      s2 += s.x[k] * s.px[k];
+    }
+    s.v = s.v + h * f.uv;
+    s.pv = s.pv - h * f.dHv;
+    s.sx = group_sum<64>(s1);
+    s.xpx = group_sum<64>(s2);
+    set_velocity(s);
+  };
+
+  {
+    double d2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < EPL; ++k) d2 += s0.x[k] * s0.px[k];
+    s0.xpx = group_sum<64>(d2);
+  }
+  set_velocity(s0);
+  // H0 with the chain's log density as given (the initial IntegratorState carries it)
+  const double H0 = -logp0 + kinetic(s0);
+
+  // ---- the trajectory: its two ends, sums (the start state included), length ----
+  FState<EPL> left = s0, right = s0;
+  double psx[EPL], vsx[EPL], psv = s0.pv, vsv = s0.uv;
+#pragma unroll
+  for (int k = 0; k < EPL; ++k) {
+    psx[k] = s0.px[k];
+    vsx[k] = s0.ux[k];
+  }
+  int ntraj = 0;
+  Prop<EPL> prop;
+#pragma unroll
+  for (int k = 0; k < EPL; ++k) prop.x[k] = s0.x[k];
+  prop.v = s0.v;
+  prop.energy = H0;
+  prop.weight = 0.0;
+  prop.slpa = -INFINITY;
+  auto set_prop = [&](Prop<EPL>& pr, const FState<EPL>& s, double en, double w) {
+#pragma unroll
+    for (int k = 0; k < EPL; ++k) pr.x[k] = s.x[k];
+    pr.v = s.v;
+    pr.energy = en;
+    pr.weight = w;
+    pr.slpa = fmin(w, 0.0);
+  };
+  double* const ckv = ck;                      // [M][64 EPL] velocities at the checkpoints
+  double* const cks = ck + M * 64 * EPL;       // [M][64 EPL] velocity sums at the checkpoints
+  int e = 0, j = 0;
+  bool diverging = false, turning = false;
+  while (e < M && !diverging && !turning) {
+    const double ud = uniform(nu.u_dir, (size_t)c * M + e, kStreamNutsDirection, (uint32_t)e);
+    const int dir = ud < 0.5 ? 1 : -1;
+    // ---- subtree of up to 2^e steps from the trajectory's end (trajectory.py:149-317) ----
+    FState<EPL> cur = dir > 0 ? right : left;
+    Prop<EPL> sp;
+    double ssx[EPL], ssv_x[EPL], ssp_v = 0.0, ssv_v = 0.0;
+#pragma unroll
+    for (int k = 0; k < EPL; ++k) {
+      ssx[k] = 0.0;
+      ssv_x[k] = 0.0;
+    }
+    int nsub = 0, step = 0;
+    const int max_steps = 1 << e;
+    bool div_s = false, term_s = false;
+    while (step < max_steps && !term_s && !div_s) {
+      one_step(cur, dir * a.eps);
+      const double en = -logdensity(cur.v, cur.sx) + kinetic(cur);
+      double w = H0 - en;
+      if (isnan(w)) w = -INFINITY;
+      div_s = -w > a.div_threshold;
+      const double us = uniform(nu.u_step, (size_t)c * S + j, kStreamNutsStep, (uint32_t)j);
+      if (step == 0) {
+#pragma unroll
+        for (int k = 0; k < EPL; ++k) {
+          ssx[k] = cur.px[k];
+          ssv_x[k] = cur.ux[k];
+        }
+        ssp_v = cur.pv;
+        ssv_v = cur.uv;
+        nsub = 1;
+        set_prop(sp, cur, en, w);
+      } else {
+#pragma unroll
+        for (int k = 0; k < EPL; ++k) {
+          ssx[k] += cur.px[k];
+          ssv_x[k] += cur.ux[k];
+        }
+        ssp_v += cur.pv;
+        ssv_v += cur.uv;
+        ++nsub;
+        // progressive uniform sampling (proposal.py:198-222)
+        const double dw = w - sp.weight;
+        const double pacc = 1.0 / (1.0 + exp(-dw));
+        const double nw = logaddexp(sp.weight, w), ns = logaddexp(sp.slpa, fmin(w, 0.0));
+        if (us < pacc) set_prop(sp, cur, en, w);
+        sp.weight = nw;
+        sp.slpa = ns;
+      }
+      ++j;
+      // numpyro checkpoints (termination.py:38-117): leaf index -> [idx_min, idx_max]
+      const int idx_max = __builtin_popcount((unsigned)step >> 1);
+      const int trailing = __builtin_ctz(~(unsigned)step);
+      const int idx_min = idx_max - trailing + 1;
+      if ((step & 1) == 0) {
+#pragma unroll
+        for (int k = 0; k < EPL; ++k) {
+          ckv[idx_max * 64 * EPL + l * EPL + k] = cur.ux[k];
+          cks[idx_max * 64 * EPL + l * EPL + k] = ssv_x[k];
+        }
+        if (l == 0) {  // the replicated v coordinate rides in lane 0's slot past the x entries
+          ckv[M * 64 * EPL * 2 + idx_max * 2] = cur.uv;
+          ckv[M * 64 * EPL * 2 + idx_max * 2 + 1] = ssv_v;
+        }
+      }
+      __syncthreads();  // one wave: orders the checkpoint writes before the reads below
+      for (int i = idx_max; i >= idx_min && !term_s; --i) {
+        double rho[EPL];
+#pragma unroll
+        for (int k = 0; k < EPL; ++k) rho[k] = ssv_x[k] - cks[i * 64 * EPL + l * EPL + k] + ckv[i * 64 * EPL + l * EPL + k];
+        const double ckv_v = ckv[M * 64 * EPL * 2 + i * 2], cks_v = ckv[M * 64 * EPL * 2 + i * 2 + 1];
+        const double rho_v = ssv_v - cks_v + ckv_v;
+        double lv[EPL];
+#pragma unroll
+        for (int k = 0; k < EPL; ++k) lv[k] = ckv[i * 64 * EPL + l * EPL + k];
+        term_s = dot_u(lv, ckv_v, rho, rho_v) <= 0.0 || dot_u(cur.ux, cur.uv, rho, rho_v) <= 0.0;
+      }
+      ++step;
+    }
+    // ---- merge (trajectory.py:580-660) ----
+    if (div_s || term_s) {
+      prop.slpa = logaddexp(prop.slpa, sp.slpa);
+    } else {
+      // progressive biased sampling (proposal.py:225-254)
+      const double um = uniform(nu.u_merge, (size_t)c * M + e, kStreamNutsMerge, (uint32_t)e);
+      const double pacc = fmin(exp(sp.weight - prop.weight), 1.0);
+      const double nw = logaddexp(prop.weight, sp.weight), ns = logaddexp(prop.slpa, sp.slpa);
+      if (um < pacc) prop = sp;
+      prop.weight = nw;
+      prop.slpa = ns;
+    }
+    if (dir > 0)
+      right = cur;
+    else
+      left = cur;
+#pragma unroll
+    for (int k = 0; k < EPL; ++k) {
+      psx[k] += ssx[k];
+      vsx[k] += ssv_x[k];
+    }
+    psv += ssp_v;
+    vsv += ssv_v;
+    ntraj += nsub;
+    const bool turn_full = dot_u(left.ux, left.uv, vsx, vsv) <= 0.0 || dot_u(right.ux, right.uv, vsx, vsv) <= 0.0;
+    ++e;
+    diverging = div_s;
+    turning = term_s || turn_full;
+  }
+  (void)psx;
+  (void)psv;
+
+  // ---- outputs: the sampled proposal's state (RMHMCState) and NUTSInfo ----
+  double sxp = 0.0;
+#pragma unroll
+  for (int k = 0; k < EPL; ++k) sxp += prop.x[k] * prop.x[k];
+  sxp = group_sum<64>(sxp);
+  const double ep = exp(-prop.v);
+#pragma unroll
+  for (int k = 0; k < EPL; ++k) {
+    if (!on(k)) continue;
+    static_cast<double*>(a.q_out)[base + j0 + k] = prop.x[k];
+    if (a.g_out) static_cast<double*>(a.g_out)[base + j0 + k] = -ep * prop.x[k];
+    if (nu.left_position) static_cast<double*>(nu.left_position)[base + j0 + k] = left.x[k];
+    if (nu.right_position) static_cast<double*>(nu.right_position)[base + j0 + k] = right.x[k];
+  }
+  if (l == 0) {
+    static_cast<double*>(a.q_out)[base + Dx] = prop.v;
+    if (a.g_out)
+      static_cast<double*>(a.g_out)[base + Dx] = -prop.v * inv_s2 - halfDx + 0.5 * ep * sxp;
+    if (nu.left_position) static_cast<double*>(nu.left_position)[base + Dx] = left.v;
+    if (nu.right_position) static_cast<double*>(nu.right_position)[base + Dx] = right.v;
+    a.logp_out[c] = logdensity(prop.v, sxp);
+    if (nu.energy) nu.energy[c] = prop.energy;
+    if (nu.acceptance_rate) nu.acceptance_rate[c] = exp(prop.slpa) / (double)ntraj;
+    if (nu.is_divergent) nu.is_divergent[c] = (uint8_t)diverging;
+    if (nu.is_turning) nu.is_turning[c] = (uint8_t)turning;
+    if (nu.num_expansions) nu.num_expansions[c] = e;
+    if (nu.num_steps) nu.num_steps[c] = ntraj;
+  }
+}
+
+hipError_t launch_funnel_nuts(const FunnelArgs& a, const NutsArgs& n, hipStream_t stream) {
+  const int epl = (a.D - 1 + 63) / 64;
+  const size_t smem = ((size_t)2 * n.max_doublings * 64 * (epl <= 1 ? 1 : (epl <= 2 ? 2 : 4)) +
+                       (size_t)2 * n.max_doublings) * sizeof(double);
+  if (epl <= 1)
+    hipLaunchKernelGGL(funnel_nuts_kernel<1>, dim3(a.C), dim3(64), smem, stream, a, n);
+  else if (epl <= 2)
+    hipLaunchKernelGGL(funnel_nuts_kernel<2>, dim3(a.C), dim3(64), smem, stream, a, n);
+  else
+    hipLaunchKernelGGL(funnel_nuts_kernel<4>, dim3(a.C), dim3(64), smem, stream, a, n);
+  return hipGetLastError();
+}
+
+}  // namespace gjx

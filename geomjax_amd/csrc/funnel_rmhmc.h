@@ -69,6 +69,24 @@ int funnel_pick_epl(int dim, int lpc);
 
 hipError_t launch_funnel_rmhmc(const FunnelArgs& a, int dtype, int lpc, int epl, hipStream_t stream);
 
+// NUTS (iterative, multinomial, numpyro-style U-turn with the "euc" criterion)
+struct NutsArgs {
+  int max_doublings;           // M
+  const double* u_dir;         // [C, M] or null (device PRNG)
+  const double* u_merge;       // [C, M]
+  const double* u_step;        // [C, 2^M - 1]
+  // outputs (optional unless noted)
+  double* energy;              // [C]
+  double* acceptance_rate;     // [C]
+  uint8_t* is_divergent;       // [C]
+  uint8_t* is_turning;         // [C]
+  int32_t* num_expansions;     // [C]
+  int32_t* num_steps;          // [C]
+  void* left_position;         // [C, D]
+  void* right_position;        // [C, D]
+};
+hipError_t launch_funnel_nuts(const FunnelArgs& a, const NutsArgs& n, hipStream_t stream);
+
 hipError_t launch_funnel_value_and_grad(int C, int D, double sigma, int dtype, const void* q, double* logp,
                                         void* grad, hipStream_t stream);
 

@@ -1,0 +1,133 @@
+"""NUTS with the RMHMC implicit-midpoint integrator (geomjax/rmhmc/nuts.py).
+
+Same API as the reference: ``nuts(logdensity_fn, step_size, metric_fn, *,
+max_num_doublings=10, divergence_threshold=1000, integrator=implicit_midpoint,
+stopping_criterion="euc")`` with ``init`` (= rmhmc.init) and ``step``; the
+kernel ``build_kernel(integrator, divergence_threshold)(rng_key, state,
+logdensity_fn, step_size, metric_fn, max_num_doublings, stopping_criterion)``.
+The whole transition (momentum draw, iterative tree doubling, progressive
+multinomial sampling, U-turn checks) runs in one HIP kernel, one wave per
+chain, float64; served for Neal's funnel with the Monge metric and the "euc"
+criterion (the reference's default).
+
+``rng_key`` is a ``PRNGKey`` (device Philox) or ``NUTSNoise(z, u_direction,
+u_merge, u_step)`` with the draws the reference takes from its keys: the
+momentum normals [C, D], per expansion e the direction uniform (+1 when
+< 0.5) and the biased-merge uniform [C, M], and per integration step j of the
+transition the progressive-sampling uniform [C, 2^M - 1].
+
+``NUTSInfo.trajectory_leftmost_state`` / ``trajectory_rightmost_state`` carry
+the positions of the trajectory's two ends (the reference keeps the whole
+IntegratorState there).
+"""
+import functools
+from typing import NamedTuple
+
+import torch
+
+from . import _lib
+from .metrics import MongeMetric
+from .models import NealFunnel
+from .random import PRNGKey
+from .rmhmc import RMHMCState, _params, _solver_kwargs, implicit_midpoint, init
+
+
+class NUTSNoise(NamedTuple):
+    z: torch.Tensor            # [C, D]
+    u_direction: torch.Tensor  # [C, M]
+    u_merge: torch.Tensor      # [C, M]
+    u_step: torch.Tensor       # [C, 2^M - 1]
+
+
+class NUTSInfo(NamedTuple):  # geomjax/rmhmc/nuts.py:36-75 (same fields and order)
+    momentum: torch.Tensor
+    is_divergent: torch.Tensor
+    is_turning: torch.Tensor
+    energy: torch.Tensor
+    trajectory_leftmost_state: torch.Tensor
+    trajectory_rightmost_state: torch.Tensor
+    num_trajectory_expansions: torch.Tensor
+    num_integration_steps: torch.Tensor
+    acceptance_rate: torch.Tensor
+
+
+def build_kernel(integrator=implicit_midpoint, divergence_threshold: int = 1000):
+    """geomjax/rmhmc/nuts.py:78-161."""
+    solver = _solver_kwargs(integrator)
+
+    def kernel(rng_key, state, logdensity_fn, step_size, metric_fn, max_num_doublings: int = 10,
+               stopping_criterion: str = "euc"):
+        if stopping_criterion != "euc":
+            raise NotImplementedError("the NUTS kernel implements the 'euc' U-turn criterion (the default)")
+        if not isinstance(logdensity_fn, NealFunnel) or not isinstance(metric_fn, MongeMetric):
+            raise NotImplementedError("the NUTS kernel serves Neal's funnel with the Monge metric")
+        position = state.position
+        single = position.dim() == 1
+        if single:
+            position = position[None]
+        position = position.contiguous()
+        if position.dtype != torch.float64 or not position.is_cuda:
+            raise ValueError("NUTS positions must be float64 tensors on a HIP device")
+        C, D = position.shape
+        if D != logdensity_fn.D:
+            raise ValueError(f"position has {D} coordinates, the model has D={logdensity_fn.D}")
+        dev = position.device
+        M = int(max_num_doublings)
+        logdensity = state.logdensity.reshape(C).to(torch.float64).contiguous()
+        if isinstance(rng_key, NUTSNoise):
+            f = lambda t, *shape: t.reshape(*shape).to(device=dev, dtype=torch.float64).contiguous()
+            noise = (f(rng_key.z, C, D), f(rng_key.u_direction, C, M), f(rng_key.u_merge, C, M),
+                     f(rng_key.u_step, C, 2 ** M - 1))
+            seed, counter = 0, 0
+        elif isinstance(rng_key, PRNGKey):
+            noise = (None, None, None, None)
+            seed, counter = rng_key.seed, rng_key.counter
+        else:
+            raise TypeError("rng_key must be NUTSNoise or PRNGKey")
+        vec = lambda: torch.empty(C, D, dtype=torch.float64, device=dev)
+        sca = lambda dt=torch.float64: torch.empty(C, dtype=dt, device=dev)
+        out = dict(new_position=vec(), new_logdensity=sca(), new_logdensity_grad=vec(), momentum=vec(),
+                   energy=sca(), acceptance_rate=sca(), is_divergent=sca(torch.uint8), is_turning=sca(torch.uint8),
+                   num_trajectory_expansions=sca(torch.int32), num_integration_steps=sca(torch.int32),
+                   leftmost_position=vec(), rightmost_position=vec())
+        io = _lib.NutsIO(position=position.data_ptr(), logdensity=logdensity.data_ptr(), z=_lib.ptr(noise[0]),
+                         u_direction=_lib.ptr(noise[1]), u_merge=_lib.ptr(noise[2]), u_step=_lib.ptr(noise[3]),
+                         **{k: v.data_ptr() for k, v in out.items()})
+        params = _params(C, D, _lib.GJX_F64, step_size, 0, solver, metric_fn.alpha2, logdensity_fn.sigma,
+                         divergence_threshold, seed=seed, counter=counter)
+        rc = _lib.load().gjx_rmhmc_monge_funnel_nuts(params, M, io, _lib.stream_handle(dev))
+        _lib.check(rc, "gjx_rmhmc_monge_funnel_nuts")
+        sq = (lambda t: t[0]) if single else (lambda t: t)
+        new_state = RMHMCState(sq(out["new_position"]), sq(out["new_logdensity"]), sq(out["new_logdensity_grad"]))
+        info = NUTSInfo(sq(out["momentum"]), sq(out["is_divergent"].bool()), sq(out["is_turning"].bool()),
+                        sq(out["energy"]), sq(out["leftmost_position"]), sq(out["rightmost_position"]),
+                        sq(out["num_trajectory_expansions"]), sq(out["num_integration_steps"]),
+                        sq(out["acceptance_rate"]))
+        return new_state, info
+
+    return kernel
+
+
+class nuts:
+    """geomjax/rmhmc/nuts.py:164-250."""
+
+    init = staticmethod(init)
+    build_kernel = staticmethod(build_kernel)
+
+    def __new__(cls, logdensity_fn, step_size, metric_fn, *, max_num_doublings: int = 10,
+                divergence_threshold: int = 1000, integrator=implicit_midpoint, stopping_criterion: str = "euc"):
+        from .base import SamplingAlgorithm
+
+        kernel = cls.build_kernel(integrator, divergence_threshold)
+
+        def init_fn(position):
+            return cls.init(position, logdensity_fn)
+
+        def step_fn(rng_key, state):
+            return kernel(rng_key, state, logdensity_fn, step_size, metric_fn, max_num_doublings,
+                          stopping_criterion=stopping_criterion)
+
+        return SamplingAlgorithm(init_fn, step_fn)
+
+
+__all__ = ["nuts", "build_kernel", "NUTSInfo", "NUTSNoise", "init"]

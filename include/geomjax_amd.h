@@ -382,6 +382,42 @@ int gjx_rmhmc_monge_funnel_tangent(const gjx_funnel_params* params, const gjx_ta
 int gjx_rmhmc_fisher_logistic_tangent(const gjx_logistic_params* params, const gjx_logistic_data* data,
                                       const gjx_tangent_io* io, void* stream);
 
+/* ---------------------------------------------------------------------------
+ * NUTS with the RMHMC implicit-midpoint integrator (geomjax/rmhmc/nuts.py:78-161,
+ * 253-348): iterative trajectory doubling in a random direction, progressive
+ * multinomial sampling (uniform inside a subtree, biased at the merge), the
+ * numpyro-style iterative U-turn check with the "euc" criterion.  Monge metric
+ * on Neal's funnel, float64, one wave per chain, D - 1 <= 256, M =
+ * max_num_doublings in [1, 12] (the trajectory holds at most 2^M - 1 steps
+ * past its start).  The random
+ * draws of one transition per chain: z (momentum), u_direction[e] and u_merge[e]
+ * for expansion e < M, u_step[j] for the j-th integration step (all NULL:
+ * device PRNG at (params->seed, params->counter)).
+ * ------------------------------------------------------------------------- */
+typedef struct {
+  const void* position;          /* [C, D] */
+  const double* logdensity;      /* [C] */
+  const void* z;                 /* [C, D] or NULL */
+  const double* u_direction;     /* [C, M] or NULL */
+  const double* u_merge;         /* [C, M] or NULL */
+  const double* u_step;          /* [C, 2^M - 1] or NULL */
+  void* new_position;            /* [C, D] */
+  double* new_logdensity;        /* [C] */
+  void* new_logdensity_grad;     /* [C, D] optional */
+  void* momentum;                /* [C, D] optional: NUTSInfo.momentum */
+  double* energy;                /* [C] optional */
+  double* acceptance_rate;       /* [C] optional */
+  uint8_t* is_divergent;         /* [C] optional */
+  uint8_t* is_turning;           /* [C] optional */
+  int32_t* num_trajectory_expansions; /* [C] optional */
+  int32_t* num_integration_steps;     /* [C] optional: states in the trajectory */
+  void* leftmost_position;       /* [C, D] optional */
+  void* rightmost_position;      /* [C, D] optional */
+} gjx_nuts_io;
+
+int gjx_rmhmc_monge_funnel_nuts(const gjx_funnel_params* params, int32_t max_num_doublings, const gjx_nuts_io* io,
+                                void* stream);
+
 /* Log density and its gradient of Neal's funnel for C positions: the
  * jax.value_and_grad(logdensity_fn)(position) of the reference's init
  * (geomjax/rmhmc/rmhmc.py:96-98, geomjax/lmcmonge/lmc.py:101-109) with the

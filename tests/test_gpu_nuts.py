@@ -1,0 +1,100 @@
+"""GPU: NUTS with the RMHMC integrator (geomjax_amd.nuts) on Neal's funnel.
+
+* parity with the float64 oracle (oracle/nuts.py) fed the same draws: new
+  position, log density, energy, acceptance statistic and the trajectory's
+  ends within 1e-10 norm-wise relative; numbers of integration steps and of
+  expansions, and the turning / divergence flags identical;
+* stationarity with the device PRNG: chains started from exact draws of the
+  funnel (v ~ N(0, 9), x_i | v ~ N(0, e^v)) keep v's mean and variance after
+  ten transitions (within 5 standard errors).
+"""
+import functools
+
+import pytest
+import torch
+
+import geomjax_amd as gj
+from geomjax_amd import random as gr
+from geomjax_amd.nuts import NUTSNoise
+from geomjax_amd.nuts import build_kernel as nuts_kernel
+from geomjax_amd.rmhmc import RMHMCState, implicit_midpoint
+from oracle import models as om
+from oracle import nuts as onuts
+from oracle import rmhmc as orm
+from tests.helpers import rel_err, seeded_normal, seeded_uniform
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+
+
+@pytest.mark.parametrize("D,C,eps,M,T,seed", [
+    (10, 4, 0.1, 5, 3, 0),     # the 10-dimensional funnel, three chained transitions
+    (2, 6, 0.3, 6, 2, 1),      # minimal funnel, long trajectories
+    (100, 2, 0.05, 4, 1, 2),   # two coordinates per lane
+])
+def test_nuts_matches_oracle(D, C, eps, M, T, seed):
+    print(f"case D={D} C={C} eps={eps} M={M} T={T}", flush=True)
+    model = gj.NealFunnel(D)
+    Mo = om.NealFunnel(D)
+    metric = om.monge_metric_fn(Mo.logp, 1.0, D)
+    q0 = seeded_normal(10 + seed, C, D)
+    so = orm.init(q0, Mo.logp)
+    s = RMHMCState(so.position.to(DEV), so.logdensity.to(DEV), so.logdensity_grad.to(DEV))
+    kern = nuts_kernel(functools.partial(implicit_midpoint, max_iters=8))
+    okern = onuts.build_kernel(max_iters=8)
+    for t in range(T):
+        noise = (seeded_normal(20 + 10 * seed + t, C, D), seeded_uniform(30 + 10 * seed + t, C, M),
+                 seeded_uniform(40 + 10 * seed + t, C, M), seeded_uniform(50 + 10 * seed + t, C, 2 ** M - 1))
+        s1, info = kern(NUTSNoise(*(x.to(DEV) for x in noise)), s, model, eps, gj.MongeMetric(1.0), M)
+        so1, oi = okern(noise, so, Mo.logp, eps, metric, max_num_doublings=M)
+        torch.cuda.synchronize()
+        steps = [o.num_integration_steps for o in oi]
+        print(f"  t={t}: steps {steps}, turning {[o.is_turning for o in oi]}", flush=True)
+        assert info.num_integration_steps.cpu().tolist() == steps
+        assert info.num_trajectory_expansions.cpu().tolist() == [o.num_trajectory_expansions for o in oi]
+        assert info.is_turning.cpu().tolist() == [o.is_turning for o in oi]
+        assert info.is_divergent.cpu().tolist() == [o.is_divergent for o in oi]
+        assert rel_err(s1.position, so1.position) < 1e-10
+        assert rel_err(s1.logdensity, so1.logdensity) < 1e-10
+        assert rel_err(s1.logdensity_grad, so1.logdensity_grad) < 1e-10
+        assert rel_err(info.energy, torch.tensor([o.energy for o in oi], dtype=torch.float64)) < 1e-10
+        assert rel_err(info.acceptance_rate, torch.tensor([o.acceptance_rate for o in oi], dtype=torch.float64)) < 1e-10
+        left = torch.cat([o.trajectory_leftmost_state.position for o in oi])
+        right = torch.cat([o.trajectory_rightmost_state.position for o in oi])
+        for c, o in enumerate(oi):
+            el = rel_err(info.trajectory_leftmost_state[c], left[c])
+            er = rel_err(info.trajectory_rightmost_state[c], right[c])
+            print(f"    chain {c}: divergent {o.is_divergent}, ends {el:.1e} {er:.1e}", flush=True)
+            # The last state of a divergent trajectory is where the energy error
+            # passed the threshold: the fixed-point iterates that produced it
+            # run away, which amplifies rounding differences.
+            bar = 1e-3 if o.is_divergent else 1e-10
+            assert el < bar and er < bar, c
+        assert rel_err(info.momentum, torch.stack([o.momentum for o in oi])) < 1e-10
+        so = so1
+        s = RMHMCState(so.position.to(DEV), so.logdensity.to(DEV), so.logdensity_grad.to(DEV))
+
+
+def test_nuts_keeps_the_funnel_stationary():
+    D, C = 10, 4096
+    g = torch.Generator().manual_seed(77)
+    v = 3.0 * torch.randn(C, 1, generator=g, dtype=torch.float64)
+    x = torch.randn(C, D - 1, generator=g, dtype=torch.float64) * torch.exp(0.5 * v)
+    q = torch.cat([x, v], dim=1).to(DEV)
+    model = gj.NealFunnel(D)
+    alg = gj.nuts(model, 0.2, gj.MongeMetric(1.0), max_num_doublings=6,
+                  integrator=functools.partial(implicit_midpoint, max_iters=8))
+    s = alg.init(q)
+    accs, steps = [], []
+    for t in range(10):
+        s, info = alg.step(gr.key(500 + t), s)
+        accs.append(float(info.acceptance_rate.mean()))
+        steps.append(float(info.num_integration_steps.double().mean()))
+    torch.cuda.synchronize()
+    vv = s.position[:, -1].double().cpu()
+    m, var = float(vv.mean()), float(vv.var())
+    print(f"v mean {m:.3f} (0), var {var:.3f} (9), acceptance {sum(accs) / 10:.3f}, mean steps {sum(steps) / 10:.1f}")
+    assert torch.isfinite(s.position).all()
+    se_m, se_v = 3.0 / C ** 0.5, 9.0 * (2.0 / C) ** 0.5
+    assert abs(m) < 5 * se_m
+    assert abs(var - 9.0) < 5 * se_v
