@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "lib
 
 GJX_F64 = 0
 GJX_F32 = 1
+GJX_UTURN = {"euc": 0, "bet": 1, "riem": 2}  # stopping criteria of the NUTS entry point
 
 _c_int32 = ctypes.c_int32
 _c_double = ctypes.c_double
@@ -159,7 +160,7 @@ _SIGNATURES = {
     "gjx_lmcmonge_funnel_step": [ctypes.POINTER(FunnelParams), ctypes.POINTER(LmcStepIO), _vp],
     "gjx_lmcmonge_funnel_sample": [ctypes.POINTER(FunnelParams), ctypes.POINTER(LmcSampleIO), _vp],
     "gjx_rmhmc_monge_funnel_tangent": [ctypes.POINTER(FunnelParams), ctypes.POINTER(TangentIO), _vp],
-    "gjx_rmhmc_monge_funnel_nuts": [ctypes.POINTER(FunnelParams), _c_int32, ctypes.POINTER(NutsIO), _vp],
+    "gjx_rmhmc_monge_funnel_nuts": [ctypes.POINTER(FunnelParams), _c_int32, _c_int32, ctypes.POINTER(NutsIO), _vp],
     "gjx_funnel_value_and_grad": [_c_int32, _c_int32, _c_double, _c_int32, _vp, _vp, _vp, _vp],
     "gjx_rmhmc_fisher_logistic_step": [ctypes.POINTER(LogisticParams), ctypes.POINTER(LogisticData),
                                        ctypes.POINTER(RmhmcStepIO), _vp],

@@ -195,8 +195,8 @@ int gjx_rmhmc_monge_funnel_tangent(const gjx_funnel_params* params, const gjx_ta
   return GJX_OK;
 }
 
-int gjx_rmhmc_monge_funnel_nuts(const gjx_funnel_params* params, int32_t max_num_doublings, const gjx_nuts_io* io,
-                                void* stream) {
+int gjx_rmhmc_monge_funnel_nuts(const gjx_funnel_params* params, int32_t max_num_doublings,
+                                int32_t stopping_criterion, const gjx_nuts_io* io, void* stream) {
   GJX_REQUIRE(params != nullptr && io != nullptr, "params and io are required");
   GJX_REQUIRE(params->dtype == GJX_F64, "the NUTS kernel runs in float64");
   GJX_REQUIRE(params->num_chains > 0, "num_chains must be > 0 (got %d)", params->num_chains);
@@ -204,6 +204,10 @@ int gjx_rmhmc_monge_funnel_nuts(const gjx_funnel_params* params, int32_t max_num
   if (params->dim - 1 > 256) return fail(GJX_EUNSUPPORTED, "dim %d > 257 for the NUTS kernel", params->dim);
   GJX_REQUIRE(max_num_doublings >= 1 && max_num_doublings <= 12, "max_num_doublings must be in [1, 12] (got %d)",
               max_num_doublings);
+  GJX_REQUIRE(stopping_criterion == GJX_UTURN_EUC || stopping_criterion == GJX_UTURN_BET ||
+                  stopping_criterion == GJX_UTURN_RIEM,
+              "stopping_criterion must be GJX_UTURN_EUC, GJX_UTURN_BET or GJX_UTURN_RIEM (got %d)",
+              stopping_criterion);
   GJX_REQUIRE(params->max_iters >= 0, "max_iters must be >= 0");
   GJX_REQUIRE(std::isfinite(params->step_size), "step_size must be finite");
   GJX_REQUIRE(std::isfinite(params->alpha2) && params->alpha2 >= 0.0, "alpha2 must be finite and >= 0");
@@ -226,6 +230,7 @@ int gjx_rmhmc_monge_funnel_nuts(const gjx_funnel_params* params, int32_t max_num
   gjx::NutsArgs n;
   std::memset(&n, 0, sizeof(n));
   n.max_doublings = max_num_doublings;
+  n.criterion = stopping_criterion;
   n.u_dir = io->u_direction;
   n.u_merge = io->u_merge;
   n.u_step = io->u_step;

@@ -7,14 +7,16 @@
 // stopping at divergence or an iterative U-turn), :517-668
 // (dynamic_multiplicative_expansion: random direction, biased merge, U-turn
 // of the merged trajectory), geomjax/mcmc/termination.py:30-119 (numpyro
-// checkpoints), geomjax/rmhmc/metrics.py:104-108 (criterion "euc": the
-// velocity sum against the end velocities).
+// checkpoints), geomjax/rmhmc/metrics.py:100-117 (the U-turn criteria: "euc"
+// the velocity sum against the end velocities, "bet" the momentum sum against
+// the end velocities, "riem" the velocity sum against the end momenta).
 //
 // One wave per chain: every branch of the tree building is wave-uniform.  Lane
 // l holds x coordinates [l EPL, (l+1) EPL) of the D-1 "x" part; the log-scale
 // coordinate v is replicated.  The integrator, energies and velocities are the
 // closed forms of funnel_rmhmc.hip.  The U-turn checkpoints (velocity and
-// velocity sum at up to M leaves) live in LDS.  The trajectory keeps its two
+// velocity sum at up to M leaves, and momentum and momentum sum for "bet" and
+// "riem") live in LDS.  The trajectory keeps its two
 // ends, its momentum and velocity sums and its length; the subtree being built
 // keeps its last state, sums and length; proposals keep their position,
 // energy, weight and accumulated acceptance.
@@ -53,7 +55,9 @@ using namespace nuts;
 
 template <int EPL>
 __global__ void __launch_bounds__(64) funnel_nuts_kernel(const FunnelArgs a, const NutsArgs nu) {
-  extern __shared__ __align__(16) double ck[];  // [2][M][64 * EPL]: checkpoint velocities, velocity sums
+  // checkpoints of the iterative U-turn check: [NA][M][64 EPL] coordinates x, then [NA][M] for v;
+  // arrays 0, 1 = velocity, velocity sum; 2, 3 = momentum, momentum sum ("bet", "riem")
+  extern __shared__ __align__(16) double ck[];
   const int c = blockIdx.x;
   if (c >= a.C) return;
   const int l = threadIdx.x;
@@ -75,7 +79,10 @@ __global__ void __launch_bounds__(64) funnel_nuts_kernel(const FunnelArgs a, con
     return u01_f64(r.x, r.y) - 0.5 * (1.0 / 9007199254740992.0);
   };
 
-  for (int i = l; i < 2 * M * 64 * EPL + 2 * M; i += 64) ck[i] = 0.0;
+  const int crit = nu.criterion;
+  const int NA = nuts_checkpoint_arrays(crit);
+  const int XS = M * 64 * EPL;  // one checkpoint array's x part
+  for (int i = l; i < NA * (XS + M); i += 64) ck[i] = 0.0;
   __syncthreads();
 
   // ---- chain state and momentum p0 = chol(G) z (funnel_rmhmc.hip) ----
@@ -185,6 +192,26 @@ __global__ void __launch_bounds__(64) funnel_nuts_kernel(const FunnelArgs a, con
     for (int k = 0; k < EPL; ++k) d += ux[k] * wx[k];
     return group_sum<64>(d) + uv * wv;
   };
+  // the U-turn criteria (geomjax/rmhmc/metrics.py:100-117) on the two ends' momenta ml, mr
+  // and velocities vl, vr and the sums ms, vs over the states between them; every dot
+  // product is a group total, so the result is the same in every lane
+  auto is_turning = [&](const double (&mlx)[EPL], double mlv, const double (&mrx)[EPL], double mrv,
+                        const double (&msx)[EPL], double msv, const double (&vlx)[EPL], double vlv,
+                        const double (&vrx)[EPL], double vrv, const double (&vsx)[EPL], double vsv) -> bool {
+    if (crit == 1) return dot_u(vlx, vlv, msx, msv) <= 0.0 || dot_u(vrx, vrv, msx, msv) <= 0.0;  // "bet"
+    if (crit == 2) return dot_u(mlx, mlv, vsx, vsv) <= 0.0 || dot_u(mrx, mrv, vsx, vsv) <= 0.0;  // "riem"
+    return dot_u(vlx, vlv, vsx, vsv) <= 0.0 || dot_u(vrx, vrv, vsx, vsv) <= 0.0;                 // "euc"
+  };
+  auto ck_put = [&](int arr, int idx, const double (&x)[EPL], double v) {
+#pragma unroll
+    for (int k = 0; k < EPL; ++k) ck[arr * XS + idx * 64 * EPL + l * EPL + k] = x[k];
+    if (l == 0) ck[NA * XS + arr * M + idx] = v;  // v is replicated: lane 0 stores it
+  };
+  auto ck_get = [&](int arr, int idx, double (&x)[EPL]) -> double {
+#pragma unroll
+    for (int k = 0; k < EPL; ++k) x[k] = ck[arr * XS + idx * 64 * EPL + l * EPL + k];
+    return ck[NA * XS + arr * M + idx];
+  };
   // one implicit-midpoint step of size hs = direction * eps (integrators.py:92-156)
   auto one_step = [&](FState<EPL>& s, double hstep) {
     const double h = 0.5 * hstep;
@@ -273,8 +300,6 @@ __global__ void __launch_bounds__(64) funnel_nuts_kernel(const FunnelArgs a, con
     pr.weight = w;
     pr.slpa = fmin(w, 0.0);
   };
-  double* const ckv = ck;                      // [M][64 EPL] velocities at the checkpoints
-  double* const cks = ck + M * 64 * EPL;       // [M][64 EPL] velocity sums at the checkpoints
   int e = 0, j = 0;
   bool diverging = false, turning = false;
   while (e < M && !diverging && !turning) {
@@ -332,27 +357,34 @@ __global__ void __launch_bounds__(64) funnel_nuts_kernel(const FunnelArgs a, con
       const int trailing = __builtin_ctz(~(unsigned)step);
       const int idx_min = idx_max - trailing + 1;
       if ((step & 1) == 0) {
-#pragma unroll
-        for (int k = 0; k < EPL; ++k) {
-          ckv[idx_max * 64 * EPL + l * EPL + k] = cur.ux[k];
-          cks[idx_max * 64 * EPL + l * EPL + k] = ssv_x[k];
-        }
-        if (l == 0) {  // the replicated v coordinate rides in lane 0's slot past the x entries
-          ckv[M * 64 * EPL * 2 + idx_max * 2] = cur.uv;
-          ckv[M * 64 * EPL * 2 + idx_max * 2 + 1] = ssv_v;
+        ck_put(0, idx_max, cur.ux, cur.uv);
+        ck_put(1, idx_max, ssv_x, ssv_v);
+        if (NA == 4) {
+          ck_put(2, idx_max, cur.px, cur.pv);
+          ck_put(3, idx_max, ssx, ssp_v);
         }
       }
       __syncthreads();  // one wave: orders the checkpoint writes before the reads below
       for (int i = idx_max; i >= idx_min && !term_s; --i) {
-        double rho[EPL];
+        // the sub-subtree from checkpoint i to the current state (termination.py:89-117)
+        double vl[EPL], tmp[EPL], vs[EPL], ml[EPL], ms[EPL];
+        const double vlv = ck_get(0, i, vl);
+        const double tv = ck_get(1, i, tmp);
 #pragma unroll
-        for (int k = 0; k < EPL; ++k) rho[k] = ssv_x[k] - cks[i * 64 * EPL + l * EPL + k] + ckv[i * 64 * EPL + l * EPL + k];
-        const double ckv_v = ckv[M * 64 * EPL * 2 + i * 2], cks_v = ckv[M * 64 * EPL * 2 + i * 2 + 1];
-        const double rho_v = ssv_v - cks_v + ckv_v;
-        double lv[EPL];
+        for (int k = 0; k < EPL; ++k) vs[k] = ssv_x[k] - tmp[k] + vl[k];
+        const double vsv = ssv_v - tv + vlv;
+        double mlv = 0.0, msv = 0.0;
+        if (NA == 4) {
+          mlv = ck_get(2, i, ml);
+          const double tm = ck_get(3, i, tmp);
 #pragma unroll
-        for (int k = 0; k < EPL; ++k) lv[k] = ckv[i * 64 * EPL + l * EPL + k];
-        term_s = dot_u(lv, ckv_v, rho, rho_v) <= 0.0 || dot_u(cur.ux, cur.uv, rho, rho_v) <= 0.0;
+          for (int k = 0; k < EPL; ++k) ms[k] = ssx[k] - tmp[k] + ml[k];
+          msv = ssp_v - tm + mlv;
+        } else {
+#pragma unroll
+          for (int k = 0; k < EPL; ++k) ml[k] = ms[k] = 0.0;
+        }
+        term_s = is_turning(ml, mlv, cur.px, cur.pv, ms, msv, vl, vlv, cur.ux, cur.uv, vs, vsv);
       }
       ++step;
     }
@@ -380,13 +412,12 @@ __global__ void __launch_bounds__(64) funnel_nuts_kernel(const FunnelArgs a, con
     psv += ssp_v;
     vsv += ssv_v;
     ntraj += nsub;
-    const bool turn_full = dot_u(left.ux, left.uv, vsx, vsv) <= 0.0 || dot_u(right.ux, right.uv, vsx, vsv) <= 0.0;
+    const bool turn_full = is_turning(left.px, left.pv, right.px, right.pv, psx, psv, left.ux, left.uv, right.ux,
+                                      right.uv, vsx, vsv);
     ++e;
     diverging = div_s;
     turning = term_s || turn_full;
   }
-  (void)psx;
-  (void)psv;
 
   // ---- outputs: the sampled proposal's state (RMHMCState) and NUTSInfo ----
   double sxp = 0.0;
@@ -420,15 +451,22 @@ __global__ void __launch_bounds__(64) funnel_nuts_kernel(const FunnelArgs a, con
 
 hipError_t launch_funnel_nuts(const FunnelArgs& a, const NutsArgs& n, hipStream_t stream) {
   const int epl = (a.D - 1 + 63) / 64;
-  const size_t smem = ((size_t)2 * n.max_doublings * 64 * (epl <= 1 ? 1 : (epl <= 2 ? 2 : 4)) +
-                       (size_t)2 * n.max_doublings) * sizeof(double);
-  if (epl <= 1)
-    hipLaunchKernelGGL(funnel_nuts_kernel<1>, dim3(a.C), dim3(64), smem, stream, a, n);
-  else if (epl <= 2)
-    hipLaunchKernelGGL(funnel_nuts_kernel<2>, dim3(a.C), dim3(64), smem, stream, a, n);
-  else
-    hipLaunchKernelGGL(funnel_nuts_kernel<4>, dim3(a.C), dim3(64), smem, stream, a, n);
-  return hipGetLastError();
+  const int na = nuts_checkpoint_arrays(n.criterion);
+  const size_t smem = ((size_t)na * n.max_doublings * 64 * (epl <= 1 ? 1 : (epl <= 2 ? 2 : 4)) +
+                       (size_t)na * n.max_doublings) * sizeof(double);
+  // up to 4 x 12 x 256 x 8 B = 96 KiB of checkpoints: past the 64 KiB default of dynamic LDS
+  auto launch = [&](auto kern) {
+    if (smem > 65536) {
+      const hipError_t e =
+          hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+      if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(kern, dim3(a.C), dim3(64), smem, stream, a, n);
+    return hipGetLastError();
+  };
+  if (epl <= 1) return launch(funnel_nuts_kernel<1>);
+  if (epl <= 2) return launch(funnel_nuts_kernel<2>);
+  return launch(funnel_nuts_kernel<4>);
 }
 
 }  // namespace gjx

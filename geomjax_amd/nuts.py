@@ -7,8 +7,9 @@ kernel ``build_kernel(integrator, divergence_threshold)(rng_key, state,
 logdensity_fn, step_size, metric_fn, max_num_doublings, stopping_criterion)``.
 The whole transition (momentum draw, iterative tree doubling, progressive
 multinomial sampling, U-turn checks) runs in one HIP kernel, one wave per
-chain, float64; served for Neal's funnel with the Monge metric and the "euc"
-criterion (the reference's default).
+chain, float64; served for Neal's funnel with the Monge metric and the three
+U-turn criteria of geomjax/rmhmc/metrics.py:100-117 ("euc", the default;
+"bet"; "riem").
 
 ``rng_key`` is a ``PRNGKey`` (device Philox) or ``NUTSNoise(z, u_direction,
 u_merge, u_step)`` with the draws the reference takes from its keys: the
@@ -57,8 +58,8 @@ def build_kernel(integrator=implicit_midpoint, divergence_threshold: int = 1000)
 
     def kernel(rng_key, state, logdensity_fn, step_size, metric_fn, max_num_doublings: int = 10,
                stopping_criterion: str = "euc"):
-        if stopping_criterion != "euc":
-            raise NotImplementedError("the NUTS kernel implements the 'euc' U-turn criterion (the default)")
+        if stopping_criterion not in _lib.GJX_UTURN:
+            raise ValueError(f"stopping_criterion must be one of {sorted(_lib.GJX_UTURN)} (got {stopping_criterion!r})")
         if not isinstance(logdensity_fn, NealFunnel) or not isinstance(metric_fn, MongeMetric):
             raise NotImplementedError("the NUTS kernel serves Neal's funnel with the Monge metric")
         position = state.position
@@ -95,7 +96,8 @@ def build_kernel(integrator=implicit_midpoint, divergence_threshold: int = 1000)
                          **{k: v.data_ptr() for k, v in out.items()})
         params = _params(C, D, _lib.GJX_F64, step_size, 0, solver, metric_fn.alpha2, logdensity_fn.sigma,
                          divergence_threshold, seed=seed, counter=counter)
-        rc = _lib.load().gjx_rmhmc_monge_funnel_nuts(params, M, io, _lib.stream_handle(dev))
+        rc = _lib.load().gjx_rmhmc_monge_funnel_nuts(params, M, _lib.GJX_UTURN[stopping_criterion], io,
+                                                     _lib.stream_handle(dev))
         _lib.check(rc, "gjx_rmhmc_monge_funnel_nuts")
         sq = (lambda t: t[0]) if single else (lambda t: t)
         new_state = RMHMCState(sq(out["new_position"]), sq(out["new_logdensity"]), sq(out["new_logdensity_grad"]))

@@ -386,7 +386,9 @@ int gjx_rmhmc_fisher_logistic_tangent(const gjx_logistic_params* params, const g
  * NUTS with the RMHMC implicit-midpoint integrator (geomjax/rmhmc/nuts.py:78-161,
  * 253-348): iterative trajectory doubling in a random direction, progressive
  * multinomial sampling (uniform inside a subtree, biased at the merge), the
- * numpyro-style iterative U-turn check with the "euc" criterion.  Monge metric
+ * numpyro-style iterative U-turn check with the stopping criterion
+ * (geomjax/rmhmc/metrics.py:100-117: GJX_UTURN_EUC "euc", the default; BET
+ * "bet"; RIEM "riem").  Monge metric
  * on Neal's funnel, float64, one wave per chain, D - 1 <= 256, M =
  * max_num_doublings in [1, 12] (the trajectory holds at most 2^M - 1 steps
  * past its start).  The random
@@ -394,6 +396,8 @@ int gjx_rmhmc_fisher_logistic_tangent(const gjx_logistic_params* params, const g
  * for expansion e < M, u_step[j] for the j-th integration step (all NULL:
  * device PRNG at (params->seed, params->counter)).
  * ------------------------------------------------------------------------- */
+enum { GJX_UTURN_EUC = 0, GJX_UTURN_BET = 1, GJX_UTURN_RIEM = 2 };
+
 typedef struct {
   const void* position;          /* [C, D] */
   const double* logdensity;      /* [C] */
@@ -415,8 +419,8 @@ typedef struct {
   void* rightmost_position;      /* [C, D] optional */
 } gjx_nuts_io;
 
-int gjx_rmhmc_monge_funnel_nuts(const gjx_funnel_params* params, int32_t max_num_doublings, const gjx_nuts_io* io,
-                                void* stream);
+int gjx_rmhmc_monge_funnel_nuts(const gjx_funnel_params* params, int32_t max_num_doublings,
+                                int32_t stopping_criterion, const gjx_nuts_io* io, void* stream);
 
 /* Log density and its gradient of Neal's funnel for C positions: the
  * jax.value_and_grad(logdensity_fn)(position) of the reference's init

@@ -7,7 +7,7 @@ geomjax/mcmc/trajectory.py:40-103 (trajectory helpers), :149-317
 (dynamic_progressive_integration), :517-668 (dynamic_multiplicative_expansion),
 geomjax/mcmc/termination.py:30-119 (numpyro-style iterative U-turn),
 geomjax/mcmc/proposal.py:43-121,198-255 (proposals, progressive uniform and
-biased sampling) and the U-turn criterion of geomjax/rmhmc/metrics.py:76-118.
+biased sampling) and the U-turn criteria of geomjax/rmhmc/metrics.py:100-117.
 
 Chains run one at a time (each has its own trajectory); the integrator is the
 batched implicit midpoint of oracle.rmhmc on a batch of one.  The JAX keys
@@ -51,9 +51,19 @@ def _row(state, i=0):
     return IntegratorState(*(t[i:i + 1] for t in state))
 
 
-def _is_turning_euc(v_left, v_right, v_sum):
-    """geomjax/rmhmc/metrics.py:104-108 ("euc"): rho = velocity sum."""
-    return bool(torch.dot(v_left, v_sum) <= 0) or bool(torch.dot(v_right, v_sum) <= 0)
+def _is_turning(criterion, m_left, m_right, m_sum, v_left, v_right, v_sum):
+    """geomjax/rmhmc/metrics.py:100-117: "euc" rho = velocity sum against the end
+    velocities, "bet" rho = momentum sum against the end velocities, "riem"
+    rho = velocity sum against the end momenta."""
+    if criterion == "euc":
+        rho, left, right = v_sum, v_left, v_right
+    elif criterion == "bet":
+        rho, left, right = m_sum, v_left, v_right
+    elif criterion == "riem":
+        rho, left, right = v_sum, m_left, m_right
+    else:
+        raise ValueError(criterion)
+    return bool(torch.dot(left, rho) <= 0) or bool(torch.dot(right, rho) <= 0)
 
 
 def _leaf_idx_to_ckpt_idxs(n):
@@ -98,7 +108,7 @@ def build_kernel(divergence_threshold=1000.0, **solver_kwargs):
     """geomjax/rmhmc/nuts.py:78-161 with the keys replaced by
     noise = (z [C, D], u_dir [C, M], u_merge [C, M], u_step [C, 2^M - 1])."""
 
-    def kernel(noise, state, logdensity_fn, step_size, metric_fn, max_num_doublings=10):
+    def kernel(noise, state, logdensity_fn, step_size, metric_fn, max_num_doublings=10, stopping_criterion="euc"):
         z, u_dir, u_merge, u_step = noise
         momentum_generator, kinetic_energy_fn, imvp = gaussian_riemannian(metric_fn)
         one_step = implicit_midpoint(logdensity_fn, kinetic_energy_fn, imvp, **solver_kwargs)
@@ -115,6 +125,8 @@ def build_kernel(divergence_threshold=1000.0, **solver_kwargs):
             H0 = float(energy(s0)[0])
             proposal = Proposal(s0, H0, 0.0, -math.inf)
             traj = Trajectory(s0, s0, s0.momentum[0].clone(), s0.velocity[0].clone(), 0)
+            r_ck = torch.zeros(M, D, dtype=position.dtype)
+            rs_ck = torch.zeros(M, D, dtype=position.dtype)
             v_ck = torch.zeros(M, D, dtype=position.dtype)
             vs_ck = torch.zeros(M, D, dtype=position.dtype)
             idx = (0, 0)
@@ -144,12 +156,15 @@ def build_kernel(divergence_threshold=1000.0, **solver_kwargs):
                     # iterative U-turn (termination.py:30-69, 89-117)
                     idx = _leaf_idx_to_ckpt_idxs(step)
                     if step % 2 == 0:
+                        r_ck[idx[1]] = new.momentum[0]
+                        rs_ck[idx[1]] = sub.momentum_sum
                         v_ck[idx[1]] = new.velocity[0]
                         vs_ck[idx[1]] = sub.velocity_sum
                     i = idx[1]
                     while i >= idx[0] and not term_s:
-                        rho = sub.velocity_sum - vs_ck[i] + v_ck[i]
-                        term_s = _is_turning_euc(v_ck[i], new.velocity[0], rho)
+                        term_s = _is_turning(stopping_criterion, r_ck[i], new.momentum[0],
+                                             sub.momentum_sum - rs_ck[i] + r_ck[i], v_ck[i], new.velocity[0],
+                                             sub.velocity_sum - vs_ck[i] + v_ck[i])
                         i -= 1
                     step += 1
                 if direction < 0:
@@ -164,8 +179,10 @@ def build_kernel(divergence_threshold=1000.0, **solver_kwargs):
                 left, right = (traj, sub) if direction > 0 else (sub, traj)
                 traj = Trajectory(left.leftmost_state, right.rightmost_state, left.momentum_sum + right.momentum_sum,
                                   left.velocity_sum + right.velocity_sum, left.num_states + right.num_states)
-                turn_full = _is_turning_euc(traj.leftmost_state.velocity[0], traj.rightmost_state.velocity[0],
-                                            traj.velocity_sum)
+                turn_full = _is_turning(stopping_criterion, traj.leftmost_state.momentum[0],
+                                        traj.rightmost_state.momentum[0], traj.momentum_sum,
+                                        traj.leftmost_state.velocity[0], traj.rightmost_state.velocity[0],
+                                        traj.velocity_sum)
                 e += 1
                 diverging = div_s
                 turning = term_s or turn_full

@@ -4,6 +4,7 @@
   position, log density, energy, acceptance statistic and the trajectory's
   ends within 1e-10 norm-wise relative; numbers of integration steps and of
   expansions, and the turning / divergence flags identical;
+* the same for the "bet" and "riem" criteria;
 * stationarity with the device PRNG: chains started from exact draws of the
   funnel (v ~ N(0, 9), x_i | v ~ N(0, e^v)) keep v's mean and variance after
   ten transitions (within 5 standard errors).
@@ -27,13 +28,16 @@ pytestmark = pytest.mark.gpu
 DEV = "cuda:0"
 
 
-@pytest.mark.parametrize("D,C,eps,M,T,seed", [
-    (10, 4, 0.1, 5, 3, 0),     # the 10-dimensional funnel, three chained transitions
-    (2, 6, 0.3, 6, 2, 1),      # minimal funnel, long trajectories
-    (100, 2, 0.05, 4, 1, 2),   # two coordinates per lane
+@pytest.mark.parametrize("D,C,eps,M,T,seed,crit", [
+    (10, 4, 0.1, 5, 3, 0, "euc"),     # the 10-dimensional funnel, three chained transitions
+    (2, 6, 0.3, 6, 2, 1, "euc"),      # minimal funnel, long trajectories
+    (100, 2, 0.05, 4, 1, 2, "euc"),   # two coordinates per lane
+    (4, 4, 0.15, 5, 2, 3, "bet"),     # the other U-turn criteria
+    (4, 4, 0.15, 5, 2, 3, "riem"),
+    (160, 2, 0.04, 4, 1, 4, "riem"),  # three (of four) coordinates per lane
 ])
-def test_nuts_matches_oracle(D, C, eps, M, T, seed):
-    print(f"case D={D} C={C} eps={eps} M={M} T={T}", flush=True)
+def test_nuts_matches_oracle(D, C, eps, M, T, seed, crit):
+    print(f"case D={D} C={C} eps={eps} M={M} T={T} criterion={crit}", flush=True)
     model = gj.NealFunnel(D)
     Mo = om.NealFunnel(D)
     metric = om.monge_metric_fn(Mo.logp, 1.0, D)
@@ -45,8 +49,9 @@ def test_nuts_matches_oracle(D, C, eps, M, T, seed):
     for t in range(T):
         noise = (seeded_normal(20 + 10 * seed + t, C, D), seeded_uniform(30 + 10 * seed + t, C, M),
                  seeded_uniform(40 + 10 * seed + t, C, M), seeded_uniform(50 + 10 * seed + t, C, 2 ** M - 1))
-        s1, info = kern(NUTSNoise(*(x.to(DEV) for x in noise)), s, model, eps, gj.MongeMetric(1.0), M)
-        so1, oi = okern(noise, so, Mo.logp, eps, metric, max_num_doublings=M)
+        s1, info = kern(NUTSNoise(*(x.to(DEV) for x in noise)), s, model, eps, gj.MongeMetric(1.0), M,
+                        stopping_criterion=crit)
+        so1, oi = okern(noise, so, Mo.logp, eps, metric, max_num_doublings=M, stopping_criterion=crit)
         torch.cuda.synchronize()
         steps = [o.num_integration_steps for o in oi]
         print(f"  t={t}: steps {steps}, turning {[o.is_turning for o in oi]}", flush=True)
@@ -73,6 +78,23 @@ def test_nuts_matches_oracle(D, C, eps, M, T, seed):
         assert rel_err(info.momentum, torch.stack([o.momentum for o in oi])) < 1e-10
         so = so1
         s = RMHMCState(so.position.to(DEV), so.logdensity.to(DEV), so.logdensity_grad.to(DEV))
+
+
+def test_nuts_large_checkpoint_tables_launch():
+    """M = 12 doublings with four checkpoint arrays at four coordinates per lane
+    need 96 KiB of LDS per chain (past the 64 KiB default): the launch raises the
+    kernel's limit and runs; a short step size keeps every trajectory finite."""
+    D, C = 200, 3
+    model = gj.NealFunnel(D)
+    q = (0.3 * seeded_normal(90, C, D)).to(DEV)
+    s = gj.rmhmc.init(q, model)
+    kern = nuts_kernel(functools.partial(implicit_midpoint, max_iters=8))
+    for crit in ("bet", "riem", "euc"):
+        s1, info = kern(gr.key(91), s, model, 0.01, gj.MongeMetric(1.0), 12, stopping_criterion=crit)
+        torch.cuda.synchronize()
+        print(f"criterion {crit}: steps {info.num_integration_steps.tolist()}", flush=True)
+        assert torch.isfinite(s1.position).all()
+        assert (info.num_integration_steps >= 1).all() and (info.num_integration_steps <= 2 ** 12 - 1).all()
 
 
 def test_nuts_keeps_the_funnel_stationary():

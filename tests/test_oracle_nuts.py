@@ -8,7 +8,11 @@
   (termination.py:71-87: 6 -> (3, 2), 7 -> (0, 2), 13 -> (2, 2));
 * trajectory bookkeeping: num_integration_steps <= 2^M - 1, the acceptance
   statistic is in [0, 1], and a trajectory stopped only by the doubling cap
-  has exactly 2^M - 1 states.
+  has exactly 2^M - 1 states;
+* the U-turn criteria (geomjax/rmhmc/metrics.py:100-117): a truth table on
+  vectors where each criterion has one answer; under the identity metric
+  (momentum = velocity) the three give identical trajectories, under the
+  Monge metric three different ones.
 """
 import math
 
@@ -63,3 +67,45 @@ def test_trajectory_bookkeeping():
         assert 0.0 <= info.acceptance_rate <= 1.0 + 1e-12
         if not info.is_turning and not info.is_divergent:
             assert info.num_integration_steps == 2 ** Mx - 1 and info.num_trajectory_expansions == Mx
+
+
+def test_criteria_definitions():
+    """geomjax/rmhmc/metrics.py:100-117 on vectors where each criterion has one answer."""
+    e1, m1 = torch.tensor([1.0, 0.0], dtype=torch.float64), torch.tensor([-1.0, 0.0], dtype=torch.float64)
+    # momenta point back, velocities forward: only the criteria that read the momenta turn
+    args = dict(m_left=m1, m_right=m1, m_sum=m1, v_left=e1, v_right=e1, v_sum=e1)
+    assert onuts._is_turning("euc", **args) is False
+    assert onuts._is_turning("bet", **args) is True    # v_left . m_sum < 0
+    assert onuts._is_turning("riem", **args) is True   # m_left . v_sum < 0
+    # only the right end's momentum points back: "riem" turns at the right end alone
+    args = dict(m_left=e1, m_right=m1, m_sum=e1, v_left=e1, v_right=e1, v_sum=e1)
+    assert [onuts._is_turning(c, **args) for c in ("euc", "bet", "riem")] == [False, False, True]
+    # orthogonal counts as turning (<= 0)
+    e2 = torch.tensor([0.0, 1.0], dtype=torch.float64)
+    assert onuts._is_turning("euc", m_left=e1, m_right=e1, m_sum=e1, v_left=e2, v_right=e1, v_sum=e1) is True
+
+
+def _run(D, alpha2, criterion, seed, C=4, Mx=5, eps=0.15):
+    M = om.NealFunnel(D)
+    metric = om.monge_metric_fn(M.logp, alpha2, D)
+    q0 = 0.5 * seeded_normal(seed, C, D)
+    noise = (seeded_normal(seed + 1, C, D), seeded_uniform(seed + 2, C, Mx), seeded_uniform(seed + 3, C, Mx),
+             seeded_uniform(seed + 4, C, 2 ** Mx - 1))
+    return onuts.build_kernel(max_iters=8)(noise, orm.init(q0, M.logp), M.logp, eps, metric, max_num_doublings=Mx,
+                                           stopping_criterion=criterion)
+
+
+def test_criteria_coincide_under_the_identity_metric():
+    """With alpha2 = 0 the metric is I, momentum = velocity, and the three
+    criteria are the same test: identical trajectories."""
+    runs = [_run(4, 0.0, c, 20) for c in ("euc", "bet", "riem")]
+    for s, infos in runs[1:]:
+        assert torch.equal(s.position, runs[0][0].position)
+        assert [i.num_integration_steps for i in infos] == [i.num_integration_steps for i in runs[0][1]]
+
+
+def test_criteria_differ_under_the_monge_metric():
+    """With alpha2 = 1 momentum and velocity differ, and so do the trajectory
+    lengths the three criteria give on this seed."""
+    steps = {c: tuple(i.num_integration_steps for i in _run(4, 1.0, c, 30)[1]) for c in ("euc", "bet", "riem")}
+    assert len(set(steps.values())) == 3, steps
