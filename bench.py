@@ -22,9 +22,9 @@ A "step" is one transition of all 4096 chains (10 integrator steps each).
 resident in HBM).  min-ESS/s uses the multi-chain ESS of
 geomjax/diagnostics.py:78-209 over the timed samples.  Extra fields report the
 same comparison at 1024 chains, for 256 logistic posteriors of uneven size in
-one launch (8192 and 4096 chains), for Neal's funnel D=100 (RMHMC and LMC with
-the Monge metric, 8192 chains), and the cost of the tuning derivatives (tangent
-launch vs primal transition).  The CPU baseline is the float64 oracle
+one launch (8192 and 4096 chains), for Neal's funnel D=100 (RMHMC, NUTS-RMHMC
+and LMC with the Monge metric, 8192 chains), and the cost of the tuning
+derivatives (tangent launch vs primal transition).  The CPU baseline is the float64 oracle
 restatement (dense metric + autodiff, as the reference) timed on this host's
 cores on a bounded sample of the headline workload.
 
@@ -459,6 +459,60 @@ def funnel_lmc_forms(K, W, dev, plain=True):
     return res
 
 
+def nuts_forms(K, W, dev, static_plain=None):
+    """NUTS with the RMHMC integrator (float64, one launch per transition) on
+    Neal's funnel D=100 with 8192 chains: Monge metric, the funnel workload's
+    step size, max_num_doublings 10, "euc" criterion, device PRNG.  Reports
+    integration steps/s, transitions/s and min-ESS/s, and min-ESS/s relative to
+    the plain static-length RMHMC form on the same posterior when given."""
+    import torch
+
+    import geomjax_amd as gj
+    from geomjax_amd import random as gr
+    from geomjax_amd.nuts import build_kernel as nuts_kernel
+    from geomjax_amd.rmhmc import implicit_midpoint
+
+    M = 10
+    model, metric = gj.NealFunnel(D_F, sigma=SIGMA), gj.MongeMetric(ALPHA2)
+    kern = nuts_kernel(functools.partial(implicit_midpoint, max_iters=MAXIT_F))
+    q_init = torch.randn(CHAINS_F, D_F, generator=torch.Generator().manual_seed(SEED_F), dtype=torch.float64)
+    s = gj.rmhmc.init(q_init.to(dev), model)
+    for t in range(W):
+        s, _ = kern(gr.PRNGKey(SEED_F + 7, t), s, model, EPS_F, metric, M)
+    samples = torch.empty(K, CHAINS_F, D_F, dtype=torch.float64, device=dev)
+    steps, accs, turned = [], [], []
+    stream = torch.cuda.current_stream(dev)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for t in range(K):
+        s, info = kern(gr.PRNGKey(SEED_F + 7, W + t), s, model, EPS_F, metric, M)
+        samples[t].copy_(s.position)
+        steps.append(info.num_integration_steps)
+        accs.append(info.acceptance_rate)
+        turned.append(info.is_turning)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_s = ev0.elapsed_time(ev1) / 1e3
+    total_steps = int(torch.stack(steps).long().sum().item())
+    ess, _ = min_ess(samples)
+    del samples
+    res = {"value": total_steps / wall, "unit": "integration-steps/s", "transitions_per_sec": K / wall,
+           "min_ess_per_sec": ess / wall, "min_ess": ess, "mean_integration_steps": total_steps / (K * CHAINS_F),
+           "max_integration_steps": int(torch.stack(steps).max().item()),
+           "acceptance_rate": float(torch.stack(accs).mean().item()),
+           "turned_fraction": float(torch.stack(turned).float().mean().item()),
+           "ms_per_transition": wall / K * 1e3, "kernel_ms_per_transition": kernel_s / K * 1e3,
+           "kernel": "funnel_nuts_kernel<2>", "dtype": "f64", "max_num_doublings": M, "steps": K}
+    if static_plain is not None:
+        res["min_ess_per_sec_vs_static_rmhmc_plain"] = res["min_ess_per_sec"] / static_plain["min_ess_per_sec"]
+    log(f"funnel NUTS: {res['value']:.3e} integration-steps/s, {res['mean_integration_steps']:.1f} steps per "
+        f"transition, min-ESS/s {res['min_ess_per_sec']:.3e}")
+    return res
+
+
 def tangent_forms(dev, repeats=3):
     """Cost of the derivative of acceptance x squared jump w.r.t. (log eps, log alpha2):
     one tangent launch (one transition carried with its forward derivative) against one
@@ -550,6 +604,8 @@ def main():
         extra["uneven_batch_8192_chains"] = uneven_batch_forms(32, K, W, dev, plain=not args.no_plain)
         extra["uneven_batch_4096_chains"] = uneven_batch_forms(16, K, W, dev, plain=not args.no_plain)
         extra["funnel_d100_8192_chains"] = funnel_forms(300, 100, dev, plain=not args.no_plain)
+        extra["funnel_nuts_d100_8192_chains"] = nuts_forms(100, 10, dev,
+                                                           extra["funnel_d100_8192_chains"].get("plain"))
         extra["funnel_lmcmonge_d100_8192_chains"] = funnel_lmc_forms(300, 100, dev, plain=not args.no_plain)
         extra["tangent"] = tangent_forms(dev)
 
