@@ -161,6 +161,8 @@ _SIGNATURES = {
     "gjx_lmcmonge_funnel_sample": [ctypes.POINTER(FunnelParams), ctypes.POINTER(LmcSampleIO), _vp],
     "gjx_rmhmc_monge_funnel_tangent": [ctypes.POINTER(FunnelParams), ctypes.POINTER(TangentIO), _vp],
     "gjx_rmhmc_monge_funnel_nuts": [ctypes.POINTER(FunnelParams), _c_int32, _c_int32, ctypes.POINTER(NutsIO), _vp],
+    "gjx_rmhmc_fisher_logistic_nuts": [ctypes.POINTER(LogisticParams), ctypes.POINTER(LogisticData), _c_int32,
+                                       _c_int32, ctypes.POINTER(NutsIO), _vp],
     "gjx_funnel_value_and_grad": [_c_int32, _c_int32, _c_double, _c_int32, _vp, _vp, _vp, _vp],
     "gjx_rmhmc_fisher_logistic_step": [ctypes.POINTER(LogisticParams), ctypes.POINTER(LogisticData),
                                        ctypes.POINTER(RmhmcStepIO), _vp],

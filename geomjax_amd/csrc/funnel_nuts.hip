@@ -43,12 +43,6 @@ struct Prop {
   double energy, weight, slpa;
 };
 
-__device__ __forceinline__ double logaddexp(double a, double b) {
-  const double m = fmax(a, b);
-  if (m == -INFINITY) return -INFINITY;
-  return m + log(exp(a - m) + exp(b - m));
-}
-
 }  // namespace nuts
 
 using namespace nuts;
@@ -353,9 +347,7 @@ __global__ void __launch_bounds__(64) funnel_nuts_kernel(const FunnelArgs a, con
       }
       ++j;
       // numpyro checkpoints (termination.py:38-117): leaf index -> [idx_min, idx_max]
-      const int idx_max = __builtin_popcount((unsigned)step >> 1);
-      const int trailing = __builtin_ctz(~(unsigned)step);
-      const int idx_min = idx_max - trailing + 1;
+      const int idx_max = ckpt_idx_max(step), idx_min = ckpt_idx_min(step);
       if ((step & 1) == 0) {
         ck_put(0, idx_max, cur.ux, cur.uv);
         ck_put(1, idx_max, ssv_x, ssv_v);

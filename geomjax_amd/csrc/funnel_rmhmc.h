@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "nuts.h"
+
 namespace gjx {
 
 // Everything one launch needs, passed by value as the kernel argument.
@@ -70,25 +72,6 @@ int funnel_pick_epl(int dim, int lpc);
 hipError_t launch_funnel_rmhmc(const FunnelArgs& a, int dtype, int lpc, int epl, hipStream_t stream);
 
 // NUTS (iterative, multinomial, numpyro-style U-turn with the "euc" criterion)
-struct NutsArgs {
-  int max_doublings;           // M
-  int criterion;               // U-turn criterion: 0 "euc", 1 "bet", 2 "riem"
-  const double* u_dir;         // [C, M] or null (device PRNG)
-  const double* u_merge;       // [C, M]
-  const double* u_step;        // [C, 2^M - 1]
-  // outputs (optional unless noted)
-  double* energy;              // [C]
-  double* acceptance_rate;     // [C]
-  uint8_t* is_divergent;       // [C]
-  uint8_t* is_turning;         // [C]
-  int32_t* num_expansions;     // [C]
-  int32_t* num_steps;          // [C]
-  void* left_position;         // [C, D]
-  void* right_position;        // [C, D]
-};
-// checkpoint arrays of the iterative U-turn check: velocities and velocity sums,
-// plus momenta and momentum sums for the criteria that read them
-__host__ __device__ inline int nuts_checkpoint_arrays(int criterion) { return criterion == 0 ? 2 : 4; }
 hipError_t launch_funnel_nuts(const FunnelArgs& a, const NutsArgs& n, hipStream_t stream);
 
 hipError_t launch_funnel_value_and_grad(int C, int D, double sigma, int dtype, const void* q, double* logp,

@@ -6,10 +6,12 @@ stopping_criterion="euc")`` with ``init`` (= rmhmc.init) and ``step``; the
 kernel ``build_kernel(integrator, divergence_threshold)(rng_key, state,
 logdensity_fn, step_size, metric_fn, max_num_doublings, stopping_criterion)``.
 The whole transition (momentum draw, iterative tree doubling, progressive
-multinomial sampling, U-turn checks) runs in one HIP kernel, one wave per
-chain, float64; served for Neal's funnel with the Monge metric and the three
-U-turn criteria of geomjax/rmhmc/metrics.py:100-117 ("euc", the default;
-"bet"; "riem").
+multinomial sampling, U-turn checks) runs in one HIP kernel, float64, with the
+three U-turn criteria of geomjax/rmhmc/metrics.py:100-117 ("euc", the
+default; "bet"; "riem").  Served: Neal's funnel with the Monge metric (one
+wave per chain) and Bayesian logistic regression with the Fisher metric, one
+posterior or a ``LogisticBatch``, every posterior with D <= 64 (one
+workgroup per chain).
 
 ``rng_key`` is a ``PRNGKey`` (device Philox) or ``NUTSNoise(z, u_direction,
 u_merge, u_step)`` with the draws the reference takes from its keys: the
@@ -27,8 +29,8 @@ from typing import NamedTuple
 import torch
 
 from . import _lib
-from .metrics import MongeMetric
-from .models import NealFunnel
+from .metrics import FisherMetric, MongeMetric
+from .models import LogisticBatch, LogisticRegression, NealFunnel, logistic_params
 from .random import PRNGKey
 from .rmhmc import RMHMCState, _params, _solver_kwargs, implicit_midpoint, init
 
@@ -60,8 +62,11 @@ def build_kernel(integrator=implicit_midpoint, divergence_threshold: int = 1000)
                stopping_criterion: str = "euc"):
         if stopping_criterion not in _lib.GJX_UTURN:
             raise ValueError(f"stopping_criterion must be one of {sorted(_lib.GJX_UTURN)} (got {stopping_criterion!r})")
-        if not isinstance(logdensity_fn, NealFunnel) or not isinstance(metric_fn, MongeMetric):
-            raise NotImplementedError("the NUTS kernel serves Neal's funnel with the Monge metric")
+        funnel = isinstance(logdensity_fn, NealFunnel) and isinstance(metric_fn, MongeMetric)
+        logistic = isinstance(logdensity_fn, (LogisticRegression, LogisticBatch)) and isinstance(metric_fn, FisherMetric)
+        if not (funnel or logistic):
+            raise NotImplementedError("the NUTS kernels serve Neal's funnel with the Monge metric and logistic "
+                                      "regression with the Fisher metric")
         position = state.position
         single = position.dim() == 1
         if single:
@@ -71,7 +76,9 @@ def build_kernel(integrator=implicit_midpoint, divergence_threshold: int = 1000)
             raise ValueError("NUTS positions must be float64 tensors on a HIP device")
         C, D = position.shape
         if D != logdensity_fn.D:
-            raise ValueError(f"position has {D} coordinates, the model has D={logdensity_fn.D}")
+            raise ValueError(f"position has {D} coordinates, the model has {logdensity_fn.D}")
+        if isinstance(logdensity_fn, LogisticBatch) and C != logdensity_fn.num_chains:
+            raise ValueError(f"the batch has {logdensity_fn.num_chains} chains, got {C}")
         dev = position.device
         M = int(max_num_doublings)
         logdensity = state.logdensity.reshape(C).to(torch.float64).contiguous()
@@ -94,11 +101,20 @@ def build_kernel(integrator=implicit_midpoint, divergence_threshold: int = 1000)
         io = _lib.NutsIO(position=position.data_ptr(), logdensity=logdensity.data_ptr(), z=_lib.ptr(noise[0]),
                          u_direction=_lib.ptr(noise[1]), u_merge=_lib.ptr(noise[2]), u_step=_lib.ptr(noise[3]),
                          **{k: v.data_ptr() for k, v in out.items()})
-        params = _params(C, D, _lib.GJX_F64, step_size, 0, solver, metric_fn.alpha2, logdensity_fn.sigma,
-                         divergence_threshold, seed=seed, counter=counter)
-        rc = _lib.load().gjx_rmhmc_monge_funnel_nuts(params, M, _lib.GJX_UTURN[stopping_criterion], io,
-                                                     _lib.stream_handle(dev))
-        _lib.check(rc, "gjx_rmhmc_monge_funnel_nuts")
+        crit = _lib.GJX_UTURN[stopping_criterion]
+        if funnel:
+            params = _params(C, D, _lib.GJX_F64, step_size, 0, solver, metric_fn.alpha2, logdensity_fn.sigma,
+                             divergence_threshold, seed=seed, counter=counter)
+            rc = _lib.load().gjx_rmhmc_monge_funnel_nuts(params, M, crit, io, _lib.stream_handle(dev))
+            _lib.check(rc, "gjx_rmhmc_monge_funnel_nuts")
+        else:
+            params = logistic_params(logdensity_fn, C, step_size=step_size, max_iters=solver["max_iters"],
+                                     convergence_tol=solver["convergence_tol"],
+                                     divergence_tol=solver["divergence_tol"],
+                                     divergence_threshold=divergence_threshold, seed=seed, counter=counter)
+            data = logdensity_fn.data(dtype=torch.float64, num_chains=C)
+            rc = _lib.load().gjx_rmhmc_fisher_logistic_nuts(params, data, M, crit, io, _lib.stream_handle(dev))
+            _lib.check(rc, "gjx_rmhmc_fisher_logistic_nuts")
         sq = (lambda t: t[0]) if single else (lambda t: t)
         new_state = RMHMCState(sq(out["new_position"]), sq(out["new_logdensity"]), sq(out["new_logdensity_grad"]))
         info = NUTSInfo(sq(out["momentum"]), sq(out["is_divergent"].bool()), sq(out["is_turning"].bool()),

@@ -422,6 +422,17 @@ typedef struct {
 int gjx_rmhmc_monge_funnel_nuts(const gjx_funnel_params* params, int32_t max_num_doublings,
                                 int32_t stopping_criterion, const gjx_nuts_io* io, void* stream);
 
+/* NUTS with the RMHMC integrator for Bayesian logistic regression with the
+ * Fisher metric (geomjax/rmhmc/nuts.py:78-161 over the float64 logistic
+ * integrator of gjx_rmhmc_fisher_logistic_step): float64 (params->dtype =
+ * GJX_F64), every posterior D <= 64, one workgroup per chain; a batch of
+ * posteriors as for the step entry point.  io as for the funnel entry point, with
+ * rows of stride params->dim; new_logdensity_grad is recomputed at the chosen
+ * state.  params->num_integration_steps and num_transitions are not used. */
+int gjx_rmhmc_fisher_logistic_nuts(const gjx_logistic_params* params, const gjx_logistic_data* data,
+                                   int32_t max_num_doublings, int32_t stopping_criterion, const gjx_nuts_io* io,
+                                   void* stream);
+
 /* Log density and its gradient of Neal's funnel for C positions: the
  * jax.value_and_grad(logdensity_fn)(position) of the reference's init
  * (geomjax/rmhmc/rmhmc.py:96-98, geomjax/lmcmonge/lmc.py:101-109) with the

@@ -120,3 +120,77 @@ def test_nuts_keeps_the_funnel_stationary():
     se_m, se_v = 3.0 / C ** 0.5, 9.0 * (2.0 / C) ** 0.5
     assert abs(m) < 5 * se_m
     assert abs(var - 9.0) < 5 * se_v
+
+
+@pytest.mark.parametrize("N,D,C,eps,M,T,seed,crit", [
+    (120, 6, 3, 0.1, 4, 2, 10, "euc"),
+    (120, 6, 3, 0.1, 4, 2, 10, "riem"),
+    (256, 64, 2, 0.05, 3, 1, 11, "bet"),   # the kernel's full feature capacity
+])
+def test_logistic_nuts_matches_oracle(N, D, C, eps, M, T, seed, crit):
+    from geomjax_amd.synthetic import make_logistic
+
+    print(f"case logistic N={N} D={D} C={C} eps={eps} M={M} T={T} criterion={crit}", flush=True)
+    X, y, _ = make_logistic(N, D, seed)
+    model = gj.LogisticRegression(X, y, 100.0)
+    Mo = om.LogisticRegression(X, y, 100.0)
+    q0 = 0.3 * seeded_normal(100 + seed, C, D)
+    so = orm.init(q0, Mo.logp)
+    s = gj.rmhmc.init(q0.to(DEV), model)
+    kern = nuts_kernel(functools.partial(implicit_midpoint, max_iters=6))
+    okern = onuts.build_kernel(max_iters=6)
+    for t in range(T):
+        noise = (seeded_normal(200 + 10 * seed + t, C, D), seeded_uniform(300 + 10 * seed + t, C, M),
+                 seeded_uniform(400 + 10 * seed + t, C, M), seeded_uniform(500 + 10 * seed + t, C, 2 ** M - 1))
+        s1, info = kern(NUTSNoise(*(x.to(DEV) for x in noise)), s, model, eps, gj.FisherMetric(), M,
+                        stopping_criterion=crit)
+        so1, oi = okern(noise, so, Mo.logp, eps, Mo.fisher_metric_fn, max_num_doublings=M, stopping_criterion=crit)
+        torch.cuda.synchronize()
+        print(f"  t={t}: steps {[o.num_integration_steps for o in oi]}, turning {[o.is_turning for o in oi]}",
+              flush=True)
+        assert info.num_integration_steps.cpu().tolist() == [o.num_integration_steps for o in oi]
+        assert info.num_trajectory_expansions.cpu().tolist() == [o.num_trajectory_expansions for o in oi]
+        assert info.is_turning.cpu().tolist() == [o.is_turning for o in oi]
+        assert info.is_divergent.cpu().tolist() == [o.is_divergent for o in oi]
+        assert rel_err(s1.position, so1.position) < 1e-10
+        assert rel_err(s1.logdensity, so1.logdensity) < 1e-10
+        assert rel_err(s1.logdensity_grad, so1.logdensity_grad) < 1e-10
+        assert rel_err(info.energy, torch.tensor([o.energy for o in oi], dtype=torch.float64)) < 1e-10
+        assert rel_err(info.acceptance_rate, torch.tensor([o.acceptance_rate for o in oi], dtype=torch.float64)) < 1e-10
+        assert rel_err(info.momentum, torch.stack([o.momentum for o in oi])) < 1e-10
+        left = torch.cat([o.trajectory_leftmost_state.position for o in oi])
+        right = torch.cat([o.trajectory_rightmost_state.position for o in oi])
+        assert rel_err(info.trajectory_leftmost_state, left) < 1e-10
+        assert rel_err(info.trajectory_rightmost_state, right) < 1e-10
+        so = so1
+        s = RMHMCState(so.position.to(DEV), so.logdensity.to(DEV), so.logdensity_grad.to(DEV))
+
+
+def test_logistic_nuts_batch_equals_each_posterior_alone():
+    from geomjax_amd.synthetic import make_logistic
+
+    Xa, ya, _ = make_logistic(150, 5, 21)
+    Xb, yb, _ = make_logistic(90, 9, 22)
+    ma, mb = gj.LogisticRegression(Xa, ya, 100.0), gj.LogisticRegression(Xb, yb, 100.0)
+    batch = gj.LogisticBatch([ma, mb], [3, 2])
+    C, Dm, M = 5, 9, 5
+    q = torch.zeros(C, Dm, dtype=torch.float64)
+    q[:3, :5] = 0.3 * seeded_normal(31, 3, 5)
+    q[3:, :9] = 0.3 * seeded_normal(32, 2, 9)
+    z = torch.zeros(C, Dm, dtype=torch.float64)
+    z[:3, :5] = seeded_normal(33, 3, 5)
+    z[3:, :9] = seeded_normal(34, 2, 9)
+    ud, um, us = seeded_uniform(35, C, M), seeded_uniform(36, C, M), seeded_uniform(37, C, 2 ** M - 1)
+    kern = nuts_kernel(functools.partial(implicit_midpoint, max_iters=6))
+    sb = gj.rmhmc.init(q.to(DEV), batch)
+    nb, ib = kern(NUTSNoise(z.to(DEV), ud.to(DEV), um.to(DEV), us.to(DEV)), sb, batch, 0.1, gj.FisherMetric(), M)
+    for model, rows, D in ((ma, slice(0, 3), 5), (mb, slice(3, 5), 9)):
+        s = gj.rmhmc.init(q[rows, :D].contiguous().to(DEV), model)
+        n1, i1 = kern(NUTSNoise(*(x[rows].contiguous().to(DEV) for x in (z[:, :D], ud, um, us))), s, model, 0.1,
+                      gj.FisherMetric(), M)
+        torch.cuda.synchronize()
+        assert torch.equal(ib.num_integration_steps[rows], i1.num_integration_steps)
+        assert rel_err(nb.position[rows, :D], n1.position) <= 1e-12
+        assert rel_err(nb.logdensity_grad[rows, :D], n1.logdensity_grad) <= 1e-12
+        assert rel_err(ib.acceptance_rate[rows], i1.acceptance_rate) <= 1e-12
+        assert torch.count_nonzero(nb.position[rows, D:]) == 0  # row tails past the posterior's D
