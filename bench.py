@@ -513,6 +513,50 @@ def nuts_forms(K, W, dev, static_plain=None):
     return res
 
 
+def logistic_nuts_forms(K, W, dev, chains=1024):
+    """NUTS with the RMHMC integrator on the headline posterior (logistic
+    regression 2048 x 64, Fisher metric, float64, one launch per transition):
+    the headline step size and fixed-point cap, max_num_doublings 5, "euc",
+    device PRNG.  Reports integration steps/s and min-ESS/s."""
+    import torch
+
+    import geomjax_amd as gj
+    from geomjax_amd import random as gr
+    from geomjax_amd.nuts import build_kernel as nuts_kernel
+    from geomjax_amd.rmhmc import implicit_midpoint
+    from geomjax_amd.synthetic import make_logistic
+
+    M = 5
+    X, y, _ = make_logistic(N_ROWS, D_LOG, DATA_SEED)
+    model = gj.LogisticRegression(X, y, PRIOR, device=dev)
+    kern = nuts_kernel(functools.partial(implicit_midpoint, max_iters=MAXIT_LOG))
+    q_init = 0.1 * torch.randn(chains, D_LOG, generator=torch.Generator().manual_seed(INIT_SEED), dtype=torch.float64)
+    s = gj.rmhmc.init(q_init.to(dev), model)
+    for t in range(W):
+        s, _ = kern(gr.PRNGKey(INIT_SEED + 11, t), s, model, EPS_LOG, gj.FisherMetric(), M)
+    samples = torch.empty(K, chains, D_LOG, dtype=torch.float64, device=dev)
+    steps, accs = [], []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for t in range(K):
+        s, info = kern(gr.PRNGKey(INIT_SEED + 11, W + t), s, model, EPS_LOG, gj.FisherMetric(), M)
+        samples[t].copy_(s.position)
+        steps.append(info.num_integration_steps)
+        accs.append(info.acceptance_rate)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    total_steps = int(torch.stack(steps).long().sum().item())
+    ess, _ = min_ess(samples)
+    del samples
+    res = {"value": total_steps / wall, "unit": "integration-steps/s", "min_ess_per_sec": ess / wall, "min_ess": ess,
+           "mean_integration_steps": total_steps / (K * chains), "acceptance_rate": float(torch.stack(accs).mean()),
+           "ms_per_transition": wall / K * 1e3, "chains": chains, "dtype": "f64", "max_num_doublings": M, "steps": K,
+           "kernel": "logistic_nuts_f64_kernel"}
+    log(f"logistic NUTS: {res['value']:.3e} integration-steps/s, {res['mean_integration_steps']:.1f} steps per "
+        f"transition, min-ESS/s {res['min_ess_per_sec']:.3e}")
+    return res
+
+
 def tangent_forms(dev, repeats=3):
     """Cost of the derivative of acceptance x squared jump w.r.t. (log eps, log alpha2):
     one tangent launch (one transition carried with its forward derivative) against one
@@ -606,6 +650,7 @@ def main():
         extra["funnel_d100_8192_chains"] = funnel_forms(300, 100, dev, plain=not args.no_plain)
         extra["funnel_nuts_d100_8192_chains"] = nuts_forms(100, 10, dev,
                                                            extra["funnel_d100_8192_chains"].get("plain"))
+        extra["logistic_nuts_1024_chains"] = logistic_nuts_forms(10, 2, dev)
         extra["funnel_lmcmonge_d100_8192_chains"] = funnel_lmc_forms(300, 100, dev, plain=not args.no_plain)
         extra["tangent"] = tangent_forms(dev)
 
