@@ -7,15 +7,22 @@ from . import diagnostics, metrics, models, random, sampling
 from .base import SamplingAlgorithm
 from .metrics import FisherMetric, MongeMetric
 from .models import LogisticBatch, LogisticRegression, NealFunnel
-from .lmcmc import lmc
+from .lmcmc import dynamic_lmc, lmc
+from .lmcmonge import dynamic_lmc as dynamic_lmcmonge
 from .lmcmonge import lmc as lmcmonge
 from .rmhmc import dynamic_rmhmc, rmhmc
-from .nuts import nuts
+from .nuts import nuts as nutsrmhmc
+from .adaptation import chees_adaptation as chees_adaptation_riemanian
+
+# ``nuts`` also names the RMHMC NUTS here (the reference's ``geomjax.nuts`` is its
+# Euclidean NUTS, which this build does not serve)
+nuts = nutsrmhmc
 
 __version__ = "0.1.0"
 
 ess = diagnostics.effective_sample_size
 rhat = diagnostics.potential_scale_reduction
 
-__all__ = ["rmhmc", "dynamic_rmhmc", "nuts", "lmc", "lmcmonge", "SamplingAlgorithm", "NealFunnel", "LogisticRegression", "LogisticBatch", "MongeMetric", "FisherMetric", "metrics", "models", "random", "sampling",
+__all__ = ["rmhmc", "dynamic_rmhmc", "nutsrmhmc", "nuts", "lmc", "dynamic_lmc", "lmcmonge", "dynamic_lmcmonge",
+           "chees_adaptation_riemanian", "SamplingAlgorithm", "NealFunnel", "LogisticRegression", "LogisticBatch", "MongeMetric", "FisherMetric", "metrics", "models", "random", "sampling",
            "diagnostics", "ess", "rhat", "__version__"]

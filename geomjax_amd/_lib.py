@@ -84,7 +84,7 @@ class LmcStepIO(ctypes.Structure):
         "proposal_position", "proposal_momentum", "proposal_velocity", "proposal_logdensity",
         "proposal_logdensity_grad_norm", "proposal_dl_ig", "proposal_Hdl_ig", "proposal_ig_Hdl_ig",
         "proposal_logdensity_hvp_velocity_norm", "proposal_determinant_metric", "proposal_volume_adjustment",
-        "proposal_weight", "proposal_sum_log_p_accept",
+        "proposal_weight", "proposal_sum_log_p_accept", "num_integration_steps",
     )]
 
 
@@ -92,7 +92,7 @@ class LmcSampleIO(ctypes.Structure):
     _fields_ = [(n, _vp) for n in (
         "inverse_mass_matrix", "position", "logdensity", "volume_adjustment", "z", "u",
         "final_position", "final_logdensity", "final_logdensity_grad", "final_volume_adjustment",
-        "samples", "acceptance_rate", "num_accepted",
+        "samples", "acceptance_rate", "num_accepted", "num_integration_steps",
     )]
 
 
@@ -140,6 +140,7 @@ class LmcmcStepIO(ctypes.Structure):
         "velocity", "acceptance_rate", "is_accepted", "is_divergent", "energy",
         "proposal_position", "proposal_momentum", "proposal_velocity", "proposal_logdensity",
         "proposal_logdensity_grad", "proposal_volume_adjustment", "proposal_weight", "proposal_sum_log_p_accept",
+        "num_integration_steps",
     )]
 
 
@@ -147,6 +148,7 @@ class LmcmcSampleIO(ctypes.Structure):
     _fields_ = [(n, _vp) for n in (
         "position", "logdensity", "volume_adjustment", "z", "u", "final_position", "final_logdensity",
         "final_logdensity_grad", "final_volume_adjustment", "samples", "acceptance_rate", "num_accepted",
+        "num_integration_steps",
     )]
 
 

@@ -301,6 +301,7 @@ int gjx_lmcmonge_funnel_step(const gjx_funnel_params* params, const gjx_lmc_step
   a.logp_in = io->logdensity;
   a.g_in = io->logdensity_grad;
   a.J_in = io->volume_adjustment;
+  a.Lc = io->num_integration_steps;
   a.z = io->z;
   a.u = io->u;
   a.q_out = io->new_position;
@@ -347,6 +348,7 @@ int gjx_lmcmonge_funnel_sample(const gjx_funnel_params* params, const gjx_lmc_sa
   a.q_in = io->position;
   a.logp_in = io->logdensity;
   a.J_in = io->volume_adjustment;
+  a.Lc = io->num_integration_steps;
   a.z = io->z;
   a.u = io->u;
   a.final_q = io->final_position;
@@ -565,6 +567,7 @@ int gjx_lmc_fisher_logistic_step(const gjx_logistic_params* params, const gjx_lo
   a.logp_in = io->logdensity;
   a.g_in = io->logdensity_grad;
   a.J_in = io->volume_adjustment;
+  a.Lc = io->num_integration_steps;
   a.z = io->z;
   a.u = io->u;
   a.q_out = io->new_position;
@@ -603,6 +606,7 @@ int gjx_lmc_fisher_logistic_sample(const gjx_logistic_params* params, const gjx_
   a.q_in = io->position;
   a.logp_in = io->logdensity;
   a.J_in = io->volume_adjustment;
+  a.Lc = io->num_integration_steps;
   a.z = io->z;
   a.u = io->u;
   a.final_q = io->final_position;

@@ -21,6 +21,7 @@ struct FunnelLmcArgs {
   const double* J_in;
   const void* z;    // [T, C, D] or null
   const double* u;  // [T, C] or null
+  const int32_t* Lc;  // [T, C] per-chain integration steps (dynamic kernel) or null: L for all
   // single-transition outputs
   void* q_out;
   double* logp_out;

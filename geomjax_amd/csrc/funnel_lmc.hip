@@ -276,20 +276,28 @@ __global__ void __launch_bounds__(256) funnel_lmc_kernel(const FunnelLmcArgs a) 
 #pragma unroll
     for (int k = 0; k < EPL; ++k) xc[k] = x[k];
     double dJ = 0.0;
+    // chains of the dynamic kernel stop after their own number of steps; every
+    // cross-lane sum stays inside a chain's aligned lane group, so a group whose
+    // chain has stopped can sit out the step while the others reduce
+    const int Ls = live ? chain_steps(a.Lc, a.L, a.C, STEP ? 0 : t, (int)c) : 0;
     for (int s = 0; s < a.L; ++s) {
-      half_step<R, LPC, EPL>(P, xc, mi, vx, vv, dJ, a2, inv_s2, h, exact_sm);
-      R sx = R(0), smx = R(0);
+      const bool on = s < Ls;
+      if (!__any(on)) break;
+      if (on) {
+        half_step<R, LPC, EPL>(P, xc, mi, vx, vv, dJ, a2, inv_s2, h, exact_sm);
+        R sx = R(0), smx = R(0);
 #pragma unroll
-      for (int k = 0; k < EPL; ++k) {
-        xc[k] = xc[k] + eps * vx[k];
-        sx += xc[k] * xc[k];
-        smx += mi[k] * xc[k] * xc[k];
+        for (int k = 0; k < EPL; ++k) {
+          xc[k] = xc[k] + eps * vx[k];
+          sx += xc[k] * xc[k];
+          smx += mi[k] * xc[k] * xc[k];
+        }
+        vc = vc + eps * vv;
+        sx = group_sum<LPC>(sx);
+        smx = group_sum<LPC>(smx);
+        P = pos_scalars<R>(vc, sx, smx, a2, inv_s2, halfDx, miv);
+        half_step<R, LPC, EPL>(P, xc, mi, vx, vv, dJ, a2, inv_s2, h, exact_sm);
       }
-      vc = vc + eps * vv;
-      sx = group_sum<LPC>(sx);
-      smx = group_sum<LPC>(smx);
-      P = pos_scalars<R>(vc, sx, smx, a2, inv_s2, halfDx, miv);
-      half_step<R, LPC, EPL>(P, xc, mi, vx, vv, dJ, a2, inv_s2, h, exact_sm);
     }
 
     // ---- energy at the end (lmcmonge_energy, metrics.py:267-284) ----

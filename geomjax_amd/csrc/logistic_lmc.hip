@@ -151,7 +151,8 @@ __global__ void __launch_bounds__(NT, 2) logistic_lmc_f64_kernel(const LogisticA
 
     // ---- trajectory ----
     double dJ = 0.0, ll = ll0;
-    for (int s = 0; s < a.L; ++s) {
+    const int Ls = chain_steps(a.Lc, a.L, a.C, STEP ? 0 : tr, c);
+    for (int s = 0; s < Ls; ++s) {
       half_step(dJ);
       if (t < D) S.q[t] = S.q[t] + eps * S.p[t];
       __syncthreads();

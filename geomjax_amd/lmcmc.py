@@ -19,7 +19,8 @@ from .models import LogisticBatch, LogisticRegression, logistic_params
 from .proposal import Proposal
 from .random import HostNoise, PRNGKey
 
-__all__ = ["LMCState", "LMCInfo", "IntegratorState", "init", "build_kernel", "lmc", "lan_integrator", "sample"]
+__all__ = ["LMCState", "LMCInfo", "IntegratorState", "init", "build_kernel", "lmc", "lan_integrator", "sample",
+           "DynamicLMCState", "init_dynamic", "build_dynamic_kernel", "dynamic_lmc"]
 
 
 class IntegratorState(NamedTuple):  # geomjax/lmcmc/integrators.py:26-38
@@ -88,7 +89,12 @@ def build_kernel(integrator=lan_integrator, divergence_threshold: float = 1000):
     if integrator is not lan_integrator:
         raise NotImplementedError("only lan_integrator is implemented on the GPU")
 
-    def kernel(rng_key, state, logdensity_fn, step_size, metric_fn, num_integration_steps):
+    def kernel(rng_key, state, logdensity_fn, step_size, metric_fn, num_integration_steps, *,
+               max_num_integration_steps=None):
+        """``num_integration_steps``: an int, or a [C] integer tensor of per-chain
+        counts (the dynamic kernel's form), bounded by ``max_num_integration_steps``."""
+        from .rmhmc import _steps
+
         single = state.position.dim() == 1
         position = (state.position[None] if single else state.position).contiguous()
         _check(logdensity_fn, metric_fn, position)
@@ -98,7 +104,7 @@ def build_kernel(integrator=lan_integrator, divergence_threshold: float = 1000):
         grad = state.logdensity_grad.reshape(C, D).to(torch.float64).contiguous()
         J = torch.as_tensor(state.volume_adjustment, dtype=torch.float64, device=dev).expand(C).contiguous()
         z, u, seed, counter = _noise(rng_key, None, C, D, dev)
-        L = int(num_integration_steps)
+        L, steps = _steps(num_integration_steps, (C,), dev, max_num_integration_steps)
 
         def vec():
             return torch.zeros(C, D, dtype=torch.float64, device=dev)
@@ -113,7 +119,8 @@ def build_kernel(integrator=lan_integrator, divergence_threshold: float = 1000):
                    proposal_volume_adjustment=sca(), proposal_weight=sca(), proposal_sum_log_p_accept=sca())
         io = _lib.LmcmcStepIO(position=position.data_ptr(), logdensity=logdensity.data_ptr(),
                               logdensity_grad=grad.data_ptr(), volume_adjustment=J.data_ptr(), z=_lib.ptr(z),
-                              u=_lib.ptr(u), **{k: v.data_ptr() for k, v in out.items()})
+                              u=_lib.ptr(u), num_integration_steps=_lib.ptr(steps),
+                              **{k: v.data_ptr() for k, v in out.items()})
         params = logistic_params(logdensity_fn, C, L=L, step_size=step_size,
                                  divergence_threshold=divergence_threshold, seed=seed, counter=counter)
         rc = _lib.load().gjx_lmc_fisher_logistic_step(params, logdensity_fn.data(), io, _lib.stream_handle(dev))
@@ -130,15 +137,18 @@ def build_kernel(integrator=lan_integrator, divergence_threshold: float = 1000):
         proposal = Proposal(pstate, sq(out["energy"]), sq(out["proposal_weight"]),
                             sq(out["proposal_sum_log_p_accept"]))
         info = LMCInfo(sq(out["velocity"]), sq(out["acceptance_rate"]), sq(out["is_accepted"].bool()),
-                       sq(out["is_divergent"].bool()), sq(out["energy"]), proposal, L)
+                       sq(out["is_divergent"].bool()), sq(out["energy"]), proposal,
+                       L if steps is None else sq(steps))
         return new_state, info
 
     return kernel
 
 
 def sample(rng_key, state, logdensity_fn, step_size, metric_fn, num_integration_steps, num_samples, *,
-           divergence_threshold=1000, store_samples=True, samples_out=None):
-    """``num_samples`` LMC transitions of every chain in one persistent launch."""
+           divergence_threshold=1000, store_samples=True, samples_out=None, max_num_integration_steps=None):
+    """``num_samples`` LMC transitions of every chain in one persistent launch
+    (``num_integration_steps`` an int or a [T, C] integer tensor)."""
+    from .rmhmc import _steps
     from .sampling import SampleResult
 
     position = state.position
@@ -154,6 +164,7 @@ def sample(rng_key, state, logdensity_fn, step_size, metric_fn, num_integration_
     logdensity = state.logdensity.reshape(C).to(torch.float64).contiguous()
     J = torch.as_tensor(state.volume_adjustment, dtype=torch.float64, device=dev).expand(C).contiguous()
     z, u, seed, counter = _noise(rng_key, T, C, D, dev)
+    L, steps = _steps(num_integration_steps, (T, C), dev, max_num_integration_steps)
     final_q = torch.zeros_like(position)
     final_l = torch.empty(C, dtype=torch.float64, device=dev)
     final_g = torch.zeros_like(position)
@@ -168,12 +179,16 @@ def sample(rng_key, state, logdensity_fn, step_size, metric_fn, num_integration_
                             volume_adjustment=J.data_ptr(), z=_lib.ptr(z), u=_lib.ptr(u),
                             final_position=final_q.data_ptr(), final_logdensity=final_l.data_ptr(),
                             final_logdensity_grad=final_g.data_ptr(), final_volume_adjustment=final_J.data_ptr(),
-                            samples=_lib.ptr(samples), acceptance_rate=acc.data_ptr(), num_accepted=n_acc.data_ptr())
-    params = logistic_params(logdensity_fn, C, L=num_integration_steps, T=T, step_size=step_size,
+                            samples=_lib.ptr(samples), acceptance_rate=acc.data_ptr(), num_accepted=n_acc.data_ptr(),
+                            num_integration_steps=_lib.ptr(steps))
+    params = logistic_params(logdensity_fn, C, L=L, T=T, step_size=step_size,
                              divergence_threshold=divergence_threshold, seed=seed, counter=counter)
     rc = _lib.load().gjx_lmc_fisher_logistic_sample(params, logdensity_fn.data(), io, _lib.stream_handle(dev))
     _lib.check(rc, "gjx_lmc_fisher_logistic_sample")
-    fev = torch.full((C,), T * int(num_integration_steps), dtype=torch.int64, device=dev)
+    if steps is None:
+        fev = torch.full((C,), T * L, dtype=torch.int64, device=dev)
+    else:
+        fev = steps.clamp(0, L).to(torch.int64).sum(0)
     return SampleResult(LMCState(final_q, final_l, final_g, final_J), samples, acc, n_acc, fev)
 
 
@@ -198,5 +213,88 @@ class lmc:
         def sample_fn(rng_key, state, num_samples, **kw):
             return sample(rng_key, state, logdensity_fn, step_size, metric_fn, num_integration_steps, num_samples,
                           divergence_threshold=divergence_threshold, **kw)
+
+        return BoundAlgorithm(init_fn, step_fn, sample_fn)
+
+
+# ---------------------------------------------------------------------------
+# Dynamic LMC (geomjax/lmcmc/lmc.py:45-57,104-111,185-252,349-408)
+# ---------------------------------------------------------------------------
+
+
+class DynamicLMCState(NamedTuple):  # geomjax/lmcmc/lmc.py:45-57 (same fields and order)
+    position: torch.Tensor
+    logdensity: torch.Tensor
+    logdensity_grad: torch.Tensor
+    volume_adjustment: torch.Tensor
+    random_generator_arg: torch.Tensor  # [C] int64, one per chain
+
+
+def init_dynamic(position, logdensity_fn, random_generator_arg):
+    """geomjax/lmcmc/lmc.py:104-111; ``random_generator_arg`` is a scalar or a [C]
+    int64 tensor."""
+    s = init(position, logdensity_fn)
+    arg = torch.as_tensor(random_generator_arg, dtype=torch.int64, device=s.position.device)
+    C = s.position.shape[0] if s.position.dim() == 2 else 1
+    return DynamicLMCState(s.position, s.logdensity, s.logdensity_grad, s.volume_adjustment, arg.expand(C).clone())
+
+
+def build_dynamic_kernel(integrator=lan_integrator, divergence_threshold: float = 1000, next_random_arg_fn=None,
+                         integration_steps_fn=None, max_num_integration_steps=9):
+    """geomjax/lmcmc/lmc.py:185-252: the LMC kernel with a number of integration
+    steps drawn per chain from its ``random_generator_arg``."""
+    from .rmhmc import _integration_steps, _next_random_arg
+
+    next_random_arg_fn = next_random_arg_fn or _next_random_arg
+    integration_steps_fn = integration_steps_fn or _integration_steps
+    base = build_kernel(integrator, divergence_threshold)
+
+    def kernel(rng_key, state, logdensity_fn, step_size, metric_fn, **integration_steps_kwargs):
+        steps = integration_steps_fn(state.random_generator_arg, **integration_steps_kwargs)
+        lmc_state = LMCState(state.position, state.logdensity, state.logdensity_grad, state.volume_adjustment)
+        new, info = base(rng_key, lmc_state, logdensity_fn, step_size, metric_fn, steps,
+                         max_num_integration_steps=max_num_integration_steps)
+        return (DynamicLMCState(new.position, new.logdensity, new.logdensity_grad, new.volume_adjustment,
+                                next_random_arg_fn(state.random_generator_arg)), info)
+
+    return kernel
+
+
+class dynamic_lmc:
+    """geomjax/lmcmc/lmc.py:349-408."""
+
+    init = staticmethod(init_dynamic)
+    build_kernel = staticmethod(build_dynamic_kernel)
+
+    def __new__(cls, logdensity_fn, step_size, metric_fn, *, divergence_threshold: int = 1000,
+                integrator=lan_integrator, next_random_arg_fn=None, integration_steps_fn=None,
+                max_num_integration_steps=9):
+        from .rmhmc import _integration_steps, _next_random_arg
+        from .sampling import BoundAlgorithm
+
+        next_random_arg_fn = next_random_arg_fn or _next_random_arg
+        integration_steps_fn = integration_steps_fn or _integration_steps
+        kernel = cls.build_kernel(integrator, divergence_threshold, next_random_arg_fn, integration_steps_fn,
+                                  max_num_integration_steps)
+
+        def init_fn(position, random_generator_arg):
+            return cls.init(position, logdensity_fn, random_generator_arg)
+
+        def step_fn(rng_key, state):
+            return kernel(rng_key, state, logdensity_fn, step_size, metric_fn)
+
+        def sample_fn(rng_key, state, num_samples, **kw):
+            """``num_samples`` transitions in one persistent launch; the step counts
+            of all transitions are drawn up front from the successive arguments."""
+            args, a = [], state.random_generator_arg
+            for _ in range(int(num_samples)):
+                args.append(integration_steps_fn(a))
+                a = next_random_arg_fn(a)
+            r = sample(rng_key, LMCState(state.position, state.logdensity, state.logdensity_grad,
+                                         state.volume_adjustment), logdensity_fn, step_size, metric_fn,
+                       torch.stack(args), num_samples, divergence_threshold=divergence_threshold,
+                       max_num_integration_steps=max_num_integration_steps, **kw)
+            return r._replace(state=DynamicLMCState(r.state.position, r.state.logdensity, r.state.logdensity_grad,
+                                                    r.state.volume_adjustment, a))
 
         return BoundAlgorithm(init_fn, step_fn, sample_fn)

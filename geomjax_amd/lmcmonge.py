@@ -7,6 +7,14 @@ step_size, inverse_mass_matrix, num_integration_steps, alpha2)`` and the
 ``lmc`` algorithm class (exported as ``geomjax_amd.lmcmonge`` like the
 reference's alias, geomjax/__init__.py:24).
 
+``dynamic_lmc`` (``DynamicLMCState``, ``init_dynamic``,
+``build_dynamic_kernel``) is the reference's geomjax/lmcmonge/lmc.py:47-61,
+112-127,240-309,408-469 with its state handling fixed: as written there the
+kernel builds ``LMCState`` with an extra alpha2 field and never passes alpha2 to
+the base kernel, so it cannot run; here the state's alpha2 is passed through
+and kept.  Step counts are drawn per chain on the device (as
+``geomjax_amd.rmhmc.dynamic_rmhmc``).
+
 ``reference_faithful=True`` (default) reproduces the reference bit for bit in
 algebra, including the Sherman-Morrison correction without alpha2
 (integrators.py:186-189) and the momentum's extra factor L (lmc.py:202-204);
@@ -24,7 +32,7 @@ from .proposal import Proposal
 from .random import HostNoise, PRNGKey
 
 __all__ = ["LMCState", "LMCInfo", "RiemannianIntegratorState", "init", "build_kernel", "lmc", "lan_integrator",
-           "sample"]
+           "sample", "DynamicLMCState", "init_dynamic", "build_dynamic_kernel", "dynamic_lmc"]
 
 
 class RiemannianIntegratorState(NamedTuple):  # geomjax/lmcmonge/integrators.py:26-44
@@ -116,7 +124,12 @@ def build_kernel(integrator=lan_integrator, divergence_threshold: float = 1000, 
     if integrator is not lan_integrator:
         raise NotImplementedError("only lan_integrator is implemented on the GPU")
 
-    def kernel(rng_key, state, logdensity_fn, step_size, inverse_mass_matrix, num_integration_steps, alpha2):
+    def kernel(rng_key, state, logdensity_fn, step_size, inverse_mass_matrix, num_integration_steps, alpha2, *,
+               max_num_integration_steps=None):
+        """``num_integration_steps``: an int, or a [C] integer tensor of per-chain
+        counts (the dynamic kernel's form), bounded by ``max_num_integration_steps``."""
+        from .rmhmc import _steps
+
         _check_alpha2(alpha2)
         single = state.position.dim() == 1
         position = (state.position[None] if single else state.position).contiguous()
@@ -127,7 +140,7 @@ def build_kernel(integrator=lan_integrator, divergence_threshold: float = 1000, 
         grad = state.logdensity_grad.reshape(C, D).to(position.dtype).contiguous()
         J = torch.as_tensor(state.volume_adjustment, dtype=torch.float64, device=dev).expand(C).contiguous()
         z, u, seed, counter = _noise(rng_key, None, C, D, dev, position.dtype)
-        L = int(num_integration_steps)
+        L, steps = _steps(num_integration_steps, (C,), dev, max_num_integration_steps)
 
         def vec():
             return torch.empty(C, D, dtype=position.dtype, device=dev)
@@ -145,7 +158,7 @@ def build_kernel(integrator=lan_integrator, divergence_threshold: float = 1000, 
         io = _lib.LmcStepIO(inverse_mass_matrix=im.data_ptr(), position=position.data_ptr(),
                             logdensity=logdensity.data_ptr(), logdensity_grad=grad.data_ptr(),
                             volume_adjustment=J.data_ptr(), z=_lib.ptr(z), u=_lib.ptr(u),
-                            **{k: v.data_ptr() for k, v in out.items()})
+                            num_integration_steps=_lib.ptr(steps), **{k: v.data_ptr() for k, v in out.items()})
         params = _params(C, D, _dtype_code(position), step_size, L, alpha2, logdensity_fn.sigma,
                          divergence_threshold, reference_faithful, lpc=lanes_per_chain, seed=seed, counter=counter)
         rc = _lib.load().gjx_lmcmonge_funnel_step(params, io, _lib.stream_handle(dev))
@@ -165,7 +178,8 @@ def build_kernel(integrator=lan_integrator, divergence_threshold: float = 1000, 
         proposal = Proposal(pstate, sq(out["energy"]), sq(out["proposal_weight"]),
                             sq(out["proposal_sum_log_p_accept"]))
         info = LMCInfo(sq(out["velocity"]), sq(out["acceptance_rate"]), sq(out["is_accepted"].bool()),
-                       sq(out["is_divergent"].bool()), sq(out["energy"]), proposal, L)
+                       sq(out["is_divergent"].bool()), sq(out["energy"]), proposal,
+                       L if steps is None else sq(steps))
         return new_state, info
 
     return kernel
@@ -173,8 +187,10 @@ def build_kernel(integrator=lan_integrator, divergence_threshold: float = 1000, 
 
 def sample(rng_key, state, logdensity_fn, step_size, inverse_mass_matrix, num_integration_steps, alpha2,
            num_samples, *, divergence_threshold=1000, reference_faithful=True, store_samples=True, lanes_per_chain=0,
-           samples_out=None):
-    """``num_samples`` LMC-Monge transitions of every chain in one persistent launch."""
+           samples_out=None, max_num_integration_steps=None):
+    """``num_samples`` LMC-Monge transitions of every chain in one persistent launch
+    (``num_integration_steps`` an int or a [T, C] integer tensor)."""
+    from .rmhmc import _steps
     from .sampling import SampleResult
 
     _check_alpha2(alpha2)
@@ -191,6 +207,7 @@ def sample(rng_key, state, logdensity_fn, step_size, inverse_mass_matrix, num_in
     logdensity = state.logdensity.reshape(C).to(torch.float64).contiguous()
     J = torch.as_tensor(state.volume_adjustment, dtype=torch.float64, device=dev).expand(C).contiguous()
     z, u, seed, counter = _noise(rng_key, T, C, D, dev, position.dtype)
+    L, steps = _steps(num_integration_steps, (T, C), dev, max_num_integration_steps)
     final_q = torch.empty_like(position)
     final_l = torch.empty(C, dtype=torch.float64, device=dev)
     final_g = torch.empty_like(position)
@@ -207,13 +224,17 @@ def sample(rng_key, state, logdensity_fn, step_size, inverse_mass_matrix, num_in
                           logdensity=logdensity.data_ptr(), volume_adjustment=J.data_ptr(), z=_lib.ptr(z),
                           u=_lib.ptr(u), final_position=final_q.data_ptr(), final_logdensity=final_l.data_ptr(),
                           final_logdensity_grad=final_g.data_ptr(), final_volume_adjustment=final_J.data_ptr(),
-                          samples=_lib.ptr(samples), acceptance_rate=acc.data_ptr(), num_accepted=n_acc.data_ptr())
-    params = _params(C, D, _dtype_code(position), step_size, num_integration_steps, alpha2, logdensity_fn.sigma,
+                          samples=_lib.ptr(samples), acceptance_rate=acc.data_ptr(), num_accepted=n_acc.data_ptr(),
+                          num_integration_steps=_lib.ptr(steps))
+    params = _params(C, D, _dtype_code(position), step_size, L, alpha2, logdensity_fn.sigma,
                      divergence_threshold, reference_faithful, T=T, lpc=lanes_per_chain, seed=seed, counter=counter)
     rc = _lib.load().gjx_lmcmonge_funnel_sample(params, io, _lib.stream_handle(dev))
     _lib.check(rc, "gjx_lmcmonge_funnel_sample")
     # every LMC step evaluates the vector field once (one gradient + Hessian-vector products)
-    fev = torch.full((C,), T * int(num_integration_steps), dtype=torch.int64, device=dev)
+    if steps is None:
+        fev = torch.full((C,), T * L, dtype=torch.int64, device=dev)
+    else:
+        fev = steps.clamp(0, L).to(torch.int64).sum(0)
     return SampleResult(LMCState(final_q, final_l, final_g, final_J), samples, acc, n_acc, fev)
 
 
@@ -240,5 +261,93 @@ class lmc:
             return sample(rng_key, state, logdensity_fn, step_size, inverse_mass_matrix, num_integration_steps,
                           alpha2, num_samples, divergence_threshold=divergence_threshold,
                           reference_faithful=reference_faithful, **kw)
+
+        return BoundAlgorithm(init_fn, step_fn, sample_fn)
+
+
+# ---------------------------------------------------------------------------
+# Dynamic LMC-Monge (geomjax/lmcmonge/lmc.py:47-61,112-127,240-309,408-469)
+# ---------------------------------------------------------------------------
+
+
+class DynamicLMCState(NamedTuple):  # geomjax/lmcmonge/lmc.py:47-61 (same fields and order)
+    alpha2: float
+    position: torch.Tensor
+    logdensity: torch.Tensor
+    logdensity_grad: torch.Tensor
+    volume_adjustment: torch.Tensor
+    random_generator_arg: torch.Tensor  # [C] int64, one per chain
+
+
+def init_dynamic(position, logdensity_fn, alpha2, random_generator_arg):
+    """geomjax/lmcmonge/lmc.py:112-127; ``random_generator_arg`` is a scalar or a
+    [C] int64 tensor."""
+    _check_alpha2(alpha2)
+    s = init(position, logdensity_fn)
+    arg = torch.as_tensor(random_generator_arg, dtype=torch.int64, device=s.position.device)
+    C = s.position.shape[0] if s.position.dim() == 2 else 1
+    return DynamicLMCState(float(alpha2), s.position, s.logdensity, s.logdensity_grad, s.volume_adjustment,
+                           arg.expand(C).clone())
+
+
+def build_dynamic_kernel(integrator=lan_integrator, divergence_threshold: float = 1000, next_random_arg_fn=None,
+                         integration_steps_fn=None, max_num_integration_steps=9, **kernel_kwargs):
+    """geomjax/lmcmonge/lmc.py:240-309: the LMC-Monge kernel with a number of
+    integration steps drawn per chain from its ``random_generator_arg``, run
+    at the state's alpha2 (which the new state keeps)."""
+    from .rmhmc import _integration_steps, _next_random_arg
+
+    next_random_arg_fn = next_random_arg_fn or _next_random_arg
+    integration_steps_fn = integration_steps_fn or _integration_steps
+    base = build_kernel(integrator, divergence_threshold, **kernel_kwargs)
+
+    def kernel(rng_key, state, logdensity_fn, step_size, inverse_mass_matrix, **integration_steps_kwargs):
+        steps = integration_steps_fn(state.random_generator_arg, **integration_steps_kwargs)
+        lmc_state = LMCState(state.position, state.logdensity, state.logdensity_grad, state.volume_adjustment)
+        new, info = base(rng_key, lmc_state, logdensity_fn, step_size, inverse_mass_matrix, steps, state.alpha2,
+                         max_num_integration_steps=max_num_integration_steps)
+        return (DynamicLMCState(state.alpha2, new.position, new.logdensity, new.logdensity_grad,
+                                new.volume_adjustment, next_random_arg_fn(state.random_generator_arg)), info)
+
+    return kernel
+
+
+class dynamic_lmc:
+    """geomjax/lmcmonge/lmc.py:408-469."""
+
+    init = staticmethod(init_dynamic)
+    build_kernel = staticmethod(build_dynamic_kernel)
+
+    def __new__(cls, logdensity_fn, step_size, inverse_mass_matrix, *, alpha2=0.001, divergence_threshold: int = 1000,
+                integrator=lan_integrator, next_random_arg_fn=None, integration_steps_fn=None,
+                max_num_integration_steps=9, reference_faithful=True):
+        from .rmhmc import _integration_steps, _next_random_arg
+        from .sampling import BoundAlgorithm
+
+        next_random_arg_fn = next_random_arg_fn or _next_random_arg
+        integration_steps_fn = integration_steps_fn or _integration_steps
+        kernel = cls.build_kernel(integrator, divergence_threshold, next_random_arg_fn, integration_steps_fn,
+                                  max_num_integration_steps, reference_faithful=reference_faithful)
+
+        def init_fn(position, random_generator_arg):
+            return cls.init(position, logdensity_fn, alpha2, random_generator_arg)
+
+        def step_fn(rng_key, state):
+            return kernel(rng_key, state, logdensity_fn, step_size, inverse_mass_matrix)
+
+        def sample_fn(rng_key, state, num_samples, **kw):
+            """``num_samples`` transitions in one persistent launch; the step counts
+            of all transitions are drawn up front from the successive arguments."""
+            args, a = [], state.random_generator_arg
+            for _ in range(int(num_samples)):
+                args.append(integration_steps_fn(a))
+                a = next_random_arg_fn(a)
+            r = sample(rng_key, LMCState(state.position, state.logdensity, state.logdensity_grad,
+                                         state.volume_adjustment), logdensity_fn, step_size, inverse_mass_matrix,
+                       torch.stack(args), state.alpha2, num_samples, divergence_threshold=divergence_threshold,
+                       reference_faithful=reference_faithful, max_num_integration_steps=max_num_integration_steps,
+                       **kw)
+            return r._replace(state=DynamicLMCState(state.alpha2, r.state.position, r.state.logdensity,
+                                                    r.state.logdensity_grad, r.state.volume_adjustment, a))
 
         return BoundAlgorithm(init_fn, step_fn, sample_fn)

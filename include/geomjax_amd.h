@@ -175,6 +175,7 @@ typedef struct {
   double* proposal_volume_adjustment;  /* [C] optional */
   double* proposal_weight;          /* [C] optional */
   double* proposal_sum_log_p_accept;/* [C] optional */
+  const int32_t* num_integration_steps; /* [C] optional: per-chain counts (dynamic kernel), each in [0, params->num_integration_steps] */
 } gjx_lmc_step_io;
 
 /* Buffers of T LMC-Monge transitions in one persistent launch. */
@@ -192,6 +193,7 @@ typedef struct {
   void* samples;                    /* [T, C, D] optional */
   float* acceptance_rate;           /* [T, C] optional */
   int32_t* num_accepted;            /* [C] optional */
+  const int32_t* num_integration_steps; /* [T, C] optional: per-chain, per-transition counts (dynamic kernel) */
 } gjx_lmc_sample_io;
 
 /* One LMC-Monge transition per chain: the reference kernel
@@ -313,6 +315,7 @@ typedef struct {
   double* proposal_volume_adjustment; /* [C] optional */
   double* proposal_weight;          /* [C] optional */
   double* proposal_sum_log_p_accept;/* [C] optional */
+  const int32_t* num_integration_steps; /* [C] optional: per-chain counts (dynamic kernel), each in [0, params->num_integration_steps] */
 } gjx_lmcmc_step_io;
 
 typedef struct {
@@ -328,6 +331,7 @@ typedef struct {
   void* samples;                    /* [T, C, dim] optional */
   float* acceptance_rate;           /* [T, C] optional */
   int32_t* num_accepted;            /* [C] optional */
+  const int32_t* num_integration_steps; /* [T, C] optional: per-chain, per-transition counts (dynamic kernel) */
 } gjx_lmcmc_sample_io;
 
 /* One LMC transition per chain with metric_fn = Fisher metric: the reference
