@@ -96,6 +96,13 @@ class LmcSampleIO(ctypes.Structure):
     )]
 
 
+class LmcAlpha2TangentIO(ctypes.Structure):
+    _fields_ = [(n, _vp) for n in (
+        "inverse_mass_matrix", "position", "z", "num_integration_steps", "proposal_position",
+        "d_proposal_position_d_alpha2",
+    )]
+
+
 class LogisticParams(ctypes.Structure):
     _fields_ = [
         ("num_chains", _c_int32),
@@ -161,6 +168,7 @@ _SIGNATURES = {
     "gjx_rmhmc_monge_funnel_sample": [ctypes.POINTER(FunnelParams), ctypes.POINTER(RmhmcSampleIO), _vp],
     "gjx_lmcmonge_funnel_step": [ctypes.POINTER(FunnelParams), ctypes.POINTER(LmcStepIO), _vp],
     "gjx_lmcmonge_funnel_sample": [ctypes.POINTER(FunnelParams), ctypes.POINTER(LmcSampleIO), _vp],
+    "gjx_lmcmonge_funnel_alpha2_tangent": [ctypes.POINTER(FunnelParams), ctypes.POINTER(LmcAlpha2TangentIO), _vp],
     "gjx_rmhmc_monge_funnel_tangent": [ctypes.POINTER(FunnelParams), ctypes.POINTER(TangentIO), _vp],
     "gjx_rmhmc_monge_funnel_nuts": [ctypes.POINTER(FunnelParams), _c_int32, _c_int32, ctypes.POINTER(NutsIO), _vp],
     "gjx_rmhmc_fisher_logistic_nuts": [ctypes.POINTER(LogisticParams), ctypes.POINTER(LogisticData), _c_int32,

@@ -331,6 +331,30 @@ int gjx_lmcmonge_funnel_step(const gjx_funnel_params* params, const gjx_lmc_step
   return GJX_OK;
 }
 
+int gjx_lmcmonge_funnel_alpha2_tangent(const gjx_funnel_params* params, const gjx_lmc_alpha2_tangent_io* io,
+                                       void* stream) {
+  int lpc = 0, epl = 0;
+  int rc = check_funnel_params(params, false, &lpc, &epl);
+  if (rc != GJX_OK) return rc;
+  GJX_REQUIRE(params->dtype == GJX_F64, "the alpha2 tangent runs in float64");
+  GJX_REQUIRE((params->flags & ~3) == 0, "unknown flags 0x%x", params->flags);
+  GJX_REQUIRE(params->alpha2 > 0.0, "alpha2 must be > 0 for its derivative (got %g)", params->alpha2);
+  GJX_REQUIRE(io != nullptr, "io is NULL");
+  GJX_REQUIRE(io->inverse_mass_matrix && io->position, "inverse_mass_matrix and position are required");
+  GJX_REQUIRE(io->d_proposal_position_d_alpha2, "d_proposal_position_d_alpha2 is required");
+  gjx::FunnelLmcArgs a = lmc_args(params);
+  a.T = 1;
+  a.inv_mass = io->inverse_mass_matrix;
+  a.q_in = io->position;
+  a.z = io->z;
+  a.Lc = io->num_integration_steps;
+  a.prop_q = io->proposal_position;
+  a.d_prop_q = io->d_proposal_position_d_alpha2;
+  hipError_t e = gjx::launch_funnel_lmc_alpha2_tangent(a, lpc, epl, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "gjx_lmcmonge_funnel_alpha2_tangent launch");
+  return GJX_OK;
+}
+
 int gjx_lmcmonge_funnel_sample(const gjx_funnel_params* params, const gjx_lmc_sample_io* io, void* stream) {
   int lpc = 0, epl = 0;
   int rc = check_funnel_params(params, true, &lpc, &epl);

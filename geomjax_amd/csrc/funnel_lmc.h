@@ -53,8 +53,12 @@ struct FunnelLmcArgs {
   void* samples;
   float* sample_acc;
   int32_t* num_accepted;
+  // tangent kernel: d(proposal position)/d alpha2, [C, D] float64
+  double* d_prop_q;
 };
 
 hipError_t launch_funnel_lmc(const FunnelLmcArgs& a, int dtype, int lpc, int epl, hipStream_t stream);
+// forward-mode derivative of the proposal position in alpha2 (float64)
+hipError_t launch_funnel_lmc_alpha2_tangent(const FunnelLmcArgs& a, int lpc, int epl, hipStream_t stream);
 
 }  // namespace gjx

@@ -23,6 +23,7 @@
 //   a_j = sqrt(alpha2) dl_j / sqrt(mi_j),  t_j = 1 - sum_{k<=j} a_k^2,
 //   v_j = sqrt(mi_j) (sqrt(t_j/t_{j-1}) z_j - a_j sum_{k<j} a_k z_k / sqrt(t_k t_{k-1})).
 #include "common.h"
+#include "dual.h"
 #include "funnel_lmc.h"
 
 namespace gjx {
@@ -98,18 +99,130 @@ __device__ __forceinline__ void half_step(const PosScalars<R>& P, const R (&x)[E
   const R P7 = s7 + P.Hdlv * P.dlv;   // Hdl . dl
   // 1st volume adjustment (:178-179)
   const R d1 = R(1) + h * a2 * P1;
-  dJ -= log(fabs((double)d1));
+  dJ -= log(fabs((double)val(d1)));
   // v_t = v - (eps/2) sqrt(L) (dphi_ig - alpha2 (dphi.dl) dl)   (:182-185)
   const R vtH = P3 - h * P.sL * (P4 - a2 * P2 * P1);
   R coef = h * vtH / d1;  // (:186-189); the reference has no alpha2 here
   if (exact_sm) coef *= a2;
   const R HdlVn = P5 - h * P.sL * (P6 - a2 * P2 * P7) - coef * P7;
   // 2nd volume adjustment (:192)
-  dJ += log(fabs((double)(R(1) - h * a2 * HdlVn)));
+  dJ += log(fabs((double)val(R(1) - h * a2 * HdlVn)));
   const R kq = h * P.sL, kd = a2 * P2;
 #pragma unroll
   for (int k = 0; k < EPL; ++k) vx[k] = vx[k] - kq * (dphig[k] - kd * dl[k]) - coef * dl[k];
   vv = vv - kq * (dphig_v - kd * P.dlv) - coef * P.dlv;
+}
+
+// Scalars of the position (x, v) and the inclusive prefix of mi x^2 over the
+// chain's coordinates in lane order (the velocity draw's running sums).
+template <class R, int LPC, int EPL>
+__device__ __forceinline__ PosScalars<R> position_scalars(const R (&x)[EPL], R v, const R (&mi)[EPL], R (&loc_inc)[EPL],
+                                                          int lg, R a2, R inv_s2, R halfDx, R miv) {
+  R sx = R(0), smx = R(0);
+#pragma unroll
+  for (int k = 0; k < EPL; ++k) {
+    sx += x[k] * x[k];
+    smx += mi[k] * x[k] * x[k];
+    loc_inc[k] = smx;
+  }
+  const R off = group_exclusive_scan<LPC>(smx, lg);
+#pragma unroll
+  for (int k = 0; k < EPL; ++k) loc_inc[k] += off;
+  sx = group_sum<LPC>(sx);
+  smx = group_sum<LPC>(smx);
+  return pos_scalars<R>(v, sx, smx, a2, inv_s2, halfDx, miv);
+}
+
+// The transition's standard normals for the velocity draw and its accept
+// uniform: host-fed ([T, C, D] in storage type T, [T, C]) or the device Philox
+// streams at counter + t.
+template <class T, int EPL>
+__device__ __forceinline__ void transition_noise(const FunnelLmcArgs& a, size_t c, int t, int j0, int Dx, bool live,
+                                                 T (&z)[EPL], T& zv, double& uacc) {
+  const int D = Dx + 1;
+  const uint64_t tt = a.counter + (uint64_t)t;
+  const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
+  if (a.z != nullptr) {
+    const size_t base = ((size_t)t * a.C + c) * D;
+#pragma unroll
+    for (int k = 0; k < EPL; ++k) z[k] = (live && j0 + k < Dx) ? ld<T>(a.z, base + j0 + k) : T(0);
+    zv = live ? ld<T>(a.z, base + Dx) : T(0);
+    uacc = (live && a.u != nullptr) ? a.u[(size_t)t * a.C + c] : 1.0;
+    return;
+  }
+#pragma unroll
+  for (int k = 0; k < EPL; k += 2) {
+    u32x4 r = philox4x32_10(u32x4{(uint32_t)c, (uint32_t)tt, kStreamMomentum + (uint32_t)(j0 + k), (uint32_t)(tt >> 32)},
+                            k0, k1);
+    T n0, n1;
+    normal2(r, n0, n1);
+    z[k] = j0 + k < Dx ? n0 : T(0);
+    if (k + 1 < EPL) z[k + 1] = j0 + k + 1 < Dx ? n1 : T(0);
+  }
+  {
+    u32x4 r = philox4x32_10(u32x4{(uint32_t)c, (uint32_t)tt, kStreamMomentumV, (uint32_t)(tt >> 32)}, k0, k1);
+    T n0, n1;
+    normal2(r, n0, n1);
+    zv = n0;
+  }
+  u32x4 r = philox4x32_10(u32x4{(uint32_t)c, (uint32_t)tt, kStreamAccept, (uint32_t)(tt >> 32)}, k0, k1);
+  uacc = u01_f64(r.x, r.y) - 0.5 * (1.0 / 9007199254740992.0);
+}
+
+// Velocity v0 = chol(M^-1 - alpha2 dl dl^T) z (geomjax/lmcmonge/metrics.py:155-166)
+// with the exact lower factor of a diagonal-minus-rank-one matrix.
+template <class R, int LPC, int EPL>
+__device__ __forceinline__ void draw_velocity(const PosScalars<R>& P, const R (&x)[EPL], const R (&mi)[EPL],
+                                              const R (&loc_inc)[EPL], const R (&z)[EPL], R zv, R a2, R alpha,
+                                              R miv, int lg, int j0, int Dx, R (&vx)[EPL], R& vv) {
+  // a_j^2 = alpha2 dl_j^2 / mi_j = (alpha2 e^2 / L) mi_j x_j^2
+  const R w = a2 * P.e * P.e / P.L;
+  R cj[EPL];
+  R acc2 = R(0);
+#pragma unroll
+  for (int k = 0; k < EPL; ++k) {
+    const R tj = R(1) - w * loc_inc[k];
+    const R tjm = R(1) - w * (loc_inc[k] - mi[k] * x[k] * x[k]);
+    const R aj = -alpha * P.e * sqrt(mi[k]) * x[k] * P.isL;
+    cj[k] = aj * z[k] / sqrt(tj * tjm);
+    vx[k] = sqrt(tj / tjm) * z[k];
+    acc2 += cj[k];
+  }
+  const R off2 = group_exclusive_scan<LPC>(acc2, lg);
+  const R tot2 = group_sum<LPC>(acc2);
+  R run = off2;
+#pragma unroll
+  for (int k = 0; k < EPL; ++k) {
+    const R aj = -alpha * P.e * sqrt(mi[k]) * x[k] * P.isL;
+    vx[k] = j0 + k < Dx ? sqrt(mi[k]) * (vx[k] - aj * run) : R(0);
+    run += cj[k];
+  }
+  // t after all x coordinates is 1 - alpha2 e^2 Smx / L; then the log-scale coordinate
+  const R txe = R(1) - w * P.Smx;
+  const R av = alpha * sqrt(miv) * P.gv * P.isL;
+  const R tv = txe - av * av;
+  vv = sqrt(miv) * (sqrt(tv / txe) * zv - av * tot2);
+}
+
+// One lan_integrator step (geomjax/lmcmonge/integrators.py:52-156): half step of
+// the velocity, full step of the position, half step of the velocity.
+template <class R, int LPC, int EPL>
+__device__ __forceinline__ void lmc_step(PosScalars<R>& P, R (&xc)[EPL], R& vc, const R (&mi)[EPL], R (&vx)[EPL],
+                                         R& vv, double& dJ, R eps, R h, R a2, R inv_s2, R halfDx, R miv,
+                                         bool exact_sm) {
+  half_step<R, LPC, EPL>(P, xc, mi, vx, vv, dJ, a2, inv_s2, h, exact_sm);
+  R sx = R(0), smx = R(0);
+#pragma unroll
+  for (int k = 0; k < EPL; ++k) {
+    xc[k] = xc[k] + eps * vx[k];
+    sx += xc[k] * xc[k];
+    smx += mi[k] * xc[k] * xc[k];
+  }
+  vc = vc + eps * vv;
+  sx = group_sum<LPC>(sx);
+  smx = group_sum<LPC>(smx);
+  P = pos_scalars<R>(vc, sx, smx, a2, inv_s2, halfDx, miv);
+  half_step<R, LPC, EPL>(P, xc, mi, vx, vv, dJ, a2, inv_s2, h, exact_sm);
 }
 
 }  // namespace
@@ -163,91 +276,17 @@ __global__ void __launch_bounds__(256) funnel_lmc_kernel(const FunnelLmcArgs a) 
 
   for (int t = 0; t < T_steps; ++t) {
     // ---- position quantities at q0 (lmc.py:177-200) ----
-    PosScalars<R> P;
     R loc_inc[EPL];
-    {
-      R sx = R(0), smx = R(0);
-#pragma unroll
-      for (int k = 0; k < EPL; ++k) {
-        sx += x[k] * x[k];
-        smx += mi[k] * x[k] * x[k];
-        loc_inc[k] = smx;
-      }
-      const R Smx_lane = smx;
-      const R off = group_exclusive_scan<LPC>(Smx_lane, lg);
-#pragma unroll
-      for (int k = 0; k < EPL; ++k) loc_inc[k] += off;  // inclusive prefix of mi x^2 over coordinates
-      sx = group_sum<LPC>(sx);
-      smx = group_sum<LPC>(smx);
-      P = pos_scalars<R>(v, sx, smx, a2, inv_s2, halfDx, miv);
-    }
+    PosScalars<R> P = position_scalars<R, LPC, EPL>(x, v, mi, loc_inc, lg, a2, inv_s2, halfDx, miv);
 
-    // ---- noise ----
+    // ---- noise and velocity v0 = chol(M^-1 - alpha2 dl dl^T) z  (metrics.py:155-166) ----
     R z[EPL];
     R zv;
     double uacc;
-    {
-      const uint64_t tt = a.counter + (uint64_t)t;
-      const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
-      if (a.z != nullptr) {
-        const size_t base = ((size_t)t * a.C + c) * D;
-#pragma unroll
-        for (int k = 0; k < EPL; ++k) z[k] = (live && GJX_ON(k)) ? ld<T>(a.z, base + j0 + k) : R(0);
-        zv = live ? ld<T>(a.z, base + Dx) : R(0);
-        uacc = live ? a.u[(size_t)t * a.C + c] : 1.0;
-      } else {
-#pragma unroll
-        for (int k = 0; k < EPL; k += 2) {
-          u32x4 r = philox4x32_10(
-              u32x4{(uint32_t)c, (uint32_t)tt, kStreamMomentum + (uint32_t)(j0 + k), (uint32_t)(tt >> 32)}, k0, k1);
-          R n0, n1;
-          normal2(r, n0, n1);
-          z[k] = GJX_ON(k) ? n0 : R(0);
-          if (k + 1 < EPL) z[k + 1] = GJX_ON(k + 1) ? n1 : R(0);
-        }
-        {
-          u32x4 r = philox4x32_10(u32x4{(uint32_t)c, (uint32_t)tt, kStreamMomentumV, (uint32_t)(tt >> 32)}, k0, k1);
-          R n0, n1;
-          normal2(r, n0, n1);
-          zv = n0;
-        }
-        u32x4 r = philox4x32_10(u32x4{(uint32_t)c, (uint32_t)tt, kStreamAccept, (uint32_t)(tt >> 32)}, k0, k1);
-        uacc = u01_f64(r.x, r.y) - 0.5 * (1.0 / 9007199254740992.0);
-      }
-    }
-
-    // ---- velocity v0 = chol(M^-1 - alpha2 dl dl^T) z  (metrics.py:155-166) ----
+    transition_noise<T, EPL>(a, c, t, j0, Dx, live, z, zv, uacc);
     R vx[EPL];
     R vv;
-    {
-      // a_j^2 = alpha2 dl_j^2 / mi_j = (alpha2 e^2 / L) mi_j x_j^2
-      const R w = a2 * P.e * P.e / P.L;
-      R cj[EPL];
-      R acc2 = R(0);
-#pragma unroll
-      for (int k = 0; k < EPL; ++k) {
-        const R tj = R(1) - w * loc_inc[k];
-        const R tjm = R(1) - w * (loc_inc[k] - mi[k] * x[k] * x[k]);
-        const R aj = -alpha * P.e * sqrt(mi[k]) * x[k] * P.isL;
-        cj[k] = aj * z[k] / sqrt(tj * tjm);
-        vx[k] = sqrt(tj / tjm) * z[k];
-        acc2 += cj[k];
-      }
-      const R off2 = group_exclusive_scan<LPC>(acc2, lg);
-      const R tot2 = group_sum<LPC>(acc2);
-      R run = off2;
-#pragma unroll
-      for (int k = 0; k < EPL; ++k) {
-        const R aj = -alpha * P.e * sqrt(mi[k]) * x[k] * P.isL;
-        vx[k] = GJX_ON(k) ? sqrt(mi[k]) * (vx[k] - aj * run) : R(0);
-        run += cj[k];
-      }
-      // t after all x coordinates is 1 - alpha2 e^2 Smx / L; then the log-scale coordinate
-      const R txe = R(1) - w * P.Smx;
-      const R av = alpha * sqrt(miv) * P.gv * P.isL;
-      const R tv = txe - av * av;
-      vv = sqrt(miv) * (sqrt(tv / txe) * zv - av * tot2);
-    }
+    draw_velocity<R, LPC, EPL>(P, x, mi, loc_inc, z, zv, a2, alpha, miv, lg, j0, Dx, vx, vv);
     if (STEP && a.velocity != nullptr && live) {
 #pragma unroll
       for (int k = 0; k < EPL; ++k)
@@ -283,21 +322,7 @@ __global__ void __launch_bounds__(256) funnel_lmc_kernel(const FunnelLmcArgs a) 
     for (int s = 0; s < a.L; ++s) {
       const bool on = s < Ls;
       if (!__any(on)) break;
-      if (on) {
-        half_step<R, LPC, EPL>(P, xc, mi, vx, vv, dJ, a2, inv_s2, h, exact_sm);
-        R sx = R(0), smx = R(0);
-#pragma unroll
-        for (int k = 0; k < EPL; ++k) {
-          xc[k] = xc[k] + eps * vx[k];
-          sx += xc[k] * xc[k];
-          smx += mi[k] * xc[k] * xc[k];
-        }
-        vc = vc + eps * vv;
-        sx = group_sum<LPC>(sx);
-        smx = group_sum<LPC>(smx);
-        P = pos_scalars<R>(vc, sx, smx, a2, inv_s2, halfDx, miv);
-        half_step<R, LPC, EPL>(P, xc, mi, vx, vv, dJ, a2, inv_s2, h, exact_sm);
-      }
+      if (on) lmc_step<R, LPC, EPL>(P, xc, vc, mi, vx, vv, dJ, eps, h, a2, inv_s2, halfDx, miv, exact_sm);
     }
 
     // ---- energy at the end (lmcmonge_energy, metrics.py:267-284) ----
@@ -434,6 +459,74 @@ __global__ void __launch_bounds__(256) funnel_lmc_kernel(const FunnelLmcArgs a) 
 #undef GJX_ON
 }
 
+// Forward-mode tangent of one LMC-Monge transition in alpha2: the derivative of
+// the proposal position that geomjax/adaptation/chees_adaptation_lmcmonge.py:
+// 475-519 takes with jax.jacfwd.  The float64 transition runs on
+// Dual<double, 1> seeded d alpha2 = 1, through the velocity draw and every
+// integrator step (the helpers above, as in funnel_lmc_kernel); the noise is
+// the primal kernel's at the same (seed, counter), so value and tangent belong
+// to the trajectory the primal launch integrates.  Outputs the proposal position
+// (prop_q, optional) and its derivative (d_prop_q).
+template <int LPC, int EPL>
+__global__ void __launch_bounds__(256) funnel_lmc_alpha2_tangent_kernel(const FunnelLmcArgs a) {
+  using R = Dual<double, 1>;
+  const int tid = blockIdx.x * 256 + threadIdx.x;
+  const int chain = tid / LPC;
+  const int lg = tid % LPC;
+  const bool live = chain < a.C;
+  const size_t c = live ? (size_t)chain : 0;
+  const int D = a.D, Dx = a.D - 1;
+  const int j0 = lg * EPL;
+  const R eps = R(a.eps);
+  const R h = R(0.5 * a.eps);
+  R a2 = R(a.alpha2);
+  a2.d[0] = 1.0;
+  const R alpha = sqrt(a2);
+  const R inv_s2 = R(1.0 / (a.sigma * a.sigma));
+  const R halfDx = R(0.5 * Dx);
+  const bool exact_sm = a.exact_sm != 0;
+  R mi[EPL];
+#pragma unroll
+  for (int k = 0; k < EPL; ++k) mi[k] = R(j0 + k < Dx ? ld<double>(a.inv_mass, j0 + k) : 1.0);
+  const R miv = R(ld<double>(a.inv_mass, Dx));
+  R x[EPL];
+#pragma unroll
+  for (int k = 0; k < EPL; ++k) x[k] = R((live && j0 + k < Dx) ? ld<double>(a.q_in, c * D + j0 + k) : 0.0);
+  const R v = R(live ? ld<double>(a.q_in, c * D + Dx) : 0.0);
+  R loc_inc[EPL];
+  PosScalars<R> P = position_scalars<R, LPC, EPL>(x, v, mi, loc_inc, lg, a2, inv_s2, halfDx, miv);
+  double zd[EPL], zvd, uacc;
+  transition_noise<double, EPL>(a, c, 0, j0, Dx, live, zd, zvd, uacc);
+  R z[EPL];
+#pragma unroll
+  for (int k = 0; k < EPL; ++k) z[k] = R(zd[k]);
+  R vx[EPL];
+  R vv;
+  draw_velocity<R, LPC, EPL>(P, x, mi, loc_inc, z, R(zvd), a2, alpha, miv, lg, j0, Dx, vx, vv);
+  R xc[EPL];
+  R vc = v;
+#pragma unroll
+  for (int k = 0; k < EPL; ++k) xc[k] = x[k];
+  double dJ = 0.0;
+  const int Ls = live ? chain_steps(a.Lc, a.L, a.C, 0, (int)c) : 0;
+  for (int s = 0; s < a.L; ++s) {
+    const bool on = s < Ls;
+    if (!__any(on)) break;
+    if (on) lmc_step<R, LPC, EPL>(P, xc, vc, mi, vx, vv, dJ, eps, h, a2, inv_s2, halfDx, miv, exact_sm);
+  }
+  if (!live) return;
+#pragma unroll
+  for (int k = 0; k < EPL; ++k) {
+    if (j0 + k >= Dx) continue;
+    if (a.prop_q) st<double>(a.prop_q, c * D + j0 + k, xc[k].v);
+    a.d_prop_q[c * D + j0 + k] = xc[k].d[0];
+  }
+  if (lg == 0) {
+    if (a.prop_q) st<double>(a.prop_q, c * D + Dx, vc.v);
+    a.d_prop_q[c * D + Dx] = vc.d[0];
+  }
+}
+
 namespace {
 
 template <class T, int LPC, int EPL>
@@ -472,6 +565,36 @@ hipError_t launch_lpc(const FunnelLmcArgs& a, int lpc, int epl, hipStream_t stre
 }
 
 }  // namespace
+
+namespace {
+
+template <int LPC>
+hipError_t launch_tangent_epl(const FunnelLmcArgs& a, int epl, hipStream_t stream) {
+  const long long lanes = (long long)a.C * LPC;
+  const unsigned blocks = (unsigned)((lanes + 255) / 256);
+  switch (epl) {
+    case 1: hipLaunchKernelGGL((funnel_lmc_alpha2_tangent_kernel<LPC, 1>), dim3(blocks), dim3(256), 0, stream, a); break;
+    case 2: hipLaunchKernelGGL((funnel_lmc_alpha2_tangent_kernel<LPC, 2>), dim3(blocks), dim3(256), 0, stream, a); break;
+    case 4: hipLaunchKernelGGL((funnel_lmc_alpha2_tangent_kernel<LPC, 4>), dim3(blocks), dim3(256), 0, stream, a); break;
+    case 8: hipLaunchKernelGGL((funnel_lmc_alpha2_tangent_kernel<LPC, 8>), dim3(blocks), dim3(256), 0, stream, a); break;
+    case 16: hipLaunchKernelGGL((funnel_lmc_alpha2_tangent_kernel<LPC, 16>), dim3(blocks), dim3(256), 0, stream, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_funnel_lmc_alpha2_tangent(const FunnelLmcArgs& a, int lpc, int epl, hipStream_t stream) {
+  switch (lpc) {
+    case 4: return launch_tangent_epl<4>(a, epl, stream);
+    case 8: return launch_tangent_epl<8>(a, epl, stream);
+    case 16: return launch_tangent_epl<16>(a, epl, stream);
+    case 32: return launch_tangent_epl<32>(a, epl, stream);
+    case 64: return launch_tangent_epl<64>(a, epl, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
 
 hipError_t launch_funnel_lmc(const FunnelLmcArgs& a, int dtype, int lpc, int epl, hipStream_t stream) {
   if (dtype == 0) return launch_lpc<double>(a, lpc, epl, stream);

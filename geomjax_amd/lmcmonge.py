@@ -32,7 +32,8 @@ from .proposal import Proposal
 from .random import HostNoise, PRNGKey
 
 __all__ = ["LMCState", "LMCInfo", "RiemannianIntegratorState", "init", "build_kernel", "lmc", "lan_integrator",
-           "sample", "DynamicLMCState", "init_dynamic", "build_dynamic_kernel", "dynamic_lmc"]
+           "sample", "DynamicLMCState", "init_dynamic", "build_dynamic_kernel", "dynamic_lmc",
+           "proposal_alpha2_derivative"]
 
 
 class RiemannianIntegratorState(NamedTuple):  # geomjax/lmcmonge/integrators.py:26-44
@@ -183,6 +184,39 @@ def build_kernel(integrator=lan_integrator, divergence_threshold: float = 1000, 
         return new_state, info
 
     return kernel
+
+
+def proposal_alpha2_derivative(rng_key, state, logdensity_fn, step_size, inverse_mass_matrix, num_integration_steps,
+                               alpha2, *, reference_faithful=True, lanes_per_chain=0, max_num_integration_steps=None):
+    """(proposal position, d proposal position / d alpha2) of one LMC-Monge
+    transition per chain: the jax.jacfwd of the proposal position in alpha2
+    that geomjax/adaptation/chees_adaptation_lmcmonge.py:475-519 takes, in
+    forward mode on the device (float64, alpha2 > 0).  ``rng_key`` as for the
+    step kernel (only the normals of a ``HostNoise`` are used), so the result
+    belongs to the trajectory the step kernel integrates with the same key."""
+    from .rmhmc import _steps
+
+    _check_alpha2(alpha2)
+    position = state.position
+    single = position.dim() == 1
+    position = (position[None] if single else position).contiguous()
+    if position.dtype != torch.float64:
+        raise TypeError("the alpha2 derivative runs in float64")
+    im = _check(logdensity_fn, position, inverse_mass_matrix)
+    dev = position.device
+    C, D = position.shape
+    z, _, seed, counter = _noise(rng_key, None, C, D, dev, torch.float64)
+    L, steps = _steps(num_integration_steps, (C,), dev, max_num_integration_steps)
+    prop = torch.empty(C, D, dtype=torch.float64, device=dev)
+    dprop = torch.empty(C, D, dtype=torch.float64, device=dev)
+    io = _lib.LmcAlpha2TangentIO(inverse_mass_matrix=im.data_ptr(), position=position.data_ptr(), z=_lib.ptr(z),
+                                 num_integration_steps=_lib.ptr(steps), proposal_position=prop.data_ptr(),
+                                 d_proposal_position_d_alpha2=dprop.data_ptr())
+    params = _params(C, D, _lib.GJX_F64, step_size, L, alpha2, logdensity_fn.sigma, 1000.0, reference_faithful,
+                     lpc=lanes_per_chain, seed=seed, counter=counter)
+    rc = _lib.load().gjx_lmcmonge_funnel_alpha2_tangent(params, io, _lib.stream_handle(dev))
+    _lib.check(rc, "gjx_lmcmonge_funnel_alpha2_tangent")
+    return (prop[0], dprop[0]) if single else (prop, dprop)
 
 
 def sample(rng_key, state, logdensity_fn, step_size, inverse_mass_matrix, num_integration_steps, alpha2,

@@ -206,6 +206,27 @@ int gjx_lmcmonge_funnel_step(const gjx_funnel_params* params, const gjx_lmc_step
 /* T LMC-Monge transitions per chain in one launch. */
 int gjx_lmcmonge_funnel_sample(const gjx_funnel_params* params, const gjx_lmc_sample_io* io, void* stream);
 
+/* Derivative in alpha2 of the proposal position of one LMC-Monge transition
+ * per chain: the jax.jacfwd(trajectory_proposal_position, argnums=0) that the
+ * reference's ChEES adaptation for lmcmonge takes
+ * (geomjax/adaptation/chees_adaptation_lmcmonge.py:475-519), through the
+ * velocity draw and every integrator step, in forward mode.  float64
+ * (params->dtype = GJX_F64), params->alpha2 > 0; the noise (z, or the device
+ * PRNG at params->seed / counter) and flags are those of the step entry point,
+ * so the result belongs to the trajectory gjx_lmcmonge_funnel_step integrates
+ * with the same arguments. */
+typedef struct {
+  const void* inverse_mass_matrix;       /* [D] float64 */
+  const void* position;                  /* [C, D] float64 */
+  const void* z;                         /* [C, D] float64 standard normals, or NULL: device PRNG */
+  const int32_t* num_integration_steps;  /* [C] optional: per-chain counts in [0, params->num_integration_steps] */
+  void* proposal_position;               /* [C, D] optional: the proposal position itself */
+  double* d_proposal_position_d_alpha2;  /* [C, D] */
+} gjx_lmc_alpha2_tangent_io;
+
+int gjx_lmcmonge_funnel_alpha2_tangent(const gjx_funnel_params* params, const gjx_lmc_alpha2_tangent_io* io,
+                                       void* stream);
+
 /* ---------------------------------------------------------------------------
  * Bayesian logistic regression with the Fisher metric
  *   l(q) = sum_n [y_n eta_n - log(1 + e^eta_n)] - |q|^2 / (2 prior_var)
