@@ -13,6 +13,7 @@ from .lmcmonge import lmc as lmcmonge
 from .rmhmc import dynamic_rmhmc, rmhmc
 from .nuts import nuts as nutsrmhmc
 from .adaptation import chees_adaptation as chees_adaptation_riemanian
+from .adaptation import chees_adaptation_lmcmonge
 
 # ``nuts`` also names the RMHMC NUTS here (the reference's ``geomjax.nuts`` is its
 # Euclidean NUTS, which this build does not serve)
@@ -24,5 +25,5 @@ ess = diagnostics.effective_sample_size
 rhat = diagnostics.potential_scale_reduction
 
 __all__ = ["rmhmc", "dynamic_rmhmc", "nutsrmhmc", "nuts", "lmc", "dynamic_lmc", "lmcmonge", "dynamic_lmcmonge",
-           "chees_adaptation_riemanian", "SamplingAlgorithm", "NealFunnel", "LogisticRegression", "LogisticBatch", "MongeMetric", "FisherMetric", "metrics", "models", "random", "sampling",
+           "chees_adaptation_riemanian", "chees_adaptation_lmcmonge", "SamplingAlgorithm", "NealFunnel", "LogisticRegression", "LogisticBatch", "MongeMetric", "FisherMetric", "metrics", "models", "random", "sampling",
            "diagnostics", "ess", "rhat", "__version__"]

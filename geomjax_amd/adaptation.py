@@ -1,7 +1,10 @@
-"""ChEES adaptation of the step size and trajectory length of jittered RMHMC.
+"""ChEES adaptation of the step size and trajectory length of jittered RMHMC
+and LMC-Monge (and of LMC-Monge's alpha2).
 
 Reference: geomjax/adaptation/chees_adaptation_riemanian.py (the driver,
-``dynamics="rmhmc"``), geomjax/optimizers/dual_averaging.py (step size), and
+``dynamics="rmhmc"``), geomjax/adaptation/chees_adaptation_lmcmonge.py (the
+LMC-Monge driver with its alpha2 update, ``chees_adaptation_lmcmonge`` at the
+end of this module), geomjax/optimizers/dual_averaging.py (step size), and
 the optax Adam its callers pass for the trajectory length.  Every iteration
 runs one RMHMC transition of all chains with the HIP kernel (the same number
 of integration steps for every chain: ceil(jitter * trajectory_length /
@@ -267,3 +270,152 @@ class chees_adaptation:
 __all__ = ["chees_adaptation", "chees_base", "chees_trajectory_gradient", "dual_averaging", "adam", "halton",
            "dynamic_rmhmc",
            "OPTIMAL_TARGET_ACCEPTANCE_RATE"]
+
+
+# ---------------------------------------------------------------------------
+# ChEES for LMC-Monge (geomjax/adaptation/chees_adaptation_lmcmonge.py)
+# ---------------------------------------------------------------------------
+
+
+class ChEESLMCAdaptationState(NamedTuple):  # chees_adaptation_lmcmonge.py:21-58 (same fields)
+    step_size: float
+    log_step_size_moving_average: float
+    trajectory_length: float
+    log_trajectory_length_moving_average: float
+    da_state: DualAveragingState
+    optim_state: AdamState
+    random_generator_arg: int
+    step: int
+    alpha2: float
+    log_alpha2_moving_average: float
+    optim_state_alpha2: AdamState
+
+
+def chees_lmcmonge_base(jitter_generator, next_random_arg_fn, optim, target_acceptance_rate, decay_rate,
+                        update_alpha=False, reference_faithful=True):
+    """(init, update) of chees_adaptation_lmcmonge.py:59-341: the step size and
+    trajectory length as in the RMHMC driver, and with ``update_alpha`` one Adam
+    step on log alpha2 per iteration from the gradient
+    jitter * T * (|q_L - m_L|^2 - |q_0 - m_0|^2) * ((q_L - m_L) . dq_L/dalpha2),
+    acceptance-weighted over non-divergent chains (:218-252), reported as the
+    moving average of log alpha2 with the same weights.
+
+    ``reference_faithful=True`` keeps the reference's choices: the adaptation's
+    alpha2 starts at 1.0 whatever alpha2 the caller gives (:282-284), and the
+    gradients go to Adam as computed (descending the criterion).  ``False``
+    starts from the caller's alpha2 and ascends."""
+    init_rm, update_rm = chees_base(jitter_generator, next_random_arg_fn, optim, target_acceptance_rate, decay_rate,
+                                    reference_faithful)
+    sign = 1.0 if reference_faithful else -1.0
+
+    def init_fn(random_generator_arg, step_size, alpha2):
+        a0 = 1.0 if reference_faithful else float(alpha2)
+        log_ma = 0.0 if reference_faithful else math.log(a0)
+        return ChEESLMCAdaptationState(*init_rm(random_generator_arg, step_size), a0, log_ma, optim.init(a0))
+
+    def update_fn(st: ChEESLMCAdaptationState, proposed_positions, proposed_velocities, proposed_derivatives_alpha2,
+                  initial_positions, acceptance_probabilities, is_divergent):
+        base = update_rm(ChEESAdaptationState(*st[:8]), proposed_positions, proposed_velocities, initial_positions,
+                         acceptance_probabilities, is_divergent)
+        alpha2, log_ma, opt_a = st.alpha2, st.log_alpha2_moving_average, st.optim_state_alpha2
+        if update_alpha:
+            g = chees_trajectory_gradient(jitter_generator(st.random_generator_arg), st.trajectory_length,
+                                          proposed_positions, proposed_derivatives_alpha2, initial_positions,
+                                          acceptance_probabilities, is_divergent)
+            log_a = math.log(st.alpha2)
+            upd, opt_ = optim.update(sign * g, st.optim_state_alpha2, log_a)
+            log_a_ = log_a + upd
+            new_log_a, opt_a = (log_a_, opt_) if math.isfinite(log_a_) else (log_a, st.optim_state_alpha2)
+            w = st.step ** (-decay_rate)
+            log_ma = (1.0 - w) * st.log_alpha2_moving_average + w * new_log_a
+            alpha2 = _exp(log_ma)
+        return ChEESLMCAdaptationState(*base, alpha2, log_ma, opt_a)
+
+    return init_fn, update_fn
+
+
+class chees_adaptation_lmcmonge:
+    """geomjax/adaptation/chees_adaptation_lmcmonge.py:344-557 over the HIP LMC-Monge kernels.
+
+    ``chees_adaptation_lmcmonge(logprob_fn, inverse_mass_matrix, num_chains, *,
+    jitter_amount=1.0, target_acceptance_rate=0.651, decay_rate=0.5,
+    update_alpha=False, alpha2=0.001)``; ``run(rng_key, positions, step_size,
+    optim, num_steps, max_sampling_steps=1000)`` returns
+    ``(AdaptationResults(last_states, parameters), history)`` with parameters
+    that build ``geomjax_amd.dynamic_lmcmonge(logprob_fn, **parameters)``.
+
+    Every iteration runs one LMC-Monge transition of all chains (the same number
+    of integration steps for all: ceil(jitter * T / step size)) and, with
+    ``update_alpha``, the alpha2 tangent of the same trajectories (the
+    reference's ``jax.jacfwd`` of the proposal position, :475-519).  The
+    reference's driver calls a dynamic LMC-Monge kernel that cannot run as
+    written; here the chains run at the alpha2 their states carry: the
+    caller's alpha2 with ``reference_faithful=True`` (the reference never
+    writes the adapted alpha2 back into the states), the adaptation's current
+    alpha2 with ``False``.  ``history`` holds per-iteration scalars (step size,
+    trajectory length, number of integration steps, mean acceptance,
+    harmonic-mean acceptance, alpha2).
+    """
+
+    def __init__(self, logprob_fn, inverse_mass_matrix, num_chains, *, jitter_amount=1.0,
+                 target_acceptance_rate=OPTIMAL_TARGET_ACCEPTANCE_RATE, decay_rate=0.5, update_alpha=False,
+                 alpha2=0.001, reference_faithful=True):
+        self.logprob_fn, self.inverse_mass_matrix, self.num_chains = logprob_fn, inverse_mass_matrix, int(num_chains)
+        self.jitter_amount, self.target, self.decay_rate = jitter_amount, target_acceptance_rate, decay_rate
+        self.update_alpha, self.alpha2, self.reference_faithful = update_alpha, float(alpha2), reference_faithful
+
+    def run(self, rng_key: PRNGKey, positions, step_size, optim, num_steps=1000, *, max_sampling_steps=1000):
+        from .lmcmonge import DynamicLMCState, LMCState, init_dynamic, proposal_alpha2_derivative
+        from .lmcmonge import build_kernel as lmc_build_kernel
+
+        if positions.shape[0] != self.num_chains:
+            raise ValueError("initial positions' leading dimension must be num_chains")
+        max_bits = math.ceil(math.log2(num_steps + max_sampling_steps))
+        ja = self.jitter_amount
+
+        def jitter(i):
+            return halton(i, max_bits) * ja + (1.0 - ja)
+
+        def next_arg(i):
+            return i + 1
+
+        init_fn, update_fn = chees_lmcmonge_base(jitter, next_arg, optim, self.target, self.decay_rate,
+                                                 self.update_alpha, self.reference_faithful)
+        kernel = lmc_build_kernel(reference_faithful=self.reference_faithful)
+        states = init_dynamic(positions, self.logprob_fn, self.alpha2, 0)
+        im = torch.as_tensor(self.inverse_mass_matrix, device=positions.device)
+        ast = init_fn(0, float(step_size), self.alpha2)
+        history = []
+        for it in range(int(num_steps)):
+            key = PRNGKey(rng_key.seed, rng_key.counter + it)
+            L = int(math.ceil(jitter(ast.random_generator_arg) * ast.trajectory_length / ast.step_size))
+            a2 = states.alpha2 if self.reference_faithful else ast.alpha2
+            base_state = LMCState(states.position, states.logdensity, states.logdensity_grad, states.volume_adjustment)
+            new, info = kernel(key, base_state, self.logprob_fn, ast.step_size, im, L, a2)
+            if self.update_alpha:
+                _, dq = proposal_alpha2_derivative(key, base_state, self.logprob_fn, ast.step_size, im, L, a2,
+                                                   reference_faithful=self.reference_faithful)
+            else:
+                dq = torch.zeros_like(states.position)
+            ast_new = update_fn(ast, info.proposal.state.position, info.proposal.state.velocity, dq, states.position,
+                                info.acceptance_rate, info.is_divergent)
+            history.append((ast.step_size, ast.trajectory_length, L, float(info.acceptance_rate.mean()),
+                            harmonic_acceptance(info.acceptance_rate, info.is_divergent), ast.alpha2))
+            states = DynamicLMCState(a2, new.position, new.logdensity, new.logdensity_grad, new.volume_adjustment,
+                                     states.random_generator_arg + 1)
+            ast = ast_new
+
+        traj_adj = math.exp(ast.log_trajectory_length_moving_average - ast.log_step_size_moving_average)
+
+        def integration_steps_fn(arg):
+            return torch.ceil(jitter(arg) * traj_adj).to(torch.int32)
+
+        parameters = {"alpha2": math.exp(ast.log_alpha2_moving_average),
+                      "step_size": math.exp(ast.log_step_size_moving_average),
+                      "inverse_mass_matrix": im, "next_random_arg_fn": next_arg,
+                      "integration_steps_fn": integration_steps_fn,
+                      "max_num_integration_steps": int(math.ceil(traj_adj)),
+                      "reference_faithful": self.reference_faithful}
+        last = states._replace(random_generator_arg=torch.full((states.position.shape[0],), ast.random_generator_arg,
+                                                               dtype=torch.int64, device=states.position.device))
+        return AdaptationResults(last, parameters), history

@@ -99,3 +99,42 @@ def test_chees_update_sign_of_the_trajectory_step():
         moved[faithful] = new.log_trajectory_length_moving_average - math.log(0.5)
     assert math.copysign(1.0, moved[True]) == -math.copysign(1.0, grad)
     assert math.copysign(1.0, moved[False]) == math.copysign(1.0, grad)
+
+
+def test_chees_lmcmonge_alpha2_update():
+    """chees_adaptation_lmcmonge.py:59-341: the faithful state starts at alpha2 =
+    1.0 whatever the caller gives (:282-284) and, with update_alpha, moves log
+    alpha2 against the alpha2 gradient (Adam on the gradient as computed); the
+    gradient is the trajectory gradient's formula with dq/dalpha2 in place of the
+    velocity; the step size and trajectory length updates equal the RMHMC
+    driver's; without update_alpha alpha2 does not move."""
+    from geomjax_amd.adaptation import chees_base, chees_lmcmonge_base
+
+    g = torch.Generator().manual_seed(6)
+    C, D = 16, 4
+    q0 = torch.randn(C, D, generator=g, dtype=torch.float64)
+    v = torch.randn(C, D, generator=g, dtype=torch.float64)
+    dq = torch.randn(C, D, generator=g, dtype=torch.float64)
+    qL = q0 + 0.5 * v
+    acc = torch.full((C,), 0.7, dtype=torch.float64)
+    div = torch.zeros(C, dtype=torch.bool)
+    ga = chees_trajectory_gradient(1.0, 0.5, qL, dq, q0, acc, div)
+    assert ga != 0.0
+    for faithful in (True, False):
+        init_fn, update_fn = chees_lmcmonge_base(lambda i: 1.0, lambda i: i + 1, adam(0.1), 0.651, 0.5,
+                                                 update_alpha=True, reference_faithful=faithful)
+        st = init_fn(0, 0.5, 0.01)
+        assert st.alpha2 == (1.0 if faithful else 0.01)
+        new = update_fn(st, qL, v, dq, q0, acc, div)
+        moved = new.log_alpha2_moving_average - math.log(st.alpha2)
+        # step weight 1 ** -0.5 = 1: the moving average is the new log alpha2; Adam's
+        # first step has size lr = 0.1 whatever the gradient's scale
+        assert abs(abs(moved) - 0.1) < 1e-6
+        assert math.copysign(1.0, moved) == (-1.0 if faithful else 1.0) * math.copysign(1.0, ga)
+        assert math.isclose(new.alpha2, math.exp(new.log_alpha2_moving_average))
+        rm_init, rm_update = chees_base(lambda i: 1.0, lambda i: i + 1, adam(0.1), 0.651, 0.5, faithful)
+        rm = rm_update(rm_init(0, 0.5), qL, v, q0, acc, div)
+        assert tuple(new[:8]) == tuple(rm)
+    init_fn, update_fn = chees_lmcmonge_base(lambda i: 1.0, lambda i: i + 1, adam(0.1), 0.651, 0.5)
+    st = init_fn(0, 0.5, 0.01)
+    assert update_fn(st, qL, v, dq, q0, acc, div).alpha2 == 1.0

@@ -1,7 +1,9 @@
 """GPU: the ChEES driver (geomjax_amd.adaptation.chees_adaptation) over the HIP
 RMHMC kernel: the step size moves the harmonic-mean acceptance towards the
 0.651 target, every parameter stays finite, and the adapted parameters build
-a working dynamic RMHMC sampler.  (The driver's arithmetic is pinned on the
+a working dynamic RMHMC sampler; the same for the LMC-Monge driver
+(chees_adaptation_lmcmonge) with its alpha2 update, whose per-iteration
+derivative comes from the alpha2 tangent kernel.  (The drivers' arithmetic is pinned on the
 CPU in tests/test_adaptation.py; sampling noise makes the trajectory of the
 adaptation itself a statistical, not an exact, check.)"""
 import functools
@@ -54,3 +56,33 @@ def test_chees_adapts_and_the_result_samples(kind):
     assert torch.isfinite(state.position).all()
     assert float(info.acceptance_rate.mean()) > 0.2
 
+
+
+@pytest.mark.parametrize("faithful", [True, False])
+def test_chees_lmcmonge_adapts_and_the_result_samples(faithful):
+    from geomjax_amd.adaptation import chees_adaptation_lmcmonge
+
+    D, C = 10, 128
+    model = gj.NealFunnel(D)
+    im = torch.ones(D, dtype=torch.float64, device=DEV)
+    warm = chees_adaptation_lmcmonge(model, im, C, update_alpha=True, alpha2=0.5, reference_faithful=faithful)
+    q0 = (0.3 * seeded_normal(19, C, D)).to(DEV)
+    (last, params), hist = warm.run(gr.key(21), q0, 0.05, adam(0.05), num_steps=40)
+    torch.cuda.synchronize()
+    harm = [h[4] for h in hist]
+    a2s = [h[5] for h in hist]
+    print(f"lmcmonge faithful={faithful}: step size {hist[0][0]:.3f} -> {params['step_size']:.3f}, "
+          f"L {min(h[2] for h in hist)}..{max(h[2] for h in hist)}, harmonic first 5 {sum(harm[:5]) / 5:.3f} "
+          f"last 15 {sum(harm[-15:]) / 15:.3f}, alpha2 {a2s[0]:.3f} -> {params['alpha2']:.3f}", flush=True)
+    assert all(math.isfinite(h[0]) and math.isfinite(h[1]) and math.isfinite(h[5]) for h in hist)
+    assert math.isfinite(params["step_size"]) and params["step_size"] > 0
+    assert math.isfinite(params["alpha2"]) and params["alpha2"] > 0
+    assert a2s[0] == (1.0 if faithful else 0.5)
+    assert params["alpha2"] != a2s[0]  # the alpha2 update ran
+    assert last.alpha2 == 0.5 or not faithful  # the faithful driver never writes alpha2 back to the chains
+    alg = gj.dynamic_lmcmonge(model, **params)
+    state = last
+    for t in range(5):
+        state, info = alg.step(gr.key(200 + t), state)
+    torch.cuda.synchronize()
+    assert torch.isfinite(state.position).all()
