@@ -96,6 +96,15 @@ class LmcSampleIO(ctypes.Structure):
     )]
 
 
+class LmcNutsIO(ctypes.Structure):
+    _fields_ = [(n, _vp) for n in (
+        "inverse_mass_matrix", "position", "logdensity", "volume_adjustment", "z", "u_direction", "u_merge", "u_step",
+        "new_position", "new_logdensity", "new_logdensity_grad", "new_volume_adjustment", "velocity", "energy",
+        "acceptance_rate", "is_divergent", "is_turning", "num_trajectory_expansions", "num_integration_steps",
+        "leftmost_position", "rightmost_position",
+    )]
+
+
 class LmcAlpha2TangentIO(ctypes.Structure):
     _fields_ = [(n, _vp) for n in (
         "inverse_mass_matrix", "position", "z", "num_integration_steps", "proposal_position",
@@ -169,6 +178,7 @@ _SIGNATURES = {
     "gjx_lmcmonge_funnel_step": [ctypes.POINTER(FunnelParams), ctypes.POINTER(LmcStepIO), _vp],
     "gjx_lmcmonge_funnel_sample": [ctypes.POINTER(FunnelParams), ctypes.POINTER(LmcSampleIO), _vp],
     "gjx_lmcmonge_funnel_alpha2_tangent": [ctypes.POINTER(FunnelParams), ctypes.POINTER(LmcAlpha2TangentIO), _vp],
+    "gjx_lmcmonge_funnel_nuts": [ctypes.POINTER(FunnelParams), _c_int32, _c_int32, ctypes.POINTER(LmcNutsIO), _vp],
     "gjx_rmhmc_monge_funnel_tangent": [ctypes.POINTER(FunnelParams), ctypes.POINTER(TangentIO), _vp],
     "gjx_rmhmc_monge_funnel_nuts": [ctypes.POINTER(FunnelParams), _c_int32, _c_int32, ctypes.POINTER(NutsIO), _vp],
     "gjx_rmhmc_fisher_logistic_nuts": [ctypes.POINTER(LogisticParams), ctypes.POINTER(LogisticData), _c_int32,

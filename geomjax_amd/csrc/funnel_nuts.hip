@@ -1,7 +1,8 @@
-// NUTS on Neal's funnel with the Monge metric and the RMHMC implicit-midpoint
-// integrator, float64.
+// NUTS on Neal's funnel, float64, with the RMHMC implicit-midpoint integrator
+// (Monge metric) or the LMC-Monge Lagrangian integrator.
 //
-// Reference: geomjax/rmhmc/nuts.py:78-161,253-348 (kernel, iterative proposal),
+// Reference: geomjax/rmhmc/nuts.py:78-161,253-348 and
+// geomjax/lmcmonge/nuts.py:80-207,300-404 (kernels, iterative proposal),
 // geomjax/mcmc/trajectory.py:149-317 (dynamic_progressive_integration: a
 // subtree of up to 2^e steps in one direction, progressive uniform sampling,
 // stopping at divergence or an iterative U-turn), :517-668
@@ -13,74 +14,333 @@
 //
 // One wave per chain: every branch of the tree building is wave-uniform.  Lane
 // l holds x coordinates [l EPL, (l+1) EPL) of the D-1 "x" part; the log-scale
-// coordinate v is replicated.  The integrator, energies and velocities are the
-// closed forms of funnel_rmhmc.hip.  The U-turn checkpoints (velocity and
-// velocity sum at up to M leaves, and momentum and momentum sum for "bet" and
-// "riem") live in LDS.  The trajectory keeps its two
-// ends, its momentum and velocity sums and its length; the subtree being built
-// keeps its last state, sums and length; proposals keep their position,
-// energy, weight and accumulated acceptance.
+// coordinate v is replicated.  The tree building (nuts_tree) is written once
+// over a chain state with position (x, v), momentum (px, pv) and velocity
+// (ux, uv); each kernel supplies its integrator step and energy:
+//   RMHMC: the closed forms of funnel_rmhmc.hip (energy -l + T(q, p));
+//   LMC  : the lmc_ops of funnel_lmc_ops.h (energy -l + K(q, v) - J with the
+//          volume adjustment J carried along the trajectory,
+//          geomjax/lmcmonge/metrics.py:267-284).
+// The U-turn checkpoints (velocity and velocity sum at up to M leaves, and
+// momentum and momentum sum for "bet" and "riem") live in LDS; the
+// trajectory's ends, sums and the proposals live in registers.
 #include "common.h"
 #include "funnel_field.h"
+#include "funnel_lmc.h"
+#include "funnel_lmc_ops.h"
 #include "funnel_rmhmc.h"
+#include "nuts.h"
 
 #include <math.h>
 
 namespace gjx {
 namespace nuts {
 
+// RMHMC chain state: |x|^2 and x . p_x are kept as group totals
 template <int EPL>
-struct FState {
+struct RState {
   double x[EPL], px[EPL], ux[EPL];
   double v, pv, uv;
-  double sx, xpx;  // |x|^2 and x . p_x (group totals)
+  double sx, xpx;
+  double J;  // no volume adjustment in RMHMC (kept at 0 for the shared proposal code)
+};
+
+// LMC-Monge chain state: the position's scalars and the carried volume adjustment
+template <int EPL>
+struct LState {
+  double x[EPL], px[EPL], ux[EPL];
+  double v, pv, uv;
+  lmc_ops::PosScalars<double> P;
+  double J;
 };
 
 template <int EPL>
 struct Prop {
   double x[EPL];
   double v;
+  double J;
   double energy, weight, slpa;
 };
+
+struct TreeOut {
+  int expansions, num_states;
+  bool diverging, turning;
+};
+
+template <int EPL>
+__device__ __forceinline__ double dot64(const double (&ux)[EPL], double uv, const double (&wx)[EPL], double wv) {
+  double d = 0.0;
+#pragma unroll
+  for (int k = 0; k < EPL; ++k) d += ux[k] * wx[k];
+  return group_sum<64>(d) + uv * wv;
+}
+
+// The transition's uniforms: host-fed or the device Philox stream of the purpose tag.
+struct NutsUniform {
+  int c;
+  uint64_t tt;
+  uint32_t k0, k1;
+  __device__ double operator()(const double* host, size_t idx, uint32_t stream, uint32_t i) const {
+    if (host != nullptr) return host[idx];
+    u32x4 r = philox4x32_10(u32x4{(uint32_t)c, (uint32_t)tt, stream + i, (uint32_t)(tt >> 32)}, k0, k1);
+    return u01_f64(r.x, r.y) - 0.5 * (1.0 / 9007199254740992.0);
+  }
+};
+
+// The iterative NUTS expansion of one chain's trajectory from s0 (energy H0):
+// geomjax/mcmc/trajectory.py:149-317,517-668 with the numpyro U-turn check of
+// geomjax/mcmc/termination.py:30-119.  step(s, signed_eps) advances a state by
+// one integrator step (its velocity and momentum current), energy(s) is its
+// energy.  On return left / right are the trajectory's ends and prop the
+// sampled proposal.
+template <int EPL, class S, class Step, class Energy>
+__device__ TreeOut nuts_tree(const S& s0, double H0, Step step, Energy energy, const NutsUniform& uniform,
+                             const NutsArgs& nu, double eps, double div_threshold, double* ck, int c, S& left,
+                             S& right, Prop<EPL>& prop) {
+  const int l = threadIdx.x;
+  const int M = nu.max_doublings;
+  const int NS = (1 << M) - 1;
+  const int crit = nu.criterion;
+  const int NA = nuts_checkpoint_arrays(crit);
+  const int XS = M * 64 * EPL;  // one checkpoint array's x part
+  // U-turn criteria (geomjax/rmhmc/metrics.py:100-117) on the two ends' momenta ml, mr and
+  // velocities vl, vr and the sums ms, vs over the states between them; the dot products
+  // are group totals, so the answer is the same in every lane
+  auto is_turning = [&](const double (&mlx)[EPL], double mlv, const double (&mrx)[EPL], double mrv,
+                        const double (&msx)[EPL], double msv, const double (&vlx)[EPL], double vlv,
+                        const double (&vrx)[EPL], double vrv, const double (&vsx)[EPL], double vsv) -> bool {
+    if (crit == 1) return dot64(vlx, vlv, msx, msv) <= 0.0 || dot64(vrx, vrv, msx, msv) <= 0.0;  // "bet"
+    if (crit == 2) return dot64(mlx, mlv, vsx, vsv) <= 0.0 || dot64(mrx, mrv, vsx, vsv) <= 0.0;  // "riem"
+    return dot64(vlx, vlv, vsx, vsv) <= 0.0 || dot64(vrx, vrv, vsx, vsv) <= 0.0;                 // "euc"
+  };
+  // checkpoints: [NA][M][64 EPL] coordinates x, then [NA][M] for v; arrays 0, 1 =
+  // velocity, velocity sum; 2, 3 = momentum, momentum sum
+  auto ck_put = [&](int arr, int idx, const double (&x)[EPL], double v) {
+#pragma unroll
+    for (int k = 0; k < EPL; ++k) ck[arr * XS + idx * 64 * EPL + l * EPL + k] = x[k];
+    if (l == 0) ck[NA * XS + arr * M + idx] = v;  // v is replicated: lane 0 stores it
+  };
+  auto ck_get = [&](int arr, int idx, double (&x)[EPL]) -> double {
+#pragma unroll
+    for (int k = 0; k < EPL; ++k) x[k] = ck[arr * XS + idx * 64 * EPL + l * EPL + k];
+    return ck[NA * XS + arr * M + idx];
+  };
+  auto set_prop = [&](Prop<EPL>& pr, const S& s, double en, double w) {
+#pragma unroll
+    for (int k = 0; k < EPL; ++k) pr.x[k] = s.x[k];
+    pr.v = s.v;
+    pr.J = s.J;
+    pr.energy = en;
+    pr.weight = w;
+    pr.slpa = fmin(w, 0.0);
+  };
+
+  for (int i = l; i < NA * (XS + M); i += 64) ck[i] = 0.0;
+  __syncthreads();
+  // the trajectory: its two ends, sums (the start state included), length
+  left = s0;
+  right = s0;
+  double psx[EPL], vsx[EPL], psv = s0.pv, vsv = s0.uv;
+#pragma unroll
+  for (int k = 0; k < EPL; ++k) {
+    psx[k] = s0.px[k];
+    vsx[k] = s0.ux[k];
+  }
+  int ntraj = 0;
+  set_prop(prop, s0, H0, 0.0);
+  prop.slpa = -INFINITY;
+  int e = 0, j = 0;
+  bool diverging = false, turning = false;
+  while (e < M && !diverging && !turning) {
+    const double ud = uniform(nu.u_dir, (size_t)c * M + e, kStreamNutsDirection, (uint32_t)e);
+    const int dir = ud < 0.5 ? 1 : -1;
+    // ---- subtree of up to 2^e steps from the trajectory's end (trajectory.py:149-317) ----
+    S cur = dir > 0 ? right : left;
+    Prop<EPL> sp;
+    double ssx[EPL], ssv_x[EPL], ssp_v = 0.0, ssv_v = 0.0;
+#pragma unroll
+    for (int k = 0; k < EPL; ++k) {
+      ssx[k] = 0.0;
+      ssv_x[k] = 0.0;
+    }
+    int nsub = 0, st = 0;
+    const int max_steps = 1 << e;
+    bool div_s = false, term_s = false;
+    while (st < max_steps && !term_s && !div_s) {
+      step(cur, dir * eps);
+      const double en = energy(cur);
+      double w = H0 - en;
+      if (isnan(w)) w = -INFINITY;
+      div_s = -w > div_threshold;
+      const double us = uniform(nu.u_step, (size_t)c * NS + j, kStreamNutsStep, (uint32_t)j);
+      if (st == 0) {
+#pragma unroll
+        for (int k = 0; k < EPL; ++k) {
+          ssx[k] = cur.px[k];
+          ssv_x[k] = cur.ux[k];
+        }
+        ssp_v = cur.pv;
+        ssv_v = cur.uv;
+        nsub = 1;
+        set_prop(sp, cur, en, w);
+      } else {
+#pragma unroll
+        for (int k = 0; k < EPL; ++k) {
+          ssx[k] += cur.px[k];
+          ssv_x[k] += cur.ux[k];
+        }
+        ssp_v += cur.pv;
+        ssv_v += cur.uv;
+        ++nsub;
+        // progressive uniform sampling (proposal.py:198-222)
+        const double dw = w - sp.weight;
+        const double pacc = 1.0 / (1.0 + exp(-dw));
+        const double nw = logaddexp(sp.weight, w), ns = logaddexp(sp.slpa, fmin(w, 0.0));
+        if (us < pacc) set_prop(sp, cur, en, w);
+        sp.weight = nw;
+        sp.slpa = ns;
+      }
+      ++j;
+      // iterative U-turn check (termination.py:38-117)
+      const int idx_max = ckpt_idx_max(st), idx_min = ckpt_idx_min(st);
+      if ((st & 1) == 0) {
+        ck_put(0, idx_max, cur.ux, cur.uv);
+        ck_put(1, idx_max, ssv_x, ssv_v);
+        if (NA == 4) {
+          ck_put(2, idx_max, cur.px, cur.pv);
+          ck_put(3, idx_max, ssx, ssp_v);
+        }
+      }
+      __syncthreads();  // one wave: orders the checkpoint writes before the reads below
+      for (int i = idx_max; i >= idx_min && !term_s; --i) {
+        // the sub-subtree from checkpoint i to the current state (termination.py:89-117)
+        double vl[EPL], tmp[EPL], vs[EPL], ml[EPL], ms[EPL];
+        const double vlv = ck_get(0, i, vl);
+        const double tv = ck_get(1, i, tmp);
+#pragma unroll
+        for (int k = 0; k < EPL; ++k) vs[k] = ssv_x[k] - tmp[k] + vl[k];
+        const double vsv_i = ssv_v - tv + vlv;
+        double mlv = 0.0, msv = 0.0;
+        if (NA == 4) {
+          mlv = ck_get(2, i, ml);
+          const double tm = ck_get(3, i, tmp);
+#pragma unroll
+          for (int k = 0; k < EPL; ++k) ms[k] = ssx[k] - tmp[k] + ml[k];
+          msv = ssp_v - tm + mlv;
+        } else {
+#pragma unroll
+          for (int k = 0; k < EPL; ++k) ml[k] = ms[k] = 0.0;
+        }
+        term_s = is_turning(ml, mlv, cur.px, cur.pv, ms, msv, vl, vlv, cur.ux, cur.uv, vs, vsv_i);
+      }
+      ++st;
+    }
+    // ---- merge (trajectory.py:580-660) ----
+    if (div_s || term_s) {
+      prop.slpa = logaddexp(prop.slpa, sp.slpa);
+    } else {
+      // progressive biased sampling (proposal.py:225-254)
+      const double um = uniform(nu.u_merge, (size_t)c * M + e, kStreamNutsMerge, (uint32_t)e);
+      const double pacc = fmin(exp(sp.weight - prop.weight), 1.0);
+      const double nw = logaddexp(prop.weight, sp.weight), ns = logaddexp(prop.slpa, sp.slpa);
+      if (um < pacc) prop = sp;
+      prop.weight = nw;
+      prop.slpa = ns;
+    }
+    if (dir > 0)
+      right = cur;
+    else
+      left = cur;
+#pragma unroll
+    for (int k = 0; k < EPL; ++k) {
+      psx[k] += ssx[k];
+      vsx[k] += ssv_x[k];
+    }
+    psv += ssp_v;
+    vsv += ssv_v;
+    ntraj += nsub;
+    const bool turn_full = is_turning(left.px, left.pv, right.px, right.pv, psx, psv, left.ux, left.uv, right.ux,
+                                      right.uv, vsx, vsv);
+    ++e;
+    diverging = div_s;
+    turning = term_s || turn_full;
+  }
+  return TreeOut{e, ntraj, diverging, turning};
+}
+
+// Funnel log density (examples/funnel/main.py:34-39) from v and |x|^2.
+__device__ __forceinline__ double funnel_logp(double v, double sx, double sigma, int Dx) {
+  return -0.5 * v * v / (sigma * sigma) - log(sigma) - 0.5 * kLog2Pi - 0.5 * exp(-v) * sx - 0.5 * (double)Dx * v -
+         0.5 * (double)Dx * kLog2Pi;
+}
+
+// Outputs shared by both kernels: the proposal's position, log density and
+// gradient (closed forms at its position), optionally its volume adjustment,
+// the trajectory's ends and the rest of NUTSInfo.
+template <int EPL>
+__device__ void write_outputs(const Prop<EPL>& prop, const double (&lx)[EPL], double lv, const double (&rx)[EPL],
+                              double rv, const TreeOut& t, int c, int D, double sigma, void* q_out, double* logp_out,
+                              void* g_out, double* J_out, const NutsArgs& nu) {
+  const int l = threadIdx.x, Dx = D - 1, j0 = l * EPL;
+  const size_t base = (size_t)c * D;
+  double sxp = 0.0;
+#pragma unroll
+  for (int k = 0; k < EPL; ++k) sxp += prop.x[k] * prop.x[k];
+  sxp = group_sum<64>(sxp);
+  const double ep = exp(-prop.v);
+#pragma unroll
+  for (int k = 0; k < EPL; ++k) {
+    if (j0 + k >= Dx) continue;
+    static_cast<double*>(q_out)[base + j0 + k] = prop.x[k];
+    if (g_out) static_cast<double*>(g_out)[base + j0 + k] = -ep * prop.x[k];
+    if (nu.left_position) static_cast<double*>(nu.left_position)[base + j0 + k] = lx[k];
+    if (nu.right_position) static_cast<double*>(nu.right_position)[base + j0 + k] = rx[k];
+  }
+  if (l == 0) {
+    static_cast<double*>(q_out)[base + Dx] = prop.v;
+    if (g_out)
+      static_cast<double*>(g_out)[base + Dx] = -prop.v / (sigma * sigma) - 0.5 * Dx + 0.5 * ep * sxp;
+    if (nu.left_position) static_cast<double*>(nu.left_position)[base + Dx] = lv;
+    if (nu.right_position) static_cast<double*>(nu.right_position)[base + Dx] = rv;
+    logp_out[c] = funnel_logp(prop.v, sxp, sigma, Dx);
+    if (J_out) J_out[c] = prop.J;
+    if (nu.energy) nu.energy[c] = prop.energy;
+    if (nu.acceptance_rate) nu.acceptance_rate[c] = exp(prop.slpa) / (double)t.num_states;
+    if (nu.is_divergent) nu.is_divergent[c] = (uint8_t)t.diverging;
+    if (nu.is_turning) nu.is_turning[c] = (uint8_t)t.turning;
+    if (nu.num_expansions) nu.num_expansions[c] = t.expansions;
+    if (nu.num_steps) nu.num_steps[c] = t.num_states;
+  }
+}
 
 }  // namespace nuts
 
 using namespace nuts;
 
+// ---------------------------------------------------------------------------
+// NUTS with the RMHMC implicit midpoint, Monge metric (geomjax/rmhmc/nuts.py)
+// ---------------------------------------------------------------------------
 template <int EPL>
 __global__ void __launch_bounds__(64) funnel_nuts_kernel(const FunnelArgs a, const NutsArgs nu) {
-  // checkpoints of the iterative U-turn check: [NA][M][64 EPL] coordinates x, then [NA][M] for v;
-  // arrays 0, 1 = velocity, velocity sum; 2, 3 = momentum, momentum sum ("bet", "riem")
   extern __shared__ __align__(16) double ck[];
   const int c = blockIdx.x;
   if (c >= a.C) return;
   const int l = threadIdx.x;
   const int D = a.D, Dx = a.D - 1;
   const int j0 = l * EPL;
-  const int M = nu.max_doublings;
-  const int S = (1 << M) - 1;
   const double a2 = a.alpha2;
   const double alpha = sqrt(a.alpha2);
   const double inv_s2 = 1.0 / (a.sigma * a.sigma);
   const double halfDx = 0.5 * Dx;
   const uint64_t tt = a.counter;
   const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
+  const NutsUniform uniform{c, tt, k0, k1};
   const size_t base = (size_t)c * D;
   auto on = [&](int k) { return j0 + k < Dx; };
-  auto uniform = [&](const double* host, size_t idx, uint32_t stream, uint32_t i) -> double {
-    if (host != nullptr) return host[idx];
-    u32x4 r = philox4x32_10(u32x4{(uint32_t)c, (uint32_t)tt, stream + i, (uint32_t)(tt >> 32)}, k0, k1);
-    return u01_f64(r.x, r.y) - 0.5 * (1.0 / 9007199254740992.0);
-  };
-
-  const int crit = nu.criterion;
-  const int NA = nuts_checkpoint_arrays(crit);
-  const int XS = M * 64 * EPL;  // one checkpoint array's x part
-  for (int i = l; i < NA * (XS + M); i += 64) ck[i] = 0.0;
-  __syncthreads();
 
   // ---- chain state and momentum p0 = chol(G) z (funnel_rmhmc.hip) ----
-  FState<EPL> s0;
+  RState<EPL> s0;
+  s0.J = 0.0;
   double z[EPL], zv;
 #pragma unroll
   for (int k = 0; k < EPL; ++k) s0.x[k] = on(k) ? static_cast<const double*>(a.q_in)[base + j0 + k] : 0.0;
@@ -148,15 +408,14 @@ __global__ void __launch_bounds__(64) funnel_nuts_kernel(const FunnelArgs a, con
   }
 
   // velocity u = G^-1 p at a state whose sums sx, xpx are current
-  auto set_velocity = [&](FState<EPL>& s) {
+  auto set_velocity = [&](RState<EPL>& s) {
     const FieldCoef<double> f = field_coef<double>(s.v, s.pv, s.sx, s.xpx, a2, inv_s2, halfDx);
 #pragma unroll
     for (int k = 0; k < EPL; ++k) s.ux[k] = s.px[k] + f.ce * s.x[k];
     s.uv = f.uv;
   };
-  // kinetic energy T(q, p) = -log N(p; 0, G(q)) (geomjax/rmhmc/metrics.py:66-69) and
-  // the log density (examples/funnel/main.py:34-39) at a state with current sums
-  auto kinetic = [&](const FState<EPL>& s) -> double {
+  // kinetic energy T(q, p) = -log N(p; 0, G(q)) (geomjax/rmhmc/metrics.py:66-69)
+  auto kinetic = [&](const RState<EPL>& s) -> double {
     double d3 = 0.0;
 #pragma unroll
     for (int k = 0; k < EPL; ++k) d3 += s.px[k] * s.px[k];
@@ -176,38 +435,9 @@ __global__ void __launch_bounds__(64) funnel_nuts_kernel(const FunnelArgs a, con
     const double perp2 = group_sum<64>(d4) + rv * rv;
     return 0.5 * monge_quad(pp, perp2, gg, a2) + 0.5 * log(1.0 + a2 * gg) + 0.5 * (double)D * kLog2Pi;
   };
-  auto logdensity = [&](double v, double sx) -> double {
-    return -0.5 * v * v * inv_s2 - log(a.sigma) - 0.5 * kLog2Pi - 0.5 * exp(-v) * sx - 0.5 * (double)Dx * v -
-           0.5 * (double)Dx * kLog2Pi;
-  };
-  auto dot_u = [&](const double (&ux)[EPL], double uv, const double (&wx)[EPL], double wv) -> double {
-    double d = 0.0;
-#pragma unroll
-    for (int k = 0; k < EPL; ++k) d += ux[k] * wx[k];
-    return group_sum<64>(d) + uv * wv;
-  };
-  // the U-turn criteria (geomjax/rmhmc/metrics.py:100-117) on the two ends' momenta ml, mr
-  // and velocities vl, vr and the sums ms, vs over the states between them; every dot
-  // product is a group total, so the result is the same in every lane
-  auto is_turning = [&](const double (&mlx)[EPL], double mlv, const double (&mrx)[EPL], double mrv,
-                        const double (&msx)[EPL], double msv, const double (&vlx)[EPL], double vlv,
-                        const double (&vrx)[EPL], double vrv, const double (&vsx)[EPL], double vsv) -> bool {
-    if (crit == 1) return dot_u(vlx, vlv, msx, msv) <= 0.0 || dot_u(vrx, vrv, msx, msv) <= 0.0;  // "bet"
-    if (crit == 2) return dot_u(mlx, mlv, vsx, vsv) <= 0.0 || dot_u(mrx, mrv, vsx, vsv) <= 0.0;  // "riem"
-    return dot_u(vlx, vlv, vsx, vsv) <= 0.0 || dot_u(vrx, vrv, vsx, vsv) <= 0.0;                 // "euc"
-  };
-  auto ck_put = [&](int arr, int idx, const double (&x)[EPL], double v) {
-#pragma unroll
-    for (int k = 0; k < EPL; ++k) ck[arr * XS + idx * 64 * EPL + l * EPL + k] = x[k];
-    if (l == 0) ck[NA * XS + arr * M + idx] = v;  // v is replicated: lane 0 stores it
-  };
-  auto ck_get = [&](int arr, int idx, double (&x)[EPL]) -> double {
-#pragma unroll
-    for (int k = 0; k < EPL; ++k) x[k] = ck[arr * XS + idx * 64 * EPL + l * EPL + k];
-    return ck[NA * XS + arr * M + idx];
-  };
-  // one implicit-midpoint step of size hs = direction * eps (integrators.py:92-156)
-  auto one_step = [&](FState<EPL>& s, double hstep) {
+  auto energy = [&](const RState<EPL>& s) -> double { return -funnel_logp(s.v, s.sx, a.sigma, Dx) + kinetic(s); };
+  // one implicit-midpoint step of size hstep = direction * eps (integrators.py:92-156)
+  auto one_step = [&](RState<EPL>& s, double hstep) {
     const double h = 0.5 * hstep;
     double x0[EPL], p0[EPL];
 #pragma unroll
@@ -270,195 +500,147 @@ __global__ void __launch_bounds__(64) funnel_nuts_kernel(const FunnelArgs a, con
   // H0 with the chain's log density as given (the initial IntegratorState carries it)
   const double H0 = -logp0 + kinetic(s0);
 
-  // ---- the trajectory: its two ends, sums (the start state included), length ----
-  FState<EPL> left = s0, right = s0;
-  double psx[EPL], vsx[EPL], psv = s0.pv, vsv = s0.uv;
-#pragma unroll
-  for (int k = 0; k < EPL; ++k) {
-    psx[k] = s0.px[k];
-    vsx[k] = s0.ux[k];
-  }
-  int ntraj = 0;
+  RState<EPL> left, right;
   Prop<EPL> prop;
-#pragma unroll
-  for (int k = 0; k < EPL; ++k) prop.x[k] = s0.x[k];
-  prop.v = s0.v;
-  prop.energy = H0;
-  prop.weight = 0.0;
-  prop.slpa = -INFINITY;
-  auto set_prop = [&](Prop<EPL>& pr, const FState<EPL>& s, double en, double w) {
-#pragma unroll
-    for (int k = 0; k < EPL; ++k) pr.x[k] = s.x[k];
-    pr.v = s.v;
-    pr.energy = en;
-    pr.weight = w;
-    pr.slpa = fmin(w, 0.0);
-  };
-  int e = 0, j = 0;
-  bool diverging = false, turning = false;
-  while (e < M && !diverging && !turning) {
-    const double ud = uniform(nu.u_dir, (size_t)c * M + e, kStreamNutsDirection, (uint32_t)e);
-    const int dir = ud < 0.5 ? 1 : -1;
-    // ---- subtree of up to 2^e steps from the trajectory's end (trajectory.py:149-317) ----
-    FState<EPL> cur = dir > 0 ? right : left;
-    Prop<EPL> sp;
-    double ssx[EPL], ssv_x[EPL], ssp_v = 0.0, ssv_v = 0.0;
-#pragma unroll
-    for (int k = 0; k < EPL; ++k) {
-      ssx[k] = 0.0;
-      ssv_x[k] = 0.0;
-    }
-    int nsub = 0, step = 0;
-    const int max_steps = 1 << e;
-    bool div_s = false, term_s = false;
-    while (step < max_steps && !term_s && !div_s) {
-      one_step(cur, dir * a.eps);
-      const double en = -logdensity(cur.v, cur.sx) + kinetic(cur);
-      double w = H0 - en;
-      if (isnan(w)) w = -INFINITY;
-      div_s = -w > a.div_threshold;
-      const double us = uniform(nu.u_step, (size_t)c * S + j, kStreamNutsStep, (uint32_t)j);
-      if (step == 0) {
-#pragma unroll
-        for (int k = 0; k < EPL; ++k) {
-          ssx[k] = cur.px[k];
-          ssv_x[k] = cur.ux[k];
-        }
-        ssp_v = cur.pv;
-        ssv_v = cur.uv;
-        nsub = 1;
-        set_prop(sp, cur, en, w);
-      } else {
-#pragma unroll
-        for (int k = 0; k < EPL; ++k) {
-          ssx[k] += cur.px[k];
-          ssv_x[k] += cur.ux[k];
-        }
-        ssp_v += cur.pv;
-        ssv_v += cur.uv;
-        ++nsub;
-        // progressive uniform sampling (proposal.py:198-222)
-        const double dw = w - sp.weight;
-        const double pacc = 1.0 / (1.0 + exp(-dw));
-        const double nw = logaddexp(sp.weight, w), ns = logaddexp(sp.slpa, fmin(w, 0.0));
-        if (us < pacc) set_prop(sp, cur, en, w);
-        sp.weight = nw;
-        sp.slpa = ns;
-      }
-      ++j;
-      // numpyro checkpoints (termination.py:38-117): leaf index -> [idx_min, idx_max]
-      const int idx_max = ckpt_idx_max(step), idx_min = ckpt_idx_min(step);
-      if ((step & 1) == 0) {
-        ck_put(0, idx_max, cur.ux, cur.uv);
-        ck_put(1, idx_max, ssv_x, ssv_v);
-        if (NA == 4) {
-          ck_put(2, idx_max, cur.px, cur.pv);
-          ck_put(3, idx_max, ssx, ssp_v);
-        }
-      }
-      __syncthreads();  // one wave: orders the checkpoint writes before the reads below
-      for (int i = idx_max; i >= idx_min && !term_s; --i) {
-        // the sub-subtree from checkpoint i to the current state (termination.py:89-117)
-        double vl[EPL], tmp[EPL], vs[EPL], ml[EPL], ms[EPL];
-        const double vlv = ck_get(0, i, vl);
-        const double tv = ck_get(1, i, tmp);
-#pragma unroll
-        for (int k = 0; k < EPL; ++k) vs[k] = ssv_x[k] - tmp[k] + vl[k];
-        const double vsv = ssv_v - tv + vlv;
-        double mlv = 0.0, msv = 0.0;
-        if (NA == 4) {
-          mlv = ck_get(2, i, ml);
-          const double tm = ck_get(3, i, tmp);
-#pragma unroll
-          for (int k = 0; k < EPL; ++k) ms[k] = ssx[k] - tmp[k] + ml[k];
-          msv = ssp_v - tm + mlv;
-        } else {
-#pragma unroll
-          for (int k = 0; k < EPL; ++k) ml[k] = ms[k] = 0.0;
-        }
-        term_s = is_turning(ml, mlv, cur.px, cur.pv, ms, msv, vl, vlv, cur.ux, cur.uv, vs, vsv);
-      }
-      ++step;
-    }
-    // ---- merge (trajectory.py:580-660) ----
-    if (div_s || term_s) {
-      prop.slpa = logaddexp(prop.slpa, sp.slpa);
-    } else {
-      // progressive biased sampling (proposal.py:225-254)
-      const double um = uniform(nu.u_merge, (size_t)c * M + e, kStreamNutsMerge, (uint32_t)e);
-      const double pacc = fmin(exp(sp.weight - prop.weight), 1.0);
-      const double nw = logaddexp(prop.weight, sp.weight), ns = logaddexp(prop.slpa, sp.slpa);
-      if (um < pacc) prop = sp;
-      prop.weight = nw;
-      prop.slpa = ns;
-    }
-    if (dir > 0)
-      right = cur;
-    else
-      left = cur;
-#pragma unroll
-    for (int k = 0; k < EPL; ++k) {
-      psx[k] += ssx[k];
-      vsx[k] += ssv_x[k];
-    }
-    psv += ssp_v;
-    vsv += ssv_v;
-    ntraj += nsub;
-    const bool turn_full = is_turning(left.px, left.pv, right.px, right.pv, psx, psv, left.ux, left.uv, right.ux,
-                                      right.uv, vsx, vsv);
-    ++e;
-    diverging = div_s;
-    turning = term_s || turn_full;
-  }
-
-  // ---- outputs: the sampled proposal's state (RMHMCState) and NUTSInfo ----
-  double sxp = 0.0;
-#pragma unroll
-  for (int k = 0; k < EPL; ++k) sxp += prop.x[k] * prop.x[k];
-  sxp = group_sum<64>(sxp);
-  const double ep = exp(-prop.v);
-#pragma unroll
-  for (int k = 0; k < EPL; ++k) {
-    if (!on(k)) continue;
-    static_cast<double*>(a.q_out)[base + j0 + k] = prop.x[k];
-    if (a.g_out) static_cast<double*>(a.g_out)[base + j0 + k] = -ep * prop.x[k];
-    if (nu.left_position) static_cast<double*>(nu.left_position)[base + j0 + k] = left.x[k];
-    if (nu.right_position) static_cast<double*>(nu.right_position)[base + j0 + k] = right.x[k];
-  }
-  if (l == 0) {
-    static_cast<double*>(a.q_out)[base + Dx] = prop.v;
-    if (a.g_out)
-      static_cast<double*>(a.g_out)[base + Dx] = -prop.v * inv_s2 - halfDx + 0.5 * ep * sxp;
-    if (nu.left_position) static_cast<double*>(nu.left_position)[base + Dx] = left.v;
-    if (nu.right_position) static_cast<double*>(nu.right_position)[base + Dx] = right.v;
-    a.logp_out[c] = logdensity(prop.v, sxp);
-    if (nu.energy) nu.energy[c] = prop.energy;
-    if (nu.acceptance_rate) nu.acceptance_rate[c] = exp(prop.slpa) / (double)ntraj;
-    if (nu.is_divergent) nu.is_divergent[c] = (uint8_t)diverging;
-    if (nu.is_turning) nu.is_turning[c] = (uint8_t)turning;
-    if (nu.num_expansions) nu.num_expansions[c] = e;
-    if (nu.num_steps) nu.num_steps[c] = ntraj;
-  }
+  const TreeOut t = nuts_tree<EPL>(s0, H0, one_step, energy, uniform, nu, a.eps, a.div_threshold, ck, c, left, right,
+                                   prop);
+  write_outputs<EPL>(prop, left.x, left.v, right.x, right.v, t, c, D, a.sigma, a.q_out, a.logp_out, a.g_out,
+                     nullptr, nu);
 }
 
-hipError_t launch_funnel_nuts(const FunnelArgs& a, const NutsArgs& n, hipStream_t stream) {
-  const int epl = (a.D - 1 + 63) / 64;
-  const int na = nuts_checkpoint_arrays(n.criterion);
-  const size_t smem = ((size_t)na * n.max_doublings * 64 * (epl <= 1 ? 1 : (epl <= 2 ? 2 : 4)) +
-                       (size_t)na * n.max_doublings) * sizeof(double);
-  // up to 4 x 12 x 256 x 8 B = 96 KiB of checkpoints: past the 64 KiB default of dynamic LDS
-  auto launch = [&](auto kern) {
-    if (smem > 65536) {
-      const hipError_t e =
-          hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
-      if (e != hipSuccess) return e;
-    }
-    hipLaunchKernelGGL(kern, dim3(a.C), dim3(64), smem, stream, a, n);
-    return hipGetLastError();
+// ---------------------------------------------------------------------------
+// NUTS with the LMC-Monge integrator (geomjax/lmcmonge/nuts.py:80-207)
+// ---------------------------------------------------------------------------
+template <int EPL>
+__global__ void __launch_bounds__(64) funnel_nuts_lmc_kernel(const FunnelLmcArgs a, const NutsArgs nu) {
+  using namespace lmc_ops;
+  extern __shared__ __align__(16) double ck[];
+  const int c = blockIdx.x;
+  if (c >= a.C) return;
+  const int l = threadIdx.x;
+  const int D = a.D, Dx = a.D - 1;
+  const int j0 = l * EPL;
+  const double a2 = a.alpha2;
+  const double alpha = sqrt(a.alpha2);
+  const double inv_s2 = 1.0 / (a.sigma * a.sigma);
+  const double halfDx = 0.5 * Dx;
+  const bool exact_sm = a.exact_sm != 0;
+  const NutsUniform uniform{c, a.counter, (uint32_t)a.seed, (uint32_t)(a.seed >> 32)};
+  const size_t base = (size_t)c * D;
+
+  // diagonal inverse mass (the reference's inverse_mass_matrix)
+  double mi[EPL];
+#pragma unroll
+  for (int k = 0; k < EPL; ++k) mi[k] = j0 + k < Dx ? ld<double>(a.inv_mass, j0 + k) : 1.0;
+  const double miv = ld<double>(a.inv_mass, Dx);
+  double sum_log_mi;
+  {
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < EPL; ++k)
+      if (j0 + k < Dx) s += log(mi[k]);
+    sum_log_mi = group_sum<64>(s) + log(miv);
+  }
+
+  // momentum = M v + alpha2 L (v . g) g (the reference's metric_vector_product with its
+  // extra L, geomjax/lmcmonge/metrics.py:243-255; exact: alpha2 (v . g) g) from the
+  // velocity and the position's scalars
+  auto set_momentum = [&](LState<EPL>& s) {
+    double xv = 0.0;
+#pragma unroll
+    for (int k = 0; k < EPL; ++k) xv += s.x[k] * s.ux[k];
+    xv = group_sum<64>(xv);
+    const double vg = -s.P.e * xv + s.uv * s.P.gv;
+    const double coef = a.exact_momentum ? a2 * vg : a2 * s.P.L * vg;
+#pragma unroll
+    for (int k = 0; k < EPL; ++k) s.px[k] = j0 + k < Dx ? s.ux[k] / mi[k] + coef * (-s.P.e * s.x[k]) : 0.0;
+    s.pv = s.uv / miv + coef * s.P.gv;
   };
-  if (epl <= 1) return launch(funnel_nuts_kernel<1>);
-  if (epl <= 2) return launch(funnel_nuts_kernel<2>);
-  return launch(funnel_nuts_kernel<4>);
+  // energy -l + K - J (lmcmonge_energy, geomjax/lmcmonge/metrics.py:267-284)
+  auto energy = [&](const LState<EPL>& s) -> double {
+    return -funnel_logp(s.v, s.P.Sx, a.sigma, Dx) +
+           lmc_kinetic<double, 64, EPL>(s.P, s.x, s.ux, s.uv, mi, miv, j0, Dx, sum_log_mi, a2) - s.J;
+  };
+  // one lan_integrator step of size hstep = direction * eps (integrators.py:52-156)
+  auto one_step = [&](LState<EPL>& s, double hstep) {
+    double dJ = 0.0;
+    lmc_step<double, 64, EPL>(s.P, s.x, s.v, mi, s.ux, s.uv, dJ, hstep, 0.5 * hstep, a2, inv_s2, halfDx, miv,
+                              exact_sm);
+    s.J += dJ;
+    set_momentum(s);
+  };
+
+  // ---- state, velocity draw v0 = chol(M^-1 - alpha2 dl dl^T) z, momentum, H0 (lmc.py:151-210) ----
+  LState<EPL> s0;
+#pragma unroll
+  for (int k = 0; k < EPL; ++k) s0.x[k] = j0 + k < Dx ? ld<double>(a.q_in, base + j0 + k) : 0.0;
+  s0.v = ld<double>(a.q_in, base + Dx);
+  s0.J = a.J_in[c];
+  double loc_inc[EPL];
+  s0.P = position_scalars<double, 64, EPL>(s0.x, s0.v, mi, loc_inc, l, a2, inv_s2, halfDx, miv);
+  double z[EPL], zv, uacc;
+  transition_noise<double, EPL>(a, (size_t)c, 0, j0, Dx, true, z, zv, uacc);
+  draw_velocity<double, 64, EPL>(s0.P, s0.x, mi, loc_inc, z, zv, a2, alpha, miv, l, j0, Dx, s0.ux, s0.uv);
+  set_momentum(s0);
+  if (a.velocity != nullptr) {  // NUTSInfo.momentum holds the initial velocity (lmcmonge/nuts.py:386-396)
+#pragma unroll
+    for (int k = 0; k < EPL; ++k)
+      if (j0 + k < Dx) st<double>(a.velocity, base + j0 + k, s0.ux[k]);
+    if (l == 0) st<double>(a.velocity, base + Dx, s0.uv);
+  }
+  // H0 with the chain's log density and volume adjustment as given
+  const double H0 =
+      -a.logp_in[c] + lmc_kinetic<double, 64, EPL>(s0.P, s0.x, s0.ux, s0.uv, mi, miv, j0, Dx, sum_log_mi, a2) - s0.J;
+
+  LState<EPL> left, right;
+  Prop<EPL> prop;
+  const TreeOut t = nuts_tree<EPL>(s0, H0, one_step, energy, uniform, nu, a.eps, a.div_threshold, ck, c, left, right,
+                                   prop);
+  write_outputs<EPL>(prop, left.x, left.v, right.x, right.v, t, c, D, a.sigma, a.q_out, a.logp_out, a.g_out, a.J_out,
+                     nu);
+}
+
+namespace {
+
+// coordinates per lane of a 64-lane chain: 1, 2 or 4 (D - 1 <= 256)
+int nuts_epl(int D) {
+  const int epl = (D - 1 + 63) / 64;
+  return epl <= 1 ? 1 : (epl <= 2 ? 2 : 4);
+}
+
+// [NA][M][64 EPL] + [NA][M] doubles of checkpoints
+size_t nuts_smem(const NutsArgs& n, int epl) {
+  const int na = nuts_checkpoint_arrays(n.criterion);
+  return ((size_t)na * n.max_doublings * 64 * epl + (size_t)na * n.max_doublings) * sizeof(double);
+}
+
+// up to 4 x 12 x 256 x 8 B = 96 KiB of checkpoints: past the 64 KiB default of dynamic LDS
+template <class K, class A>
+hipError_t launch_one_nuts(K kern, const A& a, const NutsArgs& n, size_t smem, hipStream_t stream) {
+  if (smem > 65536) {
+    const hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(kern, dim3(a.C), dim3(64), smem, stream, a, n);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_funnel_nuts(const FunnelArgs& a, const NutsArgs& n, hipStream_t stream) {
+  const int epl = nuts_epl(a.D);
+  const size_t smem = nuts_smem(n, epl);
+  if (epl == 1) return launch_one_nuts(funnel_nuts_kernel<1>, a, n, smem, stream);
+  if (epl == 2) return launch_one_nuts(funnel_nuts_kernel<2>, a, n, smem, stream);
+  return launch_one_nuts(funnel_nuts_kernel<4>, a, n, smem, stream);
+}
+
+hipError_t launch_funnel_nuts_lmc(const FunnelLmcArgs& a, const NutsArgs& n, hipStream_t stream) {
+  const int epl = nuts_epl(a.D);
+  const size_t smem = nuts_smem(n, epl);
+  if (epl == 1) return launch_one_nuts(funnel_nuts_lmc_kernel<1>, a, n, smem, stream);
+  if (epl == 2) return launch_one_nuts(funnel_nuts_lmc_kernel<2>, a, n, smem, stream);
+  return launch_one_nuts(funnel_nuts_lmc_kernel<4>, a, n, smem, stream);
 }
 
 }  // namespace gjx

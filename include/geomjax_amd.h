@@ -206,6 +206,41 @@ int gjx_lmcmonge_funnel_step(const gjx_funnel_params* params, const gjx_lmc_step
 /* T LMC-Monge transitions per chain in one launch. */
 int gjx_lmcmonge_funnel_sample(const gjx_funnel_params* params, const gjx_lmc_sample_io* io, void* stream);
 
+/* NUTS with the LMC-Monge integrator (geomjax/lmcmonge/nuts.py:80-207,300-404):
+ * the iterative expansion of gjx_rmhmc_monge_funnel_nuts over lan_integrator
+ * steps, with the energy -l + K - J (the volume adjustment J carried along the
+ * trajectory, geomjax/lmcmonge/metrics.py:267-284).  Funnel, float64, one wave
+ * per chain, D - 1 <= 256, max_num_doublings in [1, 12], stopping criterion
+ * GJX_UTURN_*; flags and alpha2 as for gjx_lmcmonge_funnel_step.  The draws
+ * of one transition per chain: z (velocity), u_direction[e], u_merge[e] for
+ * expansion e, u_step[j] for integration step j (all NULL: device PRNG). */
+typedef struct {
+  const void* inverse_mass_matrix;   /* [D] float64 */
+  const void* position;              /* [C, D] */
+  const double* logdensity;          /* [C] */
+  const double* volume_adjustment;   /* [C] */
+  const void* z;                     /* [C, D] or NULL */
+  const double* u_direction;         /* [C, M] or NULL */
+  const double* u_merge;             /* [C, M] or NULL */
+  const double* u_step;              /* [C, 2^M - 1] or NULL */
+  void* new_position;                /* [C, D] */
+  double* new_logdensity;            /* [C] */
+  void* new_logdensity_grad;         /* [C, D] optional */
+  double* new_volume_adjustment;     /* [C] optional */
+  void* velocity;                    /* [C, D] optional: NUTSInfo.velocity (the v0 drawn) */
+  double* energy;                    /* [C] optional */
+  double* acceptance_rate;           /* [C] optional */
+  uint8_t* is_divergent;             /* [C] optional */
+  uint8_t* is_turning;               /* [C] optional */
+  int32_t* num_trajectory_expansions; /* [C] optional */
+  int32_t* num_integration_steps;    /* [C] optional */
+  void* leftmost_position;           /* [C, D] optional */
+  void* rightmost_position;          /* [C, D] optional */
+} gjx_lmc_nuts_io;
+
+int gjx_lmcmonge_funnel_nuts(const gjx_funnel_params* params, int32_t max_num_doublings, int32_t stopping_criterion,
+                             const gjx_lmc_nuts_io* io, void* stream);
+
 /* Derivative in alpha2 of the proposal position of one LMC-Monge transition
  * per chain: the jax.jacfwd(trajectory_proposal_position, argnums=0) that the
  * reference's ChEES adaptation for lmcmonge takes

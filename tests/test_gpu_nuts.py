@@ -4,8 +4,11 @@
   position, log density, energy, acceptance statistic and the trajectory's
   ends within 1e-10 norm-wise relative; numbers of integration steps and of
   expansions, and the turning / divergence flags identical;
-* the same for the "bet" and "riem" criteria;
-* stationarity with the device PRNG: chains started from exact draws of the
+* the same for the "bet" and "riem" criteria, for logistic regression with the
+  Fisher metric (and a batch of posteriors against each alone), and for the
+  LMC-Monge integrator (geomjax_amd.lmcmonge_nuts) with the volume adjustment
+  and both algebras;
+* stationarity with the device PRNG (RMHMC, and LMC-Monge in the exact algebra): chains started from exact draws of the
   funnel (v ~ N(0, 9), x_i | v ~ N(0, e^v)) keep v's mean and variance after
   ten transitions (within 5 standard errors).
 """
@@ -194,3 +197,80 @@ def test_logistic_nuts_batch_equals_each_posterior_alone():
         assert rel_err(nb.logdensity_grad[rows, :D], n1.logdensity_grad) <= 1e-12
         assert rel_err(ib.acceptance_rate[rows], i1.acceptance_rate) <= 1e-12
         assert torch.count_nonzero(nb.position[rows, D:]) == 0  # row tails past the posterior's D
+
+
+@pytest.mark.parametrize("D,C,eps,M,T,a2,unit_mass,faithful,crit,seed", [
+    (10, 4, 0.1, 5, 2, 1.0, True, True, "euc", 20),
+    (10, 4, 0.1, 5, 2, 0.6, False, False, "riem", 21),
+    (100, 2, 0.05, 4, 1, 1.0, True, True, "bet", 22),
+    (100, 2, 0.05, 4, 1, 1.0, True, True, "euc", 23),
+])
+def test_lmcmonge_nuts_matches_oracle(D, C, eps, M, T, a2, unit_mass, faithful, crit, seed):
+    from geomjax_amd.lmcmonge import LMCState
+    from geomjax_amd.lmcmonge import init as lmc_init
+    from geomjax_amd.lmcmonge_nuts import build_kernel as lnuts_kernel
+    from oracle import lmcmonge as olm
+
+    print(f"case lmcmonge D={D} C={C} eps={eps} M={M} T={T} alpha2={a2} unit_mass={unit_mass} faithful={faithful} "
+          f"criterion={crit}", flush=True)
+    Mo = om.NealFunnel(D)
+    model = gj.NealFunnel(D)
+    im = torch.ones(D, dtype=torch.float64) if unit_mass else 0.5 + seeded_uniform(600 + seed, D)
+    q0 = 0.5 * seeded_normal(610 + seed, C, D)
+    so = olm.init(q0, Mo.logp)
+    s = lmc_init(q0.to(DEV), model)
+    kern = lnuts_kernel(max_num_doublings=M, reference_faithful=faithful)
+    okern = onuts.build_lmcmonge_kernel(max_num_doublings=M, reference_faithful=faithful)
+    for t in range(T):
+        noise = (seeded_normal(620 + 10 * seed + t, C, D), seeded_uniform(630 + 10 * seed + t, C, M),
+                 seeded_uniform(640 + 10 * seed + t, C, M), seeded_uniform(650 + 10 * seed + t, C, 2 ** M - 1))
+        s1, info = kern(NUTSNoise(*(x.to(DEV) for x in noise)), s, model, eps, im.to(DEV), a2,
+                        stopping_criterion=crit)
+        so1, oi = okern(noise, so, Mo.logp, eps, im, a2, stopping_criterion=crit)
+        torch.cuda.synchronize()
+        print(f"  t={t}: steps {[o.num_integration_steps for o in oi]}, turning {[o.is_turning for o in oi]}, "
+              f"divergent {[o.is_divergent for o in oi]}", flush=True)
+        assert info.num_integration_steps.cpu().tolist() == [o.num_integration_steps for o in oi]
+        assert info.num_trajectory_expansions.cpu().tolist() == [o.num_trajectory_expansions for o in oi]
+        assert info.is_turning.cpu().tolist() == [o.is_turning for o in oi]
+        assert info.is_divergent.cpu().tolist() == [o.is_divergent for o in oi]
+        assert rel_err(s1.position, so1.position) < 1e-10
+        assert rel_err(s1.logdensity, so1.logdensity) < 1e-10
+        assert rel_err(s1.logdensity_grad, so1.logdensity_grad) < 1e-10
+        jscale = max(1.0, so1.volume_adjustment.abs().max().item())
+        assert (s1.volume_adjustment.cpu() - so1.volume_adjustment).abs().max().item() < 1e-10 * jscale
+        assert rel_err(info.energy, torch.tensor([o.energy for o in oi], dtype=torch.float64)) < 1e-10
+        assert rel_err(info.acceptance_rate, torch.tensor([o.acceptance_rate for o in oi], dtype=torch.float64)) < 1e-10
+        assert rel_err(info.velocity, torch.stack([o.momentum for o in oi])) < 1e-10
+        for c, o in enumerate(oi):
+            bar = 1e-3 if o.is_divergent else 1e-10
+            assert rel_err(info.trajectory_leftmost_state[c], o.trajectory_leftmost_state.position[0]) < bar, c
+            assert rel_err(info.trajectory_rightmost_state[c], o.trajectory_rightmost_state.position[0]) < bar, c
+        so = so1
+        s = LMCState(so.position.to(DEV), so.logdensity.to(DEV), so.logdensity_grad.to(DEV),
+                     so.volume_adjustment.to(DEV))
+
+
+def test_lmcmonge_nuts_keeps_the_funnel_stationary():
+    """Exact algebra (reference_faithful=False): chains started from exact funnel
+    draws keep v's mean and variance after ten transitions (5 standard errors)."""
+    D, C = 10, 4096
+    g = torch.Generator().manual_seed(78)
+    v = 3.0 * torch.randn(C, 1, generator=g, dtype=torch.float64)
+    x = torch.randn(C, D - 1, generator=g, dtype=torch.float64) * torch.exp(0.5 * v)
+    q = torch.cat([x, v], dim=1).to(DEV)
+    model = gj.NealFunnel(D)
+    im = torch.ones(D, dtype=torch.float64, device=DEV)
+    alg = gj.nutslmcmonge(model, 0.1, im, alpha2=1.0, max_num_doublings=6, reference_faithful=False)
+    s = alg.init(q)
+    accs = []
+    for t in range(10):
+        s, info = alg.step(gr.key(700 + t), s)
+        accs.append(float(info.acceptance_rate.mean()))
+    torch.cuda.synchronize()
+    vv = s.position[:, -1].cpu()
+    m, var = float(vv.mean()), float(vv.var())
+    print(f"v mean {m:.3f} (0), var {var:.3f} (9), acceptance {sum(accs) / 10:.3f}")
+    assert torch.isfinite(s.position).all()
+    assert abs(m) < 5 * 3.0 / C ** 0.5
+    assert abs(var - 9.0) < 5 * 9.0 * (2.0 / C) ** 0.5

@@ -109,3 +109,53 @@ def test_criteria_differ_under_the_monge_metric():
     lengths the three criteria give on this seed."""
     steps = {c: tuple(i.num_integration_steps for i in _run(4, 1.0, c, 30)[1]) for c in ("euc", "bet", "riem")}
     assert len(set(steps.values())) == 3, steps
+
+
+def test_lmcmonge_one_doubling_is_one_lmc_step():
+    """NUTS-LMC-Monge (geomjax/lmcmonge/nuts.py) with M = 1 is one lan_integrator
+    step of +-eps and the Metropolis test on -l + K - J: the lmcmonge oracle with
+    one integration step of that signed size, fed the same velocity normals and
+    the merge uniform as its accept uniform (non-unit mass, alpha2 = 0.6, a
+    carried volume adjustment; both algebras)."""
+    from oracle import lmcmonge as olm
+
+    D, C, eps = 5, 6, 0.1
+    M = om.NealFunnel(D)
+    im = 0.5 + seeded_uniform(11, D)
+    q0 = 0.5 * seeded_normal(12, C, D)
+    z, u = seeded_normal(13, C, D), seeded_uniform(14, C)
+    u_dir = torch.tensor([[0.2], [0.7], [0.4], [0.9], [0.1], [0.6]], dtype=torch.float64)
+    for faithful in (True, False):
+        s0 = olm.init(q0, M.logp)
+        s0 = s0._replace(volume_adjustment=0.01 * seeded_normal(15, C))
+        kern = onuts.build_lmcmonge_kernel(max_num_doublings=1, reference_faithful=faithful)
+        new, infos = kern((z, u_dir, u[:, None], torch.zeros(C, 1, dtype=torch.float64)), s0, M.logp, eps, im, 0.6)
+        k = olm.build_kernel(reference_faithful=faithful)
+        for c in range(C):
+            sgn = 1.0 if float(u_dir[c, 0]) < 0.5 else -1.0
+            sc = olm.LMCState(*(t[c:c + 1] for t in s0))
+            sr, ir = k((z[c:c + 1], u[c:c + 1]), sc, M.logp, sgn * eps, im, 1, 0.6)
+            assert rel_err(new.position[c:c + 1], sr.position) < 1e-12, (faithful, c)
+            assert abs(float(new.volume_adjustment[c]) - float(sr.volume_adjustment[0])) < 1e-12, (faithful, c)
+            assert abs(infos[c].acceptance_rate - float(ir.acceptance_rate[0])) < 1e-12, (faithful, c)
+            assert rel_err(infos[c].momentum, ir.velocity[0]) < 1e-14  # NUTSInfo carries the initial velocity
+
+
+def test_lmcmonge_criteria_coincide_under_the_identity_metric():
+    """alpha2 = 0 and unit mass: the LMC-Monge momentum M v + alpha2 L (v.g) g is
+    the velocity, and the three criteria give identical trajectories."""
+    from oracle import lmcmonge as olm
+
+    D, C, Mx = 4, 4, 5
+    M = om.NealFunnel(D)
+    q0 = 0.5 * seeded_normal(40, C, D)
+    noise = (seeded_normal(41, C, D), seeded_uniform(42, C, Mx), seeded_uniform(43, C, Mx),
+             seeded_uniform(44, C, 2 ** Mx - 1))
+    im = torch.ones(D, dtype=torch.float64)
+    runs = []
+    for crit in ("euc", "bet", "riem"):
+        kern = onuts.build_lmcmonge_kernel(max_num_doublings=Mx)
+        runs.append(kern(noise, olm.init(q0, M.logp), M.logp, 0.15, im, 0.0, stopping_criterion=crit))
+    for s, infos in runs[1:]:
+        assert torch.equal(s.position, runs[0][0].position)
+        assert [i.num_integration_steps for i in infos] == [i.num_integration_steps for i in runs[0][1]]
