@@ -557,6 +557,42 @@ def logistic_nuts_forms(K, W, dev, chains=1024):
     return res
 
 
+def lmc_alpha2_forms(dev, repeats=5):
+    """Cost of the LMC-Monge alpha2 tangent (float64, the derivative ChEES-lmcmonge
+    takes every iteration) against one float64 LMC-Monge transition on the funnel
+    D=100 with 8192 chains, 32 integration steps."""
+    import torch
+
+    import geomjax_amd as gj
+    from geomjax_amd import random as gr
+    from geomjax_amd.lmcmonge import build_kernel as lmc_kernel
+    from geomjax_amd.lmcmonge import init as lmc_init
+    from geomjax_amd.lmcmonge import proposal_alpha2_derivative
+
+    model = gj.NealFunnel(D_F, sigma=SIGMA)
+    q = torch.randn(CHAINS_F, D_F, generator=torch.Generator().manual_seed(SEED_F), dtype=torch.float64).to(dev)
+    s = lmc_init(q, model)
+    im = torch.ones(D_F, dtype=torch.float64, device=dev)
+    kern = lmc_kernel()
+    proposal_alpha2_derivative(gr.PRNGKey(3, 0), s, model, EPS_F, im, L_F, ALPHA2)
+    kern(gr.PRNGKey(3, 0), s, model, EPS_F, im, L_F, ALPHA2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(repeats):
+        proposal_alpha2_derivative(gr.PRNGKey(3, i), s, model, EPS_F, im, L_F, ALPHA2)
+    torch.cuda.synchronize()
+    t_tan = (time.perf_counter() - t0) / repeats
+    t0 = time.perf_counter()
+    for i in range(repeats):
+        kern(gr.PRNGKey(3, i), s, model, EPS_F, im, L_F, ALPHA2)
+    torch.cuda.synchronize()
+    t_pri = (time.perf_counter() - t0) / repeats
+    res = {"tangent_ms": t_tan * 1e3, "primal_f64_transition_ms": t_pri * 1e3, "tangent_over_primal": t_tan / t_pri,
+           "chains": CHAINS_F, "dim": D_F, "num_integration_steps": L_F}
+    log(f"lmc alpha2 tangent: {t_tan * 1e3:.2f} ms vs primal {t_pri * 1e3:.2f} ms ({t_tan / t_pri:.2f}x)")
+    return res
+
+
 def tangent_forms(dev, repeats=3):
     """Cost of the derivative of acceptance x squared jump w.r.t. (log eps, log alpha2):
     one tangent launch (one transition carried with its forward derivative) against one
@@ -610,6 +646,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the 1024-chain and funnel comparisons")
+    ap.add_argument("--only", choices=["funnel_nuts", "logistic_nuts", "lmc_alpha2"],
+                    help="run one secondary workload alone and print its JSON (for profiling it)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 256 MiB three times (reference kernel for the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -631,6 +669,15 @@ def main():
         log(f"copied {x.numel() * 4} bytes x 3")
         return
     K, W = max(1, args.steps), max(0, args.warmup)
+    if args.only == "funnel_nuts":
+        print(json.dumps(nuts_forms(K, W, dev)), flush=True)
+        return
+    if args.only == "logistic_nuts":
+        print(json.dumps(logistic_nuts_forms(K, W, dev)), flush=True)
+        return
+    if args.only == "lmc_alpha2":
+        print(json.dumps(lmc_alpha2_forms(dev)), flush=True)
+        return
 
     head = logistic_forms(CHAINS, K, W, dev, plain=not args.no_plain)
     tuned = head["tuned"]
@@ -653,6 +700,7 @@ def main():
         extra["logistic_nuts_1024_chains"] = logistic_nuts_forms(10, 2, dev)
         extra["funnel_lmcmonge_d100_8192_chains"] = funnel_lmc_forms(300, 100, dev, plain=not args.no_plain)
         extra["tangent"] = tangent_forms(dev)
+        extra["lmc_alpha2_tangent"] = lmc_alpha2_forms(dev)
 
     cpu = None
     if not args.no_cpu_baseline:
