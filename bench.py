@@ -557,6 +557,44 @@ def logistic_nuts_forms(K, W, dev, chains=1024):
     return res
 
 
+def logistic_lmc_forms(K, W, dev, chains=1024, L=10):
+    """The general-metric LMC kernel (lmcmc, Fisher metric, float64, one launch per
+    transition) on the headline posterior (logistic 2048 x 64): step size 0.1, 10
+    integration steps, device PRNG.  Reports leapfrog-steps/s and min-ESS/s."""
+    import torch
+
+    import geomjax_amd as gj
+    from geomjax_amd import random as gr
+    from geomjax_amd.lmcmc import build_kernel as lmc_kernel
+    from geomjax_amd.lmcmc import init as lmc_init
+    from geomjax_amd.synthetic import make_logistic
+
+    X, y, _ = make_logistic(N_ROWS, D_LOG, DATA_SEED)
+    model = gj.LogisticRegression(X, y, PRIOR, device=dev)
+    kern = lmc_kernel()
+    q = 0.1 * torch.randn(chains, D_LOG, generator=torch.Generator().manual_seed(INIT_SEED), dtype=torch.float64)
+    s = lmc_init(q.to(dev), model)
+    for t in range(W):
+        s, _ = kern(gr.PRNGKey(INIT_SEED + 13, t), s, model, EPS_LOG, gj.FisherMetric(), L)
+    samples = torch.empty(K, chains, D_LOG, dtype=torch.float64, device=dev)
+    accs = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for t in range(K):
+        s, info = kern(gr.PRNGKey(INIT_SEED + 13, W + t), s, model, EPS_LOG, gj.FisherMetric(), L)
+        samples[t].copy_(s.position)
+        accs.append(info.acceptance_rate)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ess, _ = min_ess(samples)
+    del samples
+    res = {"value": chains * L * K / wall, "unit": "leapfrog-steps/s", "min_ess_per_sec": ess / wall, "min_ess": ess,
+           "acceptance_rate": float(torch.stack(accs).mean()), "ms_per_transition": wall / K * 1e3, "chains": chains,
+           "num_integration_steps": L, "dtype": "f64", "steps": K, "kernel": "logistic_lmc_f64_kernel"}
+    log(f"logistic LMC (Fisher): {res['value']:.3e} leapfrog-steps/s, min-ESS/s {res['min_ess_per_sec']:.3e}")
+    return res
+
+
 def lmc_alpha2_forms(dev, repeats=5):
     """Cost of the LMC-Monge alpha2 tangent (float64, the derivative ChEES-lmcmonge
     takes every iteration) against one float64 LMC-Monge transition on the funnel
@@ -646,7 +684,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the 1024-chain and funnel comparisons")
-    ap.add_argument("--only", choices=["funnel_nuts", "logistic_nuts", "lmc_alpha2"],
+    ap.add_argument("--only", choices=["funnel_nuts", "logistic_nuts", "lmc_alpha2", "logistic_lmc"],
                     help="run one secondary workload alone and print its JSON (for profiling it)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 256 MiB three times (reference kernel for the FETCH_SIZE / WRITE_SIZE counters)")
@@ -675,6 +713,9 @@ def main():
     if args.only == "logistic_nuts":
         print(json.dumps(logistic_nuts_forms(K, W, dev)), flush=True)
         return
+    if args.only == "logistic_lmc":
+        print(json.dumps(logistic_lmc_forms(K, W, dev)), flush=True)
+        return
     if args.only == "lmc_alpha2":
         print(json.dumps(lmc_alpha2_forms(dev)), flush=True)
         return
@@ -698,6 +739,7 @@ def main():
         extra["funnel_nuts_d100_8192_chains"] = nuts_forms(100, 10, dev,
                                                            extra["funnel_d100_8192_chains"].get("plain"))
         extra["logistic_nuts_1024_chains"] = logistic_nuts_forms(10, 2, dev)
+        extra["logistic_lmc_1024_chains"] = logistic_lmc_forms(20, 5, dev)
         extra["funnel_lmcmonge_d100_8192_chains"] = funnel_lmc_forms(300, 100, dev, plain=not args.no_plain)
         extra["tangent"] = tangent_forms(dev)
         extra["lmc_alpha2_tangent"] = lmc_alpha2_forms(dev)

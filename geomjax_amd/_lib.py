@@ -179,6 +179,8 @@ _SIGNATURES = {
     "gjx_lmcmonge_funnel_sample": [ctypes.POINTER(FunnelParams), ctypes.POINTER(LmcSampleIO), _vp],
     "gjx_lmcmonge_funnel_alpha2_tangent": [ctypes.POINTER(FunnelParams), ctypes.POINTER(LmcAlpha2TangentIO), _vp],
     "gjx_lmcmonge_funnel_nuts": [ctypes.POINTER(FunnelParams), _c_int32, _c_int32, ctypes.POINTER(LmcNutsIO), _vp],
+    "gjx_lmc_fisher_logistic_nuts": [ctypes.POINTER(LogisticParams), ctypes.POINTER(LogisticData), _c_int32, _c_int32,
+                                     ctypes.POINTER(LmcNutsIO), _vp],
     "gjx_rmhmc_monge_funnel_tangent": [ctypes.POINTER(FunnelParams), ctypes.POINTER(TangentIO), _vp],
     "gjx_rmhmc_monge_funnel_nuts": [ctypes.POINTER(FunnelParams), _c_int32, _c_int32, ctypes.POINTER(NutsIO), _vp],
     "gjx_rmhmc_fisher_logistic_nuts": [ctypes.POINTER(LogisticParams), ctypes.POINTER(LogisticData), _c_int32,

@@ -602,6 +602,56 @@ int gjx_rmhmc_fisher_logistic_nuts(const gjx_logistic_params* params, const gjx_
   return GJX_OK;
 }
 
+int gjx_lmc_fisher_logistic_nuts(const gjx_logistic_params* params, const gjx_logistic_data* data,
+                                 int32_t max_num_doublings, int32_t stopping_criterion, const gjx_lmc_nuts_io* io,
+                                 void* stream) {
+  gjx::LogisticArgs a;
+  int rc = check_logistic(params, data, false, &a, 64);
+  if (rc != GJX_OK) return rc;
+  GJX_REQUIRE(params->dtype == GJX_F64, "the NUTS kernel runs in float64");
+  GJX_REQUIRE(max_num_doublings >= 1 && max_num_doublings <= 12, "max_num_doublings must be in [1, 12] (got %d)",
+              max_num_doublings);
+  GJX_REQUIRE(stopping_criterion == GJX_UTURN_EUC || stopping_criterion == GJX_UTURN_BET ||
+                  stopping_criterion == GJX_UTURN_RIEM,
+              "stopping_criterion must be GJX_UTURN_EUC, GJX_UTURN_BET or GJX_UTURN_RIEM (got %d)",
+              stopping_criterion);
+  GJX_REQUIRE(io != nullptr, "io is NULL");
+  GJX_REQUIRE(io->position && io->logdensity && io->volume_adjustment,
+              "position, logdensity and volume_adjustment are required");
+  GJX_REQUIRE(io->new_position && io->new_logdensity, "new_position and new_logdensity are required");
+  const bool host = io->z != nullptr;
+  GJX_REQUIRE((io->u_direction != nullptr) == host && (io->u_merge != nullptr) == host &&
+                  (io->u_step != nullptr) == host,
+              "z, u_direction, u_merge and u_step must all be given or all be NULL");
+  a.q_in = io->position;
+  a.logp_in = io->logdensity;
+  a.J_in = io->volume_adjustment;
+  a.z = io->z;
+  a.q_out = io->new_position;
+  a.logp_out = io->new_logdensity;
+  a.g_out = io->new_logdensity_grad;
+  a.J_out = io->new_volume_adjustment;
+  a.momentum = io->velocity;  // NUTSInfo.velocity: the v0 drawn
+  gjx::NutsArgs n;
+  std::memset(&n, 0, sizeof(n));
+  n.max_doublings = max_num_doublings;
+  n.criterion = stopping_criterion;
+  n.u_dir = io->u_direction;
+  n.u_merge = io->u_merge;
+  n.u_step = io->u_step;
+  n.energy = io->energy;
+  n.acceptance_rate = io->acceptance_rate;
+  n.is_divergent = io->is_divergent;
+  n.is_turning = io->is_turning;
+  n.num_expansions = io->num_trajectory_expansions;
+  n.num_steps = io->num_integration_steps;
+  n.left_position = io->leftmost_position;
+  n.right_position = io->rightmost_position;
+  hipError_t e = gjx::launch_logistic_nuts_lmc_f64(a, n, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "gjx_lmc_fisher_logistic_nuts launch");
+  return GJX_OK;
+}
+
 int gjx_rmhmc_fisher_logistic_sample(const gjx_logistic_params* params, const gjx_logistic_data* data,
                                      const gjx_rmhmc_sample_io* io, void* stream) {
   gjx::LogisticArgs a;

@@ -125,6 +125,8 @@ hipError_t launch_logistic_rmhmc_f64(const LogisticArgs& a, hipStream_t stream);
 hipError_t launch_logistic_rmhmc_tangent(const LogisticArgs& a, const TangentOut& o, hipStream_t stream);
 // NUTS with the RMHMC integrator, float64, D <= 64 (logistic_nuts.hip)
 hipError_t launch_logistic_nuts_f64(const LogisticArgs& a, const NutsArgs& n, hipStream_t stream);
+// NUTS with the general-metric LMC integrator, float64, D <= 64 (logistic_nuts.hip)
+hipError_t launch_logistic_nuts_lmc_f64(const LogisticArgs& a, const NutsArgs& n, hipStream_t stream);
 hipError_t launch_logistic_value_and_grad(const LogisticArgs& a, hipStream_t stream);
 hipError_t launch_logistic_lmc_f64(const LogisticArgs& a, hipStream_t stream);
 hipError_t launch_logistic_rmhmc_f32(const LogisticArgs& a, int max_features, hipStream_t stream);

@@ -21,13 +21,13 @@
 #include "common.h"
 #include "logistic.h"
 #include "logistic_common.h"
+#include "logistic_lmc_ops.h"
 
 namespace gjx {
 
 using namespace logistic;
 
-using C64 = Cfg<64>;  // the LMC kernel serves D <= 64
-constexpr int MAXD = C64::MAXD, LDX = C64::LDX;
+constexpr int MAXD = C64::MAXD;  // the LMC kernel serves D <= 64
 
 template <bool STEP>
 __global__ void __launch_bounds__(NT, 2) logistic_lmc_f64_kernel(const LogisticArgs a) {
@@ -60,63 +60,11 @@ __global__ void __launch_bounds__(NT, 2) logistic_lmc_f64_kernel(const LogisticA
   double A[4][4];
   __syncthreads();
 
-  // position quantities at S.q: S.Gi = G^-1, S.g = grad l, S.dH = grad log det G;
-  // returns log det G and (through loglik) the log-likelihood sum
   auto position_pass = [&](bool chol_draw, double& loglik) __attribute__((always_inline)) -> double {
-    pass_gram(S, X, y, N, D, prior, nullptr, 0.0, true, false, loglik);
-    load_block(S, A);
-    __syncthreads();
-    if (chol_draw) {
-      // velocity v0 = L^-T z, L = chol(G)   (metrics.py:75-91)
-      cholesky(S, D);
-      if (t < MAXD) S.vt[t] = t < D ? S.z[t] : 0.0;
-      __syncthreads();
-      for (int i = D - 1; i >= 0; --i) {
-        const double vi = S.vt[i] / S.Gi[i * LDX + i];
-        if (t < i) S.vt[t] -= S.Gi[i * LDX + t] * vi;
-        if (t == i) S.p[i] = vi;
-        __syncthreads();
-      }
-    }
-    const double ld = sweep(S, A, D);
-    __syncthreads();
-    store_neg_block(S, A);
-    __syncthreads();
-    pass_quad(S, X, y, N, D, prior, false);  // S.dH = X^T (w a)
-    return ld;
+    return lmc_position_pass(S, A, X, y, N, D, prior, chol_draw, loglik);
   };
-
-  // one half step (integrators.py:61-91) with velocity S.p; accumulates dJ
   auto half_step = [&](double& dJ) __attribute__((always_inline)) {
-    double dd = 0.0;
-    // S.vt = grad log det G and S.u = grad l survive the two Gram passes below
-    if (t < D) {
-      S.u[t] = S.g[t];
-      S.vt[t] = S.dH[t];
-    }
-    __syncthreads();
-    // A+ = G + (eps/2) Omega~(v) and G v
-    pass_gram(S, X, y, N, D, prior, S.p, 0.25 * eps, false, true, dd);
-    load_block(S, A);
-    __syncthreads();
-    dJ -= sweep(S, A, D);
-    __syncthreads();
-    store_neg_block(S, A);  // S.Gi = A+^-1
-    // v_temp = G v - (eps/2) (-g + grad log det G / 2)
-    if (t < D) S.qs[t] = S.gv[t] - 0.5 * eps * (-S.u[t] + 0.5 * S.vt[t]);
-    __syncthreads();
-    ginv_times(S, S.qs, S.p, D);  // v' = A+^-1 v_temp
-    // A- = G - (eps/2) Omega~(v')
-    pass_gram(S, X, y, N, D, prior, S.p, -0.25 * eps, false, false, dd);
-    load_block(S, A);
-    __syncthreads();
-    dJ += sweep(S, A, D);
-    __syncthreads();
-    if (t < D) {
-      S.g[t] = S.u[t];
-      S.dH[t] = S.vt[t];
-    }
-    __syncthreads();
+    lmc_half_step(S, A, X, y, N, D, prior, eps, dJ);
   };
 
   for (int tr = 0; tr < T_steps; ++tr) {

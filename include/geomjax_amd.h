@@ -493,6 +493,17 @@ int gjx_rmhmc_fisher_logistic_nuts(const gjx_logistic_params* params, const gjx_
                                    int32_t max_num_doublings, int32_t stopping_criterion, const gjx_nuts_io* io,
                                    void* stream);
 
+/* NUTS with the general-metric LMC integrator (geomjax/lmcmc/nuts.py:78-161)
+ * for Bayesian logistic regression with the Fisher metric: the iterative
+ * expansion over the lan_integrator steps of gjx_lmc_fisher_logistic_step, with
+ * the energy -l - log det G / 2 + v^T G v / 2 - J (geomjax/lmcmc/metrics.py:
+ * 209-221).  float64, every posterior D <= 64, one workgroup per chain; io as
+ * gjx_lmc_nuts_io with rows of stride params->dim (inverse_mass_matrix unused;
+ * velocity = NUTSInfo.velocity, the v0 drawn). */
+int gjx_lmc_fisher_logistic_nuts(const gjx_logistic_params* params, const gjx_logistic_data* data,
+                                 int32_t max_num_doublings, int32_t stopping_criterion, const gjx_lmc_nuts_io* io,
+                                 void* stream);
+
 /* Log density and its gradient of Neal's funnel for C positions: the
  * jax.value_and_grad(logdensity_fn)(position) of the reference's init
  * (geomjax/rmhmc/rmhmc.py:96-98, geomjax/lmcmonge/lmc.py:101-109) with the

@@ -4,6 +4,7 @@ TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).
 
 Restates geomjax/rmhmc/nuts.py:78-161,253-348 (kernel, iterative proposal),
 geomjax/lmcmonge/nuts.py:80-207,300-404 (the LMC-Monge kernel),
+geomjax/lmcmc/nuts.py:78-161,253-351 (the general-metric LMC kernel),
 geomjax/mcmc/trajectory.py:40-103 (trajectory helpers), :149-317
 (dynamic_progressive_integration), :517-668 (dynamic_multiplicative_expansion),
 geomjax/mcmc/termination.py:30-119 (numpyro-style iterative U-turn),
@@ -252,6 +253,45 @@ def build_lmcmonge_kernel(divergence_threshold=1000.0, max_num_doublings=10, ref
             torch.cat([s.position for s in outs]), torch.cat([s.logdensity for s in outs]),
             torch.cat([s.logdensity_grad_norm * torch.sqrt(s.determinant_metric)[:, None] for s in outs]),
             torch.cat([s.volume_adjustment.reshape(1) for s in outs]))
+        return new_state, infos
+
+    return kernel
+
+
+def build_lmcmc_kernel(divergence_threshold=1000.0):
+    """geomjax/lmcmc/nuts.py:78-161,253-351: the same expansion over the
+    general-metric lan_integrator (oracle.lmcmc) with the energy
+    -l - log det G / 2 + v^T G v / 2 - J, noise as for ``build_kernel``.  Returns
+    (LMCState, list of NUTSInfo) with NUTSInfo's first field holding the initial
+    velocity, as the reference's does."""
+    from . import lmcmc as olc
+
+    def kernel(noise, state, logdensity_fn, step_size, metric_fn, max_num_doublings=10, stopping_criterion="euc"):
+        z, u_dir, u_merge, u_step = noise
+        (velocity_generator, kinetic_energy, omega_tilde_fn, grad_logdetmetric,
+         metric_vector_product) = olc.gaussian_riemannian(metric_fn)
+        integrator = olc.lan_integrator(logdensity_fn, omega_tilde_fn, grad_logdetmetric, metric_vector_product)
+        energy = olc.lmc_energy(kinetic_energy)
+        position, logdensity, logdensity_grad, volume_adjustment = state
+        velocity = vmap(velocity_generator)(z, position)
+        momentum = vmap(metric_vector_product)(position, velocity)
+        J = torch.as_tensor(volume_adjustment, dtype=position.dtype).expand(position.shape[0])
+        M = int(max_num_doublings)
+        outs, infos = [], []
+        for c in range(position.shape[0]):
+            r = slice(c, c + 1)
+            s0 = olc.IntegratorState(position[r], momentum[r], velocity[r], logdensity[r], logdensity_grad[r], J[r])
+            H0 = float(energy(s0)[0])
+            proposal, traj, e, diverging, turning = _expand(s0, H0, integrator, energy, step_size, u_dir[c],
+                                                            u_merge[c], u_step[c], M, stopping_criterion,
+                                                            divergence_threshold)
+            acc = math.exp(proposal.sum_log_p_accept) / traj.num_states
+            outs.append(proposal.state)
+            infos.append(NUTSInfo(velocity[c], diverging, turning, proposal.energy, traj.leftmost_state,
+                                  traj.rightmost_state, e, traj.num_states, acc))
+        new_state = olc.LMCState(torch.cat([s.position for s in outs]), torch.cat([s.logdensity for s in outs]),
+                                 torch.cat([s.logdensity_grad for s in outs]),
+                                 torch.cat([s.volume_adjustment.reshape(1) for s in outs]))
         return new_state, infos
 
     return kernel

@@ -274,3 +274,107 @@ def test_lmcmonge_nuts_keeps_the_funnel_stationary():
     assert torch.isfinite(s.position).all()
     assert abs(m) < 5 * 3.0 / C ** 0.5
     assert abs(var - 9.0) < 5 * 9.0 * (2.0 / C) ** 0.5
+
+
+@pytest.mark.parametrize("N,D,C,eps,M,T,seed,crit", [
+    (120, 6, 3, 0.6, 4, 2, 40, "euc"),
+    (120, 6, 3, 0.6, 4, 2, 41, "riem"),
+    (200, 12, 3, 0.5, 5, 2, 42, "bet"),
+    (120, 6, 3, 3.0, 4, 2, 44, "euc"),     # divergent trajectories
+    (256, 64, 2, 0.05, 3, 1, 43, "euc"),   # the kernel's full feature capacity
+])
+def test_lmcmc_nuts_matches_oracle(N, D, C, eps, M, T, seed, crit):
+    from geomjax_amd.lmcmc import LMCState
+    from geomjax_amd.lmcmc import init as lmc_init
+    from geomjax_amd.lmcmc_nuts import build_kernel as lnuts_kernel
+    from geomjax_amd.synthetic import make_logistic
+    from oracle import lmcmc as olc
+
+    print(f"case lmcmc-nuts N={N} D={D} C={C} eps={eps} M={M} T={T} criterion={crit}", flush=True)
+    X, y, _ = make_logistic(N, D, seed)
+    model = gj.LogisticRegression(X, y, 100.0)
+    Mo = om.LogisticRegression(X, y, 100.0)
+    q0 = 0.3 * seeded_normal(700 + seed, C, D)
+    so = olc.init(q0, Mo.logp)
+    s = lmc_init(q0.to(DEV), model)
+    kern = lnuts_kernel()
+    okern = onuts.build_lmcmc_kernel()
+    for t in range(T):
+        noise = (seeded_normal(720 + 10 * seed + t, C, D), seeded_uniform(730 + 10 * seed + t, C, M),
+                 seeded_uniform(740 + 10 * seed + t, C, M), seeded_uniform(750 + 10 * seed + t, C, 2 ** M - 1))
+        s1, info = kern(NUTSNoise(*(x.to(DEV) for x in noise)), s, model, eps, gj.FisherMetric(), M,
+                        stopping_criterion=crit)
+        so1, oi = okern(noise, so, Mo.logp, eps, Mo.fisher_metric_fn, max_num_doublings=M, stopping_criterion=crit)
+        torch.cuda.synchronize()
+        print(f"  t={t}: steps {[o.num_integration_steps for o in oi]}, turning {[o.is_turning for o in oi]}, "
+              f"divergent {[o.is_divergent for o in oi]}", flush=True)
+        assert info.num_integration_steps.cpu().tolist() == [o.num_integration_steps for o in oi]
+        assert info.num_trajectory_expansions.cpu().tolist() == [o.num_trajectory_expansions for o in oi]
+        assert info.is_turning.cpu().tolist() == [o.is_turning for o in oi]
+        assert info.is_divergent.cpu().tolist() == [o.is_divergent for o in oi]
+        assert rel_err(s1.position, so1.position) < 1e-10
+        assert rel_err(s1.logdensity, so1.logdensity) < 1e-10
+        assert rel_err(s1.logdensity_grad, so1.logdensity_grad) < 1e-10
+        jscale = max(1.0, so1.volume_adjustment.abs().max().item())
+        assert (s1.volume_adjustment.cpu() - so1.volume_adjustment).abs().max().item() < 1e-10 * jscale
+        assert rel_err(info.energy, torch.tensor([o.energy for o in oi], dtype=torch.float64)) < 1e-10
+        assert rel_err(info.acceptance_rate, torch.tensor([o.acceptance_rate for o in oi], dtype=torch.float64)) < 1e-10
+        assert rel_err(info.velocity, torch.stack([o.momentum for o in oi])) < 1e-10
+        for c, o in enumerate(oi):
+            bar = 1e-3 if o.is_divergent else 1e-10
+            assert rel_err(info.trajectory_leftmost_state[c], o.trajectory_leftmost_state.position[0]) < bar, c
+            assert rel_err(info.trajectory_rightmost_state[c], o.trajectory_rightmost_state.position[0]) < bar, c
+        so = so1
+        s = LMCState(so.position.to(DEV), so.logdensity.to(DEV), so.logdensity_grad.to(DEV),
+                     so.volume_adjustment.to(DEV))
+
+
+def test_lmcmc_nuts_batch_equals_each_posterior_alone():
+    from geomjax_amd.lmcmc import init as lmc_init
+    from geomjax_amd.lmcmc_nuts import build_kernel as lnuts_kernel
+    from geomjax_amd.synthetic import make_logistic
+
+    Xa, ya, _ = make_logistic(150, 5, 51)
+    Xb, yb, _ = make_logistic(90, 9, 52)
+    ma, mb = gj.LogisticRegression(Xa, ya, 100.0), gj.LogisticRegression(Xb, yb, 100.0)
+    batch = gj.LogisticBatch([ma, mb], [3, 2])
+    C, Dm, M = 5, 9, 5
+    q = torch.zeros(C, Dm, dtype=torch.float64)
+    q[:3, :5] = 0.3 * seeded_normal(61, 3, 5)
+    q[3:, :9] = 0.3 * seeded_normal(62, 2, 9)
+    z = torch.zeros(C, Dm, dtype=torch.float64)
+    z[:3, :5] = seeded_normal(63, 3, 5)
+    z[3:, :9] = seeded_normal(64, 2, 9)
+    ud, um, us = seeded_uniform(65, C, M), seeded_uniform(66, C, M), seeded_uniform(67, C, 2 ** M - 1)
+    kern = lnuts_kernel()
+    sb = lmc_init(q.to(DEV), batch)
+    nb, ib = kern(NUTSNoise(z.to(DEV), ud.to(DEV), um.to(DEV), us.to(DEV)), sb, batch, 0.2, gj.FisherMetric(), M)
+    for model, rows, D in ((ma, slice(0, 3), 5), (mb, slice(3, 5), 9)):
+        s = lmc_init(q[rows, :D].contiguous().to(DEV), model)
+        n1, i1 = kern(NUTSNoise(*(x[rows].contiguous().to(DEV) for x in (z[:, :D], ud, um, us))), s, model, 0.2,
+                      gj.FisherMetric(), M)
+        torch.cuda.synchronize()
+        assert torch.equal(ib.num_integration_steps[rows], i1.num_integration_steps)
+        assert rel_err(nb.position[rows, :D], n1.position) <= 1e-12
+        assert rel_err(nb.volume_adjustment[rows], n1.volume_adjustment) <= 1e-12
+        assert rel_err(ib.acceptance_rate[rows], i1.acceptance_rate) <= 1e-12
+        assert torch.count_nonzero(nb.position[rows, D:]) == 0  # row tails past the posterior's D
+
+
+def test_nutslmc_sampling_algorithm_runs_on_device_noise():
+    """gj.nutslmc's SamplingAlgorithm with a device Philox key: finite states,
+    step counts within 2^M - 1, acceptance in [0, 1]."""
+    from geomjax_amd.synthetic import make_logistic
+
+    X, y, _ = make_logistic(300, 10, 71)
+    model = gj.LogisticRegression(X, y, 100.0)
+    alg = gj.nutslmc(model, 0.2, gj.FisherMetric(), max_num_doublings=6)
+    s = alg.init(0.1 * seeded_normal(72, 256, 10).to(DEV))
+    for t in range(3):
+        s, info = alg.step(gr.key(800 + t), s)
+    torch.cuda.synchronize()
+    assert torch.isfinite(s.position).all() and torch.isfinite(s.volume_adjustment).all()
+    steps = info.num_integration_steps.cpu()
+    assert (steps >= 1).all() and (steps <= 2 ** 6 - 1).all()
+    acc = info.acceptance_rate.cpu()
+    assert ((acc >= 0) & (acc <= 1)).all()

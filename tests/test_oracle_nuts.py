@@ -159,3 +159,32 @@ def test_lmcmonge_criteria_coincide_under_the_identity_metric():
     for s, infos in runs[1:]:
         assert torch.equal(s.position, runs[0][0].position)
         assert [i.num_integration_steps for i in infos] == [i.num_integration_steps for i in runs[0][1]]
+
+
+def test_lmcmc_one_doubling_is_one_lmc_step():
+    """NUTS-LMC (general metric, geomjax/lmcmc/nuts.py) with M = 1 is one
+    lan_integrator step of +-eps with the Metropolis test on -l + K - J: the lmcmc
+    oracle with one integration step of that signed size (logistic regression,
+    Fisher metric, a carried volume adjustment)."""
+    from oracle import lmcmc as olc
+    from geomjax_amd.synthetic import make_logistic
+
+    X, y, _ = make_logistic(60, 4, 3)
+    Mo = om.LogisticRegression(X, y, 100.0)
+    C, eps = 5, 0.2
+    q0 = 0.3 * seeded_normal(50, C, 4)
+    z, u = seeded_normal(51, C, 4), seeded_uniform(52, C)
+    u_dir = torch.tensor([[0.2], [0.7], [0.4], [0.9], [0.1]], dtype=torch.float64)
+    s0 = olc.init(q0, Mo.logp)
+    s0 = s0._replace(volume_adjustment=0.01 * seeded_normal(53, C))
+    new, infos = onuts.build_lmcmc_kernel()((z, u_dir, u[:, None], torch.zeros(C, 1, dtype=torch.float64)), s0,
+                                            Mo.logp, eps, Mo.fisher_metric_fn, max_num_doublings=1)
+    k = olc.build_kernel()
+    for c in range(C):
+        sgn = 1.0 if float(u_dir[c, 0]) < 0.5 else -1.0
+        sr, ir = k((z[c:c + 1], u[c:c + 1]), olc.LMCState(*(t[c:c + 1] for t in s0)), Mo.logp, sgn * eps,
+                   Mo.fisher_metric_fn, 1)
+        assert rel_err(new.position[c:c + 1], sr.position) < 1e-12, c
+        assert abs(float(new.volume_adjustment[c]) - float(sr.volume_adjustment[0])) < 1e-12, c
+        assert abs(infos[c].acceptance_rate - float(ir.acceptance_rate[0])) < 1e-12, c
+        assert rel_err(infos[c].momentum, ir.velocity[0]) < 1e-14  # NUTSInfo carries the initial velocity
