@@ -513,25 +513,33 @@ def nuts_forms(K, W, dev, static_plain=None):
     return res
 
 
-def logistic_nuts_forms(K, W, dev, chains=1024):
-    """NUTS with the RMHMC integrator on the headline posterior (logistic
-    regression 2048 x 64, Fisher metric, float64, one launch per transition):
-    the headline step size and fixed-point cap, max_num_doublings 5, "euc",
-    device PRNG.  Reports integration steps/s and min-ESS/s."""
+def logistic_nuts_forms(K, W, dev, chains=1024, integrator="rmhmc"):
+    """NUTS on the headline posterior (logistic regression 2048 x 64, Fisher
+    metric, float64, one launch per transition) with the RMHMC integrator (the
+    headline step size and fixed-point cap) or the general-metric LMC integrator
+    (``integrator="lmc"``, same step size): max_num_doublings 5, "euc", device
+    PRNG.  Reports integration steps/s and min-ESS/s."""
     import torch
 
     import geomjax_amd as gj
     from geomjax_amd import random as gr
-    from geomjax_amd.nuts import build_kernel as nuts_kernel
-    from geomjax_amd.rmhmc import implicit_midpoint
     from geomjax_amd.synthetic import make_logistic
 
     M = 5
     X, y, _ = make_logistic(N_ROWS, D_LOG, DATA_SEED)
     model = gj.LogisticRegression(X, y, PRIOR, device=dev)
-    kern = nuts_kernel(functools.partial(implicit_midpoint, max_iters=MAXIT_LOG))
     q_init = 0.1 * torch.randn(chains, D_LOG, generator=torch.Generator().manual_seed(INIT_SEED), dtype=torch.float64)
-    s = gj.rmhmc.init(q_init.to(dev), model)
+    if integrator == "lmc":
+        from geomjax_amd.lmcmc import init as lmc_init
+        from geomjax_amd.lmcmc_nuts import build_kernel as lnuts_kernel
+
+        kern, kname, s = lnuts_kernel(), "logistic_nuts_lmc_f64_kernel", lmc_init(q_init.to(dev), model)
+    else:
+        from geomjax_amd.nuts import build_kernel as nuts_kernel
+        from geomjax_amd.rmhmc import implicit_midpoint
+
+        kern = nuts_kernel(functools.partial(implicit_midpoint, max_iters=MAXIT_LOG))
+        kname, s = "logistic_nuts_f64_kernel", gj.rmhmc.init(q_init.to(dev), model)
     for t in range(W):
         s, _ = kern(gr.PRNGKey(INIT_SEED + 11, t), s, model, EPS_LOG, gj.FisherMetric(), M)
     samples = torch.empty(K, chains, D_LOG, dtype=torch.float64, device=dev)
@@ -551,8 +559,8 @@ def logistic_nuts_forms(K, W, dev, chains=1024):
     res = {"value": total_steps / wall, "unit": "integration-steps/s", "min_ess_per_sec": ess / wall, "min_ess": ess,
            "mean_integration_steps": total_steps / (K * chains), "acceptance_rate": float(torch.stack(accs).mean()),
            "ms_per_transition": wall / K * 1e3, "chains": chains, "dtype": "f64", "max_num_doublings": M, "steps": K,
-           "kernel": "logistic_nuts_f64_kernel"}
-    log(f"logistic NUTS: {res['value']:.3e} integration-steps/s, {res['mean_integration_steps']:.1f} steps per "
+           "kernel": kname}
+    log(f"logistic NUTS ({integrator}): {res['value']:.3e} integration-steps/s, {res['mean_integration_steps']:.1f} steps per "
         f"transition, min-ESS/s {res['min_ess_per_sec']:.3e}")
     return res
 
@@ -684,7 +692,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the 1024-chain and funnel comparisons")
-    ap.add_argument("--only", choices=["funnel_nuts", "logistic_nuts", "lmc_alpha2", "logistic_lmc"],
+    ap.add_argument("--only", choices=["funnel_nuts", "logistic_nuts", "logistic_nuts_lmc", "lmc_alpha2", "logistic_lmc"],
                     help="run one secondary workload alone and print its JSON (for profiling it)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 256 MiB three times (reference kernel for the FETCH_SIZE / WRITE_SIZE counters)")
@@ -713,6 +721,9 @@ def main():
     if args.only == "logistic_nuts":
         print(json.dumps(logistic_nuts_forms(K, W, dev)), flush=True)
         return
+    if args.only == "logistic_nuts_lmc":
+        print(json.dumps(logistic_nuts_forms(K, W, dev, integrator="lmc")), flush=True)
+        return
     if args.only == "logistic_lmc":
         print(json.dumps(logistic_lmc_forms(K, W, dev)), flush=True)
         return
@@ -740,6 +751,7 @@ def main():
                                                            extra["funnel_d100_8192_chains"].get("plain"))
         extra["logistic_nuts_1024_chains"] = logistic_nuts_forms(10, 2, dev)
         extra["logistic_lmc_1024_chains"] = logistic_lmc_forms(20, 5, dev)
+        extra["logistic_nuts_lmc_1024_chains"] = logistic_nuts_forms(10, 2, dev, integrator="lmc")
         extra["funnel_lmcmonge_d100_8192_chains"] = funnel_lmc_forms(300, 100, dev, plain=not args.no_plain)
         extra["tangent"] = tangent_forms(dev)
         extra["lmc_alpha2_tangent"] = lmc_alpha2_forms(dev)
