@@ -168,6 +168,21 @@ class LmcmcSampleIO(ctypes.Structure):
     )]
 
 
+class MmalaStepIO(ctypes.Structure):
+    _fields_ = [(n, _vp) for n in (
+        "position", "logdensity", "logdensity_grad", "z", "u", "new_position", "new_logdensity",
+        "new_logdensity_grad", "new_metric", "new_gamma", "acceptance_rate", "is_accepted", "is_divergent", "weight",
+        "proposal_position", "proposal_logdensity", "proposal_logdensity_grad", "proposal_metric", "proposal_gamma",
+    )]
+
+
+class MmalaSampleIO(ctypes.Structure):
+    _fields_ = [(n, _vp) for n in (
+        "position", "logdensity", "z", "u", "final_position", "final_logdensity", "final_logdensity_grad",
+        "final_metric", "final_gamma", "samples", "acceptance_rate", "num_accepted",
+    )]
+
+
 LMC_EXACT_SHERMAN_MORRISON = 1
 LMC_EXACT_MOMENTUM = 2
 
@@ -196,6 +211,12 @@ _SIGNATURES = {
                                        ctypes.POINTER(LmcmcSampleIO), _vp],
     "gjx_logistic_value_and_grad": [ctypes.POINTER(LogisticParams), ctypes.POINTER(LogisticData), _vp, _vp, _vp,
                                     _vp],
+    "gjx_mmala_fisher_logistic_init": [ctypes.POINTER(LogisticParams), ctypes.POINTER(LogisticData), _vp, _vp, _vp,
+                                       _vp, _vp, _vp],
+    "gjx_mmala_fisher_logistic_step": [ctypes.POINTER(LogisticParams), ctypes.POINTER(LogisticData), _c_int32,
+                                       ctypes.POINTER(MmalaStepIO), _vp],
+    "gjx_mmala_fisher_logistic_sample": [ctypes.POINTER(LogisticParams), ctypes.POINTER(LogisticData), _c_int32,
+                                         ctypes.POINTER(MmalaSampleIO), _vp],
     "gjx_rmhmc_fisher_logistic_tangent": [ctypes.POINTER(LogisticParams), ctypes.POINTER(LogisticData),
                                           ctypes.POINTER(TangentIO), _vp],
     "gjx_logistic_pack_size": [_c_int32, _vp, _vp, ctypes.POINTER(ctypes.c_uint64)],

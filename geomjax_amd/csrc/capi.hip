@@ -750,6 +750,89 @@ int gjx_lmc_fisher_logistic_sample(const gjx_logistic_params* params, const gjx_
   return GJX_OK;
 }
 
+int gjx_mmala_fisher_logistic_init(const gjx_logistic_params* params, const gjx_logistic_data* data,
+                                   const void* position, double* logdensity, void* logdensity_grad, double* metric,
+                                   double* gamma, void* stream) {
+  if (int rc0 = require_f64(params, "gjx_mmala_fisher_logistic_init")) return rc0;
+  gjx::LogisticArgs a;
+  int rc = check_logistic(params, data, false, &a, gjx::kLogisticMaxDLmc);
+  if (rc != GJX_OK) return rc;
+  GJX_REQUIRE(position && logdensity && logdensity_grad, "position, logdensity and logdensity_grad are required");
+  a.q_in = position;
+  a.logp_out = logdensity;
+  a.g_out = logdensity_grad;
+  a.metric_out = metric;
+  a.gamma_out = gamma;
+  hipError_t e = gjx::launch_logistic_mmala_init(a, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "gjx_mmala_fisher_logistic_init launch");
+  return GJX_OK;
+}
+
+int gjx_mmala_fisher_logistic_step(const gjx_logistic_params* params, const gjx_logistic_data* data,
+                                   int reference_faithful, const gjx_mmala_step_io* io, void* stream) {
+  if (int rc0 = require_f64(params, "gjx_mmala_fisher_logistic_step")) return rc0;
+  gjx::LogisticArgs a;
+  int rc = check_logistic(params, data, false, &a, gjx::kLogisticMaxDLmc);
+  if (rc != GJX_OK) return rc;
+  GJX_REQUIRE(params->step_size > 0.0, "step_size must be > 0 (the proposal covariance is 2 step_size G^-1)");
+  GJX_REQUIRE(io != nullptr, "io is NULL");
+  GJX_REQUIRE(io->position && io->logdensity, "position and logdensity are required");
+  GJX_REQUIRE(io->new_position && io->new_logdensity, "new_position and new_logdensity are required");
+  GJX_REQUIRE((io->z == nullptr) == (io->u == nullptr), "z and u must both be given or both be NULL");
+  a.q_in = io->position;
+  a.logp_in = io->logdensity;
+  a.g_in = io->logdensity_grad;
+  a.z = io->z;
+  a.u = io->u;
+  a.q_out = io->new_position;
+  a.logp_out = io->new_logdensity;
+  a.g_out = io->new_logdensity_grad;
+  a.metric_out = io->new_metric;
+  a.gamma_out = io->new_gamma;
+  a.acc_rate = io->acceptance_rate;
+  a.accepted = io->is_accepted;
+  a.divergent = io->is_divergent;
+  a.prop_weight = io->weight;
+  a.prop_q = io->proposal_position;
+  a.prop_logp = io->proposal_logdensity;
+  a.prop_g = io->proposal_logdensity_grad;
+  a.prop_metric = io->proposal_metric;
+  a.prop_gamma = io->proposal_gamma;
+  a.mmala_exact = reference_faithful ? 0 : 1;
+  hipError_t e = gjx::launch_logistic_mmala_f64(a, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "gjx_mmala_fisher_logistic_step launch");
+  return GJX_OK;
+}
+
+int gjx_mmala_fisher_logistic_sample(const gjx_logistic_params* params, const gjx_logistic_data* data,
+                                     int reference_faithful, const gjx_mmala_sample_io* io, void* stream) {
+  if (int rc0 = require_f64(params, "gjx_mmala_fisher_logistic_sample")) return rc0;
+  gjx::LogisticArgs a;
+  int rc = check_logistic(params, data, true, &a, gjx::kLogisticMaxDLmc);
+  if (rc != GJX_OK) return rc;
+  GJX_REQUIRE(params->step_size > 0.0, "step_size must be > 0 (the proposal covariance is 2 step_size G^-1)");
+  GJX_REQUIRE(io != nullptr, "io is NULL");
+  GJX_REQUIRE(io->position && io->logdensity, "position and logdensity are required");
+  GJX_REQUIRE(io->final_position && io->final_logdensity, "final_position and final_logdensity are required");
+  GJX_REQUIRE((io->z == nullptr) == (io->u == nullptr), "z and u must both be given or both be NULL");
+  a.q_in = io->position;
+  a.logp_in = io->logdensity;
+  a.z = io->z;
+  a.u = io->u;
+  a.final_q = io->final_position;
+  a.final_logp = io->final_logdensity;
+  a.final_g = io->final_logdensity_grad;
+  a.metric_out = io->final_metric;
+  a.gamma_out = io->final_gamma;
+  a.samples = io->samples;
+  a.sample_acc = io->acceptance_rate;
+  a.num_accepted = io->num_accepted;
+  a.mmala_exact = reference_faithful ? 0 : 1;
+  hipError_t e = gjx::launch_logistic_mmala_f64(a, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "gjx_mmala_fisher_logistic_sample launch");
+  return GJX_OK;
+}
+
 int gjx_logistic_value_and_grad(const gjx_logistic_params* params, const gjx_logistic_data* data,
                                 const void* position, double* logdensity, void* logdensity_grad, void* stream) {
   if (int rc0 = require_f64(params, "gjx_logistic_value_and_grad")) return rc0;

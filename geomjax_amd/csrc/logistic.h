@@ -76,6 +76,14 @@ struct LogisticArgs {
   const float* packed;
   float* work;
   int64_t work_rows;
+  // manifold MALA: metrics [C, Dmax, Dmax] and Gamma vectors [C, Dmax] of the
+  // returned state and of the proposal; mmala_exact selects +log alpha as the
+  // proposal weight (0: the reference's pairing, -log alpha)
+  double* metric_out;
+  double* gamma_out;
+  double* prop_metric;
+  double* prop_gamma;
+  int mmala_exact;
 };
 
 // Coordinates [D, Dmax) of chain c's rows (a posterior narrower than the
@@ -122,6 +130,11 @@ struct TangentOut {
 };
 
 hipError_t launch_logistic_rmhmc_f64(const LogisticArgs& a, hipStream_t stream);
+// manifold MALA (Fisher metric, float64, D <= 64): one transition (q_out set) or
+// T transitions (final_q set); the init kernel fills logp_out, g_out,
+// metric_out and gamma_out at q_in
+hipError_t launch_logistic_mmala_f64(const LogisticArgs& a, hipStream_t stream);
+hipError_t launch_logistic_mmala_init(const LogisticArgs& a, hipStream_t stream);
 hipError_t launch_logistic_rmhmc_tangent(const LogisticArgs& a, const TangentOut& o, hipStream_t stream);
 // NUTS with the RMHMC integrator, float64, D <= 64 (logistic_nuts.hip)
 hipError_t launch_logistic_nuts_f64(const LogisticArgs& a, const NutsArgs& n, hipStream_t stream);

@@ -401,6 +401,69 @@ int gjx_lmc_fisher_logistic_step(const gjx_logistic_params* params, const gjx_lo
 int gjx_lmc_fisher_logistic_sample(const gjx_logistic_params* params, const gjx_logistic_data* data,
                                    const gjx_lmcmc_sample_io* io, void* stream);
 
+/* Buffers of one manifold-MALA transition.  Input DiffusionState(position,
+ * logdensity, logdensity_grad, metric, Gamma) (geomjax/lmcmc/diffusions.py:27-32):
+ * only position and logdensity are read (the metric, its inverse and Gamma
+ * are functions of the position and are recomputed), logdensity_grad is copied
+ * through on rejection.  Outputs the new DiffusionState and MMALAInfo
+ * (geomjax/lmcmc/mmala.py:32-50: acceptance_rate, is_accepted, proposal,
+ * is_divergent, weight).  Metrics are [C, dim, dim] blocks (zeros past the
+ * posterior's D).  Optional outputs may be NULL. */
+typedef struct {
+  const void* position;             /* [C, dim] */
+  const double* logdensity;         /* [C] */
+  const void* logdensity_grad;      /* [C, dim] optional (else recomputed for the rejected case) */
+  const void* z;                    /* [C, dim] standard normals, or NULL: device PRNG */
+  const double* u;                  /* [C], or NULL */
+  void* new_position;               /* [C, dim] */
+  double* new_logdensity;           /* [C] */
+  void* new_logdensity_grad;        /* [C, dim] optional */
+  double* new_metric;               /* [C, dim, dim] optional */
+  double* new_gamma;                /* [C, dim] optional */
+  double* acceptance_rate;          /* [C] optional */
+  uint8_t* is_accepted;             /* [C] optional */
+  uint8_t* is_divergent;            /* [C] optional */
+  double* weight;                   /* [C] optional */
+  void* proposal_position;          /* [C, dim] optional */
+  double* proposal_logdensity;      /* [C] optional */
+  void* proposal_logdensity_grad;   /* [C, dim] optional */
+  double* proposal_metric;          /* [C, dim, dim] optional */
+  double* proposal_gamma;           /* [C, dim] optional */
+} gjx_mmala_step_io;
+
+typedef struct {
+  const void* position;             /* [C, dim] */
+  const double* logdensity;         /* [C] */
+  const void* z;                    /* [T, C, dim] or NULL */
+  const double* u;                  /* [T, C] or NULL */
+  void* final_position;             /* [C, dim] */
+  double* final_logdensity;         /* [C] */
+  void* final_logdensity_grad;      /* [C, dim] optional */
+  double* final_metric;             /* [C, dim, dim] optional */
+  double* final_gamma;              /* [C, dim] optional */
+  void* samples;                    /* [T, C, dim] optional */
+  float* acceptance_rate;           /* [T, C] optional */
+  int32_t* num_accepted;            /* [C] optional */
+} gjx_mmala_sample_io;
+
+/* init of manifold MALA (geomjax/lmcmc/mmala.py:53-60): log density, gradient,
+ * Fisher metric and Gamma_i = sum_j d(G^-1)_ij/dx_j (geomjax/lmcmc/diffusions.py:83-94)
+ * at C positions (float64, D <= 64).  metric and gamma may be NULL. */
+int gjx_mmala_fisher_logistic_init(const gjx_logistic_params* params, const gjx_logistic_data* data,
+                                   const void* position, double* logdensity, void* logdensity_grad, double* metric,
+                                   double* gamma, void* stream);
+
+/* One manifold-MALA transition per chain with metric_fn = Fisher metric
+ * (geomjax/lmcmc/mmala.py:63-148, geomjax/lmcmc/diffusions.py:35-80; float64,
+ * D <= 64).  reference_faithful != 0 keeps the reference's energy pairing
+ * (proposal weight = -log of the Metropolis-Hastings ratio); 0 uses +log. */
+int gjx_mmala_fisher_logistic_step(const gjx_logistic_params* params, const gjx_logistic_data* data,
+                                   int reference_faithful, const gjx_mmala_step_io* io, void* stream);
+
+/* T manifold-MALA transitions per chain in one launch. */
+int gjx_mmala_fisher_logistic_sample(const gjx_logistic_params* params, const gjx_logistic_data* data,
+                                     int reference_faithful, const gjx_mmala_sample_io* io, void* stream);
+
 /* value_and_grad of the logistic log density for C positions (init). */
 int gjx_logistic_value_and_grad(const gjx_logistic_params* params, const gjx_logistic_data* data,
                                 const void* position, double* logdensity, void* logdensity_grad, void* stream);

@@ -28,6 +28,7 @@ mcmc          proposal / accept tail, static integration, Euclidean HMC
 rmhmc         implicit midpoint RMHMC (geomjax/rmhmc/*)
 lmcmonge      Lagrangian Monte Carlo with the Monge metric (geomjax/lmcmonge/*)
 lmcmc         Lagrangian Monte Carlo with a general metric (geomjax/lmcmc/*)
+mmala         manifold MALA with a dense metric (geomjax/lmcmc/mmala.py, diffusions.py)
 diagnostics   ESS and R-hat in numpy (geomjax/diagnostics.py)
 
 How each function is pinned (checks that do not just retype its formula) is
