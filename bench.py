@@ -603,6 +603,39 @@ def logistic_lmc_forms(K, W, dev, chains=1024, L=10):
     return res
 
 
+def mmala_forms(K, W, dev, chains=1024, step_size=0.3):
+    """Manifold MALA (Fisher metric, float64, corrected Metropolis test) on the
+    headline posterior (logistic 2048 x 64): K transitions per chain in one
+    persistent launch after W warm-up transitions, device PRNG.  Reports
+    transitions/s, min-ESS/s and the acceptance rate."""
+    import torch
+
+    import geomjax_amd as gj
+    from geomjax_amd import random as gr
+    from geomjax_amd.synthetic import make_logistic
+
+    X, y, _ = make_logistic(N_ROWS, D_LOG, DATA_SEED)
+    model = gj.LogisticRegression(X, y, PRIOR, device=dev)
+    alg = gj.mmala(model, step_size, gj.FisherMetric(), reference_faithful=False)
+    q = 0.1 * torch.randn(chains, D_LOG, generator=torch.Generator().manual_seed(INIT_SEED), dtype=torch.float64)
+    s = alg.init(q.to(dev))
+    if W:
+        s = alg.sample_fn(gr.PRNGKey(INIT_SEED + 17, 0), s, W, store_samples=False).state
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = alg.sample_fn(gr.PRNGKey(INIT_SEED + 17, W), s, K)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ess, _ = min_ess(res.samples)
+    out = {"value": chains * K / wall, "unit": "transitions/s", "min_ess_per_sec": ess / wall, "min_ess": ess,
+           "acceptance_rate": float(res.acceptance_rate.double().mean()), "ms_per_transition": wall / K * 1e3,
+           "chains": chains, "step_size": step_size, "dtype": "f64", "steps": K,
+           "kernel": "logistic_mmala_f64_kernel<false>"}
+    log(f"MMALA (Fisher): {out['value']:.3e} transitions/s, acceptance {out['acceptance_rate']:.3f}, "
+        f"min-ESS/s {out['min_ess_per_sec']:.3e}")
+    return out
+
+
 def lmc_alpha2_forms(dev, repeats=5):
     """Cost of the LMC-Monge alpha2 tangent (float64, the derivative ChEES-lmcmonge
     takes every iteration) against one float64 LMC-Monge transition on the funnel
@@ -692,7 +725,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the 1024-chain and funnel comparisons")
-    ap.add_argument("--only", choices=["funnel_nuts", "logistic_nuts", "logistic_nuts_lmc", "lmc_alpha2", "logistic_lmc"],
+    ap.add_argument("--only", choices=["funnel_nuts", "logistic_nuts", "logistic_nuts_lmc", "lmc_alpha2", "logistic_lmc", "mmala"],
                     help="run one secondary workload alone and print its JSON (for profiling it)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 256 MiB three times (reference kernel for the FETCH_SIZE / WRITE_SIZE counters)")
@@ -724,6 +757,9 @@ def main():
     if args.only == "logistic_nuts_lmc":
         print(json.dumps(logistic_nuts_forms(K, W, dev, integrator="lmc")), flush=True)
         return
+    if args.only == "mmala":
+        print(json.dumps(mmala_forms(K, W, dev)), flush=True)
+        return
     if args.only == "logistic_lmc":
         print(json.dumps(logistic_lmc_forms(K, W, dev)), flush=True)
         return
@@ -752,6 +788,7 @@ def main():
         extra["logistic_nuts_1024_chains"] = logistic_nuts_forms(10, 2, dev)
         extra["logistic_lmc_1024_chains"] = logistic_lmc_forms(20, 5, dev)
         extra["logistic_nuts_lmc_1024_chains"] = logistic_nuts_forms(10, 2, dev, integrator="lmc")
+        extra["mmala_1024_chains"] = mmala_forms(200, 100, dev)
         extra["funnel_lmcmonge_d100_8192_chains"] = funnel_lmc_forms(300, 100, dev, plain=not args.no_plain)
         extra["tangent"] = tangent_forms(dev)
         extra["lmc_alpha2_tangent"] = lmc_alpha2_forms(dev)

@@ -118,6 +118,18 @@ def test_lmc_family_and_lmcmonge_chees(logistic):
     assert tuned is not None
 
 
+def test_mmala(logistic):
+    print("case mmala step + sample_fn", flush=True)
+    model, D = logistic
+    mm = gj.mmala(model, 0.3, gj.FisherMetric(), reference_faithful=False)
+    mstate = mm.init(torch.zeros(64, D, dtype=torch.float64, device=DEV))
+    mstate, minfo = mm.step(gr.key(9), mstate)
+    mres = mm.sample_fn(gr.key(10), mstate, 5)
+    torch.cuda.synchronize()
+    assert mstate.metric.shape == (64, D, D) and minfo.proposal.Gamma.shape == (64, D)
+    _finite(mstate.position, mres.samples, mres.state.metric, mres.state.Gamma)
+
+
 def test_tangents_and_adapt(logistic):
     print("case esjd_acceptance + adapt", flush=True)
     from geomjax_amd.adjoint import adapt, esjd_acceptance
