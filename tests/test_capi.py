@@ -120,3 +120,38 @@ def test_logistic_feature_capacity_per_path():
     assert step(rm, rio, _lib.GJX_F32, 129) == -2
     assert step(lmc, lio, _lib.GJX_F64, 64) == -1
     assert step(lmc, lio, _lib.GJX_F64, 65) == -2
+
+
+def test_mmala_and_logistic_nuts_entry_points_validate_on_the_host():
+    """MMALA and the logistic NUTS entry points: float64 only, D <= 64, a
+    positive step size for MMALA, required buffers, z and u given together."""
+    lib = _lib.load()
+
+    def params(D, dtype=_lib.GJX_F64, step=0.1):
+        return _lib.LogisticParams(num_chains=4, dim=D, dtype=dtype, num_integration_steps=0, max_iters=0,
+                                   num_transitions=3, step_size=step, prior_var=100.0)
+
+    def data(D):
+        return _lib.LogisticData(num_posteriors=1, max_rows=16, max_features=D, num_rows=1, num_features=1,
+                                 x_offset=1, y_offset=1, x=1, y=1)
+
+    mstep, msample = lib.gjx_mmala_fisher_logistic_step, lib.gjx_mmala_fisher_logistic_sample
+    assert mstep(params(8, _lib.GJX_F32), data(8), 1, _lib.MmalaStepIO(), None) == -2
+    assert "float32" in _lib.last_error()
+    assert mstep(params(65), data(65), 1, _lib.MmalaStepIO(), None) == -2
+    assert mstep(params(8, step=0.0), data(8), 1, _lib.MmalaStepIO(), None) == -1
+    assert "step_size" in _lib.last_error()
+    assert mstep(params(8), data(8), 1, _lib.MmalaStepIO(), None) == -1
+    assert "position" in _lib.last_error()
+    io = _lib.MmalaStepIO(position=1, logdensity=1, new_position=1, new_logdensity=1, z=1)
+    assert mstep(params(8), data(8), 0, io, None) == -1
+    assert "z and u" in _lib.last_error()
+    sio = _lib.MmalaSampleIO(position=1, logdensity=1)
+    assert msample(params(8), data(8), 1, sio, None) == -1
+    assert "final_position" in _lib.last_error()
+    assert lib.gjx_mmala_fisher_logistic_init(params(65), data(65), 1, 1, 1, None, None, None) == -2
+    assert lib.gjx_mmala_fisher_logistic_init(params(8), data(8), 1, None, 1, None, None, None) == -1
+    for fn, io_t in ((lib.gjx_rmhmc_fisher_logistic_nuts, _lib.NutsIO), (lib.gjx_lmc_fisher_logistic_nuts,
+                                                                        _lib.LmcNutsIO)):
+        assert fn(params(65), data(65), 5, _lib.GJX_UTURN["euc"], io_t(), None) == -2
+        assert fn(params(8), data(8), 5, 7, io_t(), None) == -1  # unknown criterion
