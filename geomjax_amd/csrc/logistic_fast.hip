@@ -482,8 +482,11 @@ __device__ __forceinline__ void field_pass(Smem& S, const float* __restrict__ Xr
   for (int kc = 0; kc < KC; ++kc)
 #pragma unroll
     for (int i = 0; i < 8; ++i) dacc[kc][i] = 0.0f;
-  const float* xp = Xr + (size_t)l * 8;
-  const uint4* sp = reinterpret_cast<const uint4*>(Xs) + (size_t)l * 2;
+  // uniform bases with 32-bit lane offsets: the loads address base + offset
+  // instead of holding a 64-bit pointer per fragment in registers
+  const uint32_t xoff = (uint32_t)l * 8u;
+  const uint4* const sbase = reinterpret_cast<const uint4*>(Xs);
+  const uint32_t soff = (uint32_t)l * 2u;
   const int nrb = npad / 32;  // even: npad is a multiple of 64
   // One 32-row block: its MFMA operand (X split into bf16 hi/lo), and the
   // float32 X fragment, eta, base and label of the lane's row with the block's
@@ -498,23 +501,28 @@ __device__ __forceinline__ void field_pass(Smem& S, const float* __restrict__ Xr
   };
   // the pipeline's tail re-reads the last block (in bounds; its results are unused)
   auto clamp_rb = [&](int rb) { return rb < nrb ? rb : nrb - 1; };
+  // The block index is made opaque (readfirstlane) so that the block's address
+  // is formed as a uniform base plus the lane's 32-bit offset, not as one
+  // 64-bit per-lane pointer per fragment stream carried across iterations.
   auto fetch_xf = [&](XF& f, int rb) {
-    rb = clamp_rb(rb);
+    rb = __builtin_amdgcn_readfirstlane(clamp_rb(rb));
+    const uint4* const blk = sbase + (size_t)rb * (KC * 128);
 #pragma unroll
     for (int kc = 0; kc < KC; ++kc) {
-      const uint4* q = sp + (size_t)(rb * KC + kc) * 128;
-      f.xh[kc] = __builtin_bit_cast(bf16x8, q[0]);
-      f.xl[kc] = __builtin_bit_cast(bf16x8, q[1]);
+      const uint32_t i = (uint32_t)kc * 128u + soff;
+      f.xh[kc] = __builtin_bit_cast(bf16x8, blk[i]);
+      f.xl[kc] = __builtin_bit_cast(bf16x8, blk[i + 1u]);
     }
   };
   auto fetch_blk = [&](Blk& B, int rb) {
-    rb = clamp_rb(rb);
+    rb = __builtin_amdgcn_readfirstlane(clamp_rb(rb));
+    const float* const blk = Xr + (size_t)rb * (KC * 512);
 #pragma unroll
-    for (int kc = 0; kc < KC; ++kc) load8(xp + (size_t)(rb * KC + kc) * 512, B.x[kc]);
-    const int row = rb * 32 + (l % 32);
-    B.eta = E1[row];
-    B.base = explicit_step ? 0.0f : E0[row];
-    B.yv = yf[row];
+    for (int kc = 0; kc < KC; ++kc) load8(blk + ((uint32_t)kc * 512u + xoff), B.x[kc]);
+    const uint32_t lr = (uint32_t)(l % 32);
+    B.eta = (E1 + (size_t)rb * 32)[lr];
+    B.base = explicit_step ? 0.0f : (E0 + (size_t)rb * 32)[lr];
+    B.yv = (yf + (size_t)rb * 32)[lr];
   };
   auto mfma_blk = [&](const XF& f, Blk& B) {
 #pragma unroll
