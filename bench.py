@@ -725,7 +725,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the 1024-chain and funnel comparisons")
-    ap.add_argument("--only", choices=["funnel_nuts", "logistic_nuts", "logistic_nuts_lmc", "lmc_alpha2", "logistic_lmc", "mmala"],
+    ap.add_argument("--only", choices=["funnel_nuts", "logistic_nuts", "logistic_nuts_lmc", "lmc_alpha2", "logistic_lmc", "mmala",
+                             "uneven"],
                     help="run one secondary workload alone and print its JSON (for profiling it)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 256 MiB three times (reference kernel for the FETCH_SIZE / WRITE_SIZE counters)")
@@ -756,6 +757,9 @@ def main():
         return
     if args.only == "logistic_nuts_lmc":
         print(json.dumps(logistic_nuts_forms(K, W, dev, integrator="lmc")), flush=True)
+        return
+    if args.only == "uneven":
+        print(json.dumps(uneven_batch_forms(32, K, W, dev, plain=not args.no_plain)), flush=True)
         return
     if args.only == "mmala":
         print(json.dumps(mmala_forms(K, W, dev)), flush=True)

@@ -44,6 +44,8 @@
 // sum (float64) to log det G.  At pivot k the not-yet-swept column k is the
 // Schur complement, so the Cholesky factor column L[:, k] = A[:, k] / sqrt(d_k)
 // is available for the momentum draw p0 = L z at no extra cost.
+#include <mutex>
+
 #include "common.h"
 #include "logistic.h"
 #include "logistic_f32.h"
@@ -963,6 +965,33 @@ hipError_t launch_logistic_pack(const double* X, const double* y, const int32_t*
   return hipGetLastError();
 }
 
+namespace {
+
+// Two side streams per device (default priority, created on first use and kept
+// for the process) on which the per-width launches of a batch run concurrently.
+struct SideStreams {
+  hipStream_t s[2];
+  bool ok;
+};
+
+SideStreams* side_streams() {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static SideStreams cache[kMaxDevices];
+  static bool tried[kMaxDevices];
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!tried[dev]) {
+    tried[dev] = true;
+    cache[dev].ok = hipStreamCreateWithFlags(&cache[dev].s[0], hipStreamNonBlocking) == hipSuccess &&
+                    hipStreamCreateWithFlags(&cache[dev].s[1], hipStreamNonBlocking) == hipSuccess;
+  }
+  return cache[dev].ok ? &cache[dev] : nullptr;
+}
+
+}  // namespace
+
 hipError_t launch_logistic_rmhmc_f32(const LogisticArgs& a, int max_features, hipStream_t stream) {
   const int blocks = (a.C + 7) / 8 * 8;
   // one launch per packed width present: a single posterior needs only its
@@ -971,19 +1000,58 @@ hipError_t launch_logistic_rmhmc_f32(const LogisticArgs& a, int max_features, hi
   const bool nb1 = max_features <= 32 || a.P > 1;
   const bool nb2 = max_features > 32 && (max_features <= 64 || a.P > 1);
   const bool nb4 = max_features > 64;
-  if (a.q_out != nullptr) {
-    if (nb1) hipLaunchKernelGGL((lfast::logistic_rmhmc_f32_kernel<1, true>), dim3(blocks), dim3(64), 0, stream, a);
-    if (nb2) hipLaunchKernelGGL((lfast::logistic_rmhmc_f32_kernel<2, true>), dim3(blocks), dim3(64), 0, stream, a);
-  } else {
-    if (nb1) hipLaunchKernelGGL((lfast::logistic_rmhmc_f32_kernel<1, false>), dim3(blocks), dim3(64), 0, stream, a);
-    if (nb2) hipLaunchKernelGGL((lfast::logistic_rmhmc_f32_kernel<2, false>), dim3(blocks), dim3(64), 0, stream, a);
+  const bool step = a.q_out != nullptr;
+  auto launch = [&](int which, hipStream_t st) -> hipError_t {
+    if (which == 0) {
+      if (step)
+        hipLaunchKernelGGL((lfast::logistic_rmhmc_f32_kernel<1, true>), dim3(blocks), dim3(64), 0, st, a);
+      else
+        hipLaunchKernelGGL((lfast::logistic_rmhmc_f32_kernel<1, false>), dim3(blocks), dim3(64), 0, st, a);
+      return hipGetLastError();
+    }
+    if (which == 1) {
+      if (step)
+        hipLaunchKernelGGL((lfast::logistic_rmhmc_f32_kernel<2, true>), dim3(blocks), dim3(64), 0, st, a);
+      else
+        hipLaunchKernelGGL((lfast::logistic_rmhmc_f32_kernel<2, false>), dim3(blocks), dim3(64), 0, st, a);
+      return hipGetLastError();
+    }
+    return launch_logistic_rmhmc_f32_wide(a, st);
+  };
+  int widths[3], n = 0;
+  if (nb4) widths[n++] = 2;  // the widest first: it holds most of a mixed batch's work
+  if (nb2) widths[n++] = 1;
+  if (nb1) widths[n++] = 0;
+  // The launches touch disjoint chains.  In a batch with several widths they
+  // run concurrently: side streams fork from the caller's stream at an event
+  // and the caller's stream waits for both before anything after this call
+  // (the pattern also holds under stream capture).  The tail of one width's
+  // launch then overlaps the others.
+  SideStreams* ss = n > 1 ? side_streams() : nullptr;
+  if (ss == nullptr) {
+    for (int i = 0; i < n; ++i) {
+      const hipError_t e = launch(widths[i], stream);
+      if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
   }
-  if (nb4) {
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    return launch_logistic_rmhmc_f32_wide(a, stream);
+  hipEvent_t fork = nullptr, join[2] = {nullptr, nullptr};
+  hipError_t e = hipEventCreateWithFlags(&fork, hipEventDisableTiming);
+  for (int i = 0; e == hipSuccess && i < n - 1; ++i) e = hipEventCreateWithFlags(&join[i], hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventRecord(fork, stream);
+  for (int i = 1; e == hipSuccess && i < n; ++i) e = hipStreamWaitEvent(ss->s[i - 1], fork, 0);
+  if (e == hipSuccess) e = launch(widths[0], stream);
+  for (int i = 1; e == hipSuccess && i < n; ++i) {
+    e = launch(widths[i], ss->s[i - 1]);
+    if (e == hipSuccess) e = hipEventRecord(join[i - 1], ss->s[i - 1]);
+    if (e == hipSuccess) e = hipStreamWaitEvent(stream, join[i - 1], 0);
   }
-  return hipGetLastError();
+  // events may be destroyed once recorded and waited on (their resources go
+  // when they complete)
+  if (fork) (void)hipEventDestroy(fork);
+  for (int i = 0; i < 2; ++i)
+    if (join[i]) (void)hipEventDestroy(join[i]);
+  return e;
 }
 
 }  // namespace gjx

@@ -39,10 +39,22 @@ def _cases():
                                  256, 20, torch.float32),
         "logistic_f64": lambda: (gj.rmhmc(gj.LogisticRegression(X, y, 100.0), 0.1, gj.FisherMetric(), 6,
                                           integrator=integ), 16, 40, torch.float64),
+        # a batch of three widths: its launches fork onto side streams and join
+        "logistic_batch_f32": lambda: (gj.rmhmc(_batch(), 0.05, gj.FisherMetric(), 4, integrator=integ), 24, 100,
+                                       torch.float32),
     }
 
 
-@pytest.mark.parametrize("name", ["logistic_f32", "logistic_wide_f32", "funnel_f32", "lmcmonge_f32", "logistic_f64"])
+def _batch():
+    models = []
+    for N, D, seed in ((200, 20, 13), (300, 40, 14), (250, 100, 15)):
+        Xb, yb, _ = make_logistic(N, D, seed)
+        models.append(gj.LogisticRegression(Xb, yb, 100.0))
+    return gj.LogisticBatch(models, 8)
+
+
+@pytest.mark.parametrize("name", ["logistic_f32", "logistic_wide_f32", "funnel_f32", "lmcmonge_f32", "logistic_f64",
+                                  "logistic_batch_f32"])
 def test_graph_replays_match_eager(name):
     alg, C, D, dt = _cases()[name]()
     T, R = 3, 3
