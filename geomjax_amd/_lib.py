@@ -139,7 +139,8 @@ class LogisticData(ctypes.Structure):
         ("max_features", _c_int32),
         ("reserved", _c_int32),
     ] + [(n, _vp) for n in ("num_rows", "num_features", "x_offset", "y_offset", "chain_posterior", "x", "y",
-                            "packed", "workspace")] + [("workspace_bytes", ctypes.c_uint64)]
+                            "packed", "workspace")] + [("workspace_bytes", ctypes.c_uint64),
+                                                       ("dispatch_order", _vp)]
 
 
 class TangentIO(ctypes.Structure):

@@ -482,6 +482,7 @@ int check_logistic(const gjx_logistic_params* p, const gjx_logistic_data* d, boo
   a->seed = p->seed;
   a->counter = p->counter;
   a->chain_post = d->chain_posterior;
+  a->dispatch_order = d->dispatch_order;
   a->post_N = d->num_rows;
   a->post_D = d->num_features;
   a->post_xoff = d->x_offset;

@@ -76,6 +76,7 @@ struct LogisticArgs {
   const float* packed;
   float* work;
   int64_t work_rows;
+  const int32_t* dispatch_order;  // [C] chain of each dispatch slot (tuned kernels) or null
   // manifold MALA: metrics [C, Dmax, Dmax] and Gamma vectors [C, Dmax] of the
   // returned state and of the proposal; mmala_exact selects +log alpha as the
   // proposal weight (0: the reference's pairing, -log alpha)

@@ -892,8 +892,10 @@ template <int NB, bool STEP>
 __global__ void __launch_bounds__(64) logistic_rmhmc_f32_kernel(const LogisticArgs a) {
   __shared__ Smem S;
   const int per_xcd = gridDim.x / 8;
-  const int c = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
-  if (c >= a.C) return;
+  const int slot = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+  if (slot >= a.C) return;
+  const int c = a.dispatch_order ? a.dispatch_order[slot] : slot;
+  if (c < 0 || c >= a.C) return;  // not a permutation entry: nothing runs in this slot
   const int post = a.chain_post ? a.chain_post[c] : 0;
   if (post < 0 || post >= a.P) return;  // invalid posterior index: chain left untouched
   if (a.packed_hdr[post].nb != NB) return;

@@ -175,6 +175,7 @@ class _posterior_table:
         self._packed = None     # float32 fragment table (built on first float32 use)
         self._workspace = None  # per-chain scratch of the float32 path
         self._retired = []      # outgrown scratch buffers: a captured HIP graph may still point at them
+        self._orders = {}       # num_chains -> dispatch order of the float32 kernels
 
     def _base(self, chain_posterior):
         return _lib.LogisticData(num_posteriors=self.P, max_rows=self.max_rows, max_features=self.max_features,
@@ -206,6 +207,52 @@ class _posterior_table:
             self._workspace = torch.empty(need, dtype=torch.uint8, device=self.x.device)
         return self._workspace
 
+    def dispatch_order(self, chain_posterior, num_chains):
+        """Order in which the float32 kernels start the chains of a batch.
+
+        The kernels deal dispatch slots to the 8 XCDs in contiguous ranges of
+        ceil(C/8) slots.  Posteriors are taken by decreasing cost per chain
+        (N times the square of the padded width) and each goes whole to the XCD
+        with the least cost so far that has room (spilling into the next when
+        none has room for all its chains); inside an XCD the chains keep that
+        order, so the costliest start first and the cheap ones fill the launch's
+        tail, and a posterior's X stays in few L2 caches.  None for a single
+        posterior (every chain costs the same)."""
+        if chain_posterior is None or self.P == 1:
+            return None
+        C = int(num_chains)
+        if C in self._orders:
+            return self._orders[C]
+        cp = chain_posterior.detach().to("cpu", torch.int64)
+        if cp.numel() != C:
+            return None
+        pad = [32 if d <= 32 else (64 if d <= 64 else 128) for d in self._host_D]
+        cost = [n * w * w for n, w in zip(self._host_N, pad)]
+        per = (C + 7) // 8
+        cap = [max(0, min(per, C - x * per)) for x in range(8)]
+        lists = [[] for _ in range(8)]
+        load = [0.0] * 8
+        chains_of = [[] for _ in range(self.P)]
+        for c, p in enumerate(cp.tolist()):
+            if 0 <= p < self.P:
+                chains_of[p].append(c)
+        for p in sorted(range(self.P), key=lambda q: -cost[q]):
+            rest = chains_of[p]
+            while rest:
+                room = [x for x in range(8) if len(lists[x]) < cap[x]]
+                fits = [x for x in room if cap[x] - len(lists[x]) >= len(rest)]
+                x = min(fits or room, key=lambda k: load[k])
+                take = rest[:cap[x] - len(lists[x])]
+                rest = rest[len(take):]
+                lists[x].extend(take)
+                load[x] += cost[p] * len(take)
+        order = [c for x in range(8) for c in lists[x]]
+        placed = set(order)
+        order += [c for c in range(C) if c not in placed]  # chains of an invalid posterior index
+        t = torch.tensor(order, dtype=torch.int32, device=self.x.device)
+        self._orders[C] = t
+        return t
+
     def struct(self, chain_posterior, dtype=torch.float64, num_chains=None):
         st = self._base(chain_posterior)
         if dtype == torch.float32:
@@ -215,6 +262,7 @@ class _posterior_table:
             st.packed = self.packed().data_ptr()
             st.workspace = ws.data_ptr()
             st.workspace_bytes = ws.numel()
+            st.dispatch_order = _lib.ptr(self.dispatch_order(chain_posterior, num_chains))
         return st
 
 

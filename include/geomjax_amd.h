@@ -305,6 +305,11 @@ typedef struct {
   const void* packed;            /* the table built by gjx_logistic_pack */
   void* workspace;               /* per-chain scratch, gjx_logistic_workspace_size bytes */
   uint64_t workspace_bytes;
+  /* GJX_F32 only, optional: [C] device permutation of the chains giving the
+   * order in which the tuned kernels start them (dispatch slot s runs chain
+   * dispatch_order[s]; slots are dealt to the 8 XCDs in ranges of
+   * ceil(C/8) slots).  NULL: chain s in slot s.  Results do not depend on it. */
+  const int32_t* dispatch_order;
 } gjx_logistic_data;
 
 /* Tuned RMHMC path (params->dtype == GJX_F32).  State arrays (positions,
