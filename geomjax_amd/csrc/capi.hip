@@ -11,6 +11,7 @@
 #include "../../include/geomjax_amd.h"
 #include "funnel_lmc.h"
 #include "funnel_rmhmc.h"
+#include "dense_metric.h"
 #include "logistic.h"
 
 namespace {
@@ -748,6 +749,56 @@ int gjx_lmc_fisher_logistic_sample(const gjx_logistic_params* params, const gjx_
   a.num_accepted = io->num_accepted;
   hipError_t e = gjx::launch_logistic_lmc_f64(a, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "gjx_lmc_fisher_logistic_sample launch");
+  return GJX_OK;
+}
+
+static int dense_args(const gjx_dense_metric_io* io, gjx::DenseMetricArgs* a, const char* fn) {
+  GJX_REQUIRE(io != nullptr, "%s: io is NULL", fn);
+  GJX_REQUIRE(io->num_chains > 0, "%s: num_chains must be > 0 (got %d)", fn, io->num_chains);
+  GJX_REQUIRE(io->dim >= 1, "%s: dim must be >= 1 (got %d)", fn, io->dim);
+  if (io->dim > gjx::kDenseMaxD)
+    return fail(GJX_EUNSUPPORTED, "%s: dim %d exceeds the dense-metric kernels' maximum of %d", fn, io->dim,
+                gjx::kDenseMaxD);
+  GJX_REQUIRE(io->metric && io->metric_jacobian, "%s: metric and metric_jacobian are required", fn);
+  std::memset(a, 0, sizeof(*a));
+  a->C = io->num_chains;
+  a->D = io->dim;
+  a->step_size = io->step_size;
+  a->metric = io->metric;
+  a->metric_jac = io->metric_jacobian;
+  a->z = io->z;
+  a->velocity = io->velocity;
+  a->logdensity_grad = io->logdensity_grad;
+  a->grad_logdet_in = io->grad_logdet_in;
+  a->velocity_out = io->velocity_out;
+  a->logdet = io->logdet;
+  a->grad_logdet = io->grad_logdet;
+  a->vgv = io->vgv;
+  a->volume_adjustment = io->volume_adjustment;
+  return GJX_OK;
+}
+
+int gjx_dense_metric_position(const gjx_dense_metric_io* io, void* stream) {
+  gjx::DenseMetricArgs a;
+  int rc = dense_args(io, &a, "gjx_dense_metric_position");
+  if (rc != GJX_OK) return rc;
+  GJX_REQUIRE(!io->velocity_out || io->z, "velocity_out (L^-T z) needs z");
+  GJX_REQUIRE(!io->vgv || io->velocity, "vgv needs velocity");
+  hipError_t e = gjx::launch_dense_metric_position(a, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "gjx_dense_metric_position launch");
+  return GJX_OK;
+}
+
+int gjx_dense_lmc_half_step(const gjx_dense_metric_io* io, void* stream) {
+  gjx::DenseMetricArgs a;
+  int rc = dense_args(io, &a, "gjx_dense_lmc_half_step");
+  if (rc != GJX_OK) return rc;
+  GJX_REQUIRE(std::isfinite(io->step_size), "step_size must be finite");
+  GJX_REQUIRE(io->velocity && io->logdensity_grad && io->grad_logdet_in && io->velocity_out &&
+                  io->volume_adjustment,
+              "velocity, logdensity_grad, grad_logdet_in, velocity_out and volume_adjustment are required");
+  hipError_t e = gjx::launch_dense_lmc_half_step(a, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "gjx_dense_lmc_half_step launch");
   return GJX_OK;
 }
 

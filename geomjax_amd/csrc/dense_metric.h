@@ -1,0 +1,31 @@
+// Internal launch interface of the dense general-metric LMC kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+namespace gjx {
+
+constexpr int kDenseMaxD = 64;
+
+struct DenseMetricArgs {
+  int C, D;
+  double step_size;
+  const double* metric;      // [C, D, D] G(q) as the metric function returns it
+  const double* metric_jac;  // [C, D, D, D]: dG_ab/dq_i at [c][a][b][i]
+  const double* z;           // [C, D] standard normals of the velocity draw, or null
+  const double* velocity;    // [C, D] (position: v for v^T G v, or null; half step: the input velocity)
+  const double* logdensity_grad;  // [C, D] (half step)
+  const double* grad_logdet_in;   // [C, D] (half step)
+  double* velocity_out;      // [C, D] (position: L^-T z; half step: the new velocity; may alias velocity)
+  double* logdet;            // [C] log det of (G + G^T)/2 (position)
+  double* grad_logdet;       // [C, D] grad log det G (position)
+  double* vgv;               // [C] v^T G v of the given velocity (position)
+  double* volume_adjustment; // [C] in/out (half step)
+};
+
+size_t dense_metric_smem_bytes();
+hipError_t launch_dense_metric_position(const DenseMetricArgs& a, hipStream_t stream);
+hipError_t launch_dense_lmc_half_step(const DenseMetricArgs& a, hipStream_t stream);
+
+}  // namespace gjx

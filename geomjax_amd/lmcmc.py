@@ -66,8 +66,13 @@ def _check(logdensity_fn, metric_fn, position):
 
 
 def init(position, logdensity_fn):
-    """geomjax/lmcmc/lmc.py:98-101."""
+    """geomjax/lmcmc/lmc.py:98-101.  A plain callable log density (a user
+    function) is evaluated by torch.func on the device (the dense-metric path)."""
     if not isinstance(logdensity_fn, (LogisticRegression, LogisticBatch)):
+        if callable(logdensity_fn):
+            from . import lmcmc_dense
+
+            return lmcmc_dense.init(position, logdensity_fn)
         raise NotImplementedError(f"no HIP log density for {logdensity_fn!r}")
     logdensity, grad = logdensity_fn.value_and_grad(position)
     return LMCState(position, logdensity, grad, torch.zeros_like(logdensity))
@@ -93,8 +98,15 @@ def build_kernel(integrator=lan_integrator, divergence_threshold: float = 1000):
                max_num_integration_steps=None):
         """``num_integration_steps``: an int, or a [C] integer tensor of per-chain
         counts (the dynamic kernel's form), bounded by ``max_num_integration_steps``."""
+        from . import lmcmc_dense
         from .rmhmc import _steps
 
+        if lmcmc_dense.is_dense_metric_fn(logdensity_fn, metric_fn):
+            # a user metric_fn: torch.func derivatives + the dense-metric HIP kernels
+            if not isinstance(num_integration_steps, int):
+                raise NotImplementedError("per-chain step counts need a closed-form metric kernel")
+            return lmcmc_dense.transition(rng_key, state, logdensity_fn, step_size, metric_fn,
+                                          num_integration_steps, divergence_threshold)
         single = state.position.dim() == 1
         position = (state.position[None] if single else state.position).contiguous()
         _check(logdensity_fn, metric_fn, position)
@@ -148,9 +160,16 @@ def sample(rng_key, state, logdensity_fn, step_size, metric_fn, num_integration_
            divergence_threshold=1000, store_samples=True, samples_out=None, max_num_integration_steps=None):
     """``num_samples`` LMC transitions of every chain in one persistent launch
     (``num_integration_steps`` an int or a [T, C] integer tensor)."""
+    from . import lmcmc_dense
     from .rmhmc import _steps
     from .sampling import SampleResult
 
+    if lmcmc_dense.is_dense_metric_fn(logdensity_fn, metric_fn):
+        if not isinstance(num_integration_steps, int):
+            raise NotImplementedError("per-chain step counts need a closed-form metric kernel")
+        return lmcmc_dense.sample(rng_key, state, logdensity_fn, step_size, metric_fn, num_integration_steps,
+                                  num_samples, divergence_threshold=divergence_threshold,
+                                  store_samples=store_samples, samples_out=samples_out)
     position = state.position
     if position.dim() != 2:
         raise ValueError("sample() runs a batch of chains: position must be [C, D]")

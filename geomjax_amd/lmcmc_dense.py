@@ -1,0 +1,188 @@
+"""LMC with a general dense user metric (geomjax/lmcmc with any ``metric_fn``).
+
+The reference's lmcmc accepts any ``metric_fn`` and differentiates it with
+``jax.jacfwd`` (geomjax/lmcmc/metrics.py:158-189).  Here the user's functions
+(``logdensity_fn`` and ``metric_fn``, written with torch operations that
+``torch.func`` can transform, on the device of their input) are evaluated and
+differentiated by ``torch.func`` on the GPU, batched over chains; the
+integrator's linear algebra runs in the HIP kernels of ``dense_metric.hip``:
+the D^3 Christoffel contractions, the LU factorisations with partial pivoting
+of G +- (eps/2) Omega~, the solve, the log-determinants and the volume change
+(``gjx_dense_lmc_half_step``), and the Cholesky velocity draw, log det G and
+grad log det G = tr(G^-1 dG) (``gjx_dense_metric_position``).  float64,
+D <= 64.
+
+With a ``PRNGKey`` the normals and uniforms are drawn by torch's device
+generator seeded from (seed, counter); ``HostNoise(z [C, D], u [C])`` feeds
+them explicitly.
+"""
+import torch
+from torch.func import grad_and_value, jacfwd, vmap
+
+from . import _lib
+from .proposal import Proposal
+from .random import HostNoise, PRNGKey
+
+__all__ = ["is_dense_metric_fn", "init", "transition", "sample"]
+
+
+def is_dense_metric_fn(logdensity_fn, metric_fn):
+    """True for plain callables (user functions), False for this package's model
+    and metric descriptors, which have closed-form kernels."""
+    from .metrics import FisherMetric, MongeMetric
+    from .models import LogisticBatch, LogisticRegression, NealFunnel
+
+    if isinstance(metric_fn, (FisherMetric, MongeMetric)):
+        return False
+    if isinstance(logdensity_fn, (LogisticRegression, LogisticBatch, NealFunnel)):
+        return False
+    return callable(metric_fn) and callable(logdensity_fn)
+
+
+def _check(position):
+    if not position.is_cuda or position.dtype != torch.float64:
+        raise ValueError("dense-metric LMC states must be float64 tensors on a HIP device")
+    if position.dim() != 2:
+        raise ValueError("dense-metric LMC runs a batch of chains: position must be [C, D]")
+    if position.shape[1] > 64:
+        raise NotImplementedError(f"dense-metric kernels take D <= 64 (got {position.shape[1]})")
+
+
+def _value_and_grad(logdensity_fn, q):
+    g, l = vmap(grad_and_value(logdensity_fn))(q)
+    return l.contiguous(), g.contiguous()
+
+
+def _metric(metric_fn, q):
+    G = vmap(metric_fn)(q).contiguous()
+    dG = vmap(jacfwd(metric_fn))(q).contiguous()  # [C, D, D, D]: dG_ab / dq_i at [c, a, b, i]
+    return G, dG
+
+
+def _io(G, dG, **kw):
+    C, D = G.shape[0], G.shape[1]
+    ptrs = {k: _lib.ptr(v) for k, v in kw.items() if k != "step_size"}
+    return _lib.DenseMetricIO(num_chains=C, dim=D, step_size=float(kw.get("step_size", 0.0)), metric=G.data_ptr(),
+                              metric_jacobian=dG.data_ptr(), **ptrs)
+
+
+def _position(G, dG, z=None, velocity=None):
+    C, D = G.shape[0], G.shape[1]
+    dev = G.device
+    logdet = torch.empty(C, dtype=torch.float64, device=dev)
+    gld = torch.empty(C, D, dtype=torch.float64, device=dev)
+    v0 = torch.empty(C, D, dtype=torch.float64, device=dev) if z is not None else None
+    vgv = torch.empty(C, dtype=torch.float64, device=dev) if velocity is not None else None
+    io = _io(G, dG, z=z, velocity=velocity, velocity_out=v0, logdet=logdet, grad_logdet=gld, vgv=vgv)
+    _lib.check(_lib.load().gjx_dense_metric_position(io, _lib.stream_handle(dev)), "gjx_dense_metric_position")
+    return logdet, gld, v0, vgv
+
+
+def _half_step(G, dG, v, g, gld, eps, J):
+    """One half step; returns the new velocity, J updated in place."""
+    out = torch.empty_like(v)
+    io = _io(G, dG, step_size=eps, velocity=v, logdensity_grad=g, grad_logdet_in=gld, velocity_out=out,
+             volume_adjustment=J)
+    _lib.check(_lib.load().gjx_dense_lmc_half_step(io, _lib.stream_handle(v.device)), "gjx_dense_lmc_half_step")
+    return out
+
+
+def init(position, logdensity_fn):
+    """geomjax/lmcmc/lmc.py:98-101 for a user log density."""
+    from .lmcmc import LMCState
+
+    single = position.dim() == 1
+    q = (position[None] if single else position).contiguous()
+    _check(q)
+    l, g = _value_and_grad(logdensity_fn, q)
+    if single:
+        return LMCState(position, l[0], g[0], torch.zeros_like(l[0]))
+    return LMCState(position, l, g, torch.zeros_like(l))
+
+
+def _noise(rng_key, C, D, dev):
+    if isinstance(rng_key, HostNoise):
+        return (rng_key.z.reshape(C, D).to(device=dev, dtype=torch.float64).contiguous(),
+                rng_key.u.reshape(C).to(device=dev, dtype=torch.float64).contiguous())
+    if isinstance(rng_key, PRNGKey):
+        gen = torch.Generator(device=dev)
+        gen.manual_seed((int(rng_key.seed) * 0x9E3779B1 + int(rng_key.counter)) & 0x7FFFFFFFFFFFFFFF)
+        return (torch.randn(C, D, generator=gen, dtype=torch.float64, device=dev),
+                torch.rand(C, generator=gen, dtype=torch.float64, device=dev))
+    raise TypeError("rng_key must be geomjax_amd.random.HostNoise or PRNGKey")
+
+
+def transition(rng_key, state, logdensity_fn, step_size, metric_fn, num_integration_steps,
+               divergence_threshold=1000.0):
+    """One LMC transition (geomjax/lmcmc/lmc.py:135-180, 411-475) of every chain."""
+    from .lmcmc import IntegratorState, LMCInfo, LMCState
+
+    q0 = state.position.contiguous()
+    _check(q0)
+    C, D = q0.shape
+    dev = q0.device
+    L = int(num_integration_steps)
+    eps = float(step_size)
+    l0 = state.logdensity.reshape(C).to(torch.float64)
+    g0 = state.logdensity_grad.reshape(C, D).to(torch.float64).contiguous()
+    J0 = torch.as_tensor(state.volume_adjustment, dtype=torch.float64, device=dev).expand(C).clone()
+    z, u = _noise(rng_key, C, D, dev)
+
+    G, dG = _metric(metric_fn, q0)
+    logdet, gld, v0, _ = _position(G, dG, z=z)
+    _, _, _, vgv0 = _position(G, dG, velocity=v0)
+    H0 = -l0 + (-0.5 * logdet + 0.5 * vgv0) - J0
+
+    q, v, l, g, J = q0, v0, l0, g0, J0.clone()
+    for _ in range(L):
+        v = _half_step(G, dG, v, g, gld, eps, J)
+        q = q + eps * v
+        l, g = _value_and_grad(logdensity_fn, q)
+        G, dG = _metric(metric_fn, q)
+        logdet, gld, _, _ = _position(G, dG)
+        v = _half_step(G, dG, v, g, gld, eps, J)
+    _, _, _, vgv1 = _position(G, dG, velocity=v)
+    H1 = -l + (-0.5 * logdet + 0.5 * vgv1) - J
+    delta = H0 - H1
+    delta = torch.where(torch.isnan(delta), torch.full_like(delta, -float("inf")), delta)
+    p_accept = torch.clamp(torch.exp(delta), max=1.0)
+    accept = u < p_accept
+    momentum = torch.bmm(G, v[:, :, None])[:, :, 0]
+    end = IntegratorState(q, -momentum, -v, l, g, J)  # flipped (geomjax/lmcmc/lmc.py:478-499)
+    proposal = Proposal(end, H1, delta, torch.minimum(delta, torch.zeros_like(delta)))
+    a2 = accept[:, None]
+    new_state = LMCState(torch.where(a2, q, q0), torch.where(accept, l, l0), torch.where(a2, g, g0),
+                         torch.where(accept, J, J0))
+    info = LMCInfo(v0, p_accept, accept, -delta > divergence_threshold, H1, proposal, L)
+    return new_state, info
+
+
+def sample(rng_key, state, logdensity_fn, step_size, metric_fn, num_integration_steps, num_samples, *,
+           divergence_threshold=1000, store_samples=True, samples_out=None):
+    """``num_samples`` transitions (one Python-driven loop of kernel launches)."""
+    from .lmcmc import LMCState
+    from .sampling import SampleResult
+
+    q = state.position
+    C, D = q.shape
+    T = int(num_samples)
+    if T < 1:
+        raise ValueError("num_samples must be >= 1")
+    dev = q.device
+    samples = samples_out if samples_out is not None else (
+        torch.zeros(T, C, D, dtype=torch.float64, device=dev) if store_samples else None)
+    acc = torch.empty(T, C, dtype=torch.float32, device=dev)
+    n_acc = torch.zeros(C, dtype=torch.int32, device=dev)
+    s = LMCState(*state)
+    for t in range(T):
+        key = rng_key if isinstance(rng_key, PRNGKey) else HostNoise(rng_key.z[t], rng_key.u[t])
+        if isinstance(key, PRNGKey):
+            key = PRNGKey(key.seed, key.counter + t)
+        s, info = transition(key, s, logdensity_fn, step_size, metric_fn, num_integration_steps,
+                             divergence_threshold)
+        if samples is not None:
+            samples[t].copy_(s.position)
+        acc[t] = info.acceptance_rate.float()
+        n_acc += info.is_accepted.int()
+    fev = torch.full((C,), T * int(num_integration_steps), dtype=torch.int64, device=dev)
+    return SampleResult(s, samples, acc, n_acc, fev)

@@ -469,6 +469,41 @@ int gjx_mmala_fisher_logistic_step(const gjx_logistic_params* params, const gjx_
 int gjx_mmala_fisher_logistic_sample(const gjx_logistic_params* params, const gjx_logistic_data* data,
                                      int reference_faithful, const gjx_mmala_sample_io* io, void* stream);
 
+/* ---------------------------------------------------------------------------
+ * General dense metrics for LMC (lmcmc with a user metric_fn).  The caller
+ * evaluates G(q) [C, dim, dim] and its Jacobian [C, dim, dim, dim]
+ * (dG_ab/dq_i at [c][a][b][i], the layout of jax.jacfwd / torch.func.jacfwd);
+ * these entry points do the per-chain linear algebra of the integrator.
+ * float64, dim <= 64.  Buffers are device memory, [C, dim] row-major.
+ * ------------------------------------------------------------------------- */
+typedef struct {
+  int32_t num_chains;
+  int32_t dim;
+  double step_size;                 /* half step: the signed step size eps */
+  const double* metric;             /* [C, dim, dim] */
+  const double* metric_jacobian;    /* [C, dim, dim, dim] */
+  const double* z;                  /* position: [C, dim] normals of the velocity draw, or NULL */
+  const double* velocity;           /* position: [C, dim] velocity for v^T G v, or NULL; half step: input */
+  const double* logdensity_grad;    /* half step: [C, dim] */
+  const double* grad_logdet_in;     /* half step: [C, dim] grad log det G at the position */
+  double* velocity_out;             /* position: L^-T z (needs z); half step: new velocity (may alias velocity) */
+  double* logdet;                   /* position: [C] log det (G + G^T)/2, optional */
+  double* grad_logdet;              /* position: [C, dim] grad log det G, optional */
+  double* vgv;                      /* position: [C] v^T G v (needs velocity), optional */
+  double* volume_adjustment;        /* half step: [C] in/out, J - log|det A+| + log|det A-| */
+} gjx_dense_metric_io;
+
+/* Position quantities of a dense metric (geomjax/lmcmc/metrics.py:75-112,181-189):
+ * log det G, grad log det G = tr(G^-1 dG/dq_k), the velocity draw L^-T z
+ * (L = chol((G + G^T)/2)) and v^T G v. */
+int gjx_dense_metric_position(const gjx_dense_metric_io* io, void* stream);
+
+/* One half step of lan_integrator for a dense metric
+ * (geomjax/lmcmc/integrators.py:61-91 with omega_tilde_fn, geomjax/lmcmc/metrics.py:158-179):
+ * v' = (G + (eps/2) Omega~(v))^-1 (G v - (eps/2)(-grad l + grad log det G / 2)) by LU with
+ * partial pivoting, J += -log|det(G + (eps/2) Omega~(v))| + log|det(G - (eps/2) Omega~(v'))|. */
+int gjx_dense_lmc_half_step(const gjx_dense_metric_io* io, void* stream);
+
 /* value_and_grad of the logistic log density for C positions (init). */
 int gjx_logistic_value_and_grad(const gjx_logistic_params* params, const gjx_logistic_data* data,
                                 const void* position, double* logdensity, void* logdensity_grad, void* stream);

@@ -155,3 +155,21 @@ def test_mmala_and_logistic_nuts_entry_points_validate_on_the_host():
                                                                         _lib.LmcNutsIO)):
         assert fn(params(65), data(65), 5, _lib.GJX_UTURN["euc"], io_t(), None) == -2
         assert fn(params(8), data(8), 5, 7, io_t(), None) == -1  # unknown criterion
+
+
+def test_dense_metric_entry_points_validate_on_the_host():
+    lib = _lib.load()
+    io = _lib.DenseMetricIO(num_chains=0, dim=8)
+    assert lib.gjx_dense_metric_position(io, None) == -1
+    assert "num_chains" in _lib.last_error()
+    io.num_chains, io.dim = 4, 65
+    assert lib.gjx_dense_metric_position(io, None) == -2
+    io.dim = 8
+    assert lib.gjx_dense_metric_position(io, None) == -1
+    assert "metric" in _lib.last_error()
+    io.metric, io.metric_jacobian, io.velocity_out = 1, 1, 1
+    assert lib.gjx_dense_metric_position(io, None) == -1
+    assert "needs z" in _lib.last_error()
+    io.velocity_out = None
+    assert lib.gjx_dense_lmc_half_step(io, None) == -1
+    assert "required" in _lib.last_error()
