@@ -39,10 +39,22 @@ def _cases():
                                  256, 20, torch.float32),
         "logistic_f64": lambda: (gj.rmhmc(gj.LogisticRegression(X, y, 100.0), 0.1, gj.FisherMetric(), 6,
                                           integrator=integ), 16, 40, torch.float64),
+        # a user dense metric: torch.func derivatives and the dense-metric kernels in one graph
+        "dense_lmc_f64": lambda: (gj.lmc(_funnel_logp, 0.05, _monge_metric, 3), 16, 10, torch.float64),
         # a batch of three widths: its launches fork onto side streams and join
         "logistic_batch_f32": lambda: (gj.rmhmc(_batch(), 0.05, gj.FisherMetric(), 4, integrator=integ), 24, 100,
                                        torch.float32),
     }
+
+
+def _funnel_logp(q):
+    x, v = q[:-1], q[-1]
+    return -0.5 * v * v / 9.0 + torch.sum(-0.5 * x * x * torch.exp(-v) - 0.5 * v)
+
+
+def _monge_metric(q):
+    g = torch.func.grad(_funnel_logp)(q)
+    return torch.eye(q.shape[0], dtype=q.dtype, device=q.device) + torch.outer(g, g)
 
 
 def _batch():
@@ -54,7 +66,7 @@ def _batch():
 
 
 @pytest.mark.parametrize("name", ["logistic_f32", "logistic_wide_f32", "funnel_f32", "lmcmonge_f32", "logistic_f64",
-                                  "logistic_batch_f32"])
+                                  "logistic_batch_f32", "dense_lmc_f64"])
 def test_graph_replays_match_eager(name):
     alg, C, D, dt = _cases()[name]()
     T, R = 3, 3
