@@ -759,7 +759,7 @@ static int dense_args(const gjx_dense_metric_io* io, gjx::DenseMetricArgs* a, co
   if (io->dim > gjx::kDenseMaxD)
     return fail(GJX_EUNSUPPORTED, "%s: dim %d exceeds the dense-metric kernels' maximum of %d", fn, io->dim,
                 gjx::kDenseMaxD);
-  GJX_REQUIRE(io->metric && io->metric_jacobian, "%s: metric and metric_jacobian are required", fn);
+  GJX_REQUIRE(io->metric, "%s: metric is required", fn);
   std::memset(a, 0, sizeof(*a));
   a->C = io->num_chains;
   a->D = io->dim;
@@ -775,6 +775,9 @@ static int dense_args(const gjx_dense_metric_io* io, gjx::DenseMetricArgs* a, co
   a->grad_logdet = io->grad_logdet;
   a->vgv = io->vgv;
   a->volume_adjustment = io->volume_adjustment;
+  a->gamma = io->gamma;
+  a->drift = io->drift;
+  a->chol_inv_z = io->chol_inv_z;
   return GJX_OK;
 }
 
@@ -782,6 +785,7 @@ int gjx_dense_metric_position(const gjx_dense_metric_io* io, void* stream) {
   gjx::DenseMetricArgs a;
   int rc = dense_args(io, &a, "gjx_dense_metric_position");
   if (rc != GJX_OK) return rc;
+  GJX_REQUIRE(io->metric_jacobian, "metric_jacobian is required");
   GJX_REQUIRE(!io->velocity_out || io->z, "velocity_out (L^-T z) needs z");
   GJX_REQUIRE(!io->vgv || io->velocity, "vgv needs velocity");
   hipError_t e = gjx::launch_dense_metric_position(a, (hipStream_t)stream);
@@ -793,12 +797,25 @@ int gjx_dense_lmc_half_step(const gjx_dense_metric_io* io, void* stream) {
   gjx::DenseMetricArgs a;
   int rc = dense_args(io, &a, "gjx_dense_lmc_half_step");
   if (rc != GJX_OK) return rc;
+  GJX_REQUIRE(io->metric_jacobian, "metric_jacobian is required");
   GJX_REQUIRE(std::isfinite(io->step_size), "step_size must be finite");
   GJX_REQUIRE(io->velocity && io->logdensity_grad && io->grad_logdet_in && io->velocity_out &&
                   io->volume_adjustment,
               "velocity, logdensity_grad, grad_logdet_in, velocity_out and volume_adjustment are required");
   hipError_t e = gjx::launch_dense_lmc_half_step(a, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "gjx_dense_lmc_half_step launch");
+  return GJX_OK;
+}
+
+int gjx_dense_mmala_drift(const gjx_dense_metric_io* io, void* stream) {
+  gjx::DenseMetricArgs a;
+  int rc = dense_args(io, &a, "gjx_dense_mmala_drift");
+  if (rc != GJX_OK) return rc;
+  GJX_REQUIRE(!io->gamma || io->metric_jacobian, "gamma needs metric_jacobian");
+  GJX_REQUIRE(!io->drift || io->logdensity_grad, "drift needs logdensity_grad");
+  GJX_REQUIRE(!io->chol_inv_z || io->z, "chol_inv_z needs z");
+  hipError_t e = gjx::launch_dense_mmala_drift(a, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "gjx_dense_mmala_drift launch");
   return GJX_OK;
 }
 

@@ -22,10 +22,16 @@ struct DenseMetricArgs {
   double* grad_logdet;       // [C, D] grad log det G (position)
   double* vgv;               // [C] v^T G v of the given velocity (position)
   double* volume_adjustment; // [C] in/out (half step)
+  // manifold MALA (drift kernel): Gamma_i = sum_j d(G^-1)_ij/dq_j (needs metric_jac),
+  // drift G^-1 grad l + Gamma, and chol(G^-1) z
+  double* gamma;             // [C, D] or null
+  double* drift;             // [C, D] or null
+  double* chol_inv_z;        // [C, D] or null (needs z)
 };
 
 size_t dense_metric_smem_bytes();
 hipError_t launch_dense_metric_position(const DenseMetricArgs& a, hipStream_t stream);
 hipError_t launch_dense_lmc_half_step(const DenseMetricArgs& a, hipStream_t stream);
+hipError_t launch_dense_mmala_drift(const DenseMetricArgs& a, hipStream_t stream);
 
 }  // namespace gjx

@@ -266,9 +266,103 @@ __global__ void __launch_bounds__(NT) dense_lmc_half_step_kernel(const DenseMetr
   if (t == 0) a.volume_adjustment[c] = a.volume_adjustment[c] - ld_plus + ld_minus;
 }
 
+// Manifold MALA quantities of a state (geomjax/lmcmc/mmala.py:91-113,
+// geomjax/lmcmc/diffusions.py:55-94): with G^-1 from the Cholesky factor of
+// (G + G^T)/2, Gamma = -G^-1 h with h_a = sum_bj dG_ab/dq_j (G^-1)_bj (so that
+// Gamma_i = sum_j d(G^-1)_ij/dq_j), the drift G^-1 grad l + Gamma, log det G and
+// the draw chol(G^-1) z (the lower factor of the inverse, which the reference
+// factors as the proposal covariance 2 eps G^-1).
+__global__ void __launch_bounds__(NT) dense_mmala_drift_kernel(const DenseMetricArgs a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  Smem& S = *reinterpret_cast<Smem*>(smem_raw);
+  const int c = blockIdx.x;
+  if (c >= a.C) return;
+  const int t = threadIdx.x, D = a.D, DD = D * D;
+  const double* G = a.metric + (size_t)c * DD;
+  for (int ij = t; ij < DD; ij += NT) {
+    const int i = ij / D, j = ij % D;
+    S.A[i * LDA + j] = 0.5 * (G[i * D + j] + G[j * D + i]);
+  }
+  if (t < D) S.v[t] = a.logdensity_grad ? a.logdensity_grad[(size_t)c * D + t] : 0.0;
+  __syncthreads();
+  const double logdet = cholesky(S, D);
+  if (t < D) {  // W = L^-1, one column per lane
+    const int j = t;
+    for (int i = 0; i < D; ++i) {
+      double s = i == j ? 1.0 : 0.0;
+      for (int k = j; k < i; ++k) s = fma(-S.A[i * LDA + k], S.W[k * LDA + j], s);
+      S.W[i * LDA + j] = i < j ? 0.0 : s / S.A[i * LDA + i];
+    }
+  }
+  __syncthreads();
+  for (int ab = t; ab < DD; ab += NT) {  // G^-1 = W^T W into S.T
+    const int ai = ab / D, bi = ab % D;
+    double s = 0.0;
+    for (int k = (ai > bi ? ai : bi); k < D; ++k) s = fma(S.W[k * LDA + ai], S.W[k * LDA + bi], s);
+    S.T[ai * LDA + bi] = s;
+  }
+  __syncthreads();
+  // h_a = sum_b sum_j dG[a][b][j] (G^-1)_bj: one (a, b) per lane pass (contiguous j) into S.A
+  if (a.metric_jac != nullptr) {
+    const double* dG = a.metric_jac + (size_t)c * DD * D;
+    for (int ab = t; ab < DD; ab += NT) {
+      const int ai = ab / D, bi = ab % D;
+      const double* p = dG + (size_t)ab * D;
+      double s = 0.0;
+      for (int j = 0; j < D; ++j) s = fma(p[j], S.T[bi * LDA + j], s);
+      S.A[ai * LDA + bi] = s;
+    }
+    __syncthreads();
+    if (t < D) {
+      double h = 0.0;
+      for (int bi = 0; bi < D; ++bi) h += S.A[t * LDA + bi];
+      S.rhs[t] = h;
+    }
+  } else if (t < D) {
+    S.rhs[t] = 0.0;
+  }
+  __syncthreads();
+  // Gamma = -G^-1 h, drift = G^-1 grad l + Gamma
+  if (t < D) {
+    double gh = 0.0, gg = 0.0;
+    for (int j = 0; j < D; ++j) {
+      gh = fma(S.T[t * LDA + j], S.rhs[j], gh);
+      gg = fma(S.T[t * LDA + j], S.v[j], gg);
+    }
+    const size_t r = (size_t)c * D + t;
+    if (a.gamma) a.gamma[r] = -gh;
+    if (a.drift) a.drift[r] = gg - gh;
+  }
+  // chol(G^-1) z: the lower Cholesky factor of the inverse, formed in S.A
+  if (a.z != nullptr && a.chol_inv_z != nullptr) {
+    __syncthreads();
+    for (int ij = t; ij < DD; ij += NT) {
+      const int i = ij / D, j = ij % D;
+      S.A[i * LDA + j] = 0.5 * (S.T[i * LDA + j] + S.T[j * LDA + i]);
+    }
+    if (t < D) S.x[t] = a.z[(size_t)c * D + t];
+    __syncthreads();
+    (void)cholesky(S, D);
+    if (t < D) {
+      double s = 0.0;
+      for (int k = 0; k <= t; ++k) s = fma(S.A[t * LDA + k], S.x[k], s);
+      a.chol_inv_z[(size_t)c * D + t] = s;
+    }
+  }
+  if (t == 0 && a.logdet) a.logdet[c] = logdet;
+}
+
 }  // namespace
 
 size_t dense_metric_smem_bytes() { return sizeof(Smem); }
+
+hipError_t launch_dense_mmala_drift(const DenseMetricArgs& a, hipStream_t stream) {
+  const size_t smem = sizeof(Smem);
+  (void)hipFuncSetAttribute((const void*)dense_mmala_drift_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)smem);
+  hipLaunchKernelGGL(dense_mmala_drift_kernel, dim3(a.C), dim3(NT), smem, stream, a);
+  return hipGetLastError();
+}
 
 hipError_t launch_dense_metric_position(const DenseMetricArgs& a, hipStream_t stream) {
   const size_t smem = sizeof(Smem);

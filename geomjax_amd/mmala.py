@@ -14,7 +14,9 @@ float64: Bayesian logistic regression with the Fisher metric, one posterior or a
 ``reference_faithful=True`` (the default) keeps the reference's pairing of the
 transition energies, whose proposal weight is minus the log Metropolis-Hastings
 ratio (geomjax/lmcmc/mmala.py:74-113 with geomjax/mcmc/proposal.py:151-158);
-``False`` uses the ratio itself.  ``rng_key`` is a ``PRNGKey`` (device Philox)
+``False`` uses the ratio itself.  A plain callable ``metric_fn`` (with a
+plain callable log density) runs the dense-metric path (``mmala_dense``).
+``rng_key`` is a ``PRNGKey`` (device Philox)
 or ``HostNoise(z [C, D], u [C])`` (the proposal's normals and the accept
 uniform).
 """
@@ -56,7 +58,14 @@ def _check(logdensity_fn, metric_fn, position):
 
 
 def init(position, logdensity_fn, metric_fn=FisherMetric()):
-    """geomjax/lmcmc/mmala.py:53-60: log density, gradient, metric and Gamma."""
+    """geomjax/lmcmc/mmala.py:53-60: log density, gradient, metric and Gamma.
+    Plain callables (a user log density and metric) take the dense-metric path."""
+    from .lmcmc_dense import is_dense_metric_fn
+
+    if is_dense_metric_fn(logdensity_fn, metric_fn):
+        from . import mmala_dense
+
+        return mmala_dense.init(position, logdensity_fn, metric_fn)
     single = position.dim() == 1
     pos = (position[None] if single else position).contiguous()
     _check(logdensity_fn, metric_fn, pos)
@@ -92,6 +101,13 @@ def build_kernel(*, reference_faithful=True, divergence_threshold: float = 1000)
     threshold at 1000)."""
 
     def kernel(rng_key, state, logdensity_fn, step_size, metric_fn):
+        from .lmcmc_dense import is_dense_metric_fn
+
+        if is_dense_metric_fn(logdensity_fn, metric_fn):
+            from . import mmala_dense
+
+            return mmala_dense.transition(rng_key, state, logdensity_fn, step_size, metric_fn, reference_faithful,
+                                          divergence_threshold)
         single = state.position.dim() == 1
         position = (state.position[None] if single else state.position).contiguous()
         _check(logdensity_fn, metric_fn, position)
@@ -142,8 +158,15 @@ def sample(rng_key, state, logdensity_fn, step_size, metric_fn, num_samples, *, 
            divergence_threshold=1000, store_samples=True, samples_out=None):
     """``num_samples`` MMALA transitions of every chain in one persistent launch;
     the final state's gradient, metric and Gamma are evaluated at its position."""
+    from .lmcmc_dense import is_dense_metric_fn
     from .sampling import SampleResult
 
+    if is_dense_metric_fn(logdensity_fn, metric_fn):
+        from . import mmala_dense
+
+        return mmala_dense.sample(rng_key, state, logdensity_fn, step_size, metric_fn, num_samples,
+                                  reference_faithful=reference_faithful, divergence_threshold=divergence_threshold,
+                                  store_samples=store_samples, samples_out=samples_out)
     position = state.position
     if position.dim() != 2:
         raise ValueError("sample() runs a batch of chains: position must be [C, D]")

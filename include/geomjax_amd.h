@@ -491,6 +491,9 @@ typedef struct {
   double* grad_logdet;              /* position: [C, dim] grad log det G, optional */
   double* vgv;                      /* position: [C] v^T G v (needs velocity), optional */
   double* volume_adjustment;        /* half step: [C] in/out, J - log|det A+| + log|det A-| */
+  double* gamma;                    /* mmala: [C, dim] Gamma_i = sum_j d(G^-1)_ij/dq_j (needs metric_jacobian), optional */
+  double* drift;                    /* mmala: [C, dim] G^-1 grad l + Gamma (needs logdensity_grad), optional */
+  double* chol_inv_z;               /* mmala: [C, dim] chol(G^-1) z (needs z), optional */
 } gjx_dense_metric_io;
 
 /* Position quantities of a dense metric (geomjax/lmcmc/metrics.py:75-112,181-189):
@@ -503,6 +506,12 @@ int gjx_dense_metric_position(const gjx_dense_metric_io* io, void* stream);
  * v' = (G + (eps/2) Omega~(v))^-1 (G v - (eps/2)(-grad l + grad log det G / 2)) by LU with
  * partial pivoting, J += -log|det(G + (eps/2) Omega~(v))| + log|det(G - (eps/2) Omega~(v'))|. */
 int gjx_dense_lmc_half_step(const gjx_dense_metric_io* io, void* stream);
+
+/* Manifold-MALA quantities of a dense metric (geomjax/lmcmc/mmala.py:91-113,
+ * geomjax/lmcmc/diffusions.py:55-94): log det G, Gamma (from the Jacobian),
+ * the drift G^-1 grad l + Gamma and the draw chol(G^-1) z.  metric_jacobian
+ * may be NULL when gamma is not requested (the drift then omits Gamma). */
+int gjx_dense_mmala_drift(const gjx_dense_metric_io* io, void* stream);
 
 /* value_and_grad of the logistic log density for C positions (init). */
 int gjx_logistic_value_and_grad(const gjx_logistic_params* params, const gjx_logistic_data* data,

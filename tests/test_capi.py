@@ -173,3 +173,16 @@ def test_dense_metric_entry_points_validate_on_the_host():
     io.velocity_out = None
     assert lib.gjx_dense_lmc_half_step(io, None) == -1
     assert "required" in _lib.last_error()
+
+
+def test_dense_mmala_drift_validates_on_the_host():
+    lib = _lib.load()
+    io = _lib.DenseMetricIO(num_chains=2, dim=4, metric=1, gamma=1)
+    assert lib.gjx_dense_mmala_drift(io, None) == -1
+    assert "metric_jacobian" in _lib.last_error()
+    io.gamma, io.drift = None, 1
+    assert lib.gjx_dense_mmala_drift(io, None) == -1
+    assert "logdensity_grad" in _lib.last_error()
+    io.drift, io.chol_inv_z = None, 1
+    assert lib.gjx_dense_mmala_drift(io, None) == -1
+    assert "needs z" in _lib.last_error()

@@ -147,3 +147,72 @@ def test_exact_mode_agrees_with_rmhmc_on_the_posterior_moments():
     se = (qa.var(0) / C + qb.var(0) / C).sqrt()
     assert ((qa.mean(0) - qb.mean(0)).abs() < 6 * se).all()
     assert ((qa.var(0) / qb.var(0) - 1).abs() < 0.15).all()
+
+
+def _funnel_logp(q):
+    import math
+
+    x, v = q[:-1], q[-1]
+    return (-0.5 * v * v / 9.0 - 0.5 * math.log(2 * math.pi * 9.0)
+            + torch.sum(-0.5 * x * x * torch.exp(-v) - 0.5 * v - 0.5 * math.log(2 * math.pi)))
+
+
+def _monge(q):
+    g = torch.func.grad(_funnel_logp)(q)
+    return torch.eye(q.shape[0], dtype=q.dtype, device=q.device) + 0.5 * torch.outer(g, g)
+
+
+@pytest.mark.parametrize("faithful", [True, False])
+def test_user_dense_metric_matches_oracle(faithful):
+    """A user metric (the funnel's Monge metric by autodiff): the dense-metric
+    kernels and torch.func against the oracle's jacfwd of the inverse metric."""
+    print(f"case mmala dense funnel-Monge faithful={faithful}", flush=True)
+    C, D, eps, T = 4, 8, 0.05, 2
+    q0 = 0.5 * seeded_normal(70, C, D)
+    so = omm.init(q0, _funnel_logp, _monge)
+    s = mm_init(q0.to(DEV), _funnel_logp, _monge)
+    for f in ("logdensity", "logdensity_grad", "metric", "Gamma"):
+        assert rel_err(getattr(s, f), getattr(so, f)) < 1e-10, ("init", f)
+    kern, okern = build_kernel(reference_faithful=faithful), omm.build_kernel(reference_faithful=faithful)
+    for t in range(T):
+        z, u = seeded_normal(71 + t, C, D), seeded_uniform(81 + t, C)
+        s1, info = kern(gr.HostNoise(z.to(DEV), u.to(DEV)), s, _funnel_logp, eps, _monge)
+        so1, oi = okern((z, u), so, _funnel_logp, eps, _monge)
+        torch.cuda.synchronize()
+        assert info.is_accepted.cpu().tolist() == oi.is_accepted.tolist()
+        assert (info.weight.cpu() - oi.weight).abs().max().item() < 1e-10 * max(1.0, oi.weight.abs().max().item())
+        for f in DiffusionState._fields:
+            assert rel_err(getattr(s1, f), getattr(so1, f)) < 1e-10, ("state", f)
+            assert rel_err(getattr(info.proposal, f), getattr(oi.proposal, f)) < 1e-10, ("proposal", f)
+        so, s = so1, s1
+
+
+def test_fisher_as_user_function_matches_the_closed_form_kernel():
+    import math
+
+    X, y, _ = make_logistic(100, 6, 13)
+    Xd = torch.as_tensor(X, dtype=torch.float64, device=DEV)
+    yd = torch.as_tensor(y, dtype=torch.float64, device=DEV)
+
+    def logp(b):
+        eta = Xd @ b
+        return (torch.sum(yd * eta - torch.nn.functional.softplus(eta)) - 0.5 * torch.dot(b, b) / 100.0
+                - 0.5 * b.shape[0] * math.log(2 * math.pi * 100.0))
+
+    def fisher(b):
+        s = torch.sigmoid(Xd @ b)
+        return (Xd.T * (s * (1 - s))) @ Xd + torch.eye(b.shape[0], dtype=b.dtype, device=b.device) / 100.0
+
+    C, D = 8, 6
+    q0 = (0.3 * seeded_normal(90, C, D)).to(DEV)
+    z, u = seeded_normal(91, C, D).to(DEV), seeded_uniform(92, C).to(DEV)
+    model = gj.LogisticRegression(X, y, 100.0)
+    su, sc = mm_init(q0, logp, fisher), mm_init(q0, model)
+    assert rel_err(su.Gamma, sc.Gamma) < 1e-10 and rel_err(su.metric, sc.metric) < 1e-12
+    a, ia = build_kernel(reference_faithful=False)(gr.HostNoise(z, u), su, logp, 0.3, fisher)
+    b, ib = build_kernel(reference_faithful=False)(gr.HostNoise(z, u), sc, model, 0.3, gj.FisherMetric())
+    torch.cuda.synchronize()
+    assert torch.equal(ia.is_accepted, ib.is_accepted)
+    assert rel_err(ia.weight, ib.weight) < 1e-9
+    for f in DiffusionState._fields:
+        assert rel_err(getattr(a, f), getattr(b, f)) < 1e-10, f
