@@ -187,7 +187,8 @@ class MmalaSampleIO(ctypes.Structure):
 class DenseMetricIO(ctypes.Structure):
     _fields_ = [("num_chains", _c_int32), ("dim", _c_int32), ("step_size", _c_double)] + [(n, _vp) for n in (
         "metric", "metric_jacobian", "z", "velocity", "logdensity_grad", "grad_logdet_in", "velocity_out",
-        "logdet", "grad_logdet", "vgv", "volume_adjustment", "gamma", "drift", "chol_inv_z",
+        "logdet", "grad_logdet", "vgv", "volume_adjustment", "gamma", "drift", "chol_inv_z", "momentum",
+        "momentum_out", "dtdq", "pu",
     )]
 
 
@@ -222,6 +223,7 @@ _SIGNATURES = {
     "gjx_dense_metric_position": [ctypes.POINTER(DenseMetricIO), _vp],
     "gjx_dense_lmc_half_step": [ctypes.POINTER(DenseMetricIO), _vp],
     "gjx_dense_mmala_drift": [ctypes.POINTER(DenseMetricIO), _vp],
+    "gjx_dense_rmhmc_field": [ctypes.POINTER(DenseMetricIO), _vp],
     "gjx_mmala_fisher_logistic_init": [ctypes.POINTER(LogisticParams), ctypes.POINTER(LogisticData), _vp, _vp, _vp,
                                        _vp, _vp, _vp],
     "gjx_mmala_fisher_logistic_step": [ctypes.POINTER(LogisticParams), ctypes.POINTER(LogisticData), _c_int32,

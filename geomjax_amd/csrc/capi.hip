@@ -778,6 +778,10 @@ static int dense_args(const gjx_dense_metric_io* io, gjx::DenseMetricArgs* a, co
   a->gamma = io->gamma;
   a->drift = io->drift;
   a->chol_inv_z = io->chol_inv_z;
+  a->momentum = io->momentum;
+  a->momentum_out = io->momentum_out;
+  a->dtdq = io->dtdq;
+  a->pu = io->pu;
   return GJX_OK;
 }
 
@@ -816,6 +820,17 @@ int gjx_dense_mmala_drift(const gjx_dense_metric_io* io, void* stream) {
   GJX_REQUIRE(!io->chol_inv_z || io->z, "chol_inv_z needs z");
   hipError_t e = gjx::launch_dense_mmala_drift(a, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "gjx_dense_mmala_drift launch");
+  return GJX_OK;
+}
+
+int gjx_dense_rmhmc_field(const gjx_dense_metric_io* io, void* stream) {
+  gjx::DenseMetricArgs a;
+  int rc = dense_args(io, &a, "gjx_dense_rmhmc_field");
+  if (rc != GJX_OK) return rc;
+  GJX_REQUIRE(io->momentum || io->z, "momentum or z is required");
+  GJX_REQUIRE(!io->dtdq || io->metric_jacobian, "dtdq needs metric_jacobian");
+  hipError_t e = gjx::launch_dense_rmhmc_field(a, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "gjx_dense_rmhmc_field launch");
   return GJX_OK;
 }
 

@@ -27,11 +27,18 @@ struct DenseMetricArgs {
   double* gamma;             // [C, D] or null
   double* drift;             // [C, D] or null
   double* chol_inv_z;        // [C, D] or null (needs z)
+  // RMHMC field (field kernel): momentum p (or chol(G) z when null and z is
+  // given), u = G^-1 p, dT/dq and p^T u; the momentum used goes to momentum_out
+  const double* momentum;    // [C, D] or null
+  double* momentum_out;      // [C, D] or null
+  double* dtdq;              // [C, D] or null (needs metric_jac)
+  double* pu;                // [C] or null
 };
 
 size_t dense_metric_smem_bytes();
 hipError_t launch_dense_metric_position(const DenseMetricArgs& a, hipStream_t stream);
 hipError_t launch_dense_lmc_half_step(const DenseMetricArgs& a, hipStream_t stream);
 hipError_t launch_dense_mmala_drift(const DenseMetricArgs& a, hipStream_t stream);
+hipError_t launch_dense_rmhmc_field(const DenseMetricArgs& a, hipStream_t stream);
 
 }  // namespace gjx

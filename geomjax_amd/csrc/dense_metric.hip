@@ -352,9 +352,94 @@ __global__ void __launch_bounds__(NT) dense_mmala_drift_kernel(const DenseMetric
   if (t == 0 && a.logdet) a.logdet[c] = logdet;
 }
 
+// The implicit-midpoint vector field of RMHMC for a dense metric
+// (geomjax/rmhmc/metrics.py:42-129, geomjax/rmhmc/integrators.py:125-135):
+// with T(q, p) = p^T G^-1 p / 2 + log det G / 2 + (D/2) log 2 pi and L the
+// Cholesky factor of (G + G^T)/2, dT/dp = u = G^-1 p and
+// dT/dq_k = 1/2 sum_ab ((G^-1)_ba - u_a u_b) dG_ab/dq_k; the momentum draw is
+// p = L z.
+__global__ void __launch_bounds__(NT) dense_rmhmc_field_kernel(const DenseMetricArgs a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  Smem& S = *reinterpret_cast<Smem*>(smem_raw);
+  const int c = blockIdx.x;
+  if (c >= a.C) return;
+  const int t = threadIdx.x, D = a.D, DD = D * D;
+  const double* G = a.metric + (size_t)c * DD;
+  for (int ij = t; ij < DD; ij += NT) {
+    const int i = ij / D, j = ij % D;
+    S.A[i * LDA + j] = 0.5 * (G[i * D + j] + G[j * D + i]);
+  }
+  if (t < D) S.rhs[t] = a.z ? a.z[(size_t)c * D + t] : 0.0;
+  __syncthreads();
+  const double logdet = cholesky(S, D);
+  // the momentum: given, or p = L z
+  if (t < D) {
+    double pv;
+    if (a.momentum != nullptr) {
+      pv = a.momentum[(size_t)c * D + t];
+    } else {
+      pv = 0.0;
+      for (int k = 0; k <= t; ++k) pv = fma(S.A[t * LDA + k], S.rhs[k], pv);
+    }
+    S.v[t] = pv;
+    const int j = t;  // W = L^-1, one column per lane
+    for (int i = 0; i < D; ++i) {
+      double s = i == j ? 1.0 : 0.0;
+      for (int k = j; k < i; ++k) s = fma(-S.A[i * LDA + k], S.W[k * LDA + j], s);
+      S.W[i * LDA + j] = i < j ? 0.0 : s / S.A[i * LDA + i];
+    }
+  }
+  __syncthreads();
+  for (int ab = t; ab < DD; ab += NT) {  // G^-1 = W^T W into S.T
+    const int ai = ab / D, bi = ab % D;
+    double s = 0.0;
+    for (int k = (ai > bi ? ai : bi); k < D; ++k) s = fma(S.W[k * LDA + ai], S.W[k * LDA + bi], s);
+    S.T[ai * LDA + bi] = s;
+  }
+  __syncthreads();
+  if (t < D) {  // u = G^-1 p
+    double s = 0.0;
+    for (int j = 0; j < D; ++j) s = fma(S.T[t * LDA + j], S.v[j], s);
+    S.x[t] = s;
+  }
+  __syncthreads();
+  const double pu = block_sum(S, t < D ? S.v[t] * S.x[t] : 0.0);
+  if (a.metric_jac != nullptr) {
+    const double* dG = a.metric_jac + (size_t)c * DD * D;
+    const int k = t % 64, part = t / 64;
+    double s = 0.0;
+    if (k < D)
+      for (int ai = part; ai < D; ai += 4) {
+        const double ua = S.x[ai];
+        for (int bi = 0; bi < D; ++bi)
+          s = fma(S.T[bi * LDA + ai] - ua * S.x[bi], dG[((size_t)ai * D + bi) * D + k], s);
+      }
+    S.part[part][k] = s;
+    __syncthreads();
+  }
+  if (t < D) {
+    const size_t r = (size_t)c * D + t;
+    if (a.velocity_out) a.velocity_out[r] = S.x[t];
+    if (a.momentum_out) a.momentum_out[r] = S.v[t];
+    if (a.dtdq && a.metric_jac) a.dtdq[r] = 0.5 * (S.part[0][t] + S.part[1][t] + S.part[2][t] + S.part[3][t]);
+  }
+  if (t == 0) {
+    if (a.logdet) a.logdet[c] = logdet;
+    if (a.pu) a.pu[c] = pu;
+  }
+}
+
 }  // namespace
 
 size_t dense_metric_smem_bytes() { return sizeof(Smem); }
+
+hipError_t launch_dense_rmhmc_field(const DenseMetricArgs& a, hipStream_t stream) {
+  const size_t smem = sizeof(Smem);
+  (void)hipFuncSetAttribute((const void*)dense_rmhmc_field_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)smem);
+  hipLaunchKernelGGL(dense_rmhmc_field_kernel, dim3(a.C), dim3(NT), smem, stream, a);
+  return hipGetLastError();
+}
 
 hipError_t launch_dense_mmala_drift(const DenseMetricArgs& a, hipStream_t stream) {
   const size_t smem = sizeof(Smem);

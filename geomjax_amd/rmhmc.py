@@ -12,7 +12,9 @@ geomjax/adaptation/step_size_adaptation.py:147-153) work unchanged.
 Differences forced by the GPU (documented, not silent):
   * states are batched: ``position`` is [C, D] (a [D] vector is one chain);
   * ``logdensity_fn`` / ``metric_fn`` are model and metric descriptors
-    (``geomjax_amd.models``, ``geomjax_amd.metrics``), not callables;
+    (``geomjax_amd.models``, ``geomjax_amd.metrics``) for the closed-form
+    kernels; plain callables (a user log density and dense metric written with
+    torch operations) run the dense-metric path (``rmhmc_dense``);
   * ``rng_key`` is a ``geomjax_amd.random.HostNoise`` or ``PRNGKey``.
 The fixed-point solver's keywords (``max_iters``, ``convergence_tol``,
 ``divergence_tol``) are bound with ``functools.partial(implicit_midpoint,
@@ -114,8 +116,13 @@ def _steps(num_integration_steps, shape, dev, max_num_integration_steps=None):
 
 
 def init(position, logdensity_fn):
-    """geomjax/rmhmc/rmhmc.py:96-98: value and gradient of the log density."""
+    """geomjax/rmhmc/rmhmc.py:96-98: value and gradient of the log density.  A
+    plain callable (a user function) is evaluated by torch.func on the device."""
     if not isinstance(logdensity_fn, (NealFunnel, LogisticRegression, LogisticBatch)):
+        if callable(logdensity_fn):
+            from . import rmhmc_dense
+
+            return rmhmc_dense.init(position, logdensity_fn)
         raise NotImplementedError(f"no HIP log density for {logdensity_fn!r}")
     logdensity, grad = logdensity_fn.value_and_grad(position)
     return RMHMCState(position, logdensity, grad)
@@ -162,6 +169,28 @@ def build_kernel(integrator=implicit_midpoint, divergence_threshold: float = 100
 
     def kernel(rng_key, state, logdensity_fn, step_size, metric_fn, num_integration_steps, *,
                max_num_integration_steps=None):
+        from .lmcmc_dense import is_dense_metric_fn
+
+        if is_dense_metric_fn(logdensity_fn, metric_fn):
+            # a user metric_fn: torch.func derivatives + the dense-metric field kernel
+            from . import rmhmc_dense
+
+            if not isinstance(num_integration_steps, int):
+                raise NotImplementedError("per-chain step counts need a closed-form metric kernel")
+            new, info, fps = rmhmc_dense.transition(rng_key, state, logdensity_fn, step_size, metric_fn,
+                                                    num_integration_steps, solver, divergence_threshold)
+            if return_fixed_point_info:
+                C = state.position.shape[0]
+                if fps:
+                    fp = FixedPointIterationInfo(torch.stack([f[0] for f in fps], 1), torch.stack([f[1] for f in fps], 1),
+                                                 torch.stack([f[2] for f in fps], 1).to(torch.int32))
+                else:
+                    dev = state.position.device
+                    fp = FixedPointIterationInfo(torch.zeros(C, 0, dtype=torch.bool, device=dev),
+                                                 torch.zeros(C, 0, dtype=torch.float64, device=dev),
+                                                 torch.zeros(C, 0, dtype=torch.int32, device=dev))
+                return new, info, fp
+            return new, info
         position, single = _as_batch(state.position)
         kind = _problem(logdensity_fn, metric_fn, position)
         position = position.contiguous()
@@ -252,7 +281,17 @@ def sample(rng_key, state, logdensity_fn, step_size, metric_fn, num_integration_
     int or, for the dynamic kernel, an integer tensor [T, C] of per-transition,
     per-chain counts.  Returns ``SampleResult``.
     """
+    from .lmcmc_dense import is_dense_metric_fn
     from .sampling import SampleResult
+
+    if is_dense_metric_fn(logdensity_fn, metric_fn):
+        from . import rmhmc_dense
+
+        if not isinstance(num_integration_steps, int):
+            raise NotImplementedError("per-chain step counts need a closed-form metric kernel")
+        return rmhmc_dense.sample(rng_key, state, logdensity_fn, step_size, metric_fn, num_integration_steps,
+                                  num_samples, _solver_kwargs(integrator), divergence_threshold=divergence_threshold,
+                                  store_samples=store_samples, samples_out=samples_out)
 
     solver = _solver_kwargs(integrator)
     position, single = _as_batch(state.position)

@@ -494,6 +494,10 @@ typedef struct {
   double* gamma;                    /* mmala: [C, dim] Gamma_i = sum_j d(G^-1)_ij/dq_j (needs metric_jacobian), optional */
   double* drift;                    /* mmala: [C, dim] G^-1 grad l + Gamma (needs logdensity_grad), optional */
   double* chol_inv_z;               /* mmala: [C, dim] chol(G^-1) z (needs z), optional */
+  const double* momentum;           /* rmhmc: [C, dim] p, or NULL: p = chol(G) z (needs z) */
+  double* momentum_out;             /* rmhmc: [C, dim] the p used, optional */
+  double* dtdq;                     /* rmhmc: [C, dim] dT/dq (needs metric_jacobian), optional */
+  double* pu;                       /* rmhmc: [C] p^T G^-1 p, optional */
 } gjx_dense_metric_io;
 
 /* Position quantities of a dense metric (geomjax/lmcmc/metrics.py:75-112,181-189):
@@ -512,6 +516,12 @@ int gjx_dense_lmc_half_step(const gjx_dense_metric_io* io, void* stream);
  * the drift G^-1 grad l + Gamma and the draw chol(G^-1) z.  metric_jacobian
  * may be NULL when gamma is not requested (the drift then omits Gamma). */
 int gjx_dense_mmala_drift(const gjx_dense_metric_io* io, void* stream);
+
+/* The implicit-midpoint vector field of RMHMC for a dense metric
+ * (geomjax/rmhmc/metrics.py:42-129 with integrators.py:125-135): the momentum
+ * p (given, or drawn as chol(G) z), u = G^-1 p (into velocity_out), dT/dq_k =
+ * tr(G^-1 dG/dq_k)/2 - u^T (dG/dq_k) u / 2, log det G and p^T u. */
+int gjx_dense_rmhmc_field(const gjx_dense_metric_io* io, void* stream);
 
 /* value_and_grad of the logistic log density for C positions (init). */
 int gjx_logistic_value_and_grad(const gjx_logistic_params* params, const gjx_logistic_data* data,

@@ -186,3 +186,13 @@ def test_dense_mmala_drift_validates_on_the_host():
     io.drift, io.chol_inv_z = None, 1
     assert lib.gjx_dense_mmala_drift(io, None) == -1
     assert "needs z" in _lib.last_error()
+
+
+def test_dense_rmhmc_field_validates_on_the_host():
+    lib = _lib.load()
+    io = _lib.DenseMetricIO(num_chains=2, dim=4, metric=1)
+    assert lib.gjx_dense_rmhmc_field(io, None) == -1
+    assert "momentum or z" in _lib.last_error()
+    io.z, io.dtdq = 1, 1
+    assert lib.gjx_dense_rmhmc_field(io, None) == -1
+    assert "metric_jacobian" in _lib.last_error()
