@@ -690,6 +690,59 @@ def dense_lmc_forms(dev, chains=1024, D=20, L=4, T=5, step_size=0.05):
     return out
 
 
+def dense_rmhmc_forms(dev, chains=1024, D=20, L=4, T=3, step_size=0.05, max_iters=8):
+    """RMHMC with a general dense user metric: Neal's funnel (D = 20) with the
+    Monge metric I + g g^T as a plain torch function, implicit midpoint with at
+    most 8 fixed-point iterations; T transitions captured in a HIP graph
+    (the fixed point then always sweeps max_iters times) and, for comparison,
+    run eagerly (early exit when every chain has converged)."""
+    import math
+
+    import torch
+
+    import geomjax_amd as gj
+    from geomjax_amd import random as gr
+    from geomjax_amd.rmhmc import implicit_midpoint
+    from geomjax_amd.sampling import GraphSampler
+
+    def logp(q):
+        x, v = q[:-1], q[-1]
+        return (-0.5 * v * v / 9.0 - 0.5 * math.log(2 * math.pi * 9.0)
+                + torch.sum(-0.5 * x * x * torch.exp(-v) - 0.5 * v - 0.5 * math.log(2 * math.pi)))
+
+    grad = torch.func.grad(logp)
+
+    def metric(q):
+        g = grad(q)
+        return torch.eye(q.shape[0], dtype=q.dtype, device=q.device) + torch.outer(g, g)
+
+    alg = gj.rmhmc(logp, step_size, metric, L, integrator=functools.partial(implicit_midpoint, max_iters=max_iters))
+    q = torch.randn(chains, D, generator=torch.Generator().manual_seed(INIT_SEED), dtype=torch.float64)
+    s = alg.sample_fn(gr.PRNGKey(INIT_SEED + 23, 0), alg.init(q.to(dev)), 1, store_samples=False).state
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = alg.sample_fn(gr.PRNGKey(INIT_SEED + 23, 1), s, T, store_samples=False)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gs = GraphSampler(alg, res.state, T, store_samples=False)
+    gs.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        gs.replay()
+    torch.cuda.synchronize()
+    gwall = (time.perf_counter() - t0) / 3
+    out = {"value": chains * L * T / gwall, "unit": "leapfrog-steps/s", "ms_per_transition": gwall / T * 1e3,
+           "eager_ms_per_transition": wall / T * 1e3, "form": f"HIP graph of {T} transitions, replayed",
+           "acceptance_rate": float(res.acceptance_rate.double().mean()),
+           "fevals_per_chain_transition_eager": float(res.num_fevals.double().mean()) / T, "chains": chains,
+           "dim": D, "num_integration_steps": L, "max_iters": max_iters, "dtype": "f64", "steps": T,
+           "kernels": "dense_rmhmc_field_kernel (+ torch.func for the user functions)"}
+    log(f"dense-metric RMHMC (funnel D={D}, Monge by autodiff): {out['value']:.3e} leapfrog-steps/s, "
+        f"{out['ms_per_transition']:.1f} ms per transition in a graph ({out['eager_ms_per_transition']:.1f} eager)")
+    return out
+
+
 def lmc_alpha2_forms(dev, repeats=5):
     """Cost of the LMC-Monge alpha2 tangent (float64, the derivative ChEES-lmcmonge
     takes every iteration) against one float64 LMC-Monge transition on the funnel
@@ -780,7 +833,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the 1024-chain and funnel comparisons")
     ap.add_argument("--only", choices=["funnel_nuts", "logistic_nuts", "logistic_nuts_lmc", "lmc_alpha2", "logistic_lmc", "mmala",
-                             "uneven", "dense_lmc"],
+                             "uneven", "dense_lmc", "dense_rmhmc"],
                     help="run one secondary workload alone and print its JSON (for profiling it)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 256 MiB three times (reference kernel for the FETCH_SIZE / WRITE_SIZE counters)")
@@ -814,6 +867,9 @@ def main():
         return
     if args.only == "uneven":
         print(json.dumps(uneven_batch_forms(32, K, W, dev, plain=not args.no_plain)), flush=True)
+        return
+    if args.only == "dense_rmhmc":
+        print(json.dumps(dense_rmhmc_forms(dev)), flush=True)
         return
     if args.only == "dense_lmc":
         print(json.dumps(dense_lmc_forms(dev)), flush=True)
@@ -851,6 +907,7 @@ def main():
         extra["logistic_nuts_lmc_1024_chains"] = logistic_nuts_forms(10, 2, dev, integrator="lmc")
         extra["mmala_1024_chains"] = mmala_forms(200, 100, dev)
         extra["dense_lmc_funnel_d20_1024_chains"] = dense_lmc_forms(dev)
+        extra["dense_rmhmc_funnel_d20_1024_chains"] = dense_rmhmc_forms(dev)
         extra["funnel_lmcmonge_d100_8192_chains"] = funnel_lmc_forms(300, 100, dev, plain=not args.no_plain)
         extra["tangent"] = tangent_forms(dev)
         extra["lmc_alpha2_tangent"] = lmc_alpha2_forms(dev)
