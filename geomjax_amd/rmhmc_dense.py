@@ -47,6 +47,7 @@ def init(position, logdensity_fn):
     single = position.dim() == 1
     q = (position[None] if single else position).contiguous()
     _check(q)
+    _lib.load()  # the sampler's kernels are native: fail here, not at the first step, without them
     l, g = _value_and_grad(logdensity_fn, q)
     if single:
         return RMHMCState(position, l[0], g[0])
