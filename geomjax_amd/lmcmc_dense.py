@@ -41,9 +41,9 @@ def is_dense_metric_fn(logdensity_fn, metric_fn):
 
 def _check(position):
     if not position.is_cuda or position.dtype != torch.float64:
-        raise ValueError("dense-metric LMC states must be float64 tensors on a HIP device")
+        raise ValueError("dense-metric sampler states must be float64 tensors on a HIP device")
     if position.dim() != 2:
-        raise ValueError("dense-metric LMC runs a batch of chains: position must be [C, D]")
+        raise ValueError("the dense-metric samplers run a batch of chains: position must be [C, D]")
     if position.shape[1] > 64:
         raise NotImplementedError(f"dense-metric kernels take D <= 64 (got {position.shape[1]})")
 
