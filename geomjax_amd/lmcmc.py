@@ -103,10 +103,8 @@ def build_kernel(integrator=lan_integrator, divergence_threshold: float = 1000):
 
         if lmcmc_dense.is_dense_metric_fn(logdensity_fn, metric_fn):
             # a user metric_fn: torch.func derivatives + the dense-metric HIP kernels
-            if not isinstance(num_integration_steps, int):
-                raise NotImplementedError("per-chain step counts need a closed-form metric kernel")
             return lmcmc_dense.transition(rng_key, state, logdensity_fn, step_size, metric_fn,
-                                          num_integration_steps, divergence_threshold)
+                                          num_integration_steps, divergence_threshold, max_num_integration_steps)
         single = state.position.dim() == 1
         position = (state.position[None] if single else state.position).contiguous()
         _check(logdensity_fn, metric_fn, position)
@@ -165,11 +163,10 @@ def sample(rng_key, state, logdensity_fn, step_size, metric_fn, num_integration_
     from .sampling import SampleResult
 
     if lmcmc_dense.is_dense_metric_fn(logdensity_fn, metric_fn):
-        if not isinstance(num_integration_steps, int):
-            raise NotImplementedError("per-chain step counts need a closed-form metric kernel")
         return lmcmc_dense.sample(rng_key, state, logdensity_fn, step_size, metric_fn, num_integration_steps,
                                   num_samples, divergence_threshold=divergence_threshold,
-                                  store_samples=store_samples, samples_out=samples_out)
+                                  store_samples=store_samples, samples_out=samples_out,
+                                  max_num_integration_steps=max_num_integration_steps)
     position = state.position
     if position.dim() != 2:
         raise ValueError("sample() runs a batch of chains: position must be [C, D]")

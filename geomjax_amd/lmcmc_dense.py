@@ -114,15 +114,18 @@ def _noise(rng_key, C, D, dev):
 
 
 def transition(rng_key, state, logdensity_fn, step_size, metric_fn, num_integration_steps,
-               divergence_threshold=1000.0):
-    """One LMC transition (geomjax/lmcmc/lmc.py:135-180, 411-475) of every chain."""
+               divergence_threshold=1000.0, max_num_integration_steps=None):
+    """One LMC transition (geomjax/lmcmc/lmc.py:135-180, 411-475) of every chain;
+    ``num_integration_steps`` an int or a [C] tensor of per-chain counts (a chain
+    whose count is reached keeps its state while the others step on)."""
     from .lmcmc import IntegratorState, LMCInfo, LMCState
+    from .rmhmc_dense import _per_chain_steps
 
     q0 = state.position.contiguous()
     _check(q0)
     C, D = q0.shape
     dev = q0.device
-    L = int(num_integration_steps)
+    L, Ls = _per_chain_steps(num_integration_steps, C, dev, max_num_integration_steps)
     eps = float(step_size)
     l0 = state.logdensity.reshape(C).to(torch.float64)
     g0 = state.logdensity_grad.reshape(C, D).to(torch.float64).contiguous()
@@ -135,13 +138,22 @@ def transition(rng_key, state, logdensity_fn, step_size, metric_fn, num_integrat
     H0 = -l0 + (-0.5 * logdet + 0.5 * vgv0) - J0
 
     q, v, l, g, J = q0, v0, l0, g0, J0.clone()
-    for _ in range(L):
-        v = _half_step(G, dG, v, g, gld, eps, J)
-        q = q + eps * v
-        l, g = _value_and_grad(logdensity_fn, q)
-        G, dG = _metric(metric_fn, q)
-        logdet, gld, _, _ = _position(G, dG)
-        v = _half_step(G, dG, v, g, gld, eps, J)
+    for st in range(L):
+        Jn = J.clone()
+        vn = _half_step(G, dG, v, g, gld, eps, Jn)
+        qn = q + eps * vn
+        ln, gn = _value_and_grad(logdensity_fn, qn)
+        Gn, dGn = _metric(metric_fn, qn)
+        logdetn, gldn, _, _ = _position(Gn, dGn)
+        vn = _half_step(Gn, dGn, vn, gn, gldn, eps, Jn)
+        if Ls is None:
+            q, v, l, g, J, G, dG, logdet, gld = qn, vn, ln, gn, Jn, Gn, dGn, logdetn, gldn
+        else:
+            a = st < Ls
+            a1, a2, a3 = a[:, None], a[:, None, None], a[:, None, None, None]
+            q, v, g, gld = (torch.where(a1, x, y) for x, y in ((qn, q), (vn, v), (gn, g), (gldn, gld)))
+            l, J, logdet = (torch.where(a, x, y) for x, y in ((ln, l), (Jn, J), (logdetn, logdet)))
+            G, dG = torch.where(a2, Gn, G).contiguous(), torch.where(a3, dGn, dG).contiguous()
     _, _, _, vgv1 = _position(G, dG, velocity=v)
     H1 = -l + (-0.5 * logdet + 0.5 * vgv1) - J
     delta = H0 - H1
@@ -154,12 +166,12 @@ def transition(rng_key, state, logdensity_fn, step_size, metric_fn, num_integrat
     a2 = accept[:, None]
     new_state = LMCState(torch.where(a2, q, q0), torch.where(accept, l, l0), torch.where(a2, g, g0),
                          torch.where(accept, J, J0))
-    info = LMCInfo(v0, p_accept, accept, -delta > divergence_threshold, H1, proposal, L)
+    info = LMCInfo(v0, p_accept, accept, -delta > divergence_threshold, H1, proposal, L if Ls is None else Ls)
     return new_state, info
 
 
 def sample(rng_key, state, logdensity_fn, step_size, metric_fn, num_integration_steps, num_samples, *,
-           divergence_threshold=1000, store_samples=True, samples_out=None):
+           divergence_threshold=1000, store_samples=True, samples_out=None, max_num_integration_steps=None):
     """``num_samples`` transitions (one Python-driven loop of kernel launches)."""
     from .lmcmc import LMCState
     from .sampling import SampleResult
@@ -174,16 +186,19 @@ def sample(rng_key, state, logdensity_fn, step_size, metric_fn, num_integration_
         torch.zeros(T, C, D, dtype=torch.float64, device=dev) if store_samples else None)
     acc = torch.empty(T, C, dtype=torch.float32, device=dev)
     n_acc = torch.zeros(C, dtype=torch.int32, device=dev)
+    fev = torch.zeros(C, dtype=torch.int64, device=dev)
     s = LMCState(*state)
     for t in range(T):
         key = rng_key if isinstance(rng_key, PRNGKey) else HostNoise(rng_key.z[t], rng_key.u[t])
         if isinstance(key, PRNGKey):
             key = PRNGKey(key.seed, key.counter + t)
-        s, info = transition(key, s, logdensity_fn, step_size, metric_fn, num_integration_steps,
-                             divergence_threshold)
+        steps = num_integration_steps[t] if isinstance(num_integration_steps, torch.Tensor) else num_integration_steps
+        s, info = transition(key, s, logdensity_fn, step_size, metric_fn, steps, divergence_threshold,
+                             max_num_integration_steps)
         if samples is not None:
             samples[t].copy_(s.position)
         acc[t] = info.acceptance_rate.float()
         n_acc += info.is_accepted.int()
-    fev = torch.full((C,), T * int(num_integration_steps), dtype=torch.int64, device=dev)
+        n = info.num_integration_steps
+        fev += n.to(torch.int64) if isinstance(n, torch.Tensor) else n
     return SampleResult(s, samples, acc, n_acc, fev)

@@ -175,10 +175,9 @@ def build_kernel(integrator=implicit_midpoint, divergence_threshold: float = 100
             # a user metric_fn: torch.func derivatives + the dense-metric field kernel
             from . import rmhmc_dense
 
-            if not isinstance(num_integration_steps, int):
-                raise NotImplementedError("per-chain step counts need a closed-form metric kernel")
             new, info, fps = rmhmc_dense.transition(rng_key, state, logdensity_fn, step_size, metric_fn,
-                                                    num_integration_steps, solver, divergence_threshold)
+                                                    num_integration_steps, solver, divergence_threshold,
+                                                    max_num_integration_steps)
             if return_fixed_point_info:
                 C = state.position.shape[0]
                 if fps:
@@ -287,11 +286,10 @@ def sample(rng_key, state, logdensity_fn, step_size, metric_fn, num_integration_
     if is_dense_metric_fn(logdensity_fn, metric_fn):
         from . import rmhmc_dense
 
-        if not isinstance(num_integration_steps, int):
-            raise NotImplementedError("per-chain step counts need a closed-form metric kernel")
         return rmhmc_dense.sample(rng_key, state, logdensity_fn, step_size, metric_fn, num_integration_steps,
                                   num_samples, _solver_kwargs(integrator), divergence_threshold=divergence_threshold,
-                                  store_samples=store_samples, samples_out=samples_out)
+                                  store_samples=store_samples, samples_out=samples_out,
+                                  max_num_integration_steps=max_num_integration_steps)
 
     solver = _solver_kwargs(integrator)
     position, single = _as_batch(state.position)

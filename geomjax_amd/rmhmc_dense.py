@@ -95,9 +95,22 @@ def _implicit_midpoint_step(q0, p0, eps, logdensity_fn, metric_fn, solver):
     return q1, p1, (success, norm, n)
 
 
+def _per_chain_steps(num_integration_steps, C, dev, max_num_integration_steps):
+    """(L, per-chain counts or None): an int, or a [C] integer tensor (the dynamic
+    kernels' form) clamped to ``max_num_integration_steps`` when given."""
+    if isinstance(num_integration_steps, torch.Tensor):
+        Ls = num_integration_steps.to(device=dev, dtype=torch.int64).reshape(C)
+        L = int(max_num_integration_steps) if max_num_integration_steps is not None else int(Ls.max().item())
+        L = max(L, 0)
+        return L, Ls.clamp(0, L)
+    return int(num_integration_steps), None
+
+
 def transition(rng_key, state, logdensity_fn, step_size, metric_fn, num_integration_steps, solver,
-               divergence_threshold=1000.0):
-    """One RMHMC transition (geomjax/rmhmc/rmhmc.py:131-174, 376-462) of every chain.
+               divergence_threshold=1000.0, max_num_integration_steps=None):
+    """One RMHMC transition (geomjax/rmhmc/rmhmc.py:131-174, 376-462) of every chain;
+    ``num_integration_steps`` an int or a [C] tensor of per-chain counts (a chain
+    whose count is reached keeps its state while the others step on).
     Returns (RMHMCState, RMHMCInfo, per-step FixedPointIterationInfo list)."""
     from .proposal import Proposal
     from .rmhmc import IntegratorState, RMHMCInfo, RMHMCState
@@ -106,7 +119,7 @@ def transition(rng_key, state, logdensity_fn, step_size, metric_fn, num_integrat
     _check(q0)
     C, D = q0.shape
     dev = q0.device
-    L = int(num_integration_steps)
+    L, Ls = _per_chain_steps(num_integration_steps, C, dev, max_num_integration_steps)
     eps = float(step_size)
     l0 = state.logdensity.reshape(C).to(torch.float64)
     g0 = state.logdensity_grad.reshape(C, D).to(torch.float64).contiguous()
@@ -116,12 +129,22 @@ def transition(rng_key, state, logdensity_fn, step_size, metric_fn, num_integrat
     H0 = -l0 + _kinetic(logdet0, pu0, D)
     q, p, l, g = q0, p0, l0, g0
     infos = []
-    for _ in range(L):
-        q, p, info = _implicit_midpoint_step(q, p, eps, logdensity_fn, metric_fn, solver)
+    for st in range(L):
+        qn, pn, info = _implicit_midpoint_step(q, p, eps, logdensity_fn, metric_fn, solver)
+        if Ls is None:
+            q, p = qn, pn
+        else:
+            act = st < Ls
+            q, p = torch.where(act[:, None], qn, q), torch.where(act[:, None], pn, p)
+            info = (info[0] & act, torch.where(act, info[1], torch.zeros_like(info[1])),
+                    torch.where(act, info[2], torch.zeros_like(info[2])))
         infos.append(info)
     if L > 0:
         l, g = _value_and_grad(logdensity_fn, q)
         G, dG = _metric(metric_fn, q)
+        if Ls is not None:  # chains that took no step keep the state's own values
+            moved = Ls > 0
+            l, g = torch.where(moved, l, l0), torch.where(moved[:, None], g, g0)
     _, v, _, logdet1, pu1 = _field(G, None, p=p)
     H1 = -l + _kinetic(logdet1, pu1, D)
     delta = H0 - H1
@@ -132,12 +155,12 @@ def transition(rng_key, state, logdensity_fn, step_size, metric_fn, num_integrat
     proposal = Proposal(end, H1, delta, torch.minimum(delta, torch.zeros_like(delta)))
     a1 = accept[:, None]
     new_state = RMHMCState(torch.where(a1, q, q0), torch.where(accept, l, l0), torch.where(a1, g, g0))
-    info = RMHMCInfo(p0, p_accept, accept, -delta > divergence_threshold, H1, proposal, L)
+    info = RMHMCInfo(p0, p_accept, accept, -delta > divergence_threshold, H1, proposal, L if Ls is None else Ls)
     return new_state, info, infos
 
 
 def sample(rng_key, state, logdensity_fn, step_size, metric_fn, num_integration_steps, num_samples, solver, *,
-           divergence_threshold=1000, store_samples=True, samples_out=None):
+           divergence_threshold=1000, store_samples=True, samples_out=None, max_num_integration_steps=None):
     """``num_samples`` transitions (a Python-driven loop; capture it with
     ``GraphSampler`` to remove the host dispatch)."""
     from .rmhmc import RMHMCState
@@ -158,12 +181,16 @@ def sample(rng_key, state, logdensity_fn, step_size, metric_fn, num_integration_
     for t in range(T):
         key = PRNGKey(rng_key.seed, rng_key.counter + t) if isinstance(rng_key, PRNGKey) else \
             HostNoise(rng_key.z[t], rng_key.u[t])
-        s, info, fp = transition(key, s, logdensity_fn, step_size, metric_fn, num_integration_steps, solver,
-                                 divergence_threshold)
+        steps = num_integration_steps[t] if isinstance(num_integration_steps, torch.Tensor) else num_integration_steps
+        s, info, fp = transition(key, s, logdensity_fn, step_size, metric_fn, steps, solver, divergence_threshold,
+                                 max_num_integration_steps)
         if samples is not None:
             samples[t].copy_(s.position)
         acc[t] = info.acceptance_rate.float()
         n_acc += info.is_accepted.int()
+        # per step: the first evaluation, the n iterations, the explicit half step
         for (_, _, n) in fp:
-            fev += n + 2  # the first evaluation, the n iterations, the explicit half step
+            fev += n
+        fev += 2 * (info.num_integration_steps if isinstance(info.num_integration_steps, torch.Tensor)
+                    else torch.full_like(fev, info.num_integration_steps))
     return SampleResult(s, samples, acc, n_acc, fev)

@@ -114,3 +114,21 @@ def test_dense_sampler_api_runs_with_device_draws():
     assert res.samples.shape == (5, 32, 10)
     assert torch.isfinite(res.samples).all()
     assert 0.0 < float(res.acceptance_rate.mean()) <= 1.0
+
+
+def test_per_chain_step_counts_match_the_oracle_chain_by_chain():
+    D, C, eps = 10, 3, 0.05
+    metric_fn = monge_metric(1.0)
+    q0 = 0.5 * seeded_normal(60, C, D)
+    z, u = seeded_normal(61, C, D), seeded_uniform(62, C)
+    counts = torch.tensor([3, 0, 5])
+    s1, info = lmc_kernel()(gr.HostNoise(z.to(DEV), u.to(DEV)), lmc_init(q0.to(DEV), funnel_logp), funnel_logp, eps,
+                            metric_fn, counts.to(DEV), max_num_integration_steps=5)
+    torch.cuda.synchronize()
+    for c in range(C):
+        so1, oi = olc.build_kernel()((z[c:c + 1], u[c:c + 1]), olc.init(q0[c:c + 1], funnel_logp), funnel_logp, eps,
+                                     metric_fn, int(counts[c]))
+        assert bool(info.is_accepted[c]) == bool(oi.is_accepted[0]), c
+        assert rel_err(s1.position[c:c + 1], so1.position) < 1e-10, c
+        assert abs(float(s1.volume_adjustment[c]) - float(so1.volume_adjustment[0])) < 1e-10, c
+        assert rel_err(info.acceptance_rate[c:c + 1], oi.acceptance_rate) < 1e-10, c

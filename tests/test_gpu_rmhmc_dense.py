@@ -117,3 +117,34 @@ def test_dense_rmhmc_sampler_api_and_graph_capture():
     assert torch.isfinite(res.samples).all()
     assert rel_err(gs.samples, eager.samples) <= 1e-12
     assert torch.equal(gs.acceptance_rate, eager.acceptance_rate)
+
+
+def test_per_chain_step_counts_match_the_oracle_chain_by_chain():
+    """Dynamic form: each chain with its own count equals the oracle run alone
+    with that count (a count of 0 returns the state itself)."""
+    D, C, eps = 8, 3, 0.05
+    metric_fn = monge(1.0)
+    q0 = 0.5 * seeded_normal(60, C, D)
+    z, u = seeded_normal(61, C, D), seeded_uniform(62, C)
+    counts = torch.tensor([0, 2, 4])
+    kern = build_kernel(functools.partial(implicit_midpoint, max_iters=8))
+    s1, info = kern(gr.HostNoise(z.to(DEV), u.to(DEV)), rm_init(q0.to(DEV), funnel_logp), funnel_logp, eps,
+                    metric_fn, counts.to(DEV), max_num_integration_steps=4)
+    torch.cuda.synchronize()
+    assert info.num_integration_steps.cpu().tolist() == counts.tolist()
+    okern = orm.build_kernel(max_iters=8)
+    for c in range(C):
+        so = orm.init(q0[c:c + 1], funnel_logp)
+        so1, oi = okern((z[c:c + 1], u[c:c + 1]), so, funnel_logp, eps, metric_fn, int(counts[c]))
+        assert bool(info.is_accepted[c]) == bool(oi.is_accepted[0]), c
+        assert rel_err(s1.position[c:c + 1], so1.position) < 1e-10, c
+        assert rel_err(info.acceptance_rate[c:c + 1], oi.acceptance_rate) < 1e-10, c
+
+
+def test_dynamic_rmhmc_api_with_a_user_metric():
+    alg = gj.dynamic_rmhmc(funnel_logp, 0.05, monge(1.0), integrator=functools.partial(implicit_midpoint, max_iters=6))
+    s = alg.init(0.5 * seeded_normal(63, 16, 8).to(DEV), torch.arange(16, device=DEV))
+    s, info = alg.step(gr.key(4), s)
+    torch.cuda.synchronize()
+    assert info.num_integration_steps.shape == (16,)
+    assert torch.isfinite(s.position).all()
