@@ -148,3 +148,20 @@ def test_dynamic_rmhmc_api_with_a_user_metric():
     torch.cuda.synchronize()
     assert info.num_integration_steps.shape == (16,)
     assert torch.isfinite(s.position).all()
+
+
+def test_chees_adaptation_runs_with_a_user_metric():
+    """The ChEES driver (dual averaging + Adam on log T) over the dense path:
+    finite parameters, and the adapted dynamic sampler steps."""
+    from geomjax_amd.adaptation import adam, chees_adaptation
+
+    C = 64
+    integ = functools.partial(implicit_midpoint, max_iters=6)
+    warm = chees_adaptation(funnel_logp, monge(1.0), C, integrator=integ)
+    q0 = (0.5 * seeded_normal(64, C, 8)).to(DEV)
+    (last, params), hist = warm.run(gr.key(5), q0, 0.1, adam(0.05), 8)
+    assert math.isfinite(params["step_size"]) and params["step_size"] > 0
+    alg = gj.dynamic_rmhmc(funnel_logp, **params)
+    s, info = alg.step(gr.key(6), last)
+    torch.cuda.synchronize()
+    assert torch.isfinite(s.position).all()
