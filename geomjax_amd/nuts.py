@@ -55,13 +55,47 @@ class NUTSInfo(NamedTuple):  # geomjax/rmhmc/nuts.py:36-75 (same fields and orde
 
 
 def build_kernel(integrator=implicit_midpoint, divergence_threshold: int = 1000):
-    """geomjax/rmhmc/nuts.py:78-161."""
+    """geomjax/rmhmc/nuts.py:78-161.  Plain callables (a user log density and
+    dense metric) run the batched dense-metric NUTS of ``nuts_dense``."""
     solver = _solver_kwargs(integrator)
+
+    def _dense_kernel(rng_key, state, logdensity_fn, step_size, metric_fn, max_num_doublings, stopping_criterion):
+        from . import nuts_dense
+        from .rmhmc import RMHMCState
+
+        single = state.position.dim() == 1
+        st = RMHMCState(*((t[None] if single else t) for t in state)) if single else state
+        C, D = st.position.shape
+        dev = st.position.device
+        M = int(max_num_doublings)
+        if isinstance(rng_key, NUTSNoise):
+            f = lambda t, *shape: t.reshape(*shape).to(device=dev, dtype=torch.float64).contiguous()
+            noise = (f(rng_key.z, C, D), f(rng_key.u_direction, C, M), f(rng_key.u_merge, C, M),
+                     f(rng_key.u_step, C, 2 ** M - 1))
+        elif isinstance(rng_key, PRNGKey):
+            gen = torch.Generator(device=dev)
+            gen.manual_seed((int(rng_key.seed) * 0x9E3779B1 + int(rng_key.counter)) & 0x7FFFFFFFFFFFFFFF)
+            noise = (torch.randn(C, D, generator=gen, dtype=torch.float64, device=dev),
+                     *(torch.rand(C, n, generator=gen, dtype=torch.float64, device=dev) for n in (M, M, 2 ** M - 1)))
+        else:
+            raise TypeError("rng_key must be NUTSNoise or PRNGKey")
+        q, l, g, inf = nuts_dense.transition(noise, st, logdensity_fn, step_size, metric_fn, solver, M,
+                                             stopping_criterion, divergence_threshold)
+        sq = (lambda t: t[0]) if single else (lambda t: t)
+        info = NUTSInfo(sq(inf["momentum"]), sq(inf["is_divergent"]), sq(inf["is_turning"]), sq(inf["energy"]),
+                        sq(inf["left"]), sq(inf["right"]), sq(inf["num_trajectory_expansions"]),
+                        sq(inf["num_integration_steps"]), sq(inf["acceptance_rate"]))
+        return RMHMCState(sq(q), sq(l), sq(g)), info
 
     def kernel(rng_key, state, logdensity_fn, step_size, metric_fn, max_num_doublings: int = 10,
                stopping_criterion: str = "euc"):
         if stopping_criterion not in _lib.GJX_UTURN:
             raise ValueError(f"stopping_criterion must be one of {sorted(_lib.GJX_UTURN)} (got {stopping_criterion!r})")
+        from .lmcmc_dense import is_dense_metric_fn
+
+        if is_dense_metric_fn(logdensity_fn, metric_fn):
+            return _dense_kernel(rng_key, state, logdensity_fn, step_size, metric_fn, max_num_doublings,
+                                 stopping_criterion)
         funnel = isinstance(logdensity_fn, NealFunnel) and isinstance(metric_fn, MongeMetric)
         logistic = isinstance(logdensity_fn, (LogisticRegression, LogisticBatch)) and isinstance(metric_fn, FisherMetric)
         if not (funnel or logistic):

@@ -61,8 +61,9 @@ def _kinetic(logdet, pu, D):
 
 def _implicit_midpoint_step(q0, p0, eps, logdensity_fn, metric_fn, solver):
     """geomjax/rmhmc/integrators.py:92-156 for every chain: the fixed point of
-    x = z0 + (eps/2) F(x), then one explicit half step from the last iterate."""
-    h = 0.5 * eps
+    x = z0 + (eps/2) F(x), then one explicit half step from the last iterate.
+    ``eps`` is a float or a [C] tensor of per-chain signed step sizes."""
+    h = (0.5 * eps)[:, None] if torch.is_tensor(eps) else 0.5 * eps
 
     def F(q, p, initial):
         _, g = _value_and_grad(logdensity_fn, q)
