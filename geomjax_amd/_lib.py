@@ -188,7 +188,7 @@ class DenseMetricIO(ctypes.Structure):
     _fields_ = [("num_chains", _c_int32), ("dim", _c_int32), ("step_size", _c_double)] + [(n, _vp) for n in (
         "metric", "metric_jacobian", "z", "velocity", "logdensity_grad", "grad_logdet_in", "velocity_out",
         "logdet", "grad_logdet", "vgv", "volume_adjustment", "gamma", "drift", "chol_inv_z", "momentum",
-        "momentum_out", "dtdq", "pu",
+        "momentum_out", "dtdq", "pu", "step_sizes",
     )]
 
 

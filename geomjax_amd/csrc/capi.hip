@@ -782,6 +782,7 @@ static int dense_args(const gjx_dense_metric_io* io, gjx::DenseMetricArgs* a, co
   a->momentum_out = io->momentum_out;
   a->dtdq = io->dtdq;
   a->pu = io->pu;
+  a->step_sizes = io->step_sizes;
   return GJX_OK;
 }
 

@@ -11,6 +11,7 @@ constexpr int kDenseMaxD = 64;
 struct DenseMetricArgs {
   int C, D;
   double step_size;
+  const double* step_sizes;  // [C] per-chain signed step sizes (half step), or null: step_size
   const double* metric;      // [C, D, D] G(q) as the metric function returns it
   const double* metric_jac;  // [C, D, D, D]: dG_ab/dq_i at [c][a][b][i]
   const double* z;           // [C, D] standard normals of the velocity draw, or null

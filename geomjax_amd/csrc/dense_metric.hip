@@ -242,7 +242,7 @@ __global__ void __launch_bounds__(NT) dense_lmc_half_step_kernel(const DenseMetr
   const int c = blockIdx.x;
   if (c >= a.C) return;
   const int t = threadIdx.x, D = a.D, DD = D * D;
-  const double eps = a.step_size;
+  const double eps = a.step_sizes ? a.step_sizes[c] : a.step_size;
   const double* G = a.metric + (size_t)c * DD;
   const double* dG = a.metric_jac + (size_t)c * DD * D;
   for (int ij = t; ij < DD; ij += NT) S.W[(ij / D) * LDA + ij % D] = G[ij];

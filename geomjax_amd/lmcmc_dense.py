@@ -79,12 +79,37 @@ def _position(G, dG, z=None, velocity=None):
 
 
 def _half_step(G, dG, v, g, gld, eps, J):
-    """One half step; returns the new velocity, J updated in place."""
+    """One half step (``eps`` a float or a [C] tensor of signed step sizes);
+    returns the new velocity, J updated in place."""
     out = torch.empty_like(v)
-    io = _io(G, dG, step_size=eps, velocity=v, logdensity_grad=g, grad_logdet_in=gld, velocity_out=out,
-             volume_adjustment=J)
+    if torch.is_tensor(eps):
+        io = _io(G, dG, step_size=0.0, step_sizes=eps.to(torch.float64).contiguous(), velocity=v,
+                 logdensity_grad=g, grad_logdet_in=gld, velocity_out=out, volume_adjustment=J)
+    else:
+        io = _io(G, dG, step_size=eps, velocity=v, logdensity_grad=g, grad_logdet_in=gld, velocity_out=out,
+                 volume_adjustment=J)
     _lib.check(_lib.load().gjx_dense_lmc_half_step(io, _lib.stream_handle(v.device)), "gjx_dense_lmc_half_step")
     return out
+
+
+def lmc_step(state, eps, logdensity_fn, metric_fn):
+    """One lan_integrator step (integrators.py:93-142) of every chain from the tuple
+    (q, momentum, velocity, l, g, J), ``eps`` a [C] tensor of signed step sizes;
+    returns the new tuple and its energy -l - log det G / 2 + v^T G v / 2 - J
+    (metrics.py:209-221)."""
+    q, _, v, l, g, J = state
+    G, dG = _metric(metric_fn, q)
+    _, gld, _, _ = _position(G, dG)
+    J = J.clone()
+    v = _half_step(G, dG, v, g, gld, eps, J)
+    q = q + eps[:, None] * v
+    l, g = _value_and_grad(logdensity_fn, q)
+    G, dG = _metric(metric_fn, q)
+    logdet, gld, _, _ = _position(G, dG)
+    v = _half_step(G, dG, v, g, gld, eps, J)
+    _, _, _, vgv = _position(G, dG, velocity=v)
+    momentum = torch.bmm(G, v[:, :, None])[:, :, 0]
+    return (q, momentum, v, l, g, J), -l + (-0.5 * logdet + 0.5 * vgv) - J
 
 
 def init(position, logdensity_fn):

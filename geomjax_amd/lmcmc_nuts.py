@@ -50,6 +50,11 @@ def build_kernel(integrator=lan_integrator, divergence_threshold: int = 1000):
                stopping_criterion: str = "euc"):
         if stopping_criterion not in _lib.GJX_UTURN:
             raise ValueError(f"stopping_criterion must be one of {sorted(_lib.GJX_UTURN)} (got {stopping_criterion!r})")
+        from .lmcmc_dense import is_dense_metric_fn
+
+        if is_dense_metric_fn(logdensity_fn, metric_fn):
+            return _dense(rng_key, state, logdensity_fn, step_size, metric_fn, max_num_doublings, stopping_criterion,
+                          divergence_threshold)
         single = state.position.dim() == 1
         position = (state.position[None] if single else state.position).contiguous()
         if position.dtype != torch.float64:
@@ -96,6 +101,37 @@ def build_kernel(integrator=lan_integrator, divergence_threshold: int = 1000):
         return new_state, info
 
     return kernel
+
+
+def _dense(rng_key, state, logdensity_fn, step_size, metric_fn, max_num_doublings, stopping_criterion,
+           divergence_threshold):
+    """A user dense metric: the batched expansion of ``nuts_dense`` over the
+    dense-metric LMC steps."""
+    from . import nuts_dense
+
+    single = state.position.dim() == 1
+    st = LMCState(*((torch.as_tensor(t)[None] if single else t) for t in state))
+    C, D = st.position.shape
+    dev = st.position.device
+    M = int(max_num_doublings)
+    if isinstance(rng_key, NUTSNoise):
+        f = lambda t, *shape: t.reshape(*shape).to(device=dev, dtype=torch.float64).contiguous()
+        noise = (f(rng_key.z, C, D), f(rng_key.u_direction, C, M), f(rng_key.u_merge, C, M),
+                 f(rng_key.u_step, C, 2 ** M - 1))
+    elif isinstance(rng_key, PRNGKey):
+        gen = torch.Generator(device=dev)
+        gen.manual_seed((int(rng_key.seed) * 0x9E3779B1 + int(rng_key.counter)) & 0x7FFFFFFFFFFFFFFF)
+        noise = (torch.randn(C, D, generator=gen, dtype=torch.float64, device=dev),
+                 *(torch.rand(C, n, generator=gen, dtype=torch.float64, device=dev) for n in (M, M, 2 ** M - 1)))
+    else:
+        raise TypeError("rng_key must be NUTSNoise or PRNGKey")
+    q, l, g, J, inf = nuts_dense.lmc_transition(noise, st, logdensity_fn, step_size, metric_fn, M,
+                                                stopping_criterion, divergence_threshold)
+    sq = (lambda t: t[0]) if single else (lambda t: t)
+    info = NUTSInfo(sq(inf["momentum"]), sq(inf["is_divergent"]), sq(inf["is_turning"]), sq(inf["energy"]),
+                    sq(inf["left"]), sq(inf["right"]), sq(inf["num_trajectory_expansions"]),
+                    sq(inf["num_integration_steps"]), sq(inf["acceptance_rate"]))
+    return LMCState(sq(q), sq(l), sq(g), sq(J)), info
 
 
 class nuts:

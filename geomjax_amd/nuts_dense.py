@@ -19,7 +19,7 @@ import torch
 from .lmcmc_dense import _check, _metric, _value_and_grad
 from .rmhmc_dense import _field, _implicit_midpoint_step, _kinetic
 
-__all__ = ["transition"]
+__all__ = ["transition", "expand", "lmc_transition"]
 
 _CRITERIA = ("euc", "bet", "riem")
 
@@ -55,30 +55,43 @@ def _ckpt_idxs(n):
 
 def transition(noise, state, logdensity_fn, step_size, metric_fn, solver, max_num_doublings=10,
                stopping_criterion="euc", divergence_threshold=1000.0):
-    """One NUTS transition of every chain.  ``noise`` = (z [C, D], u_direction
-    [C, M], u_merge [C, M], u_step [C, 2^M - 1]).  Returns (position, logdensity,
-    grad, info dict)."""
-    if stopping_criterion not in _CRITERIA:
-        raise ValueError(f"stopping_criterion must be one of {list(_CRITERIA)} (got {stopping_criterion!r})")
+    """One NUTS transition of every chain with the RMHMC implicit midpoint.
+    ``noise`` = (z [C, D], u_direction [C, M], u_merge [C, M], u_step [C, 2^M - 1]).
+    Returns (position, logdensity, grad, info dict)."""
     q0 = state.position.contiguous()
     _check(q0)
     C, D = q0.shape
-    dev = q0.device
-    M = int(max_num_doublings)
-    eps = float(step_size)
-    z, u_dir, u_merge, u_step = noise
+    z = noise[0]
     l0 = state.logdensity.reshape(C).to(torch.float64)
     g0 = state.logdensity_grad.reshape(C, D).to(torch.float64).contiguous()
     G, _ = _metric(metric_fn, q0)
     p0, v0, _, logdet0, pu0 = _field(G, None, z=z)
     H0 = -l0 + _kinetic(logdet0, pu0, D)
 
-    def energy_state(q, p):
-        """(v, l, g, energy) at a new integrator state (integrators.py:150-154)."""
-        l, g = _value_and_grad(logdensity_fn, q)
-        Gq, _ = _metric(metric_fn, q)
-        _, v, _, logdet, pu = _field(Gq, None, p=p)
-        return v, l, g, -l + _kinetic(logdet, pu, D)
+    def step(cur, eps_vec):
+        qn, pn, _ = _implicit_midpoint_step(cur[0], cur[1], eps_vec, logdensity_fn, metric_fn, solver)
+        ln, gn = _value_and_grad(logdensity_fn, qn)
+        Gq, _ = _metric(metric_fn, qn)
+        _, vn, _, logdet, pu = _field(Gq, None, p=pn)
+        return (qn, pn, vn, ln, gn), -ln + _kinetic(logdet, pu, D)
+
+    P_s, info = expand((q0, p0, v0, l0, g0), H0, step, float(step_size), noise, max_num_doublings,
+                       stopping_criterion, divergence_threshold)
+    info["momentum"] = p0
+    return P_s[0], P_s[3], P_s[4], info
+
+
+def expand(S0, H0, step, eps, noise, max_num_doublings, stopping_criterion, divergence_threshold):
+    """The batched iterative expansion from the state tuple S0 = (position,
+    momentum, velocity, ...) with energies H0 [C]; ``step(cur, signed_eps [C])``
+    returns (new tuple, its energy).  Returns (proposal tuple, info dict)."""
+    if stopping_criterion not in _CRITERIA:
+        raise ValueError(f"stopping_criterion must be one of {list(_CRITERIA)} (got {stopping_criterion!r})")
+    q0, p0, v0 = S0[0], S0[1], S0[2]
+    C, D = q0.shape
+    dev = q0.device
+    M = int(max_num_doublings)
+    _, u_dir, u_merge, u_step = noise
 
     def weight(E):
         w = H0 - E
@@ -86,7 +99,6 @@ def transition(noise, state, logdensity_fn, step_size, metric_fn, solver, max_nu
 
     ninf = torch.full((C,), -math.inf, dtype=torch.float64, device=dev)
     zeros = torch.zeros(C, dtype=torch.float64, device=dev)
-    S0 = (q0, p0, v0, l0, g0)
     # proposal: (state, energy, weight, sum_log_p_accept)
     P_s, P_e, P_w, P_a = S0, H0.clone(), zeros.clone(), ninf.clone()
     left, right = S0, S0
@@ -116,16 +128,15 @@ def transition(noise, state, logdensity_fn, step_size, metric_fn, solver, max_nu
         sub_active = active.clone()
         div_s = torch.zeros_like(done)
         term_s = torch.zeros_like(done)
-        for step in range(2 ** e):
+        for k in range(2 ** e):
             if not capturing and not bool(sub_active.any()):
                 break
-            qn, pn, _ = _implicit_midpoint_step(cur[0], cur[1], sgn * eps, logdensity_fn, metric_fn, solver)
-            vn, ln, gn, En = energy_state(qn, pn)
-            new = (qn, pn, vn, ln, gn)
+            new, En = step(cur, sgn * eps)
+            pn, vn = new[1], new[2]
             wn = weight(En)
             an = torch.minimum(wn, zeros)
             div_now = -wn > divergence_threshold
-            if step == 0:
+            if k == 0:
                 n_l, n_r, n_m, n_v, n_n = new, new, pn.clone(), vn.clone(), torch.ones_like(s_n)
                 nP = (new, En, wn, an)
             else:
@@ -144,8 +155,8 @@ def transition(noise, state, logdensity_fn, step_size, metric_fn, solver, max_nu
             sP_e, sP_w, sP_a = (torch.where(sa, x, y) for x, y in ((nP[1], sP_e), (nP[2], sP_w), (nP[3], sP_a)))
             j = j + sa.to(torch.int64)
             # iterative U-turn checks (termination.py:30-69, 89-117)
-            i_min, i_max = _ckpt_idxs(step)
-            if step % 2 == 0:
+            i_min, i_max = _ckpt_idxs(k)
+            if k % 2 == 0:
                 m3 = sa[:, None]
                 r_ck[:, i_max] = torch.where(m3, pn, r_ck[:, i_max])
                 rs_ck[:, i_max] = torch.where(m3, s_msum, rs_ck[:, i_max])
@@ -183,7 +194,38 @@ def transition(noise, state, logdensity_fn, step_size, metric_fn, solver, max_nu
         turning = torch.where(a, term_s | turn_full, turning)
         done = done | (a & (div_s | term_s | turn_full))
     acceptance = torch.exp(P_a) / nstates.to(torch.float64)
-    info = dict(momentum=p0, is_divergent=diverging, is_turning=turning, energy=P_e, left=left[0], right=right[0],
+    info = dict(is_divergent=diverging, is_turning=turning, energy=P_e, left=left[0], right=right[0],
                 num_trajectory_expansions=expansions.to(torch.int32), num_integration_steps=nstates.to(torch.int32),
                 acceptance_rate=acceptance)
-    return P_s[0], P_s[3], P_s[4], info
+    return P_s, info
+
+
+def lmc_transition(noise, state, logdensity_fn, step_size, metric_fn, max_num_doublings=10, stopping_criterion="euc",
+                   divergence_threshold=1000.0):
+    """One NUTS transition with the general-metric LMC integrator
+    (geomjax/lmcmc/nuts.py:78-161): velocity v0 = L^-T z, momentum G v0, the
+    energy -l - log det G / 2 + v^T G v / 2 - J carried along the trajectory.
+    Returns (position, logdensity, grad, volume_adjustment, info dict)."""
+    from .lmcmc_dense import _position, lmc_step
+
+    q0 = state.position.contiguous()
+    _check(q0)
+    C, D = q0.shape
+    dev = q0.device
+    z = noise[0]
+    l0 = state.logdensity.reshape(C).to(torch.float64)
+    g0 = state.logdensity_grad.reshape(C, D).to(torch.float64).contiguous()
+    J0 = torch.as_tensor(state.volume_adjustment, dtype=torch.float64, device=dev).expand(C).clone()
+    G, dG = _metric(metric_fn, q0)
+    logdet0, _, v0, _ = _position(G, dG, z=z)
+    _, _, _, vgv0 = _position(G, dG, velocity=v0)
+    m0 = torch.bmm(G, v0[:, :, None])[:, :, 0]
+    H0 = -l0 + (-0.5 * logdet0 + 0.5 * vgv0) - J0
+
+    def step(cur, eps_vec):
+        return lmc_step(cur, eps_vec, logdensity_fn, metric_fn)
+
+    P_s, info = expand((q0, m0, v0, l0, g0, J0), H0, step, float(step_size), noise, max_num_doublings,
+                       stopping_criterion, divergence_threshold)
+    info["momentum"] = v0  # NUTSInfo's first field holds the initial velocity (lmcmc/nuts.py:36-77)
+    return P_s[0], P_s[3], P_s[4], P_s[5], info

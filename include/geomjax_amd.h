@@ -498,6 +498,7 @@ typedef struct {
   double* momentum_out;             /* rmhmc: [C, dim] the p used, optional */
   double* dtdq;                     /* rmhmc: [C, dim] dT/dq (needs metric_jacobian), optional */
   double* pu;                       /* rmhmc: [C] p^T G^-1 p, optional */
+  const double* step_sizes;         /* half step: [C] per-chain signed step sizes, or NULL: step_size */
 } gjx_dense_metric_io;
 
 /* Position quantities of a dense metric (geomjax/lmcmc/metrics.py:75-112,181-189):

@@ -78,3 +78,39 @@ def test_dense_nuts_api_with_device_draws():
     assert torch.isfinite(s.position).all()
     steps = info.num_integration_steps.cpu()
     assert (steps >= 1).all() and (steps <= 15).all()
+
+
+@pytest.mark.parametrize("crit", ["euc", "riem"])
+def test_dense_lmc_nuts_matches_oracle(crit):
+    """NUTS with the general-metric LMC integrator and a user metric (funnel,
+    Monge metric by autodiff) against oracle/nuts.py's lmcmc expansion."""
+    from geomjax_amd.lmcmc import LMCState
+    from geomjax_amd.lmcmc import init as lmc_init
+    from geomjax_amd.lmcmc_nuts import build_kernel as lnuts_kernel
+    from oracle import lmcmc as olc
+
+    D, C, eps, M = 6, 3, 0.2, 4
+    print(f"case dense lmc-nuts D={D} C={C} eps={eps} M={M} criterion={crit}", flush=True)
+    q0 = 0.5 * seeded_normal(70, C, D)
+    so = olc.init(q0, funnel_logp)
+    s = lmc_init(q0.to(DEV), funnel_logp)
+    okern = onuts.build_lmcmc_kernel()
+    for t in range(2):
+        noise = (seeded_normal(71 + t, C, D), seeded_uniform(73 + t, C, M), seeded_uniform(75 + t, C, M),
+                 seeded_uniform(77 + t, C, 2 ** M - 1))
+        s1, info = lnuts_kernel()(NUTSNoise(*(x.to(DEV) for x in noise)), s, funnel_logp, eps, monge, M,
+                                  stopping_criterion=crit)
+        so1, oi = okern(noise, so, funnel_logp, eps, monge, max_num_doublings=M, stopping_criterion=crit)
+        torch.cuda.synchronize()
+        print(f"  t={t}: steps {[o.num_integration_steps for o in oi]}", flush=True)
+        assert info.num_integration_steps.cpu().tolist() == [o.num_integration_steps for o in oi]
+        assert info.is_turning.cpu().tolist() == [o.is_turning for o in oi]
+        assert info.is_divergent.cpu().tolist() == [o.is_divergent for o in oi]
+        for f in ("position", "logdensity", "logdensity_grad"):
+            assert rel_err(getattr(s1, f), getattr(so1, f)) < 1e-10, f
+        assert (s1.volume_adjustment.cpu() - so1.volume_adjustment).abs().max() < 1e-10 * max(
+            1.0, so1.volume_adjustment.abs().max().item())
+        assert rel_err(info.acceptance_rate, torch.tensor([o.acceptance_rate for o in oi], dtype=torch.float64)) < 1e-10
+        assert rel_err(info.velocity, torch.stack([o.momentum for o in oi])) < 1e-10
+        so = so1
+        s = LMCState(*(x.to(DEV) for x in so))
