@@ -23,19 +23,22 @@ namespace gjx {
 namespace {
 
 constexpr int NT = 256;
-constexpr int MAXD = kDenseMaxD;
-constexpr int LDA = MAXD + 1;
 
-struct Smem {
-  double A[MAXD * LDA];  // the matrix being factored
-  double W[MAXD * LDA];  // G (half step) / L^-1, then G^-1 (position)
-  double T[MAXD * LDA];  // the contraction T (half step)
-  double v[MAXD], rhs[MAXD], x[MAXD];
-  double part[4][MAXD];
+// LDS of one chain for a capacity MD (32 or 64): a D <= 32 problem takes a
+// quarter of the LDS, so four times as many chains are resident per CU.
+template <int MD>
+struct SmemT {
+  static constexpr int LDA = MD + 1;
+  double A[MD * LDA];  // the matrix being factored
+  double W[MD * LDA];  // G (half step) / L^-1, then G^-1 (position)
+  double T[MD * LDA];  // the contraction T (half step)
+  double v[MD], rhs[MD], x[MD];
+  double part[4][64];
   double red[8];
   int piv;
 };
 
+template <class Smem>
 __device__ __forceinline__ double block_sum(Smem& S, double v) {
   v = group_sum<64>(v);
   const int w = threadIdx.x / 64, l = threadIdx.x % 64;
@@ -49,7 +52,9 @@ __device__ __forceinline__ double block_sum(Smem& S, double v) {
 // coef = step / 4 (the reference's metric + 0.5 step * 0.5 (P1 + P2 - P3)).
 // U is formed in S.A and overwritten in place pair by pair: the lane that owns
 // {(l, j), (j, l)} reads both U entries before writing either.
+template <class Smem>
 __device__ void omega_into_A(Smem& S, const double* __restrict__ dG, int D, double coef) {
+  constexpr int LDA = Smem::LDA;
   const int t = threadIdx.x;
   const int DD = D * D;
   for (int ab = t; ab < DD; ab += NT) {  // T_ab = sum_i v_i dG[a][b][i]
@@ -78,7 +83,9 @@ __device__ void omega_into_A(Smem& S, const double* __restrict__ dG, int D, doub
 // LU with partial pivoting of the leading D x D block of S.A, the forward
 // elimination applied to S.rhs when solve is set; then S.x = A^-1 rhs.
 // Returns log |det A|.  Every lane leaves after the same D steps.
+template <class Smem>
 __device__ double lu_logdet_solve(Smem& S, int D, bool solve) {
+  constexpr int LDA = Smem::LDA;
   const int t = threadIdx.x;
   double logdet = 0.0;
   for (int k = 0; k < D; ++k) {
@@ -143,7 +150,9 @@ __device__ double lu_logdet_solve(Smem& S, int D, bool solve) {
 // Right-looking Cholesky of the symmetric part of S.A (lower, in place).
 // Returns log det.  A non-positive pivot gives NaN (as the reference's failed
 // factorisation does), never a hang.
+template <class Smem>
 __device__ double cholesky(Smem& S, int D) {
+  constexpr int LDA = Smem::LDA;
   const int t = threadIdx.x;
   double logdet = 0.0;
   for (int k = 0; k < D; ++k) {
@@ -163,7 +172,10 @@ __device__ double cholesky(Smem& S, int D) {
   return logdet;
 }
 
+template <int MD>
 __global__ void __launch_bounds__(NT) dense_metric_position_kernel(const DenseMetricArgs a) {
+  using Smem = SmemT<MD>;
+  constexpr int LDA = Smem::LDA;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Smem& S = *reinterpret_cast<Smem*>(smem_raw);
   const int c = blockIdx.x;
@@ -236,7 +248,10 @@ __global__ void __launch_bounds__(NT) dense_metric_position_kernel(const DenseMe
   }
 }
 
+template <int MD>
 __global__ void __launch_bounds__(NT) dense_lmc_half_step_kernel(const DenseMetricArgs a) {
+  using Smem = SmemT<MD>;
+  constexpr int LDA = Smem::LDA;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Smem& S = *reinterpret_cast<Smem*>(smem_raw);
   const int c = blockIdx.x;
@@ -272,7 +287,10 @@ __global__ void __launch_bounds__(NT) dense_lmc_half_step_kernel(const DenseMetr
 // Gamma_i = sum_j d(G^-1)_ij/dq_j), the drift G^-1 grad l + Gamma, log det G and
 // the draw chol(G^-1) z (the lower factor of the inverse, which the reference
 // factors as the proposal covariance 2 eps G^-1).
+template <int MD>
 __global__ void __launch_bounds__(NT) dense_mmala_drift_kernel(const DenseMetricArgs a) {
+  using Smem = SmemT<MD>;
+  constexpr int LDA = Smem::LDA;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Smem& S = *reinterpret_cast<Smem*>(smem_raw);
   const int c = blockIdx.x;
@@ -358,7 +376,10 @@ __global__ void __launch_bounds__(NT) dense_mmala_drift_kernel(const DenseMetric
 // Cholesky factor of (G + G^T)/2, dT/dp = u = G^-1 p and
 // dT/dq_k = 1/2 sum_ab ((G^-1)_ba - u_a u_b) dG_ab/dq_k; the momentum draw is
 // p = L z.
+template <int MD>
 __global__ void __launch_bounds__(NT) dense_rmhmc_field_kernel(const DenseMetricArgs a) {
+  using Smem = SmemT<MD>;
+  constexpr int LDA = Smem::LDA;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Smem& S = *reinterpret_cast<Smem*>(smem_raw);
   const int c = blockIdx.x;
@@ -431,38 +452,37 @@ __global__ void __launch_bounds__(NT) dense_rmhmc_field_kernel(const DenseMetric
 
 }  // namespace
 
-size_t dense_metric_smem_bytes() { return sizeof(Smem); }
+size_t dense_metric_smem_bytes() { return sizeof(SmemT<64>); }
+
+namespace {
+
+// the capacity-32 instantiation for D <= 32 (a quarter of the LDS), else capacity 64
+hipError_t launch_dense(const DenseMetricArgs& a, hipStream_t stream, void (*k32)(DenseMetricArgs),
+                        void (*k64)(DenseMetricArgs)) {
+  const bool small = a.D <= 32;
+  const size_t smem = small ? sizeof(SmemT<32>) : sizeof(SmemT<64>);
+  void (*k)(DenseMetricArgs) = small ? k32 : k64;
+  (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+  hipLaunchKernelGGL(k, dim3(a.C), dim3(NT), smem, stream, a);
+  return hipGetLastError();
+}
+
+}  // namespace
 
 hipError_t launch_dense_rmhmc_field(const DenseMetricArgs& a, hipStream_t stream) {
-  const size_t smem = sizeof(Smem);
-  (void)hipFuncSetAttribute((const void*)dense_rmhmc_field_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)smem);
-  hipLaunchKernelGGL(dense_rmhmc_field_kernel, dim3(a.C), dim3(NT), smem, stream, a);
-  return hipGetLastError();
+  return launch_dense(a, stream, dense_rmhmc_field_kernel<32>, dense_rmhmc_field_kernel<64>);
 }
 
 hipError_t launch_dense_mmala_drift(const DenseMetricArgs& a, hipStream_t stream) {
-  const size_t smem = sizeof(Smem);
-  (void)hipFuncSetAttribute((const void*)dense_mmala_drift_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)smem);
-  hipLaunchKernelGGL(dense_mmala_drift_kernel, dim3(a.C), dim3(NT), smem, stream, a);
-  return hipGetLastError();
+  return launch_dense(a, stream, dense_mmala_drift_kernel<32>, dense_mmala_drift_kernel<64>);
 }
 
 hipError_t launch_dense_metric_position(const DenseMetricArgs& a, hipStream_t stream) {
-  const size_t smem = sizeof(Smem);
-  (void)hipFuncSetAttribute((const void*)dense_metric_position_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)smem);
-  hipLaunchKernelGGL(dense_metric_position_kernel, dim3(a.C), dim3(NT), smem, stream, a);
-  return hipGetLastError();
+  return launch_dense(a, stream, dense_metric_position_kernel<32>, dense_metric_position_kernel<64>);
 }
 
 hipError_t launch_dense_lmc_half_step(const DenseMetricArgs& a, hipStream_t stream) {
-  const size_t smem = sizeof(Smem);
-  (void)hipFuncSetAttribute((const void*)dense_lmc_half_step_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)smem);
-  hipLaunchKernelGGL(dense_lmc_half_step_kernel, dim3(a.C), dim3(NT), smem, stream, a);
-  return hipGetLastError();
+  return launch_dense(a, stream, dense_lmc_half_step_kernel<32>, dense_lmc_half_step_kernel<64>);
 }
 
 }  // namespace gjx

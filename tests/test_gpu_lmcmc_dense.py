@@ -45,7 +45,8 @@ def monge_metric(alpha2):
     return metric_fn
 
 
-@pytest.mark.parametrize("D,C,eps,L,a2,T,seed", [(10, 4, 0.05, 4, 1.0, 2, 1), (20, 3, 0.1, 3, 0.5, 2, 2)])
+@pytest.mark.parametrize("D,C,eps,L,a2,T,seed", [(10, 4, 0.05, 4, 1.0, 2, 1), (20, 3, 0.1, 3, 0.5, 2, 2),
+                                                 (40, 2, 0.05, 2, 1.0, 1, 3)])  # 40: the capacity-64 kernels
 def test_funnel_monge_autodiff_matches_oracle(D, C, eps, L, a2, T, seed):
     print(f"case dense funnel-Monge D={D} C={C} eps={eps} L={L} alpha2={a2}", flush=True)
     metric_fn = monge_metric(a2)

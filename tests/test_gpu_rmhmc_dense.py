@@ -42,7 +42,8 @@ def monge(alpha2):
     return metric_fn
 
 
-@pytest.mark.parametrize("D,C,eps,L,a2,max_iters,T,seed", [(8, 4, 0.05, 3, 1.0, 8, 2, 1), (16, 3, 0.1, 2, 0.5, 6, 2, 2)])
+@pytest.mark.parametrize("D,C,eps,L,a2,max_iters,T,seed", [(8, 4, 0.05, 3, 1.0, 8, 2, 1), (16, 3, 0.1, 2, 0.5, 6, 2, 2),
+                                                           (40, 2, 0.05, 2, 1.0, 6, 1, 3)])
 def test_funnel_monge_autodiff_matches_oracle(D, C, eps, L, a2, max_iters, T, seed):
     print(f"case dense rmhmc funnel-Monge D={D} C={C} eps={eps} L={L} alpha2={a2}", flush=True)
     metric_fn = monge(a2)
