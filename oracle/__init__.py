@@ -2,7 +2,7 @@
 
 TEST INFRASTRUCTURE ONLY. This package is the correctness oracle for the HIP
 kernels in ``geomjax_amd``.  Only ``tests/``, ``__graft_entry__.smoke()`` and
-the ``cpu_baseline`` leg of ``bench.py`` may import it, and only as the checker
+the ``cpu_baseline`` leg of ``bench.py`` may import it, and only to compare results with
 (or, for ``bench.py``, as the timed CPU baseline).  The product path never
 imports, calls or falls back to anything in here.
 
