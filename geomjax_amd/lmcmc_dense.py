@@ -54,9 +54,17 @@ def _value_and_grad(logdensity_fn, q):
 
 
 def _metric(metric_fn, q):
-    G = vmap(metric_fn)(q).contiguous()
-    dG = vmap(jacfwd(metric_fn))(q).contiguous()  # [C, D, D, D]: dG_ab / dq_i at [c, a, b, i]
-    return G, dG
+    """G [C, D, D] and dG [C, D, D, D] (dG_ab / dq_i at [c, a, b, i]).  A metric_fn
+    returning a vector is a diagonal metric (the reference's ndim == 1 branch,
+    geomjax/rmhmc/metrics.py:45-65): it is embedded as diag(m), dG_aa/dq_i = dm_a/dq_i."""
+    G = vmap(metric_fn)(q)
+    J = vmap(jacfwd(metric_fn))(q)
+    if G.dim() == 2:  # [C, D]: diagonal
+        G = torch.diag_embed(G)
+        J = torch.diag_embed(J.transpose(1, 2)).permute(0, 2, 3, 1)  # [C, i, a, a] -> [C, a, b, i]
+    elif G.dim() != 3:
+        raise ValueError(f"metric_fn must return a [D] or [D, D] tensor (got {tuple(G.shape[1:])})")
+    return G.contiguous(), J.contiguous()
 
 
 def _io(G, dG, **kw):

@@ -166,3 +166,26 @@ def test_chees_adaptation_runs_with_a_user_metric():
     s, info = alg.step(gr.key(6), last)
     torch.cuda.synchronize()
     assert torch.isfinite(s.position).all()
+
+
+def test_diagonal_user_metric_matches_the_oracles_diagonal_branch():
+    """A metric_fn returning a vector (the reference's ndim == 1 branch:
+    momentum sqrt(m) z, T = sum log(2 pi m)/2 + p^2/(2m)) against the oracle."""
+    g = torch.func.grad(funnel_logp)
+
+    def diag_metric(q):
+        return 1.0 + 0.5 * g(q) ** 2
+
+    D, C, eps, L = 8, 3, 0.05, 3
+    q0 = 0.5 * seeded_normal(80, C, D)
+    z, u = seeded_normal(81, C, D), seeded_uniform(82, C)
+    kern = build_kernel(functools.partial(implicit_midpoint, max_iters=8))
+    s1, info = kern(gr.HostNoise(z.to(DEV), u.to(DEV)), rm_init(q0.to(DEV), funnel_logp), funnel_logp, eps,
+                    diag_metric, L)
+    so1, oi = orm.build_kernel(max_iters=8)((z, u), orm.init(q0, funnel_logp), funnel_logp, eps, diag_metric, L)
+    torch.cuda.synchronize()
+    assert info.is_accepted.cpu().tolist() == oi.is_accepted.tolist()
+    for f in ("position", "logdensity", "logdensity_grad"):
+        assert rel_err(getattr(s1, f), getattr(so1, f)) < 1e-10, f
+    assert rel_err(info.momentum, oi.momentum) < 1e-10
+    assert rel_err(info.energy, oi.energy) < 1e-10
