@@ -27,16 +27,37 @@ __all__ = ["is_dense_metric_fn", "init", "transition", "sample"]
 
 
 def is_dense_metric_fn(logdensity_fn, metric_fn):
-    """True for plain callables (user functions), False for this package's model
-    and metric descriptors, which have closed-form kernels."""
+    """True for plain callables (user functions) -- including ``MongeMetric``
+    paired with a user log density, whose metric is then built by autodiff --
+    False for this package's model descriptors, which have closed-form kernels."""
     from .metrics import FisherMetric, MongeMetric
     from .models import LogisticBatch, LogisticRegression, NealFunnel
 
-    if isinstance(metric_fn, (FisherMetric, MongeMetric)):
-        return False
     if isinstance(logdensity_fn, (LogisticRegression, LogisticBatch, NealFunnel)):
         return False
+    if isinstance(metric_fn, FisherMetric):
+        return False
+    if isinstance(metric_fn, MongeMetric):
+        return callable(logdensity_fn)
     return callable(metric_fn) and callable(logdensity_fn)
+
+
+def resolve_metric_fn(logdensity_fn, metric_fn):
+    """The metric as a function of the position: ``MongeMetric(alpha2)`` with a
+    user log density becomes I + alpha2 grad l grad l^T (the Monge metric the
+    reference's examples pass as metric_fn, examples/funnel/main.py:34-39)."""
+    from .metrics import MongeMetric
+
+    if not isinstance(metric_fn, MongeMetric):
+        return metric_fn
+    alpha2 = metric_fn.alpha2
+    g = torch.func.grad(logdensity_fn)
+
+    def monge(q):
+        gq = g(q)
+        return torch.eye(q.shape[0], dtype=q.dtype, device=q.device) + alpha2 * torch.outer(gq, gq)
+
+    return monge
 
 
 def _check(position):
@@ -153,6 +174,8 @@ def transition(rng_key, state, logdensity_fn, step_size, metric_fn, num_integrat
     whose count is reached keeps its state while the others step on)."""
     from .lmcmc import IntegratorState, LMCInfo, LMCState
     from .rmhmc_dense import _per_chain_steps
+
+    metric_fn = resolve_metric_fn(logdensity_fn, metric_fn)
 
     q0 = state.position.contiguous()
     _check(q0)

@@ -37,8 +37,10 @@ def _drift(G, dG=None, g=None, z=None):
 
 def init(position, logdensity_fn, metric_fn):
     """geomjax/lmcmc/mmala.py:53-60: log density, gradient, metric and Gamma."""
+    from .lmcmc_dense import resolve_metric_fn
     from .mmala import DiffusionState
 
+    metric_fn = resolve_metric_fn(logdensity_fn, metric_fn)
     single = position.dim() == 1
     q = (position[None] if single else position).contiguous()
     _check(q)
@@ -55,6 +57,9 @@ def transition(rng_key, state, logdensity_fn, step_size, metric_fn, reference_fa
     """geomjax/lmcmc/mmala.py:115-148 for every chain.  The current state's metric
     and Gamma are taken from ``state`` (as the reference's transition energy does)."""
     from .mmala import DiffusionState, MMALAInfo
+    from .lmcmc_dense import resolve_metric_fn
+
+    metric_fn = resolve_metric_fn(logdensity_fn, metric_fn)
 
     x = state.position.contiguous()
     _check(x)

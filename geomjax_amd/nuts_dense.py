@@ -58,6 +58,9 @@ def transition(noise, state, logdensity_fn, step_size, metric_fn, solver, max_nu
     """One NUTS transition of every chain with the RMHMC implicit midpoint.
     ``noise`` = (z [C, D], u_direction [C, M], u_merge [C, M], u_step [C, 2^M - 1]).
     Returns (position, logdensity, grad, info dict)."""
+    from .lmcmc_dense import resolve_metric_fn
+
+    metric_fn = resolve_metric_fn(logdensity_fn, metric_fn)
     q0 = state.position.contiguous()
     _check(q0)
     C, D = q0.shape
@@ -206,7 +209,9 @@ def lmc_transition(noise, state, logdensity_fn, step_size, metric_fn, max_num_do
     (geomjax/lmcmc/nuts.py:78-161): velocity v0 = L^-T z, momentum G v0, the
     energy -l - log det G / 2 + v^T G v / 2 - J carried along the trajectory.
     Returns (position, logdensity, grad, volume_adjustment, info dict)."""
-    from .lmcmc_dense import _position, lmc_step
+    from .lmcmc_dense import _position, lmc_step, resolve_metric_fn
+
+    metric_fn = resolve_metric_fn(logdensity_fn, metric_fn)
 
     q0 = state.position.contiguous()
     _check(q0)

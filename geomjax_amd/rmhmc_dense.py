@@ -115,6 +115,9 @@ def transition(rng_key, state, logdensity_fn, step_size, metric_fn, num_integrat
     Returns (RMHMCState, RMHMCInfo, per-step FixedPointIterationInfo list)."""
     from .proposal import Proposal
     from .rmhmc import IntegratorState, RMHMCInfo, RMHMCState
+    from .lmcmc_dense import resolve_metric_fn
+
+    metric_fn = resolve_metric_fn(logdensity_fn, metric_fn)
 
     q0 = state.position.contiguous()
     _check(q0)

@@ -189,3 +189,19 @@ def test_diagonal_user_metric_matches_the_oracles_diagonal_branch():
         assert rel_err(getattr(s1, f), getattr(so1, f)) < 1e-10, f
     assert rel_err(info.momentum, oi.momentum) < 1e-10
     assert rel_err(info.energy, oi.energy) < 1e-10
+
+
+def test_monge_descriptor_with_a_user_logdensity_equals_the_explicit_metric_fn():
+    """MongeMetric(alpha2) paired with a plain log density builds I + alpha2 g g^T by
+    autodiff: the same transition as passing that metric_fn explicitly."""
+    D, C, eps, L = 8, 4, 0.05, 3
+    q0 = (0.5 * seeded_normal(90, C, D)).to(DEV)
+    z, u = seeded_normal(91, C, D).to(DEV), seeded_uniform(92, C).to(DEV)
+    kern = build_kernel(functools.partial(implicit_midpoint, max_iters=8))
+    s0 = rm_init(q0, funnel_logp)
+    a, ia = kern(gr.HostNoise(z, u), s0, funnel_logp, eps, gj.MongeMetric(0.5), L)
+    b, ib = kern(gr.HostNoise(z, u), s0, funnel_logp, eps, monge(0.5), L)
+    torch.cuda.synchronize()
+    assert torch.equal(ia.is_accepted, ib.is_accepted)
+    assert rel_err(a.position, b.position) <= 1e-14
+    assert rel_err(ia.energy, ib.energy) <= 1e-14
