@@ -285,9 +285,12 @@ def check(rc, what):
 
 
 def ptr(t):
-    """Device pointer of a tensor (None -> NULL)."""
+    """Device pointer of a tensor (None -> NULL).  A host tensor raises: its
+    address is not a device address, and a kernel reading it would fault."""
     if t is None:
         return None
+    if not t.is_cuda:
+        raise ValueError(f"expected a tensor on a HIP device, got a {t.dtype} tensor on {t.device}")
     return t.data_ptr()
 
 

@@ -103,8 +103,10 @@ def build_kernel(integrator=lan_integrator, divergence_threshold: float = 1000):
 
         if lmcmc_dense.is_dense_metric_fn(logdensity_fn, metric_fn):
             # a user metric_fn: torch.func derivatives + the dense-metric HIP kernels
-            return lmcmc_dense.transition(rng_key, state, logdensity_fn, step_size, metric_fn,
-                                          num_integration_steps, divergence_threshold, max_num_integration_steps)
+            state, single = lmcmc_dense.as_batch(state)
+            out = lmcmc_dense.transition(rng_key, state, logdensity_fn, step_size, metric_fn,
+                                         num_integration_steps, divergence_threshold, max_num_integration_steps)
+            return lmcmc_dense.unbatch(out) if single else out
         single = state.position.dim() == 1
         position = (state.position[None] if single else state.position).contiguous()
         _check(logdensity_fn, metric_fn, position)

@@ -69,30 +69,55 @@ def _check(position):
         raise NotImplementedError(f"dense-metric kernels take D <= 64 (got {position.shape[1]})")
 
 
+def _as_kernel_input(t, q, shape, what):
+    """A user function's output as the float64 kernels read it: the exact
+    ``shape``, float64 and on ``q``'s device (a float32 result is widened, a
+    host result moved), contiguous.  Anything that is not a real floating
+    tensor of that shape raises before a launch."""
+    if not torch.is_tensor(t):
+        raise TypeError(f"{what} must return a torch tensor (got {type(t).__name__})")
+    if not (t.is_floating_point() and not t.is_complex()):
+        raise TypeError(f"{what} must return a real floating-point tensor (got {t.dtype})")
+    if tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{what} returned shape {tuple(t.shape)}, expected {tuple(shape)}")
+    return t.to(device=q.device, dtype=torch.float64).contiguous()
+
+
 def _value_and_grad(logdensity_fn, q):
     g, l = vmap(grad_and_value(logdensity_fn))(q)
-    return l.contiguous(), g.contiguous()
+    C, D = q.shape
+    return (_as_kernel_input(l, q, (C,), "logdensity_fn (a scalar per chain)"),
+            _as_kernel_input(g, q, (C, D), "the gradient of logdensity_fn"))
 
 
 def _metric(metric_fn, q):
     """G [C, D, D] and dG [C, D, D, D] (dG_ab / dq_i at [c, a, b, i]).  A metric_fn
     returning a vector is a diagonal metric (the reference's ndim == 1 branch,
-    geomjax/rmhmc/metrics.py:45-65): it is embedded as diag(m), dG_aa/dq_i = dm_a/dq_i."""
+    geomjax/rmhmc/metrics.py:45-65): it is embedded as diag(m), dG_aa/dq_i = dm_a/dq_i.
+    The results are float64 on q's device whatever the function returned
+    (float32 results are widened, host results moved)."""
+    C, D = q.shape
     G = vmap(metric_fn)(q)
     J = vmap(jacfwd(metric_fn))(q)
-    if G.dim() == 2:  # [C, D]: diagonal
+    if torch.is_tensor(G) and G.dim() == 2:  # [C, D]: diagonal
+        G = _as_kernel_input(G, q, (C, D), "metric_fn (diagonal)")
+        J = _as_kernel_input(J, q, (C, D, D), "the Jacobian of metric_fn (diagonal)")
         G = torch.diag_embed(G)
         J = torch.diag_embed(J.transpose(1, 2)).permute(0, 2, 3, 1)  # [C, i, a, a] -> [C, a, b, i]
-    elif G.dim() != 3:
-        raise ValueError(f"metric_fn must return a [D] or [D, D] tensor (got {tuple(G.shape[1:])})")
+    elif torch.is_tensor(G) and G.dim() == 3:
+        G = _as_kernel_input(G, q, (C, D, D), "metric_fn")
+        J = _as_kernel_input(J, q, (C, D, D, D), "the Jacobian of metric_fn")
+    else:
+        got = tuple(G.shape[1:]) if torch.is_tensor(G) else type(G).__name__
+        raise ValueError(f"metric_fn must return a [D] or [D, D] tensor (got {got})")
     return G.contiguous(), J.contiguous()
 
 
 def _io(G, dG, **kw):
     C, D = G.shape[0], G.shape[1]
     ptrs = {k: _lib.ptr(v) for k, v in kw.items() if k != "step_size"}
-    return _lib.DenseMetricIO(num_chains=C, dim=D, step_size=float(kw.get("step_size", 0.0)), metric=G.data_ptr(),
-                              metric_jacobian=dG.data_ptr(), **ptrs)
+    return _lib.DenseMetricIO(num_chains=C, dim=D, step_size=float(kw.get("step_size", 0.0)), metric=_lib.ptr(G),
+                              metric_jacobian=_lib.ptr(dG), **ptrs)
 
 
 def _position(G, dG, z=None, velocity=None):
@@ -153,6 +178,28 @@ def init(position, logdensity_fn):
     if single:
         return LMCState(position, l[0], g[0], torch.zeros_like(l[0]))
     return LMCState(position, l, g, torch.zeros_like(l))
+
+
+def as_batch(state):
+    """A single-chain state (position [D]) as a batch of one chain, and whether
+    it was single: every tensor field gains a leading axis of 1."""
+    if state.position.dim() != 1:
+        return state, False
+    return type(state)(*((t[None] if torch.is_tensor(t) else t) for t in state)), True
+
+
+def unbatch(x):
+    """The inverse of ``as_batch`` on a transition's outputs: the leading chain
+    axis of every tensor in (nested) tuples, lists and dicts is dropped."""
+    if torch.is_tensor(x):
+        return x[0] if x.dim() >= 1 else x
+    if isinstance(x, tuple) and hasattr(x, "_fields"):
+        return type(x)(*(unbatch(t) for t in x))
+    if isinstance(x, (tuple, list)):
+        return type(x)(unbatch(t) for t in x)
+    if isinstance(x, dict):
+        return {k: unbatch(v) for k, v in x.items()}
+    return x
 
 
 def _noise(rng_key, C, D, dev):

@@ -105,9 +105,12 @@ def build_kernel(*, reference_faithful=True, divergence_threshold: float = 1000)
 
         if is_dense_metric_fn(logdensity_fn, metric_fn):
             from . import mmala_dense
+            from .lmcmc_dense import as_batch, unbatch
 
-            return mmala_dense.transition(rng_key, state, logdensity_fn, step_size, metric_fn, reference_faithful,
-                                          divergence_threshold)
+            state, single = as_batch(state)
+            out = mmala_dense.transition(rng_key, state, logdensity_fn, step_size, metric_fn, reference_faithful,
+                                         divergence_threshold)
+            return unbatch(out) if single else out
         single = state.position.dim() == 1
         position = (state.position[None] if single else state.position).contiguous()
         _check(logdensity_fn, metric_fn, position)

@@ -28,7 +28,7 @@ def _drift(G, dG=None, g=None, z=None):
     gamma = vec() if dG is not None else None
     drift = vec() if g is not None else None
     ciz = vec() if z is not None else None
-    io = _lib.DenseMetricIO(num_chains=C, dim=D, step_size=0.0, metric=G.data_ptr(), metric_jacobian=_lib.ptr(dG),
+    io = _lib.DenseMetricIO(num_chains=C, dim=D, step_size=0.0, metric=_lib.ptr(G), metric_jacobian=_lib.ptr(dG),
                             logdensity_grad=_lib.ptr(g), z=_lib.ptr(z), logdet=logdet.data_ptr(),
                             gamma=_lib.ptr(gamma), drift=_lib.ptr(drift), chol_inv_z=_lib.ptr(ciz))
     _lib.check(_lib.load().gjx_dense_mmala_drift(io, _lib.stream_handle(dev)), "gjx_dense_mmala_drift")

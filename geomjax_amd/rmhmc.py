@@ -174,10 +174,13 @@ def build_kernel(integrator=implicit_midpoint, divergence_threshold: float = 100
         if is_dense_metric_fn(logdensity_fn, metric_fn):
             # a user metric_fn: torch.func derivatives + the dense-metric field kernel
             from . import rmhmc_dense
+            from .lmcmc_dense import as_batch, unbatch
 
+            state, single = as_batch(state)
             new, info, fps = rmhmc_dense.transition(rng_key, state, logdensity_fn, step_size, metric_fn,
                                                     num_integration_steps, solver, divergence_threshold,
                                                     max_num_integration_steps)
+            out = (new, info)
             if return_fixed_point_info:
                 C = state.position.shape[0]
                 if fps:
@@ -188,8 +191,8 @@ def build_kernel(integrator=implicit_midpoint, divergence_threshold: float = 100
                     fp = FixedPointIterationInfo(torch.zeros(C, 0, dtype=torch.bool, device=dev),
                                                  torch.zeros(C, 0, dtype=torch.float64, device=dev),
                                                  torch.zeros(C, 0, dtype=torch.int32, device=dev))
-                return new, info, fp
-            return new, info
+                out = (new, info, fp)
+            return unbatch(out) if single else out
         position, single = _as_batch(state.position)
         kind = _problem(logdensity_fn, metric_fn, position)
         position = position.contiguous()

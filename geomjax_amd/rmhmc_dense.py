@@ -32,7 +32,7 @@ def _field(G, dG=None, p=None, z=None):
     dtdq = vec() if dG is not None else None
     logdet = torch.empty(C, dtype=torch.float64, device=dev)
     pu = torch.empty(C, dtype=torch.float64, device=dev)
-    io = _lib.DenseMetricIO(num_chains=C, dim=D, step_size=0.0, metric=G.data_ptr(), metric_jacobian=_lib.ptr(dG),
+    io = _lib.DenseMetricIO(num_chains=C, dim=D, step_size=0.0, metric=_lib.ptr(G), metric_jacobian=_lib.ptr(dG),
                             momentum=_lib.ptr(p), z=_lib.ptr(z), velocity_out=u.data_ptr(),
                             momentum_out=pout.data_ptr(), dtdq=_lib.ptr(dtdq), logdet=logdet.data_ptr(),
                             pu=pu.data_ptr())

@@ -46,7 +46,10 @@ def monge_metric(alpha2):
 
 
 @pytest.mark.parametrize("D,C,eps,L,a2,T,seed", [(10, 4, 0.05, 4, 1.0, 2, 1), (20, 3, 0.1, 3, 0.5, 2, 2),
-                                                 (40, 2, 0.05, 2, 1.0, 1, 3)])  # 40: the capacity-64 kernels
+                                                 (40, 2, 0.05, 2, 1.0, 1, 3),   # 40: the capacity-64 kernels
+                                                 (32, 2, 0.05, 2, 1.0, 1, 4),   # the capacity-32 layout's last D
+                                                 (33, 2, 0.05, 2, 1.0, 1, 5),   # the capacity-64 layout's first D
+                                                 (64, 2, 0.03, 2, 1.0, 1, 6)])  # the advertised maximum
 def test_funnel_monge_autodiff_matches_oracle(D, C, eps, L, a2, T, seed):
     print(f"case dense funnel-Monge D={D} C={C} eps={eps} L={L} alpha2={a2}", flush=True)
     metric_fn = monge_metric(a2)
@@ -133,3 +136,60 @@ def test_per_chain_step_counts_match_the_oracle_chain_by_chain():
         assert rel_err(s1.position[c:c + 1], so1.position) < 1e-10, c
         assert abs(float(s1.volume_adjustment[c]) - float(so1.volume_adjustment[0])) < 1e-10, c
         assert rel_err(info.acceptance_rate[c:c + 1], oi.acceptance_rate) < 1e-10, c
+
+
+def _skewed_metric(q):
+    """G = A A^T + I/10 with A = I + 3 (subdiagonal) + diag(tanh(q))/10: column 0 of
+    G holds 1.01 + ... on the diagonal and ~3 below it, so LU with partial
+    pivoting of G +- (eps/2) Omega~ must swap rows."""
+    D = q.shape[0]
+    A = torch.eye(D, dtype=q.dtype, device=q.device) + 3.0 * torch.diag(torch.ones(D - 1, dtype=q.dtype,
+                                                                                   device=q.device), -1)
+    A = A + 0.1 * torch.diag(torch.tanh(q))
+    return A @ A.T + 0.1 * torch.eye(D, dtype=q.dtype, device=q.device)
+
+
+def _gauss_logp(q):
+    return -0.5 * torch.dot(q, q)
+
+
+@pytest.mark.parametrize("D", [6, 33])
+def test_half_step_with_row_swaps_matches_oracle(D):
+    print(f"case dense lmc skewed metric D={D} (LU row swaps)", flush=True)
+    C, eps, L = 3, 0.05, 2
+    q0 = 0.5 * seeded_normal(60 + D, C, D)
+    _, piv = torch.linalg.lu_factor(_skewed_metric(q0[0]))
+    swaps = int((piv.cpu() != torch.arange(1, D + 1, dtype=piv.dtype)).sum())
+    assert swaps > 0  # the case really pivots
+    so, s = olc.init(q0, _gauss_logp), lmc_init(q0.to(DEV), _gauss_logp)
+    z, u = seeded_normal(61 + D, C, D), seeded_uniform(62 + D, C)
+    s1, info = lmc_kernel()(gr.HostNoise(z.to(DEV), u.to(DEV)), s, _gauss_logp, eps, _skewed_metric, L)
+    so1, oi = olc.build_kernel()((z, u), so, _gauss_logp, eps, _skewed_metric, L)
+    torch.cuda.synchronize()
+    print(f"  row swaps in LU(G(q0)): {swaps}; accepted {oi.is_accepted.tolist()}", flush=True)
+    assert info.is_accepted.cpu().tolist() == oi.is_accepted.tolist()
+    for f in ("position", "logdensity", "logdensity_grad"):
+        assert rel_err(getattr(s1, f), getattr(so1, f)) < 1e-10, f
+    assert rel_err(info.velocity, oi.velocity) < 1e-10
+    assert rel_err(info.energy, oi.energy) < 1e-10
+    for f in ("position", "momentum", "velocity"):
+        assert rel_err(getattr(info.proposal.state, f), getattr(oi.proposal.state, f)) < 1e-10, f
+    assert (info.proposal.state.volume_adjustment.cpu() - oi.proposal.state.volume_adjustment).abs().max() < 1e-10
+
+
+def test_single_chain_state_equals_the_batched_row():
+    """A [D] position takes the dense path as a batch of one chain; the result
+    equals row 0 of the same step run batched, with unbatched shapes."""
+    D, eps, L = 10, 0.05, 3
+    metric_fn = monge_metric(1.0)
+    q0 = (0.5 * seeded_normal(70, 2, D)).to(DEV)
+    z, u = seeded_normal(71, 2, D).to(DEV), seeded_uniform(72, 2).to(DEV)
+    kern = lmc_kernel()
+    sb, ib = kern(gr.HostNoise(z, u), lmc_init(q0, funnel_logp), funnel_logp, eps, metric_fn, L)
+    s1, i1 = kern(gr.HostNoise(z[0], u[0]), lmc_init(q0[0], funnel_logp), funnel_logp, eps, metric_fn, L)
+    torch.cuda.synchronize()
+    assert s1.position.shape == (D,) and s1.logdensity.dim() == 0 and i1.velocity.shape == (D,)
+    assert bool(i1.is_accepted) == bool(ib.is_accepted[0])
+    for f in ("position", "logdensity", "logdensity_grad", "volume_adjustment"):
+        assert rel_err(getattr(s1, f), getattr(sb, f)[0]) < 1e-13, f
+    assert rel_err(i1.proposal.state.position, ib.proposal.state.position[0]) < 1e-13

@@ -26,6 +26,7 @@ DEV = "cuda:0"
     (256, 8, 3, 0.1, 6, 2, 0),
     (300, 13, 2, 0.1, 4, 1, 1),
     (1024, 32, 2, 0.1, 5, 1, 2),
+    (2048, 64, 2, 0.1, 6, 1, 3),   # the kernel's feature capacity, on the 2048 x 64 shape
 ])
 def test_step_f64_matches_oracle(N, D, C, eps, L, T, seed):
     print(f"case N={N} D={D} C={C} eps={eps} L={L} T={T}", flush=True)

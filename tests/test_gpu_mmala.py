@@ -162,12 +162,14 @@ def _monge(q):
     return torch.eye(q.shape[0], dtype=q.dtype, device=q.device) + 0.5 * torch.outer(g, g)
 
 
-@pytest.mark.parametrize("faithful", [True, False])
-def test_user_dense_metric_matches_oracle(faithful):
+@pytest.mark.parametrize("faithful,D", [(True, 8), (False, 8),
+                                         (True, 32), (True, 33),   # the two LDS layouts' boundary
+                                         (False, 64)])             # the maximum D
+def test_user_dense_metric_matches_oracle(faithful, D):
     """A user metric (the funnel's Monge metric by autodiff): the dense-metric
     kernels and torch.func against the oracle's jacfwd of the inverse metric."""
-    print(f"case mmala dense funnel-Monge faithful={faithful}", flush=True)
-    C, D, eps, T = 4, 8, 0.05, 2
+    print(f"case mmala dense funnel-Monge faithful={faithful} D={D}", flush=True)
+    C, eps, T = (4, 0.05, 2) if D <= 8 else (2, 0.03, 1)
     q0 = 0.5 * seeded_normal(70, C, D)
     so = omm.init(q0, _funnel_logp, _monge)
     s = mm_init(q0.to(DEV), _funnel_logp, _monge)
@@ -216,3 +218,47 @@ def test_fisher_as_user_function_matches_the_closed_form_kernel():
     assert rel_err(ia.weight, ib.weight) < 1e-9
     for f in DiffusionState._fields:
         assert rel_err(getattr(a, f), getattr(b, f)) < 1e-10, f
+
+
+def test_user_dense_metric_single_chain_equals_the_batched_row():
+    C, D, eps = 2, 8, 0.05
+    q0 = (0.5 * seeded_normal(95, C, D)).to(DEV)
+    z, u = seeded_normal(96, C, D).to(DEV), seeded_uniform(97, C).to(DEV)
+    kern = build_kernel(reference_faithful=False)
+    sb, ib = kern(gr.HostNoise(z, u), mm_init(q0, _funnel_logp, _monge), _funnel_logp, eps, _monge)
+    s1, i1 = kern(gr.HostNoise(z[0], u[0]), mm_init(q0[0], _funnel_logp, _monge), _funnel_logp, eps, _monge)
+    torch.cuda.synchronize()
+    assert s1.position.shape == (D,) and s1.metric.shape == (D, D) and s1.logdensity.dim() == 0
+    assert bool(i1.is_accepted) == bool(ib.is_accepted[0])
+    for f in DiffusionState._fields:
+        assert rel_err(getattr(s1, f), getattr(sb, f)[0]) < 1e-13, f
+
+
+def test_user_functions_returning_float32_or_host_tensors_are_cast():
+    """A metric written without dtype/device (float32 constants, a host
+    tensor) gives the same transition as the float64 device version."""
+    C, D, eps = 3, 6, 0.05
+    q0 = (0.5 * seeded_normal(98, C, D)).to(DEV)
+    z, u = seeded_normal(99, C, D).to(DEV), seeded_uniform(100, C).to(DEV)
+
+    def metric64(q):
+        return torch.eye(q.shape[0], dtype=q.dtype, device=q.device) * (2.0 + torch.tanh(q.sum()))
+
+    def metric32(q):
+        return (torch.eye(q.shape[0], device=q.device) * (2.0 + torch.tanh(q.sum()))).float()
+
+    def metric_host(q):  # a constant metric built on the host
+        return 2.0 * torch.eye(q.shape[0], dtype=torch.float64)
+
+    def metric_const(q):
+        return 2.0 * torch.eye(q.shape[0], dtype=q.dtype, device=q.device)
+
+    kern = build_kernel(reference_faithful=False)
+    a, ia = kern(gr.HostNoise(z, u), mm_init(q0, _funnel_logp, metric64), _funnel_logp, eps, metric64)
+    b, ib = kern(gr.HostNoise(z, u), mm_init(q0, _funnel_logp, metric32), _funnel_logp, eps, metric32)
+    c, ic = kern(gr.HostNoise(z, u), mm_init(q0, _funnel_logp, metric_host), _funnel_logp, eps, metric_host)
+    d, idd = kern(gr.HostNoise(z, u), mm_init(q0, _funnel_logp, metric_const), _funnel_logp, eps, metric_const)
+    torch.cuda.synchronize()
+    assert b.metric.dtype == torch.float64 and b.metric.is_cuda and c.metric.is_cuda
+    assert rel_err(b.position, a.position) < 1e-5  # float32 metric values, widened
+    assert rel_err(c.position, d.position) < 1e-15 and rel_err(c.Gamma, d.Gamma) < 1e-15

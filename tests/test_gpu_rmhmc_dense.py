@@ -43,7 +43,10 @@ def monge(alpha2):
 
 
 @pytest.mark.parametrize("D,C,eps,L,a2,max_iters,T,seed", [(8, 4, 0.05, 3, 1.0, 8, 2, 1), (16, 3, 0.1, 2, 0.5, 6, 2, 2),
-                                                           (40, 2, 0.05, 2, 1.0, 6, 1, 3)])
+                                                           (40, 2, 0.05, 2, 1.0, 6, 1, 3),
+                                                           (32, 2, 0.05, 2, 1.0, 6, 1, 4),   # capacity-32 layout, full
+                                                           (33, 2, 0.05, 2, 1.0, 6, 1, 5),   # capacity-64 layout, first
+                                                           (64, 2, 0.03, 2, 1.0, 6, 1, 6)])  # the maximum D
 def test_funnel_monge_autodiff_matches_oracle(D, C, eps, L, a2, max_iters, T, seed):
     print(f"case dense rmhmc funnel-Monge D={D} C={C} eps={eps} L={L} alpha2={a2}", flush=True)
     metric_fn = monge(a2)
@@ -205,3 +208,22 @@ def test_monge_descriptor_with_a_user_logdensity_equals_the_explicit_metric_fn()
     assert torch.equal(ia.is_accepted, ib.is_accepted)
     assert rel_err(a.position, b.position) <= 1e-14
     assert rel_err(ia.energy, ib.energy) <= 1e-14
+
+
+def test_single_chain_state_equals_the_batched_row():
+    """A [D] position takes the dense path as a batch of one chain; the result
+    equals row 0 of the same step run batched, with unbatched shapes."""
+    D, eps, L = 8, 0.05, 3
+    metric_fn = monge(1.0)
+    q0 = (0.5 * seeded_normal(75, 2, D)).to(DEV)
+    z, u = seeded_normal(76, 2, D).to(DEV), seeded_uniform(77, 2).to(DEV)
+    kern = build_kernel(functools.partial(implicit_midpoint, max_iters=6), return_fixed_point_info=True)
+    sb, ib, fb = kern(gr.HostNoise(z, u), rm_init(q0, funnel_logp), funnel_logp, eps, metric_fn, L)
+    s1, i1, f1 = kern(gr.HostNoise(z[0], u[0]), rm_init(q0[0], funnel_logp), funnel_logp, eps, metric_fn, L)
+    torch.cuda.synchronize()
+    assert s1.position.shape == (D,) and s1.logdensity.dim() == 0 and i1.momentum.shape == (D,)
+    assert f1.iters.shape == (L,) and f1.iters.tolist() == fb.iters[0].tolist()
+    assert bool(i1.is_accepted) == bool(ib.is_accepted[0])
+    for f in ("position", "logdensity", "logdensity_grad"):
+        assert rel_err(getattr(s1, f), getattr(sb, f)[0]) < 1e-13, f
+    assert rel_err(i1.proposal.state.position, ib.proposal.state.position[0]) < 1e-13

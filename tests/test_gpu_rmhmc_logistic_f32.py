@@ -2,10 +2,11 @@
 contractions with split operands, float32 sweep + refinement, float64 energies)
 against float64 on the same q0, z, u.
 
-The reference for small shapes is the float64 oracle (dense Fisher metric,
-Cholesky inside the Gaussian log-density, autodiff); for the 2048 x 64 shape
-and the multi-posterior batch it is the float64 HIP kernel, itself pinned to
-the oracle at 1e-10.  Stated tolerances of the tuned path:
+The reference is the float64 oracle (dense Fisher metric, Cholesky inside the
+Gaussian log-density, autodiff) up to the 2048 x 64 configuration and the
+128-feature capacity; for more chains, several transitions and the
+multi-posterior batches it is the float64 HIP kernel, itself pinned to the
+oracle at 1e-10.  Stated tolerances of the tuned path:
   * end state (new position, proposal position): 1e-5 norm-wise relative;
   * momentum p0 = chol(G) z: 1e-5; velocity and gradient at the proposal: 1e-4;
   * energy and energy difference dH: 1e-3 absolute;
@@ -60,6 +61,8 @@ def _check_step(info, s, ref_info, ref_state, u, q_prev, label):
     (256, 8, 3, 0.1, 10, 6, 0),
     (300, 13, 2, 0.1, 6, 6, 1),    # ragged: N not a multiple of 32
     (1000, 40, 2, 0.1, 5, 6, 3),   # two 32-feature blocks with padding
+    (2048, 64, 2, 0.1, 10, 6, 2),  # the 2048 x 64 configuration itself
+    (1024, 128, 2, 0.05, 6, 6, 4), # the four-wave kernel at its capacity
 ])
 def test_step_f32_matches_oracle(N, D, C, eps, L, max_iters, seed):
     print(f"case N={N} D={D} C={C} eps={eps} L={L} max_iters={max_iters}", flush=True)
@@ -240,3 +243,51 @@ def test_pack_layout_matches_numpy_restatement():
         x32 = xr.astype(np.float32).astype(np.float64)
         assert np.all(np.abs(x32 - hi - lo) <= 2.0 ** -17 * np.abs(x32))
         assert np.all(np.abs(lo) <= 2.0 ** -8 * np.abs(hi) + 1e-45)
+
+
+def test_uneven_batch_full_size_matches_each_posterior_and_f64():
+    """The uneven-batch configuration at full size: 256 posteriors (D in
+    {8, 16, 32, 64, 128}, N up to 4096), cost-ordered dispatch and concurrent
+    per-width launches, two chains per posterior.  The float64 batch equals
+    every posterior run alone (1e-12); the tuned batch equals the float64 batch
+    at the tuned bars."""
+    from geomjax_amd.synthetic import make_uneven_batch
+
+    data = make_uneven_batch(P=256, seed=3)
+    cpp, eps, L = 2, 0.1, 10
+    models = [gj.LogisticRegression(X, y, 100.0) for X, y, _ in data]
+    batch = gj.LogisticBatch(models, cpp)
+    C, Dm = batch.num_chains, batch.dim
+    dims = [m.D for m in models]
+    print(f"case uneven batch: {len(models)} posteriors, N max {batch.N}, {C} chains", flush=True)
+    q0 = torch.zeros(C, Dm, dtype=torch.float64)
+    z = torch.zeros(C, Dm, dtype=torch.float64)
+    for c in range(C):
+        d = dims[c // cpp]
+        q0[c, :d] = 0.5 * seeded_normal(3000 + c, d)
+        z[c, :d] = seeded_normal(4000 + c, d)
+    u = seeded_uniform(5000, C)
+    kern = build_kernel(functools.partial(implicit_midpoint, max_iters=6))
+    s64 = gj.rmhmc.init(q0.to(DEV), batch)
+    s32 = gj.rmhmc.init(q0.float().to(DEV), batch)
+    s64n, i64 = kern(gr.HostNoise(z.to(DEV), u.to(DEV)), s64, batch, eps, gj.FisherMetric(), L)
+    s32n, i32 = kern(gr.HostNoise(z.float().to(DEV), u.to(DEV)), s32, batch, eps, gj.FisherMetric(), L)
+    torch.cuda.synchronize()
+    _check_step(i32, s32n, i64, s64n, u, q0, "uneven batch, tuned vs float64")
+    worst = 0.0
+    for p, m in enumerate(models):
+        rows = slice(p * cpp, (p + 1) * cpp)
+        d = m.D
+        sa = gj.rmhmc.init(q0[rows, :d].contiguous().to(DEV), m)
+        sa1, ia = kern(gr.HostNoise(z[rows, :d].contiguous().to(DEV), u[rows].to(DEV)), sa, m, eps,
+                       gj.FisherMetric(), L)
+        for a, b in ((s64n.position[rows, :d], sa1.position), (i64.momentum[rows, :d], ia.momentum),
+                     (i64.proposal.state.position[rows, :d], ia.proposal.state.position),
+                     (i64.energy[rows], ia.energy)):
+            e = rel_err(a, b)
+            worst = max(worst, e)
+            assert e < 1e-12, f"posterior {p} ({m.N}x{d}): batch differs from the posterior alone by {e:.2e}"
+        assert torch.equal(i64.is_accepted[rows].cpu(), ia.is_accepted.cpu())
+        assert torch.all(s64n.position[rows, d:] == 0) and torch.all(s32n.position[rows, d:] == 0)
+    torch.cuda.synchronize()
+    print(f"  float64 batch vs each posterior alone: worst {worst:.2e}", flush=True)

@@ -177,3 +177,98 @@ def test_logistic_autograd_function_and_adapt():
     print("adapted step size", eps, "history", [round(h[2], 4) for h in hist])
     assert a2 == 1.0 and math.isfinite(eps) and eps > 0.02
     assert torch.isfinite(s2.position).all()
+
+
+# ---------------------------------------------------------------------------
+# The adjoint configuration's own sizes: 32 integration steps on the funnel
+# with D = 100 (eps 0.03, alpha2 1) and on logistic regression 2048 x 64
+# (eps 0.1).  Derivative error can grow with trajectory length, so the bars
+# (1e-8 vs autograd through the oracle, 1e-6 vs central differences) are
+# checked at the full length, a few chains each.
+# ---------------------------------------------------------------------------
+
+def test_tangent_funnel_d100_32_steps_matches_autograd():
+    D, C, eps, alpha2, L, max_iters = 100, 3, 0.03, 1.0, 32, 8
+    print(f"case funnel D={D} C={C} L={L} vs autograd", flush=True)
+    q0, z = seeded_normal(200, C, D) * 0.8, seeded_normal(201, C, D)
+    obj_o, g_o = oracle_objective_grad(D, q0, z, eps, alpha2, L, max_iters)
+    model = gj.NealFunnel(D)
+    s = gj.rmhmc.init(q0.to(DEV), model)
+    r = esjd_acceptance(gr.HostNoise(z.to(DEV), torch.zeros(C, device=DEV)), s, model, eps, gj.MongeMetric(alpha2),
+                        L, integrator=functools.partial(implicit_midpoint, max_iters=max_iters))
+    torch.cuda.synchronize()
+    print("  objective", r.objective.cpu().tolist(), "grad", r.d_objective.cpu().tolist(), "oracle", g_o.tolist(),
+          flush=True)
+    assert rel_err(r.objective, obj_o) < 1e-10
+    assert rel_err(r.d_objective, g_o) < 1e-8
+
+
+def test_tangent_funnel_d100_32_steps_matches_central_differences():
+    D, C, eps, alpha2, L, max_iters, h = 100, 6, 0.03, 1.0, 32, 8, 1e-5
+    print(f"case funnel D={D} C={C} L={L} vs central differences", flush=True)
+    q0, z = seeded_normal(202, C, D) * 0.8, seeded_normal(203, C, D)
+    model = gj.NealFunnel(D)
+    integ = functools.partial(implicit_midpoint, max_iters=max_iters, convergence_tol=0.0)
+    s = gj.rmhmc.init(q0.to(DEV), model)
+    noise = gr.HostNoise(z.to(DEV), torch.zeros(C, device=DEV, dtype=torch.float64))
+    r = esjd_acceptance(noise, s, model, eps, gj.MongeMetric(alpha2), L, integrator=integ)
+    kern = build_kernel(integ)
+
+    def obj(le, la):
+        _, info = kern(noise, s, model, math.exp(le), gj.MongeMetric(math.exp(la)), L)
+        jump = ((info.proposal.state.position - s.position) ** 2).sum(-1)
+        return (info.acceptance_rate * jump).double()
+
+    le, la = math.log(eps), math.log(alpha2)
+    fd = torch.stack([(obj(le + h, la) - obj(le - h, la)) / (2 * h), (obj(le, la + h) - obj(le, la - h)) / (2 * h)],
+                     dim=1)
+    torch.cuda.synchronize()
+    print("  tangent", r.d_objective.cpu().tolist(), "fd", fd.cpu().tolist(), flush=True)
+    assert rel_err(r.objective, obj(le, la)) < 1e-12
+    assert rel_err(r.d_objective, fd) < 1e-6
+
+
+def test_logistic_tangent_2048x64_32_steps_matches_autograd():
+    from geomjax_amd.synthetic import make_logistic
+
+    N, D, C, eps, L, max_iters = 2048, 64, 2, 0.1, 32, 6
+    print(f"case logistic {N}x{D} C={C} L={L} vs autograd", flush=True)
+    X, y, _ = make_logistic(N, D, 2)
+    q0, z = seeded_normal(210, C, D) * 0.3, seeded_normal(211, C, D)
+    obj_o, g_o = oracle_logistic_objective_grad(X, y, q0, z, eps, L, max_iters)
+    model = gj.LogisticRegression(X, y, 100.0)
+    s = gj.rmhmc.init(q0.to(DEV), model)
+    r = esjd_acceptance(gr.HostNoise(z.to(DEV), torch.zeros(C, device=DEV)), s, model, eps, gj.FisherMetric(), L,
+                        integrator=functools.partial(implicit_midpoint, max_iters=max_iters))
+    torch.cuda.synchronize()
+    print("  objective", r.objective.cpu().tolist(), "d/dlog eps", r.d_objective[:, 0].cpu().tolist(),
+          "oracle", g_o.tolist(), flush=True)
+    assert rel_err(r.objective, obj_o) < 1e-10
+    assert rel_err(r.d_objective[:, 0], g_o) < 1e-8
+
+
+def test_logistic_tangent_2048x64_32_steps_matches_central_differences():
+    from geomjax_amd.synthetic import make_logistic
+
+    N, D, C, eps, L, max_iters, h = 2048, 64, 4, 0.1, 32, 6, 1e-5
+    print(f"case logistic {N}x{D} C={C} L={L} vs central differences", flush=True)
+    X, y, _ = make_logistic(N, D, 2)
+    q0, z = seeded_normal(212, C, D) * 0.3, seeded_normal(213, C, D)
+    model = gj.LogisticRegression(X, y, 100.0)
+    integ = functools.partial(implicit_midpoint, max_iters=max_iters, convergence_tol=0.0)
+    s = gj.rmhmc.init(q0.to(DEV), model)
+    noise = gr.HostNoise(z.to(DEV), torch.zeros(C, device=DEV, dtype=torch.float64))
+    r = esjd_acceptance(noise, s, model, eps, gj.FisherMetric(), L, integrator=integ)
+    kern = build_kernel(integ)
+
+    def obj(le):
+        _, info = kern(noise, s, model, math.exp(le), gj.FisherMetric(), L)
+        jump = ((info.proposal.state.position - s.position) ** 2).sum(-1)
+        return (info.acceptance_rate * jump).double()
+
+    le = math.log(eps)
+    fd = (obj(le + h) - obj(le - h)) / (2 * h)
+    torch.cuda.synchronize()
+    print("  tangent", r.d_objective[:, 0].cpu().tolist(), "fd", fd.cpu().tolist(), flush=True)
+    assert rel_err(r.objective, obj(le)) < 1e-12
+    assert rel_err(r.d_objective[:, 0], fd) < 1e-6
