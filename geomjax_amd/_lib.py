@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "lib
 
 GJX_F64 = 0
 GJX_F32 = 1
+GJX_OK, GJX_EINVAL, GJX_EUNSUPPORTED, GJX_EHIP = 0, -1, -2, -3  # enum gjx_status
 GJX_UTURN = {"euc": 0, "bet": 1, "riem": 2}  # stopping criteria of the NUTS entry point
 
 _c_int32 = ctypes.c_int32
@@ -212,6 +213,8 @@ _SIGNATURES = {
     "gjx_funnel_value_and_grad": [_c_int32, _c_int32, _c_double, _c_int32, _vp, _vp, _vp, _vp],
     "gjx_rmhmc_fisher_logistic_step": [ctypes.POINTER(LogisticParams), ctypes.POINTER(LogisticData),
                                        ctypes.POINTER(RmhmcStepIO), _vp],
+    "gjx_rmhmc_fisher_logistic_step_noalloc": [ctypes.POINTER(LogisticParams), ctypes.POINTER(LogisticData),
+                                               ctypes.POINTER(RmhmcStepIO), _vp],
     "gjx_rmhmc_fisher_logistic_sample": [ctypes.POINTER(LogisticParams), ctypes.POINTER(LogisticData),
                                          ctypes.POINTER(RmhmcSampleIO), _vp],
     "gjx_lmc_fisher_logistic_step": [ctypes.POINTER(LogisticParams), ctypes.POINTER(LogisticData),

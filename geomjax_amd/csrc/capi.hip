@@ -5,6 +5,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -516,8 +517,10 @@ int require_f64(const gjx_logistic_params* p, const char* what) {
 
 }  // namespace
 
-int gjx_rmhmc_fisher_logistic_step(const gjx_logistic_params* params, const gjx_logistic_data* data,
-                                   const gjx_rmhmc_step_io* io, void* stream) {
+namespace {
+
+int rmhmc_logistic_step(const gjx_logistic_params* params, const gjx_logistic_data* data, const gjx_rmhmc_step_io* io,
+                        void* stream, bool concurrent) {
   gjx::LogisticArgs a;
   int rc = check_logistic(params, data, false, &a, rmhmc_max_d(params));
   if (rc != GJX_OK) return rc;
@@ -551,10 +554,79 @@ int gjx_rmhmc_fisher_logistic_step(const gjx_logistic_params* params, const gjx_
   a.fp_norm = io->fp_norm;
   a.fp_success = io->fp_success;
   a.Lc = io->num_integration_steps;
-  hipError_t e = params->dtype == GJX_F32 ? gjx::launch_logistic_rmhmc_f32(a, data->max_features, (hipStream_t)stream)
-                                           : gjx::launch_logistic_rmhmc_f64(a, (hipStream_t)stream);
+  hipError_t e = params->dtype == GJX_F32
+                     ? gjx::launch_logistic_rmhmc_f32(a, data->max_features, (hipStream_t)stream, concurrent)
+                     : gjx::launch_logistic_rmhmc_f64(a, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "gjx_rmhmc_fisher_logistic_step launch");
   return GJX_OK;
+}
+
+// Scratch of gjx_rmhmc_fisher_logistic_step_noalloc: a file-scope device array,
+// one copy per GPU, alive until the process ends.  Host code never uses the
+// array's name as an address (on the host it is only a stand-in for the
+// symbol): its device address is looked up once per device and cached.
+constexpr uint64_t kNoAllocScratchBytes = GJX_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) float scratch[kNoAllocScratchBytes / sizeof(float)];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_addr[kMaxDevices];
+
+// Device address of `scratch` on the device that runs `stream` (which must be
+// the current device: the symbol is resolved in the current device's context).
+int scratch_address(void* stream, void** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices) return fail(GJX_EUNSUPPORTED, "device %d is beyond the %d supported", dev,
+                                                kMaxDevices);
+  if (stream != nullptr) {
+    hipDevice_t sdev = -1;
+    e = hipStreamGetDevice((hipStream_t)stream, &sdev);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamGetDevice");
+    GJX_REQUIRE(sdev == dev, "the stream belongs to device %d; call with that device current (current: %d)", sdev,
+                dev);
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_addr[dev] == nullptr) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    if (p == nullptr) return fail(GJX_EHIP, "hipGetSymbolAddress(scratch) returned NULL");
+    g_scratch_addr[dev] = p;
+  }
+  *out = g_scratch_addr[dev];
+  return GJX_OK;
+}
+
+}  // namespace
+
+int gjx_rmhmc_fisher_logistic_step(const gjx_logistic_params* params, const gjx_logistic_data* data,
+                                   const gjx_rmhmc_step_io* io, void* stream) {
+  return rmhmc_logistic_step(params, data, io, stream, true);
+}
+
+int gjx_rmhmc_fisher_logistic_step_noalloc(const gjx_logistic_params* params, const gjx_logistic_data* data,
+                                           const gjx_rmhmc_step_io* io, void* stream) {
+  GJX_REQUIRE(params != nullptr && data != nullptr, "params and data are required");
+  if (params->dtype != GJX_F32)  // the float64 kernel uses no scratch
+    return rmhmc_logistic_step(params, data, io, stream, false);
+  GJX_REQUIRE(params->num_chains > 0 && data->max_rows >= 1, "num_chains and max_rows must be >= 1");
+  uint64_t need = 0;
+  workspace_bytes(params->num_chains, data->max_rows, &need);
+  if (need > kNoAllocScratchBytes)
+    return fail(GJX_EUNSUPPORTED,
+                "%d chains x %d rows need %llu bytes of scratch; the allocation-free entry point holds %llu "
+                "(use gjx_rmhmc_fisher_logistic_step with a workspace)",
+                params->num_chains, data->max_rows, (unsigned long long)need,
+                (unsigned long long)kNoAllocScratchBytes);
+  void* ws = nullptr;
+  int rc = scratch_address(stream, &ws);
+  if (rc != GJX_OK) return rc;
+  gjx_logistic_data d = *data;
+  d.workspace = ws;
+  d.workspace_bytes = kNoAllocScratchBytes;
+  return rmhmc_logistic_step(params, &d, io, stream, false);
 }
 
 int gjx_rmhmc_fisher_logistic_nuts(const gjx_logistic_params* params, const gjx_logistic_data* data,

@@ -143,7 +143,10 @@ hipError_t launch_logistic_nuts_f64(const LogisticArgs& a, const NutsArgs& n, hi
 hipError_t launch_logistic_nuts_lmc_f64(const LogisticArgs& a, const NutsArgs& n, hipStream_t stream);
 hipError_t launch_logistic_value_and_grad(const LogisticArgs& a, hipStream_t stream);
 hipError_t launch_logistic_lmc_f64(const LogisticArgs& a, hipStream_t stream);
-hipError_t launch_logistic_rmhmc_f32(const LogisticArgs& a, int max_features, hipStream_t stream);
+// concurrent = false launches every packed width on `stream` in turn (no side
+// streams or events are created)
+hipError_t launch_logistic_rmhmc_f32(const LogisticArgs& a, int max_features, hipStream_t stream,
+                                     bool concurrent = true);
 hipError_t launch_logistic_rmhmc_f32_wide(const LogisticArgs& a, hipStream_t stream);
 hipError_t launch_logistic_pack(const double* X, const double* y, const int32_t* N, const int32_t* D,
                                 const int64_t* xoff, const int64_t* yoff, const PackedPost* hdr, float* packed,

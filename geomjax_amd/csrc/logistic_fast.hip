@@ -994,7 +994,7 @@ SideStreams* side_streams() {
 
 }  // namespace
 
-hipError_t launch_logistic_rmhmc_f32(const LogisticArgs& a, int max_features, hipStream_t stream) {
+hipError_t launch_logistic_rmhmc_f32(const LogisticArgs& a, int max_features, hipStream_t stream, bool concurrent) {
   const int blocks = (a.C + 7) / 8 * 8;
   // one launch per packed width present: a single posterior needs only its
   // own; a batch runs every width up to its widest posterior's (chains of
@@ -1029,7 +1029,7 @@ hipError_t launch_logistic_rmhmc_f32(const LogisticArgs& a, int max_features, hi
   // and the caller's stream waits for both before anything after this call
   // (the pattern also holds under stream capture).  The tail of one width's
   // launch then overlaps the others.
-  SideStreams* ss = n > 1 ? side_streams() : nullptr;
+  SideStreams* ss = n > 1 && concurrent ? side_streams() : nullptr;
   if (ss == nullptr) {
     for (int i = 0; i < n; ++i) {
       const hipError_t e = launch(widths[i], stream);

@@ -343,6 +343,26 @@ int gjx_logistic_workspace_size(const gjx_logistic_params* params, const gjx_log
 int gjx_rmhmc_fisher_logistic_step(const gjx_logistic_params* params, const gjx_logistic_data* data,
                                    const gjx_rmhmc_step_io* io, void* stream);
 
+/* Bytes of the device scratch the allocation-free entry point below holds
+ * per GPU (a static array, alive until the process ends). */
+#define GJX_NOALLOC_SCRATCH_BYTES (64ull << 20)
+
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  gjx_rmhmc_fisher_logistic_step with the tuned path's
+ * per-chain scratch taken from a static per-GPU array of
+ * GJX_NOALLOC_SCRATCH_BYTES instead of data->workspace (ignored here):
+ * 5 * 4 * ceil(max_rows / 64) * 64 bytes per chain must fit, e.g. 1638 chains
+ * at 2048 rows; larger calls return GJX_EUNSUPPORTED.  The call allocates
+ * nothing: no hipMalloc / hipFree, no streams or events (a batch's packed
+ * widths run one after another on `stream`).  Call it with the stream's
+ * device current.  Every call on a GPU shares that GPU's scratch, so calls
+ * that overlap in time on one GPU race; a call needs nothing in the scratch
+ * to start at zero (every chain writes its rows before reading them).
+ * float64 calls (no scratch) behave as gjx_rmhmc_fisher_logistic_step. */
+int gjx_rmhmc_fisher_logistic_step_noalloc(const gjx_logistic_params* params, const gjx_logistic_data* data,
+                                           const gjx_rmhmc_step_io* io, void* stream);
+
 /* T RMHMC transitions per chain in one launch (one workgroup per chain). */
 int gjx_rmhmc_fisher_logistic_sample(const gjx_logistic_params* params, const gjx_logistic_data* data,
                                      const gjx_rmhmc_sample_io* io, void* stream);
