@@ -40,6 +40,14 @@ __device__ __forceinline__ f32x16 mfma3(const bf16x8& ah, const bf16x8& al, cons
   return acc;
 }
 
+// acc += (ah + al)(bh + bl) with all four products: the lo*lo term is about
+// 2^-16 of each product and, for the diagonal of a Gram, always of one sign
+__device__ __forceinline__ f32x16 mfma4(const bf16x8& ah, const bf16x8& al, const bf16x8& bh, const bf16x8& bl,
+                                        f32x16 acc) {
+  acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bl, acc, 0, 0, 0);
+  return mfma3(ah, al, bh, bl, acc);
+}
+
 // sigmoid with the hardware exp2 and reciprocal (a few ulp: the state is float32)
 __device__ __forceinline__ float sigmoidf(float x) { return __builtin_amdgcn_rcpf(1.0f + __expf(-x)); }
 __device__ __forceinline__ float sqrt_fast(float x) { return __builtin_amdgcn_sqrtf(x); }

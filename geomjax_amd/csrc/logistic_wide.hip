@@ -15,7 +15,10 @@
 // same Gram / row / split-row fragment layouts as the one-wave kernel with
 // four 32-feature blocks (eight 16-feature chunks).
 //   Gram pass: the 10 lower 32 x 32 tiles are split over the waves (3, 3, 2,
-//     2), each wave streaming only the feature blocks its tiles touch.
+//     2), each wave streaming only the feature blocks its tiles touch.  All
+//     four split products (lo*lo included) are accumulated: at up to 128
+//     features and a few hundred rows the metric's condition number reaches
+//     ~20, and the dropped term alone would put u = G^-1 p ~1e-5 off.
 //   Sweep: each lane holds an 8 x 8 block of G (16 x 16 lanes); one barrier
 //     per pivot.  Its Schur-complement columns give the momentum draw.
 //   Field pass: wave w computes the 32 output features phi(32 w + .) of
@@ -134,7 +137,7 @@ __device__ __forceinline__ void gram_tiles(Smem& S, const float* __restrict__ Xg
   auto tiles = [&](const bf16x8 (&hi)[NS], const bf16x8 (&lo)[NS]) {
 #pragma unroll
     for (int t = 0; t < NTL; ++t)
-      acc[t] = mfma3(hi[P::ta(t)], lo[P::ta(t)], hi[P::tb(t)], lo[P::tb(t)], acc[t]);
+      acc[t] = mfma4(hi[P::ta(t)], lo[P::ta(t)], hi[P::tb(t)], lo[P::tb(t)], acc[t]);
   };
   Frag fa, fb;
   bf16x8 ha[NS], la[NS], hb[NS], lb[NS];
@@ -145,11 +148,11 @@ __device__ __forceinline__ void gram_tiles(Smem& S, const float* __restrict__ Xg
     fetch(fa, kb + 2);
     tiles(ha, la);
     split_frag(fb, hb, lb);
-    interleave_mfma_valu<3 * NTL, 8 * NS / (NTL) + 1>();
+    interleave_mfma_valu<4 * NTL, 6 * NS / (NTL) + 1>();
     fetch(fb, kb + 3);
     tiles(hb, lb);
     split_frag(fa, ha, la);
-    interleave_mfma_valu<3 * NTL, 8 * NS / (NTL) + 1>();
+    interleave_mfma_valu<4 * NTL, 6 * NS / (NTL) + 1>();
   }
 #pragma unroll
   for (int t = 0; t < NTL; ++t) {

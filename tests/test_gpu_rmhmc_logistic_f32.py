@@ -273,6 +273,15 @@ def test_uneven_batch_full_size_matches_each_posterior_and_f64():
     s64n, i64 = kern(gr.HostNoise(z.to(DEV), u.to(DEV)), s64, batch, eps, gj.FisherMetric(), L)
     s32n, i32 = kern(gr.HostNoise(z.float().to(DEV), u.to(DEV)), s32, batch, eps, gj.FisherMetric(), L)
     torch.cuda.synchronize()
+    rows_err = []
+    for p, m in enumerate(models):
+        rows = slice(p * cpp, (p + 1) * cpp)
+        e = rel_err(i32.proposal.state.position[rows, :m.D], i64.proposal.state.position[rows, :m.D])
+        ev = rel_err(i32.proposal.state.velocity[rows, :m.D], i64.proposal.state.velocity[rows, :m.D])
+        rows_err.append((e, ev, p, m.N, m.D))
+    rows_err.sort(reverse=True)
+    print("  worst posteriors (proposal position err, velocity err, index, N, D):",
+          [(f"{a:.2e}", f"{b:.2e}", p, n, d) for a, b, p, n, d in rows_err[:8]], flush=True)
     _check_step(i32, s32n, i64, s64n, u, q0, "uneven batch, tuned vs float64")
     worst = 0.0
     for p, m in enumerate(models):
