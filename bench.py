@@ -20,7 +20,11 @@ A "step" is one transition of all 4096 chains (10 integrator steps each).
 
 ``value`` = chains * 10 * steps / wall seconds of the timed region (inputs
 resident in HBM).  min-ESS/s uses the multi-chain ESS of
-geomjax/diagnostics.py:78-209 over the timed samples.  Extra fields report the
+geomjax/diagnostics.py:78-209 over a fixed window of post-warm-up transitions
+(100 for the 2048 x 64 posterior, 40 for the uneven batch; at least 20
+warm-up transitions), independent of --steps, divided by the window's own wall
+time; the tuned and the plain form run that window from the same warmed state
+with the same device draws, so their min-ESS estimates are paired.  Extra fields report the
 same comparison at 1024 chains, for 256 logistic posteriors of uneven size in
 one launch (8192 and 4096 chains), for Neal's funnel D=100 (RMHMC, NUTS-RMHMC
 and LMC with the Monge metric, 8192 chains), and the cost of the tuning
@@ -45,6 +49,9 @@ METRIC = "min-ESS/sec & leapfrog-steps/sec, 4096 chains, tuned vs plain GPU form
 # logistic regression (headline)
 N_ROWS, D_LOG, CHAINS, EPS_LOG, L_LOG, MAXIT_LOG, PRIOR, DATA_SEED, INIT_SEED = 2048, 64, 4096, 0.1, 10, 6, 100.0, 2, 1
 CHUNK_LOG = 10  # transitions per persistent launch (tuned form)
+WARM_MIN = 20  # untimed warm-up transitions at least (whatever --warmup says)
+# min-ESS/s windows: fixed numbers of post-warm-up transitions, independent of --steps
+ESS_WINDOW_LOG, ESS_WINDOW_BATCH = 100, 40
 # Neal's funnel (secondary)
 D_F, CHAINS_F, EPS_F, L_F, MAXIT_F, ALPHA2, SIGMA, SEED_F = 100, 8192, 0.03, 32, 8, 1.0, 3.0, 1
 CHUNK_F = 100
@@ -136,13 +143,88 @@ def cpu_baseline(seconds_budget=30.0):
                       f"Fisher metric, Cholesky + autodiff restatement, {t_used:.1f} s)"}
 
 
+def _warm(state, model, key, counter, W, integ, **kw):
+    """Untimed tuned-form warm-up of at least WARM_MIN transitions (the first
+    call also autotunes the launch shape)."""
+    from geomjax_amd import random as gr
+    from geomjax_amd.rmhmc import sample
+
+    for _ in range(max(WARM_MIN, W) // CHUNK_LOG):
+        r = sample(gr.PRNGKey(key.seed, counter), state, model, EPS_LOG, _fisher(), L_LOG, CHUNK_LOG, integrator=integ,
+                   store_samples=False, **kw)
+        state, counter = r.state, counter + CHUNK_LOG
+    return state, counter
+
+
+def _fisher():
+    import geomjax_amd as gj
+
+    return gj.FisherMetric()
+
+
+def paired_ess_window(model, warmed, key, integ, window, plain, ess_fn):
+    """min-ESS/s of the tuned and the plain form over the same fixed window of
+    `window` transitions after warm-up, both started from the same warmed
+    state and fed the same device draws (Philox streams keyed by chain and
+    transition), so the two min-ESS estimates are paired and their ratio is not
+    dominated by the estimator's noise.  min-ESS/s = min-ESS of the window's
+    samples / the window's wall time."""
+    import torch
+
+    import geomjax_amd as gj
+    from geomjax_amd import random as gr
+    from geomjax_amd.rmhmc import build_kernel, sample
+
+    C, D = warmed.position.shape
+    dev = warmed.position.device
+    base = 500_000
+    samples = torch.empty(window, C, D, dtype=torch.float32, device=dev)
+    state = warmed
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for c in range(window // CHUNK_LOG):
+        r = sample(gr.PRNGKey(key.seed, base + c * CHUNK_LOG), state, model, EPS_LOG, _fisher(), L_LOG, CHUNK_LOG,
+                   integrator=integ, samples_out=samples[c * CHUNK_LOG:(c + 1) * CHUNK_LOG])
+        state = r.state
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ess = ess_fn(samples)
+    del samples
+    out = {"window_transitions": window, "draws": "paired (same Philox streams)",
+           "tuned": {"min_ess": ess, "min_ess_per_sec": ess / wall, "wall_s": wall, "ms_per_step": wall / window * 1e3}}
+    if plain:
+        kern = build_kernel(integ)
+        s64 = gj.rmhmc.init(warmed.position.double(), model)
+        s64, _ = kern(gr.PRNGKey(key.seed, base - 1), s64, model, EPS_LOG, _fisher(), L_LOG)  # untimed: first launch
+        s64 = gj.rmhmc.init(warmed.position.double(), model)
+        psamples = torch.empty(window, C, D, dtype=torch.float64, device=dev)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for t in range(window):
+            s64, _ = kern(gr.PRNGKey(key.seed, base + t), s64, model, EPS_LOG, _fisher(), L_LOG)
+            psamples[t].copy_(s64.position)
+        torch.cuda.synchronize()
+        pwall = time.perf_counter() - t0
+        pess = ess_fn(psamples)
+        # how closely the paired chains track each other at the window's end
+        drift = float((psamples[-1] - state.position.double()).abs().max().item())
+        del psamples
+        out["plain"] = {"min_ess": pess, "min_ess_per_sec": pess / pwall, "wall_s": pwall,
+                        "ms_per_step": pwall / window * 1e3, "value": C * L_LOG * window / pwall,
+                        "unit": "leapfrog-steps/s", "form": "float64 kernel, one launch per transition"}
+        out["max_abs_state_difference_at_window_end"] = drift
+        out["tuned_vs_plain_min_ess_per_sec"] = out["tuned"]["min_ess_per_sec"] / out["plain"]["min_ess_per_sec"]
+    return out
+
+
 def logistic_forms(chains, K, W, dev, plain=True):
     """Tuned and plain RMHMC-Fisher on the 2048 x 64 logistic posterior with `chains` chains."""
     import torch
 
     import geomjax_amd as gj
+    from geomjax_amd import autotune
     from geomjax_amd import random as gr
-    from geomjax_amd.rmhmc import build_kernel, implicit_midpoint, sample
+    from geomjax_amd.rmhmc import implicit_midpoint, sample
     from geomjax_amd.synthetic import make_logistic
 
     X, y, _ = make_logistic(N_ROWS, D_LOG, DATA_SEED)
@@ -152,15 +234,9 @@ def logistic_forms(chains, K, W, dev, plain=True):
     key = gr.key(INIT_SEED)
     K = max(CHUNK_LOG, (K // CHUNK_LOG) * CHUNK_LOG)
 
-    # ---- tuned ----
-    state = gj.rmhmc.init(q_init.to(dev, torch.float32), model)
-    counter = 10_000
-    for _ in range(max(1, W // CHUNK_LOG)):
-        r = sample(gr.PRNGKey(key.seed, counter), state, model, EPS_LOG, metric, L_LOG, CHUNK_LOG,
-                   integrator=integ, store_samples=False)
-        state, counter = r.state, counter + CHUNK_LOG
+    # ---- tuned: warm-up, then the timed region (leapfrog-steps/s) ----
+    state, counter = _warm(gj.rmhmc.init(q_init.to(dev, torch.float32), model), model, key, 10_000, W, integ)
     warmed = state
-    samples = torch.empty(K, chains, D_LOG, dtype=torch.float32, device=dev)
     fevs, accs = [], []
     stream = torch.cuda.current_stream(dev)
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -169,7 +245,7 @@ def logistic_forms(chains, K, W, dev, plain=True):
     ev0.record(stream)
     for c in range(K // CHUNK_LOG):
         r = sample(gr.PRNGKey(key.seed, counter), state, model, EPS_LOG, metric, L_LOG, CHUNK_LOG,
-                   integrator=integ, samples_out=samples[c * CHUNK_LOG:(c + 1) * CHUNK_LOG])
+                   integrator=integ, store_samples=False)
         state, counter = r.state, counter + CHUNK_LOG
         fevs.append(r.num_fevals)
         accs.append(r.acceptance_rate)
@@ -178,42 +254,28 @@ def logistic_forms(chains, K, W, dev, plain=True):
     wall = time.perf_counter() - t0
     kernel_s = ev0.elapsed_time(ev1) / 1e3
     fev_total = int(torch.stack(fevs).sum().item())
-    ess, ess_arg = min_ess(samples)
-    del samples
     n_launch = K // CHUNK_LOG
-    tuned = {"value": chains * L_LOG * K / wall, "unit": "leapfrog-steps/s", "min_ess_per_sec": ess / wall,
-             "min_ess": ess, "min_ess_coord": ess_arg, "ms_per_step": wall / K * 1e3,
+    waves = autotune.logistic_waves_field(model, chains)
+    tuned = {"value": chains * L_LOG * K / wall, "unit": "leapfrog-steps/s", "ms_per_step": wall / K * 1e3,
              "acceptance_rate": float(torch.stack(accs).mean().item()),
              "fevals_per_chain_transition": fev_total / chains / K, "dtype": "f32 state, bf16x3 MFMA, f64 energies",
              "launches": n_launch, "avg_launch_ms": kernel_s / n_launch * 1e3,
-             "mfma_flops_per_launch": mfma_flops_logistic(fev_total) / n_launch, "steps": K}
+             "mfma_flops_per_launch": mfma_flops_logistic(fev_total) / n_launch, "steps": K,
+             "waves_per_chain": waves & 0xF or waves >> 4}
     log(f"logistic tuned, {chains} chains: {tuned['value']:.3e} leapfrog-steps/s, {tuned['ms_per_step']:.2f} ms/step, "
-        f"min-ESS/s {tuned['min_ess_per_sec']:.3e}, acc {tuned['acceptance_rate']:.3f}, "
-        f"fevals/chain/tr {tuned['fevals_per_chain_transition']:.1f}")
+        f"acc {tuned['acceptance_rate']:.3f}, fevals/chain/tr {tuned['fevals_per_chain_transition']:.1f}, "
+        f"waves per chain {tuned['waves_per_chain']} (autotuned)")
 
-    res = {"tuned": tuned, "plain": None}
+    win = paired_ess_window(model, warmed, key, integ, ESS_WINDOW_LOG, plain, lambda x: min_ess(x)[0])
+    tuned["min_ess_per_sec"] = win["tuned"]["min_ess_per_sec"]
+    tuned["min_ess"] = win["tuned"]["min_ess"]
+    res = {"tuned": tuned, "plain": win.get("plain"), "ess_window": win}
     if plain:
-        kern = build_kernel(integ)
-        s64 = gj.rmhmc.init(warmed.position.double(), model)
-        s64, _ = kern(gr.PRNGKey(key.seed + 1, 0), s64, model, EPS_LOG, metric, L_LOG)  # untimed
-        psamples = torch.empty(K, chains, D_LOG, dtype=torch.float64, device=dev)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for t in range(K):
-            s64, _ = kern(gr.PRNGKey(key.seed + 1, 1 + t), s64, model, EPS_LOG, metric, L_LOG)
-            psamples[t].copy_(s64.position)
-        torch.cuda.synchronize()
-        pwall = time.perf_counter() - t0
-        pess, _ = min_ess(psamples)
-        del psamples
-        res["plain"] = {"value": chains * L_LOG * K / pwall, "unit": "leapfrog-steps/s",
-                        "min_ess_per_sec": pess / pwall, "min_ess": pess, "ms_per_step": pwall / K * 1e3,
-                        "dtype": "f64", "steps": K,
-                        "form": "float64 kernel, one launch per transition"}
-        res["tuned_vs_plain"] = {"leapfrog_steps_per_sec": tuned["value"] / res["plain"]["value"],
-                                 "min_ess_per_sec": tuned["min_ess_per_sec"] / res["plain"]["min_ess_per_sec"]}
-        log(f"logistic plain, {chains} chains: {res['plain']['value']:.3e} leapfrog-steps/s, "
-            f"min-ESS/s {res['plain']['min_ess_per_sec']:.3e}; tuned/plain {res['tuned_vs_plain']}")
+        res["tuned_vs_plain"] = {"leapfrog_steps_per_sec": tuned["value"] / win["plain"]["value"],
+                                 "min_ess_per_sec": win["tuned_vs_plain_min_ess_per_sec"]}
+        log(f"logistic plain, {chains} chains: {win['plain']['value']:.3e} leapfrog-steps/s; "
+            f"min-ESS/s tuned {win['tuned']['min_ess_per_sec']:.3e} plain {win['plain']['min_ess_per_sec']:.3e} "
+            f"over {ESS_WINDOW_LOG} paired transitions; tuned/plain {res['tuned_vs_plain']}")
     return res
 
 
@@ -236,12 +298,14 @@ def uneven_batch_forms(chains_per_posterior, K, W, dev, plain=True):
 
     Posteriors as the uneven-batch configuration generates them: D drawn from
     {8, 16, 32, 64, 128}, N log-uniform in [256, 4096], data seed 3.  Chains of
-    posteriors up to 64 features run one wave each, wider ones four waves."""
+    posteriors up to 64 features run one workgroup of one or two waves each
+    (chosen per width by the autotuner), wider ones four waves."""
     import torch
 
     import geomjax_amd as gj
+    from geomjax_amd import autotune
     from geomjax_amd import random as gr
-    from geomjax_amd.rmhmc import build_kernel, implicit_midpoint, sample
+    from geomjax_amd.rmhmc import implicit_midpoint, sample
     from geomjax_amd.synthetic import make_uneven_batch
 
     data = make_uneven_batch(P=256, seed=3, dims=(8, 16, 32, 64, 128))
@@ -262,64 +326,52 @@ def uneven_batch_forms(chains_per_posterior, K, W, dev, plain=True):
     work = sum(n * m.N * m.D * m.D for n, m in zip(counts, models))
     log(f"uneven batch: {len(models)} posteriors, {C} chains, sum chains*N*D^2 = {work:.3e}")
 
-    state = gj.rmhmc.init(q_init.to(dev, torch.float32), batch)
-    counter = 10_000
-    for _ in range(max(1, W // CHUNK_LOG)):
-        r = sample(gr.PRNGKey(key.seed, counter), state, batch, EPS_LOG, metric, L_LOG, CHUNK_LOG,
-                   integrator=integ, store_samples=False)
-        state, counter = r.state, counter + CHUNK_LOG
+    state, counter = _warm(gj.rmhmc.init(q_init.to(dev, torch.float32), batch), batch, key, 10_000, W, integ)
     warmed = state
-    samples = torch.empty(K, C, Dm, dtype=torch.float32, device=dev)
     fevs = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for c in range(K // CHUNK_LOG):
         r = sample(gr.PRNGKey(key.seed, counter), state, batch, EPS_LOG, metric, L_LOG, CHUNK_LOG,
-                   integrator=integ, samples_out=samples[c * CHUNK_LOG:(c + 1) * CHUNK_LOG])
+                   integrator=integ, store_samples=False)
         state, counter = r.state, counter + CHUNK_LOG
         fevs.append(r.num_fevals)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    ess = batch_min_ess(samples, counts, dims)
-    del samples
-    tuned = {"value": C * L_LOG * K / wall, "unit": "leapfrog-steps/s", "min_ess_per_sec": ess / wall,
-             "min_ess": ess, "ms_per_step": wall / K * 1e3, "steps": K,
-             "fevals_per_chain_transition": float(torch.stack(fevs).double().sum().item()) / C / K}
+    waves = autotune.logistic_waves_field(batch, C)
+    tuned = {"value": C * L_LOG * K / wall, "unit": "leapfrog-steps/s", "ms_per_step": wall / K * 1e3, "steps": K,
+             "fevals_per_chain_transition": float(torch.stack(fevs).double().sum().item()) / C / K,
+             "waves_per_chain": {"D<=32": waves & 0xF, "33<=D<=64": waves >> 4, "65<=D<=128": 4,
+                                 "chosen_by": "autotune.tune_logistic (0 = library default)"}}
     log(f"uneven batch tuned: {tuned['value']:.3e} leapfrog-steps/s, {tuned['ms_per_step']:.2f} ms/step, "
-        f"min-ESS/s {tuned['min_ess_per_sec']:.3e}")
+        f"waves per chain {tuned['waves_per_chain']}")
+    win = paired_ess_window(batch, warmed, key, integ, ESS_WINDOW_BATCH, plain,
+                            lambda x: batch_min_ess(x, counts, dims))
+    tuned["min_ess_per_sec"] = win["tuned"]["min_ess_per_sec"]
+    tuned["min_ess"] = win["tuned"]["min_ess"]
     res = {"posteriors": len(models), "chains": C, "dims": "D in {8, 16, 32, 64, 128}, N in [256, 4096]",
-           "tuned": tuned, "plain": None}
+           "tuned": tuned, "plain": win.get("plain"), "ess_window": win}
     if plain:
-        kern = build_kernel(integ)
-        s64 = gj.rmhmc.init(warmed.position.double(), batch)
-        s64, _ = kern(gr.PRNGKey(key.seed + 1, 0), s64, batch, EPS_LOG, metric, L_LOG)
-        psamples = torch.empty(K, C, Dm, dtype=torch.float64, device=dev)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for t in range(K):
-            s64, _ = kern(gr.PRNGKey(key.seed + 1, 1 + t), s64, batch, EPS_LOG, metric, L_LOG)
-            psamples[t].copy_(s64.position)
-        torch.cuda.synchronize()
-        pwall = time.perf_counter() - t0
-        pess = batch_min_ess(psamples, counts, dims)
-        del psamples
-        res["plain"] = {"value": C * L_LOG * K / pwall, "unit": "leapfrog-steps/s", "min_ess_per_sec": pess / pwall,
-                        "min_ess": pess, "ms_per_step": pwall / K * 1e3, "dtype": "f64", "steps": K,
-                        "form": "float64 kernel, one launch per transition"}
-        res["tuned_vs_plain"] = {"leapfrog_steps_per_sec": tuned["value"] / res["plain"]["value"],
-                                 "min_ess_per_sec": tuned["min_ess_per_sec"] / res["plain"]["min_ess_per_sec"]}
-        log(f"uneven batch plain: {res['plain']['value']:.3e} leapfrog-steps/s; tuned/plain {res['tuned_vs_plain']}")
+        res["tuned_vs_plain"] = {"leapfrog_steps_per_sec": tuned["value"] / win["plain"]["value"],
+                                 "min_ess_per_sec": win["tuned_vs_plain_min_ess_per_sec"]}
+        log(f"uneven batch plain: {win['plain']['value']:.3e} leapfrog-steps/s; tuned/plain {res['tuned_vs_plain']}")
     return res
 
 
 def funnel_forms(K, W, dev, plain=True):
-    """Tuned and plain RMHMC-Monge on Neal's funnel D=100 with 8192 chains."""
+    """Tuned and plain RMHMC-Monge on Neal's funnel D=100 with 8192 chains.
+
+    The tuned form runs as the configuration states it: each chunk of
+    CHUNK_F transitions is one persistent launch captured with its noise draw
+    and state copy in a HIP graph (GraphSampler), replayed per chunk; the
+    launch shape (lanes per chain) comes from the autotuner."""
     import torch
 
     import geomjax_amd as gj
     from geomjax_amd import autotune
     from geomjax_amd import random as gr
     from geomjax_amd.rmhmc import build_kernel, implicit_midpoint, sample
+    from geomjax_amd.sampling import GraphSampler
 
     K = max(CHUNK_F, (K // CHUNK_F) * CHUNK_F)
     model, metric = gj.NealFunnel(D_F, sigma=SIGMA), gj.MongeMetric(ALPHA2)
@@ -336,8 +388,17 @@ def funnel_forms(K, W, dev, plain=True):
     log(f"funnel autotune lanes_per_chain: {lpc} ({ {k: round(v * 1e3, 2) for k, v in timings.items()} } ms)")
     counter, state = 10_000, s32
     for _ in range(max(1, W // CHUNK_F)):
-        r = run(lpc, state, gr.PRNGKey(key.seed, counter))
+        r = run(0, state, gr.PRNGKey(key.seed, counter))  # 0: the autotuned shape, as the product path takes it
         state, counter = r.state, counter + CHUNK_F
+    # eager persistent launches (no graph), for comparison
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = run(0, state, gr.PRNGKey(key.seed, counter))
+    torch.cuda.synchronize()
+    eager_ms = (time.perf_counter() - t0) * 1e3
+    alg = gj.rmhmc(model, EPS_F, metric, L_F, integrator=integ)
+    gs = GraphSampler(alg, state, CHUNK_F)  # captured once: noise draw + persistent launch + state copy
+    gs.replay()
     samples = torch.empty(K, CHAINS_F, D_F, dtype=torch.float32, device=dev)
     fevs, accs = [], []
     stream = torch.cuda.current_stream(dev)
@@ -346,14 +407,15 @@ def funnel_forms(K, W, dev, plain=True):
     t0 = time.perf_counter()
     ev0.record(stream)
     for c in range(K // CHUNK_F):
-        r = run(lpc, state, gr.PRNGKey(key.seed, counter), out=samples[c * CHUNK_F:(c + 1) * CHUNK_F])
-        state, counter = r.state, counter + CHUNK_F
-        fevs.append(r.num_fevals)
-        accs.append(r.acceptance_rate)
+        gs.replay()
+        samples[c * CHUNK_F:(c + 1) * CHUNK_F].copy_(gs.samples)
+        fevs.append(gs.num_fevals.clone())
+        accs.append(gs.acceptance_rate.mean())
     ev1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     kernel_s = ev0.elapsed_time(ev1) / 1e3
+    state = gs.state
     fev_total = int(torch.stack(fevs).sum().item())
     ess, _ = min_ess(samples)
     del samples
@@ -361,7 +423,8 @@ def funnel_forms(K, W, dev, plain=True):
     flops = flops_rmhmc_monge_funnel(fev_total, CHAINS_F, K)
     tuned = {"value": CHAINS_F * L_F * K / wall, "unit": "leapfrog-steps/s", "min_ess_per_sec": ess / wall,
              "min_ess": ess, "acceptance_rate": float(torch.stack(accs).mean().item()), "lanes_per_chain": lpc,
-             "kernel": f"funnel_rmhmc_kernel<float,{lpc},*,false>", "avg_launch_ms": kernel_s / n_launch * 1e3,
+             "kernel": f"funnel_rmhmc_kernel<float,{lpc},*,false>", "form": "HIP graph replays (GraphSampler)",
+             "avg_replay_ms": kernel_s / n_launch * 1e3, "eager_persistent_launch_ms": eager_ms,
              "achieved_tflops": flops / kernel_s / 1e12, "frac_fp32_peak": flops / kernel_s / FP32_PEAK,
              "steps": K}
     log(f"funnel tuned: {tuned['value']:.3e} leapfrog-steps/s, min-ESS/s {tuned['min_ess_per_sec']:.3e}")
@@ -832,7 +895,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the 1024-chain and funnel comparisons")
-    ap.add_argument("--only", choices=["funnel_nuts", "logistic_nuts", "logistic_nuts_lmc", "lmc_alpha2", "logistic_lmc", "mmala",
+    ap.add_argument("--only", choices=["headline", "uneven4096", "funnel_nuts", "logistic_nuts", "logistic_nuts_lmc", "lmc_alpha2", "logistic_lmc", "mmala",
                              "uneven", "dense_lmc", "dense_rmhmc"],
                     help="run one secondary workload alone and print its JSON (for profiling it)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -868,6 +931,12 @@ def main():
     if args.only == "uneven":
         print(json.dumps(uneven_batch_forms(32, K, W, dev, plain=not args.no_plain)), flush=True)
         return
+    if args.only == "uneven4096":
+        print(json.dumps(uneven_batch_forms(16, K, W, dev, plain=not args.no_plain)), flush=True)
+        return
+    if args.only == "headline":
+        print(json.dumps(logistic_forms(CHAINS, K, W, dev, plain=not args.no_plain)), flush=True)
+        return
     if args.only == "dense_rmhmc":
         print(json.dumps(dense_rmhmc_forms(dev)), flush=True)
         return
@@ -890,7 +959,8 @@ def main():
     roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": BF16_PEAK / 1e12, "unit": "TFLOP/s",
                 "frac": achieved / BF16_PEAK, "traffic": measured_traffic(),
                 "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT),
-                "kernel": "logistic_rmhmc_f32_kernel<2,false>", "launches": tuned["launches"],
+                "kernel": f"logistic_rmhmc_f32_kernel<2,{tuned['waves_per_chain']},false>",
+                "launches": tuned["launches"],
                 "avg_launch_ms": tuned["avg_launch_ms"],
                 "algorithmic_flops_per_launch": tuned["mfma_flops_per_launch"]}
 
@@ -938,6 +1008,8 @@ def main():
                    "transitions_per_launch": CHUNK_LOG},
         "min_ess_per_sec": tuned["min_ess_per_sec"],
         "min_ess": tuned["min_ess"],
+        "ess_window": {k: head["ess_window"][k] for k in ("window_transitions", "draws")}
+        | {"warmup_transitions": max(WARM_MIN, W) // CHUNK_LOG * CHUNK_LOG},
         "acceptance_rate": tuned["acceptance_rate"],
         "plain": head["plain"],
         "tuned_vs_plain": head.get("tuned_vs_plain"),

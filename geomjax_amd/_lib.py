@@ -122,7 +122,7 @@ class LogisticParams(ctypes.Structure):
         ("max_iters", _c_int32),
         ("num_transitions", _c_int32),
         ("flags", _c_int32),
-        ("reserved", _c_int32),
+        ("waves_per_chain", _c_int32),
         ("step_size", _c_double),
         ("prior_var", _c_double),
         ("convergence_tol", _c_double),

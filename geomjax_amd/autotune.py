@@ -54,3 +54,107 @@ def tune(kind, dim, chains, dtype, run, candidates=LANES_PER_CHAIN, repeats=3):
     with _lock:
         _table.setdefault(key, best_lpc)
     return best_lpc, timings
+
+
+# ---------------------------------------------------------------------------
+# Tuned logistic kernels: workgroup size per chain (1 or 2 waves) for the
+# posteriors of up to 32 and of 33..64 features, chosen per problem.
+# ---------------------------------------------------------------------------
+
+enabled = True  # False: every call takes the library default without timing
+_logistic = {}
+
+
+def _problem_key(model, chains):
+    from .models import LogisticBatch
+
+    if isinstance(model, LogisticBatch):
+        shape = tuple((m.N, m.D, n) for m, n in zip(model.models, model.counts))
+    else:
+        shape = ((model.N, model.D, int(chains)),)
+    return shape, int(chains)
+
+
+def logistic_waves_field(model, chains):
+    """The C ABI waves_per_chain field for the tuned path on ``model`` with
+    ``chains`` chains: the cached choice, tuned on first use unless tuning is
+    disabled or the current stream is being captured (then the library
+    default, 0)."""
+    key = _problem_key(model, chains)
+    with _lock:
+        if key in _logistic:
+            return _logistic[key]
+    if not enabled or torch.cuda.is_current_stream_capturing():
+        return 0
+    return tune_logistic(model, chains)[0]
+
+
+LOGISTIC_WAVES = (0x11, 0x22, 0x21, 0x12)  # bits 0-3: D <= 32, bits 4-7: 33..64 features
+
+
+def tune_logistic(model, chains, repeats=2, num_integration_steps=4, max_iters=6, step_size=0.05):
+    """Time one short persistent launch of the tuned RMHMC kernel per
+    waves_per_chain candidate on the caller's device (positions drawn near
+    the origin, the device PRNG; nothing of the caller's state is touched)
+    and cache the fastest for (problem, chains).  Candidates that differ only
+    in a width the problem does not have are not timed twice.  Returns
+    (field, {field: seconds})."""
+    import functools
+
+    from .random import PRNGKey
+    from .rmhmc import implicit_midpoint, init, sample
+
+    key = _problem_key(model, chains)
+    with _lock:
+        if key in _logistic:
+            return _logistic[key], {}
+    widths = {1 if d <= 32 else (2 if d <= 64 else 4) for (_, d, _) in key[0]}
+    cands, seen = [], set()
+    for f in LOGISTIC_WAVES:
+        sig = tuple((f >> (4 * (nb - 1))) & 0xF for nb in (1, 2) if nb in widths)
+        if sig and sig not in seen:
+            seen.add(sig)
+            cands.append(f)
+    if not cands:  # only wide posteriors: the four-wave kernel has no choice
+        with _lock:
+            _logistic.setdefault(key, 0)
+        return 0, {}
+    dev = torch.device("cuda", torch.cuda.current_device())
+    g = torch.Generator(device=dev)
+    g.manual_seed(12345)
+    q = 0.1 * torch.randn(int(chains), model.dim, generator=g, device=dev, dtype=torch.float32)
+    if hasattr(model, "counts"):  # zero the coordinates past each posterior's D
+        mask = torch.zeros_like(q)
+        c0 = 0
+        for m, n in zip(model.models, model.counts):
+            mask[c0:c0 + n, :m.D] = 1.0
+            c0 += n
+        q = q * mask
+    state = init(q, model)
+    integ = functools.partial(implicit_midpoint, max_iters=max_iters)
+
+    def run(field):
+        sample(PRNGKey(7, 0), state, model, step_size, _metric(), num_integration_steps, 1, integrator=integ,
+               store_samples=False, waves_per_chain=(field & 0xF, field >> 4))
+
+    timings = {}
+    for f in cands:
+        run(f)
+        torch.cuda.synchronize()
+        best = float("inf")
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            run(f)
+            torch.cuda.synchronize()
+            best = min(best, time.perf_counter() - t0)
+        timings[f] = best
+    best_f = min(timings, key=timings.get)
+    with _lock:
+        _logistic.setdefault(key, best_f)
+    return best_f, timings
+
+
+def _metric():
+    from .metrics import FisherMetric
+
+    return FisherMetric()

@@ -476,6 +476,14 @@ int check_logistic(const gjx_logistic_params* p, const gjx_logistic_data* d, boo
   a->Dmax = p->dim;
   a->P = d->num_posteriors;
   a->max_features = d->max_features;
+  {
+    const int v = p->waves_per_chain;
+    GJX_REQUIRE(v >= 0 && v <= 0xFF, "waves_per_chain must be in [0, 0xFF] (got %d)", v);
+    const int w1 = v & 0xF, w2 = (v >> 4) & 0xF;
+    GJX_REQUIRE(w1 <= 2 && w2 <= 2, "waves_per_chain: each 4-bit field must be 0, 1 or 2 (got 0x%x)", v);
+    a->waves[0] = w1 ? w1 : gjx::kDefaultWavesPerChain[0];
+    a->waves[1] = w2 ? w2 : gjx::kDefaultWavesPerChain[1];
+  }
   a->eps = p->step_size;
   a->prior_var = p->prior_var;
   a->conv_tol = p->convergence_tol;

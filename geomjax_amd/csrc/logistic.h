@@ -77,6 +77,9 @@ struct LogisticArgs {
   float* work;
   int64_t work_rows;
   const int32_t* dispatch_order;  // [C] chain of each dispatch slot (tuned kernels) or null
+  // tuned one-workgroup-per-chain kernels of posteriors up to 32 / 64
+  // features: waves per chain (1 or 2)
+  int waves[2];
   // manifold MALA: metrics [C, Dmax, Dmax] and Gamma vectors [C, Dmax] of the
   // returned state and of the proposal; mmala_exact selects +log alpha as the
   // proposal weight (0: the reference's pairing, -log alpha)
@@ -148,6 +151,9 @@ hipError_t launch_logistic_lmc_f64(const LogisticArgs& a, hipStream_t stream);
 hipError_t launch_logistic_rmhmc_f32(const LogisticArgs& a, int max_features, hipStream_t stream,
                                      bool concurrent = true);
 hipError_t launch_logistic_rmhmc_f32_wide(const LogisticArgs& a, hipStream_t stream);
+// waves per chain of the tuned kernels for posteriors of up to 32 / 64
+// features when the caller leaves the choice to the library
+constexpr int kDefaultWavesPerChain[2] = {2, 2};
 hipError_t launch_logistic_pack(const double* X, const double* y, const int32_t* N, const int32_t* D,
                                 const int64_t* xoff, const int64_t* yoff, const PackedPost* hdr, float* packed,
                                 int P, int max_npad, hipStream_t stream);

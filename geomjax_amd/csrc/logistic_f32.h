@@ -21,13 +21,24 @@ __device__ __forceinline__ void load8(const float* p, float (&v)[8]) {
   v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
 }
 
-// v = hi + lo + O(2^-17 |v|), hi and lo in bf16 (round to nearest even)
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+// v = hi + lo + O(2^-17 |v|), hi and lo in bf16 (round to nearest even).
+// Two values per conversion (v_cvt_pk_bf16_f32); hi is widened back to
+// float32 by placing its bits in the upper half of a word.
 __device__ __forceinline__ void split8(const float (&v)[8], bf16x8& hi, bf16x8& lo) {
 #pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    const __bf16 h = (__bf16)v[i];
-    hi[i] = h;
-    lo[i] = (__bf16)(v[i] - (float)h);
+  for (int i = 0; i < 8; i += 2) {
+    const bf16x2 h = __builtin_convertvector((f32x2{v[i], v[i + 1]}), bf16x2);
+    const uint32_t hb = __builtin_bit_cast(uint32_t, h);
+    const float h0 = __builtin_bit_cast(float, hb << 16);
+    const float h1 = __builtin_bit_cast(float, hb & 0xffff0000u);
+    const bf16x2 r = __builtin_convertvector((f32x2{v[i] - h0, v[i + 1] - h1}), bf16x2);
+    hi[i] = h[0];
+    hi[i + 1] = h[1];
+    lo[i] = r[0];
+    lo[i + 1] = r[1];
   }
 }
 

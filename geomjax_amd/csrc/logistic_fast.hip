@@ -11,8 +11,10 @@
 // a_n = x_n^T G^-1 x_n, b = X u):
 //   dH/dp = u = G^-1 p,   dH/dq = X^T [ w (a - b^2) / 2 - r ] + q / prior_var.
 //
-// Layout.  One wave (64 lanes) per chain, no cross-wave synchronisation: each
-// chain runs its own fixed-point iteration count.  The design matrix is
+// Layout.  One workgroup per chain, of one wave (64 lanes) or two (the rows of
+// every pass over X split between them, the inverse on the first; chosen per
+// problem by the caller's waves_per_chain): each chain runs its own
+// fixed-point iteration count.  The design matrix is
 // packed once into two float32 fragment layouts (logistic_pack_kernel):
 //   Gram layout: for a 16-row K step, lane l holds X[k0 + (l/32)*8 + i][32b + l%32]
 //     (i < 8) -- the A and B operands of v_mfma_f32_32x32x16_bf16 for the Gram
@@ -43,8 +45,11 @@
 // sweep operator (Goodnight 1979) gives -G^-1 and the pivots d_k, whose logs
 // sum (float64) to log det G.  At pivot k the not-yet-swept column k is the
 // Schur complement, so the Cholesky factor column L[:, k] = A[:, k] / sqrt(d_k)
-// is available for the momentum draw p0 = L z at no extra cost.
+// is available for the momentum draw p0 = L z at no extra cost.  Every
+// evaluation uses this sweep (a 32 x 32 blocked form with four MFMA products
+// measured the same per evaluation on one wave and twice as slow on two).
 #include <mutex>
+#include <type_traits>
 
 #include "common.h"
 #include "logistic.h"
@@ -57,39 +62,87 @@ using namespace lf32;
 
 constexpr int MAXD = 64;
 constexpr int LDG = MAXD + 4;  // LDS row stride of G^-1 (floats; rows stay 16-byte aligned)
-constexpr int LDW = 33;        // padded row stride of the 32 x 32 block W
 
 struct Smem {
   float G[MAXD * MAXD];  // Gram G (row stride 64), kept for the refinement step
-  float Gi[MAXD * LDG];  // G^-1 and the blocks of its computation; scratch of the transpose-reductions
-  float W[32 * LDW];     // W = A^-1 B of the blocked inverse
-  float bc[2][2][32];    // 32 x 32 sweep: column broadcast [pivot parity][half][row]
+  float Gi[MAXD * LDG];  // G^-1; scratch of the partial-sum exchanges between lanes and waves
+  float bc[2][64];       // sweep: column broadcast [pivot parity][row]
   float dk[MAXD];        // pivots
   float q0[MAXD], p0[MAXD], q[MAXD], p[MAXD], u[MAXD], dH[MAXD], z[MAXD], g[MAXD];
+  double red[2];         // per-wave partial sums of the two-wave form
 };
+
+// With NW waves per chain the rows of every pass over X are dealt to the
+// waves in groups of 64 (group g to wave g % NW), so that each row's scratch
+// (eta, sqrt(lambda)) is only ever read and written by one wave; the metric
+// inverse runs on wave 0 alone.  Synchronisation of work that wave 0 does on
+// its own: the whole workgroup for one wave, else the wave itself (the LDS
+// accesses of one wave complete in order; the fences and the wave barrier
+// keep the compiler from moving them across this point).
+template <int NW>
+__device__ __forceinline__ void wave0_sync() {
+  if constexpr (NW == 1) {
+    __syncthreads();
+  } else {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+
+// Index of the calling wave within the workgroup (wave-uniform: index
+// arithmetic on it stays scalar).
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane((int)threadIdx.x / 64); }
+
+// Sum of one value per wave, identical in every wave (LDS scratch S.red).
+template <int NW>
+__device__ __forceinline__ double waves_sum(Smem& S, double v) {
+  if constexpr (NW == 1) {
+    return v;
+  } else {
+    if (threadIdx.x % 64 == 0) S.red[threadIdx.x / 64] = v;
+    __syncthreads();
+    const double t = S.red[0] + S.red[1];
+    __syncthreads();
+    return t;
+  }
+}
+
+// K step j (0, 1, 2, ...) of wave w's share of the rows: 16-row steps 4g..4g+3
+// of the wave's 64-row groups g = w, w + NW, ...
+template <int NW>
+__device__ __forceinline__ int own_kstep(int j, int w) { return 4 * (w + NW * (j >> 2)) + (j & 3); }
+// 32-row block j of wave w's share: blocks 2g, 2g + 1 of its groups
+template <int NW>
+__device__ __forceinline__ int own_block(int j, int w) { return 2 * (w + NW * (j >> 1)) + (j & 1); }
+// number of 64-row groups of wave w
+template <int NW>
+__device__ __forceinline__ int own_groups(int npad, int w) { return (npad / 64 - w + NW - 1) / NW; }
 
 // ---------------------------------------------------------------------------
 // Gram pass: S.G = X^T diag(lambda) X + I / prior_var for the first D
 // features, identity on the padding, from sqrt(lambda) in `sl`.
 // ---------------------------------------------------------------------------
-template <int NB>
+template <int NB, int NW>
 __device__ __forceinline__ void gram_pass(Smem& S, const float* __restrict__ Xg, const float* __restrict__ sl, int npad, int D,
                           float inv_prior) {
   constexpr int NTL = NB * (NB + 1) / 2;
-  const int l = threadIdx.x;
+  const int l = threadIdx.x % 64, w = wave_id();
   f32x16 acc[NTL];
 #pragma unroll
   for (int t = 0; t < NTL; ++t) acc[t] = f32x16{0};
   const float* slp = sl + (l / 32) * 8;
   const float* xp = Xg + (size_t)l * 8;
-  const int nkb = npad / 16;  // even: npad is a multiple of 32
+  const int nj = 4 * own_groups<NW>(npad, w);  // this wave's K steps (none for a wave without rows)
   // fragments of K step kb: sqrt(lambda) of the lane's 8 rows and X of its NB feature blocks
   struct Frag {
     float s[8];
     float x[NB][8];
   };
-  auto fetch = [&](Frag& f, int kb) {
-    kb = kb < nkb ? kb : nkb - 1;  // the pipeline's tail re-reads the last step (in bounds, unused)
+  auto fetch = [&](Frag& f, int j) {
+    // the pipeline's tail re-reads the wave's last step (in bounds, unused)
+    const int jj = j < nj ? j : nj - 1;
+    const int kb = jj >= 0 ? own_kstep<NW>(jj, w) : 0;
     load8(slp + kb * 16, f.s);
 #pragma unroll
     for (int b = 0; b < NB; ++b) load8(xp + (size_t)(kb * NB + b) * 512, f.x[b]);
@@ -120,35 +173,54 @@ __device__ __forceinline__ void gram_pass(Smem& S, const float* __restrict__ Xg,
   fetch(fa, 0);
   fetch(fb, 1);
   split_frag(fa, ha, la);
-  for (int kb = 0; kb < nkb; kb += 2) {
-    fetch(fa, kb + 2);
+  for (int j = 0; j < nj; j += 2) {
+    fetch(fa, j + 2);
     tiles(ha, la);
     split_frag(fb, hb, lb);
     interleave_mfma_valu<3 * NTL, 64 / (3 * NTL) + 1>();
-    fetch(fb, kb + 3);
+    fetch(fb, j + 3);
     tiles(hb, lb);
     split_frag(fa, ha, la);
     interleave_mfma_valu<3 * NTL, 64 / (3 * NTL) + 1>();
   }
   __syncthreads();
-  int t = 0;
+  if constexpr (NW == 2) {
+    // wave 1 hands its partial tiles to wave 0 through S.Gi (unused during the pass)
+    float* P = S.Gi;
+    if (w == 1) {
 #pragma unroll
-  for (int bi = 0; bi < NB; ++bi)
+      for (int t = 0; t < NTL; ++t)
 #pragma unroll
-    for (int bj = 0; bj <= bi; ++bj) {
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int i = bi * 32 + (r / 4) * 8 + (l / 32) * 4 + (r % 4);
-        const int j = bj * 32 + (l % 32);
-        // a diagonal tile holds (i, j) and (j, i) rounded differently: keep the lower one
-        if (bi == bj && i < j) continue;
-        float v = acc[t][r];
-        if (i == j) v = i < D ? v + inv_prior : 1.0f;
-        S.G[i * MAXD + j] = v;
-        S.G[j * MAXD + i] = v;
-      }
-      ++t;
+        for (int r = 0; r < 16; ++r) P[(t * 16 + r) * 64 + l] = acc[t][r];
     }
+    __syncthreads();
+    if (w == 0) {
+#pragma unroll
+      for (int t = 0; t < NTL; ++t)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[t][r] += P[(t * 16 + r) * 64 + l];
+    }
+  }
+  if (w == 0) {
+    int t = 0;
+#pragma unroll
+    for (int bi = 0; bi < NB; ++bi)
+#pragma unroll
+      for (int bj = 0; bj <= bi; ++bj) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int i = bi * 32 + (r / 4) * 8 + (l / 32) * 4 + (r % 4);
+          const int j = bj * 32 + (l % 32);
+          // a diagonal tile holds (i, j) and (j, i) rounded differently: keep the lower one
+          if (bi == bj && i < j) continue;
+          float v = acc[t][r];
+          if (i == j) v = i < D ? v + inv_prior : 1.0f;
+          S.G[i * MAXD + j] = v;
+          S.G[j * MAXD + i] = v;
+        }
+        ++t;
+      }
+  }
   __syncthreads();
 }
 
@@ -184,7 +256,7 @@ __device__ __forceinline__ void col_set(f32x32 (&A)[NB], int k, float v) {
   }
 }
 
-template <int NB>
+template <int NB, int NW>
 __device__ __forceinline__ void sweep(Smem& S, f32x32 (&A)[NB], int D, bool want_chol, float& p0j) {
   constexpr int DP = 32 * NB;
   const int jl = threadIdx.x % DP;
@@ -196,12 +268,12 @@ __device__ __forceinline__ void sweep(Smem& S, f32x32 (&A)[NB], int D, bool want
   // Pivot k needs column k in every lane: entry i is lane i's register k
   // (A[k][i] = A[i][k]), broadcast through LDS.  Padding pivots (k >= D) are
   // 1 with zero off-diagonals: nothing to do.
-  float(*bcast)[64] = reinterpret_cast<float(*)[64]>(&S.bc[0][0][0]);
+  float(*bcast)[64] = S.bc;
   for (int k = 0; k < D; ++k) {
     const float ak = col_get<NB>(A, k);  // A[k][jl] = A[jl][k]
     float* bc = bcast[k & 1];
     bc[jl] = ak;
-    __syncthreads();
+    wave0_sync<NW>();
     float col[DP];
 #pragma unroll
     for (int i = 0; i < DP; i += 4) {
@@ -223,18 +295,18 @@ __device__ __forceinline__ void sweep(Smem& S, f32x32 (&A)[NB], int D, bool want
   // -G^-1 on the padding block is -I
   for (int k = D; k < DP; ++k) col_set<NB>(A, k, jl == k ? -1.0f : 0.0f);
   if (threadIdx.x < DP && threadIdx.x >= D) S.dk[threadIdx.x] = 1.0f;
-  __syncthreads();
+  wave0_sync<NW>();
 }
 
 // log det G = sum log d_k in float64 (padding pivots are 1)
 __device__ __forceinline__ double logdet_from_pivots(const Smem& S, int D) {
-  const int l = threadIdx.x;
+  const int l = threadIdx.x % 64;
   double v = l < D ? log((double)S.dk[l]) : 0.0;
   return group_sum<64>(v);
 }
 
 // S.u = G^-1 S.p with one refinement step against the Gram still in S.G.
-template <int NB>
+template <int NB, int NW>
 __device__ __forceinline__ void ginv_apply(Smem& S, const f32x32 (&A)[NB]) {
   constexpr int DP = 32 * NB;
   const int l = threadIdx.x, jl = l % DP;
@@ -244,24 +316,24 @@ __device__ __forceinline__ void ginv_apply(Smem& S, const f32x32 (&A)[NB]) {
 #pragma unroll
     for (int i = 0; i < 32; ++i) uj = fmaf(-A[b][i], S.p[32 * b + i], uj);
   if (l < DP) S.u[jl] = uj;
-  __syncthreads();
+  wave0_sync<NW>();
   float gu = 0.0f;
 #pragma unroll
   for (int i = 0; i < DP; ++i) gu = fmaf(S.G[i * MAXD + jl], S.u[i], gu);
   if (l < DP) S.dH[jl] = S.p[jl] - gu;  // residual (dH is free here)
-  __syncthreads();
+  wave0_sync<NW>();
   float du = 0.0f;
 #pragma unroll
   for (int b = 0; b < NB; ++b)
 #pragma unroll
     for (int i = 0; i < 32; ++i) du = fmaf(-A[b][i], S.dH[32 * b + i], du);
-  __syncthreads();
+  wave0_sync<NW>();
   if (l < DP) S.u[jl] = uj + du;
-  __syncthreads();
+  wave0_sync<NW>();
 }
 
 // S.Gi = G^-1
-template <int NB>
+template <int NB, int NW>
 __device__ __forceinline__ void store_ginv(Smem& S, const f32x32 (&A)[NB]) {
   const int l = threadIdx.x;
   if (l < 32 * NB) {
@@ -270,187 +342,48 @@ __device__ __forceinline__ void store_ginv(Smem& S, const f32x32 (&A)[NB]) {
 #pragma unroll
       for (int i = 0; i < 32; ++i) S.Gi[(32 * b + i) * LDG + l] = -A[b][i];
   }
-  __syncthreads();
+  wave0_sync<NW>();
 }
 
 // G (in S.G) -> pivots in S.dk, S.u = G^-1 S.p (refined), and with
 // keep_inverse S.Gi = G^-1.  With want_chol the momentum p0 = chol(G) S.z is
 // drawn first into S.p0 and S.p.
-template <int NB>
+template <int NB, int NW>
 __device__ __forceinline__ void invert(Smem& S, int D, bool want_chol, bool keep_inverse) {
   f32x32 A[NB];
   float p0j;
-  sweep<NB>(S, A, D, want_chol, p0j);
+  sweep<NB, NW>(S, A, D, want_chol, p0j);
   if (want_chol) {
     const int l = threadIdx.x;
     const float pv = l < D ? p0j : 0.0f;
     S.p0[l] = pv;
     S.p[l] = pv;
-    __syncthreads();
+    wave0_sync<NW>();
   }
-  ginv_apply<NB>(S, A);
-  if (keep_inverse) store_ginv<NB>(S, A);
-}
-
-// ---------------------------------------------------------------------------
-// Blocked inverse for the vector-field evaluations.  With G = [A B; B^T C]
-// (32 x 32 blocks): A^-1 by a 32 x 32 sweep, W = A^-1 B, the Schur complement
-// S = C - B^T W, S^-1 by a second sweep, Z = W S^-1 and
-//   G^-1 = [A^-1 + Z W^T, -Z; -Z^T, S^-1],   log det G = log det A + log det S.
-// The four 32 x 32 x 32 products run on v_mfma_f32_32x32x2f32 (float32
-// operands and accumulation, as the sweep).  The 32 x 32 sweep keeps half a
-// column per lane (16 FMAs per pivot).
-// ---------------------------------------------------------------------------
-
-// -X^-1 of the 32 x 32 symmetric block at src (row stride ld), written as X^-1
-// to dst (row stride dld; dst may equal src); the first nk pivots are real,
-// the rest are identity padding; pivots to S.dk[dk0 + k].  Lane l holds rows
-// [16 (l/32), +16) of column l % 32.
-__device__ __forceinline__ void sweep32(Smem& S, const float* src, int ld, float* dst, int dld, int nk, int dk0) {
-  const int l = threadIdx.x, j = l % 32, hf = l / 32;
-  f32x16 A;
-#pragma unroll
-  for (int r = 0; r < 16; ++r) A[r] = src[(16 * hf + r) * ld + j];
-  __syncthreads();  // src may alias dst
-  for (int k = 0; k < nk; ++k) {
-    const int ho = k >> 4, kr = k & 15;
-    // column k: entry j is A[k][j] = A[j][k], register kr of the half that holds row k
-    const float mine = A[kr];
-    S.bc[k & 1][hf][j] = mine;
-    __syncthreads();
-    const float* bc = S.bc[k & 1][ho];
-    float col[16];
-#pragma unroll
-    for (int i = 0; i < 16; i += 4) {
-      const float4 v = *reinterpret_cast<const float4*>(bc + 16 * hf + i);
-      col[i] = v.x; col[i + 1] = v.y; col[i + 2] = v.z; col[i + 3] = v.w;
-    }
-    const float ak = bc[j];
-    const float d = bc[k];
-    const float rd = __builtin_amdgcn_rcpf(d);  // 1 ulp, as the float32 sweep itself
-    // column k's own lanes hold col: f = 1 - 1/d turns their update into A[i][k] / d
-    const float f = (j == k) ? 1.0f - rd : ak * rd;
-#pragma unroll
-    for (int r = 0; r < 16; ++r) A[r] = fmaf(-col[r], f, A[r]);
-    const float keep = A[kr];
-    A[kr] = (hf == ho) ? ((j == k) ? -rd : ak * rd) : keep;
-    if (l == 0) S.dk[dk0 + k] = d;
-  }
-  for (int k = nk; k < 32; ++k) {  // padding pivots: -1 on the diagonal
-    const float keep = A[k & 15];
-    A[k & 15] = (hf == (k >> 4) && j == k) ? -1.0f : keep;
-  }
-  if (l >= nk && l < 32) S.dk[dk0 + l] = 1.0f;
-  __syncthreads();
-#pragma unroll
-  for (int r = 0; r < 16; ++r) dst[(16 * hf + r) * dld + j] = -A[r];
-  __syncthreads();
-}
-
-// acc + X Y for 32 x 32 operands read through xa(m, k) = X[m][k], yb(k, n) = Y[k][n];
-// result slot r of lane l is C[(r/4)*8 + (l/32)*4 + r%4][l%32]
-template <class FX, class FY>
-__device__ __forceinline__ f32x16 gemm32(FX xa, FY yb, f32x16 acc) {
-  const int l = threadIdx.x, m = l % 32, kh = l / 32;
-#pragma unroll
-  for (int ks = 0; ks < 16; ++ks) {
-    const int k = 2 * ks + kh;
-    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(xa(m, k), yb(k, m), acc, 0, 0, 0);
-  }
-  return acc;
-}
-__device__ __forceinline__ int crow(int r) { return (r / 4) * 8 + ((int)threadIdx.x / 32) * 4 + (r % 4); }
-
-// S.u = G^-1 S.p with G^-1 in S.Gi, one refinement step against the Gram in S.G.
-template <int NB>
-__device__ __forceinline__ void solve_refined(Smem& S) {
-  constexpr int DP = 32 * NB;
-  const int l = threadIdx.x, jl = l % DP;
-  float uj = 0.0f;
-#pragma unroll
-  for (int i = 0; i < DP; ++i) uj = fmaf(S.Gi[i * LDG + jl], S.p[i], uj);
-  if (l < DP) S.u[jl] = uj;
-  __syncthreads();
-  float gu = 0.0f;
-#pragma unroll
-  for (int i = 0; i < DP; ++i) gu = fmaf(S.G[i * MAXD + jl], S.u[i], gu);
-  if (l < DP) S.dH[jl] = S.p[jl] - gu;  // residual (dH is free here)
-  __syncthreads();
-  float du = 0.0f;
-#pragma unroll
-  for (int i = 0; i < DP; ++i) du = fmaf(S.Gi[i * LDG + jl], S.dH[i], du);
-  __syncthreads();
-  if (l < DP) S.u[jl] = uj + du;
-  __syncthreads();
-}
-
-// G (in S.G) -> S.Gi = G^-1, pivots in S.dk, S.u = G^-1 S.p (refined).
-template <int NB>
-__device__ __forceinline__ void invert_blocked(Smem& S, int D) {
-  float* Gi = S.Gi;
-  const float* G = S.G;
-  if constexpr (NB == 1) {
-    sweep32(S, G, MAXD, Gi, LDG, D, 0);
-  } else {
-    const int l = threadIdx.x, c = l % 32;
-    sweep32(S, G, MAXD, Gi, LDG, 32, 0);  // Gi_TL = A^-1
-    // W = A^-1 B (A^-1 symmetric: read it transposed, conflict-free)
-    f32x16 acc = gemm32([&](int m, int k) { return Gi[k * LDG + m]; },
-                        [&](int k, int n) { return G[k * MAXD + 32 + n]; }, f32x16{0});
-#pragma unroll
-    for (int r = 0; r < 16; ++r) S.W[crow(r) * LDW + c] = acc[r];
-    __syncthreads();
-    // S = C - B^T W  into Gi_BR
-    acc = gemm32([&](int m, int k) { return G[k * MAXD + 32 + m]; },
-                 [&](int k, int n) { return S.W[k * LDW + n]; }, f32x16{0});
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const int i = crow(r);
-      Gi[(32 + i) * LDG + 32 + c] = G[(32 + i) * MAXD + 32 + c] - acc[r];
-    }
-    __syncthreads();
-    sweep32(S, Gi + 32 * LDG + 32, LDG, Gi + 32 * LDG + 32, LDG, D - 32, 32);  // Gi_BR = S^-1
-    // Z = W S^-1 -> Gi_TR = -Z, Gi_BL = -Z^T
-    acc = gemm32([&](int m, int k) { return S.W[m * LDW + k]; },
-                 [&](int k, int n) { return Gi[(32 + k) * LDG + 32 + n]; }, f32x16{0});
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const int i = crow(r);
-      Gi[i * LDG + 32 + c] = -acc[r];
-      Gi[(32 + c) * LDG + i] = -acc[r];
-    }
-    __syncthreads();
-    // Gi_TL = A^-1 + Z W^T   (Z[m][k] = -Gi_BL[k][m], W^T[k][n] = W[n][k])
-    acc = gemm32([&](int m, int k) { return -Gi[(32 + k) * LDG + m]; },
-                 [&](int k, int n) { return S.W[n * LDW + k]; }, f32x16{0});
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const int i = crow(r);
-      Gi[i * LDG + c] += acc[r];
-    }
-    __syncthreads();
-  }
-  solve_refined<NB>(S);
+  ginv_apply<NB, NW>(S, A);
+  if (keep_inverse) store_ginv<NB, NW>(S, A);
 }
 
 // Sum the per-lane feature partials of a row pass over the 32 lanes of each
 // half wave: lane f < DP receives the total for feature f.
-template <int NB>
+template <int NB, int NW>
 __device__ __forceinline__ float reduce_features(Smem& S, const float (&acc)[2 * NB][8]) {
   constexpr int KC = 2 * NB, RS = KC * 8 + 1;
-  const int l = threadIdx.x;
+  static_assert(NW * 64 * RS <= MAXD * LDG, "the partials must fit S.Gi");
+  const int t = threadIdx.x, l = t % 64;
   float* R = S.Gi;
   __syncthreads();
 #pragma unroll
   for (int kc = 0; kc < KC; ++kc)
 #pragma unroll
-    for (int i = 0; i < 8; ++i) R[l * RS + kc * 8 + i] = acc[kc][i];
+    for (int i = 0; i < 8; ++i) R[t * RS + kc * 8 + i] = acc[kc][i];
   __syncthreads();
   float s = 0.0f;
-  if (l < 32 * NB) {
+  if (l < 32 * NB) {  // every wave forms the same totals in the same order
     const int kc = l / 16, h = (l / 8) % 2, i = l % 8;
-    for (int j = 0; j < 32; ++j) s += R[(h * 32 + j) * RS + kc * 8 + i];
+#pragma unroll
+    for (int ww = 0; ww < NW; ++ww)
+      for (int j = 0; j < 32; ++j) s += R[(ww * 64 + h * 32 + j) * RS + kc * 8 + i];
   }
   __syncthreads();
   return s;
@@ -463,7 +396,7 @@ __device__ __forceinline__ float reduce_features(Smem& S, const float (&acc)[2 *
 // the explicit half step, which also sets E0), SL <- sqrt(lambda) there.
 // ---------------------------------------------------------------------------
 template <int NB>
-__device__ __forceinline__ void field_pass(Smem& S, const float* __restrict__ Xr, const float* __restrict__ Xs,
+__device__ __forceinline__ void field_pass_w1(Smem& S, const float* __restrict__ Xr, const float* __restrict__ Xs,
                                            const float* __restrict__ yf, float* E0,
                            float* E1, float* SL, int npad, int D, float hs, float inv_prior, bool explicit_step) {
   constexpr int KC = 2 * NB;
@@ -601,19 +534,160 @@ __device__ __forceinline__ void field_pass(Smem& S, const float* __restrict__ Xr
     fetch_blk(b1, rb + 3);
     fetch_xf(f0, rb + 4);
   }
-  const float tot = reduce_features<NB>(S, dacc);
+  const float tot = reduce_features<NB, 1>(S, dacc);
   if (l < MAXD) S.dH[l] = l < D ? tot + S.q[l] * inv_prior : 0.0f;
   __syncthreads();
+}
+
+// Two-wave form of the same pass.  The A operand (rows of G^-1 permuted by
+// phi, split) is staged once in LDS over S.G, which is not read again before
+// the next Gram pass rewrites it, and each wave runs its own blocks one at a
+// time: the other wave resident on the SIMD covers the MFMA latency, and the
+// registers of a second block in flight are not needed.
+template <int NB>
+__device__ __forceinline__ void field_pass_w2(Smem& S, const float* __restrict__ Xr, const float* __restrict__ Xs,
+                                              const float* __restrict__ yf, float* E0, float* E1, float* SL, int npad,
+                                              int D, float hs, float inv_prior, bool explicit_step) {
+  constexpr int KC = 2 * NB, NTK = NB * KC;
+  const int t = threadIdx.x, l = t % 64, w = wave_id(), h = l / 32;
+  bf16x8* const AH = reinterpret_cast<bf16x8*>(S.G);  // [NTK][64 lanes]
+  bf16x8* const AL = AH + NTK * 64;
+  static_assert(2 * NTK * 64 * sizeof(bf16x8) <= sizeof(S.G), "the split G^-1 operand must fit S.G");
+  for (int tk = w; tk < NTK; tk += 2) {
+    const int tt = tk / KC, kc = tk % KC;
+    float v[8];
+    load8(S.Gi + phi(tt * 32 + (l % 32)) * LDG + kc * 16 + h * 8, v);
+    bf16x8 hi, lo;
+    split8(v, hi, lo);
+    AH[tk * 64 + l] = hi;
+    AL[tk * 64 + l] = lo;
+  }
+  __syncthreads();
+  float dacc[KC][8];
+#pragma unroll
+  for (int kc = 0; kc < KC; ++kc)
+#pragma unroll
+    for (int i = 0; i < 8; ++i) dacc[kc][i] = 0.0f;
+  const uint32_t xoff = (uint32_t)l * 8u;
+  const uint4* const sbase = reinterpret_cast<const uint4*>(Xs);
+  const uint32_t soff = (uint32_t)l * 2u;
+  const int nj = 2 * own_groups<2>(npad, w);  // this wave's 32-row blocks
+  // block of step j; past the end the wave's last block again (read only, unused)
+  auto blk_of = [&](int j) {
+    const int jj = j < nj ? j : nj - 1;
+    return __builtin_amdgcn_readfirstlane(jj >= 0 ? own_block<2>(jj, w) : 0);
+  };
+  struct XF {
+    bf16x8 xh[KC], xl[KC];
+  };
+  struct Blk {
+    float x[KC][8];
+    float eta, base, yv, b;
+    f32x16 acc[NB];
+  };
+  auto fetch_xf = [&](XF& f, int rb) {
+    const uint4* const blk = sbase + (size_t)rb * (KC * 128);
+#pragma unroll
+    for (int kc = 0; kc < KC; ++kc) {
+      const uint32_t i = (uint32_t)kc * 128u + soff;
+      f.xh[kc] = __builtin_bit_cast(bf16x8, blk[i]);
+      f.xl[kc] = __builtin_bit_cast(bf16x8, blk[i + 1u]);
+    }
+  };
+  auto fetch_blk = [&](Blk& B, int rb) {
+    const float* const blk = Xr + (size_t)rb * (KC * 512);
+#pragma unroll
+    for (int kc = 0; kc < KC; ++kc) load8(blk + ((uint32_t)kc * 512u + xoff), B.x[kc]);
+    const uint32_t lr = (uint32_t)(l % 32);
+    B.eta = (E1 + (size_t)rb * 32)[lr];
+    B.base = explicit_step ? 0.0f : (E0 + (size_t)rb * 32)[lr];
+    B.yv = (yf + (size_t)rb * 32)[lr];
+  };
+  auto mfma_blk = [&](const XF& f, Blk& B) {
+#pragma unroll
+    for (int tt = 0; tt < NB; ++tt) {
+      B.acc[tt] = f32x16{0};
+#pragma unroll
+      for (int kc = 0; kc < KC; ++kc) {
+        const int tk = tt * KC + kc;
+        B.acc[tt] = mfma3(AH[tk * 64 + l], AL[tk * 64 + l], f.xh[kc], f.xl[kc], B.acc[tt]);
+      }
+    }
+  };
+  auto bdot = [&](Blk& B) {
+    float bp = 0.0f;
+#pragma unroll
+    for (int kc = 0; kc < KC; ++kc) {
+      float u8[8];
+      load8(S.u + kc * 16 + h * 8, u8);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) bp = fmaf(B.x[kc][i], u8[i], bp);
+    }
+    B.b = bp + __shfl_xor(bp, 32, 64);
+  };
+  float* const e0_dst = explicit_step ? E0 : E1;
+  auto post = [&](const Blk& B, int rb) {
+    const int row = rb * 32 + (l % 32);
+    float ap = 0.0f;
+#pragma unroll
+    for (int tt = 0; tt < NB; ++tt)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int j = 16 * tt + r;
+        ap = fmaf(B.acc[tt][r], B.x[j / 8][j % 8], ap);
+      }
+    const float an = ap + __shfl_xor(ap, 32, 64);
+    const float s = sigmoidf(B.eta);
+    const float wv = s * (1.0f - s) * (1.0f - 2.0f * s);
+    const float hv = 0.5f * wv * (an - B.b * B.b) - (B.yv - s);
+#pragma unroll
+    for (int kc = 0; kc < KC; ++kc)
+#pragma unroll
+      for (int i = 0; i < 8; ++i) dacc[kc][i] = fmaf(hv, B.x[kc][i], dacc[kc][i]);
+    const float en = fmaf(hs, B.b, explicit_step ? B.eta : B.base);
+    const float sn = sigmoidf(en);
+    E1[row] = en;
+    e0_dst[row] = en;
+    SL[row] = sqrt_fast(sn * (1.0f - sn));
+  };
+  XF f;
+  Blk B;
+  fetch_xf(f, blk_of(0));
+  fetch_blk(B, blk_of(0));
+  for (int j = 0; j < nj; ++j) {
+    // the A operand is read from LDS per block: holding it in registers
+    // across the loop would cost 64 of them
+    asm volatile("" ::: "memory");
+    const int rb = blk_of(j);
+    mfma_blk(f, B);
+    fetch_xf(f, blk_of(j + 1));
+    bdot(B);
+    post(B, rb);
+    fetch_blk(B, blk_of(j + 1));
+  }
+  const float tot = reduce_features<NB, 2>(S, dacc);
+  if (t < MAXD) S.dH[l] = l < D ? tot + S.q[l] * inv_prior : 0.0f;
+  __syncthreads();
+}
+
+template <int NB, int NW>
+__device__ __forceinline__ void field_pass(Smem& S, const float* __restrict__ Xr, const float* __restrict__ Xs,
+                                           const float* __restrict__ yf, float* E0, float* E1, float* SL, int npad,
+                                           int D, float hs, float inv_prior, bool explicit_step) {
+  if constexpr (NW == 1)
+    field_pass_w1<NB>(S, Xr, Xs, yf, E0, E1, SL, npad, D, hs, inv_prior, explicit_step);
+  else
+    field_pass_w2<NB>(S, Xr, Xs, yf, E0, E1, SL, npad, D, hs, inv_prior, explicit_step);
 }
 
 // eta = X q (float64 accumulation) into Eout, sqrt(lambda) into SL; with
 // want_grad also the log likelihood (float64, returned) and
 // S.g = X^T (y - s) - q / prior_var.
-template <int NB>
+template <int NB, int NW>
 __device__ __forceinline__ double fresh_pass(Smem& S, const float* __restrict__ Xr, const float* __restrict__ yf, int N, int npad,
                              int D, float* Eout, float* SL, bool want_grad, float inv_prior) {
   constexpr int KC = 2 * NB;
-  const int l = threadIdx.x, h = l / 32;
+  const int t = threadIdx.x, l = t % 64, w = wave_id(), h = l / 32;
   double qv[KC][8];
   float gacc[KC][8];
 #pragma unroll
@@ -625,8 +699,9 @@ __device__ __forceinline__ double fresh_pass(Smem& S, const float* __restrict__ 
     }
   double ll = 0.0;
   const float* xp = Xr + (size_t)l * 8;
-  const int nrb = npad / 32;
-  for (int rb = 0; rb < nrb; ++rb) {
+  const int nj = 2 * own_groups<NW>(npad, w);  // this wave's 32-row blocks
+  for (int j = 0; j < nj; ++j) {
+    const int rb = own_block<NW>(j, w);
     float x[KC][8];
 #pragma unroll
     for (int kc = 0; kc < KC; ++kc) load8(xp + (size_t)(rb * KC + kc) * 512, x[kc]);
@@ -654,16 +729,20 @@ __device__ __forceinline__ double fresh_pass(Smem& S, const float* __restrict__ 
     }
   }
   if (want_grad) {
-    const float tot = reduce_features<NB>(S, gacc);
-    if (l < MAXD) S.g[l] = l < D ? tot - S.q[l] * inv_prior : 0.0f;
+    const float tot = reduce_features<NB, NW>(S, gacc);
+    if (t < MAXD) S.g[l] = l < D ? tot - S.q[l] * inv_prior : 0.0f;
   }
   __syncthreads();
-  return group_sum<64>(ll);
+  return waves_sum<NW>(S, group_sum<64>(ll));
 }
 
 // E0 = E1 = ES, SL = sqrt(lambda(ES))
+template <int NW>
 __device__ __forceinline__ void prep_pass(const float* ES, float* E0, float* E1, float* SL, int npad) {
-  for (int row = threadIdx.x; row < npad; row += 64) {
+  const int l = threadIdx.x % 64, w = wave_id();
+  const int ng = own_groups<NW>(npad, w);
+  for (int g = 0; g < ng; ++g) {
+    const int row = 64 * (w + NW * g) + l;
     const float e = ES[row];
     const float s = sigmoidf(e);
     E0[row] = e;
@@ -678,9 +757,9 @@ enum Stage { kInit, kNoise, kTraj, kEnd };
 // All transitions of one chain.  The trajectory is a flat loop of vector-field
 // evaluations with a single site for each pass (Gram, sweep, field, fresh),
 // which keeps one copy of each unrolled pass in the kernel.
-template <int NB, bool STEP>
+template <int NB, int NW, bool STEP>
 __device__ __forceinline__ void run_chain(const LogisticArgs& a, Smem& S, int c, int post) {
-  const int l = threadIdx.x;
+  const int l = threadIdx.x % 64, w = wave_id();  // lane, wave
   const PackedPost pp = a.packed_hdr[post];
   const int N = a.post_N[post], D = a.post_D[post];
   const float* Xg = a.packed + pp.xg;
@@ -706,8 +785,10 @@ __device__ __forceinline__ void run_chain(const LogisticArgs& a, Smem& S, int c,
 
   {
     const float qv = l < D ? static_cast<const float*>(a.q_in)[(size_t)c * Dp + l] : 0.0f;
-    S.q0[l] = qv;
-    S.q[l] = qv;
+    if (w == 0) {
+      S.q0[l] = qv;
+      S.q[l] = qv;
+    }
   }
   double logp = a.logp_in[c];
   int n_acc = 0;
@@ -723,10 +804,11 @@ __device__ __forceinline__ void run_chain(const LogisticArgs& a, Smem& S, int c,
     if (stage == kInit || stage == kEnd) {
       const bool end = stage == kEnd;
       float* Eout = (end ? 1 - cur : cur) ? ESb : ESa;
-      loglik = fresh_pass<NB>(S, Xr, yf, N, npad, D, Eout, SL, end || !STEP, inv_prior);
+      loglik = fresh_pass<NB, NW>(S, Xr, yf, N, npad, D, Eout, SL, end || !STEP, inv_prior);
       if (!end) {
         // the state's gradient goes straight to final_g (rewritten on every acceptance)
-        if (!STEP && a.final_g != nullptr && l < D) static_cast<float*>(a.final_g)[(size_t)c * Dp + l] = S.g[l];
+        if (!STEP && a.final_g != nullptr && w == 0 && l < D)
+          static_cast<float*>(a.final_g)[(size_t)c * Dp + l] = S.g[l];
         stage = kNoise;
       }
     }
@@ -734,7 +816,8 @@ __device__ __forceinline__ void run_chain(const LogisticArgs& a, Smem& S, int c,
     if (stage == kNoise) {
       const uint64_t tt = a.counter + (uint64_t)tr;
       if (a.z != nullptr) {
-        S.z[l] = l < D ? static_cast<const float*>(a.z)[((size_t)tr * a.C + c) * Dp + l] : 0.0f;
+        const float zv = l < D ? static_cast<const float*>(a.z)[((size_t)tr * a.C + c) * Dp + l] : 0.0f;
+        if (w == 0) S.z[l] = zv;
         uacc = a.u[(size_t)tr * a.C + c];
       } else {
         if (l < MAXD / 2) {
@@ -742,27 +825,29 @@ __device__ __forceinline__ void run_chain(const LogisticArgs& a, Smem& S, int c,
                                         (uint32_t)(tt >> 32)}, (uint32_t)a.seed, (uint32_t)(a.seed >> 32));
           double n0, n1;
           normal2(r, n0, n1);
-          S.z[2 * l] = 2 * l < D ? (float)n0 : 0.0f;
-          S.z[2 * l + 1] = 2 * l + 1 < D ? (float)n1 : 0.0f;
+          if (w == 0) {
+            S.z[2 * l] = 2 * l < D ? (float)n0 : 0.0f;
+            S.z[2 * l + 1] = 2 * l + 1 < D ? (float)n1 : 0.0f;
+          }
         }
         u32x4 r = philox4x32_10(u32x4{(uint32_t)c, (uint32_t)tt, kStreamAccept, (uint32_t)(tt >> 32)},
                                 (uint32_t)a.seed, (uint32_t)(a.seed >> 32));
         uacc = u01_f64(r.x, r.y) - 0.5 * (1.0 / 9007199254740992.0);
       }
-      S.q[l] = S.q0[l];
+      if (w == 0) S.q[l] = S.q0[l];
       __syncthreads();
-      prep_pass(cur ? ESb : ESa, E0, E1, SL, npad);
+      prep_pass<NW>(cur ? ESb : ESa, E0, E1, SL, npad);
     }
     // ---- metric at S.q: G, its sweep, u = G^-1 p (momentum draw first at kNoise) ----
     if (stage != kTraj || !have_ginv) {
-      gram_pass<NB>(S, Xg, SL, npad, D, inv_prior);
-      if (stage == kNoise)
-        invert<NB>(S, D, true, true);  // full sweep: the momentum draw needs chol(G)
-      else
-        invert_blocked<NB>(S, D);
+      gram_pass<NB, NW>(S, Xg, SL, npad, D, inv_prior);
+      // the inverse runs on wave 0; at the momentum draw its sweep also gives chol(G)
+      if (w == 0) invert<NB, NW>(S, D, stage == kNoise, true);
+      if constexpr (NW > 1) __syncthreads();
     }
     if (stage == kNoise) {
-      if (STEP && a.momentum != nullptr && l < D) static_cast<float*>(a.momentum)[(size_t)c * Dp + l] = S.p0[l];
+      if (STEP && a.momentum != nullptr && w == 0 && l < D)
+        static_cast<float*>(a.momentum)[(size_t)c * Dp + l] = S.p0[l];
       const double logdet = logdet_from_pivots(S, D);
       const double pu = group_sum<64>(l < D ? (double)S.p[l] * (double)S.u[l] : 0.0);
       H0 = -logp + 0.5 * pu + 0.5 * logdet + ke_const;
@@ -784,7 +869,7 @@ __device__ __forceinline__ void run_chain(const LogisticArgs& a, Smem& S, int c,
       if (isnan(dHe)) dHe = -INFINITY;
       const double p_acc = fmin(exp(dHe), 1.0);
       const bool accept = uacc < p_acc;
-      if (STEP) {
+      if (STEP && w == 0) {
         if (l == 0) {
           if (a.acc_rate) a.acc_rate[c] = p_acc;
           if (a.accepted) a.accepted[c] = (uint8_t)accept;
@@ -806,15 +891,16 @@ __device__ __forceinline__ void run_chain(const LogisticArgs& a, Smem& S, int c,
             static_cast<float*>(a.g_out)[i] = accept ? S.g[l] : static_cast<const float*>(a.g_in)[i];
         }
       }
-      if (accept) {
-        S.q0[l] = S.q[l];
-        if (!STEP && a.final_g != nullptr && l < D) static_cast<float*>(a.final_g)[(size_t)c * Dp + l] = S.g[l];
+      if (accept) {  // (only wave 0 reads S.q0)
+        if (w == 0) S.q0[l] = S.q[l];
+        if (!STEP && a.final_g != nullptr && w == 0 && l < D)
+          static_cast<float*>(a.final_g)[(size_t)c * Dp + l] = S.g[l];
         logp = logp1;
         cur = 1 - cur;
         ++n_acc;
       }
       __syncthreads();
-      if (!STEP) {
+      if (!STEP && w == 0) {
         if (a.samples != nullptr && l < D)
           static_cast<float*>(a.samples)[((size_t)tr * a.C + c) * Dp + l] = S.q0[l];
         if (a.sample_acc != nullptr && l == 0) a.sample_acc[(size_t)tr * a.C + c] = (float)p_acc;
@@ -828,13 +914,17 @@ __device__ __forceinline__ void run_chain(const LogisticArgs& a, Smem& S, int c,
       q0v = S.q[l];
       p0v = S.p[l];
     }
-    field_pass<NB>(S, Xr, Xs, yf, E0, E1, SL, npad, D, hs, inv_prior, mode == 2);
+    field_pass<NB, NW>(S, Xr, Xs, yf, E0, E1, SL, npad, D, hs, inv_prior, mode == 2);
     have_ginv = false;
     ++n_fev;
     if (mode == 2) {  // explicit half step from the last iterate (integrators.py:146-148)
-      if (l < D) {
-        S.q[l] = fmaf(hs, S.u[l], S.q[l]);
-        S.p[l] = fmaf(-hs, S.dH[l], S.p[l]);
+      // every wave reads before wave 0 updates in place
+      const float qn = l < D ? fmaf(hs, S.u[l], S.q[l]) : 0.0f;
+      const float pn = l < D ? fmaf(-hs, S.dH[l], S.p[l]) : 0.0f;
+      if constexpr (NW > 1) __syncthreads();
+      if (w == 0 && l < D) {
+        S.q[l] = qn;
+        S.p[l] = pn;
       }
       __syncthreads();
       mode = 0;
@@ -842,11 +932,14 @@ __device__ __forceinline__ void run_chain(const LogisticArgs& a, Smem& S, int c,
       continue;
     }
     {
-      float dq = 0.0f;
+      float dq = 0.0f, qn = 0.0f, pn = 0.0f;
       if (l < D) {
-        const float qn = fmaf(hs, S.u[l], q0v);
-        const float pn = fmaf(-hs, S.dH[l], p0v);
+        qn = fmaf(hs, S.u[l], q0v);
+        pn = fmaf(-hs, S.dH[l], p0v);
         dq = nan_max(fabsf(qn - S.q[l]), fabsf(pn - S.p[l]));
+      }
+      if constexpr (NW > 1) __syncthreads();  // every wave has read the old iterate
+      if (w == 0 && l < D) {
         S.q[l] = qn;
         S.p[l] = pn;
       }
@@ -860,7 +953,7 @@ __device__ __forceinline__ void run_chain(const LogisticArgs& a, Smem& S, int c,
       ++n;
     }
     if (!(n < a.max_iters && isfinite(norm) && norm < a.div_tol && norm > a.conv_tol)) {
-      if (STEP && l == 0) {
+      if (STEP && w == 0 && l == 0) {
         const size_t idx = (size_t)c * a.L + s;
         if (a.fp_iters) a.fp_iters[idx] = n;
         if (a.fp_norm) a.fp_norm[idx] = norm;
@@ -870,7 +963,7 @@ __device__ __forceinline__ void run_chain(const LogisticArgs& a, Smem& S, int c,
     }
   }
 
-  if (!STEP) {
+  if (!STEP && w == 0) {
     if (l < D) {
       static_cast<float*>(a.final_q)[(size_t)c * Dp + l] = S.q0[l];
     }
@@ -880,7 +973,7 @@ __device__ __forceinline__ void run_chain(const LogisticArgs& a, Smem& S, int c,
       if (a.num_fevals) a.num_fevals[c] = n_fev;
     }
   }
-  zero_row_tails<float, STEP>(a, c, D, l, 64);
+  if (w == 0) zero_row_tails<float, STEP>(a, c, D, l, 64);
 }
 
 // One wave per chain; one instantiation per tile width NB (chains of the
@@ -888,8 +981,9 @@ __device__ __forceinline__ void run_chain(const LogisticArgs& a, Smem& S, int c,
 // robin, so block b runs chain (b % 8) * (blocks / 8) + b / 8: each XCD takes
 // a contiguous range of chains -- for a multi-posterior batch, a few
 // posteriors, whose packed X then stays in that XCD's L2.
-template <int NB, bool STEP>
-__global__ void __launch_bounds__(64) logistic_rmhmc_f32_kernel(const LogisticArgs a) {
+template <int NB, int NW, bool STEP>
+__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW)))
+logistic_rmhmc_f32_kernel(const LogisticArgs a) {
   __shared__ Smem S;
   const int per_xcd = gridDim.x / 8;
   const int slot = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
@@ -899,7 +993,7 @@ __global__ void __launch_bounds__(64) logistic_rmhmc_f32_kernel(const LogisticAr
   const int post = a.chain_post ? a.chain_post[c] : 0;
   if (post < 0 || post >= a.P) return;  // invalid posterior index: chain left untouched
   if (a.packed_hdr[post].nb != NB) return;
-  run_chain<NB, STEP>(a, S, c, post);
+  run_chain<NB, NW, STEP>(a, S, c, post);
 }
 
 // Pack posterior p (blockIdx.y) into the two fragment layouts and float labels.
@@ -1003,22 +1097,26 @@ hipError_t launch_logistic_rmhmc_f32(const LogisticArgs& a, int max_features, hi
   const bool nb2 = max_features > 32 && (max_features <= 64 || a.P > 1);
   const bool nb4 = max_features > 64;
   const bool step = a.q_out != nullptr;
+  auto launch_nb = [&](auto nb, auto nw, hipStream_t st) {
+    constexpr int NB = decltype(nb)::value, NW = decltype(nw)::value;
+    if (step)
+      hipLaunchKernelGGL((lfast::logistic_rmhmc_f32_kernel<NB, NW, true>), dim3(blocks), dim3(64 * NW), 0, st, a);
+    else
+      hipLaunchKernelGGL((lfast::logistic_rmhmc_f32_kernel<NB, NW, false>), dim3(blocks), dim3(64 * NW), 0, st, a);
+  };
+  using I1 = std::integral_constant<int, 1>;
+  using I2 = std::integral_constant<int, 2>;
   auto launch = [&](int which, hipStream_t st) -> hipError_t {
+    if (which == 2) return launch_logistic_rmhmc_f32_wide(a, st);
+    const bool two = a.waves[which] == 2;
     if (which == 0) {
-      if (step)
-        hipLaunchKernelGGL((lfast::logistic_rmhmc_f32_kernel<1, true>), dim3(blocks), dim3(64), 0, st, a);
-      else
-        hipLaunchKernelGGL((lfast::logistic_rmhmc_f32_kernel<1, false>), dim3(blocks), dim3(64), 0, st, a);
-      return hipGetLastError();
+      if (two) launch_nb(I1{}, I2{}, st);
+      else launch_nb(I1{}, I1{}, st);
+    } else {
+      if (two) launch_nb(I2{}, I2{}, st);
+      else launch_nb(I2{}, I1{}, st);
     }
-    if (which == 1) {
-      if (step)
-        hipLaunchKernelGGL((lfast::logistic_rmhmc_f32_kernel<2, true>), dim3(blocks), dim3(64), 0, st, a);
-      else
-        hipLaunchKernelGGL((lfast::logistic_rmhmc_f32_kernel<2, false>), dim3(blocks), dim3(64), 0, st, a);
-      return hipGetLastError();
-    }
-    return launch_logistic_rmhmc_f32_wide(a, st);
+    return hipGetLastError();
   };
   int widths[3], n = 0;
   if (nb4) widths[n++] = 2;  // the widest first: it holds most of a mixed batch's work

@@ -30,6 +30,7 @@ from .metrics import _check_alpha2
 from .models import NealFunnel, _dtype_code
 from .proposal import Proposal
 from .random import HostNoise, PRNGKey
+from .rmhmc import _lanes
 
 __all__ = ["LMCState", "LMCInfo", "RiemannianIntegratorState", "init", "build_kernel", "lmc", "lan_integrator",
            "sample", "DynamicLMCState", "init_dynamic", "build_dynamic_kernel", "dynamic_lmc",
@@ -161,7 +162,8 @@ def build_kernel(integrator=lan_integrator, divergence_threshold: float = 1000, 
                             volume_adjustment=J.data_ptr(), z=_lib.ptr(z), u=_lib.ptr(u),
                             num_integration_steps=_lib.ptr(steps), **{k: v.data_ptr() for k, v in out.items()})
         params = _params(C, D, _dtype_code(position), step_size, L, alpha2, logdensity_fn.sigma,
-                         divergence_threshold, reference_faithful, lpc=lanes_per_chain, seed=seed, counter=counter)
+                         divergence_threshold, reference_faithful, seed=seed, counter=counter,
+                         lpc=_lanes(lanes_per_chain, "lmcmonge_funnel", D, C, position.dtype))
         rc = _lib.load().gjx_lmcmonge_funnel_step(params, io, _lib.stream_handle(dev))
         _lib.check(rc, "gjx_lmcmonge_funnel_step")
 
@@ -261,7 +263,8 @@ def sample(rng_key, state, logdensity_fn, step_size, inverse_mass_matrix, num_in
                           samples=_lib.ptr(samples), acceptance_rate=acc.data_ptr(), num_accepted=n_acc.data_ptr(),
                           num_integration_steps=_lib.ptr(steps))
     params = _params(C, D, _dtype_code(position), step_size, L, alpha2, logdensity_fn.sigma,
-                     divergence_threshold, reference_faithful, T=T, lpc=lanes_per_chain, seed=seed, counter=counter)
+                     divergence_threshold, reference_faithful, T=T, seed=seed, counter=counter,
+                     lpc=_lanes(lanes_per_chain, "lmcmonge_funnel", D, C, position.dtype))
     rc = _lib.load().gjx_lmcmonge_funnel_sample(params, io, _lib.stream_handle(dev))
     _lib.check(rc, "gjx_lmcmonge_funnel_sample")
     # every LMC step evaluates the vector field once (one gradient + Hessian-vector products)

@@ -267,10 +267,10 @@ class _posterior_table:
 
 
 def logistic_params(model, C, L=0, T=1, step_size=0.0, max_iters=0, convergence_tol=1e-6, divergence_tol=1e10,
-                    divergence_threshold=1000.0, seed=0, counter=0, dtype=torch.float64):
+                    divergence_threshold=1000.0, seed=0, counter=0, dtype=torch.float64, waves_per_chain=0):
     code = _lib.GJX_F32 if dtype == torch.float32 else _lib.GJX_F64
     return _lib.LogisticParams(num_chains=C, dim=model.dim, dtype=code, num_integration_steps=int(L),
-                               max_iters=int(max_iters), num_transitions=int(T), flags=0, reserved=0,
+                               max_iters=int(max_iters), num_transitions=int(T), flags=0, waves_per_chain=int(waves_per_chain),
                                step_size=float(step_size), prior_var=model.prior_var,
                                convergence_tol=float(convergence_tol), divergence_tol=float(divergence_tol),
                                divergence_threshold=float(divergence_threshold), seed=int(seed),

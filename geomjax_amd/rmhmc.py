@@ -157,8 +157,41 @@ def _params(C, D, dtype_code, step_size, L, solver, alpha2, sigma, divergence_th
         divergence_threshold=float(divergence_threshold), seed=int(seed), counter=int(counter))
 
 
+def _lanes(lanes_per_chain, kind, D, C, dtype):
+    """Lanes per chain of a funnel kernel: the caller's, else the launch shape
+    the autotuner measured for (kernel, D, chains, dtype), else 0 (the
+    library's default)."""
+    if lanes_per_chain:
+        return int(lanes_per_chain)
+    from .autotune import lookup
+
+    return lookup(kind, D, C, dtype) or 0
+
+
+def _waves(waves_per_chain, model, C, dtype):
+    """The C ABI's waves_per_chain field for a logistic call: an explicit 1 or
+    2 applies to both one-workgroup widths (up to 32 and up to 64 features),
+    a pair (w32, w64) to each; 0 takes the autotuner's choice for this problem
+    and chain count (tuned on first use), or the library default.  float64
+    calls ignore it."""
+    if dtype != torch.float32:
+        return 0
+    if isinstance(waves_per_chain, tuple):
+        a, b = (int(x) for x in waves_per_chain)
+        if a not in (0, 1, 2) or b not in (0, 1, 2):
+            raise ValueError(f"waves_per_chain entries must be 0, 1 or 2 (got {waves_per_chain})")
+        return a | (b << 4)
+    if waves_per_chain:
+        if waves_per_chain not in (1, 2):
+            raise ValueError(f"waves_per_chain must be 0, 1 or 2 (got {waves_per_chain})")
+        return int(waves_per_chain) | (int(waves_per_chain) << 4)
+    from .autotune import logistic_waves_field
+
+    return logistic_waves_field(model, C)
+
+
 def build_kernel(integrator=implicit_midpoint, divergence_threshold: float = 1000, *,
-                 return_fixed_point_info=False, lanes_per_chain=0):
+                 return_fixed_point_info=False, lanes_per_chain=0, waves_per_chain=0):
     """geomjax/rmhmc/rmhmc.py:110-176.
 
     With ``return_fixed_point_info=True`` the kernel returns a third element,
@@ -232,7 +265,8 @@ def build_kernel(integrator=implicit_midpoint, divergence_threshold: float = 100
                               **{k: v.data_ptr() for k, v in out.items()})
         if kind == "funnel_monge":
             params = _params(C, D, _dtype_code(position), step_size, L, solver, metric_fn.alpha2,
-                             logdensity_fn.sigma, divergence_threshold, lanes_per_chain=lanes_per_chain, seed=seed,
+                             logdensity_fn.sigma, divergence_threshold, seed=seed,
+                             lanes_per_chain=_lanes(lanes_per_chain, "rmhmc_monge_funnel", D, C, position.dtype),
                              counter=counter)
             rc = _lib.load().gjx_rmhmc_monge_funnel_step(params, io, _lib.stream_handle(dev))
             _lib.check(rc, "gjx_rmhmc_monge_funnel_step")
@@ -241,7 +275,8 @@ def build_kernel(integrator=implicit_midpoint, divergence_threshold: float = 100
                                      convergence_tol=solver["convergence_tol"],
                                      divergence_tol=solver["divergence_tol"],
                                      divergence_threshold=divergence_threshold, seed=seed, counter=counter,
-                                     dtype=position.dtype)
+                                     dtype=position.dtype,
+                                     waves_per_chain=_waves(waves_per_chain, logdensity_fn, C, position.dtype))
             data = logdensity_fn.data(dtype=position.dtype, num_chains=C)
             rc = _lib.load().gjx_rmhmc_fisher_logistic_step(params, data, io, _lib.stream_handle(dev))
             _lib.check(rc, "gjx_rmhmc_fisher_logistic_step")
@@ -271,7 +306,7 @@ def build_kernel(integrator=implicit_midpoint, divergence_threshold: float = 100
 
 def sample(rng_key, state, logdensity_fn, step_size, metric_fn, num_integration_steps, num_samples, *,
            integrator=implicit_midpoint, divergence_threshold=1000, store_samples=True, lanes_per_chain=0,
-           samples_out=None, max_num_integration_steps=None):
+           waves_per_chain=0, samples_out=None, max_num_integration_steps=None):
     """``num_samples`` transitions of every chain in ONE persistent launch.
 
     The reference runs this loop as ``jax.lax.scan`` over ``jax.vmap(kernel)``
@@ -335,7 +370,8 @@ def sample(rng_key, state, logdensity_fn, step_size, metric_fn, num_integration_
                             num_fevals=n_fev.data_ptr(), num_integration_steps=_lib.ptr(steps))
     if kind == "funnel_monge":
         params = _params(C, D, _dtype_code(position), step_size, L, solver, metric_fn.alpha2,
-                         logdensity_fn.sigma, divergence_threshold, T=T, lanes_per_chain=lanes_per_chain, seed=seed,
+                         logdensity_fn.sigma, divergence_threshold, T=T, seed=seed,
+                         lanes_per_chain=_lanes(lanes_per_chain, "rmhmc_monge_funnel", D, C, position.dtype),
                          counter=counter)
         rc = _lib.load().gjx_rmhmc_monge_funnel_sample(params, io, _lib.stream_handle(dev))
         _lib.check(rc, "gjx_rmhmc_monge_funnel_sample")
@@ -343,7 +379,8 @@ def sample(rng_key, state, logdensity_fn, step_size, metric_fn, num_integration_
         params = logistic_params(logdensity_fn, C, L=L, T=T, step_size=step_size,
                                  max_iters=solver["max_iters"], convergence_tol=solver["convergence_tol"],
                                  divergence_tol=solver["divergence_tol"], divergence_threshold=divergence_threshold,
-                                 seed=seed, counter=counter, dtype=position.dtype)
+                                 seed=seed, counter=counter, dtype=position.dtype,
+                                 waves_per_chain=_waves(waves_per_chain, logdensity_fn, C, position.dtype))
         data = logdensity_fn.data(dtype=position.dtype, num_chains=C)
         rc = _lib.load().gjx_rmhmc_fisher_logistic_sample(params, data, io, _lib.stream_handle(dev))
         _lib.check(rc, "gjx_rmhmc_fisher_logistic_sample")

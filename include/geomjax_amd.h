@@ -279,7 +279,12 @@ typedef struct {
   int32_t max_iters;             /* fixed-point iteration cap */
   int32_t num_transitions;       /* T >= 1 (sample entry point) */
   int32_t flags;                 /* 0 */
-  int32_t reserved;
+  /* Tuned RMHMC path, posteriors of up to 64 features: workgroup size per
+   * chain in waves, bits 0-3 for posteriors of up to 32 features, bits 4-7
+   * for 33..64 features; each 1 or 2, or 0 for the library's default
+   * (geomjax_amd.autotune picks it per problem).  Results do not depend on it
+   * beyond float32 summation order.  Ignored elsewhere. */
+  int32_t waves_per_chain;
   double step_size;
   double prior_var;              /* 100 in the logistic configurations */
   double convergence_tol;

@@ -196,3 +196,35 @@ def test_dense_rmhmc_field_validates_on_the_host():
     io.z, io.dtdq = 1, 1
     assert lib.gjx_dense_rmhmc_field(io, None) == -1
     assert "metric_jacobian" in _lib.last_error()
+
+
+def test_waves_per_chain_field_is_validated_on_the_host():
+    """Each 4-bit field of waves_per_chain must be 0, 1 or 2; checked before any launch."""
+    lib = _lib.load()
+    data = _lib.LogisticData(num_posteriors=1, max_rows=16, max_features=8, num_rows=1, num_features=1,
+                             x_offset=1, y_offset=1, x=1, y=1)
+    for bad in (0x3, 0x30, 0x100, -1):
+        p = _lib.LogisticParams(num_chains=4, dim=8, dtype=_lib.GJX_F32, num_integration_steps=2, max_iters=2,
+                                num_transitions=1, step_size=0.1, prior_var=100.0, waves_per_chain=bad)
+        assert lib.gjx_rmhmc_fisher_logistic_step(p, data, _lib.RmhmcStepIO(), None) == _lib.GJX_EINVAL
+        assert "waves_per_chain" in _lib.last_error()
+    p = _lib.LogisticParams(num_chains=4, dim=8, dtype=_lib.GJX_F32, num_integration_steps=2, max_iters=2,
+                            num_transitions=1, step_size=0.1, prior_var=100.0, waves_per_chain=0x21)
+    assert lib.gjx_rmhmc_fisher_logistic_step(p, data, _lib.RmhmcStepIO(), None) == _lib.GJX_EINVAL
+    assert "packed" in _lib.last_error()  # the field passed; the next check fails
+
+
+def test_waves_argument_encoding():
+    import pytest
+    import torch
+
+    from geomjax_amd.rmhmc import _waves
+
+    assert _waves(3, None, 4, torch.float64) == 0  # float64 calls ignore it
+    assert _waves(1, None, 4, torch.float32) == 0x11
+    assert _waves(2, None, 4, torch.float32) == 0x22
+    assert _waves((2, 1), None, 4, torch.float32) == 0x12
+    with pytest.raises(ValueError):
+        _waves(3, None, 4, torch.float32)
+    with pytest.raises(ValueError):
+        _waves((1, 4), None, 4, torch.float32)

@@ -300,3 +300,55 @@ def test_uneven_batch_full_size_matches_each_posterior_and_f64():
         assert torch.all(s64n.position[rows, d:] == 0) and torch.all(s32n.position[rows, d:] == 0)
     torch.cuda.synchronize()
     print(f"  float64 batch vs each posterior alone: worst {worst:.2e}", flush=True)
+
+
+@pytest.mark.parametrize("waves", [1, 2, (2, 1), (1, 2)])
+def test_every_workgroup_shape_matches_f64_kernel(waves):
+    """The tuned kernels' launch variants (one or two waves per chain, per
+    packed width) all equal the float64 kernel at the tuned bars, alone and in
+    a batch of every packed width."""
+    eps, L = 0.1, 8
+    kern64 = build_kernel(functools.partial(implicit_midpoint, max_iters=6))
+    kern32 = build_kernel(functools.partial(implicit_midpoint, max_iters=6), waves_per_chain=waves)
+    for (N, D, C) in [(256, 8, 6), (64, 20, 4), (130, 33, 4), (2048, 64, 8)]:
+        X, y, _ = make_logistic(N, D, 60 + D)
+        model = gj.LogisticRegression(X, y, 100.0)
+        q0 = (0.5 * seeded_normal(61 + D, C, D)).to(DEV)
+        z, u = seeded_normal(62 + D, C, D).to(DEV), seeded_uniform(63 + D, C).to(DEV)
+        print(f"case waves={waves} N={N} D={D} C={C}", flush=True)
+        s64n, i64 = kern64(gr.HostNoise(z, u), gj.rmhmc.init(q0, model), model, eps, gj.FisherMetric(), L)
+        s32n, i32 = kern32(gr.HostNoise(z.float(), u), gj.rmhmc.init(q0.float(), model), model, eps,
+                           gj.FisherMetric(), L)
+        torch.cuda.synchronize()
+        _check_step(i32, s32n, i64, s64n, u, q0, f"{N}x{D} waves={waves}")
+    shapes = [(200, 8), (600, 128), (1500, 40), (333, 32), (2048, 64)]
+    models = [gj.LogisticRegression(*make_logistic(n, d, 70 + i)[:2], 100.0) for i, (n, d) in enumerate(shapes)]
+    batch = gj.LogisticBatch(models, 3)
+    C, Dm = batch.num_chains, batch.dim
+    q0 = torch.zeros(C, Dm, dtype=torch.float64)
+    z = torch.zeros(C, Dm, dtype=torch.float64)
+    for c in range(C):
+        d = shapes[c // 3][1]
+        q0[c, :d] = 0.5 * seeded_normal(1100 + c, d)
+        z[c, :d] = seeded_normal(1200 + c, d)
+    u = seeded_uniform(1300, C)
+    print(f"case waves={waves} batch {shapes} x 3 chains", flush=True)
+    s64n, i64 = kern64(gr.HostNoise(z.to(DEV), u.to(DEV)), gj.rmhmc.init(q0.to(DEV), batch), batch, eps,
+                       gj.FisherMetric(), L)
+    s32n, i32 = kern32(gr.HostNoise(z.float().to(DEV), u.to(DEV)), gj.rmhmc.init(q0.float().to(DEV), batch), batch,
+                       eps, gj.FisherMetric(), L)
+    torch.cuda.synchronize()
+    _check_step(i32, s32n, i64, s64n, u, q0, f"batch waves={waves}")
+
+
+def test_autotuner_choice_is_cached_per_problem():
+    from geomjax_amd import autotune
+
+    X, y, _ = make_logistic(512, 40, 80)
+    model = gj.LogisticRegression(X, y, 100.0)
+    field, timings = autotune.tune_logistic(model, 256)
+    print(f"  chosen waves_per_chain field {field:#x}, timings {timings}", flush=True)
+    assert field in autotune.LOGISTIC_WAVES
+    assert len(timings) == 2  # one width present: two distinct candidates
+    assert autotune.logistic_waves_field(model, 256) == field
+    assert autotune.tune_logistic(model, 256) == (field, {})
