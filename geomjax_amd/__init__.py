@@ -15,6 +15,7 @@ from .nuts import nuts as nutsrmhmc
 from .lmcmonge_nuts import nutslmc as nutslmcmonge
 from .lmcmc_nuts import nuts as nutslmc
 from .mmala import mmala
+from .glmc import glmc
 from .adaptation import chees_adaptation as chees_adaptation_riemanian
 from .adaptation import chees_adaptation_lmcmonge
 
@@ -27,6 +28,6 @@ __version__ = "0.1.0"
 ess = diagnostics.effective_sample_size
 rhat = diagnostics.potential_scale_reduction
 
-__all__ = ["rmhmc", "dynamic_rmhmc", "nutsrmhmc", "nuts", "nutslmcmonge", "nutslmc", "mmala", "lmc", "dynamic_lmc", "lmcmonge", "dynamic_lmcmonge",
+__all__ = ["rmhmc", "dynamic_rmhmc", "nutsrmhmc", "nuts", "nutslmcmonge", "nutslmc", "mmala", "glmc", "lmc", "dynamic_lmc", "lmcmonge", "dynamic_lmcmonge",
            "chees_adaptation_riemanian", "chees_adaptation_lmcmonge", "SamplingAlgorithm", "NealFunnel", "LogisticRegression", "LogisticBatch", "MongeMetric", "FisherMetric", "metrics", "models", "random", "sampling",
            "diagnostics", "ess", "rhat", "__version__"]
