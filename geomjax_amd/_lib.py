@@ -144,6 +144,12 @@ class LogisticData(ctypes.Structure):
                                                        ("dispatch_order", _vp)]
 
 
+class DiagnosticsArgs(ctypes.Structure):
+    _fields_ = [("num_samples", _c_int32), ("num_chains", _c_int32), ("num_coords", _c_int32), ("dtype", _c_int32),
+                ("stride_sample", ctypes.c_int64), ("stride_chain", ctypes.c_int64),
+                ("stride_coord", ctypes.c_int64)]
+
+
 class TangentIO(ctypes.Structure):
     _fields_ = [(n, _vp) for n in (
         "position", "logdensity", "z", "objective", "acceptance", "sq_jump", "d_objective", "d_acceptance",
@@ -240,6 +246,9 @@ _SIGNATURES = {
     "gjx_logistic_workspace_size": [ctypes.POINTER(LogisticParams), ctypes.POINTER(LogisticData),
                                     ctypes.POINTER(ctypes.c_uint64)],
 }
+_SIGNATURES["gjx_diagnostics_workspace_size"] = [ctypes.POINTER(DiagnosticsArgs), ctypes.POINTER(ctypes.c_uint64)]
+_SIGNATURES["gjx_effective_sample_size"] = [ctypes.POINTER(DiagnosticsArgs), _vp, _vp, ctypes.c_uint64, _vp, _vp, _vp]
+
 _STRING_FUNCS = ("gjx_version", "gjx_last_error")
 
 _lib = None

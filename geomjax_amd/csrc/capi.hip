@@ -15,6 +15,13 @@
 #include "dense_metric.h"
 #include "logistic.h"
 
+namespace gjx {
+int diagnostics_max_samples();
+uint64_t diagnostics_workspace_bytes(int N, int C, int D);
+hipError_t launch_diagnostics(const void* x, int dtype, int N, int C, int D, int64_t st, int64_t sc, int64_t sd,
+                              void* workspace, double* ess, double* rhat, hipStream_t stream);
+}  // namespace gjx
+
 namespace {
 
 thread_local std::string g_last_error;
@@ -612,6 +619,46 @@ int scratch_address(void* stream, void** out) {
 int gjx_rmhmc_fisher_logistic_step(const gjx_logistic_params* params, const gjx_logistic_data* data,
                                    const gjx_rmhmc_step_io* io, void* stream) {
   return rmhmc_logistic_step(params, data, io, stream, true);
+}
+
+namespace {
+
+int check_diagnostics(const gjx_diagnostics_args* a) {
+  GJX_REQUIRE(a != nullptr, "args is NULL");
+  GJX_REQUIRE(a->num_samples >= 2, "num_samples must be >= 2 (got %d)", a->num_samples);
+  GJX_REQUIRE(a->num_chains >= 2, "the multi-chain estimators need num_chains >= 2 (got %d)", a->num_chains);
+  GJX_REQUIRE(a->num_coords >= 1, "num_coords must be >= 1 (got %d)", a->num_coords);
+  GJX_REQUIRE(a->dtype == GJX_F64 || a->dtype == GJX_F32, "dtype must be GJX_F64 or GJX_F32 (got %d)", a->dtype);
+  if (a->num_samples > gjx::diagnostics_max_samples())
+    return fail(GJX_EUNSUPPORTED, "num_samples %d exceeds the %d a chain's series can hold in LDS", a->num_samples,
+                gjx::diagnostics_max_samples());
+  return GJX_OK;
+}
+
+}  // namespace
+
+int gjx_diagnostics_workspace_size(const gjx_diagnostics_args* args, uint64_t* bytes) {
+  int rc = check_diagnostics(args);
+  if (rc != GJX_OK) return rc;
+  GJX_REQUIRE(bytes != nullptr, "bytes is NULL");
+  *bytes = gjx::diagnostics_workspace_bytes(args->num_samples, args->num_chains, args->num_coords);
+  return GJX_OK;
+}
+
+int gjx_effective_sample_size(const gjx_diagnostics_args* args, const void* samples, void* workspace,
+                              uint64_t workspace_bytes, double* ess, double* rhat, void* stream) {
+  int rc = check_diagnostics(args);
+  if (rc != GJX_OK) return rc;
+  GJX_REQUIRE(samples != nullptr, "samples is NULL");
+  GJX_REQUIRE(ess != nullptr || rhat != nullptr, "nothing to compute: ess and rhat are both NULL");
+  const uint64_t need = gjx::diagnostics_workspace_bytes(args->num_samples, args->num_chains, args->num_coords);
+  GJX_REQUIRE(workspace != nullptr && workspace_bytes >= need, "the workspace needs %llu bytes (got %llu)",
+              (unsigned long long)need, (unsigned long long)workspace_bytes);
+  hipError_t e = gjx::launch_diagnostics(samples, args->dtype, args->num_samples, args->num_chains, args->num_coords,
+                                         args->stride_sample, args->stride_chain, args->stride_coord, workspace, ess,
+                                         rhat, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "gjx_effective_sample_size launch");
+  return GJX_OK;
 }
 
 int gjx_rmhmc_fisher_logistic_step_noalloc(const gjx_logistic_params* params, const gjx_logistic_data* data,

@@ -1,14 +1,51 @@
-"""Convergence diagnostics on device tensors: R-hat and effective sample size.
+"""Convergence diagnostics: R-hat and effective sample size.
 
 Same estimators as the reference (geomjax/diagnostics.py:25-75 and :78-209):
-FFT autocovariance, Geyer's initial positive and initial monotone sequences,
-tau floored at 1/log10(M N).  Written with torch so that min-ESS over
-thousands of chains is computed where the samples already live (no host
-round trip of the [T, C, D] sample array).  Computed in float64.
+autocovariances of every chain, Geyer's initial positive and initial
+monotone sequences, tau floored at 1/log10(M N).  Computed in float64.
+
+Samples on a HIP device are reduced there by the library's diagnostics
+kernels (``gjx_effective_sample_size``: lag sums of each chain tile in LDS,
+then the Geyer steps per coordinate), so min-ESS over thousands of chains
+needs no host round trip of the [T, C, D] sample array.  Host tensors and
+arrays take the same estimators written with torch (FFT autocovariance).
 """
+import ctypes
 import math
 
 import torch
+
+from . import _lib
+
+
+def _device_diagnostics(x, chain_axis, sample_axis, want_ess, want_rhat):
+    """ESS and/or R-hat of a CUDA tensor through the HIP kernels; results shaped
+    like the reference's (the chain and sample axes removed, squeezed)."""
+    sample_axis = sample_axis % x.dim()
+    chain_axis = chain_axis % x.dim()
+    rest = [d for d in range(x.dim()) if d not in (sample_axis, chain_axis)]
+    rest_shape = [x.shape[d] for d in rest]
+    v = x.permute([sample_axis, chain_axis] + rest)
+    if v.dtype not in (torch.float32, torch.float64):
+        v = v.to(torch.float64)
+    N, M = v.shape[0], v.shape[1]
+    v = v.reshape(N, M, -1)  # a view where the layout allows, else a device copy
+    D = v.shape[2]
+    args = _lib.DiagnosticsArgs(num_samples=N, num_chains=M, num_coords=D,
+                                dtype=_lib.GJX_F32 if v.dtype == torch.float32 else _lib.GJX_F64,
+                                stride_sample=v.stride(0), stride_chain=v.stride(1), stride_coord=v.stride(2))
+    lib = _lib.load()
+    nbytes = ctypes.c_uint64(0)
+    _lib.check(lib.gjx_diagnostics_workspace_size(ctypes.byref(args), ctypes.byref(nbytes)),
+               "gjx_diagnostics_workspace_size")
+    ws = torch.empty(max(int(nbytes.value), 8), dtype=torch.uint8, device=x.device)
+    ess = torch.empty(D, dtype=torch.float64, device=x.device) if want_ess else None
+    rhat = torch.empty(D, dtype=torch.float64, device=x.device) if want_rhat else None
+    rc = lib.gjx_effective_sample_size(ctypes.byref(args), v.data_ptr(), ws.data_ptr(), ws.numel(), _lib.ptr(ess),
+                                       _lib.ptr(rhat), _lib.stream_handle(x.device))
+    _lib.check(rc, "gjx_effective_sample_size")
+    shape = lambda t: None if t is None else t.reshape(rest_shape).squeeze()
+    return shape(ess), shape(rhat)
 
 
 def _next_fast_len(n):
@@ -28,9 +65,12 @@ def _next_fast_len(n):
 
 
 def potential_scale_reduction(input_array, chain_axis=0, sample_axis=1):
-    x = torch.as_tensor(input_array).to(torch.float64)
+    x = torch.as_tensor(input_array)
     if x.shape[chain_axis] < 2:
         raise ValueError("potential_scale_reduction as implemented only works for two or more chains.")
+    if x.is_cuda:
+        return _device_diagnostics(x, chain_axis, sample_axis, False, True)[1]
+    x = x.to(torch.float64)
     n = x.shape[sample_axis]
     mean = x.mean(dim=sample_axis, keepdim=True)
     var = x.var(dim=sample_axis, unbiased=True, keepdim=True)
@@ -40,13 +80,16 @@ def potential_scale_reduction(input_array, chain_axis=0, sample_axis=1):
 
 
 def effective_sample_size(input_array, chain_axis=0, sample_axis=1):
-    x = torch.as_tensor(input_array).to(torch.float64)
+    x = torch.as_tensor(input_array)
     sample_axis = sample_axis % x.dim()
     chain_axis = chain_axis % x.dim()
     M = x.shape[chain_axis]
     N = x.shape[sample_axis]
     if M < 2:
         raise ValueError("effective_sample_size as implemented only works for two or more chains.")
+    if x.is_cuda:
+        return _device_diagnostics(x, chain_axis, sample_axis, True, False)[0]
+    x = x.to(torch.float64)
     # canonical layout [N, M, rest...]
     rest = [d for d in range(x.dim()) if d not in (sample_axis, chain_axis)]
     x = x.permute([sample_axis, chain_axis] + rest)

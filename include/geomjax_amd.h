@@ -660,6 +660,29 @@ int gjx_funnel_value_and_grad(int32_t num_chains, int32_t dim, double sigma, int
                               const void* position, double* logdensity, void* logdensity_grad,
                               void* stream);
 
+/* Multi-chain convergence diagnostics of a sample array on the device
+ * (geomjax/diagnostics.py:25-75 potential_scale_reduction and :78-209
+ * effective_sample_size, float64 arithmetic).  Element (t, c, d) of the
+ * samples is at samples[t * stride_sample + c * stride_chain +
+ * d * stride_coord] (element strides: any axis order, views included). */
+typedef struct {
+  int32_t num_samples;   /* N >= 2 (at most 20352) */
+  int32_t num_chains;    /* M >= 2 */
+  int32_t num_coords;    /* D >= 1 */
+  int32_t dtype;         /* GJX_F64 or GJX_F32 */
+  int64_t stride_sample;
+  int64_t stride_chain;
+  int64_t stride_coord;
+} gjx_diagnostics_args;
+
+/* Bytes of device scratch gjx_effective_sample_size needs. */
+int gjx_diagnostics_workspace_size(const gjx_diagnostics_args* args, uint64_t* bytes);
+
+/* ess [D] and/or rhat [D] (either may be NULL) of every coordinate over all
+ * chains.  Stream-ordered; workspace of gjx_diagnostics_workspace_size bytes. */
+int gjx_effective_sample_size(const gjx_diagnostics_args* args, const void* samples, void* workspace,
+                              uint64_t workspace_bytes, double* ess, double* rhat, void* stream);
+
 /* Library identification and diagnostics. */
 const char* gjx_version(void);
 const char* gjx_last_error(void);

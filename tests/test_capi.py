@@ -228,3 +228,22 @@ def test_waves_argument_encoding():
         _waves(3, None, 4, torch.float32)
     with pytest.raises(ValueError):
         _waves((1, 4), None, 4, torch.float32)
+
+
+def test_diagnostics_entry_points_validate_on_the_host():
+    import ctypes
+
+    lib = _lib.load()
+    nbytes = ctypes.c_uint64(0)
+    a = _lib.DiagnosticsArgs(num_samples=100, num_chains=8, num_coords=3, dtype=_lib.GJX_F32, stride_sample=24,
+                             stride_chain=3, stride_coord=1)
+    assert lib.gjx_diagnostics_workspace_size(ctypes.byref(a), ctypes.byref(nbytes)) == _lib.GJX_OK
+    assert nbytes.value >= 8 * 3 * 100  # at least the per-coordinate lag sums
+    bad = _lib.DiagnosticsArgs(num_samples=100, num_chains=1, num_coords=3, dtype=_lib.GJX_F32)
+    assert lib.gjx_diagnostics_workspace_size(ctypes.byref(bad), ctypes.byref(nbytes)) == _lib.GJX_EINVAL
+    assert "num_chains" in _lib.last_error()
+    long = _lib.DiagnosticsArgs(num_samples=10 ** 6, num_chains=4, num_coords=1, dtype=_lib.GJX_F64)
+    assert lib.gjx_diagnostics_workspace_size(ctypes.byref(long), ctypes.byref(nbytes)) == _lib.GJX_EUNSUPPORTED
+    # no workspace: refused before any launch
+    assert lib.gjx_effective_sample_size(ctypes.byref(a), 1, None, 0, 1, None, None) == _lib.GJX_EINVAL
+    assert "workspace" in _lib.last_error()

@@ -1,7 +1,9 @@
-"""GPU: ESS and R-hat computed on device tensors equal the oracle (float64).
+"""GPU: ESS and R-hat of device tensors (the diagnostics HIP kernels) equal the
+oracle (float64 FFT restatement of the reference) to 1e-10.
 
 The samples never leave the device for the diagnostics; the oracle runs on a
-host copy.  Inputs: AR(1) series (rho 0, 0.7, 0.98) in both axis orders, and a
+host copy.  Inputs: AR(1) series (rho 0, 0.7, 0.98) in both axis orders,
+float32 samples, a strided view of a chain range, an odd series length, and a
 [T, C, D] = [200, 4096, 64] stack of the size the sampling benchmarks produce.
 """
 import numpy as np
@@ -52,3 +54,22 @@ def test_device_ess_at_benchmark_size_matches_oracle():
     print(f"  max relative difference {err:.2e}; min ESS {float(e.min()):.1f} "
           f"(AR(1) closed form {T * C * 0.4 / 1.6:.1f})", flush=True)
     assert err < 1e-10
+
+
+def test_device_float32_strided_views_and_odd_lengths():
+    x = _ar1((301, 40, 6), 0, 0.8, 9)
+    xd = torch.from_numpy(x).to(DEV)
+    # a chain range and a coordinate range of a [T, C, D] array: a non-contiguous view
+    view = xd[:, 8:29, 1:5]
+    e = gd.effective_sample_size(view, chain_axis=1, sample_axis=0)
+    r = gd.potential_scale_reduction(view, chain_axis=1, sample_axis=0)
+    torch.cuda.synchronize()
+    xv = np.ascontiguousarray(x[:, 8:29, 1:5])
+    assert np.allclose(e.cpu().numpy(), od.effective_sample_size(xv, chain_axis=1, sample_axis=0), rtol=1e-10)
+    assert np.allclose(r.cpu().numpy(), od.potential_scale_reduction(xv, chain_axis=1, sample_axis=0), rtol=1e-12)
+    # float32 samples: the estimators run in float64 on the float32 values
+    x32 = x.astype(np.float32)
+    e32 = gd.effective_sample_size(torch.from_numpy(x32).to(DEV), chain_axis=1, sample_axis=0)
+    torch.cuda.synchronize()
+    ref = od.effective_sample_size(x32.astype(np.float64), chain_axis=1, sample_axis=0)
+    assert np.allclose(e32.cpu().numpy(), ref, rtol=1e-10)
