@@ -188,3 +188,97 @@ def test_lmcmc_one_doubling_is_one_lmc_step():
         assert abs(float(new.volume_adjustment[c]) - float(sr.volume_adjustment[0])) < 1e-12, c
         assert abs(infos[c].acceptance_rate - float(ir.acceptance_rate[0])) < 1e-12, c
         assert rel_err(infos[c].momentum, ir.velocity[0]) < 1e-14  # NUTSInfo carries the initial velocity
+
+
+def _leaf_selection_worst_z():
+    """Worst z-score of the empirical leaf frequencies against the closed form."""
+    D, eps, M, trials = 3, 0.5, 3, 6000  # a step long enough that the eight weights differ
+    Mdl, metric = _funnel(D)
+    q0 = 0.3 * seeded_normal(66, 1, D)
+    z = seeded_normal(67, 1, D)
+    mg, ke, imvp = onuts.gaussian_riemannian(metric)
+    raw_step = onuts.implicit_midpoint(Mdl.logp, ke, imvp, max_iters=20)
+    energy_fn = onuts.hmc_energy(ke)
+    momentum = torch.func.vmap(mg)(z, q0)
+    velocity = torch.func.vmap(imvp)(q0, momentum)
+    s0 = orm.init(q0, Mdl.logp)
+    s0 = onuts.IntegratorState(q0, momentum, velocity, s0.logdensity, s0.logdensity_grad)
+    cache, ecache = {}, {}
+
+    def one_step(state, step):  # the trajectory is fixed: integrate each (state, direction) once
+        key = (id(state), step)
+        if key not in cache:
+            cache[key] = raw_step(state, step)
+        return cache[key]
+
+    def energy(state):
+        if id(state) not in ecache:
+            ecache[id(state)] = energy_fn(state)
+        return ecache[id(state)]
+
+    H0 = float(energy(s0)[0])
+    # the leaves in time order: subtree 1 = one step forward, 2 = two back, 3 = four forward
+    fwd, bwd = [s0], [s0]
+    for _ in range(5):
+        fwd.append(one_step(fwd[-1], eps))
+    for _ in range(2):
+        bwd.append(one_step(bwd[-1], -eps))
+    subtrees = [fwd[1:2], bwd[1:3], fwd[2:6]]
+    w = lambda s: math.exp(H0 - float(energy(s)[0]))
+    W_prev = w(s0)
+    a = []
+    for S in subtrees:
+        WS = sum(w(s) for s in S)
+        a.append(min(1.0, WS / W_prev))
+        W_prev += WS
+    probs = {id(s0): math.prod(1 - x for x in a)}
+    for j, S in enumerate(subtrees):
+        WS = sum(w(s) for s in S)
+        keep = a[j] * math.prod(1 - x for x in a[j + 1:])
+        for s in S:
+            probs[id(s)] = w(s) / WS * keep
+    assert abs(sum(probs.values()) - 1.0) < 1e-12
+    assert max(a) < 1.0 and min(a) < 0.9  # every merge is a real coin, not a sure keep
+    leaves = {id(s): s for S in subtrees for s in S} | {id(s0): s0}
+
+    u_dir = torch.tensor([0.2, 0.8, 0.3], dtype=torch.float64)
+    g = torch.Generator().manual_seed(62)
+    counts = {k: 0 for k in probs}
+    for _ in range(trials):
+        u_merge = torch.rand(M, generator=g, dtype=torch.float64)
+        u_step = torch.rand(2 ** M - 1, generator=g, dtype=torch.float64)
+        prop, traj, e, div, turn = onuts._expand(s0, H0, one_step, energy, eps, u_dir, u_merge, u_step, M, "euc",
+                                                  1000.0)
+        assert e == M and not div and not turn and traj.num_states == 2 ** M - 1  # the fixed trajectory
+        hit = [k for k, s in leaves.items() if torch.equal(prop.state.position, s.position)]
+        assert len(hit) == 1
+        counts[hit[0]] += 1
+    worst = 0.0
+    for k, p in probs.items():
+        se = math.sqrt(p * (1 - p) / trials)
+        zval = abs(counts[k] / trials - p) / se
+        worst = max(worst, zval)
+    print("merge probabilities", [round(x, 3) for x in a], "leaf probabilities",
+          sorted(round(p, 4) for p in probs.values()), "worst z", round(worst, 2))
+    return worst
+
+
+def test_multi_doubling_leaf_selection_matches_the_closed_form():
+    """With the directions fixed (+, -, +) and no U-turn or divergence within
+    three doublings, the trajectory's eight states are fixed; only the merge
+    and in-subtree uniforms vary.  Progressive uniform sampling inside subtree
+    S_j picks leaf i with probability w_i / W(S_j), and the biased merge keeps
+    S_j's pick with probability a_j = min(1, W(S_j) / W_{j-1}) (W_{j-1}: the
+    total weight of the trajectory before the merge), so the transition ends
+    at leaf i of S_j with probability (w_i / W(S_j)) a_j prod_{k>j} (1 - a_k)
+    and at the start with prod_k (1 - a_k).  The empirical frequencies of the
+    oracle's expansion over many uniforms must match (5 standard errors)."""
+    assert _leaf_selection_worst_z() < 5.0
+
+
+def test_an_unbiased_merge_is_caught(monkeypatch):
+    """Planted mistake: the trajectory-level merge uses the progressive uniform
+    rule instead of the biased one."""
+    orig = onuts._sample
+    monkeypatch.setattr(onuts, "_sample", lambda u, proposal, new, biased: orig(u, proposal, new, False))
+    assert _leaf_selection_worst_z() > 5.0
