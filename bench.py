@@ -522,57 +522,76 @@ def funnel_lmc_forms(K, W, dev, plain=True):
     return res
 
 
-def nuts_forms(K, W, dev, static_plain=None):
-    """NUTS with the RMHMC integrator (float64, one launch per transition) on
-    Neal's funnel D=100 with 8192 chains: Monge metric, the funnel workload's
-    step size, max_num_doublings 10, "euc" criterion, device PRNG.  Reports
-    integration steps/s, transitions/s and min-ESS/s, and min-ESS/s relative to
-    the plain static-length RMHMC form on the same posterior when given."""
+def nuts_forms(K, W, dev, static_plain=None, chunk=10):
+    """NUTS with the RMHMC integrator (float64) on Neal's funnel D=100 with 8192
+    chains: Monge metric, the funnel workload's step size, max_num_doublings
+    10, "euc" criterion, device PRNG.  Timed as persistent launches of `chunk`
+    transitions (each chain keeps its state on chip, so a launch lasts as long
+    as the largest per-chain sum of trajectory lengths) and, for comparison,
+    as one launch per transition (a launch lasts as long as its longest
+    trajectory).  Reports integration steps/s, transitions/s and min-ESS/s,
+    and min-ESS/s relative to the plain static-length RMHMC form when given."""
     import torch
 
     import geomjax_amd as gj
     from geomjax_amd import random as gr
     from geomjax_amd.nuts import build_kernel as nuts_kernel
+    from geomjax_amd.nuts import sample as nuts_sample
     from geomjax_amd.rmhmc import implicit_midpoint
 
     M = 10
+    K = max(chunk, (K // chunk) * chunk)
     model, metric = gj.NealFunnel(D_F, sigma=SIGMA), gj.MongeMetric(ALPHA2)
-    kern = nuts_kernel(functools.partial(implicit_midpoint, max_iters=MAXIT_F))
+    integ = functools.partial(implicit_midpoint, max_iters=MAXIT_F)
+    kern = nuts_kernel(integ)
     q_init = torch.randn(CHAINS_F, D_F, generator=torch.Generator().manual_seed(SEED_F), dtype=torch.float64)
     s = gj.rmhmc.init(q_init.to(dev), model)
-    for t in range(W):
-        s, _ = kern(gr.PRNGKey(SEED_F + 7, t), s, model, EPS_F, metric, M)
+    r = nuts_sample(gr.PRNGKey(SEED_F + 7, 0), s, model, EPS_F, metric, max(W, chunk), max_num_doublings=M,
+                    integrator=integ, store_samples=False)
+    s, counter = r.state, max(W, chunk)
+    warmed = s
     samples = torch.empty(K, CHAINS_F, D_F, dtype=torch.float64, device=dev)
-    steps, accs, turned = [], [], []
+    steps, accs = [], []
     stream = torch.cuda.current_stream(dev)
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     ev0.record(stream)
-    for t in range(K):
-        s, info = kern(gr.PRNGKey(SEED_F + 7, W + t), s, model, EPS_F, metric, M)
-        samples[t].copy_(s.position)
-        steps.append(info.num_integration_steps)
-        accs.append(info.acceptance_rate)
-        turned.append(info.is_turning)
+    for c in range(K // chunk):
+        r = nuts_sample(gr.PRNGKey(SEED_F + 7, counter), s, model, EPS_F, metric, chunk, max_num_doublings=M,
+                        integrator=integ, samples_out=samples[c * chunk:(c + 1) * chunk])
+        s, counter = r.state, counter + chunk
+        steps.append(r.num_integration_steps)
+        accs.append(r.acceptance_rate)
     ev1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     kernel_s = ev0.elapsed_time(ev1) / 1e3
-    total_steps = int(torch.stack(steps).long().sum().item())
+    st = torch.cat(steps)
+    total_steps = int(st.long().sum().item())
     ess, _ = min_ess(samples)
     del samples
     res = {"value": total_steps / wall, "unit": "integration-steps/s", "transitions_per_sec": K / wall,
            "min_ess_per_sec": ess / wall, "min_ess": ess, "mean_integration_steps": total_steps / (K * CHAINS_F),
-           "max_integration_steps": int(torch.stack(steps).max().item()),
-           "acceptance_rate": float(torch.stack(accs).mean().item()),
-           "turned_fraction": float(torch.stack(turned).float().mean().item()),
+           "max_integration_steps": int(st.max().item()),
+           "acceptance_rate": float(torch.cat(accs).mean().item()),
            "ms_per_transition": wall / K * 1e3, "kernel_ms_per_transition": kernel_s / K * 1e3,
-           "kernel": "funnel_nuts_kernel<2>", "dtype": "f64", "max_num_doublings": M, "steps": K}
+           "kernel": "funnel_nuts_kernel<2>", "form": f"persistent launches of {chunk} transitions",
+           "dtype": "f64", "max_num_doublings": M, "steps": K}
+    # the same chains, one launch per transition
+    n1 = min(K, 10)
+    s = warmed
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for t in range(n1):
+        s, info = kern(gr.PRNGKey(SEED_F + 7, 10 ** 6 + t), s, model, EPS_F, metric, M)
+    torch.cuda.synchronize()
+    res["one_launch_per_transition_ms"] = (time.perf_counter() - t0) / n1 * 1e3
     if static_plain is not None:
         res["min_ess_per_sec_vs_static_rmhmc_plain"] = res["min_ess_per_sec"] / static_plain["min_ess_per_sec"]
     log(f"funnel NUTS: {res['value']:.3e} integration-steps/s, {res['mean_integration_steps']:.1f} steps per "
-        f"transition, min-ESS/s {res['min_ess_per_sec']:.3e}")
+        f"transition, {res['ms_per_transition']:.2f} ms per transition persistent vs "
+        f"{res['one_launch_per_transition_ms']:.2f} one launch each, min-ESS/s {res['min_ess_per_sec']:.3e}")
     return res
 
 

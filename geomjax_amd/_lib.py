@@ -144,6 +144,13 @@ class LogisticData(ctypes.Structure):
                                                        ("dispatch_order", _vp)]
 
 
+class NutsSampleIO(ctypes.Structure):
+    _fields_ = [(n, _vp) for n in (
+        "position", "logdensity", "z", "u_direction", "u_merge", "u_step", "final_position", "final_logdensity",
+        "final_logdensity_grad", "samples", "acceptance_rate", "num_integration_steps", "num_fevals",
+    )]
+
+
 class DiagnosticsArgs(ctypes.Structure):
     _fields_ = [("num_samples", _c_int32), ("num_chains", _c_int32), ("num_coords", _c_int32), ("dtype", _c_int32),
                 ("stride_sample", ctypes.c_int64), ("stride_chain", ctypes.c_int64),
@@ -246,6 +253,8 @@ _SIGNATURES = {
     "gjx_logistic_workspace_size": [ctypes.POINTER(LogisticParams), ctypes.POINTER(LogisticData),
                                     ctypes.POINTER(ctypes.c_uint64)],
 }
+_SIGNATURES["gjx_rmhmc_monge_funnel_nuts_sample"] = [ctypes.POINTER(FunnelParams), _c_int32, _c_int32,
+                                                     ctypes.POINTER(NutsSampleIO), _vp]
 _SIGNATURES["gjx_diagnostics_workspace_size"] = [ctypes.POINTER(DiagnosticsArgs), ctypes.POINTER(ctypes.c_uint64)]
 _SIGNATURES["gjx_effective_sample_size"] = [ctypes.POINTER(DiagnosticsArgs), _vp, _vp, ctypes.c_uint64, _vp, _vp, _vp]
 

@@ -204,6 +204,65 @@ int gjx_rmhmc_monge_funnel_tangent(const gjx_funnel_params* params, const gjx_ta
   return GJX_OK;
 }
 
+namespace {
+
+int check_funnel_nuts(const gjx_funnel_params* params, int32_t max_num_doublings, int32_t stopping_criterion) {
+  GJX_REQUIRE(params != nullptr, "params is required");
+  GJX_REQUIRE(params->dtype == GJX_F64, "the NUTS kernel runs in float64");
+  GJX_REQUIRE(params->num_chains > 0, "num_chains must be > 0 (got %d)", params->num_chains);
+  GJX_REQUIRE(params->dim >= 2, "the funnel needs dim >= 2 (got %d)", params->dim);
+  if (params->dim - 1 > 256) return fail(GJX_EUNSUPPORTED, "dim %d > 257 for the NUTS kernel", params->dim);
+  GJX_REQUIRE(max_num_doublings >= 1 && max_num_doublings <= 12, "max_num_doublings must be in [1, 12] (got %d)",
+              max_num_doublings);
+  GJX_REQUIRE(stopping_criterion == GJX_UTURN_EUC || stopping_criterion == GJX_UTURN_BET ||
+                  stopping_criterion == GJX_UTURN_RIEM,
+              "stopping_criterion must be GJX_UTURN_EUC, GJX_UTURN_BET or GJX_UTURN_RIEM (got %d)",
+              stopping_criterion);
+  GJX_REQUIRE(params->max_iters >= 0, "max_iters must be >= 0");
+  GJX_REQUIRE(std::isfinite(params->step_size), "step_size must be finite");
+  GJX_REQUIRE(std::isfinite(params->alpha2) && params->alpha2 >= 0.0, "alpha2 must be finite and >= 0");
+  GJX_REQUIRE(std::isfinite(params->sigma) && params->sigma > 0.0, "sigma must be finite and > 0");
+  return GJX_OK;
+}
+
+}  // namespace
+
+int gjx_rmhmc_monge_funnel_nuts_sample(const gjx_funnel_params* params, int32_t max_num_doublings,
+                                       int32_t stopping_criterion, const gjx_nuts_sample_io* io, void* stream) {
+  int rc = check_funnel_nuts(params, max_num_doublings, stopping_criterion);
+  if (rc != GJX_OK) return rc;
+  GJX_REQUIRE(io != nullptr, "io is required");
+  GJX_REQUIRE(params->num_transitions >= 1, "num_transitions must be >= 1 (got %d)", params->num_transitions);
+  GJX_REQUIRE(io->position && io->logdensity, "position and logdensity are required");
+  GJX_REQUIRE(io->final_position && io->final_logdensity, "final_position and final_logdensity are required");
+  const bool host = io->z != nullptr;
+  GJX_REQUIRE((io->u_direction != nullptr) == host && (io->u_merge != nullptr) == host &&
+                  (io->u_step != nullptr) == host,
+              "z, u_direction, u_merge and u_step must all be given or all be NULL");
+  gjx::FunnelArgs a = base_args(params);
+  a.T = params->num_transitions;
+  a.q_in = io->position;
+  a.logp_in = io->logdensity;
+  a.z = io->z;
+  a.final_q = io->final_position;
+  a.final_logp = io->final_logdensity;
+  a.final_g = io->final_logdensity_grad;
+  a.samples = io->samples;
+  a.sample_acc = io->acceptance_rate;
+  a.num_fevals = io->num_fevals;
+  gjx::NutsArgs n;
+  std::memset(&n, 0, sizeof(n));
+  n.max_doublings = max_num_doublings;
+  n.criterion = stopping_criterion;
+  n.u_dir = io->u_direction;
+  n.u_merge = io->u_merge;
+  n.u_step = io->u_step;
+  n.num_steps = io->num_integration_steps;
+  hipError_t e = gjx::launch_funnel_nuts(a, n, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "gjx_rmhmc_monge_funnel_nuts_sample launch");
+  return GJX_OK;
+}
+
 int gjx_rmhmc_monge_funnel_nuts(const gjx_funnel_params* params, int32_t max_num_doublings,
                                 int32_t stopping_criterion, const gjx_nuts_io* io, void* stream) {
   GJX_REQUIRE(params != nullptr && io != nullptr, "params and io are required");

@@ -332,80 +332,21 @@ __global__ void __launch_bounds__(64) funnel_nuts_kernel(const FunnelArgs a, con
   const double alpha = sqrt(a.alpha2);
   const double inv_s2 = 1.0 / (a.sigma * a.sigma);
   const double halfDx = 0.5 * Dx;
-  const uint64_t tt = a.counter;
   const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
-  const NutsUniform uniform{c, tt, k0, k1};
   const size_t base = (size_t)c * D;
   auto on = [&](int k) { return j0 + k < Dx; };
 
-  // ---- chain state and momentum p0 = chol(G) z (funnel_rmhmc.hip) ----
-  RState<EPL> s0;
-  s0.J = 0.0;
-  double z[EPL], zv;
+  // With a.final_q set the launch runs a.T transitions of the chain (the
+  // state stays in registers; the launch's time is the chains' summed
+  // trajectory lengths, not one transition's longest); else one transition
+  // with every NUTSInfo output.  Host-fed draws are then [T, C, ...].
+  const bool multi = a.final_q != nullptr;
+  const int T = multi ? a.T : 1;
+  double qx[EPL], qv, logp_c = a.logp_in[c];
 #pragma unroll
-  for (int k = 0; k < EPL; ++k) s0.x[k] = on(k) ? static_cast<const double*>(a.q_in)[base + j0 + k] : 0.0;
-  s0.v = static_cast<const double*>(a.q_in)[base + Dx];
-  const double logp0 = a.logp_in[c];
-  if (a.z != nullptr) {
-#pragma unroll
-    for (int k = 0; k < EPL; ++k) z[k] = on(k) ? static_cast<const double*>(a.z)[base + j0 + k] : 0.0;
-    zv = static_cast<const double*>(a.z)[base + Dx];
-  } else {
-#pragma unroll
-    for (int k = 0; k < EPL; k += 2) {
-      u32x4 r = philox4x32_10(
-          u32x4{(uint32_t)c, (uint32_t)tt, kStreamMomentum + (uint32_t)(j0 + k), (uint32_t)(tt >> 32)}, k0, k1);
-      double n0, n1;
-      normal2(r, n0, n1);
-      z[k] = on(k) ? n0 : 0.0;
-      if (k + 1 < EPL) z[k + 1] = on(k + 1) ? n1 : 0.0;
-    }
-    u32x4 r = philox4x32_10(u32x4{(uint32_t)c, (uint32_t)tt, kStreamMomentumV, (uint32_t)(tt >> 32)}, k0, k1);
-    double n0, n1;
-    normal2(r, n0, n1);
-    zv = n0;
-  }
-  {
-    double loc_inc[EPL], acc = 0.0;
-#pragma unroll
-    for (int k = 0; k < EPL; ++k) {
-      acc += s0.x[k] * s0.x[k];
-      loc_inc[k] = acc;
-    }
-    s0.sx = group_sum<64>(acc);
-    const double off1 = group_exclusive_scan<64>(acc, l);
-    const double e = exp(-s0.v);
-    const double a2e2 = a2 * e * e;
-    const double gv = -s0.v * inv_s2 - halfDx + 0.5 * e * s0.sx;
-    double cj[EPL], acc2 = 0.0;
-#pragma unroll
-    for (int k = 0; k < EPL; ++k) {
-      const double tj = 1.0 + a2e2 * (off1 + loc_inc[k]);
-      const double tjm = 1.0 + a2e2 * (off1 + loc_inc[k] - s0.x[k] * s0.x[k]);
-      const double aj = -alpha * e * s0.x[k];
-      cj[k] = aj * z[k] / sqrt(tj * tjm);
-      s0.px[k] = sqrt(tj / tjm) * z[k];
-      acc2 += cj[k];
-    }
-    const double off2 = group_exclusive_scan<64>(acc2, l);
-    const double tot2 = group_sum<64>(acc2);
-    double run = off2;
-#pragma unroll
-    for (int k = 0; k < EPL; ++k) {
-      const double aj = -alpha * e * s0.x[k];
-      s0.px[k] = on(k) ? s0.px[k] + aj * run : 0.0;
-      run += cj[k];
-    }
-    const double txe = 1.0 + a2e2 * s0.sx;
-    const double tv = txe + a2 * gv * gv;
-    s0.pv = sqrt(tv / txe) * zv + alpha * gv * tot2;
-  }
-  if (a.momentum != nullptr) {
-#pragma unroll
-    for (int k = 0; k < EPL; ++k)
-      if (on(k)) static_cast<double*>(a.momentum)[base + j0 + k] = s0.px[k];
-    if (l == 0) static_cast<double*>(a.momentum)[base + Dx] = s0.pv;
-  }
+  for (int k = 0; k < EPL; ++k) qx[k] = on(k) ? static_cast<const double*>(a.q_in)[base + j0 + k] : 0.0;
+  qv = static_cast<const double*>(a.q_in)[base + Dx];
+  int64_t n_steps = 0;
 
   // velocity u = G^-1 p at a state whose sums sx, xpx are current
   auto set_velocity = [&](RState<EPL>& s) {
@@ -490,22 +431,146 @@ __global__ void __launch_bounds__(64) funnel_nuts_kernel(const FunnelArgs a, con
     set_velocity(s);
   };
 
-  {
-    double d2 = 0.0;
-#pragma unroll
-    for (int k = 0; k < EPL; ++k) d2 += s0.x[k] * s0.px[k];
-    s0.xpx = group_sum<64>(d2);
-  }
-  set_velocity(s0);
-  // H0 with the chain's log density as given (the initial IntegratorState carries it)
-  const double H0 = -logp0 + kinetic(s0);
+  for (int tr = 0; tr < T; ++tr) {
+    const uint64_t tt = a.counter + (uint64_t)tr;
+    const NutsUniform uniform{c, tt, k0, k1};
+    NutsArgs ntr = nu;
+    const int M = nu.max_doublings;
+    if (nu.u_dir) {
+      ntr.u_dir = nu.u_dir + (size_t)tr * a.C * M;
+      ntr.u_merge = nu.u_merge + (size_t)tr * a.C * M;
+      ntr.u_step = nu.u_step + (size_t)tr * a.C * ((1 << M) - 1);
+    }
+    // ---- chain state and momentum p0 = chol(G) z (funnel_rmhmc.hip) ----
+    RState<EPL> s0;
+    s0.J = 0.0;
+    double z[EPL], zv;
+  #pragma unroll
+    for (int k = 0; k < EPL; ++k) s0.x[k] = qx[k];
+    s0.v = qv;
+    const double logp0 = logp_c;
+    if (a.z != nullptr) {
+      const size_t zb = ((size_t)tr * a.C + c) * D;
+  #pragma unroll
+      for (int k = 0; k < EPL; ++k) z[k] = on(k) ? static_cast<const double*>(a.z)[zb + j0 + k] : 0.0;
+      zv = static_cast<const double*>(a.z)[zb + Dx];
+    } else {
+  #pragma unroll
+      for (int k = 0; k < EPL; k += 2) {
+        u32x4 r = philox4x32_10(
+            u32x4{(uint32_t)c, (uint32_t)tt, kStreamMomentum + (uint32_t)(j0 + k), (uint32_t)(tt >> 32)}, k0, k1);
+        double n0, n1;
+        normal2(r, n0, n1);
+        z[k] = on(k) ? n0 : 0.0;
+        if (k + 1 < EPL) z[k + 1] = on(k + 1) ? n1 : 0.0;
+      }
+      u32x4 r = philox4x32_10(u32x4{(uint32_t)c, (uint32_t)tt, kStreamMomentumV, (uint32_t)(tt >> 32)}, k0, k1);
+      double n0, n1;
+      normal2(r, n0, n1);
+      zv = n0;
+    }
+    {
+      double loc_inc[EPL], acc = 0.0;
+  #pragma unroll
+      for (int k = 0; k < EPL; ++k) {
+        acc += s0.x[k] * s0.x[k];
+        loc_inc[k] = acc;
+      }
+      s0.sx = group_sum<64>(acc);
+      const double off1 = group_exclusive_scan<64>(acc, l);
+      const double e = exp(-s0.v);
+      const double a2e2 = a2 * e * e;
+      const double gv = -s0.v * inv_s2 - halfDx + 0.5 * e * s0.sx;
+      double cj[EPL], acc2 = 0.0;
+  #pragma unroll
+      for (int k = 0; k < EPL; ++k) {
+        const double tj = 1.0 + a2e2 * (off1 + loc_inc[k]);
+        const double tjm = 1.0 + a2e2 * (off1 + loc_inc[k] - s0.x[k] * s0.x[k]);
+        const double aj = -alpha * e * s0.x[k];
+        cj[k] = aj * z[k] / sqrt(tj * tjm);
+        s0.px[k] = sqrt(tj / tjm) * z[k];
+        acc2 += cj[k];
+      }
+      const double off2 = group_exclusive_scan<64>(acc2, l);
+      const double tot2 = group_sum<64>(acc2);
+      double run = off2;
+  #pragma unroll
+      for (int k = 0; k < EPL; ++k) {
+        const double aj = -alpha * e * s0.x[k];
+        s0.px[k] = on(k) ? s0.px[k] + aj * run : 0.0;
+        run += cj[k];
+      }
+      const double txe = 1.0 + a2e2 * s0.sx;
+      const double tv = txe + a2 * gv * gv;
+      s0.pv = sqrt(tv / txe) * zv + alpha * gv * tot2;
+    }
+    if (!multi && a.momentum != nullptr) {
+  #pragma unroll
+      for (int k = 0; k < EPL; ++k)
+        if (on(k)) static_cast<double*>(a.momentum)[base + j0 + k] = s0.px[k];
+      if (l == 0) static_cast<double*>(a.momentum)[base + Dx] = s0.pv;
+    }
 
-  RState<EPL> left, right;
-  Prop<EPL> prop;
-  const TreeOut t = nuts_tree<EPL>(s0, H0, one_step, energy, uniform, nu, a.eps, a.div_threshold, ck, c, left, right,
-                                   prop);
-  write_outputs<EPL>(prop, left.x, left.v, right.x, right.v, t, c, D, a.sigma, a.q_out, a.logp_out, a.g_out,
-                     nullptr, nu);
+    {
+      double d2 = 0.0;
+#pragma unroll
+      for (int k = 0; k < EPL; ++k) d2 += s0.x[k] * s0.px[k];
+      s0.xpx = group_sum<64>(d2);
+    }
+    set_velocity(s0);
+    // H0 with the chain's log density as given (the initial IntegratorState carries it)
+    const double H0 = -logp0 + kinetic(s0);
+
+    RState<EPL> left, right;
+    Prop<EPL> prop;
+    const TreeOut t = nuts_tree<EPL>(s0, H0, one_step, energy, uniform, ntr, a.eps, a.div_threshold, ck, c, left,
+                                     right, prop);
+    if (!multi) {
+      write_outputs<EPL>(prop, left.x, left.v, right.x, right.v, t, c, D, a.sigma, a.q_out, a.logp_out, a.g_out,
+                         nullptr, nu);
+      return;
+    }
+    // the proposal is the chain's next state (geomjax/rmhmc/nuts.py:150-159)
+    double sxp = 0.0;
+#pragma unroll
+    for (int k = 0; k < EPL; ++k) {
+      qx[k] = prop.x[k];
+      sxp += prop.x[k] * prop.x[k];
+    }
+    qv = prop.v;
+    sxp = group_sum<64>(sxp);
+    logp_c = funnel_logp(prop.v, sxp, a.sigma, Dx);
+    n_steps += t.num_states;
+    if (a.samples != nullptr) {
+      const size_t sb = ((size_t)tr * a.C + c) * D;
+#pragma unroll
+      for (int k = 0; k < EPL; ++k)
+        if (on(k)) static_cast<double*>(a.samples)[sb + j0 + k] = qx[k];
+      if (l == 0) static_cast<double*>(a.samples)[sb + Dx] = qv;
+    }
+    if (l == 0) {
+      if (a.sample_acc) a.sample_acc[(size_t)tr * a.C + c] = (float)(exp(prop.slpa) / (double)t.num_states);
+      if (nu.num_steps) nu.num_steps[(size_t)tr * a.C + c] = t.num_states;
+    }
+  }
+  // final state of a multi-transition launch
+  double sx = 0.0;
+#pragma unroll
+  for (int k = 0; k < EPL; ++k) sx += qx[k] * qx[k];
+  sx = group_sum<64>(sx);
+  const double e = exp(-qv);
+#pragma unroll
+  for (int k = 0; k < EPL; ++k) {
+    if (!on(k)) continue;
+    static_cast<double*>(a.final_q)[base + j0 + k] = qx[k];
+    if (a.final_g) static_cast<double*>(a.final_g)[base + j0 + k] = -e * qx[k];
+  }
+  if (l == 0) {
+    static_cast<double*>(a.final_q)[base + Dx] = qv;
+    if (a.final_g) static_cast<double*>(a.final_g)[base + Dx] = -qv * inv_s2 - halfDx + 0.5 * e * sx;
+    a.final_logp[c] = logp_c;
+    if (a.num_fevals) a.num_fevals[c] = n_steps;
+  }
 }
 
 // ---------------------------------------------------------------------------

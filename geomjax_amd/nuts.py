@@ -42,6 +42,16 @@ class NUTSNoise(NamedTuple):
     u_step: torch.Tensor       # [C, 2^M - 1]
 
 
+class NUTSSampleResult(NamedTuple):
+    """SampleResult of a persistent NUTS launch, with the trajectory lengths."""
+    state: RMHMCState
+    samples: torch.Tensor                # [T, C, D] or None
+    acceptance_rate: torch.Tensor        # [T, C] float32
+    num_accepted: None                   # NUTS samples from the trajectory: no accept count
+    num_fevals: torch.Tensor             # [C] int64: integration steps of the launch
+    num_integration_steps: torch.Tensor  # [T, C] int32
+
+
 class NUTSInfo(NamedTuple):  # geomjax/rmhmc/nuts.py:36-75 (same fields and order)
     momentum: torch.Tensor
     is_divergent: torch.Tensor
@@ -160,6 +170,68 @@ def build_kernel(integrator=implicit_midpoint, divergence_threshold: int = 1000)
     return kernel
 
 
+def sample(rng_key, state, logdensity_fn, step_size, metric_fn, num_samples, *, max_num_doublings=10,
+           stopping_criterion="euc", integrator=implicit_midpoint, divergence_threshold=1000, store_samples=True,
+           samples_out=None):
+    """``num_samples`` NUTS transitions of every chain in ONE launch (Neal's
+    funnel with the Monge metric): the reference's loop over the kernel
+    (examples/funnel/main.py:16-25).  The chains keep their state on chip, so
+    the launch lasts as long as the largest per-chain sum of trajectory
+    lengths.  ``rng_key``: a PRNGKey (transition t draws at counter + t) or
+    NUTSNoise with a leading [T] axis on every array.  Returns
+    ``SampleResult`` (num_fevals = integration steps per chain)."""
+    if not (isinstance(logdensity_fn, NealFunnel) and isinstance(metric_fn, MongeMetric)):
+        raise NotImplementedError("the persistent NUTS launch serves Neal's funnel with the Monge metric")
+    if stopping_criterion not in _lib.GJX_UTURN:
+        raise ValueError(f"stopping_criterion must be one of {sorted(_lib.GJX_UTURN)}")
+    solver = _solver_kwargs(integrator)
+    position = state.position.contiguous()
+    if position.dim() != 2 or position.dtype != torch.float64 or not position.is_cuda:
+        raise ValueError("sample() runs a batch of chains: position must be a float64 [C, D] tensor on a HIP device")
+    C, D = position.shape
+    if D != logdensity_fn.D:
+        raise ValueError(f"position has {D} coordinates, the model has {logdensity_fn.D}")
+    T, M = int(num_samples), int(max_num_doublings)
+    if T < 1:
+        raise ValueError("num_samples must be >= 1")
+    dev = position.device
+    logdensity = state.logdensity.reshape(C).to(torch.float64).contiguous()
+    if isinstance(rng_key, NUTSNoise):
+        f = lambda t, *shape: t.reshape(*shape).to(device=dev, dtype=torch.float64).contiguous()
+        noise = (f(rng_key.z, T, C, D), f(rng_key.u_direction, T, C, M), f(rng_key.u_merge, T, C, M),
+                 f(rng_key.u_step, T, C, 2 ** M - 1))
+        seed, counter = 0, 0
+    elif isinstance(rng_key, PRNGKey):
+        noise = (None, None, None, None)
+        seed, counter = rng_key.seed, rng_key.counter
+    else:
+        raise TypeError("rng_key must be NUTSNoise or PRNGKey")
+    if samples_out is not None:
+        if samples_out.shape != (T, C, D) or samples_out.dtype != torch.float64 or not samples_out.is_contiguous():
+            raise ValueError(f"samples_out must be a contiguous [{T}, {C}, {D}] float64 tensor")
+        samples = samples_out
+    else:
+        samples = torch.empty(T, C, D, dtype=torch.float64, device=dev) if store_samples else None
+    final_q = torch.empty_like(position)
+    final_l = torch.empty(C, dtype=torch.float64, device=dev)
+    final_g = torch.empty_like(position)
+    acc = torch.empty(T, C, dtype=torch.float32, device=dev)
+    steps = torch.empty(T, C, dtype=torch.int32, device=dev)
+    fev = torch.empty(C, dtype=torch.int64, device=dev)
+    io = _lib.NutsSampleIO(position=position.data_ptr(), logdensity=logdensity.data_ptr(), z=_lib.ptr(noise[0]),
+                           u_direction=_lib.ptr(noise[1]), u_merge=_lib.ptr(noise[2]), u_step=_lib.ptr(noise[3]),
+                           final_position=final_q.data_ptr(), final_logdensity=final_l.data_ptr(),
+                           final_logdensity_grad=final_g.data_ptr(), samples=_lib.ptr(samples),
+                           acceptance_rate=acc.data_ptr(), num_integration_steps=steps.data_ptr(),
+                           num_fevals=fev.data_ptr())
+    params = _params(C, D, _lib.GJX_F64, step_size, 0, solver, metric_fn.alpha2, logdensity_fn.sigma,
+                     divergence_threshold, T=T, seed=seed, counter=counter)
+    rc = _lib.load().gjx_rmhmc_monge_funnel_nuts_sample(params, M, _lib.GJX_UTURN[stopping_criterion], io,
+                                                         _lib.stream_handle(dev))
+    _lib.check(rc, "gjx_rmhmc_monge_funnel_nuts_sample")
+    return NUTSSampleResult(RMHMCState(final_q, final_l, final_g), samples, acc, None, fev, steps)
+
+
 class nuts:
     """geomjax/rmhmc/nuts.py:164-250."""
 
@@ -179,7 +251,16 @@ class nuts:
             return kernel(rng_key, state, logdensity_fn, step_size, metric_fn, max_num_doublings,
                           stopping_criterion=stopping_criterion)
 
+        if isinstance(logdensity_fn, NealFunnel) and isinstance(metric_fn, MongeMetric):
+            from .sampling import BoundAlgorithm
+
+            def sample_fn(rng_key, state, num_samples, **kw):
+                return sample(rng_key, state, logdensity_fn, step_size, metric_fn, num_samples,
+                              max_num_doublings=max_num_doublings, stopping_criterion=stopping_criterion,
+                              integrator=integrator, divergence_threshold=divergence_threshold, **kw)
+
+            return BoundAlgorithm(init_fn, step_fn, sample_fn)
         return SamplingAlgorithm(init_fn, step_fn)
 
 
-__all__ = ["nuts", "build_kernel", "NUTSInfo", "NUTSNoise", "init"]
+__all__ = ["nuts", "build_kernel", "sample", "NUTSInfo", "NUTSNoise", "init"]

@@ -630,6 +630,32 @@ typedef struct {
 int gjx_rmhmc_monge_funnel_nuts(const gjx_funnel_params* params, int32_t max_num_doublings,
                                 int32_t stopping_criterion, const gjx_nuts_io* io, void* stream);
 
+/* T = params->num_transitions NUTS transitions of every chain in one launch:
+ * the reference's sampling loop over the kernel above (a lax.scan over
+ * jax.vmap(step), examples/funnel/main.py:16-25).  Each chain keeps its state
+ * on chip between transitions, so the launch takes as long as the largest
+ * sum of T trajectory lengths, not T times the longest single trajectory.
+ * Transition t draws at (params->seed, params->counter + t), or from the
+ * host arrays, each with a leading [T] axis. */
+typedef struct {
+  const void* position;          /* [C, D] */
+  const double* logdensity;      /* [C] */
+  const void* z;                 /* [T, C, D] or NULL */
+  const double* u_direction;     /* [T, C, M] or NULL */
+  const double* u_merge;         /* [T, C, M] or NULL */
+  const double* u_step;          /* [T, C, 2^M - 1] or NULL */
+  void* final_position;          /* [C, D] */
+  double* final_logdensity;      /* [C] */
+  void* final_logdensity_grad;   /* [C, D] optional */
+  void* samples;                 /* [T, C, D] optional */
+  float* acceptance_rate;        /* [T, C] optional */
+  int32_t* num_integration_steps;/* [T, C] optional */
+  int64_t* num_fevals;           /* [C] optional: integration steps of the launch */
+} gjx_nuts_sample_io;
+
+int gjx_rmhmc_monge_funnel_nuts_sample(const gjx_funnel_params* params, int32_t max_num_doublings,
+                                       int32_t stopping_criterion, const gjx_nuts_sample_io* io, void* stream);
+
 /* NUTS with the RMHMC integrator for Bayesian logistic regression with the
  * Fisher metric (geomjax/rmhmc/nuts.py:78-161 over the float64 logistic
  * integrator of gjx_rmhmc_fisher_logistic_step): float64 (params->dtype =

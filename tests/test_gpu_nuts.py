@@ -378,3 +378,47 @@ def test_nutslmc_sampling_algorithm_runs_on_device_noise():
     assert (steps >= 1).all() and (steps <= 2 ** 6 - 1).all()
     acc = info.acceptance_rate.cpu()
     assert ((acc >= 0) & (acc <= 1)).all()
+
+
+@pytest.mark.parametrize("crit", ["euc", "riem"])
+def test_persistent_nuts_launch_equals_step_launches(crit):
+    """T transitions in one launch (the chains keep their state on chip) equal T
+    single-transition launches fed the same draws: host-fed NUTSNoise with a
+    leading [T] axis, and the device PRNG (transition t at counter + t)."""
+    from geomjax_amd.nuts import NUTSNoise, sample
+
+    D, C, eps, M, T = 12, 6, 0.1, 5, 4
+    print(f"case persistent NUTS D={D} C={C} M={M} T={T} criterion={crit}", flush=True)
+    model = gj.NealFunnel(D)
+    integ = functools.partial(implicit_midpoint, max_iters=8)
+    kern = nuts_kernel(integ)
+    s0 = gj.rmhmc.init((0.5 * seeded_normal(140, C, D)).to(DEV), model)
+    z = seeded_normal(141, T, C, D).to(DEV)
+    ud, um = seeded_uniform(142, T, C, M).to(DEV), seeded_uniform(143, T, C, M).to(DEV)
+    us = seeded_uniform(144, T, C, 2 ** M - 1).to(DEV)
+    res = sample(NUTSNoise(z, ud, um, us), s0, model, eps, gj.MongeMetric(1.0), T, max_num_doublings=M,
+                 stopping_criterion=crit, integrator=integ)
+    s = s0
+    for t in range(T):
+        s, info = kern(NUTSNoise(z[t], ud[t], um[t], us[t]), s, model, eps, gj.MongeMetric(1.0), M,
+                       stopping_criterion=crit)
+        torch.cuda.synchronize()
+        print(f"  t={t}: steps {info.num_integration_steps.tolist()} vs {res.num_integration_steps[t].tolist()}, "
+              f"max |position difference| {(res.samples[t] - s.position).abs().max().item():.3e}", flush=True)
+        assert torch.equal(res.num_integration_steps[t], info.num_integration_steps)
+        assert rel_err(res.samples[t], s.position) < 1e-13
+        assert torch.allclose(res.acceptance_rate[t].double(), info.acceptance_rate, rtol=1e-6)
+    torch.cuda.synchronize()
+    assert rel_err(res.state.position, s.position) < 1e-13
+    assert rel_err(res.state.logdensity, s.logdensity) < 1e-13
+    assert rel_err(res.state.logdensity_grad, s.logdensity_grad) < 1e-13
+    assert torch.equal(res.num_fevals, res.num_integration_steps.to(torch.int64).sum(0))
+    # device PRNG
+    key = gr.PRNGKey(7, 100)
+    res = sample(key, s0, model, eps, gj.MongeMetric(1.0), T, max_num_doublings=M, stopping_criterion=crit,
+                 integrator=integ)
+    s = s0
+    for t in range(T):
+        s, _ = kern(gr.PRNGKey(7, 100 + t), s, model, eps, gj.MongeMetric(1.0), M, stopping_criterion=crit)
+    torch.cuda.synchronize()
+    assert rel_err(res.state.position, s.position) < 1e-13
