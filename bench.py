@@ -914,7 +914,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the 1024-chain and funnel comparisons")
-    ap.add_argument("--only", choices=["headline", "uneven4096", "funnel_nuts", "logistic_nuts", "logistic_nuts_lmc", "lmc_alpha2", "logistic_lmc", "mmala",
+    ap.add_argument("--only", choices=["headline", "uneven4096", "funnel", "funnel_nuts", "logistic_nuts", "logistic_nuts_lmc", "lmc_alpha2", "logistic_lmc", "mmala",
                              "uneven", "dense_lmc", "dense_rmhmc"],
                     help="run one secondary workload alone and print its JSON (for profiling it)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -938,6 +938,9 @@ def main():
         log(f"copied {x.numel() * 4} bytes x 3")
         return
     K, W = max(1, args.steps), max(0, args.warmup)
+    if args.only == "funnel":
+        print(json.dumps(funnel_forms(K, W, dev, plain=not args.no_plain)), flush=True)
+        return
     if args.only == "funnel_nuts":
         print(json.dumps(nuts_forms(K, W, dev)), flush=True)
         return

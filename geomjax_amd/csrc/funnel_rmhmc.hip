@@ -42,6 +42,9 @@ namespace gjx {
 template <class T, int LPC, int EPL, bool STEP>
 __global__ void __launch_bounds__(256) funnel_rmhmc_kernel(const FunnelArgs a) {
   using R = T;
+  // float32 (the tuned form) iterates in span{x0, p0}; float64 updates every
+  // coordinate, the reference's own operation order
+  constexpr bool kSpan = sizeof(T) == sizeof(float);
   const int tid = blockIdx.x * 256 + threadIdx.x;
   const int chain = tid / LPC;
   const int lg = tid % LPC;
@@ -207,7 +210,57 @@ __global__ void __launch_bounds__(256) funnel_rmhmc_kernel(const FunnelArgs a) {
       double norm = 0.0;
       int n = -1;  // the first evaluation (integrators.py:84) is outside the loop count
       bool active = on;
-      while (true) {
+      if constexpr (kSpan) {
+        // Every iterate is an affine map of the previous one with per-chain
+        // coefficients, so (x, p) stays in span{x0, p0}:
+        //   x = ca x0 + cb p0,  p = cg x0 + cd p0,
+        // |x|^2 and x.p follow from the three sums of the step start, and only
+        // the convergence norm max |x' - x|, |p' - p| is formed per coordinate.
+        R spp = R(0);
+#pragma unroll
+        for (int k = 0; k < EPL; ++k) spp += p0[k] * p0[k];
+        const R Sxx = sx, Sxp = xpx, Spp = group_sum<LPC>(spp);
+        R ca = R(1), cb = R(0), cg = R(0), cd = R(1);
+        while (true) {
+          const FieldCoef<R> f = field_coef<R>(vc, pvc, sx, xpx, a2, inv_s2, halfDx);
+          // x' = x0 + h (p + ce x),  p' = p0 - h (A x + B p)
+          const R na = R(1) + h * (cg + f.ce * ca), nb = h * (cd + f.ce * cb);
+          const R ng = -h * (f.A * ca + f.B * cg), nd = R(1) - h * (f.A * cb + f.B * cd);
+          const R da = na - ca, db = nb - cb, dg = ng - cg, dd = nd - cd;
+          R mx = R(0);
+#pragma unroll
+          for (int k = 0; k < EPL; ++k)
+            mx = fmaxf(mx, fmaxf(fabsf(da * x0[k] + db * p0[k]), fabsf(dg * x0[k] + dd * p0[k])));
+          mx = group_reduce<LPC, OpMax>(mx);
+          const R vn = v0 + h * f.uv;
+          const R pvn = pv0 - h * f.dHv;
+          mx = nan_max(mx, nan_max(fabs(vn - vc), fabs(pvn - pvc)));
+          // the per-coordinate max drops NaN; a non-finite coefficient (or step
+          // start sum, through the field) makes the norm NaN instead
+          if (!isfinite(na + nb + ng + nd)) mx = R(NAN);
+          if (active) {
+            ca = na;
+            cb = nb;
+            cg = ng;
+            cd = nd;
+            vc = vn;
+            pvc = pvn;
+            sx = (ca * ca) * Sxx + (R(2) * ca * cb) * Sxp + (cb * cb) * Spp;
+            xpx = (ca * cg) * Sxx + (ca * cd + cb * cg) * Sxp + (cb * cd) * Spp;
+            norm = (double)mx;
+            ++n;
+            ++n_fev;
+          }
+          // cond_fn (integrators.py:67-74)
+          active = on && n < a.max_iters && isfinite(norm) && norm < a.div_tol && norm > a.conv_tol;
+          if (!__any(active)) break;
+        }
+#pragma unroll
+        for (int k = 0; k < EPL; ++k) {
+          xc[k] = ca * x0[k] + cb * p0[k];
+          pc[k] = cg * x0[k] + cd * p0[k];
+        }
+      } else while (true) {
         const FieldCoef<R> f = field_coef<R>(vc, pvc, sx, xpx, a2, inv_s2, halfDx);
         R s1 = R(0), s2 = R(0), mx = R(0);
 #pragma unroll
