@@ -167,20 +167,32 @@ __device__ __forceinline__ void gram_pass(Smem& S, const float* __restrict__ Xg,
       }
   };
   // Two-stage pipeline: the MFMAs of step k run while the operands of step
-  // k+1 are scaled and split on VALU; loads run two steps ahead.
-  Frag fa, fb;
+  // k+1 are scaled and split on VALU; loads run four steps ahead (a ring of
+  // four fragments: the sqrt(lambda) rows come from the per-chain scratch,
+  // whose latency two steps did not cover).  nj is a multiple of 4.
+  Frag f0, f1, f2, f3;
   bf16x8 ha[NB], la[NB], hb[NB], lb[NB];
-  fetch(fa, 0);
-  fetch(fb, 1);
-  split_frag(fa, ha, la);
-  for (int j = 0; j < nj; j += 2) {
-    fetch(fa, j + 2);
+  fetch(f0, 0);
+  fetch(f1, 1);
+  fetch(f2, 2);
+  fetch(f3, 3);
+  split_frag(f0, ha, la);
+  for (int j = 0; j < nj; j += 4) {
+    fetch(f0, j + 4);
     tiles(ha, la);
-    split_frag(fb, hb, lb);
+    split_frag(f1, hb, lb);
     interleave_mfma_valu<3 * NTL, 64 / (3 * NTL) + 1>();
-    fetch(fb, j + 3);
+    fetch(f1, j + 5);
     tiles(hb, lb);
-    split_frag(fa, ha, la);
+    split_frag(f2, ha, la);
+    interleave_mfma_valu<3 * NTL, 64 / (3 * NTL) + 1>();
+    fetch(f2, j + 6);
+    tiles(ha, la);
+    split_frag(f3, hb, lb);
+    interleave_mfma_valu<3 * NTL, 64 / (3 * NTL) + 1>();
+    fetch(f3, j + 7);
+    tiles(hb, lb);
+    split_frag(f0, ha, la);
     interleave_mfma_valu<3 * NTL, 64 / (3 * NTL) + 1>();
   }
   __syncthreads();
