@@ -75,10 +75,11 @@ def _problem_key(model, chains):
     return shape, int(chains)
 
 
-def logistic_waves_field(model, chains):
+def logistic_waves_field(model, chains, step_size=0.05, num_integration_steps=4, max_iters=6):
     """The C ABI waves_per_chain field for the tuned path on ``model`` with
-    ``chains`` chains: the cached choice, tuned on first use unless tuning is
-    disabled or the current stream is being captured (then the library
+    ``chains`` chains: the cached choice, tuned on first use (timed with the
+    first call's step size, number of steps and fixed-point cap) unless tuning
+    is disabled or the current stream is being captured (then the library
     default, 0)."""
     key = _problem_key(model, chains)
     with _lock:
@@ -86,19 +87,21 @@ def logistic_waves_field(model, chains):
             return _logistic[key]
     if not enabled or torch.cuda.is_current_stream_capturing():
         return 0
-    return tune_logistic(model, chains)[0]
+    return tune_logistic(model, chains, num_integration_steps=max(1, int(num_integration_steps)),
+                         max_iters=int(max_iters), step_size=float(step_size))[0]
 
 
 LOGISTIC_WAVES = (0x11, 0x22, 0x21, 0x12)  # bits 0-3: D <= 32, bits 4-7: 33..64 features
 
 
-def tune_logistic(model, chains, repeats=2, num_integration_steps=4, max_iters=6, step_size=0.05):
-    """Time one short persistent launch of the tuned RMHMC kernel per
-    waves_per_chain candidate on the caller's device (positions drawn near
-    the origin, the device PRNG; nothing of the caller's state is touched)
-    and cache the fastest for (problem, chains).  Candidates that differ only
-    in a width the problem does not have are not timed twice.  Returns
-    (field, {field: seconds})."""
+def tune_logistic(model, chains, repeats=3, num_integration_steps=4, max_iters=6, step_size=0.05, transitions=2):
+    """Time a persistent launch of ``transitions`` transitions of the tuned
+    RMHMC kernel per waves_per_chain candidate on the caller's device
+    (positions drawn near the origin, the device PRNG; nothing of the caller's
+    state is touched), the best of ``repeats`` after one untimed launch, and
+    cache the fastest for (problem, chains).  Candidates that differ only in a
+    width the problem does not have are not timed twice.  Returns (field,
+    {field: seconds})."""
     import functools
 
     from .random import PRNGKey
@@ -134,8 +137,8 @@ def tune_logistic(model, chains, repeats=2, num_integration_steps=4, max_iters=6
     integ = functools.partial(implicit_midpoint, max_iters=max_iters)
 
     def run(field):
-        sample(PRNGKey(7, 0), state, model, step_size, _metric(), num_integration_steps, 1, integrator=integ,
-               store_samples=False, waves_per_chain=(field & 0xF, field >> 4))
+        sample(PRNGKey(7, 0), state, model, step_size, _metric(), num_integration_steps, int(transitions),
+               integrator=integ, store_samples=False, waves_per_chain=(field & 0xF, field >> 4))
 
     timings = {}
     for f in cands:

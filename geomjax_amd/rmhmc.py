@@ -168,12 +168,13 @@ def _lanes(lanes_per_chain, kind, D, C, dtype):
     return lookup(kind, D, C, dtype) or 0
 
 
-def _waves(waves_per_chain, model, C, dtype):
+def _waves(waves_per_chain, model, C, dtype, step_size=0.05, L=4, max_iters=6):
     """The C ABI's waves_per_chain field for a logistic call: an explicit 1 or
     2 applies to both one-workgroup widths (up to 32 and up to 64 features),
     a pair (w32, w64) to each; 0 takes the autotuner's choice for this problem
-    and chain count (tuned on first use), or the library default.  float64
-    calls ignore it."""
+    and chain count (tuned on first use with the call's step size, number of
+    steps and fixed-point cap), or the library default.  float64 calls ignore
+    it."""
     if dtype != torch.float32:
         return 0
     if isinstance(waves_per_chain, tuple):
@@ -187,7 +188,8 @@ def _waves(waves_per_chain, model, C, dtype):
         return int(waves_per_chain) | (int(waves_per_chain) << 4)
     from .autotune import logistic_waves_field
 
-    return logistic_waves_field(model, C)
+    return logistic_waves_field(model, C, step_size=float(step_size), num_integration_steps=int(L),
+                                max_iters=int(max_iters))
 
 
 def build_kernel(integrator=implicit_midpoint, divergence_threshold: float = 1000, *,
@@ -276,7 +278,8 @@ def build_kernel(integrator=implicit_midpoint, divergence_threshold: float = 100
                                      divergence_tol=solver["divergence_tol"],
                                      divergence_threshold=divergence_threshold, seed=seed, counter=counter,
                                      dtype=position.dtype,
-                                     waves_per_chain=_waves(waves_per_chain, logdensity_fn, C, position.dtype))
+                                     waves_per_chain=_waves(waves_per_chain, logdensity_fn, C, position.dtype,
+                                                            step_size, L, solver["max_iters"]))
             data = logdensity_fn.data(dtype=position.dtype, num_chains=C)
             rc = _lib.load().gjx_rmhmc_fisher_logistic_step(params, data, io, _lib.stream_handle(dev))
             _lib.check(rc, "gjx_rmhmc_fisher_logistic_step")
@@ -380,7 +383,8 @@ def sample(rng_key, state, logdensity_fn, step_size, metric_fn, num_integration_
                                  max_iters=solver["max_iters"], convergence_tol=solver["convergence_tol"],
                                  divergence_tol=solver["divergence_tol"], divergence_threshold=divergence_threshold,
                                  seed=seed, counter=counter, dtype=position.dtype,
-                                 waves_per_chain=_waves(waves_per_chain, logdensity_fn, C, position.dtype))
+                                 waves_per_chain=_waves(waves_per_chain, logdensity_fn, C, position.dtype,
+                                                            step_size, L, solver["max_iters"]))
         data = logdensity_fn.data(dtype=position.dtype, num_chains=C)
         rc = _lib.load().gjx_rmhmc_fisher_logistic_sample(params, data, io, _lib.stream_handle(dev))
         _lib.check(rc, "gjx_rmhmc_fisher_logistic_sample")
