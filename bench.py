@@ -217,6 +217,19 @@ def paired_ess_window(model, warmed, key, integ, window, plain, ess_fn):
     return out
 
 
+# waves per chain the library uses for each width class when the field leaves it at 0
+# (kDefaultWavesPerChain of the C library: two for D <= 32, one for 33..64)
+DEFAULT_WAVES = {32: 2, 64: 1}
+
+
+def waves_for_width(field, D):
+    """Waves per chain of the tuned kernel for a posterior of D <= 64 features
+    under the C ABI's waves_per_chain field (4 bits per width class)."""
+    cls = 32 if D <= 32 else 64
+    w = (field >> (0 if cls == 32 else 4)) & 0xF
+    return w or DEFAULT_WAVES[cls]
+
+
 def logistic_forms(chains, K, W, dev, plain=True):
     """Tuned and plain RMHMC-Fisher on the 2048 x 64 logistic posterior with `chains` chains."""
     import torch
@@ -261,7 +274,7 @@ def logistic_forms(chains, K, W, dev, plain=True):
              "fevals_per_chain_transition": fev_total / chains / K, "dtype": "f32 state, bf16x3 MFMA, f64 energies",
              "launches": n_launch, "avg_launch_ms": kernel_s / n_launch * 1e3,
              "mfma_flops_per_launch": mfma_flops_logistic(fev_total) / n_launch, "steps": K,
-             "waves_per_chain": waves & 0xF or waves >> 4}
+             "waves_per_chain": waves_for_width(waves, D_LOG)}
     log(f"logistic tuned, {chains} chains: {tuned['value']:.3e} leapfrog-steps/s, {tuned['ms_per_step']:.2f} ms/step, "
         f"acc {tuned['acceptance_rate']:.3f}, fevals/chain/tr {tuned['fevals_per_chain_transition']:.1f}, "
         f"waves per chain {tuned['waves_per_chain']} (autotuned)")
@@ -341,8 +354,8 @@ def uneven_batch_forms(chains_per_posterior, K, W, dev, plain=True):
     waves = autotune.logistic_waves_field(batch, C)
     tuned = {"value": C * L_LOG * K / wall, "unit": "leapfrog-steps/s", "ms_per_step": wall / K * 1e3, "steps": K,
              "fevals_per_chain_transition": float(torch.stack(fevs).double().sum().item()) / C / K,
-             "waves_per_chain": {"D<=32": waves & 0xF, "33<=D<=64": waves >> 4, "65<=D<=128": 4,
-                                 "chosen_by": "autotune.tune_logistic (0 = library default)"}}
+             "waves_per_chain": {"D<=32": waves_for_width(waves, 32), "33<=D<=64": waves_for_width(waves, 64),
+                                 "65<=D<=128": 4, "chosen_by": "autotune.tune_logistic"}}
     log(f"uneven batch tuned: {tuned['value']:.3e} leapfrog-steps/s, {tuned['ms_per_step']:.2f} ms/step, "
         f"waves per chain {tuned['waves_per_chain']}")
     win = paired_ess_window(batch, warmed, key, integ, ESS_WINDOW_BATCH, plain,

@@ -91,7 +91,9 @@ def logistic_waves_field(model, chains, step_size=0.05, num_integration_steps=4,
                          max_iters=int(max_iters), step_size=float(step_size))[0]
 
 
-LOGISTIC_WAVES = (0x11, 0x22, 0x21, 0x12)  # bits 0-3: D <= 32, bits 4-7: 33..64 features
+LOGISTIC_WAVES = (0x12, 0x11, 0x22, 0x21)  # bits 0-3: D <= 32, bits 4-7: 33..64 features
+LOGISTIC_DEFAULT = 0x12  # the library's default shape (kDefaultWavesPerChain)
+LOGISTIC_MARGIN = 0.98   # another shape replaces the default only when at least 2 % faster
 
 
 def tune_logistic(model, chains, repeats=3, num_integration_steps=4, max_iters=6, step_size=0.05, transitions=2):
@@ -99,9 +101,11 @@ def tune_logistic(model, chains, repeats=3, num_integration_steps=4, max_iters=6
     RMHMC kernel per waves_per_chain candidate on the caller's device
     (positions drawn near the origin, the device PRNG; nothing of the caller's
     state is touched), the best of ``repeats`` after one untimed launch, and
-    cache the fastest for (problem, chains).  Candidates that differ only in a
-    width the problem does not have are not timed twice.  Returns (field,
-    {field: seconds})."""
+    cache the fastest for (problem, chains); the library default stays unless
+    another shape is faster by more than the timing noise (LOGISTIC_MARGIN),
+    so the choice does not flip between runs on near-ties.  Candidates that
+    differ only in a width the problem does not have are not timed twice.
+    Returns (field, {field: seconds})."""
     import functools
 
     from .random import PRNGKey
@@ -152,6 +156,8 @@ def tune_logistic(model, chains, repeats=3, num_integration_steps=4, max_iters=6
             best = min(best, time.perf_counter() - t0)
         timings[f] = best
     best_f = min(timings, key=timings.get)
+    if LOGISTIC_DEFAULT in timings and timings[best_f] > LOGISTIC_MARGIN * timings[LOGISTIC_DEFAULT]:
+        best_f = LOGISTIC_DEFAULT
     with _lock:
         _logistic.setdefault(key, best_f)
     return best_f, timings

@@ -152,8 +152,9 @@ hipError_t launch_logistic_rmhmc_f32(const LogisticArgs& a, int max_features, hi
                                      bool concurrent = true);
 hipError_t launch_logistic_rmhmc_f32_wide(const LogisticArgs& a, hipStream_t stream);
 // waves per chain of the tuned kernels for posteriors of up to 32 / 64
-// features when the caller leaves the choice to the library
-constexpr int kDefaultWavesPerChain[2] = {2, 2};
+// features when the caller leaves the choice to the library (the faster shape
+// of each class on the 2048 x 64 and uneven-batch workloads)
+constexpr int kDefaultWavesPerChain[2] = {2, 1};
 hipError_t launch_logistic_pack(const double* X, const double* y, const int32_t* N, const int32_t* D,
                                 const int64_t* xoff, const int64_t* yoff, const PackedPost* hdr, float* packed,
                                 int P, int max_npad, hipStream_t stream);
