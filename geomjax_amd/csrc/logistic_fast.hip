@@ -63,13 +63,26 @@ using namespace lf32;
 constexpr int MAXD = 64;
 constexpr int LDG = MAXD + 4;  // LDS row stride of G^-1 (floats; rows stay 16-byte aligned)
 
+// LDS of one chain.  With one wave per chain G^-1 overwrites G once the
+// refinement step has read G (the field pass needs only G^-1 and the next Gram
+// pass rewrites G), which keeps the workgroup at 20 KiB: eight one-wave
+// workgroups per CU, two chains per SIMD.  With two waves per chain G^-1 has
+// its own buffer (the field pass stages its split operand over G).
+template <int NW>
 struct Smem {
-  float G[MAXD * MAXD];  // Gram G (row stride 64), kept for the refinement step
-  float Gi[MAXD * LDG];  // G^-1; scratch of the partial-sum exchanges between lanes and waves
-  float bc[2][64];       // sweep: column broadcast [pivot parity][row]
-  float dk[MAXD];        // pivots
+  float G[MAXD * LDG];  // Gram G (row stride 64), kept for the refinement step; scratch of the
+                        // partial-sum exchanges of a row pass (no Gram is live then)
+  float Gi_[NW == 1 ? 4 : MAXD * LDG];
+  float bc[2][64];  // sweep: column broadcast [pivot parity][row]
+  float dk[MAXD];   // pivots
   float q0[MAXD], p0[MAXD], q[MAXD], p[MAXD], u[MAXD], dH[MAXD], z[MAXD], g[MAXD];
-  double red[2];         // per-wave partial sums of the two-wave form
+  double red[2];    // per-wave partial sums of the two-wave form
+  __device__ __forceinline__ float* gi() {  // G^-1 (row stride LDG)
+    if constexpr (NW == 1)
+      return G;
+    else
+      return Gi_;
+  }
 };
 
 // With NW waves per chain the rows of every pass over X are dealt to the
@@ -96,7 +109,7 @@ __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane
 
 // Sum of one value per wave, identical in every wave (LDS scratch S.red).
 template <int NW>
-__device__ __forceinline__ double waves_sum(Smem& S, double v) {
+__device__ __forceinline__ double waves_sum(Smem<NW>& S, double v) {
   if constexpr (NW == 1) {
     return v;
   } else {
@@ -124,7 +137,7 @@ __device__ __forceinline__ int own_groups(int npad, int w) { return (npad / 64 -
 // features, identity on the padding, from sqrt(lambda) in `sl`.
 // ---------------------------------------------------------------------------
 template <int NB, int NW>
-__device__ __forceinline__ void gram_pass(Smem& S, const float* __restrict__ Xg, const float* __restrict__ sl, int npad, int D,
+__device__ __forceinline__ void gram_pass(Smem<NW>& S, const float* __restrict__ Xg, const float* __restrict__ sl, int npad, int D,
                           float inv_prior) {
   constexpr int NTL = NB * (NB + 1) / 2;
   const int l = threadIdx.x % 64, w = wave_id();
@@ -197,8 +210,9 @@ __device__ __forceinline__ void gram_pass(Smem& S, const float* __restrict__ Xg,
   }
   __syncthreads();
   if constexpr (NW == 2) {
-    // wave 1 hands its partial tiles to wave 0 through S.Gi (unused during the pass)
-    float* P = S.Gi;
+    // wave 1 hands its partial tiles to wave 0 through G^-1's buffer (a
+    // two-wave workgroup has its own, not needed again before the next sweep)
+    float* P = S.gi();
     if (w == 1) {
 #pragma unroll
       for (int t = 0; t < NTL; ++t)
@@ -269,7 +283,7 @@ __device__ __forceinline__ void col_set(f32x32 (&A)[NB], int k, float v) {
 }
 
 template <int NB, int NW>
-__device__ __forceinline__ void sweep(Smem& S, f32x32 (&A)[NB], int D, bool want_chol, float& p0j) {
+__device__ __forceinline__ void sweep(Smem<NW>& S, f32x32 (&A)[NB], int D, bool want_chol, float& p0j) {
   constexpr int DP = 32 * NB;
   const int jl = threadIdx.x % DP;
 #pragma unroll
@@ -311,7 +325,8 @@ __device__ __forceinline__ void sweep(Smem& S, f32x32 (&A)[NB], int D, bool want
 }
 
 // log det G = sum log d_k in float64 (padding pivots are 1)
-__device__ __forceinline__ double logdet_from_pivots(const Smem& S, int D) {
+template <class SM>
+__device__ __forceinline__ double logdet_from_pivots(const SM& S, int D) {
   const int l = threadIdx.x % 64;
   double v = l < D ? log((double)S.dk[l]) : 0.0;
   return group_sum<64>(v);
@@ -319,7 +334,7 @@ __device__ __forceinline__ double logdet_from_pivots(const Smem& S, int D) {
 
 // S.u = G^-1 S.p with one refinement step against the Gram still in S.G.
 template <int NB, int NW>
-__device__ __forceinline__ void ginv_apply(Smem& S, const f32x32 (&A)[NB]) {
+__device__ __forceinline__ void ginv_apply(Smem<NW>& S, const f32x32 (&A)[NB]) {
   constexpr int DP = 32 * NB;
   const int l = threadIdx.x, jl = l % DP;
   float uj = 0.0f;
@@ -344,24 +359,24 @@ __device__ __forceinline__ void ginv_apply(Smem& S, const f32x32 (&A)[NB]) {
   wave0_sync<NW>();
 }
 
-// S.Gi = G^-1
+// G^-1 into its buffer (over G with one wave per chain: call after the last read of G)
 template <int NB, int NW>
-__device__ __forceinline__ void store_ginv(Smem& S, const f32x32 (&A)[NB]) {
+__device__ __forceinline__ void store_ginv(Smem<NW>& S, const f32x32 (&A)[NB]) {
   const int l = threadIdx.x;
   if (l < 32 * NB) {
 #pragma unroll
     for (int b = 0; b < NB; ++b)
 #pragma unroll
-      for (int i = 0; i < 32; ++i) S.Gi[(32 * b + i) * LDG + l] = -A[b][i];
+      for (int i = 0; i < 32; ++i) S.gi()[(32 * b + i) * LDG + l] = -A[b][i];
   }
   wave0_sync<NW>();
 }
 
 // G (in S.G) -> pivots in S.dk, S.u = G^-1 S.p (refined), and with
-// keep_inverse S.Gi = G^-1.  With want_chol the momentum p0 = chol(G) S.z is
+// keep_inverse G^-1 in its buffer.  With want_chol the momentum p0 = chol(G) S.z is
 // drawn first into S.p0 and S.p.
 template <int NB, int NW>
-__device__ __forceinline__ void invert(Smem& S, int D, bool want_chol, bool keep_inverse) {
+__device__ __forceinline__ void invert(Smem<NW>& S, int D, bool want_chol, bool keep_inverse) {
   f32x32 A[NB];
   float p0j;
   sweep<NB, NW>(S, A, D, want_chol, p0j);
@@ -379,11 +394,11 @@ __device__ __forceinline__ void invert(Smem& S, int D, bool want_chol, bool keep
 // Sum the per-lane feature partials of a row pass over the 32 lanes of each
 // half wave: lane f < DP receives the total for feature f.
 template <int NB, int NW>
-__device__ __forceinline__ float reduce_features(Smem& S, const float (&acc)[2 * NB][8]) {
+__device__ __forceinline__ float reduce_features(Smem<NW>& S, const float (&acc)[2 * NB][8]) {
   constexpr int KC = 2 * NB, RS = KC * 8 + 1;
-  static_assert(NW * 64 * RS <= MAXD * LDG, "the partials must fit S.Gi");
+  static_assert(NW * 64 * RS <= MAXD * LDG, "the partials must fit S.G");
   const int t = threadIdx.x, l = t % 64;
-  float* R = S.Gi;
+  float* R = S.G;  // no Gram is live during a row pass, and G^-1 is no longer read
   __syncthreads();
 #pragma unroll
   for (int kc = 0; kc < KC; ++kc)
@@ -403,176 +418,44 @@ __device__ __forceinline__ float reduce_features(Smem& S, const float (&acc)[2 *
 
 // ---------------------------------------------------------------------------
 // One evaluation of the vector field at (S.q, S.p) with S.u = G^-1 p and
-// G^-1 in S.Gi: S.dH = dH/dq.  Also advances the scratch: E1 <- eta of the
+// G^-1 in S.gi(): S.dH = dH/dq.  Also advances the scratch: E1 <- eta of the
 // next iterate (base + h X u, base = E0 for a fixed-point iterate, E1 for
 // the explicit half step, which also sets E0), SL <- sqrt(lambda) there.
 // ---------------------------------------------------------------------------
-template <int NB>
-__device__ __forceinline__ void field_pass_w1(Smem& S, const float* __restrict__ Xr, const float* __restrict__ Xs,
-                                           const float* __restrict__ yf, float* E0,
-                           float* E1, float* SL, int npad, int D, float hs, float inv_prior, bool explicit_step) {
-  constexpr int KC = 2 * NB;
-  const int l = threadIdx.x, h = l / 32;
-  bf16x8 Ah[NB][KC], Al[NB][KC];
-#pragma unroll
-  for (int t = 0; t < NB; ++t) {
-    const int row = phi(t * 32 + (l % 32));
-#pragma unroll
-    for (int kc = 0; kc < KC; ++kc) {
-      float v[8];
-      load8(S.Gi + row * LDG + kc * 16 + h * 8, v);
-      split8(v, Ah[t][kc], Al[t][kc]);
-    }
-  }
-  float dacc[KC][8];
-#pragma unroll
-  for (int kc = 0; kc < KC; ++kc)
-#pragma unroll
-    for (int i = 0; i < 8; ++i) dacc[kc][i] = 0.0f;
-  // uniform bases with 32-bit lane offsets: the loads address base + offset
-  // instead of holding a 64-bit pointer per fragment in registers
-  const uint32_t xoff = (uint32_t)l * 8u;
-  const uint4* const sbase = reinterpret_cast<const uint4*>(Xs);
-  const uint32_t soff = (uint32_t)l * 2u;
-  const int nrb = npad / 32;  // even: npad is a multiple of 64
-  // One 32-row block: its MFMA operand (X split into bf16 hi/lo), and the
-  // float32 X fragment, eta, base and label of the lane's row with the block's
-  // b = X u and V^T accumulators.
-  struct XF {
-    bf16x8 xh[KC], xl[KC];
-  };
-  struct Blk {
-    float x[KC][8];
-    float eta, base, yv, b;
-    f32x16 acc[NB];
-  };
-  // the pipeline's tail re-reads the last block (in bounds; its results are unused)
-  auto clamp_rb = [&](int rb) { return rb < nrb ? rb : nrb - 1; };
-  // The block index is made opaque (readfirstlane) so that the block's address
-  // is formed as a uniform base plus the lane's 32-bit offset, not as one
-  // 64-bit per-lane pointer per fragment stream carried across iterations.
-  auto fetch_xf = [&](XF& f, int rb) {
-    rb = __builtin_amdgcn_readfirstlane(clamp_rb(rb));
-    const uint4* const blk = sbase + (size_t)rb * (KC * 128);
-#pragma unroll
-    for (int kc = 0; kc < KC; ++kc) {
-      const uint32_t i = (uint32_t)kc * 128u + soff;
-      f.xh[kc] = __builtin_bit_cast(bf16x8, blk[i]);
-      f.xl[kc] = __builtin_bit_cast(bf16x8, blk[i + 1u]);
-    }
-  };
-  auto fetch_blk = [&](Blk& B, int rb) {
-    rb = __builtin_amdgcn_readfirstlane(clamp_rb(rb));
-    const float* const blk = Xr + (size_t)rb * (KC * 512);
-#pragma unroll
-    for (int kc = 0; kc < KC; ++kc) load8(blk + ((uint32_t)kc * 512u + xoff), B.x[kc]);
-    const uint32_t lr = (uint32_t)(l % 32);
-    B.eta = (E1 + (size_t)rb * 32)[lr];
-    B.base = explicit_step ? 0.0f : (E0 + (size_t)rb * 32)[lr];
-    B.yv = (yf + (size_t)rb * 32)[lr];
-  };
-  auto mfma_blk = [&](const XF& f, Blk& B) {
-#pragma unroll
-    for (int t = 0; t < NB; ++t) {
-      B.acc[t] = f32x16{0};
-#pragma unroll
-      for (int kc = 0; kc < KC; ++kc) B.acc[t] = mfma3(Ah[t][kc], Al[t][kc], f.xh[kc], f.xl[kc], B.acc[t]);
-    }
-  };
-  // b = X u; u is read from LDS per block (holding it in registers for the
-  // whole pass would cost 32 of them at the pass's peak)
-  auto bdot = [&](Blk& B) {
-    float bp = 0.0f;
-#pragma unroll
-    for (int kc = 0; kc < KC; ++kc) {
-      float u8[8];
-      load8(S.u + kc * 16 + h * 8, u8);
-#pragma unroll
-      for (int i = 0; i < 8; ++i) bp = fmaf(B.x[kc][i], u8[i], bp);
-    }
-    B.b = bp + __shfl_xor(bp, 32, 64);
-  };
-  float* const e0_dst = explicit_step ? E0 : E1;
-  // a_n, h_n, X^T h and the next iterate's eta / sqrt(lambda) of block rb;
-  // both lanes of a row store the same values
-  auto post = [&](const Blk& B, int rb) {
-    const int row = rb * 32 + (l % 32);
-    float ap = 0.0f;
-#pragma unroll
-    for (int t = 0; t < NB; ++t)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int j = 16 * t + r;
-        ap = fmaf(B.acc[t][r], B.x[j / 8][j % 8], ap);
-      }
-    const float an = ap + __shfl_xor(ap, 32, 64);
-    const float s = sigmoidf(B.eta);
-    const float w = s * (1.0f - s) * (1.0f - 2.0f * s);
-    const float hv = 0.5f * w * (an - B.b * B.b) - (B.yv - s);
-#pragma unroll
-    for (int kc = 0; kc < KC; ++kc)
-#pragma unroll
-      for (int i = 0; i < 8; ++i) dacc[kc][i] = fmaf(hv, B.x[kc][i], dacc[kc][i]);
-    const float en = fmaf(hs, B.b, explicit_step ? B.eta : B.base);
-    const float sn = sigmoidf(en);
-    E1[row] = en;
-    e0_dst[row] = en;  // the explicit step moves the step start; iterates rewrite E1 (no E0 traffic)
-    SL[row] = sqrt_fast(sn * (1.0f - sn));
-  };
-  // Two-stage pipeline: the MFMAs of block r+1 run while block r is finished
-  // on VALU; operand loads run one to two blocks ahead.
-  constexpr int NMF = 3 * NB * KC;  // MFMAs per block
-  XF f0, f1;
-  Blk b0, b1;
-  fetch_xf(f0, 0);
-  fetch_blk(b0, 0);
-  fetch_xf(f1, 1);
-  fetch_blk(b1, 1);
-  mfma_blk(f0, b0);
-  bdot(b0);
-  fetch_xf(f0, 2);
-  for (int rb = 0; rb < nrb; rb += 2) {
-    asm volatile("" ::: "memory");  // keep the per-block LDS reads of u inside the loop
-    mfma_blk(f1, b1);  // block rb + 1
-    post(b0, rb);
-    bdot(b1);
-    interleave_mfma_valu<NMF, 160 / NMF + 1>();
-    fetch_blk(b0, rb + 2);
-    fetch_xf(f1, rb + 3);
-    mfma_blk(f0, b0);  // block rb + 2 (past the end: unused)
-    post(b1, rb + 1);
-    bdot(b0);
-    interleave_mfma_valu<NMF, 160 / NMF + 1>();
-    fetch_blk(b1, rb + 3);
-    fetch_xf(f0, rb + 4);
-  }
-  const float tot = reduce_features<NB, 1>(S, dacc);
-  if (l < MAXD) S.dH[l] = l < D ? tot + S.q[l] * inv_prior : 0.0f;
-  __syncthreads();
-}
-
-// Two-wave form of the same pass.  The A operand (rows of G^-1 permuted by
-// phi, split) is staged once in LDS over S.G, which is not read again before
-// the next Gram pass rewrites it, and each wave runs its own blocks one at a
-// time: the other wave resident on the SIMD covers the MFMA latency, and the
-// registers of a second block in flight are not needed.
-template <int NB>
-__device__ __forceinline__ void field_pass_w2(Smem& S, const float* __restrict__ Xr, const float* __restrict__ Xs,
-                                              const float* __restrict__ yf, float* E0, float* E1, float* SL, int npad,
-                                              int D, float hs, float inv_prior, bool explicit_step) {
-  constexpr int KC = 2 * NB, NTK = NB * KC;
+//
+// The A operand (rows of G^-1 permuted by phi, split) is staged once in LDS
+// over S.G, which is not read again before the next Gram pass rewrites it
+// (with one wave per chain G^-1 itself lives there: every wave reads its share
+// before any is written), and each wave runs its own blocks one at a time: the
+// other wave resident on the SIMD covers the MFMA latency, and the registers of
+// a second block in flight are not needed.
+template <int NB, int NW>
+__device__ __forceinline__ void field_pass(Smem<NW>& S, const float* __restrict__ Xr, const float* __restrict__ Xs,
+                                           const float* __restrict__ yf, float* E0, float* E1, float* SL, int npad,
+                                           int D, float hs, float inv_prior, bool explicit_step) {
+  constexpr int KC = 2 * NB, NTK = NB * KC, PER = (NTK + NW - 1) / NW;
   const int t = threadIdx.x, l = t % 64, w = wave_id(), h = l / 32;
   bf16x8* const AH = reinterpret_cast<bf16x8*>(S.G);  // [NTK][64 lanes]
   bf16x8* const AL = AH + NTK * 64;
   static_assert(2 * NTK * 64 * sizeof(bf16x8) <= sizeof(S.G), "the split G^-1 operand must fit S.G");
-  for (int tk = w; tk < NTK; tk += 2) {
-    const int tt = tk / KC, kc = tk % KC;
-    float v[8];
-    load8(S.Gi + phi(tt * 32 + (l % 32)) * LDG + kc * 16 + h * 8, v);
-    bf16x8 hi, lo;
-    split8(v, hi, lo);
-    AH[tk * 64 + l] = hi;
-    AL[tk * 64 + l] = lo;
+  {
+    float v[PER][8];
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const int tk = w + NW * j;
+      if (tk < NTK) load8(S.gi() + phi((tk / KC) * 32 + (l % 32)) * LDG + (tk % KC) * 16 + h * 8, v[j]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const int tk = w + NW * j;
+      if (tk < NTK) {
+        bf16x8 hi, lo;
+        split8(v[j], hi, lo);
+        AH[tk * 64 + l] = hi;
+        AL[tk * 64 + l] = lo;
+      }
+    }
   }
   __syncthreads();
   float dacc[KC][8];
@@ -583,11 +466,11 @@ __device__ __forceinline__ void field_pass_w2(Smem& S, const float* __restrict__
   const uint32_t xoff = (uint32_t)l * 8u;
   const uint4* const sbase = reinterpret_cast<const uint4*>(Xs);
   const uint32_t soff = (uint32_t)l * 2u;
-  const int nj = 2 * own_groups<2>(npad, w);  // this wave's 32-row blocks
+  const int nj = 2 * own_groups<NW>(npad, w);  // this wave's 32-row blocks
   // block of step j; past the end the wave's last block again (read only, unused)
   auto blk_of = [&](int j) {
     const int jj = j < nj ? j : nj - 1;
-    return __builtin_amdgcn_readfirstlane(jj >= 0 ? own_block<2>(jj, w) : 0);
+    return __builtin_amdgcn_readfirstlane(jj >= 0 ? own_block<NW>(jj, w) : 0);
   };
   struct XF {
     bf16x8 xh[KC], xl[KC];
@@ -677,26 +560,16 @@ __device__ __forceinline__ void field_pass_w2(Smem& S, const float* __restrict__
     post(B, rb);
     fetch_blk(B, blk_of(j + 1));
   }
-  const float tot = reduce_features<NB, 2>(S, dacc);
+  const float tot = reduce_features<NB, NW>(S, dacc);
   if (t < MAXD) S.dH[l] = l < D ? tot + S.q[l] * inv_prior : 0.0f;
   __syncthreads();
-}
-
-template <int NB, int NW>
-__device__ __forceinline__ void field_pass(Smem& S, const float* __restrict__ Xr, const float* __restrict__ Xs,
-                                           const float* __restrict__ yf, float* E0, float* E1, float* SL, int npad,
-                                           int D, float hs, float inv_prior, bool explicit_step) {
-  if constexpr (NW == 1)
-    field_pass_w1<NB>(S, Xr, Xs, yf, E0, E1, SL, npad, D, hs, inv_prior, explicit_step);
-  else
-    field_pass_w2<NB>(S, Xr, Xs, yf, E0, E1, SL, npad, D, hs, inv_prior, explicit_step);
 }
 
 // eta = X q (float64 accumulation) into Eout, sqrt(lambda) into SL; with
 // want_grad also the log likelihood (float64, returned) and
 // S.g = X^T (y - s) - q / prior_var.
 template <int NB, int NW>
-__device__ __forceinline__ double fresh_pass(Smem& S, const float* __restrict__ Xr, const float* __restrict__ yf, int N, int npad,
+__device__ __forceinline__ double fresh_pass(Smem<NW>& S, const float* __restrict__ Xr, const float* __restrict__ yf, int N, int npad,
                              int D, float* Eout, float* SL, bool want_grad, float inv_prior) {
   constexpr int KC = 2 * NB;
   const int t = threadIdx.x, l = t % 64, w = wave_id(), h = l / 32;
@@ -770,7 +643,7 @@ enum Stage { kInit, kNoise, kTraj, kEnd };
 // evaluations with a single site for each pass (Gram, sweep, field, fresh),
 // which keeps one copy of each unrolled pass in the kernel.
 template <int NB, int NW, bool STEP>
-__device__ __forceinline__ void run_chain(const LogisticArgs& a, Smem& S, int c, int post) {
+__device__ __forceinline__ void run_chain(const LogisticArgs& a, Smem<NW>& S, int c, int post) {
   const int l = threadIdx.x % 64, w = wave_id();  // lane, wave
   const PackedPost pp = a.packed_hdr[post];
   const int N = a.post_N[post], D = a.post_D[post];
@@ -863,7 +736,7 @@ __device__ __forceinline__ void run_chain(const LogisticArgs& a, Smem& S, int c,
       const double logdet = logdet_from_pivots(S, D);
       const double pu = group_sum<64>(l < D ? (double)S.p[l] * (double)S.u[l] : 0.0);
       H0 = -logp + 0.5 * pu + 0.5 * logdet + ke_const;
-      have_ginv = true;  // S.Gi = G(S.q)^-1 and S.u = G^-1 S.p
+      have_ginv = true;  // S.gi() = G(S.q)^-1 and S.u = G^-1 S.p
       s = 0;
       mode = 0;
       Lcur = chain_steps(a.Lc, a.L, a.C, STEP ? 0 : tr, c);
@@ -994,9 +867,9 @@ __device__ __forceinline__ void run_chain(const LogisticArgs& a, Smem& S, int c,
 // a contiguous range of chains -- for a multi-posterior batch, a few
 // posteriors, whose packed X then stays in that XCD's L2.
 template <int NB, int NW, bool STEP>
-__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW)))
+__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(2)))
 logistic_rmhmc_f32_kernel(const LogisticArgs a) {
-  __shared__ Smem S;
+  __shared__ Smem<NW> S;
   const int per_xcd = gridDim.x / 8;
   const int slot = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
   if (slot >= a.C) return;
