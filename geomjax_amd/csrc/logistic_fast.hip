@@ -498,15 +498,26 @@ __device__ __forceinline__ void field_pass(Smem<NW>& S, const float* __restrict_
     B.base = explicit_step ? 0.0f : (E0 + (size_t)rb * 32)[lr];
     B.yv = (yf + (size_t)rb * 32)[lr];
   };
+  // The split G^-1 fragments of step tk + 1 are read from LDS while the three
+  // MFMAs of step tk run (one step ahead: two fragments in flight).
   auto mfma_blk = [&](const XF& f, Blk& B) {
+    bf16x8 ah = AH[l], al = AL[l];
 #pragma unroll
-    for (int tt = 0; tt < NB; ++tt) {
-      B.acc[tt] = f32x16{0};
-#pragma unroll
-      for (int kc = 0; kc < KC; ++kc) {
-        const int tk = tt * KC + kc;
-        B.acc[tt] = mfma3(AH[tk * 64 + l], AL[tk * 64 + l], f.xh[kc], f.xl[kc], B.acc[tt]);
+    for (int tk = 0; tk < NTK; ++tk) {
+      const int tt = tk / KC, kc = tk % KC;
+      bf16x8 nh = ah, nl = al;
+      if (tk + 1 < NTK) {
+        nh = AH[(tk + 1) * 64 + l];
+        nl = AL[(tk + 1) * 64 + l];
       }
+      if (kc == 0) B.acc[tt] = f32x16{0};
+      B.acc[tt] = mfma3(ah, al, f.xh[kc], f.xl[kc], B.acc[tt]);
+      if (tk + 1 < NTK) {
+        __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);  // the next step's two LDS reads
+        __builtin_amdgcn_sched_group_barrier(0x008, 3, 0);  // then this step's three MFMAs
+      }
+      ah = nh;
+      al = nl;
     }
   };
   auto bdot = [&](Blk& B) {
