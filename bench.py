@@ -916,6 +916,18 @@ def tangent_forms(dev, repeats=3):
                      "tangent_over_primal": t_tan / t_pri, "num_integration_steps": 32,
                      "mean_objective": float(r.objective.mean()), "mean_d_objective": r.d_objective.mean(0).tolist()}
         log(f"tangent {name}: {t_tan * 1e3:.2f} ms vs primal {t_pri * 1e3:.2f} ms ({t_tan / t_pri:.2f}x)")
+        if isinstance(model, gj.NealFunnel):  # the float32 fast path on the same positions
+            s32 = gj.rmhmc.init(q.to(dev, torch.float32), model)
+            esjd_acceptance(gr.PRNGKey(1, 0), s32, model, eps, metric, 32, integrator=integ)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for i in range(repeats):
+                r32 = esjd_acceptance(gr.PRNGKey(2, i), s32, model, eps, metric, 32, integrator=integ)
+            torch.cuda.synchronize()
+            t32 = (time.perf_counter() - t0) / repeats
+            out[name]["tangent_f32_fast_path_ms"] = t32 * 1e3
+            out[name]["mean_d_objective_f32"] = r32.d_objective.mean(0).tolist()
+            log(f"tangent {name} float32 fast path: {t32 * 1e3:.2f} ms")
     return out
 
 

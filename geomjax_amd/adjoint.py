@@ -47,22 +47,26 @@ def esjd_acceptance(rng_key, state, logdensity_fn, step_size, metric_fn, num_int
                     integrator=implicit_midpoint, lanes_per_chain=0) -> TangentResult:
     """Per-chain objective and its derivatives w.r.t. (log step_size, log alpha2).
 
-    Funnel + MongeMetric: both columns.  LogisticRegression / LogisticBatch +
-    FisherMetric: column 0 (column 1 is zero)."""
+    Funnel + MongeMetric: both columns; a float64 state runs the check path, a
+    float32 state the fast path (float32 trajectory and tangents, float64
+    energies, as the tuned sampler).  LogisticRegression / LogisticBatch +
+    FisherMetric: column 0 (column 1 is zero), float64."""
     funnel = isinstance(logdensity_fn, NealFunnel) and isinstance(metric_fn, MongeMetric)
     logistic = isinstance(logdensity_fn, (LogisticRegression, LogisticBatch)) and isinstance(metric_fn, FisherMetric)
     if not (funnel or logistic):
         raise NotImplementedError("tangent kernels: RMHMC-Monge on the funnel, RMHMC-Fisher on logistic regression")
     solver = _solver_kwargs(integrator)
     q = state.position
-    if q.dim() != 2 or q.dtype != torch.float64 or not q.is_cuda:
-        raise ValueError("position must be a [C, D] float64 tensor on a HIP device")
+    if q.dim() != 2 or q.dtype not in (torch.float64, torch.float32) or not q.is_cuda:
+        raise ValueError("position must be a [C, D] float64 or float32 tensor on a HIP device")
+    if logistic and q.dtype != torch.float64:
+        raise ValueError("the logistic tangent runs in float64 (pass a float64 state)")
     q = q.contiguous()
     C, D = q.shape
     dev = q.device
     logp = state.logdensity.reshape(C).to(torch.float64).contiguous()
     if isinstance(rng_key, HostNoise):
-        z = rng_key.z.reshape(C, D).to(device=dev, dtype=torch.float64).contiguous()
+        z = rng_key.z.reshape(C, D).to(device=dev, dtype=q.dtype).contiguous()
         seed, counter = 0, 0
     elif isinstance(rng_key, PRNGKey):
         z, seed, counter = None, rng_key.seed, rng_key.counter
@@ -76,7 +80,8 @@ def esjd_acceptance(rng_key, state, logdensity_fn, step_size, metric_fn, num_int
                         d_acceptance=out.d_acceptance.data_ptr(), d_sq_jump=out.d_sq_jump.data_ptr(),
                         num_fevals=out.num_fevals.data_ptr())
     if funnel:
-        params = _params(C, D, _lib.GJX_F64, step_size, num_integration_steps, solver, metric_fn.alpha2,
+        code = _lib.GJX_F64 if q.dtype == torch.float64 else _lib.GJX_F32
+        params = _params(C, D, code, step_size, num_integration_steps, solver, metric_fn.alpha2,
                          logdensity_fn.sigma, 1000.0, lanes_per_chain=lanes_per_chain, seed=seed, counter=counter)
         rc = _lib.load().gjx_rmhmc_monge_funnel_tangent(params, io, _lib.stream_handle(dev))
         _lib.check(rc, "gjx_rmhmc_monge_funnel_tangent")

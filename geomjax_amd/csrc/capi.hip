@@ -176,7 +176,8 @@ int gjx_rmhmc_monge_funnel_sample(const gjx_funnel_params* params, const gjx_rmh
 
 int gjx_rmhmc_monge_funnel_tangent(const gjx_funnel_params* params, const gjx_tangent_io* io, void* stream) {
   GJX_REQUIRE(params != nullptr && io != nullptr, "params and io are required");
-  GJX_REQUIRE(params->dtype == GJX_F64, "the tangent kernel runs in float64");
+  GJX_REQUIRE(params->dtype == GJX_F64 || params->dtype == GJX_F32,
+              "dtype must be GJX_F64 (check path) or GJX_F32 (fast path)");
   GJX_REQUIRE(params->alpha2 > 0.0 && std::isfinite(params->alpha2), "alpha2 must be > 0 (log alpha2 is a parameter)");
   GJX_REQUIRE(params->step_size > 0.0 && std::isfinite(params->step_size),
               "step_size must be > 0 (log step_size is a parameter)");
@@ -199,7 +200,7 @@ int gjx_rmhmc_monge_funnel_tangent(const gjx_funnel_params* params, const gjx_ta
   a.z = io->z;
   gjx::FunnelGradOut o{io->objective, io->acceptance, io->sq_jump, io->d_objective, io->d_acceptance,
                        io->d_sq_jump, io->num_fevals};
-  hipError_t e = gjx::launch_funnel_rmhmc_tangent(a, o, lpc, epl, (hipStream_t)stream);
+  hipError_t e = gjx::launch_funnel_rmhmc_tangent(a, o, lpc, epl, p.dtype, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "gjx_rmhmc_monge_funnel_tangent launch");
   return GJX_OK;
 }

@@ -144,5 +144,11 @@ template <class T, int N> struct EnergyOf<Dual<T, N>> { using type = Dual<double
 __device__ __forceinline__ double widen(double x) { return x; }
 __device__ __forceinline__ double widen(float x) { return (double)x; }
 template <int N> __device__ __forceinline__ Dual<double, N> widen(const Dual<double, N>& x) { return x; }
+template <int N> __device__ __forceinline__ Dual<double, N> widen(const Dual<float, N>& x) {
+  Dual<double, N> r((double)x.v);
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.d[i] = (double)x.d[i];
+  return r;
+}
 
 }  // namespace gjx

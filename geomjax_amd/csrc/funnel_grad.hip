@@ -2,8 +2,11 @@
 // funnel with respect to (log step_size, log alpha2).
 //
 // The same closed forms as funnel_rmhmc.hip, evaluated on dual numbers
-// Dual<double, 2> seeded with d eps / d log eps = eps and
-// d alpha2 / d log alpha2 = alpha2.  Every executed operation, including each
+// Dual<T, 2> seeded with d eps / d log eps = eps and
+// d alpha2 / d log alpha2 = alpha2: T = double is the check path, T = float
+// the fast path (float32 state and trajectory as in the tuned sampler; the
+// energies, their difference and the acceptance on Dual<double, 2> from the
+// widened float32 values, as the tuned sampler keeps its energies in float64).  Every executed operation, including each
 // fixed-point iterate that the solver actually ran, carries its derivative;
 // the solver's stop decisions use values only, so the result is the exact
 // derivative of the unrolled computation at the executed iteration counts
@@ -15,6 +18,7 @@
 // squared jump |q_L - q_0|^2 and their product (the expected squared jump
 // distance times acceptance that step-size / alpha2 tuning maximises), each
 // with its two derivatives.
+#include "../../include/geomjax_amd.h"
 #include "common.h"
 #include "dual.h"
 #include "funnel_field.h"
@@ -22,10 +26,10 @@
 
 namespace gjx {
 
-using RD = Dual<double, 2>;
-
-template <int LPC, int EPL>
+template <class T, int LPC, int EPL>
 __global__ void __launch_bounds__(256) funnel_rmhmc_tangent_kernel(const FunnelArgs a, const FunnelGradOut o) {
+  using RD = Dual<T, 2>;       // state and trajectory
+  using RE = Dual<double, 2>;  // energies
   const int tid = blockIdx.x * 256 + threadIdx.x;
   const int chain = tid / LPC;
   const int lg = tid % LPC;
@@ -34,14 +38,18 @@ __global__ void __launch_bounds__(256) funnel_rmhmc_tangent_kernel(const FunnelA
   const int D = a.D, Dx = a.D - 1;
   const int j0 = lg * EPL;
 #define GJX_ON(k) (live && j0 + (k) < Dx)
-  RD eps(a.eps);
-  eps.d[0] = a.eps;
-  RD a2(a.alpha2);
-  a2.d[1] = a.alpha2;
-  const RD h = RD(0.5) * eps;
+  RD eps((T)a.eps);
+  eps.d[0] = (T)a.eps;
+  RD a2((T)a.alpha2);
+  a2.d[1] = (T)a.alpha2;
+  const RD h = RD(T(0.5)) * eps;
   const RD alpha = sqrt(a2);
-  const RD inv_s2(1.0 / (a.sigma * a.sigma));
-  const RD halfDx(0.5 * Dx);
+  const RD inv_s2((T)(1.0 / (a.sigma * a.sigma)));
+  const RD halfDx((T)(0.5 * Dx));
+  RE a2e(a.alpha2);
+  a2e.d[1] = a.alpha2;
+  const RE inv_s2e(1.0 / (a.sigma * a.sigma));
+  const RE halfDxe(0.5 * Dx);
 
   // ---- inputs (constants: zero tangents) ----
   RD x[EPL];
@@ -49,17 +57,17 @@ __global__ void __launch_bounds__(256) funnel_rmhmc_tangent_kernel(const FunnelA
   {
     const size_t base = c * D;
 #pragma unroll
-    for (int k = 0; k < EPL; ++k) x[k] = RD(GJX_ON(k) ? static_cast<const double*>(a.q_in)[base + j0 + k] : 0.0);
-    v = RD(live ? static_cast<const double*>(a.q_in)[base + Dx] : 0.0);
+    for (int k = 0; k < EPL; ++k) x[k] = RD(GJX_ON(k) ? static_cast<const T*>(a.q_in)[base + j0 + k] : T(0));
+    v = RD(live ? static_cast<const T*>(a.q_in)[base + Dx] : T(0));
   }
   const double logp = live ? a.logp_in[c] : 0.0;
-  double z[EPL];
-  double zv;
+  T z[EPL];
+  T zv;
   if (a.z != nullptr) {
     const size_t base = c * D;
 #pragma unroll
-    for (int k = 0; k < EPL; ++k) z[k] = GJX_ON(k) ? static_cast<const double*>(a.z)[base + j0 + k] : 0.0;
-    zv = live ? static_cast<const double*>(a.z)[base + Dx] : 0.0;
+    for (int k = 0; k < EPL; ++k) z[k] = GJX_ON(k) ? static_cast<const T*>(a.z)[base + j0 + k] : T(0);
+    zv = live ? static_cast<const T*>(a.z)[base + Dx] : T(0);
   } else {
     const uint64_t tt = a.counter;
     const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
@@ -69,13 +77,13 @@ __global__ void __launch_bounds__(256) funnel_rmhmc_tangent_kernel(const FunnelA
           u32x4{(uint32_t)c, (uint32_t)tt, kStreamMomentum + (uint32_t)(j0 + k), (uint32_t)(tt >> 32)}, k0, k1);
       double n0, n1;
       normal2(r, n0, n1);
-      z[k] = GJX_ON(k) ? n0 : 0.0;
-      if (k + 1 < EPL) z[k + 1] = GJX_ON(k + 1) ? n1 : 0.0;
+      z[k] = GJX_ON(k) ? (T)n0 : T(0);
+      if (k + 1 < EPL) z[k + 1] = GJX_ON(k + 1) ? (T)n1 : T(0);
     }
     u32x4 r = philox4x32_10(u32x4{(uint32_t)c, (uint32_t)tt, kStreamMomentumV, (uint32_t)(tt >> 32)}, k0, k1);
     double n0, n1;
     normal2(r, n0, n1);
-    zv = n0;
+    zv = (T)n0;
   }
 
   // ---- momentum p0 = chol(G) z (as funnel_rmhmc_kernel) ----
@@ -84,7 +92,7 @@ __global__ void __launch_bounds__(256) funnel_rmhmc_tangent_kernel(const FunnelA
   RD sx;
   {
     RD loc_inc[EPL];
-    RD acc(0.0);
+    RD acc(T(0));
 #pragma unroll
     for (int k = 0; k < EPL; ++k) {
       acc += x[k] * x[k];
@@ -94,13 +102,13 @@ __global__ void __launch_bounds__(256) funnel_rmhmc_tangent_kernel(const FunnelA
     const RD off1 = group_exclusive_scan<LPC>(acc, lg);
     const RD e = exp(-v);
     const RD a2e2 = a2 * e * e;
-    const RD gv = -v * inv_s2 - halfDx + RD(0.5) * e * sx;
+    const RD gv = -v * inv_s2 - halfDx + RD(T(0.5)) * e * sx;
     RD cj[EPL];
-    RD acc2(0.0);
+    RD acc2(T(0));
 #pragma unroll
     for (int k = 0; k < EPL; ++k) {
-      const RD tj = RD(1.0) + a2e2 * (off1 + loc_inc[k]);
-      const RD tjm = RD(1.0) + a2e2 * (off1 + loc_inc[k] - x[k] * x[k]);
+      const RD tj = RD(T(1)) + a2e2 * (off1 + loc_inc[k]);
+      const RD tjm = RD(T(1)) + a2e2 * (off1 + loc_inc[k] - x[k] * x[k]);
       const RD aj = -alpha * e * x[k];
       cj[k] = aj * z[k] / sqrt(tj * tjm);
       px[k] = sqrt(tj / tjm) * z[k];
@@ -112,46 +120,49 @@ __global__ void __launch_bounds__(256) funnel_rmhmc_tangent_kernel(const FunnelA
 #pragma unroll
     for (int k = 0; k < EPL; ++k) {
       const RD aj = -alpha * e * x[k];
-      px[k] = GJX_ON(k) ? px[k] + aj * run : RD(0.0);
+      px[k] = GJX_ON(k) ? px[k] + aj * run : RD(T(0));
       run += cj[k];
     }
-    const RD txe = RD(1.0) + a2e2 * sx;
+    const RD txe = RD(T(1)) + a2e2 * sx;
     const RD tv = txe + a2 * gv * gv;
     pv = sqrt(tv / txe) * zv + alpha * gv * tot2;
   }
 
-  auto energy = [&](const RD (&xx)[EPL], const RD (&pp)[EPL], RD vv, RD pvv, RD& logp_out) -> RD {
-    RD d1(0.0), d2(0.0), d3(0.0);
+  // energy from the state (widened to float64 duals)
+  auto energy = [&](const RD (&xx)[EPL], const RD (&pp)[EPL], RD vs, RD pvs, RE& logp_out) -> RE {
+    const RE vv = widen(vs), pvv = widen(pvs);
+    RE d1(0.0), d2(0.0), d3(0.0);
 #pragma unroll
     for (int k = 0; k < EPL; ++k) {
-      d1 += xx[k] * xx[k];
-      d2 += xx[k] * pp[k];
-      d3 += pp[k] * pp[k];
+      const RE xk = widen(xx[k]), pk = widen(pp[k]);
+      d1 += xk * xk;
+      d2 += xk * pk;
+      d3 += pk * pk;
     }
-    const RD Sx = group_sum<LPC>(d1);
-    const RD Xp = group_sum<LPC>(d2);
-    const RD P2 = group_sum<LPC>(d3) + pvv * pvv;
-    const RD e = exp(-vv);
-    const RD gv = -vv * inv_s2 - halfDx + RD(0.5) * e * Sx;
-    const RD gp = -e * Xp + gv * pvv;
-    const RD gg = e * e * Sx + gv * gv;
-    const RD cp = monge_perp_coef(gp, gg);
-    RD d4(0.0);
+    const RE Sx = group_sum<LPC>(d1);
+    const RE Xp = group_sum<LPC>(d2);
+    const RE P2 = group_sum<LPC>(d3) + pvv * pvv;
+    const RE e = exp(-vv);
+    const RE gv = -vv * inv_s2e - halfDxe + RE(0.5) * e * Sx;
+    const RE gp = -e * Xp + gv * pvv;
+    const RE gg = e * e * Sx + gv * gv;
+    const RE cp = monge_perp_coef(gp, gg);
+    RE d4(0.0);
 #pragma unroll
     for (int k = 0; k < EPL; ++k) {
-      const RD r = pp[k] + cp * e * xx[k];
+      const RE r = widen(pp[k]) + cp * e * widen(xx[k]);
       d4 += r * r;
     }
-    const RD rv = pvv - cp * gv;
-    const RD perp2 = group_sum<LPC>(d4) + rv * rv;
-    logp_out = RD(-0.5) * vv * vv * inv_s2 - RD(0.5) * e * Sx - RD(0.5 * Dx) * vv +
-               RD(-log(a.sigma) - 0.5 * kLog2Pi - 0.5 * (double)Dx * kLog2Pi);
-    return RD(0.5) * monge_quad(P2, perp2, gg, a2) + RD(0.5) * log(RD(1.0) + a2 * gg) +
-           RD(0.5 * (double)D * kLog2Pi);
+    const RE rv = pvv - cp * gv;
+    const RE perp2 = group_sum<LPC>(d4) + rv * rv;
+    logp_out = RE(-0.5) * vv * vv * inv_s2e - RE(0.5) * e * Sx - RE(0.5 * Dx) * vv +
+               RE(-log(a.sigma) - 0.5 * kLog2Pi - 0.5 * (double)Dx * kLog2Pi);
+    return RE(0.5) * monge_quad(P2, perp2, gg, a2e) + RE(0.5) * log(RE(1.0) + a2e * gg) +
+           RE(0.5 * (double)D * kLog2Pi);
   };
-  RD lp0;
-  const RD T0 = energy(x, px, v, pv, lp0);
-  const RD H0 = RD(-logp) + T0;  // the state's log density is an input constant
+  RE lp0;
+  const RE T0 = energy(x, px, v, pv, lp0);
+  const RE H0 = RE(-logp) + T0;  // the state's log density is an input constant
 
   // ---- trajectory ----
   RD xc[EPL], pc[EPL];
@@ -163,7 +174,7 @@ __global__ void __launch_bounds__(256) funnel_rmhmc_tangent_kernel(const FunnelA
   }
   RD xpx;
   {
-    RD a1(0.0);
+    RD a1(T(0));
 #pragma unroll
     for (int k = 0; k < EPL; ++k) a1 += xc[k] * pc[k];
     xpx = group_sum<LPC>(a1);
@@ -182,13 +193,13 @@ __global__ void __launch_bounds__(256) funnel_rmhmc_tangent_kernel(const FunnelA
     bool active = true;
     while (true) {
       const FieldCoef<RD> f = field_coef<RD>(vc, pvc, sx, xpx, a2, inv_s2, halfDx);
-      RD s1(0.0), s2(0.0);
+      RD s1(T(0)), s2(T(0));
       double mx = 0.0;
 #pragma unroll
       for (int k = 0; k < EPL; ++k) {
         const RD xn = x0[k] + h * (pc[k] + f.ce * xc[k]);
         const RD pn = p0[k] - h * (f.A * xc[k] + f.B * pc[k]);
-        mx = nan_max(mx, nan_max(fabs(xn.v - xc[k].v), fabs(pn.v - pc[k].v)));
+        mx = nan_max(mx, (double)nan_max(fabs(xn.v - xc[k].v), fabs(pn.v - pc[k].v)));
         s1 += xn * xn;
         s2 += xn * pn;
         if (active) {
@@ -199,7 +210,7 @@ __global__ void __launch_bounds__(256) funnel_rmhmc_tangent_kernel(const FunnelA
       const RD vn = v0 + h * f.uv;
       const RD pvn = pv0 - h * f.dHv;
       mx = group_reduce<LPC, NanMax>(mx);
-      mx = nan_max(mx, nan_max(fabs(vn.v - vc.v), fabs(pvn.v - pvc.v)));
+      mx = nan_max(mx, (double)nan_max(fabs(vn.v - vc.v), fabs(pvn.v - pvc.v)));
       s1 = group_sum<LPC>(s1);
       s2 = group_sum<LPC>(s2);
       if (active) {
@@ -216,7 +227,7 @@ __global__ void __launch_bounds__(256) funnel_rmhmc_tangent_kernel(const FunnelA
     }
     {
       const FieldCoef<RD> f = field_coef<RD>(vc, pvc, sx, xpx, a2, inv_s2, halfDx);
-      RD s1(0.0), s2(0.0);
+      RD s1(T(0)), s2(T(0));
 #pragma unroll
       for (int k = 0; k < EPL; ++k) {
         const RD xo = xc[k], po = pc[k];
@@ -234,29 +245,30 @@ __global__ void __launch_bounds__(256) funnel_rmhmc_tangent_kernel(const FunnelA
   }
 
   // ---- energy at the end, acceptance probability and squared jump ----
-  RD lp1;
-  const RD T1 = energy(xc, pc, vc, pvc, lp1);
-  const RD H1 = -lp1 + T1;
-  RD dH = H0 - H1;
-  RD pacc;
+  RE lp1;
+  const RE T1 = energy(xc, pc, vc, pvc, lp1);
+  const RE H1 = -lp1 + T1;
+  RE dH = H0 - H1;
+  RE pacc;
   if (isnan(dH.v)) {
-    pacc = RD(0.0);  // weight -inf: rejected, derivative 0
+    pacc = RE(0.0);  // weight -inf: rejected, derivative 0
   } else {
-    const RD ed = exp(dH);
-    pacc = ed.v < 1.0 ? ed : RD(1.0);  // min(1, e^dH): derivative 0 where clipped
+    const RE ed = exp(dH);
+    pacc = ed.v < 1.0 ? ed : RE(1.0);  // min(1, e^dH): derivative 0 where clipped
   }
-  RD jump(0.0);
+  RE jump(0.0);
 #pragma unroll
   for (int k = 0; k < EPL; ++k) {
-    const RD dx = xc[k] - x[k];
+    const RE dx = widen(xc[k]) - widen(x[k]);
     jump += dx * dx;
   }
-  jump = group_sum<LPC>(jump) + (vc - v) * (vc - v);
-  RD obj = pacc * jump;
+  const RE dv = widen(vc) - widen(v);
+  jump = group_sum<LPC>(jump) + dv * dv;
+  RE obj = pacc * jump;
   if (!isfinite(jump.v) || isnan(dH.v)) {
     // a divergent trajectory contributes nothing (and no derivative)
-    obj = RD(0.0);
-    pacc = RD(0.0);
+    obj = RE(0.0);
+    pacc = RE(0.0);
   }
   if (live && lg == 0) {
     o.objective[c] = obj.v;
@@ -274,29 +286,35 @@ __global__ void __launch_bounds__(256) funnel_rmhmc_tangent_kernel(const FunnelA
 
 namespace {
 
-template <int LPC>
+template <class T, int LPC>
 hipError_t launch_tangent_epl(const FunnelArgs& a, const FunnelGradOut& o, int epl, hipStream_t s) {
   const unsigned blocks = (unsigned)(((long long)a.C * LPC + 255) / 256);
   switch (epl) {
-    case 1: hipLaunchKernelGGL((funnel_rmhmc_tangent_kernel<LPC, 1>), dim3(blocks), dim3(256), 0, s, a, o); break;
-    case 2: hipLaunchKernelGGL((funnel_rmhmc_tangent_kernel<LPC, 2>), dim3(blocks), dim3(256), 0, s, a, o); break;
-    case 4: hipLaunchKernelGGL((funnel_rmhmc_tangent_kernel<LPC, 4>), dim3(blocks), dim3(256), 0, s, a, o); break;
-    case 8: hipLaunchKernelGGL((funnel_rmhmc_tangent_kernel<LPC, 8>), dim3(blocks), dim3(256), 0, s, a, o); break;
+    case 1: hipLaunchKernelGGL((funnel_rmhmc_tangent_kernel<T, LPC, 1>), dim3(blocks), dim3(256), 0, s, a, o); break;
+    case 2: hipLaunchKernelGGL((funnel_rmhmc_tangent_kernel<T, LPC, 2>), dim3(blocks), dim3(256), 0, s, a, o); break;
+    case 4: hipLaunchKernelGGL((funnel_rmhmc_tangent_kernel<T, LPC, 4>), dim3(blocks), dim3(256), 0, s, a, o); break;
+    case 8: hipLaunchKernelGGL((funnel_rmhmc_tangent_kernel<T, LPC, 8>), dim3(blocks), dim3(256), 0, s, a, o); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-}  // namespace
-
-hipError_t launch_funnel_rmhmc_tangent(const FunnelArgs& a, const FunnelGradOut& o, int lpc, int epl,
-                                       hipStream_t stream) {
+template <class T>
+hipError_t launch_tangent_lpc(const FunnelArgs& a, const FunnelGradOut& o, int lpc, int epl, hipStream_t stream) {
   switch (lpc) {
-    case 16: return launch_tangent_epl<16>(a, o, epl, stream);
-    case 32: return launch_tangent_epl<32>(a, o, epl, stream);
-    case 64: return launch_tangent_epl<64>(a, o, epl, stream);
+    case 16: return launch_tangent_epl<T, 16>(a, o, epl, stream);
+    case 32: return launch_tangent_epl<T, 32>(a, o, epl, stream);
+    case 64: return launch_tangent_epl<T, 64>(a, o, epl, stream);
     default: return hipErrorInvalidValue;
   }
+}
+
+}  // namespace
+
+hipError_t launch_funnel_rmhmc_tangent(const FunnelArgs& a, const FunnelGradOut& o, int lpc, int epl, int dtype,
+                                       hipStream_t stream) {
+  return dtype == GJX_F32 ? launch_tangent_lpc<float>(a, o, lpc, epl, stream)
+                          : launch_tangent_lpc<double>(a, o, lpc, epl, stream);
 }
 
 }  // namespace gjx

@@ -62,7 +62,7 @@ struct FunnelGradOut {
   int64_t* num_fevals;
 };
 
-hipError_t launch_funnel_rmhmc_tangent(const FunnelArgs& a, const FunnelGradOut& o, int lpc, int epl,
+hipError_t launch_funnel_rmhmc_tangent(const FunnelArgs& a, const FunnelGradOut& o, int lpc, int epl, int dtype,
                                        hipStream_t stream);
 
 // Smallest compiled elements-per-lane that covers the D-1 "x" coordinates

@@ -562,12 +562,15 @@ int gjx_logistic_value_and_grad(const gjx_logistic_params* params, const gjx_log
  * chol(G(q0; alpha2)) z: acceptance p = min(1, e^dH), squared jump
  * |q_L - q_0|^2, their product, and the exact derivatives of all three with
  * respect to (log step_size, log alpha2) through the executed fixed-point
- * iterates (forward-mode tangents).  float64 only; alpha2 > 0.
+ * iterates (forward-mode tangents).  alpha2 > 0.  The funnel entry point
+ * takes dtype GJX_F64 (the check path) or GJX_F32 (the fast path: float32
+ * state, trajectory and tangents, float64 energies and outputs); the Fisher
+ * entry point GJX_F64.
  * ------------------------------------------------------------------------- */
 typedef struct {
-  const void* position;      /* [C, D] float64 */
+  const void* position;      /* [C, D] of the params' dtype */
   const double* logdensity;  /* [C] */
-  const void* z;             /* [C, D] float64 standard normals, or NULL: device PRNG (seed, counter) */
+  const void* z;             /* [C, D] standard normals of the params' dtype, or NULL: device PRNG (seed, counter) */
   double* objective;         /* [C] */
   double* acceptance;        /* [C] */
   double* sq_jump;           /* [C] */
