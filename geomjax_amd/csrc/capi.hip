@@ -209,7 +209,8 @@ namespace {
 
 int check_funnel_nuts(const gjx_funnel_params* params, int32_t max_num_doublings, int32_t stopping_criterion) {
   GJX_REQUIRE(params != nullptr, "params is required");
-  GJX_REQUIRE(params->dtype == GJX_F64, "the NUTS kernel runs in float64");
+  GJX_REQUIRE(params->dtype == GJX_F64 || params->dtype == GJX_F32,
+              "dtype must be GJX_F64 or GJX_F32 (the tuned form)");
   GJX_REQUIRE(params->num_chains > 0, "num_chains must be > 0 (got %d)", params->num_chains);
   GJX_REQUIRE(params->dim >= 2, "the funnel needs dim >= 2 (got %d)", params->dim);
   if (params->dim - 1 > 256) return fail(GJX_EUNSUPPORTED, "dim %d > 257 for the NUTS kernel", params->dim);
@@ -259,7 +260,7 @@ int gjx_rmhmc_monge_funnel_nuts_sample(const gjx_funnel_params* params, int32_t 
   n.u_merge = io->u_merge;
   n.u_step = io->u_step;
   n.num_steps = io->num_integration_steps;
-  hipError_t e = gjx::launch_funnel_nuts(a, n, (hipStream_t)stream);
+  hipError_t e = gjx::launch_funnel_nuts(a, n, params->dtype, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "gjx_rmhmc_monge_funnel_nuts_sample launch");
   return GJX_OK;
 }
@@ -267,7 +268,8 @@ int gjx_rmhmc_monge_funnel_nuts_sample(const gjx_funnel_params* params, int32_t 
 int gjx_rmhmc_monge_funnel_nuts(const gjx_funnel_params* params, int32_t max_num_doublings,
                                 int32_t stopping_criterion, const gjx_nuts_io* io, void* stream) {
   GJX_REQUIRE(params != nullptr && io != nullptr, "params and io are required");
-  GJX_REQUIRE(params->dtype == GJX_F64, "the NUTS kernel runs in float64");
+  GJX_REQUIRE(params->dtype == GJX_F64 || params->dtype == GJX_F32,
+              "dtype must be GJX_F64 or GJX_F32 (the tuned form)");
   GJX_REQUIRE(params->num_chains > 0, "num_chains must be > 0 (got %d)", params->num_chains);
   GJX_REQUIRE(params->dim >= 2, "the funnel needs dim >= 2 (got %d)", params->dim);
   if (params->dim - 1 > 256) return fail(GJX_EUNSUPPORTED, "dim %d > 257 for the NUTS kernel", params->dim);
@@ -311,7 +313,7 @@ int gjx_rmhmc_monge_funnel_nuts(const gjx_funnel_params* params, int32_t max_num
   n.num_steps = io->num_integration_steps;
   n.left_position = io->leftmost_position;
   n.right_position = io->rightmost_position;
-  hipError_t e = gjx::launch_funnel_nuts(a, n, (hipStream_t)stream);
+  hipError_t e = gjx::launch_funnel_nuts(a, n, params->dtype, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "gjx_rmhmc_monge_funnel_nuts launch");
   return GJX_OK;
 }

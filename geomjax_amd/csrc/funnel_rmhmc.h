@@ -72,7 +72,7 @@ int funnel_pick_epl(int dim, int lpc);
 hipError_t launch_funnel_rmhmc(const FunnelArgs& a, int dtype, int lpc, int epl, hipStream_t stream);
 
 // NUTS (iterative, multinomial, numpyro-style U-turn with the "euc" criterion)
-hipError_t launch_funnel_nuts(const FunnelArgs& a, const NutsArgs& n, hipStream_t stream);
+hipError_t launch_funnel_nuts(const FunnelArgs& a, const NutsArgs& n, int dtype, hipStream_t stream);
 
 hipError_t launch_funnel_value_and_grad(int C, int D, double sigma, int dtype, const void* q, double* logp,
                                         void* grad, hipStream_t stream);

@@ -6,7 +6,8 @@ stopping_criterion="euc")`` with ``init`` (= rmhmc.init) and ``step``; the
 kernel ``build_kernel(integrator, divergence_threshold)(rng_key, state,
 logdensity_fn, step_size, metric_fn, max_num_doublings, stopping_criterion)``.
 The whole transition (momentum draw, iterative tree doubling, progressive
-multinomial sampling, U-turn checks) runs in one HIP kernel, float64, with the
+multinomial sampling, U-turn checks) runs in one HIP kernel, float64 (on the
+funnel also the tuned float32 form: float32 state, float64 energies), with the
 three U-turn criteria of geomjax/rmhmc/metrics.py:100-117 ("euc", the
 default; "bet"; "riem").  Served: Neal's funnel with the Monge metric (one
 wave per chain) and Bayesian logistic regression with the Fisher metric, one
@@ -116,8 +117,10 @@ def build_kernel(integrator=implicit_midpoint, divergence_threshold: int = 1000)
         if single:
             position = position[None]
         position = position.contiguous()
-        if position.dtype != torch.float64 or not position.is_cuda:
-            raise ValueError("NUTS positions must be float64 tensors on a HIP device")
+        if position.dtype not in (torch.float64, torch.float32) or not position.is_cuda:
+            raise ValueError("NUTS positions must be float64 (or, on the funnel, float32) tensors on a HIP device")
+        if logistic and position.dtype != torch.float64:
+            raise ValueError("NUTS on logistic regression runs in float64 (pass a float64 state)")
         C, D = position.shape
         if D != logdensity_fn.D:
             raise ValueError(f"position has {D} coordinates, the model has {logdensity_fn.D}")
@@ -128,15 +131,15 @@ def build_kernel(integrator=implicit_midpoint, divergence_threshold: int = 1000)
         logdensity = state.logdensity.reshape(C).to(torch.float64).contiguous()
         if isinstance(rng_key, NUTSNoise):
             f = lambda t, *shape: t.reshape(*shape).to(device=dev, dtype=torch.float64).contiguous()
-            noise = (f(rng_key.z, C, D), f(rng_key.u_direction, C, M), f(rng_key.u_merge, C, M),
-                     f(rng_key.u_step, C, 2 ** M - 1))
+            noise = (rng_key.z.reshape(C, D).to(device=dev, dtype=position.dtype).contiguous(),
+                     f(rng_key.u_direction, C, M), f(rng_key.u_merge, C, M), f(rng_key.u_step, C, 2 ** M - 1))
             seed, counter = 0, 0
         elif isinstance(rng_key, PRNGKey):
             noise = (None, None, None, None)
             seed, counter = rng_key.seed, rng_key.counter
         else:
             raise TypeError("rng_key must be NUTSNoise or PRNGKey")
-        vec = lambda: torch.empty(C, D, dtype=torch.float64, device=dev)
+        vec = lambda: torch.empty(C, D, dtype=position.dtype, device=dev)
         sca = lambda dt=torch.float64: torch.empty(C, dtype=dt, device=dev)
         out = dict(new_position=vec(), new_logdensity=sca(), new_logdensity_grad=vec(), momentum=vec(),
                    energy=sca(), acceptance_rate=sca(), is_divergent=sca(torch.uint8), is_turning=sca(torch.uint8),
@@ -147,7 +150,8 @@ def build_kernel(integrator=implicit_midpoint, divergence_threshold: int = 1000)
                          **{k: v.data_ptr() for k, v in out.items()})
         crit = _lib.GJX_UTURN[stopping_criterion]
         if funnel:
-            params = _params(C, D, _lib.GJX_F64, step_size, 0, solver, metric_fn.alpha2, logdensity_fn.sigma,
+            code = _lib.GJX_F64 if position.dtype == torch.float64 else _lib.GJX_F32
+            params = _params(C, D, code, step_size, 0, solver, metric_fn.alpha2, logdensity_fn.sigma,
                              divergence_threshold, seed=seed, counter=counter)
             rc = _lib.load().gjx_rmhmc_monge_funnel_nuts(params, M, crit, io, _lib.stream_handle(dev))
             _lib.check(rc, "gjx_rmhmc_monge_funnel_nuts")
@@ -178,7 +182,8 @@ def sample(rng_key, state, logdensity_fn, step_size, metric_fn, num_samples, *, 
     (examples/funnel/main.py:16-25).  The chains keep their state on chip, so
     the launch lasts as long as the largest per-chain sum of trajectory
     lengths.  ``rng_key``: a PRNGKey (transition t draws at counter + t) or
-    NUTSNoise with a leading [T] axis on every array.  Returns
+    NUTSNoise with a leading [T] axis on every array.  A float32 state runs
+    the tuned form (float32 state and trajectory, float64 energies).  Returns
     ``SampleResult`` (num_fevals = integration steps per chain)."""
     if not (isinstance(logdensity_fn, NealFunnel) and isinstance(metric_fn, MongeMetric)):
         raise NotImplementedError("the persistent NUTS launch serves Neal's funnel with the Monge metric")
@@ -186,8 +191,9 @@ def sample(rng_key, state, logdensity_fn, step_size, metric_fn, num_samples, *, 
         raise ValueError(f"stopping_criterion must be one of {sorted(_lib.GJX_UTURN)}")
     solver = _solver_kwargs(integrator)
     position = state.position.contiguous()
-    if position.dim() != 2 or position.dtype != torch.float64 or not position.is_cuda:
-        raise ValueError("sample() runs a batch of chains: position must be a float64 [C, D] tensor on a HIP device")
+    if position.dim() != 2 or position.dtype not in (torch.float64, torch.float32) or not position.is_cuda:
+        raise ValueError("sample() runs a batch of chains: position must be a float64 or float32 [C, D] tensor "
+                         "on a HIP device")
     C, D = position.shape
     if D != logdensity_fn.D:
         raise ValueError(f"position has {D} coordinates, the model has {logdensity_fn.D}")
@@ -198,8 +204,8 @@ def sample(rng_key, state, logdensity_fn, step_size, metric_fn, num_samples, *, 
     logdensity = state.logdensity.reshape(C).to(torch.float64).contiguous()
     if isinstance(rng_key, NUTSNoise):
         f = lambda t, *shape: t.reshape(*shape).to(device=dev, dtype=torch.float64).contiguous()
-        noise = (f(rng_key.z, T, C, D), f(rng_key.u_direction, T, C, M), f(rng_key.u_merge, T, C, M),
-                 f(rng_key.u_step, T, C, 2 ** M - 1))
+        noise = (rng_key.z.reshape(T, C, D).to(device=dev, dtype=position.dtype).contiguous(),
+                 f(rng_key.u_direction, T, C, M), f(rng_key.u_merge, T, C, M), f(rng_key.u_step, T, C, 2 ** M - 1))
         seed, counter = 0, 0
     elif isinstance(rng_key, PRNGKey):
         noise = (None, None, None, None)
@@ -207,11 +213,11 @@ def sample(rng_key, state, logdensity_fn, step_size, metric_fn, num_samples, *, 
     else:
         raise TypeError("rng_key must be NUTSNoise or PRNGKey")
     if samples_out is not None:
-        if samples_out.shape != (T, C, D) or samples_out.dtype != torch.float64 or not samples_out.is_contiguous():
-            raise ValueError(f"samples_out must be a contiguous [{T}, {C}, {D}] float64 tensor")
+        if samples_out.shape != (T, C, D) or samples_out.dtype != position.dtype or not samples_out.is_contiguous():
+            raise ValueError(f"samples_out must be a contiguous [{T}, {C}, {D}] {position.dtype} tensor")
         samples = samples_out
     else:
-        samples = torch.empty(T, C, D, dtype=torch.float64, device=dev) if store_samples else None
+        samples = torch.empty(T, C, D, dtype=position.dtype, device=dev) if store_samples else None
     final_q = torch.empty_like(position)
     final_l = torch.empty(C, dtype=torch.float64, device=dev)
     final_g = torch.empty_like(position)
@@ -224,7 +230,8 @@ def sample(rng_key, state, logdensity_fn, step_size, metric_fn, num_samples, *, 
                            final_logdensity_grad=final_g.data_ptr(), samples=_lib.ptr(samples),
                            acceptance_rate=acc.data_ptr(), num_integration_steps=steps.data_ptr(),
                            num_fevals=fev.data_ptr())
-    params = _params(C, D, _lib.GJX_F64, step_size, 0, solver, metric_fn.alpha2, logdensity_fn.sigma,
+    code = _lib.GJX_F64 if position.dtype == torch.float64 else _lib.GJX_F32
+    params = _params(C, D, code, step_size, 0, solver, metric_fn.alpha2, logdensity_fn.sigma,
                      divergence_threshold, T=T, seed=seed, counter=counter)
     rc = _lib.load().gjx_rmhmc_monge_funnel_nuts_sample(params, M, _lib.GJX_UTURN[stopping_criterion], io,
                                                          _lib.stream_handle(dev))
