@@ -535,10 +535,11 @@ def funnel_lmc_forms(K, W, dev, plain=True):
     return res
 
 
-def nuts_forms(K, W, dev, static_plain=None, chunk=10):
-    """NUTS with the RMHMC integrator (float64) on Neal's funnel D=100 with 8192
-    chains: Monge metric, the funnel workload's step size, max_num_doublings
-    10, "euc" criterion, device PRNG.  Timed as persistent launches of `chunk`
+def nuts_forms(K, W, dev, static_plain=None, chunk=10, tuned=False):
+    """NUTS with the RMHMC integrator (float64, or with ``tuned`` the float32
+    state with float64 energies) on Neal's funnel D=100 with 8192 chains: Monge
+    metric, the funnel workload's step size, max_num_doublings 10, "euc"
+    criterion, device PRNG.  Timed as persistent launches of `chunk`
     transitions (each chain keeps its state on chip, so a launch lasts as long
     as the largest per-chain sum of trajectory lengths) and, for comparison,
     as one launch per transition (a launch lasts as long as its longest
@@ -557,13 +558,14 @@ def nuts_forms(K, W, dev, static_plain=None, chunk=10):
     model, metric = gj.NealFunnel(D_F, sigma=SIGMA), gj.MongeMetric(ALPHA2)
     integ = functools.partial(implicit_midpoint, max_iters=MAXIT_F)
     kern = nuts_kernel(integ)
+    dt = torch.float32 if tuned else torch.float64
     q_init = torch.randn(CHAINS_F, D_F, generator=torch.Generator().manual_seed(SEED_F), dtype=torch.float64)
-    s = gj.rmhmc.init(q_init.to(dev), model)
+    s = gj.rmhmc.init(q_init.to(dev).to(dt), model)
     r = nuts_sample(gr.PRNGKey(SEED_F + 7, 0), s, model, EPS_F, metric, max(W, chunk), max_num_doublings=M,
                     integrator=integ, store_samples=False)
     s, counter = r.state, max(W, chunk)
     warmed = s
-    samples = torch.empty(K, CHAINS_F, D_F, dtype=torch.float64, device=dev)
+    samples = torch.empty(K, CHAINS_F, D_F, dtype=dt, device=dev)
     steps, accs = [], []
     stream = torch.cuda.current_stream(dev)
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -589,8 +591,9 @@ def nuts_forms(K, W, dev, static_plain=None, chunk=10):
            "max_integration_steps": int(st.max().item()),
            "acceptance_rate": float(torch.cat(accs).mean().item()),
            "ms_per_transition": wall / K * 1e3, "kernel_ms_per_transition": kernel_s / K * 1e3,
-           "kernel": "funnel_nuts_kernel<2>", "form": f"persistent launches of {chunk} transitions",
-           "dtype": "f64", "max_num_doublings": M, "steps": K}
+           "kernel": f"funnel_nuts_kernel<2, {'float' if tuned else 'double'}>",
+           "form": f"persistent launches of {chunk} transitions",
+           "dtype": "f32 state, f64 energies" if tuned else "f64", "max_num_doublings": M, "steps": K}
     # the same chains, one launch per transition
     n1 = min(K, 10)
     s = warmed
@@ -602,7 +605,7 @@ def nuts_forms(K, W, dev, static_plain=None, chunk=10):
     res["one_launch_per_transition_ms"] = (time.perf_counter() - t0) / n1 * 1e3
     if static_plain is not None:
         res["min_ess_per_sec_vs_static_rmhmc_plain"] = res["min_ess_per_sec"] / static_plain["min_ess_per_sec"]
-    log(f"funnel NUTS: {res['value']:.3e} integration-steps/s, {res['mean_integration_steps']:.1f} steps per "
+    log(f"funnel NUTS ({res['dtype']}): {res['value']:.3e} integration-steps/s, {res['mean_integration_steps']:.1f} steps per "
         f"transition, {res['ms_per_transition']:.2f} ms per transition persistent vs "
         f"{res['one_launch_per_transition_ms']:.2f} one launch each, min-ESS/s {res['min_ess_per_sec']:.3e}")
     return res
@@ -968,6 +971,7 @@ def main():
         return
     if args.only == "funnel_nuts":
         print(json.dumps(nuts_forms(K, W, dev)), flush=True)
+        print(json.dumps(nuts_forms(K, W, dev, tuned=True)), flush=True)
         return
     if args.only == "logistic_nuts":
         print(json.dumps(logistic_nuts_forms(K, W, dev)), flush=True)
@@ -1019,6 +1023,9 @@ def main():
         extra["funnel_d100_8192_chains"] = funnel_forms(300, 100, dev, plain=not args.no_plain)
         extra["funnel_nuts_d100_8192_chains"] = nuts_forms(100, 10, dev,
                                                            extra["funnel_d100_8192_chains"].get("plain"))
+        extra["funnel_nuts_d100_8192_chains_f32"] = nuts_forms(100, 10, dev,
+                                                               extra["funnel_d100_8192_chains"].get("plain"),
+                                                               tuned=True)
         extra["logistic_nuts_1024_chains"] = logistic_nuts_forms(10, 2, dev)
         extra["logistic_lmc_1024_chains"] = logistic_lmc_forms(20, 5, dev)
         extra["logistic_nuts_lmc_1024_chains"] = logistic_nuts_forms(10, 2, dev, integrator="lmc")
