@@ -586,7 +586,7 @@ int check_logistic(const gjx_logistic_params* p, const gjx_logistic_data* d, boo
 // both RMHMC kernels (float64 and float32) serve D <= 128
 int rmhmc_max_d(const gjx_logistic_params*) { return gjx::kLogisticMaxD; }
 
-// only the RMHMC entry points have a float32 path
+// the RMHMC and LMC entry points have a float32 path; these do not
 int require_f64(const gjx_logistic_params* p, const char* what) {
   if (p != nullptr && p->dtype == GJX_F32) return fail(GJX_EUNSUPPORTED, "%s has no float32 path (use GJX_F64)", what);
   return GJX_OK;
@@ -872,7 +872,6 @@ int gjx_rmhmc_fisher_logistic_sample(const gjx_logistic_params* params, const gj
 
 int gjx_lmc_fisher_logistic_step(const gjx_logistic_params* params, const gjx_logistic_data* data,
                                  const gjx_lmcmc_step_io* io, void* stream) {
-  if (int rc0 = require_f64(params, "gjx_lmc_fisher_logistic_step")) return rc0;
   gjx::LogisticArgs a;
   int rc = check_logistic(params, data, false, &a, gjx::kLogisticMaxDLmc);
   if (rc != GJX_OK) return rc;
@@ -907,14 +906,14 @@ int gjx_lmc_fisher_logistic_step(const gjx_logistic_params* params, const gjx_lo
   a.prop_J = io->proposal_volume_adjustment;
   a.prop_weight = io->proposal_weight;
   a.prop_slpa = io->proposal_sum_log_p_accept;
-  hipError_t e = gjx::launch_logistic_lmc_f64(a, (hipStream_t)stream);
+  hipError_t e = params->dtype == GJX_F32 ? gjx::launch_logistic_lmc_f32(a, data->max_features, (hipStream_t)stream)
+                                           : gjx::launch_logistic_lmc_f64(a, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "gjx_lmc_fisher_logistic_step launch");
   return GJX_OK;
 }
 
 int gjx_lmc_fisher_logistic_sample(const gjx_logistic_params* params, const gjx_logistic_data* data,
                                    const gjx_lmcmc_sample_io* io, void* stream) {
-  if (int rc0 = require_f64(params, "gjx_lmc_fisher_logistic_sample")) return rc0;
   gjx::LogisticArgs a;
   int rc = check_logistic(params, data, true, &a, gjx::kLogisticMaxDLmc);
   if (rc != GJX_OK) return rc;
@@ -936,7 +935,8 @@ int gjx_lmc_fisher_logistic_sample(const gjx_logistic_params* params, const gjx_
   a.samples = io->samples;
   a.sample_acc = io->acceptance_rate;
   a.num_accepted = io->num_accepted;
-  hipError_t e = gjx::launch_logistic_lmc_f64(a, (hipStream_t)stream);
+  hipError_t e = params->dtype == GJX_F32 ? gjx::launch_logistic_lmc_f32(a, data->max_features, (hipStream_t)stream)
+                                           : gjx::launch_logistic_lmc_f64(a, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "gjx_lmc_fisher_logistic_sample launch");
   return GJX_OK;
 }

@@ -146,6 +146,8 @@ hipError_t launch_logistic_nuts_f64(const LogisticArgs& a, const NutsArgs& n, hi
 hipError_t launch_logistic_nuts_lmc_f64(const LogisticArgs& a, const NutsArgs& n, hipStream_t stream);
 hipError_t launch_logistic_value_and_grad(const LogisticArgs& a, hipStream_t stream);
 hipError_t launch_logistic_lmc_f64(const LogisticArgs& a, hipStream_t stream);
+// LMC, tuned form (float32 state, split-bf16 MFMA Grams), D <= 64 (logistic_lmc_fast.hip)
+hipError_t launch_logistic_lmc_f32(const LogisticArgs& a, int max_features, hipStream_t stream);
 // concurrent = false launches every packed width on `stream` in turn (no side
 // streams or events are created)
 hipError_t launch_logistic_rmhmc_f32(const LogisticArgs& a, int max_features, hipStream_t stream,

@@ -7,7 +7,9 @@ num_integration_steps)``, and the ``lmc`` algorithm class (exported as
 ``geomjax_amd.lmc`` like the reference's alias, geomjax/__init__.py:17).
 
 Served problems: Bayesian logistic regression (one posterior or a batch of
-posteriors) with ``FisherMetric``; float64.
+posteriors, D <= 64) with ``FisherMetric``: float64 states run the float64
+kernel, float32 states the tuned one (split-bf16 MFMA Grams, float64
+log-determinants, volume change and energies).
 """
 from typing import NamedTuple
 
@@ -57,8 +59,8 @@ def lan_integrator(*args, **kwargs):
 def _check(logdensity_fn, metric_fn, position):
     if not isinstance(logdensity_fn, (LogisticRegression, LogisticBatch)) or not isinstance(metric_fn, FisherMetric):
         raise NotImplementedError(f"no HIP LMC kernel for logdensity_fn={logdensity_fn!r}, metric_fn={metric_fn!r}")
-    if not position.is_cuda or position.dtype != torch.float64:
-        raise ValueError("LMC states must be float64 tensors on a HIP device")
+    if not position.is_cuda or position.dtype not in (torch.float64, torch.float32):
+        raise ValueError("LMC states must be float64 (plain) or float32 (tuned) tensors on a HIP device")
     if position.shape[-1] != logdensity_fn.dim:
         raise ValueError(f"position has {position.shape[-1]} coordinates, the model has {logdensity_fn.dim}")
     if isinstance(logdensity_fn, LogisticBatch) and position.shape[0] != logdensity_fn.num_chains:
@@ -78,11 +80,11 @@ def init(position, logdensity_fn):
     return LMCState(position, logdensity, grad, torch.zeros_like(logdensity))
 
 
-def _noise(rng_key, T, C, D, dev):
+def _noise(rng_key, T, C, D, dev, dtype=torch.float64):
     if isinstance(rng_key, HostNoise):
         shape = (C, D) if T is None else (T, C, D)
         ushape = (C,) if T is None else (T, C)
-        return (rng_key.z.reshape(shape).to(device=dev, dtype=torch.float64).contiguous(),
+        return (rng_key.z.reshape(shape).to(device=dev, dtype=dtype).contiguous(),
                 rng_key.u.reshape(ushape).to(device=dev, dtype=torch.float64).contiguous(), 0, 0)
     if isinstance(rng_key, PRNGKey):
         return None, None, rng_key.seed, rng_key.counter
@@ -112,14 +114,15 @@ def build_kernel(integrator=lan_integrator, divergence_threshold: float = 1000):
         _check(logdensity_fn, metric_fn, position)
         dev = position.device
         C, D = position.shape
+        dt = position.dtype
         logdensity = state.logdensity.reshape(C).to(torch.float64).contiguous()
-        grad = state.logdensity_grad.reshape(C, D).to(torch.float64).contiguous()
+        grad = state.logdensity_grad.reshape(C, D).to(dt).contiguous()
         J = torch.as_tensor(state.volume_adjustment, dtype=torch.float64, device=dev).expand(C).contiguous()
-        z, u, seed, counter = _noise(rng_key, None, C, D, dev)
+        z, u, seed, counter = _noise(rng_key, None, C, D, dev, dt)
         L, steps = _steps(num_integration_steps, (C,), dev, max_num_integration_steps)
 
         def vec():
-            return torch.zeros(C, D, dtype=torch.float64, device=dev)
+            return torch.zeros(C, D, dtype=dt, device=dev)
 
         def sca(dt=torch.float64):
             return torch.empty(C, dtype=dt, device=dev)
@@ -134,8 +137,9 @@ def build_kernel(integrator=lan_integrator, divergence_threshold: float = 1000):
                               u=_lib.ptr(u), num_integration_steps=_lib.ptr(steps),
                               **{k: v.data_ptr() for k, v in out.items()})
         params = logistic_params(logdensity_fn, C, L=L, step_size=step_size,
-                                 divergence_threshold=divergence_threshold, seed=seed, counter=counter)
-        rc = _lib.load().gjx_lmc_fisher_logistic_step(params, logdensity_fn.data(), io, _lib.stream_handle(dev))
+                                 divergence_threshold=divergence_threshold, seed=seed, counter=counter, dtype=dt)
+        data = logdensity_fn.data(dtype=dt, num_chains=C)
+        rc = _lib.load().gjx_lmc_fisher_logistic_step(params, data, io, _lib.stream_handle(dev))
         _lib.check(rc, "gjx_lmc_fisher_logistic_step")
 
         def sq(t):
@@ -179,9 +183,10 @@ def sample(rng_key, state, logdensity_fn, step_size, metric_fn, num_integration_
     T = int(num_samples)
     if T < 1:
         raise ValueError("num_samples must be >= 1")
+    dt = position.dtype
     logdensity = state.logdensity.reshape(C).to(torch.float64).contiguous()
     J = torch.as_tensor(state.volume_adjustment, dtype=torch.float64, device=dev).expand(C).contiguous()
-    z, u, seed, counter = _noise(rng_key, T, C, D, dev)
+    z, u, seed, counter = _noise(rng_key, T, C, D, dev, dt)
     L, steps = _steps(num_integration_steps, (T, C), dev, max_num_integration_steps)
     final_q = torch.zeros_like(position)
     final_l = torch.empty(C, dtype=torch.float64, device=dev)
@@ -190,7 +195,7 @@ def sample(rng_key, state, logdensity_fn, step_size, metric_fn, num_integration_
     if samples_out is not None:
         samples = samples_out
     else:
-        samples = torch.zeros(T, C, D, dtype=torch.float64, device=dev) if store_samples else None
+        samples = torch.zeros(T, C, D, dtype=dt, device=dev) if store_samples else None
     acc = torch.empty(T, C, dtype=torch.float32, device=dev)
     n_acc = torch.empty(C, dtype=torch.int32, device=dev)
     io = _lib.LmcmcSampleIO(position=position.data_ptr(), logdensity=logdensity.data_ptr(),
@@ -200,8 +205,9 @@ def sample(rng_key, state, logdensity_fn, step_size, metric_fn, num_integration_
                             samples=_lib.ptr(samples), acceptance_rate=acc.data_ptr(), num_accepted=n_acc.data_ptr(),
                             num_integration_steps=_lib.ptr(steps))
     params = logistic_params(logdensity_fn, C, L=L, T=T, step_size=step_size,
-                             divergence_threshold=divergence_threshold, seed=seed, counter=counter)
-    rc = _lib.load().gjx_lmc_fisher_logistic_sample(params, logdensity_fn.data(), io, _lib.stream_handle(dev))
+                             divergence_threshold=divergence_threshold, seed=seed, counter=counter, dtype=dt)
+    data = logdensity_fn.data(dtype=dt, num_chains=C)
+    rc = _lib.load().gjx_lmc_fisher_logistic_sample(params, data, io, _lib.stream_handle(dev))
     _lib.check(rc, "gjx_lmc_fisher_logistic_sample")
     if steps is None:
         fev = torch.full((C,), T * L, dtype=torch.int64, device=dev)

@@ -423,11 +423,15 @@ typedef struct {
 /* One LMC transition per chain with metric_fn = Fisher metric: the reference
  * kernel geomjax/lmcmc/lmc.py:135-180 with lan_integrator
  * (geomjax/lmcmc/integrators.py:51-144) and gaussian_riemannian
- * (geomjax/lmcmc/metrics.py:42-206). */
+ * (geomjax/lmcmc/metrics.py:42-206).  D <= 64.  params->dtype GJX_F64: the
+ * float64 kernel; GJX_F32: the tuned form (float32 state buffers, Gram
+ * matrices on split-bf16 MFMA, float64 log-determinants, volume change and
+ * energies), which needs data->packed and data->workspace as the tuned RMHMC
+ * entry point does. */
 int gjx_lmc_fisher_logistic_step(const gjx_logistic_params* params, const gjx_logistic_data* data,
                                  const gjx_lmcmc_step_io* io, void* stream);
 
-/* T LMC transitions per chain in one launch. */
+/* T LMC transitions per chain in one launch (dtype as for the step). */
 int gjx_lmc_fisher_logistic_sample(const gjx_logistic_params* params, const gjx_logistic_data* data,
                                    const gjx_lmcmc_sample_io* io, void* stream);
 

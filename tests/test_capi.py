@@ -97,13 +97,19 @@ def test_logistic_float32_requires_packed_table_and_workspace():
     data.packed, data.workspace, data.workspace_bytes = 1, 1, 10
     assert lib.gjx_rmhmc_fisher_logistic_step(p, data, io, None) == -1
     assert "workspace" in _lib.last_error()
+    # the tuned LMC entry points check the same (their io is checked after)
+    data.workspace_bytes = 10
+    assert lib.gjx_lmc_fisher_logistic_step(p, data, _lib.LmcmcStepIO(), None) == -1
+    assert "workspace" in _lib.last_error()
+    assert lib.gjx_lmc_fisher_logistic_sample(p, data, _lib.LmcmcSampleIO(), None) == -1
+    assert "workspace" in _lib.last_error()
     # entry points without a float32 path say so
-    assert lib.gjx_lmc_fisher_logistic_step(p, data, _lib.LmcmcStepIO(), None) == -2
+    assert lib.gjx_mmala_fisher_logistic_step(p, data, 1, _lib.MmalaStepIO(), None) == -2
     assert "float32" in _lib.last_error()
 
 
 def test_logistic_feature_capacity_per_path():
-    """RMHMC (float64 and float32) takes up to 128 features, LMC up to 64."""
+    """RMHMC (float64 and float32) takes up to 128 features, LMC (both) up to 64."""
     lib = _lib.load()
 
     def step(fn, io, dtype, D):
@@ -120,6 +126,8 @@ def test_logistic_feature_capacity_per_path():
     assert step(rm, rio, _lib.GJX_F32, 129) == -2
     assert step(lmc, lio, _lib.GJX_F64, 64) == -1
     assert step(lmc, lio, _lib.GJX_F64, 65) == -2
+    assert step(lmc, lio, _lib.GJX_F32, 64) == -1
+    assert step(lmc, lio, _lib.GJX_F32, 65) == -2
 
 
 def test_mmala_and_logistic_nuts_entry_points_validate_on_the_host():
