@@ -663,10 +663,13 @@ def logistic_nuts_forms(K, W, dev, chains=1024, integrator="rmhmc"):
     return res
 
 
-def logistic_lmc_forms(K, W, dev, chains=1024, L=10):
-    """The general-metric LMC kernel (lmcmc, Fisher metric, float64, one launch per
+def logistic_lmc_forms(K, W, dev, chains=1024, L=10, tuned=False):
+    """The general-metric LMC kernel (lmcmc, Fisher metric, one launch per
     transition) on the headline posterior (logistic 2048 x 64): step size 0.1, 10
-    integration steps, device PRNG.  Reports leapfrog-steps/s and min-ESS/s."""
+    integration steps, device PRNG; float64, or with ``tuned`` the float32 state
+    with split-bf16 MFMA Grams.  Reports leapfrog-steps/s, min-ESS/s and the
+    algorithmic MFMA rate (per integration step five Gram matrices and one
+    quadratic-form pass, 12 N D^2 flop; per transition one more of each, 4 N D^2)."""
     import torch
 
     import geomjax_amd as gj
@@ -675,29 +678,40 @@ def logistic_lmc_forms(K, W, dev, chains=1024, L=10):
     from geomjax_amd.lmcmc import init as lmc_init
     from geomjax_amd.synthetic import make_logistic
 
+    dt = torch.float32 if tuned else torch.float64
     X, y, _ = make_logistic(N_ROWS, D_LOG, DATA_SEED)
     model = gj.LogisticRegression(X, y, PRIOR, device=dev)
     kern = lmc_kernel()
     q = 0.1 * torch.randn(chains, D_LOG, generator=torch.Generator().manual_seed(INIT_SEED), dtype=torch.float64)
-    s = lmc_init(q.to(dev), model)
+    s = lmc_init(q.to(dev).to(dt), model)
     for t in range(W):
         s, _ = kern(gr.PRNGKey(INIT_SEED + 13, t), s, model, EPS_LOG, gj.FisherMetric(), L)
-    samples = torch.empty(K, chains, D_LOG, dtype=torch.float64, device=dev)
+    samples = torch.empty(K, chains, D_LOG, dtype=dt, device=dev)
     accs = []
+    stream = torch.cuda.current_stream(dev)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    ev0.record(stream)
     for t in range(K):
         s, info = kern(gr.PRNGKey(INIT_SEED + 13, W + t), s, model, EPS_LOG, gj.FisherMetric(), L)
         samples[t].copy_(s.position)
         accs.append(info.acceptance_rate)
+    ev1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    gpu_s = ev0.elapsed_time(ev1) / 1e3
     ess, _ = min_ess(samples)
     del samples
+    flops = (12.0 * L + 4.0) * N_ROWS * D_LOG ** 2 * chains * K
     res = {"value": chains * L * K / wall, "unit": "leapfrog-steps/s", "min_ess_per_sec": ess / wall, "min_ess": ess,
            "acceptance_rate": float(torch.stack(accs).mean()), "ms_per_transition": wall / K * 1e3, "chains": chains,
-           "num_integration_steps": L, "dtype": "f64", "steps": K, "kernel": "logistic_lmc_f64_kernel"}
-    log(f"logistic LMC (Fisher): {res['value']:.3e} leapfrog-steps/s, min-ESS/s {res['min_ess_per_sec']:.3e}")
+           "gpu_ms_per_transition": gpu_s / K * 1e3, "algorithmic_tflops": flops / gpu_s / 1e12,
+           "num_integration_steps": L, "dtype": "f32 state, f64 energies" if tuned else "f64", "steps": K,
+           "kernel": "logistic_lmc_f32_kernel<2, true>" if tuned else "logistic_lmc_f64_kernel"}
+    log(f"logistic LMC (Fisher, {res['dtype']}): {res['value']:.3e} leapfrog-steps/s, "
+        f"{res['ms_per_transition']:.2f} ms per transition, {res['algorithmic_tflops']:.1f} TFLOP/s algorithmic, "
+        f"min-ESS/s {res['min_ess_per_sec']:.3e}")
     return res
 
 
@@ -999,6 +1013,7 @@ def main():
         return
     if args.only == "logistic_lmc":
         print(json.dumps(logistic_lmc_forms(K, W, dev)), flush=True)
+        print(json.dumps(logistic_lmc_forms(K, W, dev, tuned=True)), flush=True)
         return
     if args.only == "lmc_alpha2":
         print(json.dumps(lmc_alpha2_forms(dev)), flush=True)
@@ -1028,6 +1043,10 @@ def main():
                                                                tuned=True)
         extra["logistic_nuts_1024_chains"] = logistic_nuts_forms(10, 2, dev)
         extra["logistic_lmc_1024_chains"] = logistic_lmc_forms(20, 5, dev)
+        extra["logistic_lmc_1024_chains_f32"] = logistic_lmc_forms(20, 5, dev, tuned=True)
+        extra["logistic_lmc_1024_chains_f32"]["tuned_vs_f64"] = {
+            k: extra["logistic_lmc_1024_chains_f32"][k] / extra["logistic_lmc_1024_chains"][k]
+            for k in ("value", "min_ess_per_sec")}
         extra["logistic_nuts_lmc_1024_chains"] = logistic_nuts_forms(10, 2, dev, integrator="lmc")
         extra["mmala_1024_chains"] = mmala_forms(200, 100, dev)
         extra["dense_lmc_funnel_d20_1024_chains"] = dense_lmc_forms(dev)
