@@ -6,13 +6,14 @@ stopping_criterion="euc")`` with ``init`` (= rmhmc.init) and ``step``; the
 kernel ``build_kernel(integrator, divergence_threshold)(rng_key, state,
 logdensity_fn, step_size, metric_fn, max_num_doublings, stopping_criterion)``.
 The whole transition (momentum draw, iterative tree doubling, progressive
-multinomial sampling, U-turn checks) runs in one HIP kernel, float64 (on the
-funnel also the tuned float32 form: float32 state, float64 energies), with the
-three U-turn criteria of geomjax/rmhmc/metrics.py:100-117 ("euc", the
-default; "bet"; "riem").  Served: Neal's funnel with the Monge metric (one
-wave per chain) and Bayesian logistic regression with the Fisher metric, one
-posterior or a ``LogisticBatch``, every posterior with D <= 64 (one
-workgroup per chain).
+multinomial sampling, U-turn checks) runs in one HIP kernel, float64 for a
+float64 state and the tuned form for a float32 one (float32 state and
+trajectory, float64 energies and tree weights), with the three U-turn
+criteria of geomjax/rmhmc/metrics.py:100-117 ("euc", the default; "bet";
+"riem").  Served: Neal's funnel with the Monge metric (one wave per chain) and
+Bayesian logistic regression with the Fisher metric, one posterior or a
+``LogisticBatch``, every posterior with D <= 64 (float64: one 256-lane
+workgroup per chain; tuned: one wave per chain, split-bf16 MFMA passes).
 
 ``rng_key`` is a ``PRNGKey`` (device Philox) or ``NUTSNoise(z, u_direction,
 u_merge, u_step)`` with the draws the reference takes from its keys: the
@@ -118,9 +119,7 @@ def build_kernel(integrator=implicit_midpoint, divergence_threshold: int = 1000)
             position = position[None]
         position = position.contiguous()
         if position.dtype not in (torch.float64, torch.float32) or not position.is_cuda:
-            raise ValueError("NUTS positions must be float64 (or, on the funnel, float32) tensors on a HIP device")
-        if logistic and position.dtype != torch.float64:
-            raise ValueError("NUTS on logistic regression runs in float64 (pass a float64 state)")
+            raise ValueError("NUTS positions must be float64 (plain) or float32 (tuned) tensors on a HIP device")
         C, D = position.shape
         if D != logdensity_fn.D:
             raise ValueError(f"position has {D} coordinates, the model has {logdensity_fn.D}")
@@ -159,8 +158,9 @@ def build_kernel(integrator=implicit_midpoint, divergence_threshold: int = 1000)
             params = logistic_params(logdensity_fn, C, step_size=step_size, max_iters=solver["max_iters"],
                                      convergence_tol=solver["convergence_tol"],
                                      divergence_tol=solver["divergence_tol"],
-                                     divergence_threshold=divergence_threshold, seed=seed, counter=counter)
-            data = logdensity_fn.data(dtype=torch.float64, num_chains=C)
+                                     divergence_threshold=divergence_threshold, seed=seed, counter=counter,
+                                     dtype=position.dtype)
+            data = logdensity_fn.data(dtype=position.dtype, num_chains=C)
             rc = _lib.load().gjx_rmhmc_fisher_logistic_nuts(params, data, M, crit, io, _lib.stream_handle(dev))
             _lib.check(rc, "gjx_rmhmc_fisher_logistic_nuts")
         sq = (lambda t: t[0]) if single else (lambda t: t)

@@ -664,12 +664,15 @@ int gjx_rmhmc_monge_funnel_nuts_sample(const gjx_funnel_params* params, int32_t 
                                        int32_t stopping_criterion, const gjx_nuts_sample_io* io, void* stream);
 
 /* NUTS with the RMHMC integrator for Bayesian logistic regression with the
- * Fisher metric (geomjax/rmhmc/nuts.py:78-161 over the float64 logistic
- * integrator of gjx_rmhmc_fisher_logistic_step): float64 (params->dtype =
- * GJX_F64), every posterior D <= 64, one workgroup per chain; a batch of
- * posteriors as for the step entry point.  io as for the funnel entry point, with
- * rows of stride params->dim; new_logdensity_grad is recomputed at the chosen
- * state.  params->num_integration_steps and num_transitions are not used. */
+ * Fisher metric (geomjax/rmhmc/nuts.py:78-161 over the logistic integrator of
+ * gjx_rmhmc_fisher_logistic_step): params->dtype GJX_F64 (one 256-lane
+ * workgroup per chain) or GJX_F32, the tuned form (float32 state buffers and
+ * trajectory, split-bf16 MFMA passes, float64 energies and tree weights, one
+ * wave per chain; needs data->packed and data->workspace as the tuned step);
+ * every posterior D <= 64; a batch of posteriors as for the step entry point.
+ * io as for the funnel entry point, with rows of stride params->dim;
+ * new_logdensity_grad is recomputed at the chosen state.
+ * params->num_integration_steps and num_transitions are not used. */
 int gjx_rmhmc_fisher_logistic_nuts(const gjx_logistic_params* params, const gjx_logistic_data* data,
                                    int32_t max_num_doublings, int32_t stopping_criterion, const gjx_nuts_io* io,
                                    void* stream);

@@ -131,8 +131,9 @@ def test_logistic_feature_capacity_per_path():
 
 
 def test_mmala_and_logistic_nuts_entry_points_validate_on_the_host():
-    """MMALA and the logistic NUTS entry points: float64 only, D <= 64, a
-    positive step size for MMALA, required buffers, z and u given together."""
+    """MMALA and the logistic NUTS entry points: MMALA and NUTS-LMC float64
+    only (NUTS-RMHMC also float32, with the tuned path's packed table), D <= 64,
+    a positive step size for MMALA, required buffers, z and u given together."""
     lib = _lib.load()
 
     def params(D, dtype=_lib.GJX_F64, step=0.1):
@@ -163,6 +164,16 @@ def test_mmala_and_logistic_nuts_entry_points_validate_on_the_host():
                                                                         _lib.LmcNutsIO)):
         assert fn(params(65), data(65), 5, _lib.GJX_UTURN["euc"], io_t(), None) == -2
         assert fn(params(8), data(8), 5, 7, io_t(), None) == -1  # unknown criterion
+    nuts_rm = lib.gjx_rmhmc_fisher_logistic_nuts
+    assert nuts_rm(params(8, _lib.GJX_F32), data(8), 5, _lib.GJX_UTURN["euc"], _lib.NutsIO(), None) == -1
+    assert "packed" in _lib.last_error()
+    d32 = data(8)
+    d32.packed, d32.workspace, d32.workspace_bytes = 1, 1, 1 << 40
+    assert nuts_rm(params(8, _lib.GJX_F32), d32, 5, _lib.GJX_UTURN["euc"], _lib.NutsIO(), None) == -1
+    assert "position" in _lib.last_error()
+    assert lib.gjx_lmc_fisher_logistic_nuts(params(8, _lib.GJX_F32), d32, 5, _lib.GJX_UTURN["euc"], _lib.LmcNutsIO(),
+                                            None) == -1
+    assert "float64" in _lib.last_error()
 
 
 def test_dense_metric_entry_points_validate_on_the_host():

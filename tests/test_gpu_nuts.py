@@ -511,3 +511,100 @@ def test_tuned_persistent_nuts_equals_step_launches():
             assert torch.equal(res.samples[0], s.position)
         assert err < 1e-3
     assert res.samples.dtype == torch.float32
+
+
+@pytest.mark.parametrize("N,D,C,eps,M", [(256, 8, 32, 0.1, 5), (1024, 40, 32, 0.05, 5), (2048, 64, 16, 0.05, 4)])
+def test_tuned_logistic_nuts_tracks_float64(N, D, C, eps, M):
+    """The tuned logistic form (float32 state, split-bf16 MFMA, float64 energies
+    and tree weights) on the same host draws as the float64 kernel, which
+    equals the oracle at 1e-10: on at least 90 % of the chains the step count is
+    the same and the new position agrees to 1e-3 relative (a float32 weight can
+    flip a progressive-sampling or U-turn decision on the odd chain)."""
+    from geomjax_amd.synthetic import make_logistic
+
+    print(f"case tuned logistic NUTS N={N} D={D} C={C} eps={eps} M={M}", flush=True)
+    X, y, _ = make_logistic(N, D, 30 + D)
+    model = gj.LogisticRegression(X, y, 100.0)
+    kern = nuts_kernel(functools.partial(implicit_midpoint, max_iters=6))
+    q0 = (0.3 * seeded_normal(160 + D, C, D)).float().double()
+    z = seeded_normal(161 + D, C, D).float().double()
+    ud, um, us = seeded_uniform(162, C, M), seeded_uniform(163, C, M), seeded_uniform(164, C, 2 ** M - 1)
+    noise = lambda dt: NUTSNoise(z.to(DEV, dt), ud.to(DEV), um.to(DEV), us.to(DEV))
+    s64, i64 = kern(noise(torch.float64), gj.rmhmc.init(q0.to(DEV), model), model, eps, gj.FisherMetric(), M)
+    s32, i32 = kern(noise(torch.float32), gj.rmhmc.init(q0.to(DEV).float(), model), model, eps, gj.FisherMetric(), M)
+    torch.cuda.synchronize()
+    assert s32.position.dtype == torch.float32 and s32.logdensity.dtype == torch.float64
+    same = (i32.num_integration_steps == i64.num_integration_steps).cpu()
+    per = ((s32.position.double() - s64.position).norm(dim=1) / s64.position.norm(dim=1).clamp_min(1e-300)).cpu()
+    close = same & (per < 1e-3)
+    frac = float(close.double().mean())
+    print(f"  steps {i64.num_integration_steps.tolist()}", flush=True)
+    print(f"  same steps on {float(same.double().mean()):.3f}, same steps and position within 1e-3 on {frac:.3f}; "
+          f"median position error {float(per[same].median()) if bool(same.any()) else float('nan'):.1e}; "
+          f"energy diff there {(i32.energy - i64.energy).abs().cpu()[close].max().item():.1e}", flush=True)
+    assert torch.isfinite(s32.position).all()
+    assert frac >= 0.9
+    assert (i32.energy - i64.energy).abs().cpu()[close].max().item() < 1e-2
+
+
+def test_tuned_logistic_nuts_batch_equals_each_posterior_alone():
+    from geomjax_amd.synthetic import make_logistic
+
+    shapes = [(300, 8), (400, 40)]
+    models = [gj.LogisticRegression(*make_logistic(n, d, 50 + i)[:2], 100.0) for i, (n, d) in enumerate(shapes)]
+    cpp, M = 4, 4
+    batch = gj.LogisticBatch(models, chains_per_posterior=cpp)
+    C = cpp * len(models)
+    q = torch.zeros(C, batch.dim)
+    z = torch.zeros(C, batch.dim)
+    for i, (_, d) in enumerate(shapes):
+        q[i * cpp:(i + 1) * cpp, :d] = 0.3 * seeded_normal(170 + i, cpp, d)
+        z[i * cpp:(i + 1) * cpp, :d] = seeded_normal(175 + i, cpp, d)
+    ud, um, us = seeded_uniform(180, C, M), seeded_uniform(181, C, M), seeded_uniform(182, C, 2 ** M - 1)
+    kern = nuts_kernel(functools.partial(implicit_midpoint, max_iters=6))
+    sb = gj.rmhmc.init(q.float().to(DEV), batch)
+    sb1, ib = kern(NUTSNoise(z.float().to(DEV), ud.to(DEV), um.to(DEV), us.to(DEV)), sb, batch, 0.1,
+                   gj.FisherMetric(), M)
+    for i, (_, d) in enumerate(shapes):
+        rows = slice(i * cpp, (i + 1) * cpp)
+        s = gj.rmhmc.init(q[rows, :d].float().to(DEV), models[i])
+        s1, info = kern(NUTSNoise(z[rows, :d].float().to(DEV), ud[rows].to(DEV), um[rows].to(DEV), us[rows].to(DEV)),
+                        s, models[i], 0.1, gj.FisherMetric(), M)
+        torch.cuda.synchronize()
+        err = rel_err(sb1.position[rows, :d].double(), s1.position.double())
+        print(f"posterior {i} (D={d}): steps {info.num_integration_steps.tolist()}, rel err {err:.1e}", flush=True)
+        assert torch.equal(ib.num_integration_steps[rows], info.num_integration_steps)
+        assert err < 1e-6
+        assert torch.all(sb1.position[rows, d:] == 0)
+
+
+def test_tuned_logistic_nuts_chains_track_float64_chains():
+    """Paired with the float64 kernel on the device PRNG over ten transitions:
+    chain means within 5 standard errors, mean acceptance statistics within
+    0.02 and mean step counts within 5 %."""
+    from geomjax_amd.synthetic import make_logistic
+
+    N, D, C, eps, M, T = 512, 8, 1024, 0.15, 5, 10
+    X, y, _ = make_logistic(N, D, 61)
+    model = gj.LogisticRegression(X, y, 100.0)
+    q0 = 0.1 * seeded_normal(190, C, D)
+    alg = gj.nuts(model, eps, gj.FisherMetric(), max_num_doublings=M,
+                  integrator=functools.partial(implicit_midpoint, max_iters=6))
+    out = {}
+    for dt in (torch.float64, torch.float32):
+        s = alg.init(q0.to(DEV).to(dt))
+        accs, steps = [], []
+        for t in range(T):
+            s, info = alg.step(gr.key(700 + t), s)
+            accs.append(float(info.acceptance_rate.mean()))
+            steps.append(float(info.num_integration_steps.double().mean()))
+        torch.cuda.synchronize()
+        out[dt] = (s.position.double().cpu(), sum(accs) / T, sum(steps) / T)
+    (p64, a64, n64), (p32, a32, n32) = out[torch.float64], out[torch.float32]
+    se = p64.std(0) / C ** 0.5
+    dm = (p32.mean(0) - p64.mean(0)).abs()
+    print(f"acceptance {a64:.4f} vs {a32:.4f}; steps {n64:.2f} vs {n32:.2f}; mean diff / SE max "
+          f"{(dm / se).max().item():.2f}", flush=True)
+    assert abs(a64 - a32) < 0.02
+    assert abs(n64 - n32) < 0.05 * n64
+    assert torch.all(dm < 5 * se)
