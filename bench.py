@@ -611,12 +611,13 @@ def nuts_forms(K, W, dev, static_plain=None, chunk=10, tuned=False):
     return res
 
 
-def logistic_nuts_forms(K, W, dev, chains=1024, integrator="rmhmc"):
+def logistic_nuts_forms(K, W, dev, chains=1024, integrator="rmhmc", tuned=False):
     """NUTS on the headline posterior (logistic regression 2048 x 64, Fisher
     metric, float64, one launch per transition) with the RMHMC integrator (the
     headline step size and fixed-point cap) or the general-metric LMC integrator
     (``integrator="lmc"``, same step size): max_num_doublings 5, "euc", device
-    PRNG.  Reports integration steps/s and min-ESS/s."""
+    PRNG; with ``tuned`` (RMHMC only) the float32 state on the tuned kernel.
+    Reports integration steps/s and min-ESS/s."""
     import torch
 
     import geomjax_amd as gj
@@ -626,7 +627,9 @@ def logistic_nuts_forms(K, W, dev, chains=1024, integrator="rmhmc"):
     M = 5
     X, y, _ = make_logistic(N_ROWS, D_LOG, DATA_SEED)
     model = gj.LogisticRegression(X, y, PRIOR, device=dev)
+    dt = torch.float32 if tuned else torch.float64
     q_init = 0.1 * torch.randn(chains, D_LOG, generator=torch.Generator().manual_seed(INIT_SEED), dtype=torch.float64)
+    q_init = q_init.to(dt)
     if integrator == "lmc":
         from geomjax_amd.lmcmc import init as lmc_init
         from geomjax_amd.lmcmc_nuts import build_kernel as lnuts_kernel
@@ -637,10 +640,11 @@ def logistic_nuts_forms(K, W, dev, chains=1024, integrator="rmhmc"):
         from geomjax_amd.rmhmc import implicit_midpoint
 
         kern = nuts_kernel(functools.partial(implicit_midpoint, max_iters=MAXIT_LOG))
-        kname, s = "logistic_nuts_f64_kernel", gj.rmhmc.init(q_init.to(dev), model)
+        kname = "logistic_nuts_f32_kernel<2>" if tuned else "logistic_nuts_f64_kernel"
+        s = gj.rmhmc.init(q_init.to(dev), model)
     for t in range(W):
         s, _ = kern(gr.PRNGKey(INIT_SEED + 11, t), s, model, EPS_LOG, gj.FisherMetric(), M)
-    samples = torch.empty(K, chains, D_LOG, dtype=torch.float64, device=dev)
+    samples = torch.empty(K, chains, D_LOG, dtype=dt, device=dev)
     steps, accs = [], []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -656,9 +660,10 @@ def logistic_nuts_forms(K, W, dev, chains=1024, integrator="rmhmc"):
     del samples
     res = {"value": total_steps / wall, "unit": "integration-steps/s", "min_ess_per_sec": ess / wall, "min_ess": ess,
            "mean_integration_steps": total_steps / (K * chains), "acceptance_rate": float(torch.stack(accs).mean()),
-           "ms_per_transition": wall / K * 1e3, "chains": chains, "dtype": "f64", "max_num_doublings": M, "steps": K,
+           "ms_per_transition": wall / K * 1e3, "chains": chains,
+           "dtype": "f32 state, f64 energies" if tuned else "f64", "max_num_doublings": M, "steps": K,
            "kernel": kname}
-    log(f"logistic NUTS ({integrator}): {res['value']:.3e} integration-steps/s, {res['mean_integration_steps']:.1f} steps per "
+    log(f"logistic NUTS ({integrator}, {res['dtype']}): {res['value']:.3e} integration-steps/s, {res['mean_integration_steps']:.1f} steps per "
         f"transition, min-ESS/s {res['min_ess_per_sec']:.3e}")
     return res
 
@@ -989,6 +994,7 @@ def main():
         return
     if args.only == "logistic_nuts":
         print(json.dumps(logistic_nuts_forms(K, W, dev)), flush=True)
+        print(json.dumps(logistic_nuts_forms(K, W, dev, tuned=True)), flush=True)
         return
     if args.only == "logistic_nuts_lmc":
         print(json.dumps(logistic_nuts_forms(K, W, dev, integrator="lmc")), flush=True)
@@ -1042,6 +1048,10 @@ def main():
                                                                extra["funnel_d100_8192_chains"].get("plain"),
                                                                tuned=True)
         extra["logistic_nuts_1024_chains"] = logistic_nuts_forms(10, 2, dev)
+        extra["logistic_nuts_1024_chains_f32"] = logistic_nuts_forms(10, 2, dev, tuned=True)
+        extra["logistic_nuts_1024_chains_f32"]["tuned_vs_f64"] = {
+            k: extra["logistic_nuts_1024_chains_f32"][k] / extra["logistic_nuts_1024_chains"][k]
+            for k in ("value", "min_ess_per_sec")}
         extra["logistic_lmc_1024_chains"] = logistic_lmc_forms(20, 5, dev)
         extra["logistic_lmc_1024_chains_f32"] = logistic_lmc_forms(20, 5, dev, tuned=True)
         extra["logistic_lmc_1024_chains_f32"]["tuned_vs_f64"] = {
