@@ -901,7 +901,7 @@ def tangent_forms(dev, repeats=3):
     one tangent launch (one transition carried with its forward derivative) against one
     primal float64 transition, on the tuning workloads: funnel D=100 x 4096 chains and
     logistic 2048 x 64 x 1024 chains, 32 integrator steps, as many fixed-point
-    iterations as the solver runs."""
+    iterations as the solver runs; and the float32 fast path of each."""
     import torch
 
     import geomjax_amd as gj
@@ -938,18 +938,18 @@ def tangent_forms(dev, repeats=3):
                      "tangent_over_primal": t_tan / t_pri, "num_integration_steps": 32,
                      "mean_objective": float(r.objective.mean()), "mean_d_objective": r.d_objective.mean(0).tolist()}
         log(f"tangent {name}: {t_tan * 1e3:.2f} ms vs primal {t_pri * 1e3:.2f} ms ({t_tan / t_pri:.2f}x)")
-        if isinstance(model, gj.NealFunnel):  # the float32 fast path on the same positions
-            s32 = gj.rmhmc.init(q.to(dev, torch.float32), model)
-            esjd_acceptance(gr.PRNGKey(1, 0), s32, model, eps, metric, 32, integrator=integ)
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            for i in range(repeats):
-                r32 = esjd_acceptance(gr.PRNGKey(2, i), s32, model, eps, metric, 32, integrator=integ)
-            torch.cuda.synchronize()
-            t32 = (time.perf_counter() - t0) / repeats
-            out[name]["tangent_f32_fast_path_ms"] = t32 * 1e3
-            out[name]["mean_d_objective_f32"] = r32.d_objective.mean(0).tolist()
-            log(f"tangent {name} float32 fast path: {t32 * 1e3:.2f} ms")
+        # the float32 fast path on the same positions
+        s32 = gj.rmhmc.init(q.to(dev, torch.float32), model)
+        esjd_acceptance(gr.PRNGKey(1, 0), s32, model, eps, metric, 32, integrator=integ)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(repeats):
+            r32 = esjd_acceptance(gr.PRNGKey(2, i), s32, model, eps, metric, 32, integrator=integ)
+        torch.cuda.synchronize()
+        t32 = (time.perf_counter() - t0) / repeats
+        out[name]["tangent_f32_fast_path_ms"] = t32 * 1e3
+        out[name]["mean_d_objective_f32"] = r32.d_objective.mean(0).tolist()
+        log(f"tangent {name} float32 fast path: {t32 * 1e3:.2f} ms")
     return out
 
 
@@ -961,7 +961,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the 1024-chain and funnel comparisons")
-    ap.add_argument("--only", choices=["headline", "uneven4096", "funnel", "funnel_nuts", "logistic_nuts", "logistic_nuts_lmc", "lmc_alpha2", "logistic_lmc", "mmala",
+    ap.add_argument("--only", choices=["headline", "tangent", "uneven4096", "funnel", "funnel_nuts", "logistic_nuts", "logistic_nuts_lmc", "lmc_alpha2", "logistic_lmc", "mmala",
                              "uneven", "dense_lmc", "dense_rmhmc"],
                     help="run one secondary workload alone and print its JSON (for profiling it)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -1016,6 +1016,9 @@ def main():
         return
     if args.only == "mmala":
         print(json.dumps(mmala_forms(K, W, dev)), flush=True)
+        return
+    if args.only == "tangent":
+        print(json.dumps(tangent_forms(dev)), flush=True)
         return
     if args.only == "logistic_lmc":
         print(json.dumps(logistic_lmc_forms(K, W, dev)), flush=True)

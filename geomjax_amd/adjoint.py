@@ -50,7 +50,9 @@ def esjd_acceptance(rng_key, state, logdensity_fn, step_size, metric_fn, num_int
     Funnel + MongeMetric: both columns; a float64 state runs the check path, a
     float32 state the fast path (float32 trajectory and tangents, float64
     energies, as the tuned sampler).  LogisticRegression / LogisticBatch +
-    FisherMetric: column 0 (column 1 is zero), float64."""
+    FisherMetric (D <= 64): column 0 (column 1 is zero); a float64 state runs
+    the check path, a float32 state the fast path (the tuned kernel's float32
+    state and split-bf16 MFMA passes with tangents, float64 energies)."""
     funnel = isinstance(logdensity_fn, NealFunnel) and isinstance(metric_fn, MongeMetric)
     logistic = isinstance(logdensity_fn, (LogisticRegression, LogisticBatch)) and isinstance(metric_fn, FisherMetric)
     if not (funnel or logistic):
@@ -59,8 +61,6 @@ def esjd_acceptance(rng_key, state, logdensity_fn, step_size, metric_fn, num_int
     q = state.position
     if q.dim() != 2 or q.dtype not in (torch.float64, torch.float32) or not q.is_cuda:
         raise ValueError("position must be a [C, D] float64 or float32 tensor on a HIP device")
-    if logistic and q.dtype != torch.float64:
-        raise ValueError("the logistic tangent runs in float64 (pass a float64 state)")
     q = q.contiguous()
     C, D = q.shape
     dev = q.device
@@ -90,8 +90,9 @@ def esjd_acceptance(rng_key, state, logdensity_fn, step_size, metric_fn, num_int
             raise ValueError(f"positions have {D} coordinates, the model {logdensity_fn.dim}")
         params = logistic_params(logdensity_fn, C, L=num_integration_steps, step_size=step_size,
                                  max_iters=solver["max_iters"], convergence_tol=solver["convergence_tol"],
-                                 divergence_tol=solver["divergence_tol"], seed=seed, counter=counter)
-        data = logdensity_fn.data(dtype=torch.float64, num_chains=C)
+                                 divergence_tol=solver["divergence_tol"], seed=seed, counter=counter,
+                                 dtype=q.dtype)
+        data = logdensity_fn.data(dtype=q.dtype, num_chains=C)
         rc = _lib.load().gjx_rmhmc_fisher_logistic_tangent(params, data, io, _lib.stream_handle(dev))
         _lib.check(rc, "gjx_rmhmc_fisher_logistic_tangent")
     return out

@@ -1125,7 +1125,6 @@ int gjx_logistic_value_and_grad(const gjx_logistic_params* params, const gjx_log
 
 int gjx_rmhmc_fisher_logistic_tangent(const gjx_logistic_params* params, const gjx_logistic_data* data,
                                       const gjx_tangent_io* io, void* stream) {
-  if (int rc0 = require_f64(params, "gjx_rmhmc_fisher_logistic_tangent")) return rc0;
   gjx::LogisticArgs a;
   int rc = check_logistic(params, data, false, &a, gjx::kLogisticMaxDLmc);
   if (rc != GJX_OK) return rc;
@@ -1140,7 +1139,9 @@ int gjx_rmhmc_fisher_logistic_tangent(const gjx_logistic_params* params, const g
   a.z = io->z;
   gjx::TangentOut o{io->objective, io->acceptance, io->sq_jump, io->d_objective, io->d_acceptance,
                     io->d_sq_jump, io->num_fevals};
-  hipError_t e = gjx::launch_logistic_rmhmc_tangent(a, o, (hipStream_t)stream);
+  hipError_t e = params->dtype == GJX_F32
+                     ? gjx::launch_logistic_rmhmc_tangent_f32(a, o, data->max_features, (hipStream_t)stream)
+                     : gjx::launch_logistic_rmhmc_tangent(a, o, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "gjx_rmhmc_fisher_logistic_tangent launch");
   return GJX_OK;
 }

@@ -140,6 +140,9 @@ hipError_t launch_logistic_rmhmc_f64(const LogisticArgs& a, hipStream_t stream);
 hipError_t launch_logistic_mmala_f64(const LogisticArgs& a, hipStream_t stream);
 hipError_t launch_logistic_mmala_init(const LogisticArgs& a, hipStream_t stream);
 hipError_t launch_logistic_rmhmc_tangent(const LogisticArgs& a, const TangentOut& o, hipStream_t stream);
+// tangent fast path: float32 state, split-bf16 MFMA, D <= 64 (logistic_grad_fast.hip)
+hipError_t launch_logistic_rmhmc_tangent_f32(const LogisticArgs& a, const TangentOut& o, int max_features,
+                                             hipStream_t stream);
 // NUTS with the RMHMC integrator, float64, D <= 64 (logistic_nuts.hip)
 hipError_t launch_logistic_nuts_f64(const LogisticArgs& a, const NutsArgs& n, hipStream_t stream);
 // NUTS with the RMHMC integrator, tuned form (float32 state), D <= 64 (logistic_nuts_fast.hip)

@@ -314,3 +314,43 @@ def test_funnel_fast_path_float32_matches_float64(D, C, eps, alpha2, L, max_iter
     assert e_obj < 1e-4
     assert e_d < 5e-3
     assert e_acc < 5e-3
+
+
+@pytest.mark.parametrize("N,D,C,eps,L", [(512, 8, 32, 0.1, 8), (1024, 40, 32, 0.1, 16), (2048, 64, 32, 0.1, 32)])
+def test_logistic_fast_path_float32_matches_float64(N, D, C, eps, L):
+    """The logistic float32 fast path (the tuned kernel's float32 state and
+    split-bf16 MFMA passes with tangents, float64 energies) against the float64
+    check path (pinned above against autograd through the oracle and central
+    differences) on the same float32-representable positions and normals, up
+    to the tangent configuration (2048 x 64, 32 steps): objective 2e-3 and its
+    derivative, the acceptance's derivative and the squared jump's derivative
+    1e-2, relative over the chains (observed at 2048 x 64: 7.5e-4 and up to
+    2.2e-3; the float32 fixed-point norms stop some steps one iteration earlier
+    or later than float64, so the two differentiate slightly different
+    executed computations)."""
+    from geomjax_amd.synthetic import make_logistic
+
+    print(f"case logistic N={N} D={D} C={C} L={L} float32 fast path", flush=True)
+    X, y, _ = make_logistic(N, D, 70 + D)
+    model = gj.LogisticRegression(X, y, 100.0)
+    q0 = (0.3 * seeded_normal(310 + D, C, D)).float().double()
+    z = seeded_normal(311 + D, C, D).float().double()
+    integ = functools.partial(implicit_midpoint, max_iters=6)
+    u0 = torch.zeros(C, device=DEV)
+    r64 = esjd_acceptance(gr.HostNoise(z.to(DEV), u0), gj.rmhmc.init(q0.to(DEV), model), model, eps,
+                          gj.FisherMetric(), L, integrator=integ)
+    r32 = esjd_acceptance(gr.HostNoise(z.to(DEV).float(), u0), gj.rmhmc.init(q0.to(DEV).float(), model), model, eps,
+                          gj.FisherMetric(), L, integrator=integ)
+    torch.cuda.synchronize()
+    errs = {"objective": rel_err(r32.objective, r64.objective),
+            "d objective": rel_err(r32.d_objective[:, 0], r64.d_objective[:, 0]),
+            "d acceptance": rel_err(r32.d_acceptance[:, 0], r64.d_acceptance[:, 0]),
+            "d sq jump": rel_err(r32.d_sq_jump[:, 0], r64.d_sq_jump[:, 0])}
+    print("  " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()) +
+          f"; acceptance {float(r64.acceptance.mean()):.4f}; fevals {r32.num_fevals.tolist()[:4]} vs "
+          f"{r64.num_fevals.tolist()[:4]}", flush=True)
+    assert r32.objective.dtype == torch.float64
+    assert torch.all(r32.d_objective[:, 1] == 0)
+    assert errs["objective"] < 2e-3
+    for k in ("d objective", "d acceptance", "d sq jump"):
+        assert errs[k] < 1e-2, k
