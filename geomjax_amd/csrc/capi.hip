@@ -407,7 +407,6 @@ int gjx_lmcmonge_funnel_nuts(const gjx_funnel_params* params, int32_t max_num_do
   int lpc = 0, epl = 0;
   int rc = check_funnel_params(params, false, &lpc, &epl);
   if (rc != GJX_OK) return rc;
-  GJX_REQUIRE(params->dtype == GJX_F64, "the NUTS kernel runs in float64");
   GJX_REQUIRE((params->flags & ~3) == 0, "unknown flags 0x%x", params->flags);
   if (params->dim - 1 > 256) return fail(GJX_EUNSUPPORTED, "dim %d > 257 for the NUTS kernel", params->dim);
   GJX_REQUIRE(max_num_doublings >= 1 && max_num_doublings <= 12, "max_num_doublings must be in [1, 12] (got %d)",
@@ -452,7 +451,7 @@ int gjx_lmcmonge_funnel_nuts(const gjx_funnel_params* params, int32_t max_num_do
   n.num_steps = io->num_integration_steps;
   n.left_position = io->leftmost_position;
   n.right_position = io->rightmost_position;
-  hipError_t e = gjx::launch_funnel_nuts_lmc(a, n, (hipStream_t)stream);
+  hipError_t e = gjx::launch_funnel_nuts_lmc(a, n, params->dtype, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "gjx_lmcmonge_funnel_nuts launch");
   return GJX_OK;
 }

@@ -61,7 +61,7 @@ struct FunnelLmcArgs {
 
 hipError_t launch_funnel_lmc(const FunnelLmcArgs& a, int dtype, int lpc, int epl, hipStream_t stream);
 // NUTS with the LMC-Monge integrator, float64, one wave per chain (funnel_nuts.hip)
-hipError_t launch_funnel_nuts_lmc(const FunnelLmcArgs& a, const NutsArgs& n, hipStream_t stream);
+hipError_t launch_funnel_nuts_lmc(const FunnelLmcArgs& a, const NutsArgs& n, int dtype, hipStream_t stream);
 // forward-mode derivative of the proposal position in alpha2 (float64)
 hipError_t launch_funnel_lmc_alpha2_tangent(const FunnelLmcArgs& a, int lpc, int epl, hipStream_t stream);
 

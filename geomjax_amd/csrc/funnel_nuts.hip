@@ -50,11 +50,11 @@ struct RState {
 };
 
 // LMC-Monge chain state: the position's scalars and the carried volume adjustment
-template <int EPL>
+template <int EPL, class T = double>
 struct LState {
-  double x[EPL], px[EPL], ux[EPL];
-  double v, pv, uv;
-  lmc_ops::PosScalars<double> P;
+  T x[EPL], px[EPL], ux[EPL];
+  T v, pv, uv;
+  lmc_ops::PosScalars<T> P;
   double J;
 };
 
@@ -638,89 +638,103 @@ __global__ void __launch_bounds__(64) funnel_nuts_kernel(const FunnelArgs a, con
 // ---------------------------------------------------------------------------
 // NUTS with the LMC-Monge integrator (geomjax/lmcmonge/nuts.py:80-207)
 // ---------------------------------------------------------------------------
-template <int EPL>
+template <int EPL, class T>
 __global__ void __launch_bounds__(64) funnel_nuts_lmc_kernel(const FunnelLmcArgs a, const NutsArgs nu) {
   using namespace lmc_ops;
-  extern __shared__ __align__(16) double ck[];
+  using LS = LState<EPL, T>;
+  extern __shared__ __align__(16) double ck_raw[];
+  T* const ck = reinterpret_cast<T*>(ck_raw);
   const int c = blockIdx.x;
   if (c >= a.C) return;
   const int l = threadIdx.x;
   const int D = a.D, Dx = a.D - 1;
   const int j0 = l * EPL;
-  const double a2 = a.alpha2;
-  const double alpha = sqrt(a.alpha2);
-  const double inv_s2 = 1.0 / (a.sigma * a.sigma);
-  const double halfDx = 0.5 * Dx;
+  // the state's arithmetic in T (float32: the tuned form), energies in float64
+  const T a2 = (T)a.alpha2;
+  const T alpha = (T)sqrt(a.alpha2);
+  const T inv_s2 = (T)(1.0 / (a.sigma * a.sigma));
+  const T halfDx = (T)(0.5 * Dx);
   const bool exact_sm = a.exact_sm != 0;
   const NutsUniform uniform{c, a.counter, (uint32_t)a.seed, (uint32_t)(a.seed >> 32)};
   const size_t base = (size_t)c * D;
 
   // diagonal inverse mass (the reference's inverse_mass_matrix)
-  double mi[EPL];
+  T mi[EPL];
 #pragma unroll
-  for (int k = 0; k < EPL; ++k) mi[k] = j0 + k < Dx ? ld<double>(a.inv_mass, j0 + k) : 1.0;
-  const double miv = ld<double>(a.inv_mass, Dx);
+  for (int k = 0; k < EPL; ++k) mi[k] = j0 + k < Dx ? ld<T>(a.inv_mass, j0 + k) : T(1);
+  const T miv = ld<T>(a.inv_mass, Dx);
   double sum_log_mi;
   {
     double s = 0.0;
 #pragma unroll
     for (int k = 0; k < EPL; ++k)
-      if (j0 + k < Dx) s += log(mi[k]);
-    sum_log_mi = group_sum<64>(s) + log(miv);
+      if (j0 + k < Dx) s += log((double)mi[k]);
+    sum_log_mi = group_sum<64>(s) + log((double)miv);
   }
 
   // momentum = M v + alpha2 L (v . g) g (the reference's metric_vector_product with its
   // extra L, geomjax/lmcmonge/metrics.py:243-255; exact: alpha2 (v . g) g) from the
   // velocity and the position's scalars
-  auto set_momentum = [&](LState<EPL>& s) {
-    double xv = 0.0;
+  auto set_momentum = [&](LS& s) {
+    T xv = T(0);
 #pragma unroll
     for (int k = 0; k < EPL; ++k) xv += s.x[k] * s.ux[k];
     xv = group_sum<64>(xv);
-    const double vg = -s.P.e * xv + s.uv * s.P.gv;
-    const double coef = a.exact_momentum ? a2 * vg : a2 * s.P.L * vg;
+    const T vg = -s.P.e * xv + s.uv * s.P.gv;
+    const T coef = a.exact_momentum ? a2 * vg : a2 * s.P.L * vg;
 #pragma unroll
-    for (int k = 0; k < EPL; ++k) s.px[k] = j0 + k < Dx ? s.ux[k] / mi[k] + coef * (-s.P.e * s.x[k]) : 0.0;
+    for (int k = 0; k < EPL; ++k) s.px[k] = j0 + k < Dx ? s.ux[k] / mi[k] + coef * (-s.P.e * s.x[k]) : T(0);
     s.pv = s.uv / miv + coef * s.P.gv;
   };
-  // energy -l + K - J (lmcmonge_energy, geomjax/lmcmonge/metrics.py:267-284)
-  auto energy = [&](const LState<EPL>& s) -> double {
-    return -funnel_logp(s.v, s.P.Sx, a.sigma, Dx) +
-           lmc_kinetic<double, 64, EPL>(s.P, s.x, s.ux, s.uv, mi, miv, j0, Dx, sum_log_mi, a2) - s.J;
+  // energy -l + K - J (lmcmonge_energy, geomjax/lmcmonge/metrics.py:267-284); the log
+  // density from the widened coordinates, the kinetic energy's sums in float64
+  auto energy = [&](const LS& s) -> double {
+    double sxd;
+    if constexpr (sizeof(T) == sizeof(double)) {
+      sxd = s.P.Sx;
+    } else {
+      double d1 = 0.0;
+#pragma unroll
+      for (int k = 0; k < EPL; ++k) d1 += (double)s.x[k] * (double)s.x[k];
+      sxd = group_sum<64>(d1);
+    }
+    return -funnel_logp((double)s.v, sxd, a.sigma, Dx) +
+           lmc_kinetic<T, 64, EPL>(s.P, s.x, s.ux, s.uv, mi, miv, j0, Dx, sum_log_mi, a.alpha2) - s.J;
   };
   // one lan_integrator step of size hstep = direction * eps (integrators.py:52-156)
-  auto one_step = [&](LState<EPL>& s, double hstep) {
+  auto one_step = [&](LS& s, double hstep) {
     double dJ = 0.0;
-    lmc_step<double, 64, EPL>(s.P, s.x, s.v, mi, s.ux, s.uv, dJ, hstep, 0.5 * hstep, a2, inv_s2, halfDx, miv,
-                              exact_sm);
+    lmc_step<T, 64, EPL>(s.P, s.x, s.v, mi, s.ux, s.uv, dJ, (T)hstep, (T)(0.5 * hstep), a2, inv_s2, halfDx, miv,
+                         exact_sm);
     s.J += dJ;
     set_momentum(s);
   };
 
   // ---- state, velocity draw v0 = chol(M^-1 - alpha2 dl dl^T) z, momentum, H0 (lmc.py:151-210) ----
-  LState<EPL> s0;
+  LS s0;
 #pragma unroll
-  for (int k = 0; k < EPL; ++k) s0.x[k] = j0 + k < Dx ? ld<double>(a.q_in, base + j0 + k) : 0.0;
-  s0.v = ld<double>(a.q_in, base + Dx);
+  for (int k = 0; k < EPL; ++k) s0.x[k] = j0 + k < Dx ? ld<T>(a.q_in, base + j0 + k) : T(0);
+  s0.v = ld<T>(a.q_in, base + Dx);
   s0.J = a.J_in[c];
-  double loc_inc[EPL];
-  s0.P = position_scalars<double, 64, EPL>(s0.x, s0.v, mi, loc_inc, l, a2, inv_s2, halfDx, miv);
-  double z[EPL], zv, uacc;
-  transition_noise<double, EPL>(a, (size_t)c, 0, j0, Dx, true, z, zv, uacc);
-  draw_velocity<double, 64, EPL>(s0.P, s0.x, mi, loc_inc, z, zv, a2, alpha, miv, l, j0, Dx, s0.ux, s0.uv);
+  T loc_inc[EPL];
+  s0.P = position_scalars<T, 64, EPL>(s0.x, s0.v, mi, loc_inc, l, a2, inv_s2, halfDx, miv);
+  T z[EPL], zv;
+  double uacc;
+  transition_noise<T, EPL>(a, (size_t)c, 0, j0, Dx, true, z, zv, uacc);
+  draw_velocity<T, 64, EPL>(s0.P, s0.x, mi, loc_inc, z, zv, a2, alpha, miv, l, j0, Dx, s0.ux, s0.uv);
   set_momentum(s0);
   if (a.velocity != nullptr) {  // NUTSInfo.momentum holds the initial velocity (lmcmonge/nuts.py:386-396)
 #pragma unroll
     for (int k = 0; k < EPL; ++k)
-      if (j0 + k < Dx) st<double>(a.velocity, base + j0 + k, s0.ux[k]);
-    if (l == 0) st<double>(a.velocity, base + Dx, s0.uv);
+      if (j0 + k < Dx) st<T>(a.velocity, base + j0 + k, s0.ux[k]);
+    if (l == 0) st<T>(a.velocity, base + Dx, s0.uv);
   }
   // H0 with the chain's log density and volume adjustment as given
-  const double H0 =
-      -a.logp_in[c] + lmc_kinetic<double, 64, EPL>(s0.P, s0.x, s0.ux, s0.uv, mi, miv, j0, Dx, sum_log_mi, a2) - s0.J;
+  const double H0 = -a.logp_in[c] +
+                    lmc_kinetic<T, 64, EPL>(s0.P, s0.x, s0.ux, s0.uv, mi, miv, j0, Dx, sum_log_mi, a.alpha2) - s0.J;
 
-  LState<EPL> left, right;
-  Prop<EPL> prop;
+  LS left, right;
+  Prop<EPL, T> prop;
   const TreeOut t = nuts_tree<EPL>(s0, H0, one_step, energy, uniform, nu, a.eps, a.div_threshold, ck, c, left, right,
                                    prop);
   write_outputs<EPL>(prop, left.x, left.v, right.x, right.v, t, c, D, a.sigma, a.q_out, a.logp_out, a.g_out, a.J_out,
@@ -768,12 +782,18 @@ hipError_t launch_funnel_nuts(const FunnelArgs& a, const NutsArgs& n, int dtype,
   return launch_one_nuts(funnel_nuts_kernel<4, double>, a, n, smem, stream);
 }
 
-hipError_t launch_funnel_nuts_lmc(const FunnelLmcArgs& a, const NutsArgs& n, hipStream_t stream) {
+hipError_t launch_funnel_nuts_lmc(const FunnelLmcArgs& a, const NutsArgs& n, int dtype, hipStream_t stream) {
   const int epl = nuts_epl(a.D);
+  if (dtype == GJX_F32) {
+    const size_t smem = nuts_smem(n, epl, sizeof(float));
+    if (epl == 1) return launch_one_nuts(funnel_nuts_lmc_kernel<1, float>, a, n, smem, stream);
+    if (epl == 2) return launch_one_nuts(funnel_nuts_lmc_kernel<2, float>, a, n, smem, stream);
+    return launch_one_nuts(funnel_nuts_lmc_kernel<4, float>, a, n, smem, stream);
+  }
   const size_t smem = nuts_smem(n, epl);
-  if (epl == 1) return launch_one_nuts(funnel_nuts_lmc_kernel<1>, a, n, smem, stream);
-  if (epl == 2) return launch_one_nuts(funnel_nuts_lmc_kernel<2>, a, n, smem, stream);
-  return launch_one_nuts(funnel_nuts_lmc_kernel<4>, a, n, smem, stream);
+  if (epl == 1) return launch_one_nuts(funnel_nuts_lmc_kernel<1, double>, a, n, smem, stream);
+  if (epl == 2) return launch_one_nuts(funnel_nuts_lmc_kernel<2, double>, a, n, smem, stream);
+  return launch_one_nuts(funnel_nuts_lmc_kernel<4, double>, a, n, smem, stream);
 }
 
 }  // namespace gjx

@@ -9,7 +9,9 @@ state, logdensity_fn, step_size, inverse_mass_matrix, alpha2,
 stopping_criterion="euc")`` returns ``(LMCState, NUTSInfo)``.  The whole
 transition (velocity draw, iterative doubling over lan_integrator steps,
 progressive multinomial sampling with the energy -l + K - J, U-turn checks)
-runs in one HIP kernel, one wave per chain, float64, on Neal's funnel.
+runs in one HIP kernel, one wave per chain, on Neal's funnel: float64 for a
+float64 state, the tuned form for a float32 one (float32 trajectory, log
+density, kinetic-energy sums, volume change and tree weights in float64).
 
 ``rng_key`` is a ``PRNGKey`` (device Philox) or ``geomjax_amd.nuts.NUTSNoise``
 (the velocity normals, direction / merge uniforms per expansion, progressive
@@ -56,8 +58,8 @@ def build_kernel(integrator=lan_integrator, divergence_threshold: int = 1000, ma
             raise ValueError(f"stopping_criterion must be one of {sorted(_lib.GJX_UTURN)} (got {stopping_criterion!r})")
         single = state.position.dim() == 1
         position = (state.position[None] if single else state.position).contiguous()
-        if position.dtype != torch.float64:
-            raise TypeError("NUTS-LMC positions must be float64")
+        if position.dtype not in (torch.float64, torch.float32):
+            raise TypeError("NUTS-LMC positions must be float64 (plain) or float32 (tuned)")
         im = _check(logdensity_fn, position, inverse_mass_matrix)
         dev = position.device
         C, D = position.shape
@@ -65,15 +67,15 @@ def build_kernel(integrator=lan_integrator, divergence_threshold: int = 1000, ma
         J = torch.as_tensor(state.volume_adjustment, dtype=torch.float64, device=dev).expand(C).contiguous()
         if isinstance(rng_key, NUTSNoise):
             f = lambda t, *shape: t.reshape(*shape).to(device=dev, dtype=torch.float64).contiguous()
-            noise = (f(rng_key.z, C, D), f(rng_key.u_direction, C, M), f(rng_key.u_merge, C, M),
-                     f(rng_key.u_step, C, 2 ** M - 1))
+            noise = (rng_key.z.reshape(C, D).to(device=dev, dtype=position.dtype).contiguous(),
+                     f(rng_key.u_direction, C, M), f(rng_key.u_merge, C, M), f(rng_key.u_step, C, 2 ** M - 1))
             seed, counter = 0, 0
         elif isinstance(rng_key, PRNGKey):
             noise = (None, None, None, None)
             seed, counter = rng_key.seed, rng_key.counter
         else:
             raise TypeError("rng_key must be NUTSNoise or PRNGKey")
-        vec = lambda: torch.empty(C, D, dtype=torch.float64, device=dev)
+        vec = lambda: torch.empty(C, D, dtype=position.dtype, device=dev)
         sca = lambda dt=torch.float64: torch.empty(C, dtype=dt, device=dev)
         out = dict(new_position=vec(), new_logdensity=sca(), new_logdensity_grad=vec(), new_volume_adjustment=sca(),
                    velocity=vec(), energy=sca(), acceptance_rate=sca(), is_divergent=sca(torch.uint8),
@@ -83,7 +85,8 @@ def build_kernel(integrator=lan_integrator, divergence_threshold: int = 1000, ma
                             logdensity=logdensity.data_ptr(), volume_adjustment=J.data_ptr(), z=_lib.ptr(noise[0]),
                             u_direction=_lib.ptr(noise[1]), u_merge=_lib.ptr(noise[2]), u_step=_lib.ptr(noise[3]),
                             **{k: v.data_ptr() for k, v in out.items()})
-        params = _params(C, D, _lib.GJX_F64, step_size, 0, alpha2, logdensity_fn.sigma, divergence_threshold,
+        code = _lib.GJX_F64 if position.dtype == torch.float64 else _lib.GJX_F32
+        params = _params(C, D, code, step_size, 0, alpha2, logdensity_fn.sigma, divergence_threshold,
                          reference_faithful, seed=seed, counter=counter)
         rc = _lib.load().gjx_lmcmonge_funnel_nuts(params, M, _lib.GJX_UTURN[stopping_criterion], io,
                                                   _lib.stream_handle(dev))

@@ -608,3 +608,63 @@ def test_tuned_logistic_nuts_chains_track_float64_chains():
     assert abs(a64 - a32) < 0.02
     assert abs(n64 - n32) < 0.05 * n64
     assert torch.all(dm < 5 * se)
+
+
+@pytest.mark.parametrize("D,C,eps,M,faithful", [(10, 64, 0.1, 6, True), (100, 64, 0.03, 7, False)])
+def test_tuned_lmcmonge_nuts_tracks_float64(D, C, eps, M, faithful):
+    """The float32 LMC-Monge NUTS form (float32 trajectory, float64 log density,
+    kinetic-energy sums, volume change and tree weights) on the same host draws
+    as the float64 kernel, in both algebras: on at least 90 % of the chains the
+    step count is the same and the new position and volume adjustment agree to
+    1e-3 (relative / absolute)."""
+    from geomjax_amd.lmcmonge_nuts import build_kernel as lnuts_kernel
+
+    print(f"case tuned LMC-Monge NUTS D={D} C={C} eps={eps} M={M} faithful={faithful}", flush=True)
+    model = gj.NealFunnel(D)
+    kern = lnuts_kernel(max_num_doublings=M, reference_faithful=faithful)
+    q0 = (0.5 * seeded_normal(200, C, D)).float().double()
+    z = seeded_normal(201, C, D).float().double()
+    ud, um, us = seeded_uniform(202, C, M), seeded_uniform(203, C, M), seeded_uniform(204, C, 2 ** M - 1)
+    noise = lambda dt: NUTSNoise(z.to(DEV, dt), ud.to(DEV), um.to(DEV), us.to(DEV))
+    out = {}
+    for dt in (torch.float64, torch.float32):
+        im = torch.ones(D, dtype=dt, device=DEV)
+        out[dt] = kern(noise(dt), gj.lmcmonge.init(q0.to(DEV, dt), model), model, eps, im, 1.0)
+    torch.cuda.synchronize()
+    (s64, i64), (s32, i32) = out[torch.float64], out[torch.float32]
+    assert s32.position.dtype == torch.float32 and s32.logdensity.dtype == torch.float64
+    same = (i32.num_integration_steps == i64.num_integration_steps).cpu()
+    per = ((s32.position.double() - s64.position).norm(dim=1) / s64.position.norm(dim=1)).cpu()
+    dj = (s32.volume_adjustment - s64.volume_adjustment).abs().cpu()
+    close = same & (per < 1e-3) & (dj < 1e-3)
+    frac = float(close.double().mean())
+    print(f"  same steps on {float(same.double().mean()):.3f}, close on {frac:.3f}; median position error "
+          f"{float(per[same].median()):.1e}, max J diff there {float(dj[close].max()):.1e}", flush=True)
+    assert torch.isfinite(s32.position).all()
+    assert frac >= 0.9
+
+
+def test_tuned_lmcmonge_nuts_keeps_the_funnel_stationary():
+    """Exact algebra, float32 state: exact funnel draws keep v's mean and
+    variance over ten transitions (5 standard errors), as the float64 form."""
+    D, C = 10, 4096
+    g = torch.Generator().manual_seed(79)
+    v = 3.0 * torch.randn(C, 1, generator=g, dtype=torch.float64)
+    x = torch.randn(C, D - 1, generator=g, dtype=torch.float64) * torch.exp(0.5 * v)
+    q = torch.cat([x, v], dim=1).to(DEV).float()
+    model = gj.NealFunnel(D)
+    im = torch.ones(D, dtype=torch.float32, device=DEV)
+    alg = gj.nutslmcmonge(model, 0.1, im, alpha2=1.0, max_num_doublings=6, reference_faithful=False)
+    s = alg.init(q)
+    accs = []
+    for t in range(10):
+        s, info = alg.step(gr.key(600 + t), s)
+        accs.append(float(info.acceptance_rate.mean()))
+    torch.cuda.synchronize()
+    vv = s.position[:, -1].double().cpu()
+    m, var = float(vv.mean()), float(vv.var())
+    print(f"v mean {m:.3f} (0), var {var:.3f} (9), acceptance {sum(accs) / 10:.3f}", flush=True)
+    assert s.position.dtype == torch.float32 and torch.isfinite(s.position).all()
+    se_m, se_v = 3.0 / C ** 0.5, 9.0 * (2.0 / C) ** 0.5
+    assert abs(m) < 5 * se_m
+    assert abs(var - 9.0) < 5 * se_v
