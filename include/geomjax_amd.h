@@ -209,13 +209,15 @@ int gjx_lmcmonge_funnel_sample(const gjx_funnel_params* params, const gjx_lmc_sa
 /* NUTS with the LMC-Monge integrator (geomjax/lmcmonge/nuts.py:80-207,300-404):
  * the iterative expansion of gjx_rmhmc_monge_funnel_nuts over lan_integrator
  * steps, with the energy -l + K - J (the volume adjustment J carried along the
- * trajectory, geomjax/lmcmonge/metrics.py:267-284).  Funnel, float64, one wave
- * per chain, D - 1 <= 256, max_num_doublings in [1, 12], stopping criterion
+ * trajectory, geomjax/lmcmonge/metrics.py:267-284).  Funnel, one wave per
+ * chain; params->dtype GJX_F64, or GJX_F32 for the tuned form (float32 state
+ * buffers, inverse mass, z and trajectory; float64 log density, kinetic-energy
+ * sums, volume change and weights); D - 1 <= 256, max_num_doublings in [1, 12], stopping criterion
  * GJX_UTURN_*; flags and alpha2 as for gjx_lmcmonge_funnel_step.  The draws
  * of one transition per chain: z (velocity), u_direction[e], u_merge[e] for
  * expansion e, u_step[j] for integration step j (all NULL: device PRNG). */
 typedef struct {
-  const void* inverse_mass_matrix;   /* [D] float64 */
+  const void* inverse_mass_matrix;   /* [D] of the params' dtype */
   const void* position;              /* [C, D] */
   const double* logdensity;          /* [C] */
   const double* volume_adjustment;   /* [C] */
