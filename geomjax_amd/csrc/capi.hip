@@ -799,7 +799,6 @@ int gjx_lmc_fisher_logistic_nuts(const gjx_logistic_params* params, const gjx_lo
   gjx::LogisticArgs a;
   int rc = check_logistic(params, data, false, &a, 64);
   if (rc != GJX_OK) return rc;
-  GJX_REQUIRE(params->dtype == GJX_F64, "the NUTS kernel runs in float64");
   GJX_REQUIRE(max_num_doublings >= 1 && max_num_doublings <= 12, "max_num_doublings must be in [1, 12] (got %d)",
               max_num_doublings);
   GJX_REQUIRE(stopping_criterion == GJX_UTURN_EUC || stopping_criterion == GJX_UTURN_BET ||
@@ -838,7 +837,9 @@ int gjx_lmc_fisher_logistic_nuts(const gjx_logistic_params* params, const gjx_lo
   n.num_steps = io->num_integration_steps;
   n.left_position = io->leftmost_position;
   n.right_position = io->rightmost_position;
-  hipError_t e = gjx::launch_logistic_nuts_lmc_f64(a, n, (hipStream_t)stream);
+  hipError_t e = params->dtype == GJX_F32
+                     ? gjx::launch_logistic_nuts_lmc_f32(a, n, data->max_features, (hipStream_t)stream)
+                     : gjx::launch_logistic_nuts_lmc_f64(a, n, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "gjx_lmc_fisher_logistic_nuts launch");
   return GJX_OK;
 }

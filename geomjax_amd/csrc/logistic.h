@@ -147,6 +147,9 @@ hipError_t launch_logistic_rmhmc_tangent_f32(const LogisticArgs& a, const Tangen
 hipError_t launch_logistic_nuts_f64(const LogisticArgs& a, const NutsArgs& n, hipStream_t stream);
 // NUTS with the RMHMC integrator, tuned form (float32 state), D <= 64 (logistic_nuts_fast.hip)
 hipError_t launch_logistic_nuts_f32(const LogisticArgs& a, const NutsArgs& n, int max_features, hipStream_t stream);
+// NUTS with the general-metric LMC integrator, tuned form, D <= 64 (logistic_nuts_fast.hip)
+hipError_t launch_logistic_nuts_lmc_f32(const LogisticArgs& a, const NutsArgs& n, int max_features,
+                                        hipStream_t stream);
 // NUTS with the general-metric LMC integrator, float64, D <= 64 (logistic_nuts.hip)
 hipError_t launch_logistic_nuts_lmc_f64(const LogisticArgs& a, const NutsArgs& n, hipStream_t stream);
 hipError_t launch_logistic_value_and_grad(const LogisticArgs& a, hipStream_t stream);

@@ -42,64 +42,6 @@ namespace lfast {
 
 enum LmcPhase { kLmcDraw, kLmcHalfA, kLmcHalfB, kLmcPos, kLmcEnd };
 
-// One pass over the rows of X with the velocity in S.p: b = X v, the signed
-// square roots of the weights lambda + coef w b into SS; with want_gv also
-// S.u = G v = X^T (lambda b) + v / prior_var; with step the position update
-// q += eps v carried into the rows (ETA += eps b, SL = sqrt(lambda) there).
-template <int NB>
-__device__ __forceinline__ void lmc_row_pass(Smem<1>& S, const float* __restrict__ Xr, float* ETA, float* SL,
-                                             float* SS, int npad, int D, float coef, float eps, bool want_gv,
-                                             bool step, float inv_prior) {
-  constexpr int KC = 2 * NB;
-  const int l = threadIdx.x % 64, h = l / 32;
-  float v[KC][8], gacc[KC][8];
-#pragma unroll
-  for (int kc = 0; kc < KC; ++kc) {
-    load8(S.p + kc * 16 + h * 8, v[kc]);
-#pragma unroll
-    for (int i = 0; i < 8; ++i) gacc[kc][i] = 0.0f;
-  }
-  const float* xp = Xr + (size_t)l * 8;
-  const int nj = npad / 32;
-  for (int rb = 0; rb < nj; ++rb) {
-    float x[KC][8];
-#pragma unroll
-    for (int kc = 0; kc < KC; ++kc) load8(xp + (size_t)(rb * KC + kc) * 512, x[kc]);
-    float bp = 0.0f;
-#pragma unroll
-    for (int kc = 0; kc < KC; ++kc)
-#pragma unroll
-      for (int i = 0; i < 8; ++i) bp = fmaf(x[kc][i], v[kc][i], bp);
-    const float b = bp + __shfl_xor(bp, 32, 64);
-    const int row = rb * 32 + (l % 32);
-    const float eta = ETA[row];
-    const float s = sigmoidf(eta);
-    const float lam = s * (1.0f - s);
-    const float d = fmaf(coef * lam * (1.0f - 2.0f * s), b, lam);
-    if (want_gv) {
-      const float lb = lam * b;
-#pragma unroll
-      for (int kc = 0; kc < KC; ++kc)
-#pragma unroll
-        for (int i = 0; i < 8; ++i) gacc[kc][i] = fmaf(lb, x[kc][i], gacc[kc][i]);
-    }
-    if (h == 0) {
-      SS[row] = copysignf(sqrt_fast(fabsf(d)), d);
-      if (step) {
-        const float en = fmaf(eps, b, eta);
-        const float sn = sigmoidf(en);
-        ETA[row] = en;
-        SL[row] = sqrt_fast(sn * (1.0f - sn));
-      }
-    }
-  }
-  if (want_gv) {
-    const float tot = reduce_features<NB, 1>(S, gacc);
-    S.u[l] = l < D ? tot + S.p[l] * inv_prior : 0.0f;
-  }
-  __syncthreads();
-}
-
 template <int NB, bool STEP>
 __device__ __forceinline__ void run_lmc_chain(const LogisticArgs& a, Smem<1>& S, int c, int post) {
   const int l = threadIdx.x % 64;

@@ -32,15 +32,16 @@ namespace lfast {
 
 using nuts::logaddexp;
 
-// coordinate l of a trajectory state (zero past D) and its log density
+// coordinate l of a trajectory state (zero past D): position, momentum, velocity
+// and (LMC) grad(-l + log det G / 2); its log density and (LMC) volume adjustment
 struct NPt {
-  float q, p, u;
-  double logp;
+  float q, p, u, dphi;
+  double logp, J;
 };
 
 struct NProp {
   float q;
-  double logp, energy, weight, slpa;
+  double logp, J, energy, weight, slpa;
 };
 
 struct NTree {
@@ -49,7 +50,7 @@ struct NTree {
 };
 
 __device__ __forceinline__ NProp nprop(const NPt& s, double en, double w) {
-  return NProp{s.q, s.logp, en, w, fmin(w, 0.0)};
+  return NProp{s.q, s.logp, s.J, en, w, fmin(w, 0.0)};
 }
 
 // The transition's uniforms: host-fed or the device Philox stream of the
@@ -286,7 +287,7 @@ __device__ __forceinline__ void run_nuts_chain(const LogisticArgs& a, const Nuts
     const double pu = group_sum<64>(own ? (double)S.p[l] * (double)S.u[l] : 0.0);
     const double logp1 = loglik - 0.5 * qq / prior + lp_const;
     // lanes past D hold zeros (S.u past the tile width is never written)
-    out = NPt{own ? S.q[l] : 0.0f, own ? S.p[l] : 0.0f, own ? S.u[l] : 0.0f, logp1};
+    out = NPt{own ? S.q[l] : 0.0f, own ? S.p[l] : 0.0f, own ? S.u[l] : 0.0f, 0.0f, logp1, 0.0};
     return -logp1 + 0.5 * pu + 0.5 * logdet_from_pivots(S, D) + ke_const;
   };
   auto load = [&](const NPt& e) {
@@ -297,7 +298,7 @@ __device__ __forceinline__ void run_nuts_chain(const LogisticArgs& a, const Nuts
     have_ginv = false;
   };
 
-  const NPt s0{own ? S.q[l] : 0.0f, own ? S.p[l] : 0.0f, own ? S.u[l] : 0.0f, logp0};
+  const NPt s0{own ? S.q[l] : 0.0f, own ? S.p[l] : 0.0f, own ? S.u[l] : 0.0f, 0.0f, logp0, 0.0};
   NPt left, right;
   NProp prop;
   const NTree res = tuned_nuts_tree(s0, H0, load, step, uniform, nu, (float)a.eps, a.div_threshold, ck, c, left,
@@ -332,6 +333,199 @@ __device__ __forceinline__ void run_nuts_chain(const LogisticArgs& a, const Nuts
   }
 }
 
+// NUTS over the general-metric LMC integrator (geomjax/lmcmc/nuts.py:78-161),
+// tuned form: the tree above over lan_integrator steps made of the tuned LMC
+// kernel's phases (logistic_lmc_fast.hip: signed-weight Gram passes of
+// G +- (h/2) Omega~, sweeps, refined solve, the position pass with grad phi), with
+// the energy -l - log det G / 2 + v^T G v / 2 - J (geomjax/lmcmc/metrics.py:
+// 209-221) in float64.  A trajectory end is (q, G v, v, grad phi, log density, J):
+// reloading it recomputes only its eta rows.
+template <int NB>
+__device__ __forceinline__ void run_nuts_lmc_chain(const LogisticArgs& a, const NutsArgs& nu, Smem<1>& S, double* ck,
+                                                   int c, int post) {
+  const int l = threadIdx.x % 64;
+  const PackedPost pp = a.packed_hdr[post];
+  const int N = a.post_N[post], D = a.post_D[post];
+  const float* Xg = a.packed + pp.xg;
+  const float* Xr = a.packed + pp.xr;
+  const float* Xs = a.packed + pp.xs;
+  const float* yf = a.packed + pp.yf;
+  const int npad = pp.npad;
+  const int64_t ns = a.work_rows;
+  float* const W = a.work + (size_t)c * 5 * ns;
+  float* const ETA = W;
+  float* const SL = W + ns;
+  float* const SS = W + 2 * ns;
+  const int Dp = a.Dmax;
+  const float inv_prior = (float)(1.0 / a.prior_var);
+  const double prior = a.prior_var;
+  const double lp_const = -0.5 * (double)D * log(2.0 * M_PI * prior);
+  const NUniform uniform{c, a.counter, (uint32_t)a.seed, (uint32_t)(a.seed >> 32)};
+  const bool own = l < D;
+
+  S.q[l] = own ? static_cast<const float*>(a.q_in)[(size_t)c * Dp + l] : 0.0f;
+  if (a.z != nullptr) {
+    S.z[l] = own ? static_cast<const float*>(a.z)[(size_t)c * Dp + l] : 0.0f;
+  } else if (l < MAXD / 2) {
+    const uint64_t tt = a.counter;
+    u32x4 r = philox4x32_10(u32x4{(uint32_t)c, (uint32_t)tt, kStreamMomentum + (uint32_t)(2 * l),
+                                  (uint32_t)(tt >> 32)}, (uint32_t)a.seed, (uint32_t)(a.seed >> 32));
+    double n0, n1;
+    normal2(r, n0, n1);
+    S.z[2 * l] = 2 * l < D ? (float)n0 : 0.0f;
+    S.z[2 * l + 1] = 2 * l + 1 < D ? (float)n1 : 0.0f;
+  }
+  S.p[l] = 0.0f;
+  __syncthreads();
+  const double logp0 = a.logp_in[c];
+  double Jc = a.J_in[c];  // volume adjustment of the state in LDS
+  (void)fresh_pass<NB, 1>(S, Xr, yf, N, npad, D, ETA, SL, false, inv_prior);
+
+  // The phases of one lan_integrator step of signed size h (integrators.py:61-144)
+  // from (S.q, S.p = velocity) with grad phi in S.dH, or with draw the velocity
+  // draw v0 = L^-T z at S.q; each phase is one Gram pass and one sweep.  Leaves
+  // log det G at the final position in logdetG; returns the volume change.
+  enum { kDraw, kHalfA, kHalfB, kPos, kDone };
+  double logdetG = 0.0;
+  auto phases = [&](bool draw, float h) -> double {
+    double dJ = 0.0;
+    int phase = draw ? kDraw : kHalfA;
+    bool second = false;  // the half step after the position update
+    while (phase != kDone) {
+      const float* wts = SL;
+      if (phase == kHalfA) {
+        lmc_row_pass<NB>(S, Xr, ETA, SL, SS, npad, D, 0.25f * h, h, true, false, inv_prior);
+        wts = SS;
+      } else if (phase == kHalfB) {
+        lmc_row_pass<NB>(S, Xr, ETA, SL, SS, npad, D, -0.25f * h, h, false, !second, inv_prior);
+        wts = SS;
+      } else if (phase == kPos) {
+        S.q[l] = own ? fmaf(h, S.p[l], S.q[l]) : 0.0f;
+        __syncthreads();
+      }
+      gram_pass<NB, 1, true>(S, Xg, wts, npad, D, inv_prior);
+      f32x32 A[NB];
+      float p0j;
+      sweep<NB, 1>(S, A, D, false, p0j, phase == kDraw);
+      const double lad = logabsdet_from_pivots(S, D);
+      if (phase == kHalfA) {  // v' = A+^-1 (G v - (h/2) grad phi)
+        dJ -= lad;
+        S.p0[l] = own ? fmaf(-0.5f * h, S.dH[l], S.u[l]) : 0.0f;
+        __syncthreads();
+        solve_refined<NB, 1>(S, A, S.p0, S.p, S.u);
+        phase = kHalfB;
+        continue;
+      }
+      if (phase == kHalfB) {
+        dJ += lad;
+        phase = second ? kDone : kPos;
+        continue;
+      }
+      logdetG = lad;
+      if (phase == kDraw) {  // v0 = L^-T z by back substitution over the rows of L
+        float r = S.z[l], vj = 0.0f;
+        for (int i = D - 1; i >= 0; --i) {
+          const float vi = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, r), i)) /
+                           S.G[i * LDG + i];
+          if (l == i) vj = vi;
+          if (l < i) r = fmaf(-S.G[i * LDG + l], vi, r);
+        }
+        S.p[l] = vj;
+        if (a.momentum != nullptr && own) static_cast<float*>(a.momentum)[(size_t)c * Dp + l] = vj;
+      }
+      store_ginv<NB, 1>(S, A);
+      S.u[l] = 0.0f;
+      __syncthreads();
+      field_pass<NB, 1>(S, Xr, Xs, yf, ETA, ETA, SL, npad, D, 0.0f, inv_prior, true);  // S.dH = grad phi
+      if (phase == kDraw) {
+        phase = kDone;
+      } else {
+        second = true;
+        phase = kHalfA;
+      }
+    }
+    return dJ;
+  };
+  // S.u = G v at the position; returns v^T G v
+  auto vgv = [&]() -> double {
+    lmc_row_pass<NB>(S, Xr, ETA, SL, SS, npad, D, 0.0f, 0.0f, true, false, inv_prior);
+    return group_sum<64>(own ? (double)S.p[l] * (double)S.u[l] : 0.0);
+  };
+
+  // ---- velocity v0 = L^-T z, momentum G v0 and H0 (lmcmc/nuts.py:146-158) ----
+  (void)phases(true, 0.0f);
+  const double H0 = -logp0 + (-0.5 * logdetG + 0.5 * vgv()) - Jc;
+
+  auto step = [&](float h, NPt& out) -> double {
+    Jc += phases(false, h);
+    const double ll = fresh_pass<NB, 1>(S, Xr, yf, N, npad, D, ETA, SL, true, inv_prior);
+    const double vGv = vgv();
+    const double qq = group_sum<64>(own ? (double)S.q[l] * (double)S.q[l] : 0.0);
+    const double logp1 = ll - 0.5 * qq / prior + lp_const;
+    out = NPt{own ? S.q[l] : 0.0f, own ? S.u[l] : 0.0f, own ? S.p[l] : 0.0f, own ? S.dH[l] : 0.0f, logp1, Jc};
+    return -logp1 + (-0.5 * logdetG + 0.5 * vGv) - Jc;
+  };
+  auto load = [&](const NPt& e) {
+    S.q[l] = e.q;
+    S.p[l] = e.u;
+    S.dH[l] = e.dphi;
+    Jc = e.J;
+    __syncthreads();
+    (void)fresh_pass<NB, 1>(S, Xr, yf, N, npad, D, ETA, SL, false, inv_prior);
+  };
+
+  const NPt s0{own ? S.q[l] : 0.0f, own ? S.u[l] : 0.0f, own ? S.p[l] : 0.0f, own ? S.dH[l] : 0.0f, logp0, Jc};
+  NPt left, right;
+  NProp prop;
+  const NTree res = tuned_nuts_tree(s0, H0, load, step, uniform, nu, (float)a.eps, a.div_threshold, ck, c, left,
+                                    right, prop);
+  // the proposal's gradient from X at its position
+  __syncthreads();
+  S.q[l] = own ? prop.q : 0.0f;
+  __syncthreads();
+  (void)fresh_pass<NB, 1>(S, Xr, yf, N, npad, D, ETA, SL, true, inv_prior);
+  const size_t row = (size_t)c * Dp;
+  if (own) {
+    static_cast<float*>(a.q_out)[row + l] = prop.q;
+    if (a.g_out) static_cast<float*>(a.g_out)[row + l] = S.g[l];
+    if (nu.left_position) static_cast<float*>(nu.left_position)[row + l] = left.q;
+    if (nu.right_position) static_cast<float*>(nu.right_position)[row + l] = right.q;
+  }
+  for (int k = D + l; k < Dp; k += 64) {  // row tails past this posterior's D
+    static_cast<float*>(a.q_out)[row + k] = 0.0f;
+    if (a.g_out) static_cast<float*>(a.g_out)[row + k] = 0.0f;
+    if (a.momentum) static_cast<float*>(a.momentum)[row + k] = 0.0f;
+    if (nu.left_position) static_cast<float*>(nu.left_position)[row + k] = 0.0f;
+    if (nu.right_position) static_cast<float*>(nu.right_position)[row + k] = 0.0f;
+  }
+  if (l == 0) {
+    a.logp_out[c] = prop.logp;
+    if (a.J_out) a.J_out[c] = prop.J;
+    if (nu.energy) nu.energy[c] = prop.energy;
+    if (nu.acceptance_rate) nu.acceptance_rate[c] = exp(prop.slpa) / (double)res.num_states;
+    if (nu.is_divergent) nu.is_divergent[c] = (uint8_t)res.diverging;
+    if (nu.is_turning) nu.is_turning[c] = (uint8_t)res.turning;
+    if (nu.num_expansions) nu.num_expansions[c] = res.expansions;
+    if (nu.num_steps) nu.num_steps[c] = res.num_states;
+  }
+}
+
+template <int NB>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+logistic_nuts_lmc_f32_kernel(const LogisticArgs a, const NutsArgs nu) {
+  __shared__ Smem<1> S;
+  extern __shared__ __align__(16) double ck_dyn[];
+  const int per_xcd = gridDim.x / 8;
+  const int slot = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+  if (slot >= a.C) return;
+  const int c = a.dispatch_order ? a.dispatch_order[slot] : slot;
+  if (c < 0 || c >= a.C) return;
+  const int post = a.chain_post ? a.chain_post[c] : 0;
+  if (post < 0 || post >= a.P) return;
+  if (a.packed_hdr[post].nb != NB) return;
+  run_nuts_lmc_chain<NB>(a, nu, S, ck_dyn, c, post);
+}
+
 // One wave per chain, chains dealt to the XCDs in contiguous ranges; chains of
 // the other tile width leave at once.
 template <int NB>
@@ -351,6 +545,21 @@ logistic_nuts_f32_kernel(const LogisticArgs a, const NutsArgs nu) {
 }
 
 }  // namespace lfast
+
+hipError_t launch_logistic_nuts_lmc_f32(const LogisticArgs& a, const NutsArgs& n, int max_features,
+                                        hipStream_t stream) {
+  const int blocks = (a.C + 7) / 8 * 8;
+  const size_t smem = (size_t)nuts_checkpoint_arrays(n.criterion) * n.max_doublings * 64 * sizeof(double);
+  const bool nb1 = max_features <= 32 || a.P > 1;
+  const bool nb2 = max_features > 32;
+  if (nb2) {
+    hipLaunchKernelGGL((lfast::logistic_nuts_lmc_f32_kernel<2>), dim3(blocks), dim3(64), smem, stream, a, n);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  if (nb1) hipLaunchKernelGGL((lfast::logistic_nuts_lmc_f32_kernel<1>), dim3(blocks), dim3(64), smem, stream, a, n);
+  return hipGetLastError();
+}
 
 hipError_t launch_logistic_nuts_f32(const LogisticArgs& a, const NutsArgs& n, int max_features,
                                     hipStream_t stream) {

@@ -8,9 +8,12 @@ state, logdensity_fn, step_size, metric_fn, max_num_doublings=10,
 stopping_criterion="euc")`` returns ``(LMCState, NUTSInfo)``.  The whole
 transition (velocity draw v0 = L^-T z, iterative doubling over lan_integrator
 steps, progressive multinomial sampling with the energy -l - log det G / 2 +
-v^T G v / 2 - J, U-turn checks) runs in one HIP kernel, one workgroup per chain,
-float64: Bayesian logistic regression with the Fisher metric, one posterior or
-a ``LogisticBatch``, every posterior with D <= 64.
+v^T G v / 2 - J, U-turn checks) runs in one HIP kernel: Bayesian logistic
+regression with the Fisher metric, one posterior or a ``LogisticBatch``, every
+posterior with D <= 64; float64 for a float64 state (one 256-lane workgroup per
+chain), the tuned form for a float32 one (one wave per chain, split-bf16 MFMA
+Gram passes, float64 log densities, log-determinants, volume change, energies
+and tree weights).
 
 ``rng_key`` is a ``PRNGKey`` (device Philox) or ``geomjax_amd.nuts.NUTSNoise``.
 ``NUTSInfo.trajectory_leftmost_state`` / ``trajectory_rightmost_state`` carry
@@ -57,8 +60,8 @@ def build_kernel(integrator=lan_integrator, divergence_threshold: int = 1000):
                           divergence_threshold)
         single = state.position.dim() == 1
         position = (state.position[None] if single else state.position).contiguous()
-        if position.dtype != torch.float64:
-            raise TypeError("NUTS-LMC positions must be float64")
+        if position.dtype not in (torch.float64, torch.float32):
+            raise TypeError("NUTS-LMC positions must be float64 (plain) or float32 (tuned)")
         _check(logdensity_fn, metric_fn, position)
         dev = position.device
         C, D = position.shape
@@ -67,15 +70,15 @@ def build_kernel(integrator=lan_integrator, divergence_threshold: int = 1000):
         J = torch.as_tensor(state.volume_adjustment, dtype=torch.float64, device=dev).expand(C).contiguous()
         if isinstance(rng_key, NUTSNoise):
             f = lambda t, *shape: t.reshape(*shape).to(device=dev, dtype=torch.float64).contiguous()
-            noise = (f(rng_key.z, C, D), f(rng_key.u_direction, C, M), f(rng_key.u_merge, C, M),
-                     f(rng_key.u_step, C, 2 ** M - 1))
+            noise = (rng_key.z.reshape(C, D).to(device=dev, dtype=position.dtype).contiguous(),
+                     f(rng_key.u_direction, C, M), f(rng_key.u_merge, C, M), f(rng_key.u_step, C, 2 ** M - 1))
             seed, counter = 0, 0
         elif isinstance(rng_key, PRNGKey):
             noise = (None, None, None, None)
             seed, counter = rng_key.seed, rng_key.counter
         else:
             raise TypeError("rng_key must be NUTSNoise or PRNGKey")
-        vec = lambda: torch.empty(C, D, dtype=torch.float64, device=dev)
+        vec = lambda: torch.empty(C, D, dtype=position.dtype, device=dev)
         sca = lambda dt=torch.float64: torch.empty(C, dtype=dt, device=dev)
         out = dict(new_position=vec(), new_logdensity=sca(), new_logdensity_grad=vec(), new_volume_adjustment=sca(),
                    velocity=vec(), energy=sca(), acceptance_rate=sca(), is_divergent=sca(torch.uint8),
@@ -86,8 +89,8 @@ def build_kernel(integrator=lan_integrator, divergence_threshold: int = 1000):
                             u_merge=_lib.ptr(noise[2]), u_step=_lib.ptr(noise[3]),
                             **{k: v.data_ptr() for k, v in out.items()})
         params = logistic_params(logdensity_fn, C, step_size=step_size, divergence_threshold=divergence_threshold,
-                                 seed=seed, counter=counter)
-        data = logdensity_fn.data(dtype=torch.float64, num_chains=C)
+                                 seed=seed, counter=counter, dtype=position.dtype)
+        data = logdensity_fn.data(dtype=position.dtype, num_chains=C)
         rc = _lib.load().gjx_lmc_fisher_logistic_nuts(params, data, M, _lib.GJX_UTURN[stopping_criterion], io,
                                                       _lib.stream_handle(dev))
         _lib.check(rc, "gjx_lmc_fisher_logistic_nuts")

@@ -131,8 +131,8 @@ def test_logistic_feature_capacity_per_path():
 
 
 def test_mmala_and_logistic_nuts_entry_points_validate_on_the_host():
-    """MMALA and the logistic NUTS entry points: MMALA and NUTS-LMC float64
-    only (NUTS-RMHMC also float32, with the tuned path's packed table), D <= 64,
+    """MMALA and the logistic NUTS entry points: MMALA float64 only (both NUTS
+    entry points also float32, with the tuned path's packed table), D <= 64,
     a positive step size for MMALA, required buffers, z and u given together."""
     lib = _lib.load()
 
@@ -173,7 +173,7 @@ def test_mmala_and_logistic_nuts_entry_points_validate_on_the_host():
     assert "position" in _lib.last_error()
     assert lib.gjx_lmc_fisher_logistic_nuts(params(8, _lib.GJX_F32), d32, 5, _lib.GJX_UTURN["euc"], _lib.LmcNutsIO(),
                                             None) == -1
-    assert "float64" in _lib.last_error()
+    assert "position" in _lib.last_error()
 
 
 def test_dense_metric_entry_points_validate_on_the_host():

@@ -668,3 +668,41 @@ def test_tuned_lmcmonge_nuts_keeps_the_funnel_stationary():
     se_m, se_v = 3.0 / C ** 0.5, 9.0 * (2.0 / C) ** 0.5
     assert abs(m) < 5 * se_m
     assert abs(var - 9.0) < 5 * se_v
+
+
+@pytest.mark.parametrize("N,D,C,eps,M,crit", [(256, 8, 32, 0.3, 5, "euc"), (1024, 40, 32, 0.15, 5, "riem"),
+                                              (2048, 64, 16, 0.1, 4, "bet")])
+def test_tuned_lmcmc_nuts_tracks_float64(N, D, C, eps, M, crit):
+    """The tuned NUTS-LMC form (float32 trajectory, split-bf16 MFMA Gram passes,
+    float64 log densities, log-determinants, volume change and weights) on the
+    same host draws as the float64 kernel, which equals the oracle at 1e-10: on
+    at least 90 % of the chains the step count is the same and the new position
+    (1e-3 relative) and volume adjustment (1e-2: four float32 log-determinants
+    per integration step, 15 steps here, observed up to ~1e-3) agree."""
+    from geomjax_amd.lmcmc import init as lmc_init
+    from geomjax_amd.lmcmc_nuts import build_kernel as lnuts_kernel
+    from geomjax_amd.synthetic import make_logistic
+
+    print(f"case tuned lmcmc-nuts N={N} D={D} C={C} eps={eps} M={M} criterion={crit}", flush=True)
+    X, y, _ = make_logistic(N, D, 80 + D)
+    model = gj.LogisticRegression(X, y, 100.0)
+    kern = lnuts_kernel()
+    q0 = (0.3 * seeded_normal(210 + D, C, D)).float().double()
+    z = seeded_normal(211 + D, C, D).float().double()
+    ud, um, us = seeded_uniform(212, C, M), seeded_uniform(213, C, M), seeded_uniform(214, C, 2 ** M - 1)
+    noise = lambda dt: NUTSNoise(z.to(DEV, dt), ud.to(DEV), um.to(DEV), us.to(DEV))
+    out = {dt: kern(noise(dt), lmc_init(q0.to(DEV, dt), model), model, eps, gj.FisherMetric(), M,
+                    stopping_criterion=crit) for dt in (torch.float64, torch.float32)}
+    torch.cuda.synchronize()
+    (s64, i64), (s32, i32) = out[torch.float64], out[torch.float32]
+    assert s32.position.dtype == torch.float32 and s32.logdensity.dtype == torch.float64
+    same = (i32.num_integration_steps == i64.num_integration_steps).cpu()
+    per = ((s32.position.double() - s64.position).norm(dim=1) / s64.position.norm(dim=1)).cpu()
+    dj = (s32.volume_adjustment - s64.volume_adjustment).abs().cpu()
+    close = same & (per < 1e-3) & (dj < 1e-2)
+    frac = float(close.double().mean())
+    print(f"  steps {i64.num_integration_steps.tolist()[:8]}; same steps on {float(same.double().mean()):.3f}, "
+          f"close on {frac:.3f}; median position error {float(per[same].median()):.1e}, "
+          f"max J diff there {float(dj[close].max()) if bool(close.any()) else float('nan'):.1e}", flush=True)
+    assert torch.isfinite(s32.position).all()
+    assert frac >= 0.9
