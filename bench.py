@@ -616,8 +616,8 @@ def logistic_nuts_forms(K, W, dev, chains=1024, integrator="rmhmc", tuned=False)
     metric, float64, one launch per transition) with the RMHMC integrator (the
     headline step size and fixed-point cap) or the general-metric LMC integrator
     (``integrator="lmc"``, same step size): max_num_doublings 5, "euc", device
-    PRNG; with ``tuned`` (RMHMC only) the float32 state on the tuned kernel.
-    Reports integration steps/s and min-ESS/s."""
+    PRNG; with ``tuned`` the float32 state on the tuned kernel.  Reports
+    integration steps/s and min-ESS/s."""
     import torch
 
     import geomjax_amd as gj
@@ -634,7 +634,8 @@ def logistic_nuts_forms(K, W, dev, chains=1024, integrator="rmhmc", tuned=False)
         from geomjax_amd.lmcmc import init as lmc_init
         from geomjax_amd.lmcmc_nuts import build_kernel as lnuts_kernel
 
-        kern, kname, s = lnuts_kernel(), "logistic_nuts_lmc_f64_kernel", lmc_init(q_init.to(dev), model)
+        kname = "logistic_nuts_lmc_f32_kernel<2>" if tuned else "logistic_nuts_lmc_f64_kernel"
+        kern, s = lnuts_kernel(), lmc_init(q_init.to(dev), model)
     else:
         from geomjax_amd.nuts import build_kernel as nuts_kernel
         from geomjax_amd.rmhmc import implicit_midpoint
@@ -998,6 +999,7 @@ def main():
         return
     if args.only == "logistic_nuts_lmc":
         print(json.dumps(logistic_nuts_forms(K, W, dev, integrator="lmc")), flush=True)
+        print(json.dumps(logistic_nuts_forms(K, W, dev, integrator="lmc", tuned=True)), flush=True)
         return
     if args.only == "uneven":
         print(json.dumps(uneven_batch_forms(32, K, W, dev, plain=not args.no_plain)), flush=True)
@@ -1061,6 +1063,10 @@ def main():
             k: extra["logistic_lmc_1024_chains_f32"][k] / extra["logistic_lmc_1024_chains"][k]
             for k in ("value", "min_ess_per_sec")}
         extra["logistic_nuts_lmc_1024_chains"] = logistic_nuts_forms(10, 2, dev, integrator="lmc")
+        extra["logistic_nuts_lmc_1024_chains_f32"] = logistic_nuts_forms(10, 2, dev, integrator="lmc", tuned=True)
+        extra["logistic_nuts_lmc_1024_chains_f32"]["tuned_vs_f64"] = {
+            k: extra["logistic_nuts_lmc_1024_chains_f32"][k] / extra["logistic_nuts_lmc_1024_chains"][k]
+            for k in ("value", "min_ess_per_sec")}
         extra["mmala_1024_chains"] = mmala_forms(200, 100, dev)
         extra["dense_lmc_funnel_d20_1024_chains"] = dense_lmc_forms(dev)
         extra["dense_rmhmc_funnel_d20_1024_chains"] = dense_rmhmc_forms(dev)
