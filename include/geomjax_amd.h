@@ -686,7 +686,11 @@ int gjx_rmhmc_fisher_logistic_nuts(const gjx_logistic_params* params, const gjx_
  * for Bayesian logistic regression with the Fisher metric: the iterative
  * expansion over the lan_integrator steps of gjx_lmc_fisher_logistic_step, with
  * the energy -l - log det G / 2 + v^T G v / 2 - J (geomjax/lmcmc/metrics.py:
- * 209-221).  float64, every posterior D <= 64, one workgroup per chain; io as
+ * 209-221).  params->dtype GJX_F64 (one 256-lane workgroup per chain) or
+ * GJX_F32, the tuned form (float32 state buffers and trajectory, split-bf16
+ * MFMA Gram passes, float64 log densities, log-determinants, volume change,
+ * energies and weights, one wave per chain; needs data->packed and
+ * data->workspace as the tuned step); every posterior D <= 64; io as
  * gjx_lmc_nuts_io with rows of stride params->dim (inverse_mass_matrix unused;
  * velocity = NUTSInfo.velocity, the v0 drawn). */
 int gjx_lmc_fisher_logistic_nuts(const gjx_logistic_params* params, const gjx_logistic_data* data,
