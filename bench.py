@@ -902,7 +902,9 @@ def tangent_forms(dev, repeats=3):
     one tangent launch (one transition carried with its forward derivative) against one
     primal float64 transition, on the tuning workloads: funnel D=100 x 4096 chains and
     logistic 2048 x 64 x 1024 chains, 32 integrator steps, as many fixed-point
-    iterations as the solver runs; and the float32 fast path of each."""
+    iterations as the solver runs; the float32 fast path of each; and the
+    float32 tangent of a 2048 x 128 posterior (256 chains), the width no float64
+    tangent serves."""
     import torch
 
     import geomjax_amd as gj
@@ -951,6 +953,32 @@ def tangent_forms(dev, repeats=3):
         out[name]["tangent_f32_fast_path_ms"] = t32 * 1e3
         out[name]["mean_d_objective_f32"] = r32.d_objective.mean(0).tolist()
         log(f"tangent {name} float32 fast path: {t32 * 1e3:.2f} ms")
+    # 65..128 features: the float32 tangent (no float64 one holds G^-1 and its
+    # tangent at this width) against one float64 primal transition
+    X, y, _ = make_logistic(N_ROWS, 128, DATA_SEED)
+    model, C, D = gj.LogisticRegression(X, y, PRIOR, device=dev), 256, 128
+    integ = functools.partial(implicit_midpoint, max_iters=MAXIT_LOG)
+    q = 0.5 * torch.randn(C, D, generator=torch.Generator().manual_seed(7), dtype=torch.float64)
+    s64 = gj.rmhmc.init(q.to(dev), model)
+    s32 = gj.rmhmc.init(q.to(dev, torch.float32), model)
+    kern = build_kernel(integ)
+    esjd_acceptance(gr.PRNGKey(1, 0), s32, model, EPS_LOG, gj.FisherMetric(), 32, integrator=integ)
+    kern(gr.PRNGKey(1, 0), s64, model, EPS_LOG, gj.FisherMetric(), 32)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(repeats):
+        r32 = esjd_acceptance(gr.PRNGKey(2, i), s32, model, EPS_LOG, gj.FisherMetric(), 32, integrator=integ)
+    torch.cuda.synchronize()
+    t32 = (time.perf_counter() - t0) / repeats
+    t0 = time.perf_counter()
+    for i in range(repeats):
+        kern(gr.PRNGKey(2, i), s64, model, EPS_LOG, gj.FisherMetric(), 32)
+    torch.cuda.synchronize()
+    t_pri = (time.perf_counter() - t0) / repeats
+    out["logistic_2048x128_256_chains"] = {"tangent_f32_ms": t32 * 1e3, "primal_f64_transition_ms": t_pri * 1e3,
+                                           "tangent_over_primal": t32 / t_pri, "num_integration_steps": 32,
+                                           "mean_d_objective_f32": r32.d_objective.mean(0).tolist()}
+    log(f"tangent logistic_2048x128_256_chains float32: {t32 * 1e3:.2f} ms vs primal {t_pri * 1e3:.2f} ms")
     return out
 
 

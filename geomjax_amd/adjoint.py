@@ -50,9 +50,10 @@ def esjd_acceptance(rng_key, state, logdensity_fn, step_size, metric_fn, num_int
     Funnel + MongeMetric: both columns; a float64 state runs the check path, a
     float32 state the fast path (float32 trajectory and tangents, float64
     energies, as the tuned sampler).  LogisticRegression / LogisticBatch +
-    FisherMetric (D <= 64): column 0 (column 1 is zero); a float64 state runs
-    the check path, a float32 state the fast path (the tuned kernel's float32
-    state and split-bf16 MFMA passes with tangents, float64 energies)."""
+    FisherMetric: column 0 (column 1 is zero); a float64 state runs the check
+    path (D <= 64), a float32 state the fast path (D <= 64: the tuned kernel's
+    float32 state and split-bf16 MFMA passes with tangents; 65..128: a float32
+    form of the check path on fp32 MFMA; float64 energies in both)."""
     funnel = isinstance(logdensity_fn, NealFunnel) and isinstance(metric_fn, MongeMetric)
     logistic = isinstance(logdensity_fn, (LogisticRegression, LogisticBatch)) and isinstance(metric_fn, FisherMetric)
     if not (funnel or logistic):

@@ -1126,7 +1126,9 @@ int gjx_logistic_value_and_grad(const gjx_logistic_params* params, const gjx_log
 int gjx_rmhmc_fisher_logistic_tangent(const gjx_logistic_params* params, const gjx_logistic_data* data,
                                       const gjx_tangent_io* io, void* stream) {
   gjx::LogisticArgs a;
-  int rc = check_logistic(params, data, false, &a, gjx::kLogisticMaxDLmc);
+  // float64: D <= 64; float32: D <= 64 on the tuned one-wave kernel, 65..128 on the wide one
+  int rc = check_logistic(params, data, false, &a,
+                          params != nullptr && params->dtype == GJX_F32 ? gjx::kLogisticMaxD : gjx::kLogisticMaxDLmc);
   if (rc != GJX_OK) return rc;
   GJX_REQUIRE(params->step_size > 0.0, "step_size must be > 0 (log step_size is the parameter)");
   GJX_REQUIRE(io != nullptr, "io is NULL");
@@ -1139,9 +1141,15 @@ int gjx_rmhmc_fisher_logistic_tangent(const gjx_logistic_params* params, const g
   a.z = io->z;
   gjx::TangentOut o{io->objective, io->acceptance, io->sq_jump, io->d_objective, io->d_acceptance,
                     io->d_sq_jump, io->num_fevals};
-  hipError_t e = params->dtype == GJX_F32
-                     ? gjx::launch_logistic_rmhmc_tangent_f32(a, o, data->max_features, (hipStream_t)stream)
-                     : gjx::launch_logistic_rmhmc_tangent(a, o, (hipStream_t)stream);
+  hipError_t e = hipSuccess;
+  if (params->dtype == GJX_F32) {
+    if (data->max_features <= 64 || data->num_posteriors > 1)
+      e = gjx::launch_logistic_rmhmc_tangent_f32(a, o, data->max_features, (hipStream_t)stream);
+    if (e == hipSuccess && data->max_features > 64)
+      e = gjx::launch_logistic_rmhmc_tangent_f32_wide(a, o, (hipStream_t)stream);
+  } else {
+    e = gjx::launch_logistic_rmhmc_tangent(a, o, (hipStream_t)stream);
+  }
   if (e != hipSuccess) return hip_fail(e, "gjx_rmhmc_fisher_logistic_tangent launch");
   return GJX_OK;
 }

@@ -140,6 +140,8 @@ hipError_t launch_logistic_rmhmc_f64(const LogisticArgs& a, hipStream_t stream);
 hipError_t launch_logistic_mmala_f64(const LogisticArgs& a, hipStream_t stream);
 hipError_t launch_logistic_mmala_init(const LogisticArgs& a, hipStream_t stream);
 hipError_t launch_logistic_rmhmc_tangent(const LogisticArgs& a, const TangentOut& o, hipStream_t stream);
+// tangent for 65..128 features: float32, fp32 MFMA, 256 lanes per chain (logistic_grad.hip)
+hipError_t launch_logistic_rmhmc_tangent_f32_wide(const LogisticArgs& a, const TangentOut& o, hipStream_t stream);
 // tangent fast path: float32 state, split-bf16 MFMA, D <= 64 (logistic_grad_fast.hip)
 hipError_t launch_logistic_rmhmc_tangent_f32(const LogisticArgs& a, const TangentOut& o, int max_features,
                                              hipStream_t stream);

@@ -354,3 +354,40 @@ def test_logistic_fast_path_float32_matches_float64(N, D, C, eps, L):
     assert errs["objective"] < 2e-3
     for k in ("d objective", "d acceptance", "d sq jump"):
         assert errs[k] < 1e-2, k
+
+
+@pytest.mark.parametrize("N,D,C,eps,L,max_iters", [(300, 100, 3, 0.1, 4, 5), (512, 128, 2, 0.08, 3, 4)])
+def test_logistic_wide_tangent_float32_matches_autograd_through_oracle(N, D, C, eps, L, max_iters):
+    """The tangent for 65..128 features (float32 state and tangents on fp32
+    MFMA, float64 energies and objective; no float64 tangent holds G^-1 and its
+    tangent at this width) against autograd through the float64 oracle from the
+    same float32-representable positions and normals, with convergence_tol = 0
+    on both sides so that every step runs the same number of fixed-point
+    iterations: objective 1e-4 and derivative 1e-3 relative over the chains
+    (observed 5.5e-6 / 6.8e-6 at D = 100 and 4.6e-7 / 4.8e-7 at D = 128)."""
+    from geomjax_amd.synthetic import make_logistic
+
+    print(f"case wide tangent N={N} D={D} C={C} eps={eps} L={L}", flush=True)
+    X, y, _ = make_logistic(N, D, 90 + D)
+    q0 = (0.3 * seeded_normal(320 + D, C, D)).float().double()
+    z = seeded_normal(321 + D, C, D).float().double()
+    le = torch.tensor(math.log(eps), dtype=torch.float64, requires_grad=True)
+    Mo = om.LogisticRegression(X, y, 100.0)
+    k = orm.build_kernel(max_iters=max_iters, convergence_tol=0.0)
+    _, info = k((z, torch.zeros(C, dtype=torch.float64)), orm.init(q0, Mo.logp), Mo.logp, torch.exp(le),
+                Mo.fisher_metric_fn, L)
+    obj_o = info.acceptance_rate * ((info.proposal.state.position - q0) ** 2).sum(-1)
+    g_o = torch.stack([torch.autograd.grad(o, (le,), retain_graph=True)[0] for o in obj_o])
+    model = gj.LogisticRegression(X, y, 100.0)
+    r = esjd_acceptance(gr.HostNoise(z.to(DEV).float(), torch.zeros(C, device=DEV)),
+                        gj.rmhmc.init(q0.to(DEV).float(), model), model, eps, gj.FisherMetric(), L,
+                        integrator=functools.partial(implicit_midpoint, max_iters=max_iters, convergence_tol=0.0))
+    torch.cuda.synchronize()
+    e_obj = rel_err(r.objective, obj_o.detach())
+    e_d = rel_err(r.d_objective[:, 0], g_o)
+    print(f"  objective {r.objective.cpu().tolist()} vs {obj_o.detach().tolist()} ({e_obj:.1e}); "
+          f"d/dlog eps {r.d_objective[:, 0].cpu().tolist()} vs {g_o.tolist()} ({e_d:.1e}); "
+          f"fevals {r.num_fevals.tolist()}", flush=True)
+    assert float(r.d_objective[:, 1].abs().max()) == 0.0
+    assert e_obj < 1e-4
+    assert e_d < 1e-3
