@@ -594,11 +594,12 @@ int gjx_rmhmc_monge_funnel_tangent(const gjx_funnel_params* params, const gjx_ta
  * geomjax/adaptation/chees_adaptation_lmcmonge.py:497-519).  One transition
  * per chain from io->position with momentum chol(G(q0)) z; column 0 of the
  * [C, 2] derivative arrays is d/d log step_size, column 1 is zero (the
- * Fisher metric has no alpha2).  D <= 64, io->z as for the step entry point
- * (NULL: device PRNG at params->counter).  params->dtype GJX_F64: the check
- * path (float64); GJX_F32: the fast path (float32 position buffer, the tuned
- * kernel's split-bf16 MFMA passes with their tangents, float64 energies and
- * outputs; needs data->packed and data->workspace as the tuned step). */
+ * Fisher metric has no alpha2).  io->z as for the step entry point (NULL:
+ * device PRNG at params->counter).  params->dtype GJX_F64: the check path
+ * (float64, D <= 64); GJX_F32 (float32 position buffer; data->packed and
+ * data->workspace as for the tuned step; float64 energies and outputs): for
+ * D <= 64 the tuned kernel's split-bf16 MFMA passes with their tangents, for
+ * 65 <= D <= 128 a float32 form of the check path on fp32 MFMA. */
 int gjx_rmhmc_fisher_logistic_tangent(const gjx_logistic_params* params, const gjx_logistic_data* data,
                                       const gjx_tangent_io* io, void* stream);
 
