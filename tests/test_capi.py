@@ -109,7 +109,8 @@ def test_logistic_float32_requires_packed_table_and_workspace():
 
 
 def test_logistic_feature_capacity_per_path():
-    """RMHMC (float64 and float32) takes up to 128 features, LMC (both) up to 64."""
+    """RMHMC (float64 and float32) takes up to 128 features, LMC (both) up to 64,
+    the tangent 64 in float64 and 128 in float32."""
     lib = _lib.load()
 
     def step(fn, io, dtype, D):
@@ -128,6 +129,12 @@ def test_logistic_feature_capacity_per_path():
     assert step(lmc, lio, _lib.GJX_F64, 65) == -2
     assert step(lmc, lio, _lib.GJX_F32, 64) == -1
     assert step(lmc, lio, _lib.GJX_F32, 65) == -2
+    # the tuning tangent: float64 up to 64 features, float32 up to 128
+    tan, tio = lib.gjx_rmhmc_fisher_logistic_tangent, _lib.TangentIO()
+    assert step(tan, tio, _lib.GJX_F64, 64) == -1
+    assert step(tan, tio, _lib.GJX_F64, 65) == -2
+    assert step(tan, tio, _lib.GJX_F32, 128) == -1
+    assert step(tan, tio, _lib.GJX_F32, 129) == -2
 
 
 def test_mmala_and_logistic_nuts_entry_points_validate_on_the_host():
