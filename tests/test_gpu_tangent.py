@@ -391,3 +391,30 @@ def test_logistic_wide_tangent_float32_matches_autograd_through_oracle(N, D, C, 
     assert float(r.d_objective[:, 1].abs().max()) == 0.0
     assert e_obj < 1e-4
     assert e_d < 1e-3
+
+
+def test_adapt_a_batch_with_wide_posteriors_in_float32():
+    """Step-size tuning by gradient on a batch of posteriors of 24 and 100
+    features with float32 states (the one-wave fast path serves the narrow
+    posterior, the 65..128 float32 tangent the wide one): the derivative is
+    finite for every chain and Adam raises a too-small step size."""
+    from geomjax_amd.adjoint import adapt
+    from geomjax_amd.synthetic import make_logistic
+
+    models = [gj.LogisticRegression(*make_logistic(400, d, 60 + d)[:2], 100.0) for d in (24, 100)]
+    cpp = 16
+    batch = gj.LogisticBatch(models, chains_per_posterior=cpp)
+    q = torch.zeros(2 * cpp, batch.dim)
+    q[:cpp, :24] = 0.3 * seeded_normal(150, cpp, 24)
+    q[cpp:, :100] = 0.3 * seeded_normal(151, cpp, 100)
+    s = gj.rmhmc.init(q.to(DEV).float(), batch)
+    r = esjd_acceptance(gr.key(5), s, batch, 0.02, gj.FisherMetric(), 6)
+    torch.cuda.synchronize()
+    print("d objective per posterior", r.d_objective[:cpp, 0].mean().item(), r.d_objective[cpp:, 0].mean().item(),
+          flush=True)
+    assert torch.isfinite(r.d_objective).all() and torch.isfinite(r.objective).all()
+    assert torch.all(r.num_fevals > 0)
+    eps, a2, s2, hist = adapt(gr.key(6), s, batch, 6, step_size=0.02, num_steps=12, learning_rate=0.1)
+    print("adapted step size", eps, "history", [round(h[2], 4) for h in hist], flush=True)
+    assert math.isfinite(eps) and eps > 0.02
+    assert s2.position.dtype == torch.float32 and torch.isfinite(s2.position).all()
