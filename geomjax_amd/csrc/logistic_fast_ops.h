@@ -523,7 +523,7 @@ __device__ __forceinline__ void field_pass(Smem<NW>& S, const float* __restrict_
     for (int kc = 0; kc < KC; ++kc) load8(blk + ((uint32_t)kc * 512u + xoff), B.x[kc]);
     const uint32_t lr = (uint32_t)(l % 32);
     B.eta = (E1 + (size_t)rb * 32)[lr];
-    B.base = explicit_step ? 0.0f : (E0 + (size_t)rb * 32)[lr];
+    B.base = (E0 + (size_t)rb * 32)[lr];  // unused by the explicit step (no branch in the loop)
     B.yv = (yf + (size_t)rb * 32)[lr];
   };
   // The split G^-1 fragments of step tk + 1 are read from LDS while the three
@@ -581,7 +581,7 @@ __device__ __forceinline__ void field_pass(Smem<NW>& S, const float* __restrict_
     const float en = fmaf(hs, B.b, explicit_step ? B.eta : B.base);
     const float sn = sigmoidf(en);
     E1[row] = en;
-    e0_dst[row] = en;
+    e0_dst[row] = en;  // E1 again for a fixed-point iterate (no branch in the loop)
     SL[row] = sqrt_fast(sn * (1.0f - sn));
   };
   XF f;
@@ -597,6 +597,12 @@ __device__ __forceinline__ void field_pass(Smem<NW>& S, const float* __restrict_
     fetch_xf(f, blk_of(j + 1));
     bdot(B);
     post(B, rb);
+    // The next block's rows load into the registers this block's rows leave
+    // (the loop body is one basic block, so this holds): hoisted above post's
+    // last uses of B they would land in temporaries, and the copy into the
+    // loop-carried registers would wait for every load in flight (3 % on
+    // 2048 x 64 at 4096 chains).
+    __builtin_amdgcn_sched_barrier(0);
     fetch_blk(B, blk_of(j + 1));
   }
   const float tot = reduce_features<NB, NW>(S, dacc);
