@@ -52,7 +52,7 @@ struct Smem {
   float bc[2][MAXD];       // sweep: column broadcast [pivot parity]
   float dk[MAXD];          // pivots
   float ab[2][NW][2][32];  // field pass: partial a_n, b_n [block parity][wave][a|b][row]
-  float ey[2][2][32];      // field pass: eta, y of the block's rows [block parity]
+  float ey[2][3][32];      // field pass: eta, y, base of the block's rows [block parity]
   float half[2][MAXD];     // the two row halves' partial sums of the solves
   double red[NW];
   float q0[MAXD], p0[MAXD], q[MAXD], p[MAXD], u[MAXD], dH[MAXD], z[MAXD], g[MAXD];
@@ -328,20 +328,26 @@ __device__ __forceinline__ void field_pass(Smem& S, const float* __restrict__ Xr
   bf16x8 xh[KC], xl[KC];
   float xa[2][8];
   float eta = 0.0f, base = 0.0f, yv = 0.0f;
+  // The next block's operands load before the exchange barrier, so that the
+  // barrier and the finish of this block cover their latency.  A value still
+  // live when its next load lands is copied at the end of the loop behind a
+  // wait that covers every load issued before it: the rows of X (still read
+  // by X^T h, through a copy) therefore load first, the row scalars are loaded
+  // by every wave (only wave 0 uses them, base only for a fixed-point iterate:
+  // a load under a branch is such a copy too) and pass through LDS before the
+  // fetch, and the MFMA operands load last.
   auto fetch = [&](int rb) {
+#pragma unroll
+    for (int c = 0; c < 2; ++c) load8(xp + (size_t)(rb * KC + 2 * w + c) * 512, xa[c]);
+    const int row = rb * 32 + rl;
+    eta = E1[row];
+    base = E0[row];
+    yv = yf[row];
 #pragma unroll
     for (int kc = 0; kc < KC; ++kc) {
       const uint4* q = sp + (size_t)(rb * KC + kc) * 128;
       xh[kc] = __builtin_bit_cast(bf16x8, q[0]);
       xl[kc] = __builtin_bit_cast(bf16x8, q[1]);
-    }
-#pragma unroll
-    for (int c = 0; c < 2; ++c) load8(xp + (size_t)(rb * KC + 2 * w + c) * 512, xa[c]);
-    if (w == 0) {
-      const int row = rb * 32 + rl;
-      eta = E1[row];
-      base = explicit_step ? 0.0f : E0[row];
-      yv = yf[row];
     }
   };
   fetch(0);
@@ -371,16 +377,18 @@ __device__ __forceinline__ void field_pass(Smem& S, const float* __restrict__ Xr
       if (w == 0) {
         S.ey[par][0][rl] = eta;
         S.ey[par][1][rl] = yv;
+        S.ey[par][2][rl] = base;
       }
     }
-    const float base_r = base;
     float xr[2][8];
 #pragma unroll
     for (int c = 0; c < 2; ++c)
 #pragma unroll
       for (int i = 0; i < 8; ++i) xr[c][i] = xa[c][i];
+    // the next block's operands load across the exchange barrier and the
+    // finish of this block (past the end the last block again: in bounds, unused)
+    fetch(rb + 1 < nrb ? rb + 1 : rb);
     __syncthreads();
-    if (rb + 1 < nrb) fetch(rb + 1);  // the next block's operands load during the finish of this one
     const float an = (S.ab[par][0][0][rl] + S.ab[par][1][0][rl]) + (S.ab[par][2][0][rl] + S.ab[par][3][0][rl]);
     const float bn = (S.ab[par][0][1][rl] + S.ab[par][1][1][rl]) + (S.ab[par][2][1][rl] + S.ab[par][3][1][rl]);
     const float et = S.ey[par][0][rl];
@@ -393,7 +401,7 @@ __device__ __forceinline__ void field_pass(Smem& S, const float* __restrict__ Xr
       for (int i = 0; i < 8; ++i) dacc[c][i] = fmaf(hv, xr[c][i], dacc[c][i]);
     if (w == 0 && h == 0) {
       const int row = rb * 32 + rl;
-      const float en = fmaf(hs, bn, explicit_step ? et : base_r);
+      const float en = fmaf(hs, bn, explicit_step ? et : S.ey[par][2][rl]);
       const float sn = sigmoidf(en);
       E1[row] = en;
       e0_dst[row] = en;  // the explicit step moves the step start; iterates rewrite E1 only
