@@ -700,11 +700,13 @@ __device__ __forceinline__ void lmc_row_pass(Smem<1>& S, const float* __restrict
     for (int i = 0; i < 8; ++i) gacc[kc][i] = 0.0f;
   }
   const float* xp = Xr + (size_t)l * 8;
-  const int nj = npad / 32;
-  for (int rb = 0; rb < nj; ++rb) {
-    float x[KC][8];
+  const int nj = npad / 32;  // even: npad is a multiple of 64
+  auto fetch = [&](float (&x)[KC][8], float& eta, int rb) {
 #pragma unroll
     for (int kc = 0; kc < KC; ++kc) load8(xp + (size_t)(rb * KC + kc) * 512, x[kc]);
+    eta = ETA[rb * 32 + (l % 32)];
+  };
+  auto block = [&](const float (&x)[KC][8], float eta, int rb) {
     float bp = 0.0f;
 #pragma unroll
     for (int kc = 0; kc < KC; ++kc)
@@ -712,7 +714,6 @@ __device__ __forceinline__ void lmc_row_pass(Smem<1>& S, const float* __restrict
       for (int i = 0; i < 8; ++i) bp = fmaf(x[kc][i], v[kc][i], bp);
     const float b = bp + __shfl_xor(bp, 32, 64);
     const int row = rb * 32 + (l % 32);
-    const float eta = ETA[row];
     const float s = sigmoidf(eta);
     const float lam = s * (1.0f - s);
     const float d = fmaf(coef * lam * (1.0f - 2.0f * s), b, lam);
@@ -732,6 +733,16 @@ __device__ __forceinline__ void lmc_row_pass(Smem<1>& S, const float* __restrict
         SL[row] = sqrt_fast(sn * (1.0f - sn));
       }
     }
+  };
+  // two register sets in turn: each block's rows (and eta, whose row the
+  // block itself rewrites only after reading it) load one block ahead
+  float x0[KC][8], x1[KC][8], e0, e1;
+  fetch(x0, e0, 0);
+  for (int rb = 0; rb < nj; rb += 2) {
+    fetch(x1, e1, rb + 1);
+    block(x0, e0, rb);
+    fetch(x0, e0, rb + 2 < nj ? rb + 2 : rb + 1);  // past the end the last block again (in bounds, unused)
+    block(x1, e1, rb + 1);
   }
   if (want_gv) {
     const float tot = reduce_features<NB, 1>(S, gacc);
