@@ -146,12 +146,15 @@ __device__ __forceinline__ void tangent_field_pass(TSmem& S, const float* __rest
     {
       const float* const blk = Xr + (size_t)rb * (KC * 512);
       const uint4* const sblk = sbase + (size_t)rb * (KC * 128);
+      // the MFMA operands first (the first product waits only for its own),
+      // then the rows of X, first read after the products are issued
 #pragma unroll
       for (int kc = 0; kc < KC; ++kc) {
-        load8(blk + kc * 512 + l * 8, x[kc]);
         xh[kc] = __builtin_bit_cast(bf16x8, sblk[kc * 128 + l * 2]);
         xl[kc] = __builtin_bit_cast(bf16x8, sblk[kc * 128 + l * 2 + 1]);
       }
+#pragma unroll
+      for (int kc = 0; kc < KC; ++kc) load8(blk + kc * 512 + l * 8, x[kc]);
     }
     f32x16 acc[NB], tac[NB];
 #pragma unroll
