@@ -33,6 +33,7 @@ DEV = "cuda:0"
 
 
 @pytest.mark.parametrize("N,D,C,eps,L,seed", [
+    (50, 6, 3, 0.1, 5, 5),       # one 64-row group: the smallest row loops
     (256, 8, 3, 0.1, 6, 0),
     (300, 13, 2, 0.1, 4, 1),
     (1024, 32, 2, 0.1, 5, 2),    # the one-block width's capacity

@@ -58,6 +58,7 @@ def _check_step(info, s, ref_info, ref_state, u, q_prev, label):
 
 
 @pytest.mark.parametrize("N,D,C,eps,L,max_iters,seed", [
+    (40, 6, 3, 0.1, 6, 6, 5),      # one 64-row group: the smallest row loops
     (256, 8, 3, 0.1, 10, 6, 0),
     (300, 13, 2, 0.1, 6, 6, 1),    # ragged: N not a multiple of 32
     (1000, 40, 2, 0.1, 5, 6, 3),   # two 32-feature blocks with padding
