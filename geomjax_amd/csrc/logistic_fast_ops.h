@@ -9,6 +9,8 @@
 // the LMC kernel (logistic_lmc_fast.hip).
 #pragma once
 
+#include <type_traits>
+
 #include "common.h"
 #include "logistic.h"
 #include "logistic_f32.h"
@@ -281,8 +283,13 @@ __device__ __forceinline__ void sweep(Smem<NW>& S, f32x32 (&A)[NB], int D, bool 
   // (A[k][i] = A[i][k]), broadcast through LDS.  Padding pivots (k >= D) are
   // 1 with zero off-diagonals: nothing to do.
   float(*bcast)[64] = S.bc;
-  for (int k = 0; k < D; ++k) {
-    const float ak = col_get<NB>(A, k);  // A[k][jl] = A[jl][k]
+  // One pivot; its row k lives in tile T (k in [32 T, 32 T + 32)), so the
+  // register-relative read and write of A[k] index one fixed tile: with one
+  // loop per tile the write stays in place (a runtime choice of tile made the
+  // compiler copy the whole tile out and back around the write).
+  auto pivot = [&](auto tile, int k) {
+    constexpr int T = decltype(tile)::value;
+    const float ak = A[T][k - 32 * T];  // A[k][jl] = A[jl][k]
     float* bc = bcast[k & 1];
     bc[jl] = ak;
     wave0_sync<NW>();
@@ -304,9 +311,13 @@ __device__ __forceinline__ void sweep(Smem<NW>& S, f32x32 (&A)[NB], int D, bool 
     for (int b = 0; b < NB; ++b)
 #pragma unroll
       for (int i = 0; i < 32; ++i) A[b][i] = fmaf(-col[32 * b + i], f, A[b][i]);
-    col_set<NB>(A, k, (jl == k) ? -rd : ak * rd);
+    A[T][k - 32 * T] = (jl == k) ? -rd : ak * rd;
     if (threadIdx.x == 0) S.dk[k] = d;
-  }
+  };
+  const int k0 = D < 32 ? D : 32;
+  for (int k = 0; k < k0; ++k) pivot(std::integral_constant<int, 0>{}, k);
+  if constexpr (NB == 2)
+    for (int k = 32; k < D; ++k) pivot(std::integral_constant<int, 1>{}, k);
   // -G^-1 on the padding block is -I
   for (int k = D; k < DP; ++k) col_set<NB>(A, k, jl == k ? -1.0f : 0.0f);
   if (threadIdx.x < DP && threadIdx.x >= D) S.dk[threadIdx.x] = 1.0f;
