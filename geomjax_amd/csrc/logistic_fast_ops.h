@@ -244,31 +244,6 @@ __device__ __forceinline__ void gram_pass(Smem<NW>& S, const float* __restrict__
 // update of the other entries is unrolled.  On return A = -G^-1 and S.dk the
 // pivots.  With want_chol, p0j = (L z)_jl with L = chol(G) and z in S.z.
 // ---------------------------------------------------------------------------
-template <int NB>
-__device__ __forceinline__ float col_get(const f32x32 (&A)[NB], int k) {
-  if constexpr (NB == 1) {
-    return A[0][k];
-  } else {
-    float v;
-    if (k < 32)
-      v = A[0][k];
-    else
-      v = A[1][k - 32];
-    return v;
-  }
-}
-template <int NB>
-__device__ __forceinline__ void col_set(f32x32 (&A)[NB], int k, float v) {
-  if constexpr (NB == 1) {
-    A[0][k] = v;
-  } else {
-    if (k < 32)
-      A[0][k] = v;
-    else
-      A[1][k - 32] = v;
-  }
-}
-
 template <int NB, int NW>
 __device__ __forceinline__ void sweep(Smem<NW>& S, f32x32 (&A)[NB], int D, bool want_chol, float& p0j,
                                       bool store_l = false) {
@@ -319,7 +294,9 @@ __device__ __forceinline__ void sweep(Smem<NW>& S, f32x32 (&A)[NB], int D, bool 
   if constexpr (NB == 2)
     for (int k = 32; k < D; ++k) pivot(std::integral_constant<int, 1>{}, k);
   // -G^-1 on the padding block is -I
-  for (int k = D; k < DP; ++k) col_set<NB>(A, k, jl == k ? -1.0f : 0.0f);
+  for (int k = D; k < 32; ++k) A[0][k] = jl == k ? -1.0f : 0.0f;
+  if constexpr (NB == 2)
+    for (int k = D > 32 ? D : 32; k < DP; ++k) A[1][k - 32] = jl == k ? -1.0f : 0.0f;
   if (threadIdx.x < DP && threadIdx.x >= D) S.dk[threadIdx.x] = 1.0f;
   wave0_sync<NW>();
 }

@@ -57,8 +57,12 @@ __device__ __forceinline__ void sweep_dual(TSmem& S, f32x32 (&A)[NB], f32x32 (&T
       T[b][i] = S.GT[(32 * b + i) * MAXD + jl];
     }
   p0j = 0.0f;
-  for (int k = 0; k < D; ++k) {
-    const float ak = col_get<NB>(A, k), tk = col_get<NB>(T, k);
+  // one pivot loop per 32-row tile, so the register-relative reads and writes
+  // of row k index a fixed tile of A and T and stay in place (a runtime choice
+  // of tile makes the compiler copy the tiles out and back for every pivot)
+  auto pivot = [&](auto tile, int k) {
+    constexpr int TL = decltype(tile)::value;
+    const float ak = A[TL][k - 32 * TL], tk = T[TL][k - 32 * TL];
     float* bc = S.bc[k & 1];
     float* bt = S.bcT[k & 1];
     bc[jl] = ak;
@@ -77,17 +81,26 @@ __device__ __forceinline__ void sweep_dual(TSmem& S, f32x32 (&A)[NB], f32x32 (&T
         T[b][i] = fmaf(-ti, f, fmaf(-ci, tf, T[b][i]));
         A[b][i] = fmaf(-ci, f, A[b][i]);
       }
-    col_set<NB>(A, k, (jl == k) ? -rd : ak * rd);
-    col_set<NB>(T, k, (jl == k) ? td * rd * rd : (tk - ak * td * rd) * rd);
+    A[TL][k - 32 * TL] = (jl == k) ? -rd : ak * rd;
+    T[TL][k - 32 * TL] = (jl == k) ? td * rd * rd : (tk - ak * td * rd) * rd;
     if (threadIdx.x == 0) {
       S.dk[k] = d;
       S.tdk[k] = td;
     }
+  };
+  const int k0 = D < 32 ? D : 32;
+  for (int k = 0; k < k0; ++k) pivot(std::integral_constant<int, 0>{}, k);
+  if constexpr (NB == 2)
+    for (int k = 32; k < D; ++k) pivot(std::integral_constant<int, 1>{}, k);
+  for (int k = D; k < 32; ++k) {
+    A[0][k] = jl == k ? -1.0f : 0.0f;
+    T[0][k] = 0.0f;
   }
-  for (int k = D; k < DP; ++k) {
-    col_set<NB>(A, k, jl == k ? -1.0f : 0.0f);
-    col_set<NB>(T, k, 0.0f);
-  }
+  if constexpr (NB == 2)
+    for (int k = D > 32 ? D : 32; k < DP; ++k) {
+      A[1][k - 32] = jl == k ? -1.0f : 0.0f;
+      T[1][k - 32] = 0.0f;
+    }
   if (threadIdx.x < DP && threadIdx.x >= D) {
     S.dk[threadIdx.x] = 1.0f;
     S.tdk[threadIdx.x] = 0.0f;
